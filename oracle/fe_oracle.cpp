@@ -1,0 +1,1253 @@
+// fe_oracle.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// Clean-room FP64 CPU restatement of the Fluide_Engine hybrid particle-grid
+// substep (PIC / FLIP / APIC transfers, fluid SDF + cell marking, velocity
+// extrapolation, blocked boundary conditions without colliders, gravity,
+// SinglePhasePressureSolver with Jacobi / CG, G2P and midpoint advection).
+// It is the checker for the HIP library and the CPU baseline timed beside it.
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the Fluide_Engine source tree).  Reference quirks are reproduced on purpose:
+//   * PIC/APIC P2G pass the cell dims for every component (src/pic_simulator.cpp:389,397,405;
+//     src/apic_simulator.cpp:58,71,84), FLIP passes each component's own dims
+//     (src/flip_simulator.cpp:58,66,74).
+//   * face markers are set for all 8 stencil nodes, weight or not (src/pic_simulator.cpp:394).
+//   * APIC weights use the unclamped position, the affine term the clamped one
+//     (src/apic_simulator.cpp:55-63,150-153).
+//   * FLIP snapshot taken right after P2G normalisation (src/flip_simulator.cpp:107-118).
+//   * ApplyPressure only corrects +side faces of FLUID cells
+//     (src/fluid_solvers/single_phase_pressure_solver.cpp:74-91).
+//   * RHS is the unscaled divergence, A = 1/h^2 stencil, gradient 1/h (same file :59-60,103-118).
+//   * pressure warm start from the previous solve (src/common/array3.hpp:189-197).
+//   * Cfl adds dt*(0,-9.8,0) to |u| (src/pic_simulator.cpp:88).
+// P2G is serial in particle order (the reference's threaded P2G races, so its
+// own output is not reproducible; the serial order is the deterministic one).
+//
+// Parity pinning (see tests/test_oracle_*.py): the reference's known-answer
+// tests (test/blas_test.cpp, test/math_utils_test.cpp, test/face_centered_grid3d_test.cpp,
+// test/scalar_grid3d_test.cpp, test/single_phase_pressuer_solver_test.cpp,
+// test/hash_grid_particle_searcher_test.cpp), closed forms, dense direct
+// solves and brute force on tiny inputs.
+//
+// Build: oracle/Makefile -> oracle/_build/libfe_oracle.so (C ABI, loaded by ctypes).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#include "fe_oracle.h"
+
+namespace {
+
+constexpr double kMaxDistance = 1.7976931348623157E+30;  // src/3d/collider_collection.cpp:3
+
+// ---------------------------------------------------------------------------
+// Thread model for the CPU baseline: the reference forks hardware_concurrency()
+// std::threads per loop and chunks the slowest-varying index
+// (src/common/parallel_utils.hpp:24-61,114-154).  The loops below are all
+// race-free element-wise sweeps, so the result does not depend on threads.
+// ---------------------------------------------------------------------------
+void par_for(int nthreads, int64_t n, const std::function<void(int64_t, int64_t)>& body) {
+    if (nthreads <= 1 || n < 4096) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    int64_t chunk = (n + nthreads - 1) / nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        int64_t a = t * chunk, b = std::min<int64_t>(n, a + chunk);
+        if (a >= b) break;
+        pool.emplace_back(body, a, b);
+    }
+    for (auto& th : pool) th.join();
+}
+
+struct Arr3 {
+    int sx = 0, sy = 0, sz = 0;
+    std::vector<double> d;
+    void resize(int x, int y, int z, double v = 0.0) {
+        sx = x; sy = y; sz = z;
+        d.assign((size_t)x * y * z, v);
+    }
+    // src/common/array3.hpp:65-85 -- x-fastest layout i + sx*(j + sy*k)
+    inline size_t at(int i, int j, int k) const { return (size_t)i + (size_t)sx * ((size_t)j + (size_t)sy * k); }
+    inline double& operator()(int i, int j, int k) { return d[at(i, j, k)]; }
+    inline double operator()(int i, int j, int k) const { return d[at(i, j, k)]; }
+    size_t size() const { return d.size(); }
+};
+
+struct Arr3i {
+    int sx = 0, sy = 0, sz = 0;
+    std::vector<int> d;
+    void resize(int x, int y, int z, int v = 0) {
+        sx = x; sy = y; sz = z;
+        d.assign((size_t)x * y * z, v);
+    }
+    inline size_t at(int i, int j, int k) const { return (size_t)i + (size_t)sx * ((size_t)j + (size_t)sy * k); }
+    inline int& operator()(int i, int j, int k) { return d[at(i, j, k)]; }
+    inline int operator()(int i, int j, int k) const { return d[at(i, j, k)]; }
+};
+
+// src/common/math_utils.hpp:13-28 -- Clamp(val, maxVal, minVal)
+inline double clamp_ref(double val, double maxVal, double minVal) {
+    if (val < minVal) return minVal;
+    if (val > maxVal) return maxVal;
+    return val;
+}
+
+// src/common/math_utils.hpp:30-34 -- Lerp(a, b, f) = (1-f)*a + f*b
+inline double lerp(double a, double b, double f) { return (1 - f) * a + f * b; }
+
+// src/common/math_utils.hpp:42-46
+inline double trilerp(double x000, double x100, double x010, double x110, double x001, double x101, double x011,
+                      double x111, double fx, double fy, double fz) {
+    return lerp(lerp(lerp(x000, x100, fx), lerp(x010, x110, fx), fy),
+                lerp(lerp(x001, x101, fx), lerp(x011, x111, fx), fy), fz);
+}
+
+// src/common/math_utils.hpp:48-73 -- GetBarycentric(x, iLow=0, iHigh, &i, &f)
+inline void barycentric(double x, int iLow, int iHigh, int* i, double* f) {
+    double s = std::floor(x);
+    *i = static_cast<int>(s);
+    int siLow = iLow, siHigh = iHigh;
+    int offset = -siLow;
+    siLow += offset;
+    siHigh += offset;
+    if (siLow == siHigh) {
+        *i = siLow;
+        *f = 0;
+    } else if (*i < siLow) {
+        *i = siLow;
+        *f = 0;
+    } else if (*i > siHigh - 1) {
+        *i = siHigh - 1;
+        *f = 1;
+    } else {
+        *f = x - s;
+    }
+    *i -= offset;
+}
+
+struct Stencil {
+    int idx[8][3];
+    double w[8];
+};
+
+// src/common/math_utils.hpp:75-112 -- GetCooridnatesAndWeights(size, origin, h, x, indexes, weights)
+inline void coords_weights(const int size[3], const double origin[3], const double h[3], const double x[3],
+                           Stencil* st) {
+    int i = 0, j = 0, k = 0;
+    double fx = 0, fy = 0, fz = 0;
+    const double nx = (x[0] - origin[0]) / h[0];
+    const double ny = (x[1] - origin[1]) / h[1];
+    const double nz = (x[2] - origin[2]) / h[2];
+    barycentric(nx, 0, size[0] - 1, &i, &fx);
+    barycentric(ny, 0, size[1] - 1, &j, &fy);
+    barycentric(nz, 0, size[2] - 1, &k, &fz);
+    const int ip1 = std::min(i + 1, size[0] - 1);
+    const int jp1 = std::min(j + 1, size[1] - 1);
+    const int kp1 = std::min(k + 1, size[2] - 1);
+    const int I[8] = {i, ip1, i, ip1, i, ip1, i, ip1};
+    const int J[8] = {j, j, jp1, jp1, j, j, jp1, jp1};
+    const int K[8] = {k, k, k, k, kp1, kp1, kp1, kp1};
+    for (int c = 0; c < 8; ++c) {
+        st->idx[c][0] = I[c];
+        st->idx[c][1] = J[c];
+        st->idx[c][2] = K[c];
+    }
+    st->w[0] = (1.00 - fx) * (1.00 - fy) * (1.00 - fz);
+    st->w[1] = fx * (1.00 - fy) * (1.00 - fz);
+    st->w[2] = (1.00 - fx) * fy * (1.00 - fz);
+    st->w[3] = fx * fy * (1.00 - fz);
+    st->w[4] = (1.00 - fx) * (1.00 - fy) * fz;
+    st->w[5] = fx * (1.00 - fy) * fz;
+    st->w[6] = (1.00 - fx) * fy * fz;
+    st->w[7] = fx * fy * fz;
+}
+
+struct GradStencil {
+    int idx[8][3];
+    double g[8][3];
+};
+
+// src/common/math_utils.hpp:114-176 -- GetCooridnatesAndGradientWeights
+inline void coords_grad_weights(const int size[3], const double origin[3], const double h[3], const double x[3],
+                                GradStencil* st) {
+    int i = 0, j = 0, k = 0;
+    double fx = 0, fy = 0, fz = 0;
+    const double nx = (x[0] - origin[0]) / h[0];
+    const double ny = (x[1] - origin[1]) / h[1];
+    const double nz = (x[2] - origin[2]) / h[2];
+    const double ihx = 1.0 / h[0], ihy = 1.0 / h[1], ihz = 1.0 / h[2];
+    barycentric(nx, 0, size[0] - 1, &i, &fx);
+    barycentric(ny, 0, size[1] - 1, &j, &fy);
+    barycentric(nz, 0, size[2] - 1, &k, &fz);
+    const int ip1 = std::min(i + 1, size[0] - 1);
+    const int jp1 = std::min(j + 1, size[1] - 1);
+    const int kp1 = std::min(k + 1, size[2] - 1);
+    const int I[8] = {i, ip1, i, ip1, i, ip1, i, ip1};
+    const int J[8] = {j, j, jp1, jp1, j, j, jp1, jp1};
+    const int K[8] = {k, k, k, k, kp1, kp1, kp1, kp1};
+    for (int c = 0; c < 8; ++c) {
+        st->idx[c][0] = I[c];
+        st->idx[c][1] = J[c];
+        st->idx[c][2] = K[c];
+    }
+    double(*g)[3] = st->g;
+    g[0][0] = -ihx * (1 - fy) * (1 - fz); g[0][1] = -ihy * (1 - fx) * (1 - fz); g[0][2] = -ihz * (1 - fx) * (1 - fy);
+    g[1][0] = ihx * (1 - fy) * (1 - fz);  g[1][1] = fx * (-ihy) * (1 - fz);     g[1][2] = fx * (1 - fy) * (-ihz);
+    g[2][0] = (-ihx) * fy * (1 - fz);     g[2][1] = (1 - fx) * ihy * (1 - fz);  g[2][2] = (1 - fx) * fy * (-ihz);
+    g[3][0] = ihx * fy * (1 - fz);        g[3][1] = fx * ihy * (1 - fz);        g[3][2] = fx * fy * (-ihz);
+    g[4][0] = (-ihx) * (1 - fy) * fz;     g[4][1] = (1 - fx) * (-ihy) * fz;     g[4][2] = (1 - fx) * (1 - fy) * ihz;
+    g[5][0] = ihx * (1 - fy) * fz;        g[5][1] = fx * (-ihy) * fz;           g[5][2] = fx * (1 - fy) * ihz;
+    g[6][0] = (-ihx) * fy * fz;           g[6][1] = (1 - fx) * ihy * fz;        g[6][2] = (1 - fx) * fy * ihz;
+    g[7][0] = ihx * fy * fz;              g[7][1] = fx * ihy * fz;              g[7][2] = fx * fy * ihz;
+}
+
+// src/grid_systems/face_centered_grid3d.cpp:339-371 -- SampleArray(position, origin, arr)
+// (also src/3d/scalar_grid3d.cpp:33-58 for ScalarGrid3D::Sample with the grid origin)
+inline double sample_array(const Arr3& a, const double origin[3], const double h[3], const double p[3]) {
+    int i = 0, j = 0, k = 0;
+    double fx = 0, fy = 0, fz = 0;
+    const double nx = (p[0] - origin[0]) / h[0];
+    const double ny = (p[1] - origin[1]) / h[1];
+    const double nz = (p[2] - origin[2]) / h[2];
+    barycentric(nx, 0, a.sx - 1, &i, &fx);
+    barycentric(ny, 0, a.sy - 1, &j, &fy);
+    barycentric(nz, 0, a.sz - 1, &k, &fz);
+    const int ip1 = std::min(i + 1, a.sx - 1);
+    const int jp1 = std::min(j + 1, a.sy - 1);
+    const int kp1 = std::min(k + 1, a.sz - 1);
+    return trilerp(a(i, j, k), a(ip1, j, k), a(i, jp1, k), a(ip1, jp1, k), a(i, j, kp1), a(ip1, j, kp1),
+                   a(i, jp1, kp1), a(ip1, jp1, kp1), fx, fy, fz);
+}
+
+// src/common/math_utils.hpp:186-253 -- ExtrapolateToRegion(input, valid, iterations, output)
+// Jacobi-style: an invalid node with valid 6-neighbours takes their mean and
+// becomes valid for the next iteration.  Reads of `output` only touch nodes that
+// are valid in valid0, which are never written in the same sweep, so the
+// reference's in-place parallel sweep is deterministic.
+void extrapolate_to_region(const Arr3& input, const Arr3i& valid, int iterations, Arr3& output, int nthreads) {
+    const int sx = input.sx, sy = input.sy, sz = input.sz;
+    Arr3i valid0 = valid, valid1 = valid;
+    output = input;
+    for (int it = 0; it < iterations; ++it) {
+        par_for(nthreads, sz, [&](int64_t k0, int64_t k1) {
+            for (int k = (int)k0; k < (int)k1; ++k)
+                for (int j = 0; j < sy; ++j)
+                    for (int i = 0; i < sx; ++i) {
+                        if (!valid0(i, j, k)) {
+                            double sum = 0;
+                            unsigned count = 0;
+                            if (i + 1 < sx && valid0(i + 1, j, k)) { sum += output(i + 1, j, k); ++count; }
+                            if (i > 0 && valid0(i - 1, j, k)) { sum += output(i - 1, j, k); ++count; }
+                            if (j + 1 < sy && valid0(i, j + 1, k)) { sum += output(i, j + 1, k); ++count; }
+                            if (j > 0 && valid0(i, j - 1, k)) { sum += output(i, j - 1, k); ++count; }
+                            if (k + 1 < sz && valid0(i, j, k + 1)) { sum += output(i, j, k + 1); ++count; }
+                            if (k > 0 && valid0(i, j, k - 1)) { sum += output(i, j, k - 1); ++count; }
+                            if (count > 0) {
+                                output(i, j, k) = sum / count;
+                                valid1(i, j, k) = 1;
+                            }
+                        } else {
+                            valid1(i, j, k) = 1;
+                        }
+                    }
+        });
+        std::swap(valid0.d, valid1.d);  // src/common/math_utils.hpp:251 Swap
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Scene state: the Fluid3 struct (src/fluid3.hpp:10-26) plus the solver state
+// the reference keeps in SinglePhasePressureSolver / FLIPSimulator / APICSimulator.
+// ---------------------------------------------------------------------------
+struct orc_scene {
+    int n[3];
+    double origin[3], size[3], h[3];
+    int method;
+    double blend;   // FLIP _blendingFactor = PIC weight (src/flip_simulator.cpp:5,155-159)
+    double maxCfl;  // src/hybrid_simulator.cpp:21
+    double gravity[3];
+    int nthreads = 1;
+    int has_collider = 0;
+    double xo[3], yo[3], zo[3];  // face data origins, src/grid_systems/face_centered_grid3d.cpp:324-332
+
+    Arr3 u, v, w;         // face velocities
+    Arr3i mu, mv, mw;     // face markers from P2G (Fluid3::x/y/zMarkers)
+    Arr3 ru, rv, rw;      // FLIP _velocityResiduals (snapshot, then delta)
+    Arr3 sdf;             // fluid SDF, cell centred (origin + h/2), src/hybrid_simulator.cpp:13-15
+    Arr3 csdf;            // collider SDF, node centred (origin), src/3d/collider_collection.cpp:8-11
+    std::vector<uint8_t> flags;  // SinglePhasePressureSolver::_fluidMarkers
+    Arr3 p, b;            // LinearSystem x (kept between solves: warm start) and b
+    Arr3 Ac, Ar, Au, Af;  // LinearSystem A rows {center,right,up,front}
+    std::vector<double> x, vel, C;  // particles (3N, 3N, 9N)
+    int64_t np = 0;
+    orc_solve_stats last{};
+};
+
+namespace {
+
+void face_origins(orc_scene* s) {
+    // src/grid_systems/face_centered_grid3d.cpp:324-332
+    for (int a = 0; a < 3; ++a) {
+        s->xo[a] = s->origin[a] + s->h[a] * (a == 0 ? 0.0 : 0.5);
+        s->yo[a] = s->origin[a] + s->h[a] * (a == 1 ? 0.0 : 0.5);
+        s->zo[a] = s->origin[a] + s->h[a] * (a == 2 ? 0.0 : 0.5);
+    }
+}
+
+const double* comp_origin(const orc_scene* s, int c) { return c == 0 ? s->xo : (c == 1 ? s->yo : s->zo); }
+Arr3& comp_arr(orc_scene* s, int c) { return c == 0 ? s->u : (c == 1 ? s->v : s->w); }
+Arr3i& comp_mark(orc_scene* s, int c) { return c == 0 ? s->mu : (c == 1 ? s->mv : s->mw); }
+Arr3& comp_res(orc_scene* s, int c) { return c == 0 ? s->ru : (c == 1 ? s->rv : s->rw); }
+
+// FaceCenteredGrid3D::Sample (src/grid_systems/face_centered_grid3d.cpp:138-144)
+inline void sample_vel(const orc_scene* s, const Arr3& u, const Arr3& v, const Arr3& w, const double p[3],
+                       double out[3]) {
+    out[0] = sample_array(u, s->xo, s->h, p);
+    out[1] = sample_array(v, s->yo, s->h, p);
+    out[2] = sample_array(w, s->zo, s->h, p);
+}
+
+// ScalarGrid3D::Sample on the collider SDF (node centred at the domain origin)
+inline double sample_collider(const orc_scene* s, const double p[3]) { return sample_array(s->csdf, s->origin, s->h, p); }
+
+// ---------------------------------------------------------------------------
+// P2G.  PIC: src/pic_simulator.cpp:356-437.  FLIP: src/flip_simulator.cpp:25-119.
+// APIC: src/apic_simulator.cpp:15-118.
+// ---------------------------------------------------------------------------
+void p2g(orc_scene* s) {
+    const int64_t N = s->np;
+    int cell[3] = {s->n[0], s->n[1], s->n[2]};
+    Arr3 wsum[3];
+    for (int c = 0; c < 3; ++c) {
+        Arr3& a = comp_arr(s, c);
+        std::fill(a.d.begin(), a.d.end(), 0.0);
+        wsum[c].resize(a.sx, a.sy, a.sz, 0.0);
+        Arr3i& m = comp_mark(s, c);
+        m.resize(a.sx, a.sy, a.sz, 0);
+    }
+    if (s->method == ORC_APIC && (int64_t)s->C.size() != 9 * N) s->C.assign(9 * N, 0.0);  // _cX.resize (:22-24)
+    const double hh[3] = {s->h[0] / 2.0, s->h[1] / 2.0, s->h[2] / 2.0};
+    Stencil st;
+    for (int64_t pi = 0; pi < N; ++pi) {  // serial: deterministic order
+        const double* pos = &s->x[3 * pi];
+        const double* vel = &s->vel[3 * pi];
+        for (int c = 0; c < 3; ++c) {
+            Arr3& a = comp_arr(s, c);
+            Arr3i& m = comp_mark(s, c);
+            // Q1: PIC/APIC pass the cell dims, FLIP each component's own dims
+            int own[3] = {a.sx, a.sy, a.sz};
+            const int* size = (s->method == ORC_FLIP) ? own : cell;
+            coords_weights(size, comp_origin(s, c), s->h, pos, &st);
+            if (s->method == ORC_APIC) {
+                // src/apic_simulator.cpp:54-63 -- clamp the two transverse axes to [o+h/2, o+L-h/2]
+                double pc[3] = {pos[0], pos[1], pos[2]};
+                for (int ax = 0; ax < 3; ++ax)
+                    if (ax != c)
+                        pc[ax] = clamp_ref(pc[ax], s->origin[ax] + s->size[ax] - hh[ax], s->origin[ax] + hh[ax]);
+                const double* cr = &s->C[9 * pi + 3 * c];
+                const double* co = comp_origin(s, c);
+                for (int q = 0; q < 8; ++q) {
+                    // GetXPos: dataOrigin + h * (i,j,k)  (src/grid_systems/face_centered_grid3d.cpp:303-307)
+                    double gp[3];
+                    for (int ax = 0; ax < 3; ++ax) gp[ax] = co[ax] + s->h[ax] * (double)st.idx[q][ax];
+                    // Vector3::Dot (src/common/vector3.hpp:136-139)
+                    double apic = cr[0] * (gp[0] - pc[0]) + cr[1] * (gp[1] - pc[1]) + cr[2] * (gp[2] - pc[2]);
+                    size_t id = a.at(st.idx[q][0], st.idx[q][1], st.idx[q][2]);
+                    a.d[id] += (vel[c] + apic) * st.w[q];
+                    wsum[c].d[id] += st.w[q];
+                    m.d[id] = 1;
+                }
+            } else {
+                for (int q = 0; q < 8; ++q) {
+                    size_t id = a.at(st.idx[q][0], st.idx[q][1], st.idx[q][2]);
+                    a.d[id] += vel[c] * st.w[q];
+                    wsum[c].d[id] += st.w[q];
+                    m.d[id] = 1;  // Q2
+                }
+            }
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        Arr3& a = comp_arr(s, c);
+        const Arr3& ws = wsum[c];
+        par_for(s->nthreads, (int64_t)a.size(), [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i)
+                if (ws.d[i] > 0.0) a.d[i] /= ws.d[i];
+        });
+    }
+    if (s->method == ORC_FLIP) {  // src/flip_simulator.cpp:107-118
+        s->ru = s->u;
+        s->rv = s->v;
+        s->rw = s->w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// G2P.  PIC: src/pic_simulator.cpp:439-450.  FLIP: src/flip_simulator.cpp:121-161.
+// APIC: src/apic_simulator.cpp:120-176.
+// ---------------------------------------------------------------------------
+void g2p(orc_scene* s) {
+    const int64_t N = s->np;
+    if (s->method == ORC_FLIP) {
+        for (int c = 0; c < 3; ++c) {  // delta = new - snapshot (:138-150)
+            Arr3& a = comp_arr(s, c);
+            Arr3& r = comp_res(s, c);
+            for (size_t i = 0; i < a.size(); ++i) r.d[i] = a.d[i] - r.d[i];
+        }
+    }
+    if (s->method == ORC_APIC) s->C.assign(9 * N, 0.0);  // :129-134
+    const double hh[3] = {s->h[0] / 2.0, s->h[1] / 2.0, s->h[2] / 2.0};
+    int cell[3] = {s->n[0], s->n[1], s->n[2]};
+    par_for(s->nthreads, N, [&](int64_t p0, int64_t p1) {
+        for (int64_t pi = p0; pi < p1; ++pi) {
+            const double* pos = &s->x[3 * pi];
+            double* vel = &s->vel[3 * pi];
+            double pic[3];
+            sample_vel(s, s->u, s->v, s->w, pos, pic);
+            if (s->method == ORC_PIC) {
+                vel[0] = pic[0]; vel[1] = pic[1]; vel[2] = pic[2];
+            } else if (s->method == ORC_FLIP) {
+                double d[3];
+                sample_vel(s, s->ru, s->rv, s->rw, pos, d);
+                double fl[3] = {vel[0] + d[0], vel[1] + d[1], vel[2] + d[2]};
+                if (s->blend > 0.0) {
+                    // Lerp<Vector3>(flip, pic, blend): (1-b)*flip + b*pic (src/common/math_utils.hpp:30-34)
+                    for (int a = 0; a < 3; ++a) fl[a] = (1 - s->blend) * fl[a] + s->blend * pic[a];
+                }
+                vel[0] = fl[0]; vel[1] = fl[1]; vel[2] = fl[2];
+            } else {
+                vel[0] = pic[0]; vel[1] = pic[1]; vel[2] = pic[2];
+                GradStencil gs;
+                for (int c = 0; c < 3; ++c) {
+                    double pc[3] = {pos[0], pos[1], pos[2]};
+                    for (int ax = 0; ax < 3; ++ax)
+                        if (ax != c)
+                            pc[ax] = clamp_ref(pc[ax], s->origin[ax] + s->size[ax] - hh[ax], s->origin[ax] + hh[ax]);
+                    const Arr3& a = comp_arr(s, c);
+                    coords_grad_weights(cell, comp_origin(s, c), s->h, pc, &gs);
+                    double* cr = &s->C[9 * pi + 3 * c];
+                    for (int q = 0; q < 8; ++q) {
+                        double val = a(gs.idx[q][0], gs.idx[q][1], gs.idx[q][2]);
+                        cr[0] += gs.g[q][0] * val;
+                        cr[1] += gs.g[q][1] * val;
+                        cr[2] += gs.g[q][2] * val;
+                    }
+                }
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// MoveParticles, src/pic_simulator.cpp:239-312.  Collider resolution (:297-304)
+// is only reached with colliders, which this restatement does not model.
+// ---------------------------------------------------------------------------
+void move_particles(orc_scene* s, double dt_total) {
+    const int64_t N = s->np;
+    unsigned numSub = static_cast<unsigned>(std::max(s->maxCfl, 1.0));
+    double dt = dt_total / numSub;
+    par_for(s->nthreads, N, [&](int64_t p0, int64_t p1) {
+        for (int64_t pi = p0; pi < p1; ++pi) {
+            double* pos = &s->x[3 * pi];
+            double* vel = &s->vel[3 * pi];
+            double pt0[3] = {pos[0], pos[1], pos[2]}, pt1[3] = {pos[0], pos[1], pos[2]};
+            for (unsigned t = 0; t < numSub; ++t) {
+                double v0[3], mid[3], mv[3];
+                sample_vel(s, s->u, s->v, s->w, pt0, v0);
+                for (int a = 0; a < 3; ++a) mid[a] = pt0[a] + 0.5 * dt * v0[a];
+                sample_vel(s, s->u, s->v, s->w, mid, mv);
+                for (int a = 0; a < 3; ++a) pt1[a] = pt0[a] + dt * mv[a];
+                for (int a = 0; a < 3; ++a) pt0[a] = pt1[a];
+            }
+            for (int a = 0; a < 3; ++a) {
+                if (pt1[a] <= s->origin[a]) {
+                    pt1[a] = s->origin[a];
+                    vel[a] = 0.0;
+                }
+                if (pt1[a] >= s->origin[a] + s->size[a]) {
+                    pt1[a] = s->origin[a] + s->size[a];
+                    vel[a] = 0.0;
+                }
+            }
+            pos[0] = pt1[0]; pos[1] = pt1[1]; pos[2] = pt1[2];
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// BuildSignedDistanceField, src/pic_simulator.cpp:452-475, with the searcher of
+// src/particle_systems/hash_grid_particle_searcher.cpp:97-204 (64^3 wrapped
+// buckets of width 2r, 8-bucket query).  Restated with an exact hash grid over
+// the same bucket geometry so the candidate set equals the reference's.
+// ---------------------------------------------------------------------------
+struct HashGrid {
+    double spacing;
+    std::vector<std::vector<int64_t>> buckets;
+    static constexpr int S = 64;  // src/particle_systems/particle_system.cpp:30-34
+    static int wrap(int b) {
+        // hash_grid_particle_searcher.cpp:142-152: `int % size_t` converts the int to
+        // size_t first; for S = 64 (a power of two) that equals the mathematical mod.
+        int r = (int)((size_t)(int64_t)b % (size_t)S);
+        if (r < 0) r += S;
+        return r;
+    }
+    size_t key(int bx, int by, int bz) const { return ((size_t)wrap(bz) * S + wrap(by)) * S + wrap(bx); }
+    void bucket_index(const double p[3], int b[3]) const {
+        for (int a = 0; a < 3; ++a) b[a] = static_cast<int>(std::floor(p[a] / spacing));
+    }
+    void build(const std::vector<double>& x, int64_t N, double sp) {
+        spacing = sp;
+        buckets.assign((size_t)S * S * S, {});
+        for (int64_t i = 0; i < N; ++i) {
+            int b[3];
+            bucket_index(&x[3 * i], b);
+            buckets[key(b[0], b[1], b[2])].push_back(i);
+        }
+    }
+    void nearby_keys(const double p[3], size_t keys[8]) const {
+        int o[3];
+        bucket_index(p, o);
+        int nb[8][3];
+        for (int i = 0; i < 8; ++i) { nb[i][0] = o[0]; nb[i][1] = o[1]; nb[i][2] = o[2]; }
+        // hash_grid_particle_searcher.cpp:164-198 (note the float 0.5f)
+        int dx = ((o[0] + 0.5f) * spacing <= p[0]) ? 1 : -1;
+        int dy = ((o[1] + 0.5f) * spacing <= p[1]) ? 1 : -1;
+        int dz = ((o[2] + 0.5f) * spacing <= p[2]) ? 1 : -1;
+        nb[4][0] += dx; nb[5][0] += dx; nb[6][0] += dx; nb[7][0] += dx;
+        nb[2][1] += dy; nb[3][1] += dy; nb[6][1] += dy; nb[7][1] += dy;
+        nb[1][2] += dz; nb[3][2] += dz; nb[5][2] += dz; nb[7][2] += dz;
+        for (int i = 0; i < 8; ++i) keys[i] = key(nb[i][0], nb[i][1], nb[i][2]);
+    }
+};
+
+void build_sdf(orc_scene* s) {
+    const double maxH = std::max(s->h[0], std::max(s->h[1], s->h[2]));
+    const double radius = 1.2 * maxH / std::sqrt(2.0);
+    const double band = 2.0 * radius;
+    HashGrid hg;
+    hg.build(s->x, s->np, 2 * radius);
+    const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    const double qr2 = band * band;
+    par_for(s->nthreads, nz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    // Grid3D::GridIndexToPosition (src/common/grid3d.hpp:13-16)
+                    double gp[3] = {so[0] + i * s->h[0], so[1] + j * s->h[1], so[2] + k * s->h[2]};
+                    double minD = band;
+                    size_t keys[8];
+                    hg.nearby_keys(gp, keys);
+                    for (int q = 0; q < 8; ++q) {
+                        for (int64_t pi : hg.buckets[keys[q]]) {
+                            const double* p = &s->x[3 * pi];
+                            double dx = p[0] - gp[0], dy = p[1] - gp[1], dz = p[2] - gp[2];
+                            double d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 <= qr2) minD = std::min(minD, std::sqrt(dx * dx + dy * dy + dz * dz));
+                        }
+                    }
+                    s->sdf(i, j, k) = minD - radius;
+                }
+    });
+    // ExtrapolateIntoCollider (src/pic_simulator.cpp:497-519): valid = colliderSdf >= 0
+    int depth = static_cast<int>(std::ceil(s->maxCfl));
+    Arr3i valid;
+    valid.resize(nx, ny, nz, 1);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double gp[3] = {so[0] + i * s->h[0], so[1] + j * s->h[1], so[2] + k * s->h[2]};
+                valid(i, j, k) = sample_collider(s, gp) < 0 ? 0 : 1;
+            }
+    Arr3 prev = s->sdf;
+    extrapolate_to_region(prev, valid, depth, s->sdf, s->nthreads);
+}
+
+// ExtrapolateVelocityToAir, src/pic_simulator.cpp:477-495 (validity = P2G face markers)
+void extrapolate_to_air(orc_scene* s) {
+    int iters = static_cast<int>(std::ceil(s->maxCfl));
+    for (int c = 0; c < 3; ++c) {
+        Arr3& a = comp_arr(s, c);
+        Arr3 prev = a;
+        extrapolate_to_region(prev, comp_mark(s, c), iters, a, s->nthreads);
+    }
+}
+
+// FractionInsideSdf, src/fluid_solvers/blocked_boundry_condition_solver.cpp:304-322
+double fraction_inside(double phi0, double phi1) {
+    if (phi0 < 0.0 && phi1 < 0.0) return 1;
+    if (phi0 < 0.0 && !(phi1 < 0.0)) return phi0 / (phi0 - phi1);
+    if (!(phi0 < 0.0) && phi1 < 0.0) return phi1 / (phi1 - phi0);
+    return 0;
+}
+
+// BlockedBoundryConditionSolver::ConstrainVelocity,
+// src/fluid_solvers/blocked_boundry_condition_solver.cpp:18-291.  With no
+// collider inside the domain (collider SDF >= 0 everywhere) the face-fraction
+// pass marks every face valid, the extrapolation is the identity and the
+// friction pass copies; what remains is the wall zeroing (:198-244).  The
+// collider branches are restated for an SDF that is >= 0 only; a negative
+// collider SDF is rejected by the caller.
+void constrain_velocity(orc_scene* s, int depth) {
+    for (int c = 0; c < 3; ++c) {
+        Arr3& a = comp_arr(s, c);
+        Arr3i marker;
+        marker.resize(a.sx, a.sy, a.sz, 1);
+        const double* co = comp_origin(s, c);
+        bool any_blocked = false;
+        for (int k = 0; k < a.sz; ++k)
+            for (int j = 0; j < a.sy; ++j)
+                for (int i = 0; i < a.sx; ++i) {
+                    double pos[3] = {co[0] + s->h[0] * i, co[1] + s->h[1] * j, co[2] + s->h[2] * k};
+                    double p0[3] = {pos[0], pos[1], pos[2]}, p1[3] = {pos[0], pos[1], pos[2]};
+                    p0[c] -= 0.5 * s->h[c];
+                    p1[c] += 0.5 * s->h[c];
+                    double frac = fraction_inside(sample_collider(s, p0), sample_collider(s, p1));
+                    frac = 1 - clamp_ref(frac, 1.0, 0.0);
+                    if (!(frac > 0.0)) {
+                        a(i, j, k) = 0.0;
+                        marker(i, j, k) = 0;
+                        any_blocked = true;
+                    }
+                }
+        if (any_blocked) {
+            Arr3 prev = a;
+            extrapolate_to_region(prev, marker, depth, a, s->nthreads);
+        }
+    }
+    // walls (:198-244)
+    for (int k = 0; k < s->u.sz; ++k)
+        for (int j = 0; j < s->u.sy; ++j) { s->u(0, j, k) = 0; s->u(s->u.sx - 1, j, k) = 0; }
+    for (int k = 0; k < s->v.sz; ++k)
+        for (int i = 0; i < s->v.sx; ++i) { s->v(i, 0, k) = 0; s->v(i, s->v.sy - 1, k) = 0; }
+    for (int j = 0; j < s->w.sy; ++j)
+        for (int i = 0; i < s->w.sx; ++i) { s->w(i, j, 0) = 0; s->w(i, j, s->w.sz - 1) = 0; }
+}
+
+// ExternalForce::ApplyExternalForce with a DirectionalField (src/forces/external_force.cpp:4-30,
+// src/forces/directional_field.cpp:13-16)
+void apply_gravity(orc_scene* s, double dt) {
+    for (int c = 0; c < 3; ++c) {
+        Arr3& a = comp_arr(s, c);
+        const double g = s->gravity[c];
+        par_for(s->nthreads, (int64_t)a.size(), [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; ++i) a.d[i] += dt * g;
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SinglePhasePressureSolver, src/fluid_solvers/single_phase_pressure_solver.cpp
+// ---------------------------------------------------------------------------
+// BuildMarkers (:30-48): FLUID if fluidSdf(cell centre) < 0, else BOUNDRY if
+// colliderSdf(cell centre) < 0, else AIR.  Both are trilinear samples
+// (ScalarGrid3D::Sample) at GetCellCenterPos (face_centered_grid3d.cpp:318-321).
+void build_markers(orc_scene* s) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    s->flags.assign((size_t)nx * ny * nz, ORC_AIR);
+    const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
+    par_for(s->nthreads, nz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double cc[3];
+                    for (int a = 0; a < 3; ++a) {
+                        int id = a == 0 ? i : (a == 1 ? j : k);
+                        cc[a] = s->origin[a] + 0.5 * s->h[a] + s->h[a] * id;
+                    }
+                    size_t id = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+                    if (sample_array(s->sdf, so, s->h, cc) < 0)
+                        s->flags[id] = ORC_FLUID;
+                    else if (sample_collider(s, cc) < 0)
+                        s->flags[id] = ORC_BOUNDRY;
+                    else
+                        s->flags[id] = ORC_AIR;
+                }
+    });
+}
+
+inline uint8_t flag_at(const orc_scene* s, int i, int j, int k) {
+    return s->flags[(size_t)i + (size_t)s->n[0] * ((size_t)j + (size_t)s->n[1] * k)];
+}
+
+// FaceCenteredGrid3D::DivergenceAtCallCenter (face_centered_grid3d.cpp:157-166)
+inline double divergence(const orc_scene* s, int i, int j, int k) {
+    double left = s->u(i, j, k), right = s->u(i + 1, j, k);
+    double down = s->v(i, j, k), up = s->v(i, j + 1, k);
+    double back = s->w(i, j, k), front = s->w(i, j, k + 1);
+    return (right - left) / s->h[0] + (up - down) / s->h[1] + (front - back) / s->h[2];
+}
+
+// BuildSystem (:99-164)
+void build_system(orc_scene* s) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    const double ihx2 = (1.0 / s->h[0]) * (1.0 / s->h[0]);
+    const double ihy2 = (1.0 / s->h[1]) * (1.0 / s->h[1]);
+    const double ihz2 = (1.0 / s->h[2]) * (1.0 / s->h[2]);
+    // LinearSystem::Resize keeps x (warm start); A and b are rebuilt.
+    if (s->p.sx != nx || s->p.sy != ny || s->p.sz != nz) s->p.resize(nx, ny, nz, 0.0);
+    s->b.resize(nx, ny, nz, 0.0);
+    s->Ac.resize(nx, ny, nz, 0.0); s->Ar.resize(nx, ny, nz, 0.0);
+    s->Au.resize(nx, ny, nz, 0.0); s->Af.resize(nx, ny, nz, 0.0);
+    par_for(s->nthreads, nz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double c = 0, r = 0, u = 0, f = 0, bb = 0;
+                    if (flag_at(s, i, j, k) == ORC_FLUID) {
+                        bb = divergence(s, i, j, k);
+                        if (i + 1 < nx && flag_at(s, i + 1, j, k) != ORC_BOUNDRY) {
+                            c += ihx2;
+                            if (flag_at(s, i + 1, j, k) == ORC_FLUID) r -= ihx2;
+                        }
+                        if (i > 0 && flag_at(s, i - 1, j, k) != ORC_BOUNDRY) c += ihx2;
+                        if (j + 1 < ny && flag_at(s, i, j + 1, k) != ORC_BOUNDRY) {
+                            c += ihy2;
+                            if (flag_at(s, i, j + 1, k) == ORC_FLUID) u -= ihy2;
+                        }
+                        if (j > 0 && flag_at(s, i, j - 1, k) != ORC_BOUNDRY) c += ihy2;
+                        if (k + 1 < nz && flag_at(s, i, j, k + 1) != ORC_BOUNDRY) {
+                            c += ihz2;
+                            if (flag_at(s, i, j, k + 1) == ORC_FLUID) f -= ihz2;
+                        }
+                        if (k > 0 && flag_at(s, i, j, k - 1) != ORC_BOUNDRY) c += ihz2;
+                    } else {
+                        c = 1.0;
+                    }
+                    s->Ac(i, j, k) = c; s->Ar(i, j, k) = r; s->Au(i, j, k) = u; s->Af(i, j, k) = f;
+                    s->b(i, j, k) = bb;
+                }
+    });
+}
+
+// BLAS::MatrixVectorMultiplication (src/linear_system/blas.cpp:72-83)
+void spmv(const orc_scene* s, const Arr3& v, Arr3& out, int nthreads) {
+    const int nx = v.sx, ny = v.sy, nz = v.sz;
+    par_for(nthreads, nz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    out(i, j, k) = s->Ac(i, j, k) * v(i, j, k) +
+                                   ((i > 0) ? s->Ar(i - 1, j, k) * v(i - 1, j, k) : 0.0) +
+                                   ((i + 1 < nx) ? s->Ar(i, j, k) * v(i + 1, j, k) : 0.0) +
+                                   ((j > 0) ? s->Au(i, j - 1, k) * v(i, j - 1, k) : 0.0) +
+                                   ((j + 1 < ny) ? s->Au(i, j, k) * v(i, j + 1, k) : 0.0) +
+                                   ((k > 0) ? s->Af(i, j, k - 1) * v(i, j, k - 1) : 0.0) +
+                                   ((k + 1 < nz) ? s->Af(i, j, k) * v(i, j, k + 1) : 0.0);
+                }
+    });
+}
+
+// BLAS::Residual (src/linear_system/blas.cpp:21-37)
+void residual(const orc_scene* s, const Arr3& x, const Arr3& b, Arr3& out, int nthreads) {
+    const int nx = x.sx, ny = x.sy, nz = x.sz;
+    par_for(nthreads, nz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    out(i, j, k) = b(i, j, k) - s->Ac(i, j, k) * x(i, j, k) -
+                                   ((i > 0) ? s->Ar(i - 1, j, k) * x(i - 1, j, k) : 0.0) -
+                                   ((i + 1 < nx) ? s->Ar(i, j, k) * x(i + 1, j, k) : 0.0) -
+                                   ((j > 0) ? s->Au(i, j - 1, k) * x(i, j - 1, k) : 0.0) -
+                                   ((j + 1 < ny) ? s->Au(i, j, k) * x(i, j + 1, k) : 0.0) -
+                                   ((k > 0) ? s->Af(i, j, k - 1) * x(i, j, k - 1) : 0.0) -
+                                   ((k + 1 < nz) ? s->Af(i, j, k) * x(i, j, k + 1) : 0.0);
+                }
+    });
+}
+
+// BLAS::Dot (src/linear_system/blas.cpp:4-19): serial, i-outer / k-inner order.
+double dot(const Arr3& a, const Arr3& b) {
+    double r = 0;
+    for (int i = 0; i < a.sx; ++i)
+        for (int j = 0; j < a.sy; ++j)
+            for (int k = 0; k < a.sz; ++k) r += a(i, j, k) * b(i, j, k);
+    return r;
+}
+
+// JacobiIterationSolver (src/linear_system/jacobi_iteration_solver.cpp:14-76)
+void jacobi(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    Arr3 xt, res;
+    xt.resize(nx, ny, nz, 0.0);
+    res.resize(nx, ny, nz, 0.0);
+    int it = 0;
+    int executed = 0;
+    for (it = 0; it < d->max_iterations; ++it) {
+        Arr3& x = s->p;
+        par_for(s->nthreads, nz, [&](int64_t k0, int64_t k1) {
+            for (int k = (int)k0; k < (int)k1; ++k)
+                for (int j = 0; j < ny; ++j)
+                    for (int i = 0; i < nx; ++i) {
+                        double r = ((i > 0) ? s->Ar(i - 1, j, k) * x(i - 1, j, k) : 0.0) +
+                                   ((i + 1 < nx) ? s->Ar(i, j, k) * x(i + 1, j, k) : 0.0) +
+                                   ((j > 0) ? s->Au(i, j - 1, k) * x(i, j - 1, k) : 0.0) +
+                                   ((j + 1 < ny) ? s->Au(i, j, k) * x(i, j + 1, k) : 0.0) +
+                                   ((k > 0) ? s->Af(i, j, k - 1) * x(i, j, k - 1) : 0.0) +
+                                   ((k + 1 < nz) ? s->Af(i, j, k) * x(i, j, k + 1) : 0.0);
+                        xt(i, j, k) = (s->b(i, j, k) - r) / s->Ac(i, j, k);
+                    }
+        });
+        std::swap(s->p.d, xt.d);
+        executed = it + 1;
+        if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
+            residual(s, s->p, s->b, res, s->nthreads);
+            double nrm = std::sqrt(dot(res, res));
+            if (nrm < d->tolerance) break;
+        }
+    }
+    st->iterations = executed;
+}
+
+// ConjugateGradientSolver::Solve (src/linear_system/conjugate_gradient_solver.cpp:14-55),
+// with the identity "preconditioner" (:57-63) or, as a build addition, the
+// diagonal (Jacobi) preconditioner.  Stopping rule: the reference's absolute
+// sigma <= tol^2, or (relative mode) ||r||_2 <= tol * ||b||_2.
+void cg(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    Arr3 r, dd, q, z;
+    r.resize(nx, ny, nz); dd.resize(nx, ny, nz); q.resize(nx, ny, nz); z.resize(nx, ny, nz);
+    const bool precond = d->kind == ORC_SOLVER_PCG_JACOBI;
+    auto apply_m = [&](const Arr3& in, Arr3& out) {
+        if (!precond) { out = in; return; }
+        for (size_t i = 0; i < in.size(); ++i) out.d[i] = in.d[i] / s->Ac.d[i];
+    };
+    if (!d->warm_start) std::fill(s->p.d.begin(), s->p.d.end(), 0.0);
+    residual(s, s->p, s->b, r, s->nthreads);
+    apply_m(r, dd);
+    double sigmaNew = dot(r, dd);
+    const double bnorm = std::sqrt(dot(s->b, s->b));
+    auto done = [&](double sig, const Arr3& rr) {
+        if (d->relative) {
+            double rn = precond ? std::sqrt(dot(rr, rr)) : std::sqrt(std::max(sig, 0.0));
+            return rn <= d->tolerance * bnorm;
+        }
+        return !(sig > d->tolerance * d->tolerance);
+    };
+    int it = 0;
+    bool trigger = false;
+    while (!done(sigmaNew, r) && it < d->max_iterations) {
+        spmv(s, dd, q, s->nthreads);
+        double alpha = sigmaNew / dot(dd, q);
+        for (size_t i = 0; i < s->p.size(); ++i) s->p.d[i] = alpha * dd.d[i] + s->p.d[i];
+        if (trigger || (it % 50 == 0 && it > 0)) {
+            residual(s, s->p, s->b, r, s->nthreads);
+            trigger = false;
+        } else {
+            for (size_t i = 0; i < r.size(); ++i) r.d[i] = -alpha * q.d[i] + r.d[i];
+        }
+        apply_m(r, z);
+        double sigmaOld = sigmaNew;
+        sigmaNew = dot(r, z);
+        if (sigmaNew > sigmaOld) trigger = true;
+        double beta = sigmaNew / sigmaOld;
+        for (size_t i = 0; i < dd.size(); ++i) dd.d[i] = beta * dd.d[i] + z.d[i];
+        ++it;
+    }
+    st->iterations = it;
+}
+
+// ApplyPressure (:56-92)
+void apply_pressure(orc_scene* s) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    const double scx = 1.0 / s->h[0], scy = 1.0 / s->h[1], scz = 1.0 / s->h[2];
+    Arr3 iu = s->u, iv = s->v, iw = s->w;
+    const Arr3& P = s->p;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                if (flag_at(s, i, j, k) != ORC_FLUID) continue;
+                if (i + 1 < nx && flag_at(s, i + 1, j, k) != ORC_BOUNDRY)
+                    s->u(i + 1, j, k) = iu(i + 1, j, k) + scx * (P(i + 1, j, k) - P(i, j, k));
+                if (j + 1 < ny && flag_at(s, i, j + 1, k) != ORC_BOUNDRY)
+                    s->v(i, j + 1, k) = iv(i, j + 1, k) + scy * (P(i, j + 1, k) - P(i, j, k));
+                if (k + 1 < nz && flag_at(s, i, j, k + 1) != ORC_BOUNDRY)
+                    s->w(i, j, k + 1) = iw(i, j, k + 1) + scz * (P(i, j, k + 1) - P(i, j, k));
+            }
+}
+
+void pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
+    auto t0 = std::chrono::steady_clock::now();
+    build_markers(s);
+    build_system(s);
+    orc_solve_stats local{};
+    if (d->kind == ORC_SOLVER_JACOBI)
+        jacobi(s, d, &local);
+    else
+        cg(s, d, &local);
+    // final residual in FP64
+    Arr3 r;
+    r.resize(s->n[0], s->n[1], s->n[2]);
+    residual(s, s->p, s->b, r, s->nthreads);
+    double rn = std::sqrt(dot(r, r)), bn = std::sqrt(dot(s->b, s->b));
+    apply_pressure(s);
+    auto t1 = std::chrono::steady_clock::now();
+    local.abs_residual = rn;
+    local.rel_residual = bn > 0 ? rn / bn : rn;
+    local.ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    s->last = local;
+    if (st) *st = local;
+}
+
+void apply_bc(orc_scene* s) { constrain_velocity(s, static_cast<int>(std::ceil(s->maxCfl))); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+orc_scene* orc_create(const int n[3], const double origin[3], const double size[3], int method, double blend,
+                      double max_cfl, const double gravity[3]) {
+    auto* s = new orc_scene();
+    for (int a = 0; a < 3; ++a) {
+        s->n[a] = n[a];
+        s->origin[a] = origin[a];
+        s->size[a] = size[a];
+        s->h[a] = size[a] / n[a];  // src/hybrid_simulator.cpp:7
+        s->gravity[a] = gravity[a];
+    }
+    s->method = method;
+    s->blend = blend;
+    s->maxCfl = max_cfl;
+    face_origins(s);
+    s->u.resize(n[0] + 1, n[1], n[2]); s->v.resize(n[0], n[1] + 1, n[2]); s->w.resize(n[0], n[1], n[2] + 1);
+    s->mu.resize(n[0] + 1, n[1], n[2]); s->mv.resize(n[0], n[1] + 1, n[2]); s->mw.resize(n[0], n[1], n[2] + 1);
+    s->ru = s->u; s->rv = s->v; s->rw = s->w;
+    s->sdf.resize(n[0], n[1], n[2], 0.0);
+    s->csdf.resize(n[0], n[1], n[2], kMaxDistance);  // after BuildSdf with no colliders
+    s->p.resize(n[0], n[1], n[2], 0.0);
+    s->b.resize(n[0], n[1], n[2], 0.0);
+    s->flags.assign((size_t)n[0] * n[1] * n[2], ORC_AIR);
+    return s;
+}
+
+void orc_destroy(orc_scene* s) { delete s; }
+void orc_set_threads(orc_scene* s, int nthreads) { s->nthreads = std::max(1, nthreads); }
+
+void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np) {
+    s->np = np;
+    s->x.assign(x, x + 3 * np);
+    s->vel.assign(v, v + 3 * np);
+    if (C)
+        s->C.assign(C, C + 9 * np);
+    else
+        s->C.assign(9 * np, 0.0);
+}
+
+void orc_get_particles(const orc_scene* s, double* x, double* v, double* C) {
+    if (x) std::memcpy(x, s->x.data(), sizeof(double) * 3 * s->np);
+    if (v) std::memcpy(v, s->vel.data(), sizeof(double) * 3 * s->np);
+    if (C) {
+        if ((int64_t)s->C.size() == 9 * s->np)
+            std::memcpy(C, s->C.data(), sizeof(double) * 9 * s->np);
+        else
+            std::memset(C, 0, sizeof(double) * 9 * s->np);
+    }
+}
+
+int64_t orc_num_particles(const orc_scene* s) { return s->np; }
+
+static Arr3* grid_field(orc_scene* s, int what) {
+    switch (what) {
+        case ORC_U: return &s->u;
+        case ORC_V: return &s->v;
+        case ORC_W: return &s->w;
+        case ORC_RU: return &s->ru;
+        case ORC_RV: return &s->rv;
+        case ORC_RW: return &s->rw;
+        case ORC_SDF: return &s->sdf;
+        case ORC_PRESSURE: return &s->p;
+        case ORC_RHS: return &s->b;
+        case ORC_COLLIDER_SDF: return &s->csdf;
+        default: return nullptr;
+    }
+}
+
+static Arr3i* grid_marker(orc_scene* s, int what) {
+    switch (what) {
+        case ORC_MU: return &s->mu;
+        case ORC_MV: return &s->mv;
+        case ORC_MW: return &s->mw;
+        default: return nullptr;
+    }
+}
+
+int64_t orc_grid_count(orc_scene* s, int what) {
+    if (Arr3* a = grid_field(s, what)) return (int64_t)a->size();
+    if (Arr3i* m = grid_marker(s, what)) return (int64_t)m->d.size();
+    if (what == ORC_FLAGS) return (int64_t)s->flags.size();
+    return -1;
+}
+
+// doubles for fields, int32 for markers, uint8 for cell flags
+int orc_get_grid(orc_scene* s, int what, void* out) {
+    if (Arr3* a = grid_field(s, what)) {
+        std::memcpy(out, a->d.data(), sizeof(double) * a->size());
+        return 0;
+    }
+    if (Arr3i* m = grid_marker(s, what)) {
+        std::memcpy(out, m->d.data(), sizeof(int) * m->d.size());
+        return 0;
+    }
+    if (what == ORC_FLAGS) {
+        std::memcpy(out, s->flags.data(), s->flags.size());
+        return 0;
+    }
+    return -1;
+}
+
+int orc_set_grid(orc_scene* s, int what, const void* in) {
+    if (Arr3* a = grid_field(s, what)) {
+        std::memcpy(a->d.data(), in, sizeof(double) * a->size());
+        return 0;
+    }
+    if (Arr3i* m = grid_marker(s, what)) {
+        std::memcpy(m->d.data(), in, sizeof(int) * m->d.size());
+        return 0;
+    }
+    if (what == ORC_FLAGS) {
+        std::memcpy(s->flags.data(), in, s->flags.size());
+        return 0;
+    }
+    return -1;
+}
+
+void orc_p2g(orc_scene* s) { p2g(s); }
+void orc_build_sdf(orc_scene* s) { build_sdf(s); }
+void orc_build_markers(orc_scene* s) { build_markers(s); }
+void orc_extrapolate_to_air(orc_scene* s) { extrapolate_to_air(s); }
+void orc_apply_bc(orc_scene* s) { apply_bc(s); }
+void orc_apply_gravity(orc_scene* s, double dt) { apply_gravity(s, dt); }
+void orc_build_system(orc_scene* s) { build_system(s); }
+void orc_apply_pressure(orc_scene* s) { apply_pressure(s); }
+void orc_g2p(orc_scene* s) { g2p(s); }
+void orc_move_particles(orc_scene* s, double dt) { move_particles(s, dt); }
+
+void orc_pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) { pressure(s, d, st); }
+
+// The substep: PICSimulator::OnAdvanceTimeStep (src/pic_simulator.cpp:111-154) with
+// OnBeginAdvanceTimeStep (:156-196), ComputeExternalForces (:203-212), ComputeDiffusion
+// (:214-222, skipped at viscosity 0), ComputePressure (:224-229), ComputeAdvection (:231-237).
+void orc_substep(orc_scene* s, double dt, const orc_solver_desc* d, orc_solve_stats* st) {
+    p2g(s);
+    build_sdf(s);
+    extrapolate_to_air(s);
+    apply_bc(s);
+    apply_gravity(s, dt);
+    apply_bc(s);
+    pressure(s, d, st);
+    apply_bc(s);
+    extrapolate_to_air(s);
+    apply_bc(s);
+    g2p(s);
+    move_particles(s, dt);
+}
+
+// PICSimulator::Cfl (src/pic_simulator.cpp:80-97) with ValueAtCellCenter
+// (src/grid_systems/face_centered_grid3d.cpp:146-155)
+double orc_cfl(const orc_scene* s, double dt) {
+    double maxV = 0.0;
+    for (int i = 0; i < s->n[0]; ++i)
+        for (int j = 0; j < s->n[1]; ++j)
+            for (int k = 0; k < s->n[2]; ++k) {
+                double cx = 0.5 * (s->u(i, j, k) + s->u(i + 1, j, k));
+                double cy = 0.5 * (s->v(i, j, k) + s->v(i, j + 1, k));
+                double cz = 0.5 * (s->w(i, j, k) + s->w(i, j, k + 1));
+                double vx = std::abs(cx) + dt * 0.0;
+                double vy = std::abs(cy) + dt * -9.8;
+                double vz = std::abs(cz) + dt * 0.0;
+                maxV = std::max(maxV, vx);
+                maxV = std::max(maxV, vy);
+                maxV = std::max(maxV, vz);
+            }
+    double minH = std::min(s->h[0], std::min(s->h[1], s->h[2]));
+    return maxV * dt / minH;
+}
+
+// PICSimulator::NumberOfSubTimeSteps (src/pic_simulator.cpp:349-354)
+unsigned orc_num_substeps(const orc_scene* s, double dt) {
+    double c = orc_cfl(s, dt);
+    return static_cast<unsigned>(std::max(std::ceil(c / s->maxCfl), 1.0));
+}
+
+// PhysicsAnimation::AdvanceTimeStep (src/physics_animation.cpp:64-92)
+int orc_advance_frame(orc_scene* s, double frame_dt, const orc_solver_desc* d) {
+    double remaining = frame_dt;
+    int count = 0;
+    while (remaining > 0.0000000001) {
+        unsigned n = orc_num_substeps(s, remaining);
+        double sub = remaining / static_cast<double>(n);
+        orc_substep(s, sub, d, nullptr);
+        remaining -= sub;
+        ++count;
+    }
+    return count;
+}
+
+void orc_last_stats(const orc_scene* s, orc_solve_stats* st) { *st = s->last; }
+
+// ---------------------------------------------------------------------------
+// Primitive restatements exported for the known-answer tests.
+// ---------------------------------------------------------------------------
+void orc_barycentric(double x, int iLow, int iHigh, int* i, double* f) { barycentric(x, iLow, iHigh, i, f); }
+
+void orc_coords_weights(const int size[3], const double origin[3], const double h[3], const double x[3],
+                        int idx_out[24], double w_out[8]) {
+    Stencil st;
+    coords_weights(size, origin, h, x, &st);
+    std::memcpy(idx_out, st.idx, sizeof(int) * 24);
+    std::memcpy(w_out, st.w, sizeof(double) * 8);
+}
+
+void orc_coords_grad_weights(const int size[3], const double origin[3], const double h[3], const double x[3],
+                             int idx_out[24], double g_out[24]) {
+    GradStencil st;
+    coords_grad_weights(size, origin, h, x, &st);
+    std::memcpy(idx_out, st.idx, sizeof(int) * 24);
+    std::memcpy(g_out, st.g, sizeof(double) * 24);
+}
+
+double orc_trilerp(const double v[8], double fx, double fy, double fz) {
+    return trilerp(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
+}
+
+double orc_sample_array(const double* data, const int dims[3], const double origin[3], const double h[3],
+                        const double p[3]) {
+    Arr3 a;
+    a.sx = dims[0]; a.sy = dims[1]; a.sz = dims[2];
+    a.d.assign(data, data + (size_t)dims[0] * dims[1] * dims[2]);
+    return sample_array(a, origin, h, p);
+}
+
+void orc_extrapolate_to_region(const double* input, const int* valid, const int dims[3], int iterations,
+                               double* output) {
+    Arr3 in, out;
+    in.sx = dims[0]; in.sy = dims[1]; in.sz = dims[2];
+    in.d.assign(input, input + (size_t)dims[0] * dims[1] * dims[2]);
+    Arr3i va;
+    va.sx = dims[0]; va.sy = dims[1]; va.sz = dims[2];
+    va.d.assign(valid, valid + va.sx * (size_t)va.sy * va.sz);
+    extrapolate_to_region(in, va, iterations, out, 1);
+    std::memcpy(output, out.d.data(), sizeof(double) * out.size());
+}
+
+// Nearby-point query of the restated hash grid (for the reference searcher tests).
+int orc_hash_grid_query(const double* pts, int64_t npts, int sx, double spacing, const double q[3],
+                        double radius, int64_t* out, int max_out) {
+    // The reference's HashGridParticleSearcher with an arbitrary bucket count `sx`
+    // per axis (its unit tests use 4, 6, 7).
+    struct G {
+        int S;
+        double sp;
+        std::vector<std::vector<int64_t>> b;
+        int wrap(int v) const {
+            int r = (int)((size_t)(int64_t)v % (size_t)S);
+            if (r < 0) r += S;
+            return r;
+        }
+        size_t key(int x, int y, int z) const { return ((size_t)wrap(z) * S + wrap(y)) * S + wrap(x); }
+    } g{sx, spacing, {}};
+    g.b.assign((size_t)sx * sx * sx, {});
+    for (int64_t i = 0; i < npts; ++i) {
+        int bx = (int)std::floor(pts[3 * i] / spacing), by = (int)std::floor(pts[3 * i + 1] / spacing),
+            bz = (int)std::floor(pts[3 * i + 2] / spacing);
+        g.b[g.key(bx, by, bz)].push_back(i);
+    }
+    int o[3] = {(int)std::floor(q[0] / spacing), (int)std::floor(q[1] / spacing), (int)std::floor(q[2] / spacing)};
+    int nb[8][3];
+    for (int i = 0; i < 8; ++i) { nb[i][0] = o[0]; nb[i][1] = o[1]; nb[i][2] = o[2]; }
+    int dx = ((o[0] + 0.5f) * spacing <= q[0]) ? 1 : -1;
+    int dy = ((o[1] + 0.5f) * spacing <= q[1]) ? 1 : -1;
+    int dz = ((o[2] + 0.5f) * spacing <= q[2]) ? 1 : -1;
+    nb[4][0] += dx; nb[5][0] += dx; nb[6][0] += dx; nb[7][0] += dx;
+    nb[2][1] += dy; nb[3][1] += dy; nb[6][1] += dy; nb[7][1] += dy;
+    nb[1][2] += dz; nb[3][2] += dz; nb[5][2] += dz; nb[7][2] += dz;
+    int count = 0;
+    const double r2 = radius * radius;
+    for (int i = 0; i < 8; ++i) {
+        for (int64_t pi : g.b[g.key(nb[i][0], nb[i][1], nb[i][2])]) {
+            double ddx = pts[3 * pi] - q[0], ddy = pts[3 * pi + 1] - q[1], ddz = pts[3 * pi + 2] - q[2];
+            if (ddx * ddx + ddy * ddy + ddz * ddz <= r2) {
+                if (count < max_out) out[count] = pi;
+                ++count;
+            }
+        }
+    }
+    return count;
+}
+
+// BLAS primitives on plain x-fastest arrays (src/linear_system/blas.cpp:4-83)
+double orc_blas_dot(const double* a, const double* b, const int dims[3]) {
+    Arr3 A, B;
+    A.sx = B.sx = dims[0]; A.sy = B.sy = dims[1]; A.sz = B.sz = dims[2];
+    size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    A.d.assign(a, a + n);
+    B.d.assign(b, b + n);
+    return dot(A, B);
+}
+
+// Solve a 7-point system given explicitly as {center,right,up,front} rows,
+// with the Jacobi or CG restatement.  x is read as the initial guess.
+void orc_solve_rows(const double* Ac, const double* Ar, const double* Au, const double* Af, const double* b,
+                    double* x, const int dims[3], const orc_solver_desc* d, orc_solve_stats* st) {
+    orc_scene s{};
+    s.n[0] = dims[0]; s.n[1] = dims[1]; s.n[2] = dims[2];
+    size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    auto fill = [&](Arr3& a, const double* src) {
+        a.sx = dims[0]; a.sy = dims[1]; a.sz = dims[2];
+        a.d.assign(src, src + n);
+    };
+    fill(s.Ac, Ac); fill(s.Ar, Ar); fill(s.Au, Au); fill(s.Af, Af); fill(s.b, b); fill(s.p, x);
+    orc_solve_stats local{};
+    if (d->kind == ORC_SOLVER_JACOBI)
+        jacobi(&s, d, &local);
+    else
+        cg(&s, d, &local);
+    std::memcpy(x, s.p.d.data(), sizeof(double) * n);
+    if (st) *st = local;
+}
+
+void orc_residual_rows(const double* Ac, const double* Ar, const double* Au, const double* Af, const double* b,
+                       const double* x, const int dims[3], double* out) {
+    orc_scene s{};
+    size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    auto fill = [&](Arr3& a, const double* src) {
+        a.sx = dims[0]; a.sy = dims[1]; a.sz = dims[2];
+        a.d.assign(src, src + n);
+    };
+    Arr3 X, B, R;
+    fill(s.Ac, Ac); fill(s.Ar, Ar); fill(s.Au, Au); fill(s.Af, Af); fill(B, b); fill(X, x);
+    R = X;
+    residual(&s, X, B, R, 1);
+    std::memcpy(out, R.d.data(), sizeof(double) * n);
+}
+
+void orc_spmv_rows(const double* Ac, const double* Ar, const double* Au, const double* Af, const double* v,
+                   const int dims[3], double* out) {
+    orc_scene s{};
+    size_t n = (size_t)dims[0] * dims[1] * dims[2];
+    auto fill = [&](Arr3& a, const double* src) {
+        a.sx = dims[0]; a.sy = dims[1]; a.sz = dims[2];
+        a.d.assign(src, src + n);
+    };
+    Arr3 V, O;
+    fill(s.Ac, Ac); fill(s.Ar, Ar); fill(s.Au, Au); fill(s.Af, Af); fill(V, v);
+    O = V;
+    spmv(&s, V, O, 1);
+    std::memcpy(out, O.d.data(), sizeof(double) * n);
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// fe_oracle.h -- TEST INFRASTRUCTURE: C ABI of the FP64 CPU oracle (fe_oracle.cpp).
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+#ifndef FE_ORACLE_H
+#define FE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_PIC = 0, ORC_FLIP = 1, ORC_APIC = 2 };
+// FluidMarker values, src/grid_systems/fluid_markers.hpp:8-13
+enum { ORC_AIR = 0, ORC_FLUID = 1, ORC_BOUNDRY = 2 };
+enum {
+    ORC_U = 0, ORC_V = 1, ORC_W = 2,
+    ORC_MU = 3, ORC_MV = 4, ORC_MW = 5,
+    ORC_RU = 6, ORC_RV = 7, ORC_RW = 8,
+    ORC_SDF = 9, ORC_FLAGS = 10, ORC_PRESSURE = 11, ORC_RHS = 12, ORC_COLLIDER_SDF = 13
+};
+enum { ORC_SOLVER_JACOBI = 0, ORC_SOLVER_CG = 1, ORC_SOLVER_PCG_JACOBI = 2 };
+
+typedef struct orc_solver_desc {
+    int32_t kind;
+    int32_t max_iterations;
+    int32_t check_interval;  // Jacobi residual check interval
+    int32_t relative;        // CG: stop on ||r||/||b|| <= tolerance instead of sigma <= tol^2
+    int32_t warm_start;      // CG: start from the previous pressure
+    int32_t pad;
+    double tolerance;
+} orc_solver_desc;
+
+typedef struct orc_solve_stats {
+    int32_t iterations;
+    int32_t pad;
+    double rel_residual;
+    double abs_residual;
+    double ms;
+} orc_solve_stats;
+
+typedef struct orc_scene orc_scene;
+
+orc_scene* orc_create(const int n[3], const double origin[3], const double size[3], int method, double blend,
+                      double max_cfl, const double gravity[3]);
+void orc_destroy(orc_scene* s);
+void orc_set_threads(orc_scene* s, int nthreads);
+void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np);
+void orc_get_particles(const orc_scene* s, double* x, double* v, double* C);
+int64_t orc_num_particles(const orc_scene* s);
+int64_t orc_grid_count(orc_scene* s, int what);
+int orc_get_grid(orc_scene* s, int what, void* out);
+int orc_set_grid(orc_scene* s, int what, const void* in);
+
+void orc_p2g(orc_scene* s);
+void orc_build_sdf(orc_scene* s);
+void orc_build_markers(orc_scene* s);
+void orc_extrapolate_to_air(orc_scene* s);
+void orc_apply_bc(orc_scene* s);
+void orc_apply_gravity(orc_scene* s, double dt);
+void orc_build_system(orc_scene* s);
+void orc_apply_pressure(orc_scene* s);
+void orc_g2p(orc_scene* s);
+void orc_move_particles(orc_scene* s, double dt);
+void orc_pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st);
+void orc_substep(orc_scene* s, double dt, const orc_solver_desc* d, orc_solve_stats* st);
+double orc_cfl(const orc_scene* s, double dt);
+unsigned orc_num_substeps(const orc_scene* s, double dt);
+int orc_advance_frame(orc_scene* s, double frame_dt, const orc_solver_desc* d);
+void orc_last_stats(const orc_scene* s, orc_solve_stats* st);
+
+void orc_barycentric(double x, int iLow, int iHigh, int* i, double* f);
+void orc_coords_weights(const int size[3], const double origin[3], const double h[3], const double x[3],
+                        int idx_out[24], double w_out[8]);
+void orc_coords_grad_weights(const int size[3], const double origin[3], const double h[3], const double x[3],
+                             int idx_out[24], double g_out[24]);
+double orc_trilerp(const double v[8], double fx, double fy, double fz);
+double orc_sample_array(const double* data, const int dims[3], const double origin[3], const double h[3],
+                        const double p[3]);
+void orc_extrapolate_to_region(const double* input, const int* valid, const int dims[3], int iterations,
+                               double* output);
+int orc_hash_grid_query(const double* pts, int64_t npts, int sx, double spacing, const double q[3], double radius,
+                        int64_t* out, int max_out);
+double orc_blas_dot(const double* a, const double* b, const int dims[3]);
+void orc_solve_rows(const double* Ac, const double* Ar, const double* Au, const double* Af, const double* b,
+                    double* x, const int dims[3], const orc_solver_desc* d, orc_solve_stats* st);
+void orc_residual_rows(const double* Ac, const double* Ar, const double* Au, const double* Af, const double* b,
+                       const double* x, const int dims[3], double* out);
+void orc_spmv_rows(const double* Ac, const double* Ar, const double* Au, const double* Af, const double* v,
+                   const int dims[3], double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  // FE_ORACLE_H
