@@ -1,0 +1,318 @@
+"""ctypes binding of the C ABI in include/fluide_hip.h (libfluide_hip.so).
+
+The library is the only compute path: if it is missing or no gfx950 device is
+present, every entry point raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfluide_hip.so")
+
+PIC, FLIP, APIC = 0, 1, 2
+AIR, FLUID, BOUNDRY = 0, 1, 2
+SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI = 0, 1, 2
+(GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
+ GRID_PRESSURE, GRID_RHS) = range(13)
+(STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
+ STAGE_MOVE, STAGE_MARKERS) = range(10)
+STAGE_NAMES = ["sort", "p2g", "sdf", "extrapolate", "bc", "gravity", "pressure", "g2p_move", "move", "markers"]
+
+EXPORTED = [
+    "fe_ctx_create", "fe_ctx_destroy", "fe_last_error", "fe_ctx_device_name", "fe_scene_create", "fe_scene_destroy",
+    "fe_scene_sync", "fe_particles_upload", "fe_particles_download", "fe_particles_count", "fe_substep",
+    "fe_advance_frame", "fe_cfl", "fe_run_stage", "fe_grid_download", "fe_grid_upload", "fe_grid_bytes",
+    "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
+    "fe_kernel_timer_read", "fe_last_solve_stats",
+]
+KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI = range(7)
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("max_iterations", C.c_int32), ("check_interval", C.c_int32),
+                ("relative", C.c_int32), ("warm_start", C.c_int32), ("reserved", C.c_int32),
+                ("tolerance", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("rel_residual", C.c_double),
+                ("abs_residual", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n", C.c_int32 * 3), ("method", C.c_int32), ("origin", C.c_double * 3), ("size", C.c_double * 3),
+                ("blend", C.c_double), ("max_cfl", C.c_double), ("gravity", C.c_double * 3),
+                ("viscosity", C.c_double), ("density", C.c_double), ("max_particles", C.c_int64)]
+
+
+class SeedShape(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("lo", C.c_double * 3), ("hi", C.c_double * 3)]
+
+
+class FluideError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libfluide_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FluideError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(make -C fluide_engine_amd/csrc); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.fe_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.fe_ctx_destroy.argtypes = [vp]
+    L.fe_last_error.argtypes = [vp]
+    L.fe_last_error.restype = C.c_char_p
+    L.fe_ctx_device_name.argtypes = [vp, C.c_char_p, C.c_size_t]
+    L.fe_scene_create.argtypes = [vp, C.POINTER(SceneDesc), C.POINTER(vp)]
+    L.fe_scene_destroy.argtypes = [vp]
+    L.fe_scene_sync.argtypes = [vp]
+    fp = C.POINTER(C.c_float)
+    L.fe_particles_upload.argtypes = [vp, fp, fp, fp, C.c_int64]
+    L.fe_particles_download.argtypes = [vp, fp, fp, fp, C.c_int64]
+    L.fe_particles_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.fe_substep.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(SolveStats)]
+    L.fe_advance_frame.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(C.c_int32)]
+    L.fe_cfl.argtypes = [vp, C.c_double, C.POINTER(C.c_double)]
+    L.fe_run_stage.argtypes = [vp, C.c_int32, C.c_double, C.POINTER(SolverDesc), C.POINTER(SolveStats)]
+    L.fe_grid_download.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    L.fe_grid_upload.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    L.fe_grid_bytes.argtypes = [vp, C.c_int32, C.POINTER(C.c_size_t)]
+    L.fe_particle_cells.argtypes = [vp, C.POINTER(C.c_int32), C.c_int64]
+    L.fe_seed_particles.argtypes = [C.POINTER(SceneDesc), C.POINTER(SeedShape), C.c_int32, C.c_uint64, fp,
+                                    C.c_int64, C.POINTER(C.c_int64)]
+    L.fe_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
+    L.fe_kernel_timer.argtypes = [vp, C.c_int32]
+    L.fe_last_solve_stats.argtypes = [vp, C.POINTER(SolveStats)]
+    L.fe_kernel_timer_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
+    L.fe_version.restype = C.c_char_p
+    for name in EXPORTED:
+        getattr(L, name).restype = C.c_char_p if name in ("fe_last_error", "fe_version") else C.c_int
+    _lib = L
+    return L
+
+
+def solver(kind=SOLVER_JACOBI, max_iterations=1000, check_interval=5, tolerance=1e-14, relative=False,
+           warm_start=True):
+    """Solver descriptor; the defaults are SinglePhasePressureSolver's
+    CudaJacobiIterationSolver(1000, 5, 1e-14) (single_phase_pressure_solver.cpp:11)."""
+    return SolverDesc(kind, max_iterations, check_interval, int(relative), int(warm_start), 0, tolerance)
+
+
+def scene_desc(n, method=FLIP, origin=(0.0, 0.0, 0.0), size=(1.0, 1.0, 1.0), blend=0.05, max_cfl=3.0,
+               gravity=(0.0, -9.81, 0.0), max_particles=0, viscosity=0.0, density=1.0):
+    d = SceneDesc()
+    d.n[:] = list(n)
+    d.method = method
+    d.origin[:] = list(origin)
+    d.size[:] = list(size)
+    d.blend = blend
+    d.max_cfl = max_cfl
+    d.gravity[:] = list(gravity)
+    d.viscosity = viscosity
+    d.density = density
+    d.max_particles = max_particles
+    return d
+
+
+def box(lo, hi):
+    s = SeedShape()
+    s.kind = 0
+    s.lo[:] = list(lo)
+    s.hi[:] = list(hi)
+    return s
+
+
+def sphere(centre, radius):
+    s = SeedShape()
+    s.kind = 1
+    s.lo[:] = list(centre)
+    s.hi[:] = [radius, 0.0, 0.0]
+    return s
+
+
+def seed_particles(desc, shapes, seed=0):
+    """Deterministic 2x2x2-per-cell seeding (host only; see fe_seed_particles)."""
+    L = load()
+    arr = (SeedShape * max(1, len(shapes)))(*shapes)
+    cnt = C.c_int64()
+    rc = L.fe_seed_particles(C.byref(desc), arr, len(shapes), seed, None, 0, C.byref(cnt))
+    if rc != 0:
+        raise FluideError(f"fe_seed_particles failed ({rc})")
+    x = np.empty((cnt.value, 3), dtype=np.float32)
+    rc = L.fe_seed_particles(C.byref(desc), arr, len(shapes), seed,
+                             x.ctypes.data_as(C.POINTER(C.c_float)), cnt.value, C.byref(cnt))
+    if rc != 0:
+        raise FluideError(f"fe_seed_particles failed ({rc})")
+    return x
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Context:
+    def __init__(self, device=-1):
+        L = load()
+        h = C.c_void_p()
+        rc = L.fe_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise FluideError(f"fe_ctx_create failed ({rc}): {L.fe_last_error(None).decode()}")
+        self._h = h
+        self.L = L
+
+    @property
+    def device_name(self):
+        buf = C.create_string_buffer(64)
+        self.L.fe_ctx_device_name(self._h, buf, 64)
+        return buf.value.decode()
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise FluideError(f"{what} failed ({rc}): {self.L.fe_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fe_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Scene:
+    """One fe_scene: the device state of a HybridSimulator."""
+
+    def __init__(self, ctx: Context, desc: SceneDesc):
+        self.ctx, self.desc, self.L = ctx, desc, ctx.L
+        h = C.c_void_p()
+        ctx.check(self.L.fe_scene_create(ctx._h, C.byref(desc), C.byref(h)), "fe_scene_create")
+        self._h = h
+        self.n = tuple(desc.n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fe_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def shape(self, field):
+        nx, ny, nz = self.n
+        if field in (GRID_U, GRID_MU, GRID_SU):
+            return (nz, ny, nx + 1)
+        if field in (GRID_V, GRID_MV, GRID_SV):
+            return (nz, ny + 1, nx)
+        if field in (GRID_W, GRID_MW, GRID_SW):
+            return (nz + 1, ny, nx)
+        return (nz, ny, nx)
+
+    @staticmethod
+    def dtype(field):
+        if field in (GRID_MU, GRID_MV, GRID_MW, GRID_FLAGS):
+            return np.uint8
+        if field in (GRID_SDF, GRID_PRESSURE):
+            return np.float64
+        return np.float32
+
+    def upload_particles(self, x, v, Cm=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+        cm = None if Cm is None else np.ascontiguousarray(Cm, dtype=np.float32).reshape(-1, 9)
+        self.ctx.check(self.L.fe_particles_upload(self._h, _fptr(x), _fptr(v), _fptr(cm), x.shape[0]),
+                       "fe_particles_upload")
+
+    def count(self):
+        c = C.c_int64()
+        self.ctx.check(self.L.fe_particles_count(self._h, C.byref(c)), "fe_particles_count")
+        return c.value
+
+    def download_particles(self):
+        n = self.count()
+        x = np.empty((n, 3), np.float32)
+        v = np.empty((n, 3), np.float32)
+        cm = np.empty((n, 9), np.float32)
+        self.ctx.check(self.L.fe_particles_download(self._h, _fptr(x), _fptr(v), _fptr(cm), n),
+                       "fe_particles_download")
+        return x, v, cm
+
+    def particle_cells(self):
+        n = self.count()
+        out = np.empty(n, np.int32)
+        self.ctx.check(self.L.fe_particle_cells(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), n),
+                       "fe_particle_cells")
+        return out
+
+    def get(self, field):
+        out = np.empty(self.shape(field), dtype=self.dtype(field))
+        self.ctx.check(self.L.fe_grid_download(self._h, field, out.ctypes.data_as(C.c_void_p), out.nbytes),
+                       "fe_grid_download")
+        return out
+
+    def set(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype(field)).reshape(self.shape(field))
+        self.ctx.check(self.L.fe_grid_upload(self._h, field, a.ctypes.data_as(C.c_void_p), a.nbytes),
+                       "fe_grid_upload")
+
+    def substep(self, dt, desc=None):
+        st = SolveStats()
+        desc = desc or solver()
+        self.ctx.check(self.L.fe_substep(self._h, dt, C.byref(desc), C.byref(st)), "fe_substep")
+        return st
+
+    def advance_frame(self, frame_dt, desc=None):
+        n = C.c_int32()
+        desc = desc or solver()
+        self.ctx.check(self.L.fe_advance_frame(self._h, frame_dt, C.byref(desc), C.byref(n)), "fe_advance_frame")
+        return n.value
+
+    def cfl(self, dt):
+        out = C.c_double()
+        self.ctx.check(self.L.fe_cfl(self._h, dt, C.byref(out)), "fe_cfl")
+        return out.value
+
+    def run_stage(self, stage, dt=0.0, desc=None):
+        st = SolveStats()
+        d = desc or solver()
+        self.ctx.check(self.L.fe_run_stage(self._h, stage, dt, C.byref(d), C.byref(st)),
+                       f"fe_run_stage({STAGE_NAMES[stage] if 0 <= stage < len(STAGE_NAMES) else stage})")
+        return st
+
+    def stage_times(self):
+        out = (C.c_double * 16)()
+        self.ctx.check(self.L.fe_stage_times(self._h, out, 16), "fe_stage_times")
+        return {STAGE_NAMES[i]: out[i] for i in range(len(STAGE_NAMES)) if out[i] > 0}
+
+    def sync(self):
+        self.ctx.check(self.L.fe_scene_sync(self._h), "fe_scene_sync")
+
+    def last_stats(self):
+        st = SolveStats()
+        self.ctx.check(self.L.fe_last_solve_stats(self._h, C.byref(st)), "fe_last_solve_stats")
+        return st
+
+    def kernel_timer(self, kernel_id):
+        """Time every launch of one kernel with HIP events on the scene's stream."""
+        self.ctx.check(self.L.fe_kernel_timer(self._h, kernel_id), "fe_kernel_timer")
+
+    def kernel_timer_read(self, reset=False):
+        ms = C.c_double()
+        n = C.c_int64()
+        self.ctx.check(self.L.fe_kernel_timer_read(self._h, C.byref(ms), C.byref(n), int(reset)),
+                       "fe_kernel_timer_read")
+        return ms.value, n.value

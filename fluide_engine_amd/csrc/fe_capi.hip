@@ -1,0 +1,634 @@
+// fe_capi.hip -- the C ABI of include/fluide_hip.h: scene lifetime, state
+// transfer and the substep orchestration of PICSimulator::OnAdvanceTimeStep
+// (src/pic_simulator.cpp:111-237) over the gfx950 kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "fe_kernels.hpp"
+
+using namespace fe;
+
+namespace {
+
+thread_local std::string g_err_no_ctx;
+
+int set_err(fe_ctx* ctx, int code, const std::string& msg) {
+    if (ctx)
+        ctx->err = msg;
+    else
+        g_err_no_ctx = msg;
+    return code;
+}
+
+int hip_fail(fe_ctx* ctx, hipError_t e, const char* what) {
+    return set_err(ctx, FE_E_HIP, std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+
+#define HIPCHK(ctx, call)                                   \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #call); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+void build_geom(fe_scene* s) {
+    Geom& g = s->g;
+    const fe_scene_desc& d = s->desc;
+    g.method = d.method;
+    for (int a = 0; a < 3; ++a) {
+        g.n[a] = d.n[a];
+        g.od[a] = d.origin[a];
+        g.Ld[a] = d.size[a];
+        g.hd[a] = d.size[a] / d.n[a];  // src/hybrid_simulator.cpp:7
+        g.o[a] = (float)g.od[a];
+        g.h[a] = (float)g.hd[a];
+        g.ihd[a] = 1.0 / g.hd[a];
+        g.ih[a] = (float)g.ihd[a];
+        g.L[a] = (float)g.Ld[a];
+    }
+    // FaceCenteredGrid3D::CalculateDataOrigins (face_centered_grid3d.cpp:324-332)
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            g.cod[c][a] = g.od[a] + g.hd[a] * (a == c ? 0.0 : 0.5);
+            g.co[c][a] = (float)g.cod[c][a];
+        }
+}
+
+size_t field_bytes(const fe_scene* s, int32_t field) {
+    const Geom& g = s->g;
+    switch (field) {
+        case FE_GRID_U: case FE_GRID_SU: return face_count(g, 0) * sizeof(float);
+        case FE_GRID_V: case FE_GRID_SV: return face_count(g, 1) * sizeof(float);
+        case FE_GRID_W: case FE_GRID_SW: return face_count(g, 2) * sizeof(float);
+        case FE_GRID_MU: return face_count(g, 0);
+        case FE_GRID_MV: return face_count(g, 1);
+        case FE_GRID_MW: return face_count(g, 2);
+        case FE_GRID_SDF: case FE_GRID_PRESSURE: return cell_count(g) * sizeof(double);
+        case FE_GRID_FLAGS: return cell_count(g);
+        case FE_GRID_RHS: return cell_count(g) * sizeof(float);
+        default: return 0;
+    }
+}
+
+void* field_ptr(fe_scene* s, int32_t field) {
+    switch (field) {
+        case FE_GRID_U: return s->u[0];
+        case FE_GRID_V: return s->u[1];
+        case FE_GRID_W: return s->u[2];
+        case FE_GRID_SU: return s->snap[0];
+        case FE_GRID_SV: return s->snap[1];
+        case FE_GRID_SW: return s->snap[2];
+        case FE_GRID_MU: return s->mark[0];
+        case FE_GRID_MV: return s->mark[1];
+        case FE_GRID_MW: return s->mark[2];
+        case FE_GRID_SDF: return s->sdf;
+        case FE_GRID_FLAGS: return s->flags;
+        case FE_GRID_PRESSURE: return s->p;
+        case FE_GRID_RHS: return s->b;
+        default: return nullptr;
+    }
+}
+
+int check_solver(fe_ctx* ctx, const fe_solver_desc* d) {
+    if (!d) return set_err(ctx, FE_E_INVALID, "solver descriptor is NULL");
+    if (d->kind < FE_SOLVER_JACOBI || d->kind > FE_SOLVER_PCG_JACOBI)
+        return set_err(ctx, FE_E_UNSUPPORTED, "unknown solver kind " + std::to_string(d->kind));
+    if (d->max_iterations < 0 || d->max_iterations > 1000000)
+        return set_err(ctx, FE_E_INVALID, "max_iterations out of range");
+    if (!(d->tolerance >= 0.0)) return set_err(ctx, FE_E_INVALID, "tolerance must be >= 0");
+    return FE_OK;
+}
+
+void ev(fe_scene* s, int i) { (void)hipEventRecord(s->ev[i], s->stream); }
+
+// Stage sequence markers for fe_stage_times.
+struct StageClock {
+    fe_scene* s;
+    int used[kMaxStages] = {0};
+    int order[kMaxStages];
+    int n = 0;
+    explicit StageClock(fe_scene* sc) : s(sc) {}
+    void begin(int stage) {
+        order[n] = stage;
+        ev(s, n);
+        ++n;
+    }
+    void end() { ev(s, n); }
+    void collect() {
+        for (int i = 0; i < kMaxStages; ++i) s->stage_ms[i] = 0.0;
+        for (int i = 0; i < n; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]) == hipSuccess) s->stage_ms[order[i]] += ms;
+        }
+    }
+};
+
+int ensure_sorted(fe_scene* s) {
+    if (!s->sorted) HIPCHK(s->ctx, launch_sort(s, false));
+    return FE_OK;
+}
+
+// Pressure stage: SinglePhasePressureSolver::Solve (single_phase_pressure_solver.cpp:21-28)
+// minus the markers (built with the SDF), plus the FP64 residual for the stats.
+int pressure(fe_scene* s, const fe_solver_desc* d, bool walls_after, fe_solve_stats* st, int ev_start) {
+    HIPCHK(s->ctx, launch_pressure_solve(s, d));
+    const SolverScalars& h = *s->sc_host;
+    const int iters = d->kind == FE_SOLVER_JACOBI ? (h.done ? h.it : d->max_iterations) : h.it;
+    const int converged = h.converged;
+    HIPCHK(s->ctx, launch_final_residual(s));
+    HIPCHK(s->ctx, launch_apply_pressure(s, walls_after));
+    hipEvent_t stop = s->ev[kMaxStages];
+    HIPCHK(s->ctx, hipEventRecord(stop, s->stream));
+    HIPCHK(s->ctx, hipMemcpyAsync(s->sc_host, s->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+    fe_solve_stats& ls = s->last_stats;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, s->ev[ev_start], stop);
+    ls.iterations = iters;
+    const double rr = s->sc_host->rr, bb = s->sc_host->bnorm2;
+    ls.abs_residual = std::sqrt(rr);
+    ls.rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    ls.ms = ms;
+    ls.converged = converged;
+    if (st) *st = ls;
+    return FE_OK;
+}
+
+int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st) {
+    fe_ctx* ctx = s->ctx;
+    StageClock clk(s);
+    // particles binned by cell (keys come fused from the previous advection)
+    clk.begin(FE_STAGE_SORT);
+    if (!s->sorted) HIPCHK(ctx, launch_sort(s, s->keys_ready));
+    s->keys_ready = false;
+    // OnBeginAdvanceTimeStep (src/pic_simulator.cpp:156-196)
+    clk.begin(FE_STAGE_P2G);
+    HIPCHK(ctx, launch_p2g(s));
+    clk.begin(FE_STAGE_SDF);
+    HIPCHK(ctx, launch_sdf(s));
+    HIPCHK(ctx, launch_markers(s));  // BuildMarkers reads only the SDF: done once here
+    clk.begin(FE_STAGE_EXTRAPOLATE);
+    HIPCHK(ctx, launch_extrapolate(s));
+    HIPCHK(ctx, launch_walls(s));
+    // ComputeExternalForces (:203-212) = forces, then ApplyBoundryCondition
+    clk.begin(FE_STAGE_GRAVITY);
+    HIPCHK(ctx, launch_gravity(s, dt, true));
+    // ComputeDiffusion is skipped at viscosity 0 (:214-222); ComputePressure (:224-229)
+    const int pev = clk.n;
+    clk.begin(FE_STAGE_PRESSURE);
+    int rc = pressure(s, d, true, st, pev);
+    if (rc != FE_OK) return rc;
+    // ComputeAdvection (:231-237)
+    clk.begin(FE_STAGE_EXTRAPOLATE);
+    HIPCHK(ctx, launch_extrapolate(s));
+    HIPCHK(ctx, launch_walls(s));
+    clk.begin(FE_STAGE_G2P);
+    HIPCHK(ctx, launch_g2p_move(s, dt, true, true, true));
+    s->keys_ready = true;
+    clk.end();
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    clk.collect();
+    kt_flush(s);
+    return FE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fe_version(void) { return "fluide-hip 0.1 (gfx950)"; }
+
+int fe_ctx_create(int device, fe_ctx** out) {
+    if (!out) return set_err(nullptr, FE_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return set_err(nullptr, FE_E_NO_DEVICE, std::string("no HIP device: ") + hipGetErrorName(e));
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) device = 0;
+    }
+    if (device >= count) return set_err(nullptr, FE_E_INVALID, "device index out of range");
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(nullptr, e, "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(nullptr, FE_E_NO_DEVICE,
+                       std::string("device is ") + prop.gcnArchName + "; this library is built for gfx950 only");
+    auto* ctx = new fe_ctx();
+    ctx->device = device;
+    std::snprintf(ctx->name, sizeof(ctx->name), "%s", prop.gcnArchName);
+    *out = ctx;
+    return FE_OK;
+}
+
+int fe_ctx_destroy(fe_ctx* ctx) {
+    delete ctx;
+    return FE_OK;
+}
+
+const char* fe_last_error(const fe_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err_no_ctx.c_str(); }
+
+int fe_ctx_device_name(const fe_ctx* ctx, char* buf, size_t len) {
+    if (!ctx || !buf || len == 0) return FE_E_INVALID;
+    std::snprintf(buf, len, "%s", ctx->name);
+    return FE_OK;
+}
+
+int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
+    if (!ctx || !d || !out) return set_err(ctx, FE_E_INVALID, "NULL argument");
+    *out = nullptr;
+    for (int a = 0; a < 3; ++a) {
+        if (d->n[a] < 2 || d->n[a] > 2048) return set_err(ctx, FE_E_INVALID, "grid size per axis must be in [2, 2048]");
+        if (!(d->size[a] > 0.0)) return set_err(ctx, FE_E_INVALID, "domain size must be positive");
+    }
+    if ((int64_t)d->n[0] * d->n[1] * d->n[2] > (int64_t)1 << 30)
+        return set_err(ctx, FE_E_INVALID, "grid has more than 2^30 cells");
+    if (d->method < FE_PIC || d->method > FE_APIC) return set_err(ctx, FE_E_INVALID, "unknown method");
+    if (d->viscosity != 0.0)
+        return set_err(ctx, FE_E_UNSUPPORTED, "viscosity > 0 (BackwardEulerDiffusionSolver) is not on this backend");
+    if (d->max_particles < 0 || d->max_particles > ((int64_t)1 << 31) - 1)
+        return set_err(ctx, FE_E_INVALID, "max_particles out of range");
+    if (!(d->max_cfl >= 0.0) || d->max_cfl > 64) return set_err(ctx, FE_E_INVALID, "max_cfl out of range");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    auto* s = new fe_scene();
+    s->ctx = ctx;
+    s->desc = *d;
+    build_geom(s);
+    s->depth = (int)std::ceil(d->max_cfl);                               // ceil(maxCfl), src/pic_simulator.cpp:491
+    s->move_substeps = (int)(unsigned)std::max(d->max_cfl, 1.0);         // src/pic_simulator.cpp:255
+    const Geom& g = s->g;
+    const int64_t nc = cell_count(g);
+    const int64_t nf[3] = {face_count(g, 0), face_count(g, 1), face_count(g, 2)};
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (r != hipSuccess && e == hipSuccess) e = r;
+    };
+    chk(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    for (int c = 0; c < 3; ++c) {
+        chk(dalloc(&s->u[c], nf[c]));
+        chk(dalloc(&s->snap[c], nf[c]));
+        chk(dalloc(&s->mark[c], nf[c]));
+    }
+    chk(dalloc(&s->valid[0], nf[0] + nf[1] + nf[2]));
+    chk(dalloc(&s->valid[1], nf[0] + nf[1] + nf[2]));
+    chk(dalloc(&s->sdf, nc));
+    chk(dalloc(&s->flags, nc));
+    chk(dalloc(&s->p, nc));
+    chk(dalloc(&s->p2, nc));
+    chk(dalloc(&s->b, nc));
+    chk(dalloc(&s->r, nc));
+    chk(dalloc(&s->z, nc));
+    chk(dalloc(&s->d[0], nc));
+    chk(dalloc(&s->d[1], nc));
+    chk(dalloc(&s->q, nc));
+    s->npartials = pad_blocks((nc + kBlock - 1) / kBlock) * 4;
+    chk(dalloc(&s->partials, s->npartials));
+    chk(dalloc(&s->sc, 1));
+    chk(hipHostMalloc((void**)&s->sc_host, sizeof(SolverScalars), hipHostMallocDefault));
+    chk(dalloc(&s->cflmax, 1));
+    const int64_t cap = std::max<int64_t>(d->max_particles, 1);
+    s->cap = cap;
+    for (int b = 0; b < 2; ++b) {
+        chk(dalloc(&s->P[b].x, 3 * cap));
+        chk(dalloc(&s->P[b].v, 3 * cap));
+        chk(dalloc(&s->P[b].pid, cap));
+        if (d->method == FE_APIC) chk(dalloc(&s->P[b].C, 9 * cap));
+    }
+    chk(dalloc(&s->key, cap));
+    chk(dalloc(&s->order, cap));
+    chk(dalloc(&s->cell_count, nc + 1));
+    chk(dalloc(&s->cell_start, nc + 1));
+    chk(dalloc(&s->cursor, nc));
+    chk(dalloc(&s->scan_sums, (nc + 1023) / 1024 + 1));
+    for (int i = 0; i <= kMaxStages; ++i) chk(hipEventCreate(&s->ev[i]));
+    if (e != hipSuccess) {
+        int rc = hip_fail(ctx, e, "scene allocation");
+        fe_scene_destroy(s);
+        return rc == FE_E_HIP && e == hipErrorOutOfMemory ? FE_E_NOMEM : rc;
+    }
+    // zero state (fill kernels, no memset nodes)
+    for (int c = 0; c < 3; ++c) {
+        chk(launch_fill_f32(s->stream, s->u[c], 0.0f, nf[c]));
+        chk(launch_fill_f32(s->stream, s->snap[c], 0.0f, nf[c]));
+        chk(launch_fill_u8(s->stream, s->mark[c], 0, nf[c]));
+    }
+    chk(hipMemsetAsync(s->p, 0, nc * sizeof(double), s->stream));
+    chk(hipMemsetAsync(s->p2, 0, nc * sizeof(double), s->stream));
+    chk(launch_fill_f32(s->stream, s->b, 0.0f, nc));
+    chk(launch_fill_f32(s->stream, s->d[0], 0.0f, nc));
+    chk(launch_fill_f32(s->stream, s->d[1], 0.0f, nc));
+    chk(launch_fill_u8(s->stream, s->flags, FE_AIR, nc));
+    chk(hipMemsetAsync(s->sdf, 0, nc * sizeof(double), s->stream));
+    chk(hipMemsetAsync(s->sc, 0, sizeof(SolverScalars), s->stream));
+    chk(hipStreamSynchronize(s->stream));
+    if (e != hipSuccess) {
+        int rc = hip_fail(ctx, e, "scene initialisation");
+        fe_scene_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return FE_OK;
+}
+
+int fe_scene_destroy(fe_scene* s) {
+    if (!s) return FE_OK;
+    if (s->ctx) (void)hipSetDevice(s->ctx->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    auto F = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    for (int c = 0; c < 3; ++c) {
+        F(s->u[c]);
+        F(s->snap[c]);
+        F(s->mark[c]);
+    }
+    F(s->valid[0]); F(s->valid[1]);
+    F(s->sdf); F(s->flags); F(s->p); F(s->b); F(s->r); F(s->z); F(s->p2);
+    F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->sc); F(s->cflmax);
+    if (s->sc_host) (void)hipHostFree(s->sc_host);
+    for (int b = 0; b < 2; ++b) {
+        F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
+    }
+    F(s->key); F(s->order); F(s->cell_count); F(s->cell_start); F(s->cursor); F(s->scan_sums);
+    for (int i = 0; i <= kMaxStages; ++i)
+        if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
+    for (auto e : s->kt_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return FE_OK;
+}
+
+int fe_scene_sync(fe_scene* s) {
+    if (!s) return FE_E_INVALID;
+    HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+    return FE_OK;
+}
+
+int fe_particles_upload(fe_scene* s, const float* x, const float* v, const float* C, int64_t count) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (count < 0 || count > s->cap)
+        return set_err(ctx, FE_E_INVALID,
+                       "particle count " + std::to_string(count) + " exceeds max_particles " + std::to_string(s->cap));
+    if (count > 0 && (!x || !v)) return set_err(ctx, FE_E_INVALID, "x and v are required");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    s->np = count;
+    Particles& P = s->P[s->cur];
+    Particles& T = s->P[1 - s->cur];  // staging
+    if (count > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(T.x, x, sizeof(float) * 3 * count, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(ctx, launch_deinterleave(s, T.x, 3, P.x));
+        HIPCHK(ctx, hipMemcpyAsync(T.v, v, sizeof(float) * 3 * count, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(ctx, launch_deinterleave(s, T.v, 3, P.v));
+        if (s->desc.method == FE_APIC) {
+            if (C) {
+                HIPCHK(ctx, hipMemcpyAsync(T.C, C, sizeof(float) * 9 * count, hipMemcpyHostToDevice, s->stream));
+                HIPCHK(ctx, launch_deinterleave(s, T.C, 9, P.C));
+            } else {
+                HIPCHK(ctx, launch_fill_f32(s->stream, P.C, 0.0f, 9 * s->cap));
+            }
+        }
+        HIPCHK(ctx, launch_iota(s, P.pid, count));
+    }
+    s->sorted = false;
+    s->keys_ready = false;
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return FE_OK;
+}
+
+int fe_particles_download(fe_scene* s, float* x, float* v, float* C, int64_t count) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (count != s->np) return set_err(ctx, FE_E_INVALID, "count must equal the particle count");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (count == 0) return FE_OK;
+    Particles& P = s->P[s->cur];
+    Particles& T = s->P[1 - s->cur];
+    if (x) {
+        HIPCHK(ctx, launch_unpermute(s, P.x, 3, T.x));
+        HIPCHK(ctx, hipMemcpyAsync(x, T.x, sizeof(float) * 3 * count, hipMemcpyDeviceToHost, s->stream));
+    }
+    if (v) {
+        HIPCHK(ctx, launch_unpermute(s, P.v, 3, T.v));
+        HIPCHK(ctx, hipMemcpyAsync(v, T.v, sizeof(float) * 3 * count, hipMemcpyDeviceToHost, s->stream));
+    }
+    if (C) {
+        if (s->desc.method != FE_APIC) {
+            std::memset(C, 0, sizeof(float) * 9 * count);
+        } else {
+            HIPCHK(ctx, launch_unpermute(s, P.C, 9, T.C));
+            HIPCHK(ctx, hipMemcpyAsync(C, T.C, sizeof(float) * 9 * count, hipMemcpyDeviceToHost, s->stream));
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return FE_OK;
+}
+
+int fe_particles_count(const fe_scene* s, int64_t* count) {
+    if (!s || !count) return FE_E_INVALID;
+    *count = s->np;
+    return FE_OK;
+}
+
+int fe_particle_cells(fe_scene* s, int32_t* cells, int64_t count) {
+    if (!s || (!cells && count > 0)) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (count != s->np) return set_err(ctx, FE_E_INVALID, "count must equal the particle count");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (count == 0) return FE_OK;
+    int32_t* tmp = (int32_t*)s->P[1 - s->cur].x;  // staging (3*cap floats >= cap ints)
+    HIPCHK(ctx, launch_cells_by_pid(s, tmp));
+    HIPCHK(ctx, hipMemcpyAsync(cells, tmp, sizeof(int32_t) * count, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    s->keys_ready = false;  // the key buffer was overwritten
+    return FE_OK;
+}
+
+int fe_substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    int rc = check_solver(ctx, d);
+    if (rc != FE_OK) return rc;
+    if (!(dt >= 0.0) || !std::isfinite(dt)) return set_err(ctx, FE_E_INVALID, "dt must be finite and >= 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return substep(s, dt, d, st);
+}
+
+int fe_cfl(fe_scene* s, double dt, double* out) {
+    if (!s || !out) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_cfl(s, dt));
+    unsigned long long bits = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&bits, s->cflmax, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    double maxv;
+    std::memcpy(&maxv, &bits, sizeof(double));
+    const double minH = std::min(s->g.hd[0], std::min(s->g.hd[1], s->g.hd[2]));
+    *out = maxv * dt / minH;
+    return FE_OK;
+}
+
+int fe_advance_frame(fe_scene* s, double frame_dt, const fe_solver_desc* d, int32_t* substeps_out) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    int rc = check_solver(ctx, d);
+    if (rc != FE_OK) return rc;
+    if (!(frame_dt >= 0.0) || !std::isfinite(frame_dt)) return set_err(ctx, FE_E_INVALID, "frame_dt must be >= 0");
+    // PhysicsAnimation::AdvanceTimeStep (src/physics_animation.cpp:64-92)
+    double remaining = frame_dt;
+    int32_t count = 0;
+    while (remaining > 0.0000000001) {
+        double cfl = 0.0;
+        if ((rc = fe_cfl(s, remaining, &cfl)) != FE_OK) return rc;
+        // PICSimulator::NumberOfSubTimeSteps (src/pic_simulator.cpp:349-354)
+        const double ratio = s->desc.max_cfl > 0 ? std::ceil(cfl / s->desc.max_cfl) : 1.0;
+        const unsigned n = (unsigned)std::max(ratio, 1.0);
+        if (n > 100000) return set_err(ctx, FE_E_INVALID, "CFL demands more than 1e5 substeps (unstable state?)");
+        const double sub = remaining / (double)n;
+        if ((rc = substep(s, sub, d, nullptr)) != FE_OK) return rc;
+        remaining -= sub;
+        ++count;
+    }
+    if (substeps_out) *substeps_out = count;
+    return FE_OK;
+}
+
+int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d, fe_solve_stats* st) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = FE_OK;
+    StageClock clk(s);
+    clk.begin(stage >= 0 && stage < kMaxStages ? stage : 0);
+    switch (stage) {
+        case FE_STAGE_SORT:
+            HIPCHK(ctx, launch_sort(s, false));
+            break;
+        case FE_STAGE_P2G:
+            if ((rc = ensure_sorted(s)) != FE_OK) return rc;
+            HIPCHK(ctx, launch_p2g(s));
+            break;
+        case FE_STAGE_SDF:
+            if ((rc = ensure_sorted(s)) != FE_OK) return rc;
+            HIPCHK(ctx, launch_sdf(s));
+            HIPCHK(ctx, launch_markers(s));
+            break;
+        case FE_STAGE_MARKERS:
+            HIPCHK(ctx, launch_markers(s));
+            break;
+        case FE_STAGE_EXTRAPOLATE:
+            HIPCHK(ctx, launch_extrapolate(s));
+            break;
+        case FE_STAGE_BC:
+            HIPCHK(ctx, launch_walls(s));
+            break;
+        case FE_STAGE_GRAVITY:
+            HIPCHK(ctx, launch_gravity(s, dt, false));
+            break;
+        case FE_STAGE_PRESSURE:
+            if ((rc = check_solver(ctx, d)) != FE_OK) return rc;
+            HIPCHK(ctx, launch_markers(s));
+            return pressure(s, d, false, st, 0);
+        case FE_STAGE_G2P:
+            HIPCHK(ctx, launch_g2p_move(s, dt, true, false, false));
+            break;
+        case FE_STAGE_MOVE:
+            HIPCHK(ctx, launch_g2p_move(s, dt, false, true, false));
+            break;
+        default:
+            return set_err(ctx, FE_E_INVALID, "unknown stage " + std::to_string(stage));
+    }
+    s->keys_ready = false;
+    clk.end();
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    clk.collect();
+    kt_flush(s);
+    return FE_OK;
+}
+
+int fe_last_solve_stats(const fe_scene* s, fe_solve_stats* st) {
+    if (!s || !st) return FE_E_INVALID;
+    *st = s->last_stats;
+    return FE_OK;
+}
+
+int fe_kernel_timer(fe_scene* s, int32_t kernel_id) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (kernel_id < FE_KERNEL_NONE || kernel_id > FE_KERNEL_JACOBI)
+        return set_err(ctx, FE_E_INVALID, "unknown kernel id " + std::to_string(kernel_id));
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    kt_flush(s);
+    if (kernel_id != FE_KERNEL_NONE && s->kt_ev.empty()) {
+        s->kt_ev.resize(2 * kTimerPairs, nullptr);
+        for (auto& e : s->kt_ev) HIPCHK(ctx, hipEventCreate(&e));
+    }
+    s->kt_id = kernel_id;
+    s->kt_ms = 0.0;
+    s->kt_launches = 0;
+    return FE_OK;
+}
+
+int fe_kernel_timer_read(fe_scene* s, double* total_ms, int64_t* launches, int32_t reset) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    kt_flush(s);
+    if (total_ms) *total_ms = s->kt_ms;
+    if (launches) *launches = s->kt_launches;
+    if (reset) {
+        s->kt_ms = 0.0;
+        s->kt_launches = 0;
+    }
+    return FE_OK;
+}
+
+int fe_grid_bytes(const fe_scene* s, int32_t field, size_t* bytes) {
+    if (!s || !bytes) return FE_E_INVALID;
+    *bytes = field_bytes(s, field);
+    return *bytes ? FE_OK : FE_E_INVALID;
+}
+
+int fe_grid_download(fe_scene* s, int32_t field, void* dst, size_t bytes) {
+    if (!s || !dst) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    const size_t need = field_bytes(s, field);
+    if (need == 0) return set_err(ctx, FE_E_INVALID, "unknown grid field " + std::to_string(field));
+    if (bytes != need) return set_err(ctx, FE_E_INVALID, "byte count mismatch: need " + std::to_string(need));
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(dst, field_ptr(s, field), need, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return FE_OK;
+}
+
+int fe_grid_upload(fe_scene* s, int32_t field, const void* src, size_t bytes) {
+    if (!s || !src) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    const size_t need = field_bytes(s, field);
+    if (need == 0) return set_err(ctx, FE_E_INVALID, "unknown grid field " + std::to_string(field));
+    if (bytes != need) return set_err(ctx, FE_E_INVALID, "byte count mismatch: need " + std::to_string(need));
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(field_ptr(s, field), src, need, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return FE_OK;
+}
+
+int fe_stage_times(const fe_scene* s, double* ms, int32_t count) {
+    if (!s || !ms) return FE_E_INVALID;
+    for (int i = 0; i < count; ++i) ms[i] = i < kMaxStages ? s->stage_ms[i] : 0.0;
+    return FE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+// fe_internal.hpp -- shared definitions of the gfx950 backend: scene state in
+// HBM, launch helpers and the device-side geometry routines every kernel uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/fluide_hip.h"
+
+namespace fe {
+
+constexpr int kBlock = 256;        // threads per workgroup for element-wise kernels (4 waves of 64)
+constexpr int kXcds = 8;           // MI355X: workgroups are dealt round-robin to 8 XCDs, one L2 each
+constexpr int kMaxStages = 16;
+
+// ---------------------------------------------------------------------------
+// Geometry of a MAC grid, in the form the kernels need.
+// Face data origins follow FaceCenteredGrid3D::CalculateDataOrigins
+// (src/grid_systems/face_centered_grid3d.cpp:324-332): u at origin + h*(0,.5,.5),
+// v at origin + h*(.5,0,.5), w at origin + h*(.5,.5,0).
+// ---------------------------------------------------------------------------
+struct Geom {
+    int n[3];
+    int method;
+    float o[3], h[3], ih[3], L[3];
+    float co[3][3];        // co[c][a]: component c data origin on axis a (FP32)
+    double od[3], hd[3], ihd[3], Ld[3];
+    double cod[3][3];      // same in FP64 (the reference's values)
+};
+
+// Face-array dims of component c: the normal axis has n+1 entries.
+__host__ __device__ inline int face_dim(const Geom& g, int c, int a) { return g.n[a] + (a == c ? 1 : 0); }
+__host__ __device__ inline int64_t face_count(const Geom& g, int c) {
+    return (int64_t)face_dim(g, c, 0) * face_dim(g, c, 1) * face_dim(g, c, 2);
+}
+__host__ __device__ inline int64_t cell_count(const Geom& g) { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
+
+// Stencil size the reference passes to GetCooridnatesAndWeights for component c
+// on axis a: FLIP passes the component's own dims (src/flip_simulator.cpp:58,66,74),
+// PIC and APIC pass the cell dims (src/pic_simulator.cpp:389,397,405;
+// src/apic_simulator.cpp:58,71,84).
+__host__ __device__ inline int stencil_size(const Geom& g, int c, int a) {
+    return g.method == FE_FLIP ? face_dim(g, c, a) : g.n[a];
+}
+
+// ---------------------------------------------------------------------------
+// floor((p - o)/h) and its fraction, bit-exact with the reference's FP64
+// expression.  The FP32 form is exact for power-of-two h with a dyadic origin;
+// in general it is within 2^-22*((|p|+|o|)/h + |x|) of the FP64 value, so when
+// the FP32 fraction lies inside a guard band 2^-19*(...) of an integer the
+// index is recomputed in FP64 exactly as (x - origin) / gridSpacing
+// (src/common/math_utils.hpp:85-89).
+// ---------------------------------------------------------------------------
+__device__ inline void exact_floor64(float p, double od, double hd, int& s, float& f) {
+#pragma clang fp contract(off)
+    double xd = ((double)p - od) / hd;
+    xd = fmin(fmax(xd, -1.0e9), 1.0e9);
+    const double sd = floor(xd);
+    s = (int)sd;
+    f = (float)(xd - sd);
+}
+
+__device__ inline void axis_floor(float p, float o, float ih, double od, double hd, int& s, float& f) {
+    const float x = (p - o) * ih;
+    const float fs = floorf(x);
+    const float fr = x - fs;
+    const float band = 1.9073486e-6f * ((fabsf(p) + fabsf(o)) * ih + fabsf(x) + 1.0f);  // 2^-19 * (...)
+    if (fr >= band && fr <= 1.0f - band && fabsf(x) < 1.0e9f) {
+        s = (int)fs;
+        f = fr;
+        return;
+    }
+    exact_floor64(p, od, hd, s, f);
+}
+
+// GetBarycentric(x, 0, size-1) clamp rules (src/common/math_utils.hpp:48-73).
+__device__ inline void bary_clamp(int s, float fr, int size, int& i, float& f) {
+    if (size <= 1) {
+        i = 0;
+        f = 0.0f;
+    } else if (s < 0) {
+        i = 0;
+        f = 0.0f;
+    } else if (s > size - 2) {
+        i = size - 2;
+        f = 1.0f;
+    } else {
+        i = s;
+        f = fr;
+    }
+}
+
+// Exact stencil base index + fraction of position p for component c, axis a.
+__device__ inline void stencil_axis(const Geom& g, int c, int a, float p, int size, int& i, float& f) {
+    int s;
+    float fr;
+    axis_floor(p, g.co[c][a], g.ih[a], g.cod[c][a], g.hd[a], s, fr);
+    bary_clamp(s, fr, size, i, f);
+}
+
+// Cell containing p, clamped to the grid: the particle's sort bin.  Same
+// arithmetic as the normal axis of the component stencils (data origin of the
+// normal axis is the grid origin), so bin and stencil agree bit for bit.
+__device__ inline int cell_axis(const Geom& g, int a, float p) {
+    int s;
+    float fr;
+    axis_floor(p, g.o[a], g.ih[a], g.od[a], g.hd[a], s, fr);
+    return s < 0 ? 0 : (s > g.n[a] - 1 ? g.n[a] - 1 : s);
+}
+
+// FP32 fast coordinates for interpolation (values, not parity-critical indices).
+__device__ inline void fast_axis(float p, float o, float ih, int size, int& i, float& f) {
+    const float x = (p - o) * ih;
+    const float fs = floorf(x);
+    int s = (int)fmaxf(fminf(fs, 1.0e9f), -1.0e9f);
+    bary_clamp(s, x - fs, size, i, f);
+}
+
+__device__ inline float lerpf_ref(float a, float b, float f) { return (1.0f - f) * a + f * b; }
+
+// FaceCenteredGrid3D::SampleArray (src/grid_systems/face_centered_grid3d.cpp:339-371):
+// trilinear sample of a face array using its own dims for the clamp.
+__device__ inline float sample_face(const float* __restrict__ a, const Geom& g, int c, float px, float py, float pz) {
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
+    int i, j, k;
+    float fx, fy, fz;
+    fast_axis(px, g.co[c][0], g.ih[0], sx, i, fx);
+    fast_axis(py, g.co[c][1], g.ih[1], sy, j, fy);
+    fast_axis(pz, g.co[c][2], g.ih[2], sz, k, fz);
+    const int ip1 = min(i + 1, sx - 1), jp1 = min(j + 1, sy - 1), kp1 = min(k + 1, sz - 1);
+    const int64_t r00 = (int64_t)sx * ((int64_t)j + (int64_t)sy * k);
+    const int64_t r10 = (int64_t)sx * ((int64_t)jp1 + (int64_t)sy * k);
+    const int64_t r01 = (int64_t)sx * ((int64_t)j + (int64_t)sy * kp1);
+    const int64_t r11 = (int64_t)sx * ((int64_t)jp1 + (int64_t)sy * kp1);
+    const float x000 = a[r00 + i], x100 = a[r00 + ip1], x010 = a[r10 + i], x110 = a[r10 + ip1];
+    const float x001 = a[r01 + i], x101 = a[r01 + ip1], x011 = a[r11 + i], x111 = a[r11 + ip1];
+    return lerpf_ref(lerpf_ref(lerpf_ref(x000, x100, fx), lerpf_ref(x010, x110, fx), fy),
+                     lerpf_ref(lerpf_ref(x001, x101, fx), lerpf_ref(x011, x111, fx), fy), fz);
+}
+
+// XCD-aware block remap: consecutive logical tiles land on the same XCD (and
+// its L2) instead of being dealt round-robin.  nblocks is padded to a multiple
+// of 8 by the launcher; blocks mapped past the real count exit.
+__device__ inline int64_t xcd_remap(int64_t bid, int64_t padded) {
+    const int64_t per = padded / kXcds;
+    return (bid % kXcds) * per + bid / kXcds;
+}
+
+inline int64_t pad_blocks(int64_t nblocks) { return (nblocks + kXcds - 1) / kXcds * kXcds; }
+
+// ---------------------------------------------------------------------------
+// Device-resident solver scalars (FP64): updated by the last block of each
+// reduction so the CG loop never waits on the host.
+// ---------------------------------------------------------------------------
+struct SolverScalars {
+    double sigma;      // r . z
+    double sigma_old;
+    double rr;         // r . r
+    double dq;         // d . q
+    double alpha;
+    double beta;
+    double bnorm2;     // b . b
+    double tol;
+    int32_t it;        // iterations done
+    int32_t done;      // stopping rule met
+    int32_t trigger;   // recompute the residual next iteration (sigma grew)
+    int32_t relative;
+    int32_t precond;
+    int32_t max_it;
+    uint32_t counter;  // last-block-done ticket
+    int32_t converged; // the stopping rule (not the iteration cap) ended the solve
+};
+
+// ---------------------------------------------------------------------------
+// Host-side scene.
+// ---------------------------------------------------------------------------
+struct Particles {
+    float* x = nullptr;   // 3*cap: px | py | pz
+    float* v = nullptr;   // 3*cap
+    float* C = nullptr;   // 9*cap (APIC only)
+    int32_t* pid = nullptr;
+};
+
+}  // namespace fe
+
+struct fe_ctx {
+    int device = 0;
+    std::string err;
+    char name[64] = {0};
+};
+
+struct fe_scene {
+    fe_ctx* ctx = nullptr;
+    fe_scene_desc desc{};
+    fe::Geom g{};
+    hipStream_t stream = nullptr;
+    int depth = 3;            // ceil(maxCfl): extrapolation / BC depth
+    int move_substeps = 3;    // (unsigned)max(maxCfl, 1)
+
+    // grid (HBM)
+    float* u[3] = {nullptr, nullptr, nullptr};       // face velocities
+    float* snap[3] = {nullptr, nullptr, nullptr};    // FLIP snapshot
+    uint8_t* mark[3] = {nullptr, nullptr, nullptr};  // P2G face markers
+    uint8_t* valid[2] = {nullptr, nullptr};          // extrapolation validity ping-pong (all 3 comps)
+    double* sdf = nullptr;
+    uint8_t* flags = nullptr;
+    double* p = nullptr;          // pressure (LinearSystem::x), FP64: kept between solves (warm start)
+    double* p2 = nullptr;         // pressure ping-pong partner
+    float* b = nullptr;
+    float* r = nullptr;
+    float* z = nullptr;           // preconditioned residual
+    float* d[2] = {nullptr, nullptr};
+    float* q = nullptr;
+    double* partials = nullptr;   // per-block FP64 partial sums (2 per block)
+    int64_t npartials = 0;
+    fe::SolverScalars* sc = nullptr;
+    fe::SolverScalars* sc_host = nullptr;  // pinned mirror
+    unsigned long long* cflmax = nullptr;
+
+    // particles (HBM, SoA, ping-pong for the reorder)
+    int64_t np = 0, cap = 0;
+    fe::Particles P[2];
+    int cur = 0;
+    uint32_t* key = nullptr;
+    int32_t* cell_count = nullptr;   // n^3 + 1
+    int32_t* cell_start = nullptr;   // n^3 + 1
+    int32_t* cursor = nullptr;       // n^3
+    int32_t* order = nullptr;        // sorted slot -> old index
+    int32_t* scan_sums = nullptr;
+    bool sorted = false;
+    bool keys_ready = false;  // bin keys + histogram already produced by the advection kernel
+
+    // timing
+    hipEvent_t ev[fe::kMaxStages + 1] = {};
+    double stage_ms[fe::kMaxStages] = {};
+    fe_solve_stats last_stats{};
+
+    // per-kernel timer: HIP events around every launch of one selected kernel
+    int32_t kt_id = 0;
+    std::vector<hipEvent_t> kt_ev;  // pairs
+    int kt_used = 0;
+    double kt_ms = 0.0;
+    int64_t kt_launches = 0;
+};
+
+namespace fe {
+
+constexpr int kTimerPairs = 1024;
+
+// Accumulate the finished pairs (call after the stream is synchronized).
+inline void kt_flush(fe_scene* s) {
+    for (int k = 0; k < s->kt_used; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s->kt_ev[2 * k], s->kt_ev[2 * k + 1]) == hipSuccess) {
+            s->kt_ms += ms;
+            s->kt_launches += 1;
+        }
+    }
+    s->kt_used = 0;
+}
+
+inline void kt_begin(fe_scene* s, int32_t id) {
+    if (s->kt_id != id || s->kt_ev.empty()) return;
+    if (s->kt_used == kTimerPairs) {
+        (void)hipStreamSynchronize(s->stream);
+        kt_flush(s);
+    }
+    (void)hipEventRecord(s->kt_ev[2 * s->kt_used], s->stream);
+}
+
+inline void kt_end(fe_scene* s, int32_t id) {
+    if (s->kt_id != id || s->kt_ev.empty()) return;
+    (void)hipEventRecord(s->kt_ev[2 * s->kt_used + 1], s->stream);
+    ++s->kt_used;
+}
+
+}  // namespace fe

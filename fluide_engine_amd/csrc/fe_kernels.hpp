@@ -1,0 +1,39 @@
+// fe_kernels.hpp -- host launchers of the gfx950 kernels (one scene, one stream).
+#pragma once
+
+#include "fe_internal.hpp"
+
+namespace fe {
+
+struct FaceSet {
+    float* a[3];
+};
+
+// k_particles.hip
+hipError_t launch_fill_i32(hipStream_t st, int32_t* a, int32_t v, int64_t n);
+hipError_t launch_keys(fe_scene* s, bool histogram);
+hipError_t launch_scan(fe_scene* s, const int32_t* in, int32_t* out, int64_t n);
+hipError_t launch_sort(fe_scene* s, bool keys_ready);
+hipError_t launch_p2g(fe_scene* s);
+hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bool emit_keys);
+hipError_t launch_unpermute(fe_scene* s, const float* src, int ncomp, float* dst);
+hipError_t launch_deinterleave(fe_scene* s, const float* src, int ncomp, float* dst);
+hipError_t launch_iota(fe_scene* s, int32_t* a, int64_t n);
+hipError_t launch_cells_by_pid(fe_scene* s, int32_t* out);
+
+// k_grid.hip
+hipError_t launch_sdf(fe_scene* s);
+hipError_t launch_markers(fe_scene* s);
+hipError_t launch_extrapolate(fe_scene* s);               // ExtrapolateVelocityToAir, depth iterations
+hipError_t launch_walls(fe_scene* s);                      // wall layers of ConstrainVelocity
+hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after);
+hipError_t launch_apply_pressure(fe_scene* s, bool walls_after);
+hipError_t launch_cfl(fe_scene* s, double dt);             // result in s->cflmax (FP64 bits)
+hipError_t launch_fill_u8(hipStream_t st, uint8_t* a, uint8_t v, int64_t n);
+hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n);
+
+// k_solver.hip
+hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d);  // rhs + solve (no apply)
+hipError_t launch_final_residual(fe_scene* s);  // FP64 ||b - A p||^2 and ||b||^2 into s->sc
+
+}  // namespace fe

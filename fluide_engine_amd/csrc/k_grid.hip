@@ -1,0 +1,424 @@
+// k_grid.hip -- grid-side kernels of the gfx950 substep: fluid SDF and cell
+// markers (bit-exact FP64 decisions), velocity extrapolation, wall boundary
+// conditions, gravity, pressure-gradient application and the CFL reduction.
+// All are HBM-bound element or 7-point stencil sweeps; launches remap blocks
+// so neighbouring tiles share an XCD's L2.
+#include "fe_kernels.hpp"
+
+namespace fe {
+
+static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+__global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t* __restrict__ a, uint8_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f32(float* __restrict__ a, float v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+hipError_t launch_fill_u8(hipStream_t st, uint8_t* a, uint8_t v, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, grid1(n), dim3(kBlock), 0, st, a, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, grid1(n), dim3(kBlock), 0, st, a, v, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// BuildSignedDistanceField (src/pic_simulator.cpp:452-475).
+// sdf(c) = min(2r, min |p - c|) - r with r = 1.2*max(h)/sqrt(2), over the
+// particles with |p-c|^2 <= (2r)^2.  The reference searches 8 hash buckets of
+// width 2r (hash_grid_particle_searcher.cpp:154-204), which always contain
+// every particle closer than r; this kernel scans the cells within r of the
+// centre (same guarantee), so sdf < 0 -- the only thing the substep uses -- and
+// its value there are bit-exact: FP64 differences, no FMA contraction and one
+// correctly rounded sqrt of the minimum squared distance (sqrt is monotone).
+// Positive values may differ from the reference's where its buckets reach
+// particles farther than r that this scan does not (they only feed the
+// surface mesher).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float* __restrict__ X, int64_t cap,
+                                                const int32_t* __restrict__ start, double* __restrict__ sdf,
+                                                int ex, int ey, int ez, double radius, double band, double qr2,
+                                                int64_t padded) {
+#pragma clang fp contract(off)
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    const int64_t c = blk * kBlock + threadIdx.x;
+    const int64_t nc = cell_count(g);
+    if (c >= nc) return;
+    const int i = (int)(c % g.n[0]);
+    const int j = (int)((c / g.n[0]) % g.n[1]);
+    const int k = (int)(c / ((int64_t)g.n[0] * g.n[1]));
+    // Grid3D::GridIndexToPosition on the SDF grid of origin o + h/2 (src/hybrid_simulator.cpp:13-15)
+    const double gx = (g.od[0] + g.hd[0] / 2.0) + (double)i * g.hd[0];
+    const double gy = (g.od[1] + g.hd[1] / 2.0) + (double)j * g.hd[1];
+    const double gz = (g.od[2] + g.hd[2] / 2.0) + (double)k * g.hd[2];
+    const int i0 = max(i - ex, 0), i1 = min(i + ex, g.n[0] - 1);
+    const int j0 = max(j - ey, 0), j1 = min(j + ey, g.n[1] - 1);
+    const int k0 = max(k - ez, 0), k1 = min(k + ez, g.n[2] - 1);
+    double best = qr2;
+    bool found = false;
+    for (int kk = k0; kk <= k1; ++kk) {
+        for (int jj = j0; jj <= j1; ++jj) {
+            const int64_t row = (int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk);
+            const int s0 = start[row + i0], s1 = start[row + i1 + 1];
+            for (int s = s0; s < s1; ++s) {
+                const double dx = (double)X[s] - gx;
+                const double dy = (double)X[cap + s] - gy;
+                const double dz = (double)X[2 * cap + s] - gz;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 <= qr2 && (!found || d2 < best)) {
+                    best = d2;
+                    found = true;
+                }
+            }
+        }
+    }
+    double minD = band;
+    if (found) minD = fmin(minD, sqrt(best));
+    sdf[c] = minD - radius;
+}
+
+hipError_t launch_sdf(fe_scene* s) {
+    const Geom& g = s->g;
+    const double maxH = fmax(g.hd[0], fmax(g.hd[1], g.hd[2]));
+    const double radius = 1.2 * maxH / std::sqrt(2.0);  // src/pic_simulator.cpp:459
+    const double band = 2.0 * radius;
+    const double qr2 = band * band;
+    int ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = (int)std::floor(radius / g.hd[a] + 0.5);
+    const int64_t nc = cell_count(g);
+    const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
+    kt_begin(s, FE_KERNEL_SDF);
+    hipLaunchKernelGGL(k_sdf, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->P[s->cur].x, s->cap,
+                       s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, padded);
+    kt_end(s, FE_KERNEL_SDF);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// SinglePhasePressureSolver::BuildMarkers (single_phase_pressure_solver.cpp:30-48):
+// FLUID iff fluidSdf.Sample(GetCellCenterPos(i,j,k)) < 0, evaluated with the
+// reference's FP64 arithmetic (ScalarGrid3D::Sample, src/3d/scalar_grid3d.cpp:33-58).
+// Without colliders the collider SDF is positive everywhere, so the other
+// cells are AIR.
+// ---------------------------------------------------------------------------
+__device__ inline void bary64(double x, int size, int& i, double& f) {
+#pragma clang fp contract(off)
+    const double s = floor(x);
+    i = (int)fmin(fmax(s, -1.0e9), 1.0e9);
+    if (size <= 1) {
+        i = 0;
+        f = 0.0;
+    } else if (i < 0) {
+        i = 0;
+        f = 0.0;
+    } else if (i > size - 2) {
+        i = size - 2;
+        f = 1.0;
+    } else {
+        f = x - s;
+    }
+}
+
+__device__ inline double lerp64(double a, double b, double f) {
+#pragma clang fp contract(off)
+    return (1 - f) * a + f * b;
+}
+
+__global__ __launch_bounds__(kBlock) void k_markers(Geom g, const double* __restrict__ sdf,
+                                                    uint8_t* __restrict__ flags, int64_t padded) {
+#pragma clang fp contract(off)
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    const int64_t c = blk * kBlock + threadIdx.x;
+    const int64_t nc = cell_count(g);
+    if (c >= nc) return;
+    const int idx[3] = {(int)(c % g.n[0]), (int)((c / g.n[0]) % g.n[1]), (int)(c / ((int64_t)g.n[0] * g.n[1]))};
+    int ii[3];
+    double ff[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double cc = (g.od[a] + 0.5 * g.hd[a]) + g.hd[a] * (double)idx[a];  // GetCellCenterPos
+        const double so = g.od[a] + g.hd[a] / 2.0;
+        bary64((cc - so) / g.hd[a], g.n[a], ii[a], ff[a]);
+    }
+    const int ip1 = min(ii[0] + 1, g.n[0] - 1), jp1 = min(ii[1] + 1, g.n[1] - 1), kp1 = min(ii[2] + 1, g.n[2] - 1);
+    auto S = [&](int a, int b, int cc) {
+        return sdf[(int64_t)a + (int64_t)g.n[0] * ((int64_t)b + (int64_t)g.n[1] * cc)];
+    };
+    const int i = ii[0], j = ii[1], k = ii[2];
+    const double v = lerp64(lerp64(lerp64(S(i, j, k), S(ip1, j, k), ff[0]), lerp64(S(i, jp1, k), S(ip1, jp1, k), ff[0]), ff[1]),
+                            lerp64(lerp64(S(i, j, kp1), S(ip1, j, kp1), ff[0]), lerp64(S(i, jp1, kp1), S(ip1, jp1, kp1), ff[0]), ff[1]),
+                            ff[2]);
+    flags[c] = v < 0 ? (uint8_t)FE_FLUID : (uint8_t)FE_AIR;
+}
+
+hipError_t launch_markers(fe_scene* s) {
+    const int64_t nc = cell_count(s->g);
+    const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_markers, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, s->g, s->sdf, s->flags, padded);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// ExtrapolateToRegion (src/common/math_utils.hpp:186-253), one iteration over
+// all three face components.  In place: a thread writes its face only when the
+// face is invalid and reads neighbours only when they are valid, and valid
+// faces are never written in the same sweep, so there is no race.
+// ---------------------------------------------------------------------------
+struct ValidSet {
+    const uint8_t* in[3];
+    uint8_t* out[3];
+};
+
+__global__ __launch_bounds__(kBlock) void k_extrapolate(Geom g, FaceSet a, ValidSet vs, int64_t padded) {
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    int64_t f = blk * kBlock + threadIdx.x;
+    int c = 0;
+    int64_t nf = face_count(g, 0);
+    if (f >= nf) {
+        f -= nf;
+        c = 1;
+        nf = face_count(g, 1);
+        if (f >= nf) {
+            f -= nf;
+            c = 2;
+            nf = face_count(g, 2);
+            if (f >= nf) return;
+        }
+    }
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
+    const uint8_t* vin = vs.in[c];
+    if (vin[f]) {
+        vs.out[c][f] = 1;
+        return;
+    }
+    const int i = (int)(f % sx);
+    const int64_t t = f / sx;
+    const int j = (int)(t % sy);
+    const int k = (int)(t / sy);
+    const int64_t dy = sx, dz = (int64_t)sx * sy;
+    float* A = a.a[c];
+    float sum = 0.0f;
+    unsigned count = 0;
+    if (i + 1 < sx && vin[f + 1]) { sum += A[f + 1]; ++count; }
+    if (i > 0 && vin[f - 1]) { sum += A[f - 1]; ++count; }
+    if (j + 1 < sy && vin[f + dy]) { sum += A[f + dy]; ++count; }
+    if (j > 0 && vin[f - dy]) { sum += A[f - dy]; ++count; }
+    if (k + 1 < sz && vin[f + dz]) { sum += A[f + dz]; ++count; }
+    if (k > 0 && vin[f - dz]) { sum += A[f - dz]; ++count; }
+    if (count > 0) {
+        A[f] = sum / (float)count;
+        vs.out[c][f] = 1;
+    } else {
+        vs.out[c][f] = 0;
+    }
+}
+
+hipError_t launch_extrapolate(fe_scene* s) {
+    const Geom& g = s->g;
+    const int64_t tot = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
+    const int64_t padded = pad_blocks((tot + kBlock - 1) / kBlock);
+    FaceSet a{{s->u[0], s->u[1], s->u[2]}};
+    const int64_t off[3] = {0, face_count(g, 0), face_count(g, 0) + face_count(g, 1)};
+    for (int it = 0; it < s->depth; ++it) {
+        ValidSet vs;
+        for (int c = 0; c < 3; ++c) {
+            vs.in[c] = it == 0 ? s->mark[c] : s->valid[(it - 1) & 1] + off[c];
+            vs.out[c] = s->valid[it & 1] + off[c];
+        }
+        hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, a, vs, padded);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Wall layers of BlockedBoundryConditionSolver::ConstrainVelocity
+// (src/fluid_solvers/blocked_boundry_condition_solver.cpp:198-244): u at i=0
+// and i=nx, v at j=0 and j=ny, w at k=0 and k=nz.  The collider passes of the
+// same function are identities when no collider is present.
+// ---------------------------------------------------------------------------
+__device__ inline bool is_wall(const Geom& g, int c, int i, int j, int k) {
+    const int q = c == 0 ? i : (c == 1 ? j : k);
+    return q == 0 || q == g.n[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_walls(Geom g, FaceSet a) {
+    // flat index over the 6 wall layers
+    const int64_t nyz = (int64_t)g.n[1] * g.n[2], nxz = (int64_t)g.n[0] * g.n[2], nxy = (int64_t)g.n[0] * g.n[1];
+    int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t < 2 * nyz) {
+        const int side = (int)(t / nyz);
+        const int64_t r = t % nyz;
+        const int j = (int)(r % g.n[1]), k = (int)(r / g.n[1]);
+        const int i = side ? g.n[0] : 0;
+        a.a[0][(int64_t)i + (int64_t)(g.n[0] + 1) * ((int64_t)j + (int64_t)g.n[1] * k)] = 0.0f;
+        return;
+    }
+    t -= 2 * nyz;
+    if (t < 2 * nxz) {
+        const int side = (int)(t / nxz);
+        const int64_t r = t % nxz;
+        const int i = (int)(r % g.n[0]), k = (int)(r / g.n[0]);
+        const int j = side ? g.n[1] : 0;
+        a.a[1][(int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)(g.n[1] + 1) * k)] = 0.0f;
+        return;
+    }
+    t -= 2 * nxz;
+    if (t < 2 * nxy) {
+        const int side = (int)(t / nxy);
+        const int64_t r = t % nxy;
+        const int i = (int)(r % g.n[0]), j = (int)(r / g.n[0]);
+        const int k = side ? g.n[2] : 0;
+        a.a[2][(int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k)] = 0.0f;
+    }
+}
+
+hipError_t launch_walls(fe_scene* s) {
+    const Geom& g = s->g;
+    const int64_t tot = 2 * ((int64_t)g.n[1] * g.n[2] + (int64_t)g.n[0] * g.n[2] + (int64_t)g.n[0] * g.n[1]);
+    FaceSet a{{s->u[0], s->u[1], s->u[2]}};
+    hipLaunchKernelGGL(k_walls, grid1(tot), dim3(kBlock), 0, s->stream, g, a);
+    return hipGetLastError();
+}
+
+// ExternalForce::ApplyExternalForce with a DirectionalField (src/forces/external_force.cpp:4-30):
+// u += dt*g on every face, followed (optionally) by the wall layers of the
+// ApplyBoundryCondition that ComputeExternalForces runs next (src/pic_simulator.cpp:203-212).
+__global__ __launch_bounds__(kBlock) void k_gravity(Geom g, FaceSet a, float gx, float gy, float gz, int walls,
+                                                    int64_t padded) {
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    int64_t f = blk * kBlock + threadIdx.x;
+    int c = 0;
+    int64_t nf = face_count(g, 0);
+    if (f >= nf) {
+        f -= nf;
+        c = 1;
+        nf = face_count(g, 1);
+        if (f >= nf) {
+            f -= nf;
+            c = 2;
+            nf = face_count(g, 2);
+            if (f >= nf) return;
+        }
+    }
+    const float add = c == 0 ? gx : (c == 1 ? gy : gz);
+    float v = a.a[c][f] + add;
+    if (walls) {
+        const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
+        const int i = (int)(f % sx);
+        const int64_t t = f / sx;
+        const int j = (int)(t % sy), k = (int)(t / sy);
+        if (is_wall(g, c, i, j, k)) v = 0.0f;
+    }
+    a.a[c][f] = v;
+}
+
+hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after) {
+    const Geom& g = s->g;
+    const int64_t tot = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
+    const int64_t padded = pad_blocks((tot + kBlock - 1) / kBlock);
+    FaceSet a{{s->u[0], s->u[1], s->u[2]}};
+    hipLaunchKernelGGL(k_gravity, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, a,
+                       (float)(dt * s->desc.gravity[0]), (float)(dt * s->desc.gravity[1]),
+                       (float)(dt * s->desc.gravity[2]), walls_after ? 1 : 0, padded);
+    return hipGetLastError();
+}
+
+// SinglePhasePressureSolver::ApplyPressure (single_phase_pressure_solver.cpp:56-92):
+// only the +side face of each FLUID cell is corrected, when the +side cell is
+// in range and not BOUNDRY: u(i+1) += (p(i+1) - p(i)) / h.  Optionally fused
+// with the wall layers of the ApplyBoundryCondition that follows
+// (src/pic_simulator.cpp:224-229).
+__global__ __launch_bounds__(kBlock) void k_apply_pressure(Geom g, FaceSet a, const double* __restrict__ p,
+                                                           const uint8_t* __restrict__ flags, int walls,
+                                                           int64_t padded) {
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    int64_t f = blk * kBlock + threadIdx.x;
+    int c = 0;
+    int64_t nf = face_count(g, 0);
+    if (f >= nf) {
+        f -= nf;
+        c = 1;
+        nf = face_count(g, 1);
+        if (f >= nf) {
+            f -= nf;
+            c = 2;
+            nf = face_count(g, 2);
+            if (f >= nf) return;
+        }
+    }
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
+    const int i = (int)(f % sx);
+    const int64_t t = f / sx;
+    const int j = (int)(t % sy), k = (int)(t / sy);
+    const int q = c == 0 ? i : (c == 1 ? j : k);
+    float v = a.a[c][f];
+    if (q >= 1 && q <= g.n[c] - 1) {
+        const int64_t cr = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);  // cell on the + side
+        const int64_t step = c == 0 ? 1 : (c == 1 ? (int64_t)g.n[0] : (int64_t)g.n[0] * g.n[1]);
+        const int64_t cl = cr - step;
+        if (flags[cl] == FE_FLUID && flags[cr] != FE_BOUNDRY) v = (float)((double)v + g.ihd[c] * (p[cr] - p[cl]));
+    }
+    if (walls && (q == 0 || q == g.n[c])) v = 0.0f;
+    a.a[c][f] = v;
+}
+
+hipError_t launch_apply_pressure(fe_scene* s, bool walls_after) {
+    const Geom& g = s->g;
+    const int64_t tot = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
+    const int64_t padded = pad_blocks((tot + kBlock - 1) / kBlock);
+    FaceSet a{{s->u[0], s->u[1], s->u[2]}};
+    hipLaunchKernelGGL(k_apply_pressure, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, a, s->p, s->flags,
+                       walls_after ? 1 : 0, padded);
+    return hipGetLastError();
+}
+
+// PICSimulator::Cfl (src/pic_simulator.cpp:80-97): max over cells of
+// |ValueAtCellCenter| + dt*(0,-9.8,0), in FP64; max is order independent, so
+// an atomic max on the (non-negative) FP64 bit pattern is deterministic.
+__global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, unsigned long long* __restrict__ out) {
+    __shared__ double smem[kBlock / 64];
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double m = 0.0;
+    if (c < cell_count(g)) {
+        const int i = (int)(c % g.n[0]);
+        const int j = (int)((c / g.n[0]) % g.n[1]);
+        const int k = (int)(c / ((int64_t)g.n[0] * g.n[1]));
+        const int64_t iu = (int64_t)i + (int64_t)(g.n[0] + 1) * ((int64_t)j + (int64_t)g.n[1] * k);
+        const int64_t iv = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)(g.n[1] + 1) * k);
+        const int64_t iw = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);
+        const double cx = 0.5 * ((double)a.a[0][iu] + (double)a.a[0][iu + 1]);
+        const double cy = 0.5 * ((double)a.a[1][iv] + (double)a.a[1][iv + g.n[0]]);
+        const double cz = 0.5 * ((double)a.a[2][iw] + (double)a.a[2][iw + (int64_t)g.n[0] * g.n[1]]);
+        m = fmax(m, fabs(cx) + dt * 0.0);
+        m = fmax(m, fabs(cy) + dt * -9.8);
+        m = fmax(m, fabs(cz) + dt * 0.0);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = smem[0];
+        for (int w = 1; w < kBlock / 64; ++w) b = fmax(b, smem[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+hipError_t launch_cfl(fe_scene* s, double dt) {
+    const int64_t nc = cell_count(s->g);
+    hipError_t e = hipMemsetAsync(s->cflmax, 0, sizeof(unsigned long long), s->stream);
+    if (e != hipSuccess) return e;
+    FaceSet a{{s->u[0], s->u[1], s->u[2]}};
+    hipLaunchKernelGGL(k_cfl, grid1(nc), dim3(kBlock), 0, s->stream, s->g, a, dt, s->cflmax);
+    return hipGetLastError();
+}
+
+}  // namespace fe

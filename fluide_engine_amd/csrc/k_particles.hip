@@ -1,0 +1,558 @@
+// k_particles.hip -- particle-side kernels of the gfx950 substep:
+//   * counting sort of particles by cell (deterministic: bins hold their
+//     particles in ascending previous order),
+//   * P2G as a per-face gather over the sorted cell bins (no float atomics),
+//   * fused G2P + midpoint advection (+ the next substep's bin keys).
+#include "fe_kernels.hpp"
+
+namespace fe {
+
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t* __restrict__ a, int32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// Bin key of each particle + histogram.
+__global__ __launch_bounds__(kBlock) void k_keys(Geom g, const float* __restrict__ X, int64_t cap, int64_t np,
+                                                 uint32_t* __restrict__ key, int32_t* __restrict__ count) {
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const int cx = cell_axis(g, 0, X[p]);
+    const int cy = cell_axis(g, 1, X[cap + p]);
+    const int cz = cell_axis(g, 2, X[2 * cap + p]);
+    const uint32_t k = (uint32_t)cx + (uint32_t)g.n[0] * ((uint32_t)cy + (uint32_t)g.n[1] * (uint32_t)cz);
+    key[p] = k;
+    if (count) atomicAdd(&count[k], 1);
+}
+
+// Exclusive scan of int32 counts, three passes of 1024 items per block.
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ inline int block_exclusive_scan(int val, int* smem, int& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = val;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) smem[wid] = x;
+    __syncthreads();
+    int pre = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        int s = smem[w];
+        if (w < wid) pre += s;
+        total += s;
+    }
+    __syncthreads();
+    return x - val + pre;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const int32_t* __restrict__ in, int64_t n,
+                                                       int32_t* __restrict__ tile_sums) {
+    __shared__ int smem[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q)
+        if (base + q < n) s += in[base + q];
+    int total;
+    block_exclusive_scan(s, smem, total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sums(int32_t* __restrict__ sums, int64_t ntiles) {
+    __shared__ int smem[kBlock / 64];
+    int carry = 0;
+    for (int64_t off = 0; off < ntiles; off += kBlock) {
+        const int64_t i = off + threadIdx.x;
+        int v = i < ntiles ? sums[i] : 0;
+        int total;
+        int ex = block_exclusive_scan(v, smem, total);
+        if (i < ntiles) sums[i] = ex + carry;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t* __restrict__ in, int64_t n,
+                                                       const int32_t* __restrict__ tile_off,
+                                                       int32_t* __restrict__ out) {
+    __shared__ int smem[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        v[q] = base + q < n ? in[base + q] : 0;
+        s += v[q];
+    }
+    int total;
+    int ex = block_exclusive_scan(s, smem, total) + tile_off[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        if (base + q < n) out[base + q] = ex;
+        ex += v[q];
+    }
+    if (base + kScanItems >= n && base < n) out[n] = ex;  // total after the last item
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key, int64_t np,
+                                                    const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
+                                                    int32_t* __restrict__ order) {
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const uint32_t k = key[p];
+    const int slot = start[k] + atomicAdd(&cursor[k], 1);
+    order[slot] = (int32_t)p;
+}
+
+// Atomic slots give each bin its particles in arbitrary order; sorting every
+// bin by previous index restores a deterministic, stable order.
+__global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict__ start, int64_t ncells,
+                                                      int32_t* __restrict__ order) {
+    int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncells) return;
+    const int s0 = start[c], s1 = start[c + 1];
+    for (int a = s0 + 1; a < s1; ++a) {
+        const int32_t v = order[a];
+        int b = a - 1;
+        while (b >= s0 && order[b] > v) {
+            order[b + 1] = order[b];
+            --b;
+        }
+        order[b + 1] = v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ order, int64_t np, int64_t cap,
+                                                   Particles src, Particles dst, int with_c) {
+    int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= np) return;
+    const int64_t o = order[s];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        dst.x[a * cap + s] = src.x[a * cap + o];
+        dst.v[a * cap + s] = src.v[a * cap + o];
+    }
+    if (with_c) {
+#pragma unroll
+        for (int a = 0; a < 9; ++a) dst.C[a * cap + s] = src.C[a * cap + o];
+    }
+    dst.pid[s] = src.pid[o];
+}
+
+// ---------------------------------------------------------------------------
+// P2G: TransferParticles2Grid as a deterministic gather.
+// PIC: src/pic_simulator.cpp:356-437; FLIP: src/flip_simulator.cpp:25-119;
+// APIC: src/apic_simulator.cpp:15-118.
+// One thread per face of component C.  The particles whose stencil can touch
+// face F lie in cells [F-1, F+1] on the transverse axes and [F-1, F] on the
+// normal axis (plus cell n-1 for face n-2 when PIC/APIC clamp the normal axis
+// at n-2).  Cells are x-fastest and particles sorted by cell, so each (j,k)
+// row of candidate cells is one contiguous particle range.  For every
+// candidate the reference's own stencil (GetCooridnatesAndWeights with the
+// scheme's stencil size) is evaluated and the particle contributes iff the face
+// is one of its 8 nodes; the face is marked even for a zero weight.
+// ---------------------------------------------------------------------------
+template <int C, int METHOD>
+__global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ X, const float* __restrict__ V,
+                                                const float* __restrict__ Cm, int64_t cap,
+                                                const int32_t* __restrict__ start, float* __restrict__ out,
+                                                uint8_t* __restrict__ mark, float* __restrict__ snap,
+                                                int64_t padded) {
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
+    const int64_t nf = (int64_t)sx * sy * sz;
+    const int64_t f = blk * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const int I = (int)(f % sx);
+    const int64_t t = f / sx;
+    const int J = (int)(t % sy);
+    const int K = (int)(t / sy);
+    const int F[3] = {I, J, K};
+    int lo[3], hi[3], S[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a == C) {
+            lo[a] = F[a] - 1;
+            hi[a] = F[a];
+            if (METHOD != FE_FLIP && F[a] == g.n[a] - 2) hi[a] = F[a] + 1;
+        } else {
+            lo[a] = F[a] - 1;
+            hi[a] = F[a] + 1;
+        }
+        lo[a] = max(lo[a], 0);
+        hi[a] = min(hi[a], g.n[a] - 1);
+        S[a] = stencil_size(g, C, a);
+    }
+    // APIC: face position GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
+    // and the clamp box [o + h/2, o + L - h/2] of the transverse axes (apic_simulator.cpp:54-63).
+    float gp[3], clo[3], chi[3];
+    if (METHOD == FE_APIC) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            gp[a] = g.co[C][a] + g.h[a] * (float)F[a];
+            clo[a] = g.o[a] + 0.5f * g.h[a];
+            chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
+        }
+    }
+    float num = 0.0f, den = 0.0f;
+    int marked = 0;
+    for (int kc = lo[2]; kc <= hi[2]; ++kc) {
+        for (int jc = lo[1]; jc <= hi[1]; ++jc) {
+            const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
+            const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
+            for (int s = s0; s < s1; ++s) {
+                const float px = X[s], py = X[cap + s], pz = X[2 * cap + s];
+                int i0;
+                float fr;
+                stencil_axis(g, C, 0, px, S[0], i0, fr);
+                float wx;
+                if (I == i0) wx = 1.0f - fr;
+                else if (I == i0 + 1) wx = fr;
+                else continue;
+                stencil_axis(g, C, 1, py, S[1], i0, fr);
+                float wy;
+                if (J == i0) wy = 1.0f - fr;
+                else if (J == i0 + 1) wy = fr;
+                else continue;
+                stencil_axis(g, C, 2, pz, S[2], i0, fr);
+                float wz;
+                if (K == i0) wz = 1.0f - fr;
+                else if (K == i0 + 1) wz = fr;
+                else continue;
+                marked = 1;
+                const float w = (wx * wy) * wz;
+                float val = V[C * cap + s];
+                if (METHOD == FE_APIC) {
+                    float pc[3] = {px, py, pz};
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
+                    const float* cr = Cm + (int64_t)(3 * C) * cap + s;
+                    val += cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]);
+                }
+                num += val * w;
+                den += w;
+            }
+        }
+    }
+    const float res = den > 0.0f ? num / den : 0.0f;
+    out[f] = res;
+    mark[f] = (uint8_t)marked;
+    if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+}
+
+// ---------------------------------------------------------------------------
+// G2P (PIC: src/pic_simulator.cpp:439-450; FLIP: src/flip_simulator.cpp:121-161;
+// APIC: src/apic_simulator.cpp:120-176) fused with MoveParticles
+// (src/pic_simulator.cpp:239-312) and the bin keys of the next substep.
+// ---------------------------------------------------------------------------
+__device__ inline float sample_delta(const float* __restrict__ a, const float* __restrict__ s, const Geom& g, int c,
+                                     float px, float py, float pz) {
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
+    int i, j, k;
+    float fx, fy, fz;
+    fast_axis(px, g.co[c][0], g.ih[0], sx, i, fx);
+    fast_axis(py, g.co[c][1], g.ih[1], sy, j, fy);
+    fast_axis(pz, g.co[c][2], g.ih[2], sz, k, fz);
+    const int ip1 = min(i + 1, sx - 1), jp1 = min(j + 1, sy - 1), kp1 = min(k + 1, sz - 1);
+    auto at = [&](int ii, int jj, int kk) {
+        const int64_t id = (int64_t)ii + (int64_t)sx * ((int64_t)jj + (int64_t)sy * kk);
+        return a[id] - s[id];  // FLIP delta = new - snapshot, node by node
+    };
+    return lerpf_ref(lerpf_ref(lerpf_ref(at(i, j, k), at(ip1, j, k), fx), lerpf_ref(at(i, jp1, k), at(ip1, jp1, k), fx), fy),
+                     lerpf_ref(lerpf_ref(at(i, j, kp1), at(ip1, j, kp1), fx), lerpf_ref(at(i, jp1, kp1), at(ip1, jp1, kp1), fx), fy),
+                     fz);
+}
+
+// GetCooridnatesAndGradientWeights (src/common/math_utils.hpp:114-176) contracted
+// with the face values: returns sum_q gradw_q * a(node_q).
+__device__ inline void grad_contract(const float* __restrict__ a, const Geom& g, int c, const float p[3],
+                                     float out[3]) {
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
+    int i, j, k;
+    float fx, fy, fz;
+    fast_axis(p[0], g.co[c][0], g.ih[0], g.n[0], i, fx);  // APIC passes the cell dims (Q1)
+    fast_axis(p[1], g.co[c][1], g.ih[1], g.n[1], j, fy);
+    fast_axis(p[2], g.co[c][2], g.ih[2], g.n[2], k, fz);
+    const int ip1 = min(i + 1, g.n[0] - 1), jp1 = min(j + 1, g.n[1] - 1), kp1 = min(k + 1, g.n[2] - 1);
+    const float ix = g.ih[0], iy = g.ih[1], iz = g.ih[2];
+    auto A = [&](int ii, int jj, int kk) { return a[(int64_t)ii + (int64_t)sx * ((int64_t)jj + (int64_t)sy * kk)]; };
+    const float v0 = A(i, j, k), v1 = A(ip1, j, k), v2 = A(i, jp1, k), v3 = A(ip1, jp1, k);
+    const float v4 = A(i, j, kp1), v5 = A(ip1, j, kp1), v6 = A(i, jp1, kp1), v7 = A(ip1, jp1, kp1);
+    out[0] = (-ix * (1 - fy) * (1 - fz)) * v0 + (ix * (1 - fy) * (1 - fz)) * v1 + ((-ix) * fy * (1 - fz)) * v2 +
+             (ix * fy * (1 - fz)) * v3 + ((-ix) * (1 - fy) * fz) * v4 + (ix * (1 - fy) * fz) * v5 +
+             ((-ix) * fy * fz) * v6 + (ix * fy * fz) * v7;
+    out[1] = (-iy * (1 - fx) * (1 - fz)) * v0 + (fx * (-iy) * (1 - fz)) * v1 + ((1 - fx) * iy * (1 - fz)) * v2 +
+             (fx * iy * (1 - fz)) * v3 + ((1 - fx) * (-iy) * fz) * v4 + (fx * (-iy) * fz) * v5 +
+             ((1 - fx) * iy * fz) * v6 + (fx * iy * fz) * v7;
+    out[2] = (-iz * (1 - fx) * (1 - fy)) * v0 + (fx * (1 - fy) * (-iz)) * v1 + ((1 - fx) * fy * (-iz)) * v2 +
+             (fx * fy * (-iz)) * v3 + ((1 - fx) * (1 - fy) * iz) * v4 + (fx * (1 - fy) * iz) * v5 +
+             ((1 - fx) * fy * iz) * v6 + (fx * fy * iz) * v7;
+}
+
+template <int METHOD>
+__global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_t cap, int64_t np, FaceSet grid,
+                                                     FaceSet snapset, float blend, float dt, int nsub, int do_g2p,
+                                                     int do_move, uint32_t* __restrict__ key,
+                                                     int32_t* __restrict__ count) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= np) return;
+    float x = P.x[s], y = P.x[cap + s], z = P.x[2 * cap + s];
+    float v[3] = {P.v[s], P.v[cap + s], P.v[2 * cap + s]};
+    if (do_g2p) {
+        float pic[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pic[c] = sample_face(grid.a[c], g, c, x, y, z);
+        if (METHOD == FE_PIC) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = pic[c];
+        } else if (METHOD == FE_FLIP) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float fl = v[c] + sample_delta(grid.a[c], snapset.a[c], g, c, x, y, z);
+                if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
+                v[c] = fl;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = pic[c];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float pc[3] = {x, y, z};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (a == c) continue;
+                    const float lo = g.o[a] + 0.5f * g.h[a], hi = g.o[a] + g.L[a] - 0.5f * g.h[a];
+                    pc[a] = pc[a] < lo ? lo : (pc[a] > hi ? hi : pc[a]);
+                }
+                float cr[3];
+                grad_contract(grid.a[c], g, c, pc, cr);
+                P.C[(int64_t)(3 * c) * cap + s] = cr[0];
+                P.C[(int64_t)(3 * c + 1) * cap + s] = cr[1];
+                P.C[(int64_t)(3 * c + 2) * cap + s] = cr[2];
+            }
+        }
+    }
+    if (do_move) {
+        const float sdt = dt / (float)nsub;
+        float p0[3] = {x, y, z}, p1[3] = {x, y, z};
+        for (int t = 0; t < nsub; ++t) {
+            float v0[3], mid[3], mv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v0[c] = sample_face(grid.a[c], g, c, p0[0], p0[1], p0[2]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) mid[a] = p0[a] + 0.5f * sdt * v0[a];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mv[c] = sample_face(grid.a[c], g, c, mid[0], mid[1], mid[2]);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                p1[a] = p0[a] + sdt * mv[a];
+                p0[a] = p1[a];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float lo = g.o[a], hi = g.o[a] + g.L[a];
+            if (p1[a] <= lo) {
+                p1[a] = lo;
+                v[a] = 0.0f;
+            }
+            if (p1[a] >= hi) {
+                p1[a] = hi;
+                v[a] = 0.0f;
+            }
+        }
+        x = p1[0];
+        y = p1[1];
+        z = p1[2];
+        P.x[s] = x;
+        P.x[cap + s] = y;
+        P.x[2 * cap + s] = z;
+    }
+    P.v[s] = v[0];
+    P.v[cap + s] = v[1];
+    P.v[2 * cap + s] = v[2];
+    if (key) {
+        const int cx = cell_axis(g, 0, x), cy = cell_axis(g, 1, y), cz = cell_axis(g, 2, z);
+        const uint32_t k = (uint32_t)cx + (uint32_t)g.n[0] * ((uint32_t)cy + (uint32_t)g.n[1] * (uint32_t)cz);
+        key[s] = k;
+        atomicAdd(&count[k], 1);
+    }
+}
+
+// Scatter sorted particles back to upload order (download path).
+__global__ __launch_bounds__(kBlock) void k_unpermute(const float* __restrict__ src, int ncomp, int64_t cap,
+                                                      int64_t np, const int32_t* __restrict__ pid,
+                                                      float* __restrict__ dst) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= np) return;
+    const int64_t o = pid[s];
+    for (int c = 0; c < ncomp; ++c) dst[o * ncomp + c] = src[c * cap + s];
+}
+
+// Interleaved upload (xyz xyz ...) -> SoA planes.
+__global__ __launch_bounds__(kBlock) void k_deinterleave(const float* __restrict__ src, int ncomp, int64_t cap,
+                                                         int64_t np, float* __restrict__ dst) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s >= np) return;
+    for (int c = 0; c < ncomp; ++c) dst[c * cap + s] = src[s * ncomp + c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(int32_t* __restrict__ a, int64_t n) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) a[s] = (int32_t)s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cells_by_pid(const uint32_t* __restrict__ key,
+                                                         const int32_t* __restrict__ pid, int64_t np,
+                                                         int32_t* __restrict__ out) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s < np) out[pid[s]] = (int32_t)key[s];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+hipError_t launch_fill_i32(hipStream_t st, int32_t* a, int32_t v, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, grid1(n), dim3(kBlock), 0, st, a, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys(fe_scene* s, bool histogram) {
+    const Particles& P = s->P[s->cur];
+    if (s->np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P.x, s->cap, s->np, s->key,
+                       histogram ? s->cell_count : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(fe_scene* s, const int32_t* in, int32_t* out, int64_t n) {
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, s->stream, in, n, s->scan_sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s->stream, s->scan_sums, tiles);
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)tiles), dim3(kBlock), 0, s->stream, in, n, s->scan_sums, out);
+    return hipGetLastError();
+}
+
+// Bins the particles of P[cur] by cell into P[1-cur]; keys and counts must be
+// ready when keys_ready (the fused advection kernel produced them).
+hipError_t launch_sort(fe_scene* s, bool keys_ready) {
+    const int64_t nc = cell_count(s->g);
+    hipError_t e;
+    if (!keys_ready) {
+        if ((e = launch_fill_i32(s->stream, s->cell_count, 0, nc + 1)) != hipSuccess) return e;
+        if ((e = launch_keys(s, true)) != hipSuccess) return e;
+    }
+    if ((e = launch_scan(s, s->cell_count, s->cell_start, nc)) != hipSuccess) return e;
+    if (s->np > 0) {
+        if ((e = launch_fill_i32(s->stream, s->cursor, 0, nc)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scatter, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->np, s->cell_start,
+                           s->cursor, s->order);
+        hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order);
+        hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->order, s->np, s->cap,
+                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        s->cur = 1 - s->cur;
+    }
+    s->sorted = true;
+    return hipSuccess;
+}
+
+template <int METHOD>
+static hipError_t p2g_method(fe_scene* s) {
+    const Particles& P = s->P[s->cur];
+    const Geom& g = s->g;
+    for (int c = 0; c < 3; ++c) {
+        const int64_t nf = face_count(g, c);
+        const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
+        auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
+        kt_begin(s, FE_KERNEL_P2G);
+        hipLaunchKernelGGL(kern, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, P.x, P.v, P.C, s->cap,
+                           s->cell_start, s->u[c], s->mark[c], s->snap[c], padded);
+        kt_end(s, FE_KERNEL_P2G);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_p2g(fe_scene* s) {
+    switch (s->g.method) {
+        case FE_PIC: return p2g_method<FE_PIC>(s);
+        case FE_FLIP: return p2g_method<FE_FLIP>(s);
+        default: return p2g_method<FE_APIC>(s);
+    }
+}
+
+hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bool emit_keys) {
+    if (s->np == 0) return hipSuccess;
+    const Particles& P = s->P[s->cur];
+    FaceSet grid{{s->u[0], s->u[1], s->u[2]}};
+    FaceSet snaps{{s->snap[0], s->snap[1], s->snap[2]}};
+    hipError_t e;
+    if (emit_keys) {
+        if ((e = launch_fill_i32(s->stream, s->cell_count, 0, cell_count(s->g) + 1)) != hipSuccess) return e;
+    }
+    uint32_t* key = emit_keys ? s->key : nullptr;
+    int32_t* cnt = emit_keys ? s->cell_count : nullptr;
+    const float blend = (float)s->desc.blend;
+    kt_begin(s, FE_KERNEL_G2P_MOVE);
+    switch (s->g.method) {
+        case FE_PIC:
+            hipLaunchKernelGGL(k_g2p_move<FE_PIC>, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P, s->cap, s->np,
+                               grid, snaps, blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key, cnt);
+            break;
+        case FE_FLIP:
+            hipLaunchKernelGGL(k_g2p_move<FE_FLIP>, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P, s->cap,
+                               s->np, grid, snaps, blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key,
+                               cnt);
+            break;
+        default:
+            hipLaunchKernelGGL(k_g2p_move<FE_APIC>, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P, s->cap,
+                               s->np, grid, snaps, blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key,
+                               cnt);
+            break;
+    }
+    kt_end(s, FE_KERNEL_G2P_MOVE);
+    if (do_move) s->sorted = false;
+    return hipGetLastError();
+}
+
+hipError_t launch_unpermute(fe_scene* s, const float* src, int ncomp, float* dst) {
+    if (s->np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpermute, grid1(s->np), dim3(kBlock), 0, s->stream, src, ncomp, s->cap, s->np,
+                       s->P[s->cur].pid, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave(fe_scene* s, const float* src, int ncomp, float* dst) {
+    if (s->np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deinterleave, grid1(s->np), dim3(kBlock), 0, s->stream, src, ncomp, s->cap, s->np, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota(fe_scene* s, int32_t* a, int64_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iota, grid1(n), dim3(kBlock), 0, s->stream, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_by_pid(fe_scene* s, int32_t* out) {
+    if (s->np == 0) return hipSuccess;
+    hipError_t e = launch_keys(s, false);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cells_by_pid, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->P[s->cur].pid, s->np,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace fe

@@ -1,0 +1,214 @@
+/* fluide_hip.h -- C ABI of the MI355X (gfx950) backend for the Fluide_Engine
+ * hybrid particle-grid substep.
+ *
+ * The reference has no C ABI: its hot path sits behind C++ classes
+ * (PhysicsAnimation / HybridSimulator / PICSimulator / FLIPSimulator /
+ * APICSimulator and the experimental CudaPICSimulator).  These entry points are
+ * what those classes (and the PyFluidEngine module) call instead, at scene
+ * granularity: the reference's per-stage objects take host Array3s, which a GPU
+ * backend cannot sit behind without a device round trip per stage.
+ *
+ * Conventions: every function returns 0 (FE_OK) or a negative FE_E_* code and
+ * never throws; fe_last_error() holds the message.  Grids are x-fastest
+ * (i + sx*(j + sy*k)), like Array3 (src/common/array3.hpp:65-85).  Particle
+ * vectors are interleaved xyz, like std::vector<Vector3<double>> in
+ * ParticleSystem (src/particle_systems/particle_system.hpp:78-79), but FP32.
+ * All device memory is owned by the scene; host buffers by the caller.
+ * Calls on one scene are synchronous and must come from one host thread at a
+ * time; distinct scenes are independent.
+ */
+#ifndef FLUIDE_HIP_H
+#define FLUIDE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fe_status {
+    FE_OK = 0,
+    FE_E_INVALID = -1,     /* bad argument, size or state */
+    FE_E_HIP = -2,         /* a HIP runtime call failed (message names it) */
+    FE_E_NOMEM = -3,       /* device or host allocation failed */
+    FE_E_UNSUPPORTED = -4, /* a feature this backend does not implement */
+    FE_E_NO_DEVICE = -5    /* no usable gfx950 device */
+};
+
+/* Transfer scheme: PICSimulator (src/pic_simulator.hpp:27), FLIPSimulator
+ * (src/flip_simulator.hpp:7), APICSimulator (src/apic_simulator.hpp:8). */
+enum fe_method { FE_PIC = 0, FE_FLIP = 1, FE_APIC = 2 };
+
+/* Cell markers: FluidMarker (src/grid_systems/fluid_markers.hpp:8-13). */
+enum fe_marker { FE_AIR = 0, FE_FLUID = 1, FE_BOUNDRY = 2 };
+
+/* Linear solver for the pressure projection.
+ * JACOBI: JacobiIterationSolver / CudaJacobiIterationSolver semantics
+ *   (src/linear_system/jacobi_iteration_solver.cpp:14-37): max_iterations sweeps,
+ *   stop when ||b-Ax||_2 < tolerance, checked every check_interval sweeps.
+ * CG: ConjugateGradientSolver (src/linear_system/conjugate_gradient_solver.cpp:14-55),
+ *   identity preconditioner, residual replacement every 50 iterations or when
+ *   sigma grows; stop on sigma <= tol^2, or ||r||/||b|| <= tol when relative.
+ * PCG_JACOBI: CG with the diagonal preconditioner (no reference counterpart).
+ * Vectors are FP32; dot products and norms accumulate in FP64. */
+enum fe_solver_kind { FE_SOLVER_JACOBI = 0, FE_SOLVER_CG = 1, FE_SOLVER_PCG_JACOBI = 2 };
+
+typedef struct fe_solver_desc {
+    int32_t kind;
+    int32_t max_iterations;
+    int32_t check_interval; /* Jacobi residual check interval (reference default 5) */
+    int32_t relative;       /* CG: relative stopping rule */
+    int32_t warm_start;     /* start from the previous pressure (reference behaviour) */
+    int32_t reserved;
+    double tolerance;
+} fe_solver_desc;
+
+typedef struct fe_solve_stats {
+    int32_t iterations;
+    int32_t converged;
+    double rel_residual; /* ||b - A p||_2 / ||b||_2 on the final p, in FP64 */
+    double abs_residual;
+    double ms;           /* device time of build + solve + apply, HIP events */
+} fe_solve_stats;
+
+/* Scene description: HybridSimulator(gridSize, domain) (src/hybrid_simulator.cpp:4-25)
+ * plus the simulator settings the reference keeps in members. */
+typedef struct fe_scene_desc {
+    int32_t n[3];         /* cells per axis */
+    int32_t method;       /* fe_method */
+    double origin[3];     /* BoundingBox3D origin */
+    double size[3];       /* BoundingBox3D size; h = size / n */
+    double blend;         /* FLIP PIC weight (_blendingFactor, src/flip_simulator.cpp:5) */
+    double max_cfl;       /* _maxCfl (src/hybrid_simulator.cpp:21) */
+    double gravity[3];    /* DirectionalField strength (src/forces/directional_field.cpp:13-16) */
+    double viscosity;     /* must be 0: BackwardEulerDiffusionSolver is not on this backend */
+    double density;       /* unused by the reference's pressure solve (kept for the API) */
+    int64_t max_particles;
+} fe_scene_desc;
+
+/* Grid fields for fe_grid_download / fe_grid_upload. */
+enum fe_grid_field {
+    FE_GRID_U = 0,       /* float, (nx+1)*ny*nz */
+    FE_GRID_V = 1,       /* float, nx*(ny+1)*nz */
+    FE_GRID_W = 2,       /* float, nx*ny*(nz+1) */
+    FE_GRID_MU = 3,      /* uint8 face markers of u (Fluid3::xMarkers) */
+    FE_GRID_MV = 4,
+    FE_GRID_MW = 5,
+    FE_GRID_SU = 6,      /* float, FLIP snapshot of u (FLIPSimulator::_velocityResiduals) */
+    FE_GRID_SV = 7,
+    FE_GRID_SW = 8,
+    FE_GRID_SDF = 9,     /* double, fluid SDF at cell centres (Fluid3::sdf) */
+    FE_GRID_FLAGS = 10,  /* uint8 cell markers (fe_marker) */
+    FE_GRID_PRESSURE = 11, /* double, pressure (LinearSystem::x) */
+    FE_GRID_RHS = 12     /* float, divergence right-hand side (LinearSystem::b) */
+};
+
+/* Stages of one substep, for fe_run_stage (PICSimulator::OnAdvanceTimeStep,
+ * src/pic_simulator.cpp:111-237). */
+enum fe_stage {
+    FE_STAGE_SORT = 0,          /* bin particles by cell (no reference counterpart) */
+    FE_STAGE_P2G = 1,           /* TransferParticles2Grid */
+    FE_STAGE_SDF = 2,           /* BuildSignedDistanceField (+ cell markers) */
+    FE_STAGE_EXTRAPOLATE = 3,   /* ExtrapolateVelocityToAir */
+    FE_STAGE_BC = 4,            /* ApplyBoundryCondition (walls) */
+    FE_STAGE_GRAVITY = 5,       /* ExternalForce::ApplyExternalForce */
+    FE_STAGE_PRESSURE = 6,      /* SinglePhasePressureSolver::Solve */
+    FE_STAGE_G2P = 7,           /* TransferGrid2Particles */
+    FE_STAGE_MOVE = 8,          /* MoveParticles */
+    FE_STAGE_MARKERS = 9        /* SinglePhasePressureSolver::BuildMarkers only */
+};
+
+typedef struct fe_ctx fe_ctx;
+typedef struct fe_scene fe_scene;
+
+/* Context: one HIP device.  device = -1 picks the current device. */
+int fe_ctx_create(int device, fe_ctx** out);
+int fe_ctx_destroy(fe_ctx* ctx);
+const char* fe_last_error(const fe_ctx* ctx);
+/* Name of the device the context runs on ("gfx950..."), for logs. */
+int fe_ctx_device_name(const fe_ctx* ctx, char* buf, size_t len);
+
+/* Scene: device state of HybridSimulator (src/hybrid_simulator.cpp:4-25). */
+int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* desc, fe_scene** out);
+int fe_scene_destroy(fe_scene* scene);
+int fe_scene_sync(fe_scene* scene);
+
+/* Particles: ParticleSystem "postion"/"velocity" attributes
+ * (src/hybrid_simulator.hpp:50-51) and the APIC affine rows _cX/_cY/_cZ
+ * (src/apic_simulator.hpp:21-23; C is 9 floats per particle: cx, cy, cz rows,
+ * or NULL for zero).  Upload replaces all particles; download returns them in
+ * upload order. */
+int fe_particles_upload(fe_scene* scene, const float* x, const float* v, const float* C, int64_t count);
+int fe_particles_download(fe_scene* scene, float* x, float* v, float* C, int64_t count);
+int fe_particles_count(const fe_scene* scene, int64_t* count);
+
+/* One substep: PICSimulator::OnAdvanceTimeStep (src/pic_simulator.cpp:111-154). */
+int fe_substep(fe_scene* scene, double dt, const fe_solver_desc* solver, fe_solve_stats* stats);
+
+/* One frame: PhysicsAnimation::AdvanceTimeStep (src/physics_animation.cpp:64-92),
+ * CFL-adaptive substeps; substeps_out may be NULL. */
+int fe_advance_frame(fe_scene* scene, double frame_dt, const fe_solver_desc* solver, int32_t* substeps_out);
+
+/* PICSimulator::Cfl (src/pic_simulator.cpp:80-97). */
+int fe_cfl(fe_scene* scene, double dt, double* cfl_out);
+
+/* Statistics of the most recent pressure solve (SinglePhasePressureSolver
+ * keeps no statistics; this is the backend's report). */
+int fe_last_solve_stats(const fe_scene* scene, fe_solve_stats* stats);
+
+/* Run one stage of the substep on the current state (tests, profiling). */
+int fe_run_stage(fe_scene* scene, int32_t stage, double dt, const fe_solver_desc* solver, fe_solve_stats* stats);
+
+/* Grid state transfer; bytes must equal the field's size. */
+int fe_grid_download(fe_scene* scene, int32_t field, void* dst, size_t bytes);
+int fe_grid_upload(fe_scene* scene, int32_t field, const void* src, size_t bytes);
+int fe_grid_bytes(const fe_scene* scene, int32_t field, size_t* bytes);
+
+/* Cell index of each particle (i + nx*(j + ny*k)) as the sort bins them, in
+ * upload order.  Bit-exact with floor((p - origin)/h) in FP64 clamped to the grid. */
+int fe_particle_cells(fe_scene* scene, int32_t* cells, int64_t count);
+
+/* Analytic scene seeding (the reference's InitializeParticles,
+ * src/pic_simulator.cpp:521-548, draws std::rand(); this is the deterministic
+ * replacement): every cell whose centre lies inside the union of the boxes
+ * and spheres gets 2x2x2 particles, one per sub-cell at
+ * cell_lo + (s + U)*h/2 with U from splitmix64(seed, cell, sub-cell).
+ * Host only (no GPU).  Call with x == NULL to get the count. */
+typedef struct fe_seed_shape {
+    int32_t kind;       /* 0 = box [lo, hi), 1 = sphere (centre, radius in lo[0..2], hi[0]) */
+    int32_t reserved;
+    double lo[3];
+    double hi[3];
+} fe_seed_shape;
+int fe_seed_particles(const fe_scene_desc* desc, const fe_seed_shape* shapes, int32_t nshapes, uint64_t seed,
+                      float* x, int64_t capacity, int64_t* count);
+
+/* Kernel timing of the last fe_substep / fe_run_stage, in ms per stage
+ * (index = fe_stage), from HIP events on the scene's stream. */
+int fe_stage_times(const fe_scene* scene, double* ms, int32_t count);
+
+/* Per-kernel timer (no reference counterpart; for roofline measurement):
+ * HIP events on the scene's stream around every launch of one kernel.
+ * fe_kernel_timer(scene, FE_KERNEL_NONE) switches it off. */
+enum fe_kernel_id {
+    FE_KERNEL_NONE = 0,
+    FE_KERNEL_CG_SPMV = 1,    /* d = z + beta d, q = A d, d.q */
+    FE_KERNEL_CG_UPDATE = 2,  /* x += alpha d, r -= alpha q, z = M^-1 r, r.z */
+    FE_KERNEL_P2G = 3,        /* per-face gather (3 launches per substep, one per component) */
+    FE_KERNEL_G2P_MOVE = 4,   /* fused G2P + advection */
+    FE_KERNEL_SDF = 5,        /* fluid SDF */
+    FE_KERNEL_JACOBI = 6      /* one Jacobi sweep */
+};
+int fe_kernel_timer(fe_scene* scene, int32_t kernel_id);
+/* Total device ms and launch count since the last reset (reset != 0 clears). */
+int fe_kernel_timer_read(fe_scene* scene, double* total_ms, int64_t* launches, int32_t reset);
+
+/* Library version string. */
+const char* fe_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDE_HIP_H */

@@ -819,9 +819,20 @@ void cg(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
     Arr3 r, dd, q, z;
     r.resize(nx, ny, nz); dd.resize(nx, ny, nz); q.resize(nx, ny, nz); z.resize(nx, ny, nz);
     const bool precond = d->kind == ORC_SOLVER_PCG_JACOBI;
+    const int nt = s->nthreads;
+    // BLAS::AXpY and Preconditioner's ParallelFill are parallel loops in the
+    // reference (src/linear_system/blas.cpp:39-47, conjugate_gradient_solver.cpp:57-63)
+    auto pfor = [&](const std::function<void(size_t)>& f) {
+        par_for(nt, (int64_t)s->p.size(), [&](int64_t a, int64_t b) {
+            for (int64_t i = a; i < b; ++i) f((size_t)i);
+        });
+    };
     auto apply_m = [&](const Arr3& in, Arr3& out) {
-        if (!precond) { out = in; return; }
-        for (size_t i = 0; i < in.size(); ++i) out.d[i] = in.d[i] / s->Ac.d[i];
+        if (!precond) {
+            pfor([&](size_t i) { out.d[i] = in.d[i]; });
+            return;
+        }
+        pfor([&](size_t i) { out.d[i] = in.d[i] / s->Ac.d[i]; });
     };
     if (!d->warm_start) std::fill(s->p.d.begin(), s->p.d.end(), 0.0);
     residual(s, s->p, s->b, r, s->nthreads);
@@ -840,19 +851,19 @@ void cg(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
     while (!done(sigmaNew, r) && it < d->max_iterations) {
         spmv(s, dd, q, s->nthreads);
         double alpha = sigmaNew / dot(dd, q);
-        for (size_t i = 0; i < s->p.size(); ++i) s->p.d[i] = alpha * dd.d[i] + s->p.d[i];
+        pfor([&](size_t i) { s->p.d[i] = alpha * dd.d[i] + s->p.d[i]; });
         if (trigger || (it % 50 == 0 && it > 0)) {
             residual(s, s->p, s->b, r, s->nthreads);
             trigger = false;
         } else {
-            for (size_t i = 0; i < r.size(); ++i) r.d[i] = -alpha * q.d[i] + r.d[i];
+            pfor([&](size_t i) { r.d[i] = -alpha * q.d[i] + r.d[i]; });
         }
         apply_m(r, z);
         double sigmaOld = sigmaNew;
         sigmaNew = dot(r, z);
         if (sigmaNew > sigmaOld) trigger = true;
         double beta = sigmaNew / sigmaOld;
-        for (size_t i = 0; i < dd.size(); ++i) dd.d[i] = beta * dd.d[i] + z.d[i];
+        pfor([&](size_t i) { dd.d[i] = beta * dd.d[i] + z.d[i]; });
         ++it;
     }
     st->iterations = it;
