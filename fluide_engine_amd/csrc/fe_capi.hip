@@ -147,13 +147,12 @@ int pressure(fe_scene* s, const fe_solver_desc* d, bool walls_after, fe_solve_st
     HIPCHK(s->ctx, launch_apply_pressure(s, walls_after));
     hipEvent_t stop = s->ev[kMaxStages];
     HIPCHK(s->ctx, hipEventRecord(stop, s->stream));
-    HIPCHK(s->ctx, hipMemcpyAsync(s->sc_host, s->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+    double rr = 0.0, bb = 0.0;
+    HIPCHK(s->ctx, read_final_residual(s, &rr, &bb));
     fe_solve_stats& ls = s->last_stats;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, s->ev[ev_start], stop);
     ls.iterations = iters;
-    const double rr = s->sc_host->rr, bb = s->sc_host->bnorm2;
     ls.abs_residual = std::sqrt(rr);
     ls.rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
     ls.ms = ms;
@@ -291,7 +290,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(dalloc(&s->q, nc));
     s->npartials = pad_blocks((nc + kBlock - 1) / kBlock) * 4;
     chk(dalloc(&s->partials, s->npartials));
-    chk(dalloc(&s->sc, 1));
+    chk(dalloc(&s->sc, 2));  // two state slots (CG alternates them)
     chk(hipHostMalloc((void**)&s->sc_host, sizeof(SolverScalars), hipHostMallocDefault));
     chk(dalloc(&s->cflmax, 1));
     const int64_t cap = std::max<int64_t>(d->max_particles, 1);
@@ -327,7 +326,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(launch_fill_f32(s->stream, s->d[1], 0.0f, nc));
     chk(launch_fill_u8(s->stream, s->flags, FE_AIR, nc));
     chk(hipMemsetAsync(s->sdf, 0, nc * sizeof(double), s->stream));
-    chk(hipMemsetAsync(s->sc, 0, sizeof(SolverScalars), s->stream));
+    chk(hipMemsetAsync(s->sc, 0, 2 * sizeof(SolverScalars), s->stream));
     chk(hipStreamSynchronize(s->stream));
     if (e != hipSuccess) {
         int rc = hip_fail(ctx, e, "scene initialisation");
@@ -572,6 +571,7 @@ int fe_kernel_timer(fe_scene* s, int32_t kernel_id) {
     kt_flush(s);
     if (kernel_id != FE_KERNEL_NONE && s->kt_ev.empty()) {
         s->kt_ev.resize(2 * kTimerPairs, nullptr);
+        s->kt_tag.assign(kTimerPairs, 0);
         for (auto& e : s->kt_ev) HIPCHK(ctx, hipEventCreate(&e));
     }
     s->kt_id = kernel_id;

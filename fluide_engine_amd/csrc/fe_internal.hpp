@@ -153,8 +153,10 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t padded) {
 inline int64_t pad_blocks(int64_t nblocks) { return (nblocks + kXcds - 1) / kXcds * kXcds; }
 
 // ---------------------------------------------------------------------------
-// Device-resident solver scalars (FP64): updated by the last block of each
-// reduction so the CG loop never waits on the host.
+// Device-resident solver scalars (FP64).  Reductions leave per-block partials;
+// the next kernel's blocks each sum them in the same fixed order (so every
+// block derives identical scalars) and block 0 records the state here, so the
+// CG loop never waits on the host and uses no atomics.
 // ---------------------------------------------------------------------------
 struct SolverScalars {
     double sigma;      // r . z
@@ -165,13 +167,13 @@ struct SolverScalars {
     double beta;
     double bnorm2;     // b . b
     double tol;
-    int32_t it;        // iterations done
-    int32_t done;      // stopping rule met
-    int32_t trigger;   // recompute the residual next iteration (sigma grew)
+    int32_t it;        // iterations done (-1 before the first spmv)
+    int32_t done;      // stopping rule met or iteration cap reached
+    int32_t trigger;   // sigma grew: recompute the residual next iteration
     int32_t relative;
     int32_t precond;
     int32_t max_it;
-    uint32_t counter;  // last-block-done ticket
+    int32_t replace;   // the next (or last) update recomputes r = b - A x
     int32_t converged; // the stopping rule (not the iteration cap) ended the solve
 };
 
@@ -242,6 +244,7 @@ struct fe_scene {
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;
     std::vector<hipEvent_t> kt_ev;  // pairs
+    std::vector<int> kt_tag;        // one tag per pair
     int kt_used = 0;
     double kt_ms = 0.0;
     int64_t kt_launches = 0;
@@ -251,9 +254,11 @@ namespace fe {
 
 constexpr int kTimerPairs = 1024;
 
-// Accumulate the finished pairs (call after the stream is synchronized).
-inline void kt_flush(fe_scene* s) {
+// Accumulate the finished pairs whose tag is below `limit` (call after the
+// stream is synchronized); tags let a caller drop launches that did no work.
+inline void kt_flush(fe_scene* s, int limit = 1 << 30) {
     for (int k = 0; k < s->kt_used; ++k) {
+        if (s->kt_tag[k] >= limit) continue;
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, s->kt_ev[2 * k], s->kt_ev[2 * k + 1]) == hipSuccess) {
             s->kt_ms += ms;
@@ -263,12 +268,13 @@ inline void kt_flush(fe_scene* s) {
     s->kt_used = 0;
 }
 
-inline void kt_begin(fe_scene* s, int32_t id) {
+inline void kt_begin(fe_scene* s, int32_t id, int tag = 0) {
     if (s->kt_id != id || s->kt_ev.empty()) return;
     if (s->kt_used == kTimerPairs) {
         (void)hipStreamSynchronize(s->stream);
         kt_flush(s);
     }
+    s->kt_tag[s->kt_used] = tag;
     (void)hipEventRecord(s->kt_ev[2 * s->kt_used], s->stream);
 }
 

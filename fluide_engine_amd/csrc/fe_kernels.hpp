@@ -34,6 +34,7 @@ hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n);
 
 // k_solver.hip
 hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d);  // rhs + solve (no apply)
-hipError_t launch_final_residual(fe_scene* s);  // FP64 ||b - A p||^2 and ||b||^2 into s->sc
+hipError_t launch_final_residual(fe_scene* s);  // FP64 ||b - A p||^2 and ||b||^2 (device)
+hipError_t read_final_residual(fe_scene* s, double* rr, double* bb);  // waits for them
 
 }  // namespace fe

@@ -3,6 +3,8 @@
 // conditions, gravity, pressure-gradient application and the CFL reduction.
 // All are HBM-bound element or 7-point stencil sweeps; launches remap blocks
 // so neighbouring tiles share an XCD's L2.
+#include <algorithm>
+
 #include "fe_kernels.hpp"
 
 namespace fe {
@@ -386,9 +388,9 @@ hipError_t launch_apply_pressure(fe_scene* s, bool walls_after) {
 // an atomic max on the (non-negative) FP64 bit pattern is deterministic.
 __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, unsigned long long* __restrict__ out) {
     __shared__ double smem[kBlock / 64];
-    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double m = 0.0;
-    if (c < cell_count(g)) {
+    const int64_t nc = cell_count(g);
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
         const int i = (int)(c % g.n[0]);
         const int j = (int)((c / g.n[0]) % g.n[1]);
         const int k = (int)(c / ((int64_t)g.n[0] * g.n[1]));
@@ -417,7 +419,9 @@ hipError_t launch_cfl(fe_scene* s, double dt) {
     hipError_t e = hipMemsetAsync(s->cflmax, 0, sizeof(unsigned long long), s->stream);
     if (e != hipSuccess) return e;
     FaceSet a{{s->u[0], s->u[1], s->u[2]}};
-    hipLaunchKernelGGL(k_cfl, grid1(nc), dim3(kBlock), 0, s->stream, s->g, a, dt, s->cflmax);
+    // few blocks with a grid-stride loop: one atomic per block on a single address
+    const unsigned blocks = (unsigned)std::min<int64_t>((nc + kBlock - 1) / kBlock, 512);
+    hipLaunchKernelGGL(k_cfl, dim3(blocks), dim3(kBlock), 0, s->stream, s->g, a, dt, s->cflmax);
     return hipGetLastError();
 }
 

@@ -6,16 +6,22 @@
 // stored: it is rebuilt inside every stencil from the 1-byte cell markers with
 // BuildSystem's rules (:99-164) -- center = sum of 1/h^2 over in-range,
 // non-BOUNDRY neighbours; off-diagonal -1/h^2 between two FLUID cells;
-// non-FLUID rows are the identity.  Vectors are FP32; every dot product and
-// norm accumulates in FP64 with a deterministic two-level reduction whose last
-// block also updates the solver scalars on the device, so the CG loop runs
-// without host round trips.
+// non-FLUID rows are the identity.  r, z, d, q are FP32, the solution is FP64,
+// and every dot product and norm accumulates in FP64.
+//
+// Reductions: each kernel runs at most 1024 blocks over contiguous cell chunks
+// and leaves one FP64 partial per block and value.  The next kernel's blocks
+// each sum those partials in the same fixed order, so all blocks derive
+// bit-identical scalars with no atomics and no inter-block fences (the kernel
+// boundary orders the writes); block 0 records the solver state.
 #include <algorithm>
 #include <utility>
 
 #include "fe_kernels.hpp"
 
 namespace fe {
+
+constexpr int kMaxSolverBlocks = 1024;
 
 // ---------------------------------------------------------------------------
 // Cell coordinates and the matrix-free 7-point row.
@@ -81,15 +87,36 @@ __device__ inline float row_apply(const Row& r, float vc, F vnb) {
     return acc;
 }
 
+// FP64 product of the same row (residuals of the FP64 solution).  The FP32
+// center is a sum of identical powers of 1/h^2 per axis, exact for
+// power-of-two spacings.
+template <typename F>
+__device__ inline double row_apply64(const Row& r, double xc, F xnb, const double ih2[3]) {
+    if (!r.fluid) return xc;
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (r.off[q] != 0.0f) acc -= ih2[q >> 1] * xnb(r.nb[q]);
+    return (double)r.center * xc + acc;
+}
+
 // ---------------------------------------------------------------------------
-// Deterministic FP64 reduction of NV values per thread.  Every block writes its
-// partials; the last block to arrive (ticket counter) sums them in a fixed
-// order and runs `fin` on the totals.
+// Launch shape and block-level FP64 reductions.
 // ---------------------------------------------------------------------------
-template <int NV, typename Fin>
-__device__ inline void reduce_finish(double (&v)[NV], double* partials, uint32_t* counter, Fin fin) {
-    __shared__ double sm[NV][kBlock / 64];
-    __shared__ int is_last;
+struct Span {
+    int64_t c0, c1;
+};
+
+__device__ inline Span block_span(int64_t nc, int64_t chunk, int64_t padded) {
+    const int64_t lb = xcd_remap(blockIdx.x, padded);
+    Span s;
+    s.c0 = lb * chunk;
+    s.c1 = min(s.c0 + chunk, nc);
+    return s;
+}
+
+template <int NV>
+__device__ inline void block_reduce(double (&v)[NV], double (*sm)[kBlock / 64]) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
@@ -99,157 +126,147 @@ __device__ inline void reduce_finish(double (&v)[NV], double* partials, uint32_t
         if (lane == 0) sm[q][wid] = x;
     }
     __syncthreads();
-    const int64_t nblocks = gridDim.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < kBlock / 64; ++w) t += sm[q][w];
-            partials[(int64_t)blockIdx.x * NV + q] = t;
-        }
-        __threadfence();
-        const uint32_t ticket = atomicAdd(counter, 1u);
-        is_last = (ticket == (uint32_t)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (int64_t b = threadIdx.x; b < nblocks; b += kBlock) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q)
-            acc[q] += __hip_atomic_load(&partials[b * NV + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        double x = acc[q];
+        double t = 0.0;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-        if (lane == 0) sm[q][wid] = x;
+        for (int w = 0; w < kBlock / 64; ++w) t += sm[q][w];
+        v[q] = t;  // every thread holds the block total
     }
     __syncthreads();
+}
+
+// One FP64 partial per block and value.
+template <int NV>
+__device__ inline void write_partials(double (&v)[NV], double* __restrict__ part) {
+    __shared__ double sm[NV][kBlock / 64];
+    block_reduce<NV>(v, sm);
     if (threadIdx.x == 0) {
-        double tot[NV];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double t = 0.0;
-#pragma unroll
-            for (int w = 0; w < kBlock / 64; ++w) t += sm[q][w];
-            tot[q] = t;
-        }
-        fin(tot);
-        *counter = 0;
+        for (int q = 0; q < NV; ++q) part[(int64_t)blockIdx.x * NV + q] = v[q];
     }
 }
 
-__device__ inline bool stop_rule(const SolverScalars* sc, double sigma, double rr) {
-    if (sc->relative) return sqrt(rr) <= sc->tol * sqrt(sc->bnorm2);
+// Sum of the previous kernel's partials, identical in every block.
+template <int NV>
+__device__ inline void sum_partials(const double* __restrict__ part, int nparts, double (&tot)[NV]) {
+    __shared__ double sm[NV][kBlock / 64];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] += part[(int64_t)b * NV + q];
+    }
+    block_reduce<NV>(tot, sm);
+}
+
+__device__ inline bool stop_rule(const SolverScalars* sc, double bnorm2, double sigma, double rr) {
+    if (sc->relative) return sqrt(rr) <= sc->tol * sqrt(bnorm2);
     return !(sigma > sc->tol * sc->tol);
 }
 
 // ---------------------------------------------------------------------------
 // Right-hand side: DivergenceAtCallCenter (face_centered_grid3d.cpp:157-166)
-// on FLUID cells, 0 elsewhere; also ||b||^2.
+// on FLUID cells, 0 elsewhere; partials of b.b.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_rhs(Geom g, FaceSet u, const uint8_t* __restrict__ flags,
-                                                float* __restrict__ b, double* partials, SolverScalars* sc,
+                                                float* __restrict__ b, double* __restrict__ pout, int64_t chunk,
                                                 int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
     double v[1] = {0.0};
-    if (c < cell_count(g)) {
-        const Cell x = cell_of(g, c);
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         float bb = 0.0f;
         if (flags[c] == FE_FLUID) {
+            const Cell x = cell_of(g, c);
             const int64_t iu = (int64_t)x.i + (int64_t)(g.n[0] + 1) * ((int64_t)x.j + (int64_t)g.n[1] * x.k);
             const int64_t iv = (int64_t)x.i + (int64_t)g.n[0] * ((int64_t)x.j + (int64_t)(g.n[1] + 1) * x.k);
-            const int64_t iw = c;
             const float left = u.a[0][iu], right = u.a[0][iu + 1];
             const float down = u.a[1][iv], up = u.a[1][iv + g.n[0]];
-            const float back = u.a[2][iw], front = u.a[2][iw + (int64_t)g.n[0] * g.n[1]];
+            const float back = u.a[2][c], front = u.a[2][c + (int64_t)g.n[0] * g.n[1]];
             bb = (right - left) / g.h[0] + (up - down) / g.h[1] + (front - back) / g.h[2];
         }
         b[c] = bb;
-        v[0] = (double)bb * (double)bb;
+        v[0] += (double)bb * (double)bb;
     }
-    reduce_finish<1>(v, partials, &sc->counter, [&](const double* t) { sc->bnorm2 = t[0]; });
-}
-
-
-// FP64 row product with the same matrix (for residuals of the FP64 solution).
-template <typename F>
-__device__ inline double row_apply64(const Row& r, double xc, F xnb, const double ih2[3]) {
-    if (!r.fluid) return xc;
-    double center = 0.0, acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        if (r.off[q] != 0.0f) acc -= ih2[q >> 1] * xnb(r.nb[q]);
-    }
-    // center: recount in FP64 from the FP32 row (exact: sums of identical powers of 1/h^2 per axis)
-    center = (double)r.center;
-    return center * xc + acc;
+    write_partials<1>(v, pout);
 }
 
 // ---------------------------------------------------------------------------
 // Jacobi sweep (JacobiIterationSolver::Relax, src/linear_system/jacobi_iteration_solver.cpp:39-76):
 // x' = (b - sum_offdiag A x) / center, on FP32 working copies of the FP64
-// pressure.  Once the convergence check has fired the sweep only copies, so
-// the host's buffer rotation stays consistent.
+// pressure.  When a residual check ran before this sweep its partials are
+// summed first (CalculateTolerance, :78-82: ||b - A x||_2 < tol stops the
+// solve); after the stop every sweep only copies, so the buffer rotation stays
+// consistent.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_jacobi(Geom g, const uint8_t* __restrict__ flags,
                                                    const float* __restrict__ b, const float* __restrict__ xo,
-                                                   float* __restrict__ xn, const SolverScalars* sc, float ihx2,
-                                                   float ihy2, float ihz2, int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
-    if (c >= cell_count(g)) return;
-    if (sc->done) {
-        xn[c] = xo[c];
-        return;
+                                                   float* __restrict__ xn, SolverScalars* sc,
+                                                   const double* __restrict__ pcheck, int nparts, int check_it,
+                                                   float ihx2, float ihy2, float ihz2, int64_t chunk,
+                                                   int64_t padded) {
+    // block 0 may set sc->done during this kernel: read it once per block so
+    // every thread takes the same path through the barriers below
+    __shared__ int sdone;
+    if (threadIdx.x == 0) sdone = sc->done;
+    __syncthreads();
+    bool done = sdone != 0;
+    if (pcheck && !done) {
+        double t[1];
+        sum_partials<1>(pcheck, nparts, t);
+        if (sqrt(t[0]) < sc->tol) {
+            done = true;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                sc->done = 1;
+                sc->converged = 1;
+                sc->it = check_it + 1;
+            }
+        }
     }
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
-    const Row r = make_row(g, flags, cell_of(g, c), ih2);
-    float off = 0.0f;
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+        if (done) {
+            xn[c] = xo[c];
+            continue;
+        }
+        const Row r = make_row(g, flags, cell_of(g, c), ih2);
+        float off = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-        if (r.off[q] != 0.0f) off += r.off[q] * xo[r.nb[q]];
-    xn[c] = (b[c] - off) / r.center;
+        for (int q = 0; q < 6; ++q)
+            if (r.off[q] != 0.0f) off += r.off[q] * xo[r.nb[q]];
+        xn[c] = (b[c] - off) / r.center;
+    }
 }
 
-// ||b - A x||_2 < tol check (JacobiIterationSolver::CalculateTolerance, :78-82), FP64 sums.
-__global__ __launch_bounds__(kBlock) void k_jacobi_check(Geom g, const uint8_t* __restrict__ flags,
-                                                         const float* __restrict__ b, const float* __restrict__ x,
-                                                         double* partials, SolverScalars* sc, float ihx2,
-                                                         float ihy2, float ihz2, int it, int64_t padded) {
-    if (sc->done) return;  // uniform across the grid
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
+// Partials of ||b - A x||^2 for the Jacobi convergence check (FP64 sums).
+__global__ __launch_bounds__(kBlock) void k_jacobi_residual(Geom g, const uint8_t* __restrict__ flags,
+                                                            const float* __restrict__ b,
+                                                            const float* __restrict__ x, const SolverScalars* sc,
+                                                            double* __restrict__ pout, float ihx2, float ihy2,
+                                                            float ihz2, int64_t chunk, int64_t padded) {
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    const float ih2[3] = {ihx2, ihy2, ihz2};
+    const double ih2d[3] = {ihx2, ihy2, ihz2};
     double v[1] = {0.0};
-    if (c < cell_count(g)) {
-        const float ih2[3] = {ihx2, ihy2, ihz2};
-        const Row r = make_row(g, flags, cell_of(g, c), ih2);
-        const double ih2d[3] = {ihx2, ihy2, ihz2};
-        const double ax = row_apply64(r, (double)x[c], [&](int64_t n) { return (double)x[n]; }, ih2d);
-        const double res = (double)b[c] - ax;
-        v[0] = res * res;
-    }
-    reduce_finish<1>(v, partials, &sc->counter, [&](const double* t) {
-        if (sqrt(t[0]) < sc->tol) {
-            sc->done = 1;
-            sc->converged = 1;
-            sc->it = it + 1;
+    if (!sc->done) {
+        for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+            const Row r = make_row(g, flags, cell_of(g, c), ih2);
+            const double ax = row_apply64(r, (double)x[c], [&](int64_t n) { return (double)x[n]; }, ih2d);
+            const double res = (double)b[c] - ax;
+            v[0] += res * res;
         }
-    });
+    }
+    write_partials<1>(v, pout);
 }
 
 __global__ __launch_bounds__(kBlock) void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ o,
-                                                       int64_t n, int zero) {
+                                                       int64_t n) {
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) o[c] = zero ? 0.0f : (float)a[c];
+    if (c < n) o[c] = (float)a[c];
 }
 
 __global__ __launch_bounds__(kBlock) void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o,
@@ -264,172 +281,217 @@ __global__ __launch_bounds__(kBlock) void k_zero_f64(double* __restrict__ x, int
 }
 
 // ---------------------------------------------------------------------------
-// Conjugate gradient (ConjugateGradientSolver::Solve, src/linear_system/conjugate_gradient_solver.cpp:14-55)
-// in two fused kernels per iteration, FP32 r, z, d, q and an FP64 solution x:
-//   spmv:   d = z + beta*d_old (on the fly, also at the 6 neighbours),
-//           q = A d, alpha = sigma / (d.q)
-//   update: x' = alpha*d + x (FP64); r = -alpha*q + r, or r = b - A x' in FP64
-//           on residual replacement (every 50 iterations or after sigma grew,
-//           :32-49; x' at the neighbours is formed on the fly from x and d),
-//           z = M^-1 r, sigma' = r.z, beta = sigma'/sigma, stopping rule.
+// Conjugate gradient (ConjugateGradientSolver::Solve, src/linear_system/conjugate_gradient_solver.cpp:14-55),
+// two kernels per iteration:
+//   spmv:   [prologue: sigma' = r.z, r.r of the last update -> beta, trigger,
+//           stopping rule, next replacement flag]  d = z + beta*d_old (also at
+//           the 6 neighbours, on the fly), q = A d, partials of d.q
+//   update: [prologue: alpha = sigma / d.q]  x' = alpha*d + x (FP64);
+//           r = -alpha*q + r, or r = b - A x' in FP64 on residual replacement
+//           (every 50 iterations or after sigma grew, :32-49; x' at the
+//           neighbours is formed on the fly from x and d); z = M^-1 r;
+//           partials of r.z and r.r
 // With the relative rule a stop is only accepted on a freshly replaced
 // residual, so the decision is taken on b - A x, not on the recurrence.
-// x and d ping-pong between two buffers; z is r itself without a preconditioner.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint8_t* __restrict__ flags,
                                                     const float* __restrict__ b, const double* __restrict__ x,
                                                     float* __restrict__ r, float* __restrict__ z,
-                                                    float* __restrict__ d0, double* partials, SolverScalars* sc,
-                                                    float ihx2, float ihy2, float ihz2, int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
+                                                    float* __restrict__ d0, SolverScalars* sc,
+                                                    const double* __restrict__ pbb, double* __restrict__ pout,
+                                                    int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
+                                                    int64_t padded) {
+    double bb[1];
+    sum_partials<1>(pbb, nparts, bb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->bnorm2 = bb[0];
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    const float ih2[3] = {ihx2, ihy2, ihz2};
+    const double ih2d[3] = {ihx2, ihy2, ihz2};
+    const bool pre = sc->precond != 0;
     double v[2] = {0.0, 0.0};
-    if (c < cell_count(g)) {
-        const float ih2[3] = {ihx2, ihy2, ihz2};
-        const double ih2d[3] = {ihx2, ihy2, ihz2};
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const Row row = make_row(g, flags, cell_of(g, c), ih2);
         const double ax = row_apply64(row, x[c], [&](int64_t n) { return x[n]; }, ih2d);
         const float rc = (float)((double)b[c] - ax);
         r[c] = rc;
         float zc = rc;
-        if (sc->precond) {
+        if (pre) {
             zc = rc / row.center;
             z[c] = zc;
         }
         d0[c] = 0.0f;
-        v[0] = (double)rc * (double)zc;
-        v[1] = (double)rc * (double)rc;
+        v[0] += (double)rc * (double)zc;
+        v[1] += (double)rc * (double)rc;
     }
-    reduce_finish<2>(v, partials, &sc->counter, [&](const double* t) {
-        sc->sigma = t[0];
-        sc->rr = t[1];
-        sc->beta = 0.0;
-        sc->it = 0;
-        sc->trigger = 0;
-        const bool stop = stop_rule(sc, t[0], t[1]);
-        sc->converged = stop ? 1 : 0;
-        sc->done = (stop || sc->max_it <= 0) ? 1 : 0;
-    });
+    write_partials<2>(v, pout);
 }
 
+// State slots alternate with the iteration (spmv k reads slot k&1 and writes
+// slot (k+1)&1), so no block can see a half-updated state.
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint8_t* __restrict__ flags,
                                                     const float* __restrict__ z, const float* __restrict__ dold,
                                                     float* __restrict__ dnew, float* __restrict__ q,
-                                                    double* partials, SolverScalars* sc, float ihx2, float ihy2,
-                                                    float ihz2, int64_t padded) {
-    if (sc->done) return;  // uniform
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
+                                                    const SolverScalars* __restrict__ in,
+                                                    SolverScalars* __restrict__ out, const double* __restrict__ pin,
+                                                    double* __restrict__ pout, int nparts, float ihx2, float ihy2,
+                                                    float ihz2, int64_t chunk, int64_t padded) {
+    if (in->done) {  // uniform: an earlier spmv stopped the solve
+        if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
+        return;
+    }
+    // ---- state update from the last update's r.z and r.r (every block alike)
+    double t[2];
+    sum_partials<2>(pin, nparts, t);
+    const int it_prev = in->it;
+    double beta;
+    int it, trig, done, conv, repl;
+    if (it_prev < 0) {  // first iteration: partials come from k_cg_init
+        it = 0;
+        beta = 0.0;
+        trig = 0;
+        conv = stop_rule(in, in->bnorm2, t[0], t[1]) ? 1 : 0;
+        done = (conv || in->max_it <= 0) ? 1 : 0;
+    } else {
+        const double sold = in->sigma;
+        it = it_prev + 1;
+        trig = t[0] > sold ? 1 : 0;
+        beta = t[0] / sold;
+        bool stop = stop_rule(in, in->bnorm2, t[0], t[1]);
+        if (stop && in->relative && !in->replace) {  // confirm on the true residual
+            stop = false;
+            trig = 1;
+        }
+        conv = stop ? 1 : 0;
+        done = (stop || it >= in->max_it) ? 1 : 0;
+    }
+    repl = (trig || (it % 50 == 0 && it > 0)) ? 1 : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SolverScalars o = *in;
+        o.sigma_old = in->sigma;
+        o.sigma = t[0];
+        o.rr = t[1];
+        o.beta = beta;
+        o.it = it;
+        o.trigger = trig;
+        o.converged = conv;
+        o.done = done;
+        o.replace = repl;
+        *out = o;
+    }
+    if (done) return;
+    // ---- body
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    const float ih2[3] = {ihx2, ihy2, ihz2};
+    const float fb = (float)beta;
+    auto D = [&](int64_t n) { return fb * dold[n] + z[n]; };  // AXpY(beta, d, s, &d)
     double v[1] = {0.0};
-    if (c < cell_count(g)) {
-        const float beta = (float)sc->beta;
-        const float ih2[3] = {ihx2, ihy2, ihz2};
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const Row row = make_row(g, flags, cell_of(g, c), ih2);
-        auto D = [&](int64_t n) { return beta * dold[n] + z[n]; };  // AXpY(beta, d, s, &d)
         const float dc = D(c);
         const float qc = row_apply(row, dc, D);
         dnew[c] = dc;
         q[c] = qc;
-        v[0] = (double)dc * (double)qc;
+        v[0] += (double)dc * (double)qc;
     }
-    reduce_finish<1>(v, partials, &sc->counter, [&](const double* t) {
-        sc->dq = t[0];
-        sc->alpha = sc->sigma / t[0];
-    });
+    write_partials<1>(v, pout);
 }
 
 __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint8_t* __restrict__ flags,
                                                       const float* __restrict__ b, const double* __restrict__ xo,
                                                       double* __restrict__ xn, const float* __restrict__ d,
                                                       const float* __restrict__ q, float* __restrict__ r,
-                                                      float* __restrict__ z, double* partials, SolverScalars* sc,
-                                                      float ihx2, float ihy2, float ihz2, int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
-    if (sc->done) {  // uniform: keep the rotation consistent
-        if (c < cell_count(g)) xn[c] = xo[c];
+                                                      float* __restrict__ z, SolverScalars* sc,
+                                                      const double* __restrict__ pin, double* __restrict__ pout,
+                                                      int nparts, float ihx2, float ihy2, float ihz2,
+                                                      int64_t chunk, int64_t padded) {
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    if (sc->done) {  // uniform: keep the x rotation consistent
+        for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) xn[c] = xo[c];
         return;
     }
-    const int it = sc->it;
-    const bool replace = sc->trigger || (it % 50 == 0 && it > 0);
+    double t[1];
+    sum_partials<1>(pin, nparts, t);
+    const double alpha = sc->sigma / t[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->dq = t[0];
+        sc->alpha = alpha;
+    }
+    const bool replace = sc->replace != 0;
+    const bool pre = sc->precond != 0;
+    const float ih2[3] = {ihx2, ihy2, ihz2};
+    const double ih2d[3] = {ihx2, ihy2, ihz2};
+    const float fa = (float)alpha;
+    auto X = [&](int64_t n) { return alpha * (double)d[n] + xo[n]; };
     double v[2] = {0.0, 0.0};
-    if (c < cell_count(g)) {
-        const double alpha = sc->alpha;
-        const float ih2[3] = {ihx2, ihy2, ihz2};
-        const double ih2d[3] = {ihx2, ihy2, ihz2};
-        const Row row = make_row(g, flags, cell_of(g, c), ih2);
-        auto X = [&](int64_t n) { return alpha * (double)d[n] + xo[n]; };
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const double xc = X(c);
         xn[c] = xc;
         float rc;
-        if (replace)
-            rc = (float)((double)b[c] - row_apply64(row, xc, X, ih2d));
-        else
-            rc = -(float)alpha * q[c] + r[c];
+        float center = 1.0f;
+        if (replace || pre) {
+            const Row row = make_row(g, flags, cell_of(g, c), ih2);
+            center = row.center;
+            rc = replace ? (float)((double)b[c] - row_apply64(row, xc, X, ih2d)) : -fa * q[c] + r[c];
+        } else {
+            rc = -fa * q[c] + r[c];
+        }
         r[c] = rc;
         float zc = rc;
-        if (sc->precond) {
-            zc = rc / row.center;
+        if (pre) {
+            zc = rc / center;
             z[c] = zc;
         }
-        v[0] = (double)rc * (double)zc;
-        v[1] = (double)rc * (double)rc;
+        v[0] += (double)rc * (double)zc;
+        v[1] += (double)rc * (double)rc;
     }
-    reduce_finish<2>(v, partials, &sc->counter, [&](const double* t) {
-        const double sold = sc->sigma;
-        sc->sigma_old = sold;
-        sc->sigma = t[0];
-        sc->rr = t[1];
-        sc->trigger = t[0] > sold ? 1 : 0;
-        sc->beta = t[0] / sold;
-        sc->it = it + 1;
-        bool stop = stop_rule(sc, t[0], t[1]);
-        if (stop && sc->relative && !replace) {  // confirm on the true residual next iteration
-            stop = false;
-            sc->trigger = 1;
-        }
-        sc->converged = stop ? 1 : 0;
-        sc->done = (stop || sc->it >= sc->max_it) ? 1 : 0;
-    });
+    write_partials<2>(v, pout);
 }
 
-// Final ||b - A p||_2 and ||b||_2 in FP64 on the FP64 solution.
+// Final ||b - A p||^2 and ||b||^2 in FP64 on the FP64 solution, with exact
+// FP64 coefficients.
 __global__ __launch_bounds__(kBlock) void k_final_residual(Geom g, const uint8_t* __restrict__ flags,
                                                            const float* __restrict__ b, const double* __restrict__ x,
-                                                           double* partials, SolverScalars* sc, double ihx2,
-                                                           double ihy2, double ihz2, int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
+                                                           double* __restrict__ pout, double ihx2, double ihy2,
+                                                           double ihz2, int64_t chunk, int64_t padded) {
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    const double ih2[3] = {ihx2, ihy2, ihz2};
     double v[2] = {0.0, 0.0};
-    if (c < cell_count(g)) {
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const Cell cc = cell_of(g, c);
-        const float ones[3] = {1.0f, 1.0f, 1.0f};
-        const Row row = make_row(g, flags, cc, ones);  // structure only; weights in FP64 below
-        double ax;
-        if (row.fluid) {
-            const double ih2[3] = {ihx2, ihy2, ihz2};
+        double ax = x[c];
+        if (flags[c] == FE_FLUID) {
+            const int64_t sy = g.n[0], sz = (int64_t)g.n[0] * g.n[1];
+            const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
             const bool in[6] = {cc.i > 0, cc.i + 1 < g.n[0], cc.j > 0, cc.j + 1 < g.n[1], cc.k > 0, cc.k + 1 < g.n[2]};
             double center = 0.0, off = 0.0;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 if (!in[q]) continue;
-                const uint8_t fnb = flags[row.nb[q]];
+                const uint8_t fnb = flags[nb[q]];
                 if (fnb == FE_BOUNDRY) continue;
                 center += ih2[q >> 1];
-                if (fnb == FE_FLUID) off -= ih2[q >> 1] * x[row.nb[q]];
+                if (fnb == FE_FLUID) off -= ih2[q >> 1] * x[nb[q]];
             }
             ax = center * x[c] + off;
-        } else {
-            ax = x[c];
         }
         const double res = (double)b[c] - ax;
-        v[0] = res * res;
-        v[1] = (double)b[c] * (double)b[c];
+        v[0] += res * res;
+        v[1] += (double)b[c] * (double)b[c];
     }
-    reduce_finish<2>(v, partials, &sc->counter, [&](const double* t) {
-        sc->rr = t[0];
-        sc->bnorm2 = t[1];
-    });
+    write_partials<2>(v, pout);
+}
+
+// One block: totals of NV-wide partials into out[0..NV).
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_finish(const double* __restrict__ part, int nparts, double* out) {
+    double t[NV];
+    sum_partials<NV>(part, nparts, t);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NV; ++q) out[q] = t[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -437,30 +499,59 @@ __global__ __launch_bounds__(kBlock) void k_final_residual(Geom g, const uint8_t
 // ---------------------------------------------------------------------------
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
+struct SolverShape {
+    int64_t blocks;  // multiple of 8 (XCD remap)
+    int64_t chunk;   // cells per block, multiple of kBlock
+};
+
+static SolverShape solver_shape(const Geom& g) {
+    const int64_t nc = cell_count(g);
+    int64_t blocks = std::min<int64_t>(pad_blocks((nc + kBlock - 1) / kBlock), kMaxSolverBlocks);
+    int64_t chunk = (nc + blocks - 1) / blocks;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    blocks = pad_blocks((nc + chunk - 1) / chunk);
+    return {blocks, chunk};
+}
+
 hipError_t launch_final_residual(fe_scene* s) {
     const Geom& g = s->g;
-    const int64_t nc = cell_count(g);
-    const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-    const double ihd[3] = {1.0 / g.hd[0], 1.0 / g.hd[1], 1.0 / g.hd[2]};
-    hipLaunchKernelGGL(k_final_residual, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->flags, s->b, s->p,
-                       s->partials, s->sc, ihd[0] * ihd[0], ihd[1] * ihd[1], ihd[2] * ihd[2], padded);
+    const SolverShape sh = solver_shape(g);
+    double* part = s->partials;
+    hipLaunchKernelGGL(k_final_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b,
+                       s->p, part, g.ihd[0] * g.ihd[0], g.ihd[1] * g.ihd[1], g.ihd[2] * g.ihd[2], sh.chunk, sh.blocks);
+    // totals (||r||^2, ||b||^2) land after the partials; read_final_residual fetches them
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kBlock), 0, s->stream, part, (int)sh.blocks,
+                       part + 2 * sh.blocks);
     return hipGetLastError();
 }
 
-static hipError_t read_scalars(fe_scene* s) {
-    hipError_t e = hipMemcpyAsync(s->sc_host, s->sc, sizeof(SolverScalars), hipMemcpyDeviceToHost, s->stream);
+static hipError_t read_scalars(fe_scene* s, int slot) {
+    hipError_t e = hipMemcpyAsync(s->sc_host, s->sc + slot, sizeof(SolverScalars), hipMemcpyDeviceToHost, s->stream);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(s->stream);
+}
+
+hipError_t read_final_residual(fe_scene* s, double* rr, double* bb) {
+    const SolverShape sh = solver_shape(s->g);
+    double tot[2] = {0.0, 0.0};
+    hipError_t e = hipMemcpyAsync(tot, s->partials + 2 * sh.blocks, sizeof(tot), hipMemcpyDeviceToHost, s->stream);
+    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return e;
+    *rr = tot[0];
+    *bb = tot[1];
+    return hipSuccess;
 }
 
 hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     const Geom& g = s->g;
     const int64_t nc = cell_count(g);
-    const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
+    const SolverShape sh = solver_shape(g);
+    const int nparts = (int)sh.blocks;
+    double* PA = s->partials;                  // 2 per block
+    double* PB = s->partials + 2 * sh.blocks;  // 2 per block
     // BuildSystem's invH * invH (single_phase_pressure_solver.cpp:102-103); exact
     // in FP32 for power-of-two spacings
-    const double ihd[3] = {1.0 / g.hd[0], 1.0 / g.hd[1], 1.0 / g.hd[2]};
-    const float ih2[3] = {(float)(ihd[0] * ihd[0]), (float)(ihd[1] * ihd[1]), (float)(ihd[2] * ihd[2])};
+    const float ih2[3] = {(float)(g.ihd[0] * g.ihd[0]), (float)(g.ihd[1] * g.ihd[1]), (float)(g.ihd[2] * g.ihd[2])};
     hipError_t e;
 
     SolverScalars init{};
@@ -468,67 +559,95 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     init.relative = d->relative;
     init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : 0;
     init.max_it = d->max_iterations;
+    init.it = -1;
     if ((e = hipMemcpyAsync(s->sc, &init, sizeof(SolverScalars), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
         return e;
     FaceSet u{{s->u[0], s->u[1], s->u[2]}};
-    hipLaunchKernelGGL(k_rhs, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, u, s->flags, s->b, s->partials,
-                       s->sc, padded);
+    hipLaunchKernelGGL(k_rhs, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->b, PA,
+                       sh.chunk, sh.blocks);
     if (!d->warm_start) hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p, nc);
 
     if (d->kind == FE_SOLVER_JACOBI) {
         // The reference's CudaJacobiIterationSolver uploads the previous x (warm start).
         float* xa = s->r;
         float* xb = s->z;
-        hipLaunchKernelGGL(k_f64_to_f32, grid1(nc), dim3(kBlock), 0, s->stream, s->p, xa, nc, 0);
+        hipLaunchKernelGGL(k_f64_to_f32, grid1(nc), dim3(kBlock), 0, s->stream, s->p, xa, nc);
+        bool pending = false;
+        int pending_it = 0;
         for (int it = 0; it < d->max_iterations; ++it) {
             kt_begin(s, FE_KERNEL_JACOBI);
-            hipLaunchKernelGGL(k_jacobi, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->flags, s->b, xa,
-                               xb, s->sc, ih2[0], ih2[1], ih2[2], padded);
+            hipLaunchKernelGGL(k_jacobi, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b, xa,
+                               xb, s->sc, pending ? PB : nullptr, nparts, pending_it, ih2[0], ih2[1], ih2[2],
+                               sh.chunk, sh.blocks);
             kt_end(s, FE_KERNEL_JACOBI);
             std::swap(xa, xb);
-            if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0)
-                hipLaunchKernelGGL(k_jacobi_check, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->flags,
-                                   s->b, xa, s->partials, s->sc, ih2[0], ih2[1], ih2[2], it, padded);
+            pending = false;
+            if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
+                hipLaunchKernelGGL(k_jacobi_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g,
+                                   s->flags, s->b, xa, s->sc, PB, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+                pending = true;
+                pending_it = it;
+            }
         }
+        if (pending)  // a check after the last sweep: the loop ends anyway, only record convergence
+            hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kBlock), 0, s->stream, PB, nparts, PA);
         hipLaunchKernelGGL(k_f32_to_f64, grid1(nc), dim3(kBlock), 0, s->stream, xa, s->p, nc);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = read_scalars(s)) != hipSuccess) return e;
+        if ((e = read_scalars(s, 0)) != hipSuccess) return e;
+        if (pending && !s->sc_host->done) {
+            double tot = 0.0;
+            if ((e = hipMemcpy(&tot, PA, sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess) return e;
+            if (std::sqrt(tot) < d->tolerance) {
+                s->sc_host->done = 1;
+                s->sc_host->converged = 1;
+                s->sc_host->it = pending_it + 1;
+            }
+        }
         kt_flush(s);
         return hipSuccess;
     }
 
     // CG / PCG
-    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->flags, s->b, s->p, s->r,
-                       s->z, s->d[0], s->partials, s->sc, ih2[0], ih2[1], ih2[2], padded);
+    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b, s->p,
+                       s->r, s->z, s->d[0], s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
     double* xa = s->p;
     double* xb = s->p2;
     int dcur = 0;
     const int batch = 16;
-    int launched = 0;
     const float* zr = init.precond ? s->z : s->r;
-    if ((e = read_scalars(s)) != hipSuccess) return e;
-    kt_flush(s);  // pairs of earlier kernels are complete after the sync
-    while (launched < d->max_iterations && !s->sc_host->done) {
-        const int nb = std::min(batch, d->max_iterations - launched);
-        const int it0 = s->sc_host->it;
+    if ((e = read_scalars(s, 0)) != hipSuccess) return e;
+    kt_flush(s, 1 << 30);  // pairs of earlier kernels are complete after the sync
+    // spmv k decides the stop of update k-1, so every batch ends with an spmv
+    // whose state slot the host reads; launches after the stop exit at once
+    // and are left out of the kernel timer (tags >= the stopping iteration).
+    int k = 0;
+    auto update = [&](int kk) {
+        dcur = 1 - dcur;
+        kt_begin(s, FE_KERNEL_CG_UPDATE, kk);
+        hipLaunchKernelGGL(k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b, xa,
+                           xb, s->d[dcur], s->q, s->r, s->z, s->sc + ((kk + 1) & 1), PA, PB, nparts, ih2[0], ih2[1],
+                           ih2[2], sh.chunk, sh.blocks);
+        kt_end(s, FE_KERNEL_CG_UPDATE);
+        std::swap(xa, xb);
+    };
+    for (;;) {
+        const int nb = std::min(batch, d->max_iterations + 1 - k);
         for (int t = 0; t < nb; ++t) {
-            kt_begin(s, FE_KERNEL_CG_SPMV);
-            hipLaunchKernelGGL(k_cg_spmv, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->flags, zr,
-                               s->d[dcur], s->d[1 - dcur], s->q, s->partials, s->sc, ih2[0], ih2[1], ih2[2], padded);
+            kt_begin(s, FE_KERNEL_CG_SPMV, k);
+            hipLaunchKernelGGL(k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, zr,
+                               s->d[dcur], s->d[1 - dcur], s->q, s->sc + (k & 1), s->sc + ((k + 1) & 1), PB, PA,
+                               nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
             kt_end(s, FE_KERNEL_CG_SPMV);
-            dcur = 1 - dcur;
-            kt_begin(s, FE_KERNEL_CG_UPDATE);
-            hipLaunchKernelGGL(k_cg_update, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->flags, s->b, xa,
-                               xb, s->d[dcur], s->q, s->r, s->z, s->partials, s->sc, ih2[0], ih2[1], ih2[2], padded);
-            kt_end(s, FE_KERNEL_CG_UPDATE);
-            std::swap(xa, xb);
+            if (t == nb - 1) break;
+            update(k);
+            ++k;
         }
-        launched += nb;
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = read_scalars(s)) != hipSuccess) return e;
-        // only the launches that iterated count for the kernel timer (the rest exit at once)
-        s->kt_used = std::min(s->kt_used, std::max(0, s->sc_host->it - it0));
-        kt_flush(s);
+        if ((e = read_scalars(s, (k + 1) & 1)) != hipSuccess) return e;
+        kt_flush(s, s->sc_host->done ? s->sc_host->it : (1 << 30));
+        if (s->sc_host->done || nb <= 0) break;
+        update(k);
+        ++k;
     }
     if (xa != s->p) {
         s->p2 = s->p;
