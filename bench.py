@@ -30,13 +30,13 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell for one launch of each timed kernel (FP32
 # vectors, FP64 solution, 1-byte markers whose 6 neighbours come from cache).
 KERNEL_BYTES_PER_CELL = {
-    # k_cg_update with the Jacobi preconditioner: read x(8) d(4) q(4) r(4) flag(1),
+    # k_cg_update with the Jacobi preconditioner: read x(8) d(4) q(4) r(4) row code(2),
     # write x'(8) r(4) z(4)
-    "cg_update": 37,
-    # k_cg_spmv: read z(4) d(4) flag(1), write d'(4) q(4)
-    "cg_spmv": 17,
-    # k_jacobi: read x(4) b(4) flag(1), write x'(4)
-    "jacobi": 13,
+    "cg_update": 38,
+    # k_cg_spmv: read z(4) d(4) row code(2), write d'(4) q(4)
+    "cg_spmv": 18,
+    # k_jacobi: read x(4) b(4) row code(2), write x'(4)
+    "jacobi": 14,
 }
 KERNEL_IDS = {"cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5}
 
@@ -52,7 +52,22 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (torch), the reference kernel that "
+                         "scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"bench: copy calibration {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
 
 
 def main():
@@ -114,6 +129,8 @@ def main():
                 "avg_launch_us": round(kavg_ms * 1e3, 3), "launches": klaunch,
                 "bytes_per_launch": bpl}
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, capi, desc, dt, args)

@@ -280,6 +280,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(dalloc(&s->valid[1], nf[0] + nf[1] + nf[2]));
     chk(dalloc(&s->sdf, nc));
     chk(dalloc(&s->flags, nc));
+    chk(dalloc(&s->code, nc));
     chk(dalloc(&s->p, nc));
     chk(dalloc(&s->p2, nc));
     chk(dalloc(&s->b, nc));
@@ -350,7 +351,7 @@ int fe_scene_destroy(fe_scene* s) {
         F(s->mark[c]);
     }
     F(s->valid[0]); F(s->valid[1]);
-    F(s->sdf); F(s->flags); F(s->p); F(s->b); F(s->r); F(s->z); F(s->p2);
+    F(s->sdf); F(s->flags); F(s->code); F(s->p); F(s->b); F(s->r); F(s->z); F(s->p2);
     F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->sc); F(s->cflmax);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
     for (int b = 0; b < 2; ++b) {

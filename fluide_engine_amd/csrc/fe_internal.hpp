@@ -2,6 +2,7 @@
 // HBM, launch helpers and the device-side geometry routines every kernel uses.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -210,6 +211,7 @@ struct fe_scene {
     uint8_t* valid[2] = {nullptr, nullptr};          // extrapolation validity ping-pong (all 3 comps)
     double* sdf = nullptr;
     uint8_t* flags = nullptr;
+    uint16_t* code = nullptr;     // per-cell 7-point row code (BuildSystem)
     double* p = nullptr;          // pressure (LinearSystem::x), FP64: kept between solves (warm start)
     double* p2 = nullptr;         // pressure ping-pong partner
     float* b = nullptr;
@@ -268,20 +270,22 @@ inline void kt_flush(fe_scene* s, int limit = 1 << 30) {
     s->kt_used = 0;
 }
 
-inline void kt_begin(fe_scene* s, int32_t id, int tag = 0) {
-    if (s->kt_id != id || s->kt_ev.empty()) return;
-    if (s->kt_used == kTimerPairs) {
-        (void)hipStreamSynchronize(s->stream);
-        kt_flush(s);
+// Launch `kernel` on the scene's stream; when the timer selects `id`, the
+// start/stop events ride on the dispatch packet itself (hipExtLaunchKernelGGL),
+// so they time the kernel's execution as rocprofv3's kernel trace does.
+template <typename... Args, typename F>
+inline void launch_timed(fe_scene* s, int32_t id, int tag, F kernel, dim3 grid, dim3 block, Args... args) {
+    if (s->kt_id == id && !s->kt_ev.empty()) {
+        if (s->kt_used == kTimerPairs) {
+            (void)hipStreamSynchronize(s->stream);
+            kt_flush(s);
+        }
+        const int u = s->kt_used++;
+        s->kt_tag[u] = tag;
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s->stream, s->kt_ev[2 * u], s->kt_ev[2 * u + 1], 0, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, s->stream, args...);
     }
-    s->kt_tag[s->kt_used] = tag;
-    (void)hipEventRecord(s->kt_ev[2 * s->kt_used], s->stream);
-}
-
-inline void kt_end(fe_scene* s, int32_t id) {
-    if (s->kt_id != id || s->kt_ev.empty()) return;
-    (void)hipEventRecord(s->kt_ev[2 * s->kt_used + 1], s->stream);
-    ++s->kt_used;
 }
 
 }  // namespace fe

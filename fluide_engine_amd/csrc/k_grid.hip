@@ -98,10 +98,8 @@ hipError_t launch_sdf(fe_scene* s) {
     for (int a = 0; a < 3; ++a) ext[a] = (int)std::floor(radius / g.hd[a] + 0.5);
     const int64_t nc = cell_count(g);
     const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-    kt_begin(s, FE_KERNEL_SDF);
-    hipLaunchKernelGGL(k_sdf, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, s->P[s->cur].x, s->cap,
-                       s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, padded);
-    kt_end(s, FE_KERNEL_SDF);
+    launch_timed(s, FE_KERNEL_SDF, 0, k_sdf, dim3((unsigned)padded), dim3(kBlock), g, (const float*)s->P[s->cur].x,
+                 s->cap, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, padded);
     return hipGetLastError();
 }
 

@@ -477,10 +477,9 @@ static hipError_t p2g_method(fe_scene* s) {
         const int64_t nf = face_count(g, c);
         const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
         auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
-        kt_begin(s, FE_KERNEL_P2G);
-        hipLaunchKernelGGL(kern, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, P.x, P.v, P.C, s->cap,
-                           s->cell_start, s->u[c], s->mark[c], s->snap[c], padded);
-        kt_end(s, FE_KERNEL_P2G);
+        launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.x,
+                     (const float*)P.v, (const float*)P.C, s->cap, (const int32_t*)s->cell_start, s->u[c],
+                     s->mark[c], s->snap[c], padded);
     }
     return hipGetLastError();
 }
@@ -505,24 +504,10 @@ hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bo
     uint32_t* key = emit_keys ? s->key : nullptr;
     int32_t* cnt = emit_keys ? s->cell_count : nullptr;
     const float blend = (float)s->desc.blend;
-    kt_begin(s, FE_KERNEL_G2P_MOVE);
-    switch (s->g.method) {
-        case FE_PIC:
-            hipLaunchKernelGGL(k_g2p_move<FE_PIC>, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P, s->cap, s->np,
-                               grid, snaps, blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key, cnt);
-            break;
-        case FE_FLIP:
-            hipLaunchKernelGGL(k_g2p_move<FE_FLIP>, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P, s->cap,
-                               s->np, grid, snaps, blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key,
-                               cnt);
-            break;
-        default:
-            hipLaunchKernelGGL(k_g2p_move<FE_APIC>, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, P, s->cap,
-                               s->np, grid, snaps, blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key,
-                               cnt);
-            break;
-    }
-    kt_end(s, FE_KERNEL_G2P_MOVE);
+    auto kern = s->g.method == FE_PIC ? k_g2p_move<FE_PIC>
+                                      : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP> : k_g2p_move<FE_APIC>);
+    launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid, snaps,
+                 blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key, cnt);
     if (do_move) s->sorted = false;
     return hipGetLastError();
 }

@@ -2,12 +2,14 @@
 // (src/fluid_solvers/single_phase_pressure_solver.cpp) on gfx950.
 //
 // The reference stores A as {center,right,up,front} doubles per cell
-// (src/linear_system/linear_system.hpp:9-15, 32 B/cell).  Here A is never
-// stored: it is rebuilt inside every stencil from the 1-byte cell markers with
-// BuildSystem's rules (:99-164) -- center = sum of 1/h^2 over in-range,
-// non-BOUNDRY neighbours; off-diagonal -1/h^2 between two FLUID cells;
-// non-FLUID rows are the identity.  r, z, d, q are FP32, the solution is FP64,
-// and every dot product and norm accumulates in FP64.
+// (src/linear_system/linear_system.hpp:9-15, 32 B/cell).  Here BuildSystem's
+// rules (:99-164) -- center = sum of 1/h^2 over in-range, non-BOUNDRY
+// neighbours; off-diagonal -1/h^2 between two FLUID cells; non-FLUID rows are
+// the identity -- are packed once per solve into a 16-bit code per cell
+// (which neighbours are coupled, how many neighbours count per axis), so the
+// iteration kernels read 2 B/cell of matrix and never divide an index.
+// r, z, d, q are FP32, the solution is FP64, and every dot product and norm
+// accumulates in FP64.
 //
 // Reductions: each kernel runs at most 1024 blocks over contiguous cell chunks
 // and leaves one FP64 partial per block and value.  The next kernel's blocks
@@ -24,80 +26,69 @@ namespace fe {
 constexpr int kMaxSolverBlocks = 1024;
 
 // ---------------------------------------------------------------------------
-// Cell coordinates and the matrix-free 7-point row.
+// The 16-bit row code.
+//   bits 0-5  : neighbour (i-1, i+1, j-1, j+1, k-1, k+1) is coupled (both FLUID)
+//   bits 6-11 : per axis x, y, z, 2 bits: in-range non-BOUNDRY neighbour count
+//   bit 12    : the cell is FLUID
 // ---------------------------------------------------------------------------
-struct Cell {
-    int i, j, k;
-    int64_t c;
+constexpr uint16_t kFluidBit = 1u << 12;
+
+struct Stride {
+    int64_t sy, sz;  // +-1 in j, k
 };
 
-__device__ inline Cell cell_of(const Geom& g, int64_t c) {
-    Cell r;
-    r.c = c;
-    r.i = (int)(c % g.n[0]);
-    r.j = (int)((c / g.n[0]) % g.n[1]);
-    r.k = (int)(c / ((int64_t)g.n[0] * g.n[1]));
-    return r;
-}
-
-struct Row {
-    float center;
-    float off[6];   // coefficient of neighbour (i-1, i+1, j-1, j+1, k-1, k+1)
-    int64_t nb[6];  // neighbour index (valid when off != 0)
-    bool fluid;
-};
-
-__device__ inline Row make_row(const Geom& g, const uint8_t* __restrict__ flags, const Cell& x, const float ih2[3]) {
-    Row r;
-    const int64_t sy = g.n[0], sz = (int64_t)g.n[0] * g.n[1];
-    r.nb[0] = x.c - 1; r.nb[1] = x.c + 1;
-    r.nb[2] = x.c - sy; r.nb[3] = x.c + sy;
-    r.nb[4] = x.c - sz; r.nb[5] = x.c + sz;
-    const bool in[6] = {x.i > 0, x.i + 1 < g.n[0], x.j > 0, x.j + 1 < g.n[1], x.k > 0, x.k + 1 < g.n[2]};
-    r.fluid = flags[x.c] == FE_FLUID;
-    float center = 0.0f;
+__device__ inline float code_center(uint16_t code, const float ih2[3]) {
+    if (!(code & kFluidBit)) return 1.0f;
+    // BuildSystem's accumulation order: +x, -x, +y, -y, +z, -z
+    float c = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) r.off[q] = 0.0f;
-    if (r.fluid) {
-        // BuildSystem's accumulation order: +x, -x, +y, -y, +z, -z
-        const int order[6] = {1, 0, 3, 2, 5, 4};
-#pragma unroll
-        for (int t = 0; t < 6; ++t) {
-            const int q = order[t];
-            if (!in[q]) continue;
-            const uint8_t fnb = flags[r.nb[q]];
-            if (fnb == FE_BOUNDRY) continue;
-            center += ih2[q >> 1];
-            if (fnb == FE_FLUID) r.off[q] = -ih2[q >> 1];
-        }
-    } else {
-        center = 1.0f;
+    for (int a = 0; a < 3; ++a) {
+        const int n = (code >> (6 + 2 * a)) & 3;
+        if (n >= 1) c += ih2[a];
+        if (n >= 2) c += ih2[a];
     }
-    r.center = center;
-    return r;
+    return c;
 }
 
-// (A v)(c) in the reference SpMV order (src/linear_system/blas.cpp:72-83).
-template <typename F>
-__device__ inline float row_apply(const Row& r, float vc, F vnb) {
-    float acc = r.center * vc;
+__device__ inline double code_center64(uint16_t code, const double ih2[3]) {
+    if (!(code & kFluidBit)) return 1.0;
+    double c = 0.0;
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-        if (r.off[q] != 0.0f) acc += r.off[q] * vnb(r.nb[q]);
+    for (int a = 0; a < 3; ++a) {
+        const int n = (code >> (6 + 2 * a)) & 3;
+        if (n >= 1) c += ih2[a];
+        if (n >= 2) c += ih2[a];
+    }
+    return c;
+}
+
+// (A v)(c) in the reference SpMV order (src/linear_system/blas.cpp:72-83):
+// center*v(c) + the coupled neighbours i-1, i+1, j-1, j+1, k-1, k+1.
+template <typename F>
+__device__ inline float code_apply(uint16_t code, float center, int64_t c, const Stride& st, const float ih2[3],
+                                   float vc, F v) {
+    float acc = center * vc;
+    if (code & 1) acc += -ih2[0] * v(c - 1);
+    if (code & 2) acc += -ih2[0] * v(c + 1);
+    if (code & 4) acc += -ih2[1] * v(c - st.sy);
+    if (code & 8) acc += -ih2[1] * v(c + st.sy);
+    if (code & 16) acc += -ih2[2] * v(c - st.sz);
+    if (code & 32) acc += -ih2[2] * v(c + st.sz);
     return acc;
 }
 
-// FP64 product of the same row (residuals of the FP64 solution).  The FP32
-// center is a sum of identical powers of 1/h^2 per axis, exact for
-// power-of-two spacings.
 template <typename F>
-__device__ inline double row_apply64(const Row& r, double xc, F xnb, const double ih2[3]) {
-    if (!r.fluid) return xc;
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-        if (r.off[q] != 0.0f) acc -= ih2[q >> 1] * xnb(r.nb[q]);
-    return (double)r.center * xc + acc;
+__device__ inline double code_apply64(uint16_t code, int64_t c, const Stride& st, const double ih2[3], double vc,
+                                      F v) {
+    if (!(code & kFluidBit)) return vc;
+    double off = 0.0;
+    if (code & 1) off -= ih2[0] * v(c - 1);
+    if (code & 2) off -= ih2[0] * v(c + 1);
+    if (code & 4) off -= ih2[1] * v(c - st.sy);
+    if (code & 8) off -= ih2[1] * v(c + st.sy);
+    if (code & 16) off -= ih2[2] * v(c - st.sz);
+    if (code & 32) off -= ih2[2] * v(c + st.sz);
+    return code_center64(code, ih2) * vc + off;
 }
 
 // ---------------------------------------------------------------------------
@@ -166,26 +157,43 @@ __device__ inline bool stop_rule(const SolverScalars* sc, double bnorm2, double 
 }
 
 // ---------------------------------------------------------------------------
-// Right-hand side: DivergenceAtCallCenter (face_centered_grid3d.cpp:157-166)
+// BuildSystem (single_phase_pressure_solver.cpp:99-164): the row codes and the
+// right-hand side b = DivergenceAtCallCenter (face_centered_grid3d.cpp:157-166)
 // on FLUID cells, 0 elsewhere; partials of b.b.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_rhs(Geom g, FaceSet u, const uint8_t* __restrict__ flags,
-                                                float* __restrict__ b, double* __restrict__ pout, int64_t chunk,
-                                                int64_t padded) {
+__global__ __launch_bounds__(kBlock) void k_build_system(Geom g, FaceSet u, const uint8_t* __restrict__ flags,
+                                                         uint16_t* __restrict__ code, float* __restrict__ b,
+                                                         double* __restrict__ pout, int64_t chunk, int64_t padded) {
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
+    const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1], nz = (uint32_t)g.n[2];
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
     double v[1] = {0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         float bb = 0.0f;
+        uint16_t cd = 0;
         if (flags[c] == FE_FLUID) {
-            const Cell x = cell_of(g, c);
-            const int64_t iu = (int64_t)x.i + (int64_t)(g.n[0] + 1) * ((int64_t)x.j + (int64_t)g.n[1] * x.k);
-            const int64_t iv = (int64_t)x.i + (int64_t)g.n[0] * ((int64_t)x.j + (int64_t)(g.n[1] + 1) * x.k);
+            const uint32_t cu = (uint32_t)c;  // grids hold fewer than 2^30 cells
+            const uint32_t i = cu % nx, j = (cu / nx) % ny, k = cu / (nx * ny);
+            const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+            const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+            cd = kFluidBit;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!in[q]) continue;
+                const uint8_t f = flags[nb[q]];
+                if (f == FE_BOUNDRY) continue;
+                cd = (uint16_t)(cd + (1u << (6 + 2 * (q >> 1))));
+                if (f == FE_FLUID) cd |= (uint16_t)(1u << q);
+            }
+            const int64_t iu = (int64_t)i + (int64_t)(nx + 1) * ((int64_t)j + (int64_t)ny * k);
+            const int64_t iv = (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)(ny + 1) * k);
             const float left = u.a[0][iu], right = u.a[0][iu + 1];
-            const float down = u.a[1][iv], up = u.a[1][iv + g.n[0]];
-            const float back = u.a[2][c], front = u.a[2][c + (int64_t)g.n[0] * g.n[1]];
+            const float down = u.a[1][iv], up = u.a[1][iv + nx];
+            const float back = u.a[2][c], front = u.a[2][c + sz];
             bb = (right - left) / g.h[0] + (up - down) / g.h[1] + (front - back) / g.h[2];
         }
+        code[c] = cd;
         b[c] = bb;
         v[0] += (double)bb * (double)bb;
     }
@@ -200,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Geom g, FaceSet u, const uint8_t
 // solve); after the stop every sweep only copies, so the buffer rotation stays
 // consistent.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_jacobi(Geom g, const uint8_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_jacobi(Geom g, const uint16_t* __restrict__ code,
                                                    const float* __restrict__ b, const float* __restrict__ xo,
                                                    float* __restrict__ xn, SolverScalars* sc,
                                                    const double* __restrict__ pcheck, int nparts, int check_it,
@@ -227,35 +235,35 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Geom g, const uint8_t* __rest
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
+    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    auto X = [&](int64_t n) { return xo[n]; };
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         if (done) {
             xn[c] = xo[c];
             continue;
         }
-        const Row r = make_row(g, flags, cell_of(g, c), ih2);
-        float off = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (r.off[q] != 0.0f) off += r.off[q] * xo[r.nb[q]];
-        xn[c] = (b[c] - off) / r.center;
+        const uint16_t cd = code[c];
+        const float center = code_center(cd, ih2);
+        // off-diagonal sum in the reference order (i-1, i+1, j-1, j+1, k-1, k+1)
+        const float off = code_apply(cd, 0.0f, c, st, ih2, 0.0f, X);
+        xn[c] = (b[c] - off) / center;
     }
 }
 
 // Partials of ||b - A x||^2 for the Jacobi convergence check (FP64 sums).
-__global__ __launch_bounds__(kBlock) void k_jacobi_residual(Geom g, const uint8_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_jacobi_residual(Geom g, const uint16_t* __restrict__ code,
                                                             const float* __restrict__ b,
                                                             const float* __restrict__ x, const SolverScalars* sc,
                                                             double* __restrict__ pout, float ihx2, float ihy2,
                                                             float ihz2, int64_t chunk, int64_t padded) {
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
-    const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
+    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
     double v[1] = {0.0};
     if (!sc->done) {
         for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-            const Row r = make_row(g, flags, cell_of(g, c), ih2);
-            const double ax = row_apply64(r, (double)x[c], [&](int64_t n) { return (double)x[n]; }, ih2d);
+            const double ax = code_apply64(code[c], c, st, ih2d, (double)x[c], [&](int64_t n) { return (double)x[n]; });
             const double res = (double)b[c] - ax;
             v[0] += res * res;
         }
@@ -294,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_zero_f64(double* __restrict__ x, int
 // With the relative rule a stop is only accepted on a freshly replaced
 // residual, so the decision is taken on b - A x, not on the recurrence.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint8_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __restrict__ code,
                                                     const float* __restrict__ b, const double* __restrict__ x,
                                                     float* __restrict__ r, float* __restrict__ z,
                                                     float* __restrict__ d0, SolverScalars* sc,
@@ -308,16 +316,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint8_t* __res
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
+    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
     const bool pre = sc->precond != 0;
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const Row row = make_row(g, flags, cell_of(g, c), ih2);
-        const double ax = row_apply64(row, x[c], [&](int64_t n) { return x[n]; }, ih2d);
+        const uint16_t cd = code[c];
+        const double ax = code_apply64(cd, c, st, ih2d, x[c], [&](int64_t n) { return x[n]; });
         const float rc = (float)((double)b[c] - ax);
         r[c] = rc;
         float zc = rc;
         if (pre) {
-            zc = rc / row.center;
+            zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
         d0[c] = 0.0f;
@@ -329,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint8_t* __res
 
 // State slots alternate with the iteration (spmv k reads slot k&1 and writes
 // slot (k+1)&1), so no block can see a half-updated state.
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint8_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __restrict__ code,
                                                     const float* __restrict__ z, const float* __restrict__ dold,
                                                     float* __restrict__ dnew, float* __restrict__ q,
                                                     const SolverScalars* __restrict__ in,
@@ -384,13 +393,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint8_t* __res
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
+    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
     const float fb = (float)beta;
     auto D = [&](int64_t n) { return fb * dold[n] + z[n]; };  // AXpY(beta, d, s, &d)
     double v[1] = {0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const Row row = make_row(g, flags, cell_of(g, c), ih2);
+        const uint16_t cd = code[c];
         const float dc = D(c);
-        const float qc = row_apply(row, dc, D);
+        const float qc = code_apply(cd, code_center(cd, ih2), c, st, ih2, dc, D);
         dnew[c] = dc;
         q[c] = qc;
         v[0] += (double)dc * (double)qc;
@@ -398,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint8_t* __res
     write_partials<1>(v, pout);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint8_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __restrict__ code,
                                                       const float* __restrict__ b, const double* __restrict__ xo,
                                                       double* __restrict__ xn, const float* __restrict__ d,
                                                       const float* __restrict__ q, float* __restrict__ r,
@@ -423,25 +433,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint8_t* __r
     const bool pre = sc->precond != 0;
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
+    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
     const float fa = (float)alpha;
     auto X = [&](int64_t n) { return alpha * (double)d[n] + xo[n]; };
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const double xc = X(c);
         xn[c] = xc;
-        float rc;
-        float center = 1.0f;
-        if (replace || pre) {
-            const Row row = make_row(g, flags, cell_of(g, c), ih2);
-            center = row.center;
-            rc = replace ? (float)((double)b[c] - row_apply64(row, xc, X, ih2d)) : -fa * q[c] + r[c];
-        } else {
-            rc = -fa * q[c] + r[c];
-        }
+        const uint16_t cd = (replace || pre) ? code[c] : 0;
+        const float rc = replace ? (float)((double)b[c] - code_apply64(cd, c, st, ih2d, xc, X)) : -fa * q[c] + r[c];
         r[c] = rc;
         float zc = rc;
         if (pre) {
-            zc = rc / center;
+            zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
         v[0] += (double)rc * (double)zc;
@@ -450,34 +454,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint8_t* __r
     write_partials<2>(v, pout);
 }
 
-// Final ||b - A p||^2 and ||b||^2 in FP64 on the FP64 solution, with exact
-// FP64 coefficients.
-__global__ __launch_bounds__(kBlock) void k_final_residual(Geom g, const uint8_t* __restrict__ flags,
+// Final ||b - A p||^2 and ||b||^2 in FP64 on the FP64 solution, with FP64
+// coefficients 1/h^2.
+__global__ __launch_bounds__(kBlock) void k_final_residual(Geom g, const uint16_t* __restrict__ code,
                                                            const float* __restrict__ b, const double* __restrict__ x,
                                                            double* __restrict__ pout, double ihx2, double ihy2,
                                                            double ihz2, int64_t chunk, int64_t padded) {
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const double ih2[3] = {ihx2, ihy2, ihz2};
+    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const Cell cc = cell_of(g, c);
-        double ax = x[c];
-        if (flags[c] == FE_FLUID) {
-            const int64_t sy = g.n[0], sz = (int64_t)g.n[0] * g.n[1];
-            const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-            const bool in[6] = {cc.i > 0, cc.i + 1 < g.n[0], cc.j > 0, cc.j + 1 < g.n[1], cc.k > 0, cc.k + 1 < g.n[2]};
-            double center = 0.0, off = 0.0;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!in[q]) continue;
-                const uint8_t fnb = flags[nb[q]];
-                if (fnb == FE_BOUNDRY) continue;
-                center += ih2[q >> 1];
-                if (fnb == FE_FLUID) off -= ih2[q >> 1] * x[nb[q]];
-            }
-            ax = center * x[c] + off;
-        }
+        const double ax = code_apply64(code[c], c, st, ih2, x[c], [&](int64_t n) { return x[n]; });
         const double res = (double)b[c] - ax;
         v[0] += res * res;
         v[1] += (double)b[c] * (double)b[c];
@@ -517,7 +506,7 @@ hipError_t launch_final_residual(fe_scene* s) {
     const Geom& g = s->g;
     const SolverShape sh = solver_shape(g);
     double* part = s->partials;
-    hipLaunchKernelGGL(k_final_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b,
+    hipLaunchKernelGGL(k_final_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b,
                        s->p, part, g.ihd[0] * g.ihd[0], g.ihd[1] * g.ihd[1], g.ihd[2] * g.ihd[2], sh.chunk, sh.blocks);
     // totals (||r||^2, ||b||^2) land after the partials; read_final_residual fetches them
     hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kBlock), 0, s->stream, part, (int)sh.blocks,
@@ -563,8 +552,8 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     if ((e = hipMemcpyAsync(s->sc, &init, sizeof(SolverScalars), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
         return e;
     FaceSet u{{s->u[0], s->u[1], s->u[2]}};
-    hipLaunchKernelGGL(k_rhs, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->b, PA,
-                       sh.chunk, sh.blocks);
+    hipLaunchKernelGGL(k_build_system, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->code,
+                       s->b, PA, sh.chunk, sh.blocks);
     if (!d->warm_start) hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p, nc);
 
     if (d->kind == FE_SOLVER_JACOBI) {
@@ -575,16 +564,15 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         bool pending = false;
         int pending_it = 0;
         for (int it = 0; it < d->max_iterations; ++it) {
-            kt_begin(s, FE_KERNEL_JACOBI);
-            hipLaunchKernelGGL(k_jacobi, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b, xa,
-                               xb, s->sc, pending ? PB : nullptr, nparts, pending_it, ih2[0], ih2[1], ih2[2],
-                               sh.chunk, sh.blocks);
-            kt_end(s, FE_KERNEL_JACOBI);
+            launch_timed(s, FE_KERNEL_JACOBI, 0, k_jacobi, dim3((unsigned)sh.blocks), dim3(kBlock), g,
+                         (const uint16_t*)s->code, (const float*)s->b, (const float*)xa, xb, s->sc,
+                         (const double*)(pending ? PB : nullptr), nparts, pending_it, ih2[0], ih2[1], ih2[2], sh.chunk,
+                         sh.blocks);
             std::swap(xa, xb);
             pending = false;
             if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
                 hipLaunchKernelGGL(k_jacobi_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g,
-                                   s->flags, s->b, xa, s->sc, PB, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+                                   s->code, s->b, xa, s->sc, PB, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
                 pending = true;
                 pending_it = it;
             }
@@ -608,7 +596,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     }
 
     // CG / PCG
-    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b, s->p,
+    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
                        s->r, s->z, s->d[0], s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
     double* xa = s->p;
     double* xb = s->p2;
@@ -616,28 +604,26 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     const int batch = 16;
     const float* zr = init.precond ? s->z : s->r;
     if ((e = read_scalars(s, 0)) != hipSuccess) return e;
-    kt_flush(s, 1 << 30);  // pairs of earlier kernels are complete after the sync
+    kt_flush(s);  // pairs of earlier kernels are complete after the sync
     // spmv k decides the stop of update k-1, so every batch ends with an spmv
     // whose state slot the host reads; launches after the stop exit at once
     // and are left out of the kernel timer (tags >= the stopping iteration).
     int k = 0;
     auto update = [&](int kk) {
         dcur = 1 - dcur;
-        kt_begin(s, FE_KERNEL_CG_UPDATE, kk);
-        hipLaunchKernelGGL(k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, s->b, xa,
-                           xb, s->d[dcur], s->q, s->r, s->z, s->sc + ((kk + 1) & 1), PA, PB, nparts, ih2[0], ih2[1],
-                           ih2[2], sh.chunk, sh.blocks);
-        kt_end(s, FE_KERNEL_CG_UPDATE);
+        launch_timed(s, FE_KERNEL_CG_UPDATE, kk, k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), g,
+                     (const uint16_t*)s->code, (const float*)s->b, (const double*)xa, xb, (const float*)s->d[dcur],
+                     (const float*)s->q, s->r, s->z, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
+                     ih2[1], ih2[2], sh.chunk, sh.blocks);
         std::swap(xa, xb);
     };
     for (;;) {
         const int nb = std::min(batch, d->max_iterations + 1 - k);
         for (int t = 0; t < nb; ++t) {
-            kt_begin(s, FE_KERNEL_CG_SPMV, k);
-            hipLaunchKernelGGL(k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->flags, zr,
-                               s->d[dcur], s->d[1 - dcur], s->q, s->sc + (k & 1), s->sc + ((k + 1) & 1), PB, PA,
-                               nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
-            kt_end(s, FE_KERNEL_CG_SPMV);
+            launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
+                         (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
+                         (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, PA,
+                         nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
             if (t == nb - 1) break;
             update(k);
             ++k;
