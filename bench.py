@@ -3,8 +3,8 @@
 A step is one substep of the hybrid particle-grid path (PhysicsAnimation::
 AdvanceSingleFrame with frame dt = 1/480, which takes one substep at this CFL):
 P2G, fluid SDF + markers, extrapolation, walls, gravity, pressure projection
-(Jacobi-preconditioned CG to ||b-Ax||/||b|| <= 1e-6, warm-started from the
-previous pressure), G2P and midpoint advection, on 4,021,248 particles seeded
+(multigrid-preconditioned CG to ||b-Ax||/||b|| <= 1e-6, checked in FP64,
+warm-started from the previous pressure), G2P and midpoint advection, on 4,021,248 particles seeded
 2x2x2 per fluid cell in x < 0.4, y < 0.6.
 
 Prints one JSON line: Mparticle-substeps/s (whole job), the median pressure-solve
@@ -30,15 +30,24 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per cell for one launch of each timed kernel (FP32
 # vectors, FP64 solution, 1-byte markers whose 6 neighbours come from cache).
 KERNEL_BYTES_PER_CELL = {
-    # k_cg_update with the Jacobi preconditioner: read x(8) d(4) q(4) r(4) row code(2),
-    # write x'(8) r(4) z(4)
-    "cg_update": 38,
+    # k_cg_update: read x(8) d(4) q(4) r(4), write x'(8) r(4); the diagonal
+    # preconditioner adds the row code(2) and z(4) (see update_bytes)
+    "cg_update": 32,
     # k_cg_spmv: read z(4) d(4) row code(2), write d'(4) q(4)
     "cg_spmv": 18,
     # k_jacobi: read x(4) b(4) row code(2), write x'(4)
     "jacobi": 14,
 }
-KERNEL_IDS = {"cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5}
+KERNEL_IDS = {"none": 0, "cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5}
+SOLVERS = {"jacobi": 0, "cg": 1, "pcg": 2, "mgpcg": 3}
+SOLVER_TEXT = {"jacobi": "Jacobi sweeps", "cg": "CG", "pcg": "Jacobi-PCG", "mgpcg": "multigrid-PCG"}
+
+
+def kernel_bytes_per_cell(kernel, solver):
+    b = KERNEL_BYTES_PER_CELL[kernel]
+    if kernel == "cg_update" and solver == "pcg":
+        b += 6
+    return b
 
 
 def parse():
@@ -48,6 +57,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--kernel", default="cg_update", choices=sorted(KERNEL_IDS))
+    ap.add_argument("--solver", default="mgpcg", choices=sorted(SOLVERS))
     ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -85,7 +95,10 @@ def main():
     x = capi.seed_particles(desc, [capi.box((-1.0, -1.0, -1.0), (0.4, 0.6, 2.0))], seed=0)
     v = np.zeros_like(x)
     npart = x.shape[0]
-    solver = capi.solver(capi.SOLVER_PCG_JACOBI, 2000, 0, 1e-6, relative=True, warm_start=True)
+    if args.solver == "jacobi":  # the reference's CUDA default: 1000 sweeps, ||b-Ax|| < 1e-14 checked every 5
+        solver = capi.solver(capi.SOLVER_JACOBI, 1000, 5, 1e-14, warm_start=True)
+    else:
+        solver = capi.solver(SOLVERS[args.solver], 2000, 0, 1e-6, relative=True, warm_start=True)
 
     ctx = capi.Context(0)
     scene = capi.Scene(ctx, desc)
@@ -122,7 +135,7 @@ def main():
     kavg_ms = kms / max(klaunch, 1)
     roof = None
     if args.kernel in KERNEL_BYTES_PER_CELL and klaunch > 0:
-        bpl = KERNEL_BYTES_PER_CELL[args.kernel] * ncells
+        bpl = kernel_bytes_per_cell(args.kernel, args.solver) * ncells
         achieved = bpl / (kavg_ms * 1e-3) / 1e9
         roof = {"kernel": args.kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -149,7 +162,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"FLIP (PIC weight 0.03) dam break {n}^3, {npart} particles, dt=1/480, "
-                               f"Jacobi-PCG to 1e-6 relative residual (FP64-checked), warm start",
+                               f"{SOLVER_TEXT[args.solver]} to 1e-6 relative residual (FP64-checked), warm start",
+                   "solver": args.solver,
                    "grid": [n, n, n], "particles": npart, "substeps": sub_total},
         "pressure_ms_median": round(statistics.median(pressure_ms), 4),
         "pressure_iterations_median": statistics.median(iters),

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfluide_hip.so")
 
 PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
-SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI = 0, 1, 2
+SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI, SOLVER_MGPCG = 0, 1, 2, 3
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
  GRID_PRESSURE, GRID_RHS) = range(13)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,

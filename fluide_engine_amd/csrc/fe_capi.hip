@@ -6,7 +6,7 @@
 #include <cstring>
 #include <string>
 
-#include "fe_kernels.hpp"
+#include "fe_stencil.hpp"
 
 using namespace fe;
 
@@ -99,7 +99,7 @@ void* field_ptr(fe_scene* s, int32_t field) {
 
 int check_solver(fe_ctx* ctx, const fe_solver_desc* d) {
     if (!d) return set_err(ctx, FE_E_INVALID, "solver descriptor is NULL");
-    if (d->kind < FE_SOLVER_JACOBI || d->kind > FE_SOLVER_PCG_JACOBI)
+    if (d->kind < FE_SOLVER_JACOBI || d->kind > FE_SOLVER_MGPCG)
         return set_err(ctx, FE_E_UNSUPPORTED, "unknown solver kind " + std::to_string(d->kind));
     if (d->max_iterations < 0 || d->max_iterations > 1000000)
         return set_err(ctx, FE_E_INVALID, "max_iterations out of range");
@@ -289,7 +289,8 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(dalloc(&s->d[0], nc));
     chk(dalloc(&s->d[1], nc));
     chk(dalloc(&s->q, nc));
-    s->npartials = pad_blocks((nc + kBlock - 1) / kBlock) * 4;
+    // CG: 2 + 2 partials per block, multigrid: 1 more (r.z of the V-cycle)
+    s->npartials = solver_shape_n(nc).blocks * 6;
     chk(dalloc(&s->partials, s->npartials));
     chk(dalloc(&s->sc, 2));  // two state slots (CG alternates them)
     chk(hipHostMalloc((void**)&s->sc_host, sizeof(SolverScalars), hipHostMallocDefault));
@@ -353,6 +354,7 @@ int fe_scene_destroy(fe_scene* s) {
     F(s->valid[0]); F(s->valid[1]);
     F(s->sdf); F(s->flags); F(s->code); F(s->p); F(s->b); F(s->r); F(s->z); F(s->p2);
     F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->sc); F(s->cflmax);
+    mg_free(s);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);

@@ -188,6 +188,20 @@ struct Particles {
     int32_t* pid = nullptr;
 };
 
+// One level of the multigrid preconditioner (level 0 is the pressure grid).
+struct MGLevel {
+    int n[3] = {0, 0, 0};
+    int64_t cells = 0;
+    float ih2[3] = {0, 0, 0};
+    uint8_t* flags = nullptr;   // level 0: the scene's markers
+    uint16_t* code = nullptr;   // level 0: the scene's row codes
+    float* x = nullptr;         // correction and its ping-pong partner
+    float* x2 = nullptr;
+    float* b = nullptr;         // right-hand side (level 0: the CG residual, not owned)
+    float* r = nullptr;         // residual scratch
+    bool owns = false;          // flags/code/b allocated here (levels >= 1)
+};
+
 }  // namespace fe
 
 struct fe_ctx {
@@ -242,6 +256,7 @@ struct fe_scene {
     hipEvent_t ev[fe::kMaxStages + 1] = {};
     double stage_ms[fe::kMaxStages] = {};
     fe_solve_stats last_stats{};
+    std::vector<fe::MGLevel> mg;  // multigrid hierarchy, built on first use
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

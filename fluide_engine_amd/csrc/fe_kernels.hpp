@@ -37,4 +37,9 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d);  // rhs 
 hipError_t launch_final_residual(fe_scene* s);  // FP64 ||b - A p||^2 and ||b||^2 (device)
 hipError_t read_final_residual(fe_scene* s, double* rr, double* bb);  // waits for them
 
+// k_mg.hip: multigrid V-cycle preconditioner of the pressure CG
+hipError_t mg_setup(fe_scene* s);  // coarse markers and row codes from s->flags / s->code
+hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz);
+void mg_free(fe_scene* s);
+
 }  // namespace fe

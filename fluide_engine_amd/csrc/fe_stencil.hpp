@@ -1,0 +1,174 @@
+// fe_stencil.hpp -- matrix-free 7-point rows of the pressure system and the
+// deterministic block reductions shared by the CG and multigrid kernels.
+#pragma once
+
+#include <algorithm>
+
+#include "fe_kernels.hpp"
+
+namespace fe {
+
+// Grid-wide kernels with reductions run at most this many blocks over
+// contiguous cell chunks (one FP64 partial per block and value).
+constexpr int kMaxSolverBlocks = 1024;
+
+struct SolverShape {
+    int64_t blocks;  // multiple of 8 (XCD remap)
+    int64_t chunk;   // cells per block, multiple of kBlock
+};
+
+inline SolverShape solver_shape_n(int64_t nc) {
+    int64_t blocks = std::min<int64_t>(pad_blocks((nc + kBlock - 1) / kBlock), kMaxSolverBlocks);
+    int64_t chunk = (nc + blocks - 1) / blocks;
+    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
+    blocks = pad_blocks((nc + chunk - 1) / chunk);
+    return {blocks, chunk};
+}
+
+// ---------------------------------------------------------------------------
+// The 16-bit row code.
+//   bits 0-5  : neighbour (i-1, i+1, j-1, j+1, k-1, k+1) is coupled (both FLUID)
+//   bits 6-11 : per axis x, y, z, 2 bits: in-range non-BOUNDRY neighbour count
+//   bit 12    : the cell is FLUID
+// ---------------------------------------------------------------------------
+constexpr uint16_t kFluidBit = 1u << 12;
+
+struct Stride {
+    int64_t sy, sz;  // +-1 in j, k
+};
+
+__device__ inline float code_center(uint16_t code, const float ih2[3]) {
+    if (!(code & kFluidBit)) return 1.0f;
+    // BuildSystem's accumulation order: +x, -x, +y, -y, +z, -z
+    float c = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n = (code >> (6 + 2 * a)) & 3;
+        if (n >= 1) c += ih2[a];
+        if (n >= 2) c += ih2[a];
+    }
+    return c;
+}
+
+__device__ inline double code_center64(uint16_t code, const double ih2[3]) {
+    if (!(code & kFluidBit)) return 1.0;
+    double c = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n = (code >> (6 + 2 * a)) & 3;
+        if (n >= 1) c += ih2[a];
+        if (n >= 2) c += ih2[a];
+    }
+    return c;
+}
+
+// (A v)(c) in the reference SpMV order (src/linear_system/blas.cpp:72-83):
+// center*v(c) + the coupled neighbours i-1, i+1, j-1, j+1, k-1, k+1.
+template <typename F>
+__device__ inline float code_apply(uint16_t code, float center, int64_t c, const Stride& st, const float ih2[3],
+                                   float vc, F v) {
+    float acc = center * vc;
+    if (code & 1) acc += -ih2[0] * v(c - 1);
+    if (code & 2) acc += -ih2[0] * v(c + 1);
+    if (code & 4) acc += -ih2[1] * v(c - st.sy);
+    if (code & 8) acc += -ih2[1] * v(c + st.sy);
+    if (code & 16) acc += -ih2[2] * v(c - st.sz);
+    if (code & 32) acc += -ih2[2] * v(c + st.sz);
+    return acc;
+}
+
+template <typename F>
+__device__ inline double code_apply64(uint16_t code, int64_t c, const Stride& st, const double ih2[3], double vc,
+                                      F v) {
+    if (!(code & kFluidBit)) return vc;
+    double off = 0.0;
+    if (code & 1) off -= ih2[0] * v(c - 1);
+    if (code & 2) off -= ih2[0] * v(c + 1);
+    if (code & 4) off -= ih2[1] * v(c - st.sy);
+    if (code & 8) off -= ih2[1] * v(c + st.sy);
+    if (code & 16) off -= ih2[2] * v(c - st.sz);
+    if (code & 32) off -= ih2[2] * v(c + st.sz);
+    return code_center64(code, ih2) * vc + off;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shape and block-level FP64 reductions.
+// ---------------------------------------------------------------------------
+struct Span {
+    int64_t c0, c1;
+};
+
+__device__ inline Span block_span(int64_t nc, int64_t chunk, int64_t padded) {
+    const int64_t lb = xcd_remap(blockIdx.x, padded);
+    Span s;
+    s.c0 = lb * chunk;
+    s.c1 = min(s.c0 + chunk, nc);
+    return s;
+}
+
+template <int NV>
+__device__ inline void block_reduce(double (&v)[NV], double (*sm)[kBlock / 64]) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) sm[q][wid] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) t += sm[q][w];
+        v[q] = t;  // every thread holds the block total
+    }
+    __syncthreads();
+}
+
+// One FP64 partial per block and value.
+template <int NV>
+__device__ inline void write_partials(double (&v)[NV], double* __restrict__ part) {
+    __shared__ double sm[NV][kBlock / 64];
+    block_reduce<NV>(v, sm);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) part[(int64_t)blockIdx.x * NV + q] = v[q];
+    }
+}
+
+// Sum of the previous kernel's partials, identical in every block.
+template <int NV>
+__device__ inline void sum_partials(const double* __restrict__ part, int nparts, double (&tot)[NV]) {
+    __shared__ double sm[NV][kBlock / 64];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] += part[(int64_t)b * NV + q];
+    }
+    block_reduce<NV>(tot, sm);
+}
+
+// Row code of cell (i,j,k) from the markers with BuildSystem's rules
+// (src/fluid_solvers/single_phase_pressure_solver.cpp:116-161).
+__device__ inline uint16_t row_code(const uint8_t* __restrict__ flags, int64_t c, uint32_t i, uint32_t j, uint32_t k,
+                                    uint32_t nx, uint32_t ny, uint32_t nz) {
+    if (flags[c] != FE_FLUID) return 0;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
+    const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
+    uint16_t cd = kFluidBit;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (!in[q]) continue;
+        const uint8_t f = flags[nb[q]];
+        if (f == FE_BOUNDRY) continue;
+        cd = (uint16_t)(cd + (1u << (6 + 2 * (q >> 1))));
+        if (f == FE_FLUID) cd |= (uint16_t)(1u << q);
+    }
+    return cd;
+}
+
+}  // namespace fe

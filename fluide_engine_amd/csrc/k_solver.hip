@@ -19,137 +19,10 @@
 #include <algorithm>
 #include <utility>
 
-#include "fe_kernels.hpp"
+#include "fe_stencil.hpp"
 
 namespace fe {
 
-constexpr int kMaxSolverBlocks = 1024;
-
-// ---------------------------------------------------------------------------
-// The 16-bit row code.
-//   bits 0-5  : neighbour (i-1, i+1, j-1, j+1, k-1, k+1) is coupled (both FLUID)
-//   bits 6-11 : per axis x, y, z, 2 bits: in-range non-BOUNDRY neighbour count
-//   bit 12    : the cell is FLUID
-// ---------------------------------------------------------------------------
-constexpr uint16_t kFluidBit = 1u << 12;
-
-struct Stride {
-    int64_t sy, sz;  // +-1 in j, k
-};
-
-__device__ inline float code_center(uint16_t code, const float ih2[3]) {
-    if (!(code & kFluidBit)) return 1.0f;
-    // BuildSystem's accumulation order: +x, -x, +y, -y, +z, -z
-    float c = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int n = (code >> (6 + 2 * a)) & 3;
-        if (n >= 1) c += ih2[a];
-        if (n >= 2) c += ih2[a];
-    }
-    return c;
-}
-
-__device__ inline double code_center64(uint16_t code, const double ih2[3]) {
-    if (!(code & kFluidBit)) return 1.0;
-    double c = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int n = (code >> (6 + 2 * a)) & 3;
-        if (n >= 1) c += ih2[a];
-        if (n >= 2) c += ih2[a];
-    }
-    return c;
-}
-
-// (A v)(c) in the reference SpMV order (src/linear_system/blas.cpp:72-83):
-// center*v(c) + the coupled neighbours i-1, i+1, j-1, j+1, k-1, k+1.
-template <typename F>
-__device__ inline float code_apply(uint16_t code, float center, int64_t c, const Stride& st, const float ih2[3],
-                                   float vc, F v) {
-    float acc = center * vc;
-    if (code & 1) acc += -ih2[0] * v(c - 1);
-    if (code & 2) acc += -ih2[0] * v(c + 1);
-    if (code & 4) acc += -ih2[1] * v(c - st.sy);
-    if (code & 8) acc += -ih2[1] * v(c + st.sy);
-    if (code & 16) acc += -ih2[2] * v(c - st.sz);
-    if (code & 32) acc += -ih2[2] * v(c + st.sz);
-    return acc;
-}
-
-template <typename F>
-__device__ inline double code_apply64(uint16_t code, int64_t c, const Stride& st, const double ih2[3], double vc,
-                                      F v) {
-    if (!(code & kFluidBit)) return vc;
-    double off = 0.0;
-    if (code & 1) off -= ih2[0] * v(c - 1);
-    if (code & 2) off -= ih2[0] * v(c + 1);
-    if (code & 4) off -= ih2[1] * v(c - st.sy);
-    if (code & 8) off -= ih2[1] * v(c + st.sy);
-    if (code & 16) off -= ih2[2] * v(c - st.sz);
-    if (code & 32) off -= ih2[2] * v(c + st.sz);
-    return code_center64(code, ih2) * vc + off;
-}
-
-// ---------------------------------------------------------------------------
-// Launch shape and block-level FP64 reductions.
-// ---------------------------------------------------------------------------
-struct Span {
-    int64_t c0, c1;
-};
-
-__device__ inline Span block_span(int64_t nc, int64_t chunk, int64_t padded) {
-    const int64_t lb = xcd_remap(blockIdx.x, padded);
-    Span s;
-    s.c0 = lb * chunk;
-    s.c1 = min(s.c0 + chunk, nc);
-    return s;
-}
-
-template <int NV>
-__device__ inline void block_reduce(double (&v)[NV], double (*sm)[kBlock / 64]) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double x = v[q];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-        if (lane == 0) sm[q][wid] = x;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) t += sm[q][w];
-        v[q] = t;  // every thread holds the block total
-    }
-    __syncthreads();
-}
-
-// One FP64 partial per block and value.
-template <int NV>
-__device__ inline void write_partials(double (&v)[NV], double* __restrict__ part) {
-    __shared__ double sm[NV][kBlock / 64];
-    block_reduce<NV>(v, sm);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) part[(int64_t)blockIdx.x * NV + q] = v[q];
-    }
-}
-
-// Sum of the previous kernel's partials, identical in every block.
-template <int NV>
-__device__ inline void sum_partials(const double* __restrict__ part, int nparts, double (&tot)[NV]) {
-    __shared__ double sm[NV][kBlock / 64];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) tot[q] += part[(int64_t)b * NV + q];
-    }
-    block_reduce<NV>(tot, sm);
-}
 
 __device__ inline bool stop_rule(const SolverScalars* sc, double bnorm2, double sigma, double rr) {
     if (sc->relative) return sqrt(rr) <= sc->tol * sqrt(bnorm2);
@@ -317,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
     const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
-    const bool pre = sc->precond != 0;
+    const int pre = sc->precond;  // 0 none, 1 diagonal, 2 multigrid (z and r.z come later)
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const uint16_t cd = code[c];
@@ -325,12 +198,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
         const float rc = (float)((double)b[c] - ax);
         r[c] = rc;
         float zc = rc;
-        if (pre) {
+        if (pre == 1) {
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
         d0[c] = 0.0f;
-        v[0] += (double)rc * (double)zc;
+        if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
     }
     write_partials<2>(v, pout);
@@ -343,15 +216,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
                                                     float* __restrict__ dnew, float* __restrict__ q,
                                                     const SolverScalars* __restrict__ in,
                                                     SolverScalars* __restrict__ out, const double* __restrict__ pin,
-                                                    double* __restrict__ pout, int nparts, float ihx2, float ihy2,
-                                                    float ihz2, int64_t chunk, int64_t padded) {
+                                                    const double* __restrict__ pz, double* __restrict__ pout,
+                                                    int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
+                                                    int64_t padded) {
     if (in->done) {  // uniform: an earlier spmv stopped the solve
         if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
         return;
     }
-    // ---- state update from the last update's r.z and r.r (every block alike)
+    // ---- state update from the last update's r.z and r.r (every block alike);
+    // with the multigrid preconditioner r.z comes from the V-cycle's last sweep
     double t[2];
     sum_partials<2>(pin, nparts, t);
+    if (pz) {
+        double tz[1];
+        sum_partials<1>(pz, nparts, tz);
+        t[0] = tz[0];
+    }
     const int it_prev = in->it;
     double beta;
     int it, trig, done, conv, repl;
@@ -430,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
         sc->alpha = alpha;
     }
     const bool replace = sc->replace != 0;
-    const bool pre = sc->precond != 0;
+    const int pre = sc->precond;
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
     const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
@@ -440,15 +320,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const double xc = X(c);
         xn[c] = xc;
-        const uint16_t cd = (replace || pre) ? code[c] : 0;
+        const uint16_t cd = (replace || pre == 1) ? code[c] : 0;
         const float rc = replace ? (float)((double)b[c] - code_apply64(cd, c, st, ih2d, xc, X)) : -fa * q[c] + r[c];
         r[c] = rc;
         float zc = rc;
-        if (pre) {
+        if (pre == 1) {
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
-        v[0] += (double)rc * (double)zc;
+        if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
     }
     write_partials<2>(v, pout);
@@ -488,19 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double* __restrict__ pa
 // ---------------------------------------------------------------------------
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-struct SolverShape {
-    int64_t blocks;  // multiple of 8 (XCD remap)
-    int64_t chunk;   // cells per block, multiple of kBlock
-};
-
-static SolverShape solver_shape(const Geom& g) {
-    const int64_t nc = cell_count(g);
-    int64_t blocks = std::min<int64_t>(pad_blocks((nc + kBlock - 1) / kBlock), kMaxSolverBlocks);
-    int64_t chunk = (nc + blocks - 1) / blocks;
-    chunk = (chunk + kBlock - 1) / kBlock * kBlock;
-    blocks = pad_blocks((nc + chunk - 1) / chunk);
-    return {blocks, chunk};
-}
+static SolverShape solver_shape(const Geom& g) { return solver_shape_n(cell_count(g)); }
 
 hipError_t launch_final_residual(fe_scene* s) {
     const Geom& g = s->g;
@@ -546,7 +414,9 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     SolverScalars init{};
     init.tol = d->tolerance;
     init.relative = d->relative;
-    init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : 0;
+    init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
+    const bool mg = init.precond == 2;
+    double* PZ = s->partials + 4 * sh.blocks;  // r.z of the V-cycle, 1 per block
     init.max_it = d->max_iterations;
     init.it = -1;
     if ((e = hipMemcpyAsync(s->sc, &init, sizeof(SolverScalars), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
@@ -596,13 +466,17 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     }
 
     // CG / PCG
+    if (mg && (e = mg_setup(s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
                        s->r, s->z, s->d[0], s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+    if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ)) != hipSuccess) return e;
     double* xa = s->p;
     double* xb = s->p2;
     int dcur = 0;
-    const int batch = 16;
+    // iterations per host check: fewer when each carries a V-cycle
+    const int batch = mg ? 4 : 16;
     const float* zr = init.precond ? s->z : s->r;
+    const double* pz = mg ? PZ : nullptr;
     if ((e = read_scalars(s, 0)) != hipSuccess) return e;
     kt_flush(s);  // pairs of earlier kernels are complete after the sync
     // spmv k decides the stop of update k-1, so every batch ends with an spmv
@@ -616,14 +490,15 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
                      (const float*)s->q, s->r, s->z, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
                      ih2[1], ih2[2], sh.chunk, sh.blocks);
         std::swap(xa, xb);
+        if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ);
     };
     for (;;) {
         const int nb = std::min(batch, d->max_iterations + 1 - k);
         for (int t = 0; t < nb; ++t) {
             launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                          (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
-                         (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, PA,
-                         nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+                         (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, pz,
+                         PA, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
             if (t == nb - 1) break;
             update(k);
             ++k;

@@ -51,8 +51,10 @@ enum fe_marker { FE_AIR = 0, FE_FLUID = 1, FE_BOUNDRY = 2 };
  *   identity preconditioner, residual replacement every 50 iterations or when
  *   sigma grows; stop on sigma <= tol^2, or ||r||/||b|| <= tol when relative.
  * PCG_JACOBI: CG with the diagonal preconditioner (no reference counterpart).
- * Vectors are FP32; dot products and norms accumulate in FP64. */
-enum fe_solver_kind { FE_SOLVER_JACOBI = 0, FE_SOLVER_CG = 1, FE_SOLVER_PCG_JACOBI = 2 };
+ * MGPCG: CG preconditioned by one geometric multigrid V-cycle per iteration
+ *   (no reference counterpart; same stopping rules and final residual as CG).
+ * Vectors are FP32; the solution and every dot product and norm are FP64. */
+enum fe_solver_kind { FE_SOLVER_JACOBI = 0, FE_SOLVER_CG = 1, FE_SOLVER_PCG_JACOBI = 2, FE_SOLVER_MGPCG = 3 };
 
 typedef struct fe_solver_desc {
     int32_t kind;

@@ -170,16 +170,17 @@ def test_extrapolate_bc_gravity_parity(ctx):
 # ---------------------------------------------------------------------------
 # pressure projection
 # ---------------------------------------------------------------------------
-def _pressure_pair(ctx, n, seed):
-    desc = capi.scene_desc((n, n, n), method=capi.FLIP, max_particles=16)
+def _pressure_pair(ctx, n, seed, noise=0.03):
+    shape = (n, n, n) if np.isscalar(n) else tuple(n)  # (nx, ny, nz)
+    desc = capi.scene_desc(shape, method=capi.FLIP, max_particles=16)
     g = capi.Scene(ctx, desc)
-    o = orc.Scene((n, n, n), method=orc.FLIP)
+    o = orc.Scene(shape, method=orc.FLIP)
     rng = np.random.default_rng(seed)
     # fluid: a dam block plus noise; the SDF decides the markers on both sides
-    c = (np.arange(n) + 0.5) / n
-    K, J, I = np.meshgrid(c, c, c, indexing="ij")
-    sdf = np.where((I < 0.45) & (J < 0.55), -0.5 / n, 0.5 / n)
-    sdf[rng.uniform(size=sdf.shape) < 0.03] *= -1
+    K, J, I = np.meshgrid(*[(np.arange(m) + 0.5) / m for m in shape[::-1]], indexing="ij")
+    hmax = 1.0 / min(shape)
+    sdf = np.where((I < 0.45) & (J < 0.55), -0.5 * hmax, 0.5 * hmax)
+    sdf[rng.uniform(size=sdf.shape) < noise] *= -1
     g.set(capi.GRID_SDF, sdf); o.set(orc.SDF, sdf)
     for gf, of in GRID:
         vals = rng.normal(size=g.shape(gf)).astype(np.float32)
@@ -201,19 +202,53 @@ def test_pressure_jacobi_parity(ctx):
     g.close()
 
 
-@pytest.mark.parametrize("kind", [capi.SOLVER_CG, capi.SOLVER_PCG_JACOBI])
-def test_pressure_cg_reaches_tolerance(ctx, kind):
-    print(f"[gpu] pressure cg kind={kind}", flush=True)
-    g, o = _pressure_pair(ctx, 32, 13)
+ORC_SOLVER = {capi.SOLVER_CG: orc.CG, capi.SOLVER_PCG_JACOBI: orc.PCG_JACOBI,
+              capi.SOLVER_MGPCG: orc.CG}  # the oracle's CG gives the converged answer for MGPCG
+
+
+@pytest.mark.parametrize("shape", [32, (40, 24, 17)])
+@pytest.mark.parametrize("kind", [capi.SOLVER_CG, capi.SOLVER_PCG_JACOBI, capi.SOLVER_MGPCG])
+def test_pressure_cg_reaches_tolerance(ctx, kind, shape):
+    print(f"[gpu] pressure cg kind={kind} shape={shape}", flush=True)
+    g, o = _pressure_pair(ctx, shape, 13)
     sg = g.run_stage(capi.STAGE_PRESSURE, desc=capi.solver(kind, 2000, 0, 1e-6, relative=True, warm_start=False))
-    so = o.pressure(orc.solver(orc.CG if kind == capi.SOLVER_CG else orc.PCG_JACOBI, 2000, 0, 1e-10,
-                               relative=True, warm_start=False))
+    so = o.pressure(orc.solver(ORC_SOLVER[kind], 2000, 0, 1e-10, relative=True, warm_start=False))
     assert sg.converged == 1
-    assert sg.rel_residual <= 1e-6 * 1.05, sg.rel_residual  # FP64 residual of the FP32 solution
+    assert sg.rel_residual <= 1e-6 * 1.05, sg.rel_residual  # FP64 residual of the FP64 solution
     # the 1e-6 solution against the oracle's 1e-10 one
     rel_close(g.get(capi.GRID_PRESSURE), o.get(orc.PRESSURE), 1e-4, "cg pressure")
     for gf, of in GRID:
         rel_close(g.get(gf), o.get(of), 1e-4, f"projected {gf}")
+    if kind == capi.SOLVER_MGPCG:
+        # about 15-20 on these noisy systems (CPU model of the same V-cycle)
+        assert sg.iterations <= 40, sg.iterations
+    g.close()
+
+
+def test_mgpcg_iterations_and_warm_start(ctx):
+    """MGPCG against diagonal PCG on the same 64^3 system: far fewer
+    iterations, the same answer; a warm-started second solve of the solved
+    system stops at once."""
+    print("[gpu] mgpcg 64^3", flush=True)
+    g, _ = _pressure_pair(ctx, 64, 17, noise=0.0)
+    base = [g.get(f) for f, _ in GRID]
+    sj = g.run_stage(capi.STAGE_PRESSURE, desc=capi.solver(capi.SOLVER_PCG_JACOBI, 4000, 0, 1e-7, relative=True,
+                                                           warm_start=False))
+    pj = g.get(capi.GRID_PRESSURE)
+    for (f, _), b in zip(GRID, base):
+        g.set(f, b)
+    sm = g.run_stage(capi.STAGE_PRESSURE, desc=capi.solver(capi.SOLVER_MGPCG, 4000, 0, 1e-7, relative=True,
+                                                           warm_start=False))
+    pm = g.get(capi.GRID_PRESSURE)
+    assert sj.converged == 1 and sm.converged == 1
+    assert sm.rel_residual <= 1.05e-7 and sj.rel_residual <= 1.05e-7
+    assert sm.iterations * 8 < sj.iterations, (sm.iterations, sj.iterations)
+    rel_close(pm, pj, 2e-5, "MGPCG vs PCG pressure")
+    for (f, _), b in zip(GRID, base):
+        g.set(f, b)
+    sw = g.run_stage(capi.STAGE_PRESSURE, desc=capi.solver(capi.SOLVER_MGPCG, 4000, 0, 1e-6, relative=True,
+                                                           warm_start=True))
+    assert sw.converged == 1 and sw.iterations <= 2, sw.iterations
     g.close()
 
 
