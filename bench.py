@@ -37,8 +37,11 @@ KERNEL_BYTES_PER_CELL = {
     "cg_spmv": 18,
     # k_jacobi: read x(4) b(4) row code(2), write x'(4)
     "jacobi": 14,
+    # k_mg_smooth on the finest level: read x(4) b(4) row code(2), write x'(4)
+    "mg_smooth": 14,
 }
-KERNEL_IDS = {"none": 0, "cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5}
+KERNEL_IDS = {"none": 0, "cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5,
+              "mg_smooth": 7}
 SOLVERS = {"jacobi": 0, "cg": 1, "pcg": 2, "mgpcg": 3}
 SOLVER_TEXT = {"jacobi": "Jacobi sweeps", "cg": "CG", "pcg": "Jacobi-PCG", "mgpcg": "multigrid-PCG"}
 
@@ -46,6 +49,8 @@ SOLVER_TEXT = {"jacobi": "Jacobi sweeps", "cg": "CG", "pcg": "Jacobi-PCG", "mgpc
 def kernel_bytes_per_cell(kernel, solver):
     b = KERNEL_BYTES_PER_CELL[kernel]
     if kernel == "cg_update" and solver == "pcg":
+        b += 6
+    if kernel == "cg_update" and solver == "mgpcg":  # row code(2) + the V-cycle's first sweep(4)
         b += 6
     return b
 

@@ -29,7 +29,8 @@ EXPORTED = [
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
     "fe_kernel_timer_read", "fe_last_solve_stats",
 ]
-KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI = range(7)
+(KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
+ KERNEL_MG_SMOOTH) = range(8)
 
 
 class SolverDesc(C.Structure):

@@ -168,6 +168,7 @@ struct SolverScalars {
     double beta;
     double bnorm2;     // b . b
     double tol;
+    double rr_ref;     // r . r right after the last residual recomputation
     int32_t it;        // iterations done (-1 before the first spmv)
     int32_t done;      // stopping rule met or iteration cap reached
     int32_t trigger;   // sigma grew: recompute the residual next iteration
@@ -256,7 +257,8 @@ struct fe_scene {
     hipEvent_t ev[fe::kMaxStages + 1] = {};
     double stage_ms[fe::kMaxStages] = {};
     fe_solve_stats last_stats{};
-    std::vector<fe::MGLevel> mg;  // multigrid hierarchy, built on first use
+    std::vector<fe::MGLevel> mg;
+    int mg_tail = 0;  // first level handled by the one-workgroup tail kernel  // multigrid hierarchy, built on first use
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

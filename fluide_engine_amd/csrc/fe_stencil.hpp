@@ -35,7 +35,12 @@ constexpr uint16_t kFluidBit = 1u << 12;
 
 struct Stride {
     int64_t sy, sz;  // +-1 in j, k
+    int64_t last;    // cells - 1
 };
+
+__host__ __device__ inline Stride stride_of(int64_t nx, int64_t ny, int64_t nz) {
+    return Stride{nx, nx * ny, nx * ny * nz - 1};
+}
 
 __device__ inline float code_center(uint16_t code, const float ih2[3]) {
     if (!(code & kFluidBit)) return 1.0f;
@@ -63,17 +68,23 @@ __device__ inline double code_center64(uint16_t code, const double ih2[3]) {
 }
 
 // (A v)(c) in the reference SpMV order (src/linear_system/blas.cpp:72-83):
-// center*v(c) + the coupled neighbours i-1, i+1, j-1, j+1, k-1, k+1.
+// center*v(c) + the coupled neighbours i-1, i+1, j-1, j+1, k-1, k+1.  All six
+// neighbour values are fetched (at indices clamped into the grid) before the
+// row code is consulted, so the loads never wait on it; an uncoupled
+// neighbour contributes fma(0, v, acc) = acc (every vector holds finite values).
 template <typename F>
 __device__ inline float code_apply(uint16_t code, float center, int64_t c, const Stride& st, const float ih2[3],
                                    float vc, F v) {
+    const float vxm = v(c > 0 ? c - 1 : 0), vxp = v(c < st.last ? c + 1 : st.last);
+    const float vym = v(c >= st.sy ? c - st.sy : 0), vyp = v(c + st.sy <= st.last ? c + st.sy : st.last);
+    const float vzm = v(c >= st.sz ? c - st.sz : 0), vzp = v(c + st.sz <= st.last ? c + st.sz : st.last);
     float acc = center * vc;
-    if (code & 1) acc += -ih2[0] * v(c - 1);
-    if (code & 2) acc += -ih2[0] * v(c + 1);
-    if (code & 4) acc += -ih2[1] * v(c - st.sy);
-    if (code & 8) acc += -ih2[1] * v(c + st.sy);
-    if (code & 16) acc += -ih2[2] * v(c - st.sz);
-    if (code & 32) acc += -ih2[2] * v(c + st.sz);
+    acc = fmaf((code & 1) ? -ih2[0] : 0.0f, vxm, acc);
+    acc = fmaf((code & 2) ? -ih2[0] : 0.0f, vxp, acc);
+    acc = fmaf((code & 4) ? -ih2[1] : 0.0f, vym, acc);
+    acc = fmaf((code & 8) ? -ih2[1] : 0.0f, vyp, acc);
+    acc = fmaf((code & 16) ? -ih2[2] : 0.0f, vzm, acc);
+    acc = fmaf((code & 32) ? -ih2[2] : 0.0f, vzp, acc);
     return acc;
 }
 
@@ -89,6 +100,15 @@ __device__ inline double code_apply64(uint16_t code, int64_t c, const Stride& st
     if (code & 16) off -= ih2[2] * v(c - st.sz);
     if (code & 32) off -= ih2[2] * v(c + st.sz);
     return code_center64(code, ih2) * vc + off;
+}
+
+// Damped Jacobi weight omega / A(c,c) of the multigrid smoother (0 for a FLUID
+// cell with no diagonal, i.e. walled in by BOUNDRY cells on every side).
+constexpr float kMgOmega = 2.0f / 3.0f;
+
+__device__ inline float mg_weight(uint16_t code, const float ih2[3]) {
+    const float dg = code_center(code, ih2);
+    return dg > 0.0f ? kMgOmega / dg : 0.0f;
 }
 
 // ---------------------------------------------------------------------------

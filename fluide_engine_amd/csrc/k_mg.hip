@@ -7,40 +7,56 @@
 // child is FLUID, else BOUNDRY (the free surface keeps its Dirichlet cells on
 // every level, as in McAdams, Sifakis and Teran, "A parallel multigrid Poisson
 // solver for fluids simulation on large grids", SCA 2010), and its operator is
-// BuildSystem's 7-point rule on those markers with twice the spacing.  The V-cycle is symmetric, so the
-// preconditioner is SPD and CG stays CG:
-//   * damped Jacobi smoothing, the same number of sweeps before and after;
+// BuildSystem's 7-point rule on those markers with twice the spacing.
+//
+// The V-cycle is symmetric, so the preconditioner is SPD and CG stays CG:
+//   * damped Jacobi smoothing (omega = 2/3), two sweeps before and two after;
 //   * prolongation: trilinear interpolation of the cell-centred coarse
 //     correction (weights 3/4, 1/4 per axis; out-of-range coarse neighbours
 //     clamp onto the boundary cell, which is the Neumann wall);
-//   * restriction: exactly the transpose of that prolongation divided by 8;
-//   * coarsest level (<= 1024 cells): damped Jacobi sweeps from zero in one
-//     workgroup, in LDS.
+//   * restriction: the transpose of that prolongation divided by 8;
+//   * coarsest level (<= 512 cells): 16 damped Jacobi sweeps (omega = 0.9)
+//     from zero, a fixed polynomial in the operator.
 // Non-FLUID cells carry no residual into the coarse levels and take no
 // correction; on level 0 they are smoothed like the identity rows they are.
+//
+// Kernels per V-cycle on the large levels: the first sweep from zero is fused
+// into the kernel that produces the level's right-hand side (the CG update on
+// level 0, the restriction below it), then one sweep, the residual, the
+// restriction; on the way up the prolongation and two sweeps.  All levels of
+// at most 4096 cells run in one workgroup in LDS (k_mg_tail), which replaces
+// about ten latency-bound launches.
 #include <algorithm>
 
 #include "fe_stencil.hpp"
 
 namespace fe {
 
-constexpr float kOmega = 2.0f / 3.0f;       // smoother damping
-constexpr float kOmegaCoarse = 0.9f;        // coarsest-level Jacobi damping
-constexpr int kPreSweeps = 2;               // = post sweeps (symmetry)
-constexpr int kCoarseSweeps = 64;
-constexpr int kCoarseMax = 1024;            // cells of the coarsest level (one workgroup)
+constexpr float kOmegaCoarse = 0.9f;  // coarsest-level Jacobi damping
+constexpr int kCoarseSweeps = 16;
+constexpr int kCoarseMax = 512;       // cells of the coarsest level
+constexpr int kTailMax = 4096;        // levels of at most this many cells run in k_mg_tail
+constexpr int kTailLds = 4608;        // cells of all tail levels together (3 floats + a row code each in LDS)
+constexpr int kTailThreads = 1024;
+constexpr int kMaxTail = 8;
 
 // ---------------------------------------------------------------------------
 // Level construction
 // ---------------------------------------------------------------------------
+__device__ inline void cell_ijk(int64_t c, int nx, int ny, int& i, int& j, int& k) {
+    const uint32_t cu = (uint32_t)c;
+    i = (int)(cu % (uint32_t)nx);
+    j = (int)((cu / (uint32_t)nx) % (uint32_t)ny);
+    k = (int)(cu / ((uint32_t)nx * (uint32_t)ny));
+}
+
 __global__ __launch_bounds__(kBlock) void k_mg_coarsen(const uint8_t* __restrict__ ff, int fx, int fy, int fz,
                                                        uint8_t* __restrict__ cf, int cx, int cy, int cz) {
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t nc = (int64_t)cx * cy * cz;
     if (c >= nc) return;
-    const uint32_t cu = (uint32_t)c;
-    const int I = (int)(cu % (uint32_t)cx), J = (int)((cu / (uint32_t)cx) % (uint32_t)cy),
-              K = (int)(cu / ((uint32_t)cx * (uint32_t)cy));
+    int I, J, K;
+    cell_ijk(c, cx, cy, I, J, K);
     bool fluid = false, air = false;
     for (int dk = 0; dk < 2; ++dk) {
         const int k = 2 * K + dk;
@@ -65,72 +81,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_codes(const uint8_t* __restrict__
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t nc = (int64_t)nx * ny * nz;
     if (c >= nc) return;
-    const uint32_t cu = (uint32_t)c;
-    const uint32_t i = cu % (uint32_t)nx, j = (cu / (uint32_t)nx) % (uint32_t)ny,
-                   k = cu / ((uint32_t)nx * (uint32_t)ny);
-    code[c] = row_code(flags, c, i, j, k, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz);
+    int i, j, k;
+    cell_ijk(c, nx, ny, i, j, k);
+    code[c] = row_code(flags, c, (uint32_t)i, (uint32_t)j, (uint32_t)k, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz);
 }
 
 // ---------------------------------------------------------------------------
-// Smoothing and residual
+// Transfer weights
 // ---------------------------------------------------------------------------
-struct LevelArgs {
-    int64_t nc;
-    int64_t sy, sz;
-    float ih2[3];
-    int64_t chunk, padded;
-};
-
-// x_out = x_in + omega (b - A x_in) / D; x_in == nullptr means x_in = 0.
-// With prz != nullptr (last level-0 sweep) also leaves partials of b . x_out,
-// which is r . z of the CG.
-__global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
-                                                      const float* __restrict__ b, const float* __restrict__ xin,
-                                                      float* __restrict__ xout, float omega,
-                                                      const SolverScalars* __restrict__ st,
-                                                      double* __restrict__ prz) {
-    if (st && st->done) return;  // uniform: the solve has stopped
-    const Span sp = block_span(L.nc, L.chunk, L.padded);
-    const Stride stv{L.sy, L.sz};
-    double v[1] = {0.0};
-    auto X = [&](int64_t n) { return xin[n]; };
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const uint16_t cd = code[c];
-        const float dgn = code_center(cd, L.ih2);
-        const float w = dgn > 0.0f ? omega / dgn : 0.0f;  // a FLUID cell walled in on all sides has no diagonal
-        float xo;
-        if (!xin) {
-            xo = w * b[c];
-        } else {
-            const float xc = xin[c];
-            const float ax = code_apply(cd, dgn, c, stv, L.ih2, xc, X);
-            xo = xc + w * (b[c] - ax);
-        }
-        xout[c] = xo;
-        if (prz) v[0] += (double)b[c] * (double)xo;
-    }
-    if (prz) write_partials<1>(v, prz);
-}
-
-__global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint16_t* __restrict__ code,
-                                                        const float* __restrict__ b, const float* __restrict__ x,
-                                                        float* __restrict__ r, const SolverScalars* __restrict__ st) {
-    if (st && st->done) return;
-    const Span sp = block_span(L.nc, L.chunk, L.padded);
-    const Stride stv{L.sy, L.sz};
-    auto X = [&](int64_t n) { return x[n]; };
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const uint16_t cd = code[c];
-        float rc = 0.0f;
-        if (cd & kFluidBit) rc = b[c] - code_apply(cd, code_center(cd, L.ih2), c, stv, L.ih2, x[c], X);
-        r[c] = rc;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Transfer operators
-// ---------------------------------------------------------------------------
-// Prolongation weights of fine index fi on one axis: two coarse neighbours.
+// Prolongation of fine index fi on one axis: two coarse neighbours a, b.
 __device__ inline void prolong_axis(int fi, int nc, int& a, float& wa, int& b, float& wb) {
     const int m = fi >> 1;
     if (fi & 1) {
@@ -154,104 +113,305 @@ __device__ inline float restrict_w(int fi, int I, int nc) {
     return (a == I ? wa : 0.0f) + (b == I ? wb : 0.0f);
 }
 
-// b_c = (1/8) sum_f w(f, c) r_f over the fine cells whose prolongation stencil
-// reaches c: fine indices 2I-1 .. 2I+2 per axis.
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf, int fx, int fy, int fz,
-                                                        const uint16_t* __restrict__ ccode, float* __restrict__ bc,
-                                                        int cx, int cy, int cz, const SolverScalars* __restrict__ st) {
-    if (st && st->done) return;
-    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t nc = (int64_t)cx * cy * cz;
-    if (c >= nc) return;
-    if (!(ccode[c] & kFluidBit)) {
-        bc[c] = 0.0f;
-        return;
-    }
-    const uint32_t cu = (uint32_t)c;
-    const int I = (int)(cu % (uint32_t)cx), J = (int)((cu / (uint32_t)cx) % (uint32_t)cy),
-              K = (int)(cu / ((uint32_t)cx * (uint32_t)cy));
-    float wx[4], wy[4], wz[4];
+// The fine indices 2I-1 .. 2I+2 whose prolongation reaches coarse index I,
+// clamped into range, with their weights (0 outside the grid).
+__device__ inline void restrict_taps(int I, int nf, int nc, int (&idx)[4], float (&w)[4]) {
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int fi = 2 * I - 1 + t, fj = 2 * J - 1 + t, fk = 2 * K - 1 + t;
-        wx[t] = (fi >= 0 && fi < fx) ? restrict_w(fi, I, cx) : 0.0f;
-        wy[t] = (fj >= 0 && fj < fy) ? restrict_w(fj, J, cy) : 0.0f;
-        wz[t] = (fk >= 0 && fk < fz) ? restrict_w(fk, K, cz) : 0.0f;
+        const int fi = 2 * I - 1 + t;
+        const bool in = fi >= 0 && fi < nf;
+        idx[t] = min(max(fi, 0), nf - 1);
+        w[t] = in ? restrict_w(fi, I, nc) : 0.0f;
     }
-    float acc = 0.0f;
-    for (int tk = 0; tk < 4; ++tk) {
-        if (wz[tk] == 0.0f) continue;
-        const int64_t kk = 2 * K - 1 + tk;
-        for (int tj = 0; tj < 4; ++tj) {
-            const float wyz = wy[tj] * wz[tk];
-            if (wyz == 0.0f) continue;
-            const int64_t row = (int64_t)fx * ((int64_t)(2 * J - 1 + tj) + (int64_t)fy * kk);
-            for (int ti = 0; ti < 4; ++ti) {
-                if (wx[ti] == 0.0f) continue;
-                acc += (wx[ti] * wyz) * rf[row + (2 * I - 1 + ti)];
-            }
-        }
-    }
-    bc[c] = acc * 0.125f;
 }
 
-// x_f += sum_c w(f, c) x_c on FLUID fine cells.
-__global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, int cx, int cy, int cz,
-                                                       const uint16_t* __restrict__ fcode, float* __restrict__ xf,
-                                                       int fx, int fy, int fz, const SolverScalars* __restrict__ st) {
-    if (st && st->done) return;
-    const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t nf = (int64_t)fx * fy * fz;
-    if (f >= nf) return;
-    if (!(fcode[f] & kFluidBit)) return;
-    const uint32_t fu = (uint32_t)f;
-    const int i = (int)(fu % (uint32_t)fx), j = (int)((fu / (uint32_t)fx) % (uint32_t)fy),
-              k = (int)(fu / ((uint32_t)fx * (uint32_t)fy));
+// (1/8) sum of the 4x4x4 fine window, 64 independent loads.
+template <typename F>
+__device__ inline float restrict_sum(int I, int J, int K, int fx, int fy, int fz, int cx, int cy, int cz, F R) {
+    int ix[4], iy[4], iz[4];
+    float wx[4], wy[4], wz[4];
+    restrict_taps(I, fx, cx, ix, wx);
+    restrict_taps(J, fy, cy, iy, wy);
+    restrict_taps(K, fz, cz, iz, wz);
+    float acc = 0.0f;
+#pragma unroll
+    for (int tk = 0; tk < 4; ++tk) {
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj) {
+            const float wjk = wy[tj] * wz[tk];
+            const int64_t row = (int64_t)fx * ((int64_t)iy[tj] + (int64_t)fy * iz[tk]);
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) acc += (wx[ti] * wjk) * R(row + ix[ti]);
+        }
+    }
+    return acc * 0.125f;
+}
+
+// Trilinear prolongation of the coarse correction at fine cell (i, j, k).
+template <typename F>
+__device__ inline float prolong_at(int i, int j, int k, int cx, int cy, int cz, F X) {
     int ia, ib, ja, jb, ka, kb;
     float wia, wib, wja, wjb, wka, wkb;
     prolong_axis(i, cx, ia, wia, ib, wib);
     prolong_axis(j, cy, ja, wja, jb, wjb);
     prolong_axis(k, cz, ka, wka, kb, wkb);
-    auto X = [&](int a, int b, int c) { return xc[(int64_t)a + (int64_t)cx * ((int64_t)b + (int64_t)cy * c)]; };
-    const float v = wka * (wja * (wia * X(ia, ja, ka) + wib * X(ib, ja, ka)) +
-                           wjb * (wia * X(ia, jb, ka) + wib * X(ib, jb, ka))) +
-                    wkb * (wja * (wia * X(ia, ja, kb) + wib * X(ib, ja, kb)) +
-                           wjb * (wia * X(ia, jb, kb) + wib * X(ib, jb, kb)));
-    xf[f] += v;
+    auto V = [&](int a, int b, int c) { return X((int64_t)a + (int64_t)cx * ((int64_t)b + (int64_t)cy * c)); };
+    return wka * (wja * (wia * V(ia, ja, ka) + wib * V(ib, ja, ka)) + wjb * (wia * V(ia, jb, ka) + wib * V(ib, jb, ka))) +
+           wkb * (wja * (wia * V(ia, ja, kb) + wib * V(ib, ja, kb)) + wjb * (wia * V(ia, jb, kb) + wib * V(ib, jb, kb)));
 }
 
-// Coarsest level: damped Jacobi from zero, entirely in LDS (one workgroup).
-__global__ __launch_bounds__(kCoarseMax) void k_mg_coarse(int nx, int ny, int nz, const uint16_t* __restrict__ code,
-                                                           const float* __restrict__ b, float* __restrict__ x,
-                                                           float ihx2, float ihy2, float ihz2, float omega,
-                                                           int sweeps, const SolverScalars* __restrict__ st) {
-    if (st && st->done) return;
-    __shared__ float xs[2][kCoarseMax];
-    __shared__ float bs[kCoarseMax];
-    __shared__ uint16_t cs[kCoarseMax];
-    const int nc = nx * ny * nz;
-    const int c = threadIdx.x;
-    const float ih2[3] = {ihx2, ihy2, ihz2};
-    if (c < nc) {
-        cs[c] = code[c];
-        bs[c] = b[c];
-        xs[0][c] = 0.0f;
+// ---------------------------------------------------------------------------
+// Large levels: one kernel per step
+// ---------------------------------------------------------------------------
+struct LevelArgs {
+    int64_t nc;
+    Stride st;
+    float ih2[3];
+    int64_t chunk, padded;
+};
+
+// x_out = x_in + w (b - A x_in).  With prz != nullptr (the last level-0 sweep)
+// also leaves partials of b . x_out, which is r . z of the CG.
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
+                                                      const float* __restrict__ b, const float* __restrict__ xin,
+                                                      float* __restrict__ xout, const SolverScalars* __restrict__ st,
+                                                      double* __restrict__ prz) {
+    if (st->done) return;  // uniform: the solve has stopped
+    const Span sp = block_span(L.nc, L.chunk, L.padded);
+    const Stride stv = L.st;
+    double v[1] = {0.0};
+    auto X = [&](int64_t n) { return xin[n]; };
+#pragma unroll 2
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+        const uint16_t cd = code[c];
+        const float dgn = code_center(cd, L.ih2);
+        const float w = dgn > 0.0f ? kMgOmega / dgn : 0.0f;
+        const float xc = xin[c];
+        const float bc = b[c];
+        const float xo = xc + w * (bc - code_apply(cd, dgn, c, stv, L.ih2, xc, X));
+        xout[c] = xo;
+        v[0] += (double)bc * (double)xo;
+    }
+    if (prz) write_partials<1>(v, prz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint16_t* __restrict__ code,
+                                                        const float* __restrict__ b, const float* __restrict__ x,
+                                                        float* __restrict__ r, const SolverScalars* __restrict__ st) {
+    if (st->done) return;
+    const Span sp = block_span(L.nc, L.chunk, L.padded);
+    const Stride stv = L.st;
+    auto X = [&](int64_t n) { return x[n]; };
+#pragma unroll 2
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+        const uint16_t cd = code[c];
+        float rc = 0.0f;
+        if (cd & kFluidBit) rc = b[c] - code_apply(cd, code_center(cd, L.ih2), c, stv, L.ih2, x[c], X);
+        r[c] = rc;
+    }
+}
+
+// b_c = R r_f on coarse FLUID cells (0 elsewhere); with x1 != nullptr also the
+// coarse level's first sweep from zero, x1 = w b_c.  A workgroup covers an
+// 8x8x4 coarse tile and first stages the 18x18x10 fine window it reads in LDS.
+constexpr int kRtx = 8, kRty = 8, kRtz = 4;
+constexpr int kRwx = 2 * kRtx + 2, kRwy = 2 * kRty + 2, kRwz = 2 * kRtz + 2;
+static_assert(kRtx * kRty * kRtz == kBlock, "one coarse cell per thread");
+
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf, int fx, int fy, int fz,
+                                                        const uint16_t* __restrict__ ccode, float* __restrict__ bc,
+                                                        float* __restrict__ x1, int cx, int cy, int cz, float cihx2,
+                                                        float cihy2, float cihz2,
+                                                        const SolverScalars* __restrict__ st) {
+    if (st->done) return;
+    __shared__ float win[kRwz][kRwy][kRwx];
+    const int I0 = blockIdx.x * kRtx, J0 = blockIdx.y * kRty, K0 = blockIdx.z * kRtz;
+    const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1, fk0 = 2 * K0 - 1;
+    for (int e = threadIdx.x; e < kRwx * kRwy * kRwz; e += kBlock) {
+        const int wi = e % kRwx, wj = (e / kRwx) % kRwy, wk = e / (kRwx * kRwy);
+        const int fi = min(max(fi0 + wi, 0), fx - 1);
+        const int fj = min(max(fj0 + wj, 0), fy - 1);
+        const int fk = min(max(fk0 + wk, 0), fz - 1);
+        win[wk][wj][wi] = rf[(int64_t)fi + (int64_t)fx * ((int64_t)fj + (int64_t)fy * fk)];
     }
     __syncthreads();
-    const Stride stv{nx, (int64_t)nx * ny};
-    int cur = 0;
-    for (int it = 0; it < sweeps; ++it) {
-        if (c < nc) {
-            const uint16_t cd = cs[c];
-            const float dgn = code_center(cd, ih2);
-            auto X = [&](int64_t n) { return xs[cur][n]; };
-            const float xc = xs[cur][c];
-            const float ax = code_apply(cd, dgn, c, stv, ih2, xc, X);
-            xs[1 - cur][c] = dgn > 0.0f ? xc + omega * (bs[c] - ax) / dgn : 0.0f;
+    const int tx = threadIdx.x % kRtx, ty = (threadIdx.x / kRtx) % kRty, tz = threadIdx.x / (kRtx * kRty);
+    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
+    if (I >= cx || J >= cy || K >= cz) return;
+    const int64_t c = (int64_t)I + (int64_t)cx * ((int64_t)J + (int64_t)cy * K);
+    const uint16_t cd = ccode[c];
+    float v = 0.0f;
+    if (cd & kFluidBit) {
+        // window coordinates of fine index 2I-1+t are 2*tx+t (clamped taps carry weight 0)
+        int ix[4], iy[4], iz[4];
+        float wx[4], wy[4], wz[4];
+        restrict_taps(I, fx, cx, ix, wx);
+        restrict_taps(J, fy, cy, iy, wy);
+        restrict_taps(K, fz, cz, iz, wz);
+        float acc = 0.0f;
+#pragma unroll
+        for (int tk = 0; tk < 4; ++tk) {
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                const float wjk = wy[tj] * wz[tk];
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti) acc += (wx[ti] * wjk) * win[2 * tz + tk][2 * ty + tj][2 * tx + ti];
+            }
+        }
+        v = acc * 0.125f;
+    }
+    bc[c] = v;
+    if (x1) {
+        const float ih2[3] = {cihx2, cihy2, cihz2};
+        x1[c] = mg_weight(cd, ih2) * v;
+    }
+}
+
+// x_f += P x_c on FLUID fine cells.
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, int cx, int cy, int cz,
+                                                       const uint16_t* __restrict__ fcode, float* __restrict__ xf,
+                                                       int fx, int fy, int fz, const SolverScalars* __restrict__ st) {
+    if (st->done) return;
+    const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nf = (int64_t)fx * fy * fz;
+    if (f >= nf) return;
+    if (!(fcode[f] & kFluidBit)) return;
+    int i, j, k;
+    cell_ijk(f, fx, fy, i, j, k);
+    xf[f] += prolong_at(i, j, k, cx, cy, cz, [&](int64_t n) { return xc[n]; });
+}
+
+// ---------------------------------------------------------------------------
+// Small levels: the rest of the V-cycle in one workgroup
+// ---------------------------------------------------------------------------
+struct TailArgs {
+    int levels;  // tail levels; the last is the coarsest
+    int n[kMaxTail][3];
+    float ih2[kMaxTail][3];
+    const uint16_t* code[kMaxTail];
+    int off[kMaxTail];  // offset of each level in the LDS arrays
+};
+
+// Tail level t in LDS: values at V + o, row codes at Cd + o.
+struct TailLevel {
+    int nx, ny, nz, nc, o;
+    Stride st;
+    float ih2[3];
+};
+
+__device__ inline TailLevel tail_level(const TailArgs& a, int t) {
+    TailLevel l;
+    l.nx = a.n[t][0];
+    l.ny = a.n[t][1];
+    l.nz = a.n[t][2];
+    l.nc = l.nx * l.ny * l.nz;
+    l.o = a.off[t];
+    l.st = stride_of(l.nx, l.ny, l.nz);
+    for (int q = 0; q < 3; ++q) l.ih2[q] = a.ih2[t][q];
+    return l;
+}
+
+// One Jacobi sweep: out = in + w (b - A in), or out = w b when `in` is null.
+__device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, const float* in, float* out,
+                                  const float* b, float omega) {
+    for (int c = threadIdx.x; c < l.nc; c += kTailThreads) {
+        const uint16_t cd = code[c];
+        const float dgn = code_center(cd, l.ih2);
+        const float w = dgn > 0.0f ? omega / dgn : 0.0f;
+        float xo;
+        if (in) {
+            const float xc = in[c];
+            xo = xc + w * (b[c] - code_apply(cd, dgn, c, l.st, l.ih2, xc, [&](int64_t n) { return in[n]; }));
+        } else {
+            xo = w * b[c];
+        }
+        out[c] = xo;
+    }
+    __syncthreads();
+}
+
+// Solves level T's right-hand side bT approximately with the V-cycle of the
+// tail levels; the result goes to xT.  Everything lives in LDS: three values
+// and the row code per cell of every tail level (at most 4608 cells, 63 KiB).
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bT,
+                                                          float* __restrict__ xT,
+                                                          const SolverScalars* __restrict__ st) {
+    if (st->done) return;
+    __shared__ float X[kTailLds], Tm[kTailLds], B[kTailLds];
+    __shared__ uint16_t Cd[kTailLds];
+    for (int t = 0; t < a.levels; ++t) {
+        const int nc = a.n[t][0] * a.n[t][1] * a.n[t][2];
+        for (int c = threadIdx.x; c < nc; c += kTailThreads) Cd[a.off[t] + c] = a.code[t][c];
+    }
+    const int n0 = a.n[0][0] * a.n[0][1] * a.n[0][2];
+    for (int c = threadIdx.x; c < n0; c += kTailThreads) B[c] = bT[c];
+    __syncthreads();
+    // down: the solution of every level ends in Tm after its pre-smoothing
+    for (int t = 0; t < a.levels - 1; ++t) {
+        const TailLevel l = tail_level(a, t), lc = tail_level(a, t + 1);
+        const int o = l.o;
+        const uint16_t* code = Cd + o;
+        tail_sweep(l, code, nullptr, X + o, B + o, kMgOmega);
+        tail_sweep(l, code, X + o, Tm + o, B + o, kMgOmega);
+        // residual into X (free now)
+        {
+            const float* xs = Tm + o;
+            for (int c = threadIdx.x; c < l.nc; c += kTailThreads) {
+                const uint16_t cd = code[c];
+                float rc = 0.0f;
+                if (cd & kFluidBit)
+                    rc = B[o + c] - code_apply(cd, code_center(cd, l.ih2), c, l.st, l.ih2, xs[c],
+                                               [&](int64_t n) { return xs[n]; });
+                X[o + c] = rc;
+            }
+            __syncthreads();
+        }
+        // restriction into the next level's B
+        {
+            const float* rs = X + o;
+            for (int c = threadIdx.x; c < lc.nc; c += kTailThreads) {
+                float v = 0.0f;
+                if (Cd[lc.o + c] & kFluidBit) {
+                    int I, J, K;
+                    cell_ijk(c, lc.nx, lc.ny, I, J, K);
+                    v = restrict_sum(I, J, K, l.nx, l.ny, l.nz, lc.nx, lc.ny, lc.nz,
+                                     [&](int64_t n) { return rs[n]; });
+                }
+                B[lc.o + c] = v;
+            }
+            __syncthreads();
+        }
+    }
+    // coarsest level: Jacobi sweeps from zero, ending in Tm
+    {
+        const TailLevel l = tail_level(a, a.levels - 1);
+        const int o = l.o;
+        const uint16_t* code = Cd + o;
+        static_assert(kCoarseSweeps % 2 == 0, "the sweeps ping-pong X -> Tm");
+        tail_sweep(l, code, nullptr, X + o, B + o, kOmegaCoarse);
+        tail_sweep(l, code, X + o, Tm + o, B + o, kOmegaCoarse);
+        for (int s = 2; s < kCoarseSweeps; s += 2) {
+            tail_sweep(l, code, Tm + o, X + o, B + o, kOmegaCoarse);
+            tail_sweep(l, code, X + o, Tm + o, B + o, kOmegaCoarse);
+        }
+    }
+    // up: prolongation, two sweeps Tm -> X -> Tm
+    for (int t = a.levels - 2; t >= 0; --t) {
+        const TailLevel l = tail_level(a, t), lc = tail_level(a, t + 1);
+        const int o = l.o;
+        const uint16_t* code = Cd + o;
+        const float* xc = Tm + lc.o;
+        for (int f = threadIdx.x; f < l.nc; f += kTailThreads) {
+            if (!(code[f] & kFluidBit)) continue;
+            int i, j, k;
+            cell_ijk(f, l.nx, l.ny, i, j, k);
+            Tm[o + f] += prolong_at(i, j, k, lc.nx, lc.ny, lc.nz, [&](int64_t n) { return xc[n]; });
         }
         __syncthreads();
-        cur = 1 - cur;
+        tail_sweep(l, code, Tm + o, X + o, B + o, kMgOmega);
+        tail_sweep(l, code, X + o, Tm + o, B + o, kMgOmega);
     }
-    if (c < nc) x[c] = xs[cur][c];
+    for (int c = threadIdx.x; c < n0; c += kTailThreads) xT[c] = Tm[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -263,8 +423,7 @@ static LevelArgs level_args(const MGLevel& l) {
     const SolverShape sh = solver_shape_n(l.cells);
     LevelArgs a;
     a.nc = l.cells;
-    a.sy = l.n[0];
-    a.sz = (int64_t)l.n[0] * l.n[1];
+    a.st = stride_of(l.n[0], l.n[1], l.n[2]);
     for (int q = 0; q < 3; ++q) a.ih2[q] = l.ih2[q];
     a.chunk = sh.chunk;
     a.padded = sh.blocks;
@@ -272,6 +431,18 @@ static LevelArgs level_args(const MGLevel& l) {
 }
 
 void mg_free(fe_scene* s);
+
+// First level of the tail: the first level below the finest with at most
+// kTailMax cells whose tail fits the LDS arrays and the level table.
+static int tail_start(const std::vector<MGLevel>& lv) {
+    const int L = (int)lv.size();
+    for (int q = 1; q < L; ++q) {
+        int64_t sum = 0;
+        for (int t = q; t < L; ++t) sum += lv[t].cells;
+        if (lv[q].cells <= kTailMax && sum <= kTailLds && L - q <= kMaxTail) return q;
+    }
+    return L - 1;
+}
 
 static hipError_t mg_alloc(fe_scene* s) {
     const Geom& g = s->g;
@@ -283,7 +454,8 @@ static hipError_t mg_alloc(fe_scene* s) {
     }
     l0.cells = cell_count(g);
     lv.push_back(l0);
-    while (lv.back().cells > kCoarseMax) {
+    // every axis has at least 2 cells, so the first coarsening always shrinks
+    while (lv.back().cells > kCoarseMax || lv.size() < 2) {
         const MGLevel& f = lv.back();
         MGLevel c;
         for (int a = 0; a < 3; ++a) {
@@ -291,11 +463,11 @@ static hipError_t mg_alloc(fe_scene* s) {
             c.ih2[a] = f.ih2[a] * 0.25f;  // twice the spacing
         }
         c.cells = (int64_t)c.n[0] * c.n[1] * c.n[2];
-        if (c.cells == f.cells) break;  // cannot coarsen further
+        if (c.cells == f.cells) break;
         c.owns = true;
         lv.push_back(c);
     }
-    if (lv.back().cells > kCoarseMax) return hipErrorInvalidValue;
+    if (lv.size() < 2 || lv.back().cells > kCoarseMax) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
     for (size_t q = 0; q < lv.size(); ++q) {
         MGLevel& l = lv[q];
@@ -310,6 +482,7 @@ static hipError_t mg_alloc(fe_scene* s) {
         if ((e = hipMalloc((void**)&l.r, n * sizeof(float))) != hipSuccess) break;
     }
     s->mg = lv;
+    s->mg_tail = tail_start(lv);
     if (e != hipSuccess) mg_free(s);  // leaves the hierarchy empty: the next solve retries
     return e;
 }
@@ -346,59 +519,70 @@ hipError_t mg_setup(fe_scene* s) {
     return hipGetLastError();
 }
 
-// z = M^-1 r with one V-cycle; leaves partials of r.z in prz.  Every kernel
-// returns at once when `st` says the solve has stopped.
-hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz) {
+float* mg_first_sweep_target(fe_scene* s) { return s->mg.empty() ? nullptr : s->mg[0].x; }
+
+// z = M^-1 r with one V-cycle.  The caller has already written level 0's
+// first sweep (x1 = w r) into mg[0].x.  Leaves partials of r.z in prz.  Every
+// kernel returns at once when `st` says the solve has stopped.
+hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz, int tag) {
     std::vector<MGLevel>& lv = s->mg;
     const int L = (int)lv.size();
+    const int T = s->mg_tail;
     lv[0].b = const_cast<float*>(r);
-    std::vector<float*> cur(L);
-    for (int q = 0; q < L - 1; ++q) {
+    for (int q = 0; q < T; ++q) {
         MGLevel& l = lv[q];
-        const LevelArgs a = level_args(l);
-        // pre-smoothing from zero
-        hipLaunchKernelGGL(k_mg_smooth, dim3((unsigned)a.padded), dim3(kBlock), 0, s->stream, a,
-                           (const uint16_t*)l.code, (const float*)l.b, (const float*)nullptr, l.x, kOmega, st,
-                           (double*)nullptr);
-        float* xa = l.x;
-        float* xb = l.x2;
-        for (int t = 1; t < kPreSweeps; ++t) {
-            hipLaunchKernelGGL(k_mg_smooth, dim3((unsigned)a.padded), dim3(kBlock), 0, s->stream, a,
-                               (const uint16_t*)l.code, (const float*)l.b, (const float*)xa, xb, kOmega, st,
-                               (double*)nullptr);
-            std::swap(xa, xb);
-        }
-        cur[q] = xa;
-        hipLaunchKernelGGL(k_mg_residual, dim3((unsigned)a.padded), dim3(kBlock), 0, s->stream, a,
-                           (const uint16_t*)l.code, (const float*)l.b, (const float*)xa, l.r, st);
         MGLevel& c = lv[q + 1];
-        hipLaunchKernelGGL(k_mg_restrict, grid1(c.cells), dim3(kBlock), 0, s->stream, (const float*)l.r, l.n[0],
-                           l.n[1], l.n[2], (const uint16_t*)c.code, c.b, c.n[0], c.n[1], c.n[2], st);
+        const LevelArgs a = level_args(l);
+        const dim3 grid((unsigned)a.padded);
+        if (q == 0)
+            launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
+                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+        else
+            hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+        hipLaunchKernelGGL(k_mg_residual, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                           (const float*)l.b, (const float*)l.x2, l.r, st);
+        const dim3 rgrid((unsigned)((c.n[0] + kRtx - 1) / kRtx), (unsigned)((c.n[1] + kRty - 1) / kRty),
+                         (unsigned)((c.n[2] + kRtz - 1) / kRtz));
+        hipLaunchKernelGGL(k_mg_restrict, rgrid, dim3(kBlock), 0, s->stream, (const float*)l.r, l.n[0],
+                           l.n[1], l.n[2], (const uint16_t*)c.code, c.b, q + 1 < T ? c.x : (float*)nullptr, c.n[0],
+                           c.n[1], c.n[2], c.ih2[0], c.ih2[1], c.ih2[2], st);
     }
     {
-        MGLevel& c = lv[L - 1];
-        hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kCoarseMax), 0, s->stream, c.n[0], c.n[1], c.n[2],
-                           (const uint16_t*)c.code, (const float*)c.b, c.x, c.ih2[0], c.ih2[1], c.ih2[2],
-                           kOmegaCoarse, kCoarseSweeps, st);
-        cur[L - 1] = c.x;
+        TailArgs ta{};
+        ta.levels = L - T;
+        int off = 0;
+        for (int t = 0; t < ta.levels; ++t) {
+            const MGLevel& l = lv[T + t];
+            for (int q = 0; q < 3; ++q) {
+                ta.n[t][q] = l.n[q];
+                ta.ih2[t][q] = l.ih2[q];
+            }
+            ta.code[t] = l.code;
+            ta.off[t] = off;
+            off += (int)l.cells;
+        }
+        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s->stream, ta, (const float*)lv[T].b,
+                           lv[T].x2, st);
     }
-    for (int q = L - 2; q >= 0; --q) {
+    for (int q = T - 1; q >= 0; --q) {
         MGLevel& l = lv[q];
         const MGLevel& c = lv[q + 1];
         const LevelArgs a = level_args(l);
-        float* xa = cur[q];
-        float* xb = xa == l.x ? l.x2 : l.x;
-        hipLaunchKernelGGL(k_mg_prolong, grid1(l.cells), dim3(kBlock), 0, s->stream, (const float*)cur[q + 1], c.n[0],
-                           c.n[1], c.n[2], (const uint16_t*)l.code, xa, l.n[0], l.n[1], l.n[2], st);
-        for (int t = 0; t < kPreSweeps; ++t) {
-            const bool last0 = (q == 0 && t == kPreSweeps - 1);
-            float* out = last0 ? z : xb;
-            hipLaunchKernelGGL(k_mg_smooth, dim3((unsigned)a.padded), dim3(kBlock), 0, s->stream, a,
-                               (const uint16_t*)l.code, (const float*)l.b, (const float*)xa, out, kOmega, st,
-                               last0 ? prz : (double*)nullptr);
-            if (!last0) std::swap(xa, xb);
+        const dim3 grid((unsigned)a.padded);
+        hipLaunchKernelGGL(k_mg_prolong, grid1(l.cells), dim3(kBlock), 0, s->stream, (const float*)c.x2, c.n[0],
+                           c.n[1], c.n[2], (const uint16_t*)l.code, l.x2, l.n[0], l.n[1], l.n[2], st);
+        if (q == 0) {
+            launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
+                         (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr);
+            launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
+                         (const float*)l.b, (const float*)l.x, z, st, prz);
+        } else {
+            hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                               (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr);
+            hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
         }
-        cur[q] = xa;
     }
     return hipGetLastError();
 }

@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Geom g, const uint16_t* __res
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
-    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     auto X = [&](int64_t n) { return xo[n]; };
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         if (done) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_residual(Geom g, const uint16
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const double ih2d[3] = {ihx2, ihy2, ihz2};
-    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     double v[1] = {0.0};
     if (!sc->done) {
         for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
@@ -174,11 +174,17 @@ __global__ __launch_bounds__(kBlock) void k_zero_f64(double* __restrict__ x, int
 //           partials of r.z and r.r
 // With the relative rule a stop is only accepted on a freshly replaced
 // residual, so the decision is taken on b - A x, not on the recurrence.
+// The FP32 recurrence drifts from b - A x by about eps32 * ||A|| * sum|alpha d|,
+// which grows with the residual at the last recomputation: the residual is
+// also recomputed whenever ||r|| has fallen 100x below that value ("reliable
+// updates", Sleijpen and van der Vorst 1996), so the drift stays well below
+// ||r|| and beta stays meaningful across recomputations.
 // ---------------------------------------------------------------------------
+constexpr double kReliableDrop2 = 1e-4;  // (100x in ||r||)^2
 __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __restrict__ code,
                                                     const float* __restrict__ b, const double* __restrict__ x,
                                                     float* __restrict__ r, float* __restrict__ z,
-                                                    float* __restrict__ d0, SolverScalars* sc,
+                                                    float* __restrict__ d0, float* __restrict__ mgx, SolverScalars* sc,
                                                     const double* __restrict__ pbb, double* __restrict__ pout,
                                                     int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
                                                     int64_t padded) {
@@ -189,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
-    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     const int pre = sc->precond;  // 0 none, 1 diagonal, 2 multigrid (z and r.z come later)
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
@@ -202,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
+        if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
         d0[c] = 0.0f;
         if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
@@ -233,15 +240,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         t[0] = tz[0];
     }
     const int it_prev = in->it;
-    double beta;
+    double beta, rr_ref;
     int it, trig, done, conv, repl;
     if (it_prev < 0) {  // first iteration: partials come from k_cg_init
         it = 0;
         beta = 0.0;
         trig = 0;
+        rr_ref = t[1];
         conv = stop_rule(in, in->bnorm2, t[0], t[1]) ? 1 : 0;
         done = (conv || in->max_it <= 0) ? 1 : 0;
     } else {
+        rr_ref = in->replace ? t[1] : in->rr_ref;
         const double sold = in->sigma;
         it = it_prev + 1;
         trig = t[0] > sold ? 1 : 0;
@@ -254,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         conv = stop ? 1 : 0;
         done = (stop || it >= in->max_it) ? 1 : 0;
     }
-    repl = (trig || (it % 50 == 0 && it > 0)) ? 1 : 0;
+    repl = (trig || (it % 50 == 0 && it > 0) || t[1] < kReliableDrop2 * rr_ref) ? 1 : 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         SolverScalars o = *in;
         o.sigma_old = in->sigma;
@@ -266,6 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         o.converged = conv;
         o.done = done;
         o.replace = repl;
+        o.rr_ref = rr_ref;
         *out = o;
     }
     if (done) return;
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
-    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     const float fb = (float)beta;
     auto D = [&](int64_t n) { return fb * dold[n] + z[n]; };  // AXpY(beta, d, s, &d)
     double v[1] = {0.0};
@@ -292,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
                                                       const float* __restrict__ b, const double* __restrict__ xo,
                                                       double* __restrict__ xn, const float* __restrict__ d,
                                                       const float* __restrict__ q, float* __restrict__ r,
-                                                      float* __restrict__ z, SolverScalars* sc,
+                                                      float* __restrict__ z, float* __restrict__ mgx, SolverScalars* sc,
                                                       const double* __restrict__ pin, double* __restrict__ pout,
                                                       int nparts, float ihx2, float ihy2, float ihz2,
                                                       int64_t chunk, int64_t padded) {
@@ -313,14 +323,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
     const int pre = sc->precond;
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
-    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     const float fa = (float)alpha;
     auto X = [&](int64_t n) { return alpha * (double)d[n] + xo[n]; };
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const double xc = X(c);
         xn[c] = xc;
-        const uint16_t cd = (replace || pre == 1) ? code[c] : 0;
+        const uint16_t cd = (replace || pre) ? code[c] : 0;
         const float rc = replace ? (float)((double)b[c] - code_apply64(cd, c, st, ih2d, xc, X)) : -fa * q[c] + r[c];
         r[c] = rc;
         float zc = rc;
@@ -328,6 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
+        if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
         if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
     }
@@ -343,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_final_residual(Geom g, const uint16_
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const double ih2[3] = {ihx2, ihy2, ihz2};
-    const Stride st{g.n[0], (int64_t)g.n[0] * g.n[1]};
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     double v[2] = {0.0, 0.0};
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const double ax = code_apply64(code[c], c, st, ih2, x[c], [&](int64_t n) { return x[n]; });
@@ -467,9 +478,10 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
 
     // CG / PCG
     if (mg && (e = mg_setup(s)) != hipSuccess) return e;
+    float* mgx = mg ? mg_first_sweep_target(s) : nullptr;
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
-                       s->r, s->z, s->d[0], s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
-    if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ)) != hipSuccess) return e;
+                       s->r, s->z, s->d[0], mgx, s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+    if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ, -1)) != hipSuccess) return e;
     double* xa = s->p;
     double* xb = s->p2;
     int dcur = 0;
@@ -487,10 +499,10 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         dcur = 1 - dcur;
         launch_timed(s, FE_KERNEL_CG_UPDATE, kk, k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                      (const uint16_t*)s->code, (const float*)s->b, (const double*)xa, xb, (const float*)s->d[dcur],
-                     (const float*)s->q, s->r, s->z, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
+                     (const float*)s->q, s->r, s->z, mgx, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
                      ih2[1], ih2[2], sh.chunk, sh.blocks);
         std::swap(xa, xb);
-        if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ);
+        if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk);
     };
     for (;;) {
         const int nb = std::min(batch, d->max_iterations + 1 - k);

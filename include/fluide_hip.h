@@ -49,7 +49,9 @@ enum fe_marker { FE_AIR = 0, FE_FLUID = 1, FE_BOUNDRY = 2 };
  *   stop when ||b-Ax||_2 < tolerance, checked every check_interval sweeps.
  * CG: ConjugateGradientSolver (src/linear_system/conjugate_gradient_solver.cpp:14-55),
  *   identity preconditioner, residual replacement every 50 iterations or when
- *   sigma grows; stop on sigma <= tol^2, or ||r||/||b|| <= tol when relative.
+ *   sigma grows (and, for the FP32 vectors, whenever ||r|| fell 100x since the
+ *   last replacement); stop on sigma <= tol^2, or ||r||/||b|| <= tol when
+ *   relative, accepted only on a replaced residual.
  * PCG_JACOBI: CG with the diagonal preconditioner (no reference counterpart).
  * MGPCG: CG preconditioned by one geometric multigrid V-cycle per iteration
  *   (no reference counterpart; same stopping rules and final residual as CG).
@@ -200,7 +202,8 @@ enum fe_kernel_id {
     FE_KERNEL_P2G = 3,        /* per-face gather (3 launches per substep, one per component) */
     FE_KERNEL_G2P_MOVE = 4,   /* fused G2P + advection */
     FE_KERNEL_SDF = 5,        /* fluid SDF */
-    FE_KERNEL_JACOBI = 6      /* one Jacobi sweep */
+    FE_KERNEL_JACOBI = 6,     /* one Jacobi sweep */
+    FE_KERNEL_MG_SMOOTH = 7   /* multigrid smoothing sweep on the finest level (3 per V-cycle) */
 };
 int fe_kernel_timer(fe_scene* scene, int32_t kernel_id);
 /* Total device ms and launch count since the last reset (reset != 0 clears). */

@@ -206,7 +206,7 @@ ORC_SOLVER = {capi.SOLVER_CG: orc.CG, capi.SOLVER_PCG_JACOBI: orc.PCG_JACOBI,
               capi.SOLVER_MGPCG: orc.CG}  # the oracle's CG gives the converged answer for MGPCG
 
 
-@pytest.mark.parametrize("shape", [32, (40, 24, 17)])
+@pytest.mark.parametrize("shape", [32, (40, 24, 17), (6, 5, 4)])
 @pytest.mark.parametrize("kind", [capi.SOLVER_CG, capi.SOLVER_PCG_JACOBI, capi.SOLVER_MGPCG])
 def test_pressure_cg_reaches_tolerance(ctx, kind, shape):
     print(f"[gpu] pressure cg kind={kind} shape={shape}", flush=True)
