@@ -68,21 +68,15 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="also run one plain device copy of MIB MiB (torch), the reference kernel that "
-                         "scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
+                    help="also run one plain device copy of MIB MiB (k_copy_probe), the reference kernel "
+                         "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     return ap.parse_args()
 
 
-def copy_calibration(mib):
-    import torch
-
-    n = mib * 1024 * 1024 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"bench: copy calibration {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+def copy_calibration(ctx, mib):
+    ms = ctx.copy_probe(mib * 1024 * 1024)
+    print(f"bench: copy calibration k_copy_probe {mib} MiB read + {mib} MiB written, {ms:.3f} ms",
+          file=sys.stderr, flush=True)
 
 
 def main():
@@ -148,7 +142,7 @@ def main():
                 "bytes_per_launch": bpl}
 
     if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
+        copy_calibration(ctx, args.copy_calibration)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, capi, desc, dt, args)

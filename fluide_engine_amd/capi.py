@@ -27,7 +27,7 @@ EXPORTED = [
     "fe_scene_sync", "fe_particles_upload", "fe_particles_download", "fe_particles_count", "fe_substep",
     "fe_advance_frame", "fe_cfl", "fe_run_stage", "fe_grid_download", "fe_grid_upload", "fe_grid_bytes",
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
-    "fe_kernel_timer_read", "fe_last_solve_stats",
+    "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -100,6 +100,7 @@ def load():
     L.fe_kernel_timer.argtypes = [vp, C.c_int32]
     L.fe_last_solve_stats.argtypes = [vp, C.POINTER(SolveStats)]
     L.fe_kernel_timer_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
+    L.fe_copy_probe.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
     L.fe_version.restype = C.c_char_p
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name in ("fe_last_error", "fe_version") else C.c_int
@@ -185,6 +186,12 @@ class Context:
     def check(self, rc, what):
         if rc != 0:
             raise FluideError(f"{what} failed ({rc}): {self.L.fe_last_error(self._h).decode()}")
+
+    def copy_probe(self, nbytes):
+        """One plain device copy of nbytes (the byte-counter calibration kernel); returns ms."""
+        ms = C.c_double()
+        self.check(self.L.fe_copy_probe(self._h, int(nbytes), C.byref(ms)), "fe_copy_probe")
+        return ms.value
 
     def close(self):
         if getattr(self, "_h", None):

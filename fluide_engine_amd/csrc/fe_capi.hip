@@ -205,6 +205,41 @@ extern "C" {
 
 const char* fe_version(void) { return "fluide-hip 0.1 (gfx950)"; }
 
+// Plain streaming copy (16 B per lane and iteration), the reference kernel for
+// scaling hardware byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) b[i] = a[i];
+}
+
+int fe_copy_probe(fe_ctx* ctx, int64_t bytes, double* ms) {
+    if (!ctx || bytes <= 0 || bytes % 16 != 0) return set_err(ctx, FE_E_INVALID, "bytes must be a positive multiple of 16");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void**)&a, bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&b, bytes);
+    if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) {
+        const int64_t n = bytes / 16;
+        const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+        e = hipEventRecord(e0, nullptr);
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, (const float4*)a, b, n);
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (ms) *ms = t;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return hip_fail(ctx, e, "fe_copy_probe");
+    return FE_OK;
+}
+
 int fe_ctx_create(int device, fe_ctx** out) {
     if (!out) return set_err(nullptr, FE_E_INVALID, "out is NULL");
     *out = nullptr;

@@ -33,10 +33,13 @@ inline SolverShape solver_shape_n(int64_t nc) {
 // ---------------------------------------------------------------------------
 constexpr uint16_t kFluidBit = 1u << 12;
 
-struct Stride {
-    int64_t sy, sz;  // +-1 in j, k
-    int64_t last;    // cells - 1
+template <typename I>
+struct StrideT {
+    I sy, sz;  // +-1 in j, k
+    I last;    // cells - 1
 };
+using Stride = StrideT<int64_t>;
+using Stride32 = StrideT<int>;  // grids held in LDS
 
 __host__ __device__ inline Stride stride_of(int64_t nx, int64_t ny, int64_t nz) {
     return Stride{nx, nx * ny, nx * ny * nz - 1};
@@ -72,8 +75,8 @@ __device__ inline double code_center64(uint16_t code, const double ih2[3]) {
 // neighbour values are fetched (at indices clamped into the grid) before the
 // row code is consulted, so the loads never wait on it; an uncoupled
 // neighbour contributes fma(0, v, acc) = acc (every vector holds finite values).
-template <typename F>
-__device__ inline float code_apply(uint16_t code, float center, int64_t c, const Stride& st, const float ih2[3],
+template <typename I, typename F>
+__device__ inline float code_apply(uint16_t code, float center, I c, const StrideT<I>& st, const float ih2[3],
                                    float vc, F v) {
     const float vxm = v(c > 0 ? c - 1 : 0), vxp = v(c < st.last ? c + 1 : st.last);
     const float vym = v(c >= st.sy ? c - st.sy : 0), vyp = v(c + st.sy <= st.last ? c + st.sy : st.last);
@@ -106,9 +109,9 @@ __device__ inline double code_apply64(uint16_t code, int64_t c, const Stride& st
 // cell with no diagonal, i.e. walled in by BOUNDRY cells on every side).
 constexpr float kMgOmega = 2.0f / 3.0f;
 
-__device__ inline float mg_weight(uint16_t code, const float ih2[3]) {
+__device__ inline float mg_weight(uint16_t code, const float ih2[3], float omega = kMgOmega) {
     const float dg = code_center(code, ih2);
-    return dg > 0.0f ? kMgOmega / dg : 0.0f;
+    return dg > 0.0f ? omega * __builtin_amdgcn_rcpf(dg) : 0.0f;  // hardware reciprocal: a smoother weight
 }
 
 // ---------------------------------------------------------------------------

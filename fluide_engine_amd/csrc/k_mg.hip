@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         const uint16_t cd = code[c];
         const float dgn = code_center(cd, L.ih2);
-        const float w = dgn > 0.0f ? kMgOmega / dgn : 0.0f;
+        const float w = mg_weight(cd, L.ih2);
         const float xc = xin[c];
         const float bc = b[c];
         const float xo = xc + w * (bc - code_apply(cd, dgn, c, stv, L.ih2, xc, X));
@@ -227,6 +227,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
     __shared__ float win[kRwz][kRwy][kRwx];
     const int I0 = blockIdx.x * kRtx, J0 = blockIdx.y * kRty, K0 = blockIdx.z * kRtz;
     const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1, fk0 = 2 * K0 - 1;
+    const int tx = threadIdx.x % kRtx, ty = (threadIdx.x / kRtx) % kRty, tz = threadIdx.x / (kRtx * kRty);
+    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
+    const bool inside = I < cx && J < cy && K < cz;
+    const int64_t c = (int64_t)I + (int64_t)cx * ((int64_t)J + (int64_t)cy * K);
+    const uint16_t cd = inside ? ccode[c] : (uint16_t)0;  // issued with the window loads
     for (int e = threadIdx.x; e < kRwx * kRwy * kRwz; e += kBlock) {
         const int wi = e % kRwx, wj = (e / kRwx) % kRwy, wk = e / (kRwx * kRwy);
         const int fi = min(max(fi0 + wi, 0), fx - 1);
@@ -235,11 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
         win[wk][wj][wi] = rf[(int64_t)fi + (int64_t)fx * ((int64_t)fj + (int64_t)fy * fk)];
     }
     __syncthreads();
-    const int tx = threadIdx.x % kRtx, ty = (threadIdx.x / kRtx) % kRty, tz = threadIdx.x / (kRtx * kRty);
-    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
-    if (I >= cx || J >= cy || K >= cz) return;
-    const int64_t c = (int64_t)I + (int64_t)cx * ((int64_t)J + (int64_t)cy * K);
-    const uint16_t cd = ccode[c];
+    if (!inside) return;
     float v = 0.0f;
     if (cd & kFluidBit) {
         // window coordinates of fine index 2I-1+t are 2*tx+t (clamped taps carry weight 0)
@@ -295,7 +296,7 @@ struct TailArgs {
 // Tail level t in LDS: values at V + o, row codes at Cd + o.
 struct TailLevel {
     int nx, ny, nz, nc, o;
-    Stride st;
+    Stride32 st;
     float ih2[3];
 };
 
@@ -306,7 +307,7 @@ __device__ inline TailLevel tail_level(const TailArgs& a, int t) {
     l.nz = a.n[t][2];
     l.nc = l.nx * l.ny * l.nz;
     l.o = a.off[t];
-    l.st = stride_of(l.nx, l.ny, l.nz);
+    l.st = Stride32{l.nx, l.nx * l.ny, l.nc - 1};
     for (int q = 0; q < 3; ++q) l.ih2[q] = a.ih2[t][q];
     return l;
 }
@@ -317,11 +318,11 @@ __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, cons
     for (int c = threadIdx.x; c < l.nc; c += kTailThreads) {
         const uint16_t cd = code[c];
         const float dgn = code_center(cd, l.ih2);
-        const float w = dgn > 0.0f ? omega / dgn : 0.0f;
+        const float w = mg_weight(cd, l.ih2, omega);
         float xo;
         if (in) {
             const float xc = in[c];
-            xo = xc + w * (b[c] - code_apply(cd, dgn, c, l.st, l.ih2, xc, [&](int64_t n) { return in[n]; }));
+            xo = xc + w * (b[c] - code_apply(cd, dgn, c, l.st, l.ih2, xc, [&](int n) { return in[n]; }));
         } else {
             xo = w * b[c];
         }
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
                 float rc = 0.0f;
                 if (cd & kFluidBit)
                     rc = B[o + c] - code_apply(cd, code_center(cd, l.ih2), c, l.st, l.ih2, xs[c],
-                                               [&](int64_t n) { return xs[n]; });
+                                               [&](int n) { return xs[n]; });
                 X[o + c] = rc;
             }
             __syncthreads();

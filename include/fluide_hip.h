@@ -209,6 +209,12 @@ int fe_kernel_timer(fe_scene* scene, int32_t kernel_id);
 /* Total device ms and launch count since the last reset (reset != 0 clears). */
 int fe_kernel_timer_read(fe_scene* scene, double* total_ms, int64_t* launches, int32_t reset);
 
+/* One plain device copy of `bytes` bytes (a multiple of 16) between two
+ * scratch buffers the call allocates and frees; *ms = its duration.  No
+ * reference counterpart: the known-size kernel that hardware byte counters
+ * (FETCH_SIZE, WRITE_SIZE) are scaled by. */
+int fe_copy_probe(fe_ctx* ctx, int64_t bytes, double* ms);
+
 /* Library version string. */
 const char* fe_version(void);
 
