@@ -15,17 +15,53 @@ __global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t* __restrict__ a, in
     if (i < n) a[i] = v;
 }
 
+// Runs of equal keys in consecutive lanes of a wavefront.  Particles arrive
+// sorted by cell, so a wavefront holds few runs: one atomic per run instead of
+// one per particle keeps same-address atomics from serialising.  The active
+// lanes of a wavefront are a prefix (one particle per lane, in index order);
+// every lane, active or not, must call these.
+struct KeyRun {
+    bool head;  // first lane of its run
+    int first;  // lane of this lane's run head
+    int len;    // run length (valid on the head lane)
+};
+
+__device__ inline KeyRun key_run(uint32_t k, bool active) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t kp = __shfl_up(k, 1, 64);
+    const bool head = active && (lane == 0 || kp != k);
+    const uint64_t heads = __ballot(head);
+    const uint64_t act = __ballot(active);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes 0..lane
+    KeyRun r;
+    r.head = head;
+    r.first = 63 - __clzll(heads & upto);
+    const uint64_t later = heads & ~upto;
+    const int end = later ? __ffsll((long long)later) - 1 : 64 - __clzll(act);
+    r.len = end - lane;
+    return r;
+}
+
+// Adds 1 per active lane to count[k] with one atomic per run.
+__device__ inline void count_runs(int32_t* __restrict__ count, uint32_t k, bool active) {
+    const KeyRun r = key_run(k, active);
+    if (r.head) atomicAdd(&count[k], r.len);
+}
+
 // Bin key of each particle + histogram.
 __global__ __launch_bounds__(kBlock) void k_keys(Geom g, const float* __restrict__ X, int64_t cap, int64_t np,
                                                  uint32_t* __restrict__ key, int32_t* __restrict__ count) {
-    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= np) return;
-    const int cx = cell_axis(g, 0, X[p]);
-    const int cy = cell_axis(g, 1, X[cap + p]);
-    const int cz = cell_axis(g, 2, X[2 * cap + p]);
-    const uint32_t k = (uint32_t)cx + (uint32_t)g.n[0] * ((uint32_t)cy + (uint32_t)g.n[1] * (uint32_t)cz);
-    key[p] = k;
-    if (count) atomicAdd(&count[k], 1);
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = p < np;
+    uint32_t k = 0;
+    if (active) {
+        const int cx = cell_axis(g, 0, X[p]);
+        const int cy = cell_axis(g, 1, X[cap + p]);
+        const int cz = cell_axis(g, 2, X[2 * cap + p]);
+        k = (uint32_t)cx + (uint32_t)g.n[0] * ((uint32_t)cy + (uint32_t)g.n[1] * (uint32_t)cz);
+        key[p] = k;
+    }
+    if (count) count_runs(count, k, active);
 }
 
 // Exclusive scan of int32 counts, three passes of 1024 items per block.
@@ -102,14 +138,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t* __restrict
     if (base + kScanItems >= n && base < n) out[n] = ex;  // total after the last item
 }
 
+// Slots in the bins: the head of each run reserves the run's slots with one
+// atomic and the run's lanes take them in lane order.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key, int64_t np,
                                                     const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
                                                     int32_t* __restrict__ order) {
-    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p >= np) return;
-    const uint32_t k = key[p];
-    const int slot = start[k] + atomicAdd(&cursor[k], 1);
-    order[slot] = (int32_t)p;
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = p < np;
+    const uint32_t k = active ? key[p] : 0u;
+    const KeyRun r = key_run(k, active);
+    int base = 0;
+    if (r.head) base = start[k] + atomicAdd(&cursor[k], r.len);
+    base = __shfl(base, r.first, 64);
+    if (active) order[base + ((int)(threadIdx.x & 63) - r.first)] = (int32_t)p;
 }
 
 // Atomic slots give each bin its particles in arbitrary order; sorting every
@@ -204,43 +245,57 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
     }
     float num = 0.0f, den = 0.0f;
     int marked = 0;
+    // one candidate particle, in ascending particle order
+    auto consider = [&](int s, float px, float py, float pz, float val) {
+        int i0;
+        float fr;
+        stencil_axis(g, C, 0, px, S[0], i0, fr);
+        float wx;
+        if (I == i0) wx = 1.0f - fr;
+        else if (I == i0 + 1) wx = fr;
+        else return;
+        stencil_axis(g, C, 1, py, S[1], i0, fr);
+        float wy;
+        if (J == i0) wy = 1.0f - fr;
+        else if (J == i0 + 1) wy = fr;
+        else return;
+        stencil_axis(g, C, 2, pz, S[2], i0, fr);
+        float wz;
+        if (K == i0) wz = 1.0f - fr;
+        else if (K == i0 + 1) wz = fr;
+        else return;
+        marked = 1;
+        const float w = (wx * wy) * wz;
+        if (METHOD == FE_APIC) {
+            float pc[3] = {px, py, pz};
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
+            const float* cr = Cm + (int64_t)(3 * C) * cap + s;
+            val += cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]);
+        }
+        num += val * w;
+        den += w;
+    };
+    constexpr int kBatch = 4;  // candidates whose loads are issued together
     for (int kc = lo[2]; kc <= hi[2]; ++kc) {
         for (int jc = lo[1]; jc <= hi[1]; ++jc) {
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
-            for (int s = s0; s < s1; ++s) {
-                const float px = X[s], py = X[cap + s], pz = X[2 * cap + s];
-                int i0;
-                float fr;
-                stencil_axis(g, C, 0, px, S[0], i0, fr);
-                float wx;
-                if (I == i0) wx = 1.0f - fr;
-                else if (I == i0 + 1) wx = fr;
-                else continue;
-                stencil_axis(g, C, 1, py, S[1], i0, fr);
-                float wy;
-                if (J == i0) wy = 1.0f - fr;
-                else if (J == i0 + 1) wy = fr;
-                else continue;
-                stencil_axis(g, C, 2, pz, S[2], i0, fr);
-                float wz;
-                if (K == i0) wz = 1.0f - fr;
-                else if (K == i0 + 1) wz = fr;
-                else continue;
-                marked = 1;
-                const float w = (wx * wy) * wz;
-                float val = V[C * cap + s];
-                if (METHOD == FE_APIC) {
-                    float pc[3] = {px, py, pz};
+            int s = s0;
+            for (; s + kBatch <= s1; s += kBatch) {
+                float px[kBatch], py[kBatch], pz[kBatch], pv[kBatch];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a)
-                        if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
-                    const float* cr = Cm + (int64_t)(3 * C) * cap + s;
-                    val += cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]);
+                for (int q = 0; q < kBatch; ++q) {
+                    px[q] = X[s + q];
+                    py[q] = X[cap + s + q];
+                    pz[q] = X[2 * cap + s + q];
+                    pv[q] = V[C * cap + s + q];
                 }
-                num += val * w;
-                den += w;
+#pragma unroll
+                for (int q = 0; q < kBatch; ++q) consider(s + q, px[q], py[q], pz[q], pv[q]);
             }
+            for (; s < s1; ++s) consider(s, X[s], X[cap + s], X[2 * cap + s], V[C * cap + s]);
         }
     }
     const float res = den > 0.0f ? num / den : 0.0f;
@@ -304,88 +359,91 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
                                                      int do_move, uint32_t* __restrict__ key,
                                                      int32_t* __restrict__ count) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= np) return;
-    float x = P.x[s], y = P.x[cap + s], z = P.x[2 * cap + s];
-    float v[3] = {P.v[s], P.v[cap + s], P.v[2 * cap + s]};
-    if (do_g2p) {
-        float pic[3];
+    const bool active = s < np;
+    uint32_t k = 0;
+    if (active) {  // inactive lanes stay for the wavefront-wide key runs at the end
+        float x = P.x[s], y = P.x[cap + s], z = P.x[2 * cap + s];
+        float v[3] = {P.v[s], P.v[cap + s], P.v[2 * cap + s]};
+        if (do_g2p) {
+            float pic[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pic[c] = sample_face(grid.a[c], g, c, x, y, z);
-        if (METHOD == FE_PIC) {
+            for (int c = 0; c < 3; ++c) pic[c] = sample_face(grid.a[c], g, c, x, y, z);
+            if (METHOD == FE_PIC) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = pic[c];
-        } else if (METHOD == FE_FLIP) {
+                for (int c = 0; c < 3; ++c) v[c] = pic[c];
+            } else if (METHOD == FE_FLIP) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float fl = v[c] + sample_delta(grid.a[c], snapset.a[c], g, c, x, y, z);
-                if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
-                v[c] = fl;
+                for (int c = 0; c < 3; ++c) {
+                    float fl = v[c] + sample_delta(grid.a[c], snapset.a[c], g, c, x, y, z);
+                    if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
+                    v[c] = fl;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[c] = pic[c];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    float pc[3] = {x, y, z};
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        if (a == c) continue;
+                        const float lo = g.o[a] + 0.5f * g.h[a], hi = g.o[a] + g.L[a] - 0.5f * g.h[a];
+                        pc[a] = pc[a] < lo ? lo : (pc[a] > hi ? hi : pc[a]);
+                    }
+                    float cr[3];
+                    grad_contract(grid.a[c], g, c, pc, cr);
+                    P.C[(int64_t)(3 * c) * cap + s] = cr[0];
+                    P.C[(int64_t)(3 * c + 1) * cap + s] = cr[1];
+                    P.C[(int64_t)(3 * c + 2) * cap + s] = cr[2];
+                }
             }
-        } else {
+        }
+        if (do_move) {
+            const float sdt = dt / (float)nsub;
+            float p0[3] = {x, y, z}, p1[3] = {x, y, z};
+            for (int t = 0; t < nsub; ++t) {
+                float v0[3], mid[3], mv[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c] = pic[c];
+                for (int c = 0; c < 3; ++c) v0[c] = sample_face(grid.a[c], g, c, p0[0], p0[1], p0[2]);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float pc[3] = {x, y, z};
+                for (int a = 0; a < 3; ++a) mid[a] = p0[a] + 0.5f * sdt * v0[a];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) mv[c] = sample_face(grid.a[c], g, c, mid[0], mid[1], mid[2]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    if (a == c) continue;
-                    const float lo = g.o[a] + 0.5f * g.h[a], hi = g.o[a] + g.L[a] - 0.5f * g.h[a];
-                    pc[a] = pc[a] < lo ? lo : (pc[a] > hi ? hi : pc[a]);
+                    p1[a] = p0[a] + sdt * mv[a];
+                    p0[a] = p1[a];
                 }
-                float cr[3];
-                grad_contract(grid.a[c], g, c, pc, cr);
-                P.C[(int64_t)(3 * c) * cap + s] = cr[0];
-                P.C[(int64_t)(3 * c + 1) * cap + s] = cr[1];
-                P.C[(int64_t)(3 * c + 2) * cap + s] = cr[2];
             }
-        }
-    }
-    if (do_move) {
-        const float sdt = dt / (float)nsub;
-        float p0[3] = {x, y, z}, p1[3] = {x, y, z};
-        for (int t = 0; t < nsub; ++t) {
-            float v0[3], mid[3], mv[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v0[c] = sample_face(grid.a[c], g, c, p0[0], p0[1], p0[2]);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) mid[a] = p0[a] + 0.5f * sdt * v0[a];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) mv[c] = sample_face(grid.a[c], g, c, mid[0], mid[1], mid[2]);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                p1[a] = p0[a] + sdt * mv[a];
-                p0[a] = p1[a];
+                const float lo = g.o[a], hi = g.o[a] + g.L[a];
+                if (p1[a] <= lo) {
+                    p1[a] = lo;
+                    v[a] = 0.0f;
+                }
+                if (p1[a] >= hi) {
+                    p1[a] = hi;
+                    v[a] = 0.0f;
+                }
             }
+            x = p1[0];
+            y = p1[1];
+            z = p1[2];
+            P.x[s] = x;
+            P.x[cap + s] = y;
+            P.x[2 * cap + s] = z;
         }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float lo = g.o[a], hi = g.o[a] + g.L[a];
-            if (p1[a] <= lo) {
-                p1[a] = lo;
-                v[a] = 0.0f;
-            }
-            if (p1[a] >= hi) {
-                p1[a] = hi;
-                v[a] = 0.0f;
-            }
+        P.v[s] = v[0];
+        P.v[cap + s] = v[1];
+        P.v[2 * cap + s] = v[2];
+        if (key) {
+            const int cx = cell_axis(g, 0, x), cy = cell_axis(g, 1, y), cz = cell_axis(g, 2, z);
+            k = (uint32_t)cx + (uint32_t)g.n[0] * ((uint32_t)cy + (uint32_t)g.n[1] * (uint32_t)cz);
+            key[s] = k;
         }
-        x = p1[0];
-        y = p1[1];
-        z = p1[2];
-        P.x[s] = x;
-        P.x[cap + s] = y;
-        P.x[2 * cap + s] = z;
     }
-    P.v[s] = v[0];
-    P.v[cap + s] = v[1];
-    P.v[2 * cap + s] = v[2];
-    if (key) {
-        const int cx = cell_axis(g, 0, x), cy = cell_axis(g, 1, y), cz = cell_axis(g, 2, z);
-        const uint32_t k = (uint32_t)cx + (uint32_t)g.n[0] * ((uint32_t)cy + (uint32_t)g.n[1] * (uint32_t)cz);
-        key[s] = k;
-        atomicAdd(&count[k], 1);
-    }
+    if (key) count_runs(count, k, active);
 }
 
 // Scatter sorted particles back to upload order (download path).
