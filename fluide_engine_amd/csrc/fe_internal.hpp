@@ -123,24 +123,46 @@ __device__ inline void fast_axis(float p, float o, float ih, int size, int& i, f
 
 __device__ inline float lerpf_ref(float a, float b, float f) { return (1.0f - f) * a + f * b; }
 
-// FaceCenteredGrid3D::SampleArray (src/grid_systems/face_centered_grid3d.cpp:339-371):
-// trilinear sample of a face array using its own dims for the clamp.
-__device__ inline float sample_face(const float* __restrict__ a, const Geom& g, int c, float px, float py, float pz) {
+// Trilinear stencil of FaceCenteredGrid3D::SampleArray
+// (src/grid_systems/face_centered_grid3d.cpp:339-371) on face array c, clamped
+// with the array's own dims.  Every face dim is >= 2, so the clamp keeps
+// i <= dim-2 and the x-neighbour i+1 always exists: each (i, i+1) pair is one
+// 8-byte load, four loads per sample.
+struct FaceStencil {
+    int64_t r00, r10, r01, r11;  // row offsets (j,k), (j+1,k), (j,k+1), (j+1,k+1), plus i
+    float fx, fy, fz;
+};
+
+__device__ inline FaceStencil face_stencil(const Geom& g, int c, float px, float py, float pz) {
     const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
     int i, j, k;
-    float fx, fy, fz;
-    fast_axis(px, g.co[c][0], g.ih[0], sx, i, fx);
-    fast_axis(py, g.co[c][1], g.ih[1], sy, j, fy);
-    fast_axis(pz, g.co[c][2], g.ih[2], sz, k, fz);
-    const int ip1 = min(i + 1, sx - 1), jp1 = min(j + 1, sy - 1), kp1 = min(k + 1, sz - 1);
-    const int64_t r00 = (int64_t)sx * ((int64_t)j + (int64_t)sy * k);
-    const int64_t r10 = (int64_t)sx * ((int64_t)jp1 + (int64_t)sy * k);
-    const int64_t r01 = (int64_t)sx * ((int64_t)j + (int64_t)sy * kp1);
-    const int64_t r11 = (int64_t)sx * ((int64_t)jp1 + (int64_t)sy * kp1);
-    const float x000 = a[r00 + i], x100 = a[r00 + ip1], x010 = a[r10 + i], x110 = a[r10 + ip1];
-    const float x001 = a[r01 + i], x101 = a[r01 + ip1], x011 = a[r11 + i], x111 = a[r11 + ip1];
-    return lerpf_ref(lerpf_ref(lerpf_ref(x000, x100, fx), lerpf_ref(x010, x110, fx), fy),
-                     lerpf_ref(lerpf_ref(x001, x101, fx), lerpf_ref(x011, x111, fx), fy), fz);
+    FaceStencil t;
+    fast_axis(px, g.co[c][0], g.ih[0], sx, i, t.fx);
+    fast_axis(py, g.co[c][1], g.ih[1], sy, j, t.fy);
+    fast_axis(pz, g.co[c][2], g.ih[2], sz, k, t.fz);
+    const int jp1 = min(j + 1, sy - 1), kp1 = min(k + 1, sz - 1);
+    t.r00 = (int64_t)sx * ((int64_t)j + (int64_t)sy * k) + i;
+    t.r10 = (int64_t)sx * ((int64_t)jp1 + (int64_t)sy * k) + i;
+    t.r01 = (int64_t)sx * ((int64_t)j + (int64_t)sy * kp1) + i;
+    t.r11 = (int64_t)sx * ((int64_t)jp1 + (int64_t)sy * kp1) + i;
+    return t;
+}
+
+__device__ inline float2 load_pair(const float* __restrict__ p) {
+    float2 v;
+    __builtin_memcpy(&v, p, sizeof(v));  // 4-byte aligned 8-byte load
+    return v;
+}
+
+// The reference's lerp nesting: x first, then y, then z.
+__device__ inline float trilerp_pairs(float2 v00, float2 v10, float2 v01, float2 v11, const FaceStencil& t) {
+    return lerpf_ref(lerpf_ref(lerpf_ref(v00.x, v00.y, t.fx), lerpf_ref(v10.x, v10.y, t.fx), t.fy),
+                     lerpf_ref(lerpf_ref(v01.x, v01.y, t.fx), lerpf_ref(v11.x, v11.y, t.fx), t.fy), t.fz);
+}
+
+__device__ inline float sample_face(const float* __restrict__ a, const Geom& g, int c, float px, float py, float pz) {
+    const FaceStencil t = face_stencil(g, c, px, py, pz);
+    return trilerp_pairs(load_pair(a + t.r00), load_pair(a + t.r10), load_pair(a + t.r01), load_pair(a + t.r11), t);
 }
 
 // XCD-aware block remap: consecutive logical tiles land on the same XCD (and

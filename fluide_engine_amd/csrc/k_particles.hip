@@ -309,22 +309,18 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 // APIC: src/apic_simulator.cpp:120-176) fused with MoveParticles
 // (src/pic_simulator.cpp:239-312) and the bin keys of the next substep.
 // ---------------------------------------------------------------------------
-__device__ inline float sample_delta(const float* __restrict__ a, const float* __restrict__ s, const Geom& g, int c,
-                                     float px, float py, float pz) {
-    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
-    int i, j, k;
-    float fx, fy, fz;
-    fast_axis(px, g.co[c][0], g.ih[0], sx, i, fx);
-    fast_axis(py, g.co[c][1], g.ih[1], sy, j, fy);
-    fast_axis(pz, g.co[c][2], g.ih[2], sz, k, fz);
-    const int ip1 = min(i + 1, sx - 1), jp1 = min(j + 1, sy - 1), kp1 = min(k + 1, sz - 1);
-    auto at = [&](int ii, int jj, int kk) {
-        const int64_t id = (int64_t)ii + (int64_t)sx * ((int64_t)jj + (int64_t)sy * kk);
-        return a[id] - s[id];  // FLIP delta = new - snapshot, node by node
-    };
-    return lerpf_ref(lerpf_ref(lerpf_ref(at(i, j, k), at(ip1, j, k), fx), lerpf_ref(at(i, jp1, k), at(ip1, jp1, k), fx), fy),
-                     lerpf_ref(lerpf_ref(at(i, j, kp1), at(ip1, j, kp1), fx), lerpf_ref(at(i, jp1, kp1), at(ip1, jp1, kp1), fx), fy),
-                     fz);
+// PIC sample of a and the FLIP delta sample of a - snapshot (node by node,
+// flip_simulator.cpp:121-161) on one stencil: 8 pair loads instead of 24 loads.
+__device__ inline void sample_pic_delta(const float* __restrict__ a, const float* __restrict__ sn, const Geom& g,
+                                        int c, float px, float py, float pz, float& pic, float& delta) {
+    const FaceStencil t = face_stencil(g, c, px, py, pz);
+    const float2 a00 = load_pair(a + t.r00), a10 = load_pair(a + t.r10), a01 = load_pair(a + t.r01),
+                 a11 = load_pair(a + t.r11);
+    const float2 s00 = load_pair(sn + t.r00), s10 = load_pair(sn + t.r10), s01 = load_pair(sn + t.r01),
+                 s11 = load_pair(sn + t.r11);
+    pic = trilerp_pairs(a00, a10, a01, a11, t);
+    auto d = [](float2 x, float2 y) { return make_float2(x.x - y.x, x.y - y.y); };
+    delta = trilerp_pairs(d(a00, s00), d(a10, s10), d(a01, s01), d(a11, s11), t);
 }
 
 // GetCooridnatesAndGradientWeights (src/common/math_utils.hpp:114-176) contracted
@@ -366,19 +362,21 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
         float v[3] = {P.v[s], P.v[cap + s], P.v[2 * cap + s]};
         if (do_g2p) {
             float pic[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) pic[c] = sample_face(grid.a[c], g, c, x, y, z);
             if (METHOD == FE_PIC) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = pic[c];
+                for (int c = 0; c < 3; ++c) v[c] = sample_face(grid.a[c], g, c, x, y, z);
             } else if (METHOD == FE_FLIP) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    float fl = v[c] + sample_delta(grid.a[c], snapset.a[c], g, c, x, y, z);
+                    float delta;
+                    sample_pic_delta(grid.a[c], snapset.a[c], g, c, x, y, z, pic[c], delta);
+                    float fl = v[c] + delta;
                     if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
                     v[c] = fl;
                 }
             } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) pic[c] = sample_face(grid.a[c], g, c, x, y, z);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = pic[c];
 #pragma unroll
