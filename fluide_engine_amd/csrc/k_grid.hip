@@ -149,18 +149,39 @@ __global__ __launch_bounds__(kBlock) void k_sdf_tile(Geom g, const float* __rest
     bool found = false;
     for (int c0 = 0; c0 < total; c0 += kSChunk) {
         const int cn = min(kSChunk, total - c0);
-        for (int q = tid; q < cn; q += kBlock) {
-            const int slot = c0 + q;
-            int lo = 0, hi = nrows - 1;  // last row with seg_off <= slot
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (seg_off[mid] <= slot) lo = mid;
-                else hi = mid - 1;
+        constexpr int kPer = kSChunk / kBlock;  // all loads of a thread issued together
+        int64_t gi[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int q = tid + u * kBlock;
+            int64_t v = -1;
+            if (q < cn) {
+                const int slot = c0 + q;
+                int lo = 0, hi = nrows - 1;  // last row with seg_off <= slot
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (seg_off[mid] <= slot) lo = mid;
+                    else hi = mid - 1;
+                }
+                v = (int64_t)seg_g0[lo] + (slot - seg_off[lo]);
             }
-            const int64_t gi = (int64_t)seg_g0[lo] + (slot - seg_off[lo]);
-            px[q] = X[gi];
-            py[q] = X[cap + gi];
-            pz[q] = X[2 * cap + gi];
+            gi[u] = v;
+        }
+        float lx[kPer], ly[kPer], lz[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int64_t i = gi[u] < 0 ? 0 : gi[u];
+            lx[u] = X[i];
+            ly[u] = X[cap + i];
+            lz[u] = X[2 * cap + i];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int q = tid + u * kBlock;
+            if (gi[u] < 0) continue;
+            px[q] = lx[u];
+            py[q] = ly[u];
+            pz[q] = lz[u];
         }
         __syncthreads();
         if (inside) {

@@ -221,6 +221,18 @@ struct P2GChunk {
     static constexpr int kSize = METHOD == FE_APIC ? 512 : 1024;
 };
 
+// Particle index of concatenated halo slot `slot`: the last row whose first
+// slot is <= slot holds it (empty rows share their successor's first slot).
+__device__ inline int64_t halo_particle(const int* seg_off, const int* seg_g0, int nrows, int slot) {
+    int lo = 0, hi = nrows - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg_off[mid] <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    return (int64_t)seg_g0[lo] + (slot - seg_off[lo]);
+}
+
 // Stencil of one particle on one axis: base relative to (region origin - 1)
 // and fraction, for the normal and the transverse data origin.
 __device__ inline void p2g_axis(const Geom& g, int a, float p, int rel0, uint32_t& packed, float& frn, float& frt) {
@@ -307,29 +319,40 @@ __global__ __launch_bounds__(kBlock) void k_p2g_tile(Geom g, const float* __rest
 
     for (int c0 = 0; c0 < total; c0 += CH) {
         const int cn = min(CH, total - c0);
-        // ---- stage the chunk
-        for (int q = tid; q < cn; q += kBlock) {
-            const int slot = c0 + q;
-            int lo = 0, hi = nrows - 1;  // last row with seg_off <= slot
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (seg_off[mid] <= slot) lo = mid;
-                else hi = mid - 1;
+        // ---- stage the chunk: all loads of a thread's particles issued together
+        constexpr int kPer = CH / kBlock;
+        int64_t gi[kPer];
+        float p[kPer][3], vv[kPer][3];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int q = tid + u * kBlock;
+            gi[u] = q < cn ? halo_particle(seg_off, seg_g0, nrows, c0 + q) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int64_t i = gi[u] < 0 ? 0 : gi[u];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                p[u][a] = X[a * cap + i];
+                vv[u][a] = V[a * cap + i];
             }
-            const int64_t gi = (int64_t)seg_g0[lo] + (slot - seg_off[lo]);
-            const float p[3] = {X[gi], X[cap + gi], X[2 * cap + gi]};
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int q = tid + u * kBlock;
+            if (gi[u] < 0) continue;
             uint32_t pk = 0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                p2g_axis(g, a, p[a], rel0[a], pk, s_fn[a][q], s_ft[a][q]);
-                s_v[a][q] = V[a * cap + gi];
+                p2g_axis(g, a, p[u][a], rel0[a], pk, s_fn[a][q], s_ft[a][q]);
+                s_v[a][q] = vv[u][a];
             }
             s_pk[q] = pk;
             if (kApic) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) s_x[a][q] = p[a];
+                for (int a = 0; a < 3; ++a) s_x[a][q] = p[u][a];
 #pragma unroll
-                for (int e = 0; e < 9; ++e) s_c[e][q] = Cm[e * cap + gi];
+                for (int e = 0; e < 9; ++e) s_c[e][q] = Cm[e * cap + gi[u]];
             }
         }
         __syncthreads();
@@ -361,18 +384,22 @@ __global__ __launch_bounds__(kBlock) void k_p2g_tile(Geom g, const float* __rest
                     const int q0 = max(base + g0, c0) - c0, q1 = min(base + g1, c0 + cn) - c0;
                     for (int sl = q0; sl < q1; ++sl) {
                         const uint32_t pk = s_pk[sl];
-                        float w = 1.0f;
+                        int d[3];
                         bool hit = true;
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
                             const int sh = a == C ? 5 * a : 15 + 5 * a;
-                            const int d = fr[a] - (int)((pk >> sh) & 31u);
-                            const float f = a == C ? s_fn[a][sl] : s_ft[a][sl];
-                            hit = hit && (d == 0 || d == 1);
-                            const float wa = d == 0 ? 1.0f - f : f;
-                            w = a == 0 ? wa : w * wa;  // (wx * wy) * wz
+                            d[a] = fr[a] - (int)((pk >> sh) & 31u);
+                            hit = hit && (d[a] == 0 || d[a] == 1);
                         }
                         if (!hit) continue;
+                        float w = 1.0f;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const float f = a == C ? s_fn[a][sl] : s_ft[a][sl];
+                            const float wa = d[a] == 0 ? 1.0f - f : f;
+                            w = a == 0 ? wa : w * wa;  // (wx * wy) * wz
+                        }
                         marked[q] = 1;
                         float val = s_v[C][sl];
                         if (kApic) {
