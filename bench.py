@@ -37,15 +37,11 @@ KERNEL_BYTES_PER_CELL = {
     "cg_spmv": 18,
     # k_jacobi: read x(4) b(4) row code(2), write x'(4)
     "jacobi": 14,
-    # k_mg_down on the finest level: read b(4) row code(2), write x2(4) and the
-    # coarse right-hand side (4 per 8 cells)
-    "mg_down": 10.5,
-    # k_mg_up on the finest level: read x2(4) b(4) row code(2) and the coarse
-    # solution (4 per 8 cells), write z(4)
-    "mg_up": 14.5,
+    # k_mg_smooth on the finest level: read x(4) b(4) row code(2), write x'(4)
+    "mg_smooth": 14,
 }
 KERNEL_IDS = {"none": 0, "cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5,
-              "mg_down": 7, "mg_up": 8}
+              "mg_smooth": 7}
 SOLVERS = {"jacobi": 0, "cg": 1, "pcg": 2, "mgpcg": 3}
 SOLVER_TEXT = {"jacobi": "Jacobi sweeps", "cg": "CG", "pcg": "Jacobi-PCG", "mgpcg": "multigrid-PCG"}
 
@@ -53,6 +49,8 @@ SOLVER_TEXT = {"jacobi": "Jacobi sweeps", "cg": "CG", "pcg": "Jacobi-PCG", "mgpc
 def kernel_bytes_per_cell(kernel, solver):
     b = KERNEL_BYTES_PER_CELL[kernel]
     if kernel == "cg_update" and solver == "pcg":
+        b += 6
+    if kernel == "cg_update" and solver == "mgpcg":  # row code(2) + the V-cycle's first sweep(4)
         b += 6
     return b
 

@@ -30,7 +30,7 @@ EXPORTED = [
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
- KERNEL_MG_DOWN, KERNEL_MG_UP) = range(9)
+ KERNEL_MG_SMOOTH) = range(8)
 
 
 class SolverDesc(C.Structure):

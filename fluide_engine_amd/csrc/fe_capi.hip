@@ -602,7 +602,7 @@ int fe_last_solve_stats(const fe_scene* s, fe_solve_stats* st) {
 int fe_kernel_timer(fe_scene* s, int32_t kernel_id) {
     if (!s) return FE_E_INVALID;
     fe_ctx* ctx = s->ctx;
-    if (kernel_id < FE_KERNEL_NONE || kernel_id > FE_KERNEL_MG_UP)
+    if (kernel_id < FE_KERNEL_NONE || kernel_id > FE_KERNEL_MG_SMOOTH)
         return set_err(ctx, FE_E_INVALID, "unknown kernel id " + std::to_string(kernel_id));
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));

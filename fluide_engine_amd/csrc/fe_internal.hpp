@@ -280,9 +280,7 @@ struct fe_scene {
     double stage_ms[fe::kMaxStages] = {};
     fe_solve_stats last_stats{};
     std::vector<fe::MGLevel> mg;
-    int mg_tail = 0;          // first level handled by the one-workgroup tail kernel
-    double* mg_pz = nullptr;  // partials of r.z left by the finest V-cycle kernel
-    int64_t mg_npz = 0;  // multigrid hierarchy, built on first use
+    int mg_tail = 0;  // first level handled by the one-workgroup tail kernel  // multigrid hierarchy, built on first use
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

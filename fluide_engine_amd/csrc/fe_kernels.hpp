@@ -40,6 +40,7 @@ hipError_t read_final_residual(fe_scene* s, double* rr, double* bb);  // waits f
 // k_mg.hip: multigrid V-cycle preconditioner of the pressure CG
 hipError_t mg_setup(fe_scene* s);  // coarse markers and row codes from s->flags / s->code
 hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz, int tag);
+float* mg_first_sweep_target(fe_scene* s);  // where the CG kernels write level 0's first sweep
 void mg_free(fe_scene* s);
 
 }  // namespace fe

@@ -88,131 +88,6 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float* __restrict_
     sdf[c] = minD - radius;
 }
 
-// The same scan with the particles of an 8x8x4 tile of cells and its halo of
-// ext cells staged in LDS (positions only, in chunks walking the halo rows in
-// order); every cell reads its candidates from LDS.  The minimum is over the
-// same set of FP64 values, so the result is identical to k_sdf's.
-constexpr int kStx = 8, kSty = 8, kStz = 4;
-static_assert(kStx * kSty * kStz == kBlock, "one cell per thread");
-constexpr int kSdfMaxExt = 2;
-constexpr int kSrx = kStx + 2 * kSdfMaxExt, kSry = kSty + 2 * kSdfMaxExt, kSrz = kStz + 2 * kSdfMaxExt;
-constexpr int kSRows = kSry * kSrz;
-constexpr int kSChunk = 2048;
-
-__global__ __launch_bounds__(kBlock) void k_sdf_tile(Geom g, const float* __restrict__ X, int64_t cap,
-                                                     const int32_t* __restrict__ start, double* __restrict__ sdf,
-                                                     int ex, int ey, int ez, double radius, double band,
-                                                     double qr2) {
-#pragma clang fp contract(off)
-    __shared__ int seg_off[kSRows + 1];
-    __shared__ int seg_g0[kSRows];
-    __shared__ int cst[kSRows * (kSrx + 1)];
-    __shared__ float px[kSChunk], py[kSChunk], pz[kSChunk];
-    const int t0[3] = {(int)blockIdx.x * kStx, (int)blockIdx.y * kSty, (int)blockIdx.z * kStz};
-    const int T[3] = {kStx, kSty, kStz}, E[3] = {ex, ey, ez};
-    int r0[3], r1[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        r0[a] = max(t0[a] - E[a], 0);
-        r1[a] = min(t0[a] + T[a] - 1 + E[a], g.n[a] - 1);
-    }
-    const int nrx = r1[0] - r0[0] + 1, nry = r1[1] - r0[1] + 1, nrz = r1[2] - r0[2] + 1;
-    const int nrows = nry * nrz;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < nrows * (nrx + 1); e += kBlock) {
-        const int r = e / (nrx + 1), x = e % (nrx + 1);
-        const int jj = r0[1] + r % nry, kk = r0[2] + r / nry;
-        cst[r * (kSrx + 1) + x] = start[(int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk) + r0[0] + x];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int r = 0; r < nrows; ++r) {
-            seg_off[r] = acc;
-            seg_g0[r] = cst[r * (kSrx + 1)];
-            acc += cst[r * (kSrx + 1) + nrx] - cst[r * (kSrx + 1)];
-        }
-        seg_off[nrows] = acc;
-    }
-    __syncthreads();
-    const int total = seg_off[nrows];
-    const int i = t0[0] + tid % kStx, j = t0[1] + (tid / kStx) % kSty, k = t0[2] + tid / (kStx * kSty);
-    const bool inside = i < g.n[0] && j < g.n[1] && k < g.n[2];
-    // Grid3D::GridIndexToPosition on the SDF grid of origin o + h/2 (src/hybrid_simulator.cpp:13-15)
-    const double gx = (g.od[0] + g.hd[0] / 2.0) + (double)i * g.hd[0];
-    const double gy = (g.od[1] + g.hd[1] / 2.0) + (double)j * g.hd[1];
-    const double gz = (g.od[2] + g.hd[2] / 2.0) + (double)k * g.hd[2];
-    const int i0 = max(i - ex, 0), i1 = min(i + ex, g.n[0] - 1);
-    const int j0 = max(j - ey, 0), j1 = min(j + ey, g.n[1] - 1);
-    const int k0 = max(k - ez, 0), k1 = min(k + ez, g.n[2] - 1);
-    double best = qr2;
-    bool found = false;
-    for (int c0 = 0; c0 < total; c0 += kSChunk) {
-        const int cn = min(kSChunk, total - c0);
-        constexpr int kPer = kSChunk / kBlock;  // all loads of a thread issued together
-        int64_t gi[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int q = tid + u * kBlock;
-            int64_t v = -1;
-            if (q < cn) {
-                const int slot = c0 + q;
-                int lo = 0, hi = nrows - 1;  // last row with seg_off <= slot
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (seg_off[mid] <= slot) lo = mid;
-                    else hi = mid - 1;
-                }
-                v = (int64_t)seg_g0[lo] + (slot - seg_off[lo]);
-            }
-            gi[u] = v;
-        }
-        float lx[kPer], ly[kPer], lz[kPer];
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int64_t i = gi[u] < 0 ? 0 : gi[u];
-            lx[u] = X[i];
-            ly[u] = X[cap + i];
-            lz[u] = X[2 * cap + i];
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int q = tid + u * kBlock;
-            if (gi[u] < 0) continue;
-            px[q] = lx[u];
-            py[q] = ly[u];
-            pz[q] = lz[u];
-        }
-        __syncthreads();
-        if (inside) {
-            for (int kk = k0; kk <= k1; ++kk) {
-                for (int jj = j0; jj <= j1; ++jj) {
-                    const int r = (kk - r0[2]) * nry + (jj - r0[1]);
-                    const int* cs = cst + r * (kSrx + 1);
-                    const int base = seg_off[r] - seg_g0[r];
-                    const int q0 = max(base + cs[i0 - r0[0]], c0) - c0;
-                    const int q1 = min(base + cs[i1 + 1 - r0[0]], c0 + cn) - c0;
-                    for (int q = q0; q < q1; ++q) {
-                        const double dx = (double)px[q] - gx;
-                        const double dy = (double)py[q] - gy;
-                        const double dz = (double)pz[q] - gz;
-                        const double d2 = dx * dx + dy * dy + dz * dz;
-                        if (d2 <= qr2 && (!found || d2 < best)) {
-                            best = d2;
-                            found = true;
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (!inside) return;
-    double minD = band;
-    if (found) minD = fmin(minD, sqrt(best));
-    sdf[(int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k)] = minD - radius;
-}
-
 hipError_t launch_sdf(fe_scene* s) {
     const Geom& g = s->g;
     const double maxH = fmax(g.hd[0], fmax(g.hd[1], g.hd[2]));
@@ -221,18 +96,10 @@ hipError_t launch_sdf(fe_scene* s) {
     const double qr2 = band * band;
     int ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = (int)std::floor(radius / g.hd[a] + 0.5);
-    if (ext[0] <= kSdfMaxExt && ext[1] <= kSdfMaxExt && ext[2] <= kSdfMaxExt) {
-        const dim3 grid((unsigned)((g.n[0] + kStx - 1) / kStx), (unsigned)((g.n[1] + kSty - 1) / kSty),
-                        (unsigned)((g.n[2] + kStz - 1) / kStz));
-        launch_timed(s, FE_KERNEL_SDF, 0, k_sdf_tile, grid, dim3(kBlock), g, (const float*)s->P[s->cur].x, s->cap,
-                     (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2);
-    } else {  // strongly anisotropic cells: a wider halo than the LDS tile holds
-        const int64_t nc = cell_count(g);
-        const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-        launch_timed(s, FE_KERNEL_SDF, 0, k_sdf, dim3((unsigned)padded), dim3(kBlock), g,
-                     (const float*)s->P[s->cur].x, s->cap, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1],
-                     ext[2], radius, band, qr2, padded);
-    }
+    const int64_t nc = cell_count(g);
+    const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
+    launch_timed(s, FE_KERNEL_SDF, 0, k_sdf, dim3((unsigned)padded), dim3(kBlock), g, (const float*)s->P[s->cur].x,
+                 s->cap, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, padded);
     return hipGetLastError();
 }
 

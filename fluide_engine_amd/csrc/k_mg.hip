@@ -161,129 +161,94 @@ __device__ inline float prolong_at(int i, int j, int k, int cx, int cy, int cz, 
 }
 
 // ---------------------------------------------------------------------------
-// Large levels: two kernels per level and V-cycle.
-// A workgroup owns a 16x16x8 tile of fine cells (an 8x8x4 tile of the coarse
-// level) and recomputes the halo it needs from LDS instead of making one pass
-// over the level per step:
-//   down: x1 = w b on the tile + 3 cells, x2 = x1 + w (b - A x1) on the tile + 2
-//         (written out for the tile), r = b - A x2 on the tile + 1, and the
-//         restriction of r onto the coarse tile (the coarse right-hand side);
-//   up:   xp = x2 + P xc on the tile + 2, one sweep on the tile + 1 and one on
-//         the tile, written out (on level 0 as z, with partials of r . z).
-// Grid cells outside the domain hold 0 in the windows; the row codes never
-// couple to them.
+// Large levels: one kernel per step
 // ---------------------------------------------------------------------------
-constexpr int kRtx = 8, kRty = 8, kRtz = 4;  // coarse tile, one cell per thread
-static_assert(kRtx * kRty * kRtz == kBlock, "one coarse cell per thread");
-constexpr int kFtx = 2 * kRtx, kFty = 2 * kRty, kFtz = 2 * kRtz;  // fine tile
-
-// Fine window of the tile widened by H cells on every side.
-template <int H>
-struct Win {
-    static constexpr int x = kFtx + 2 * H, y = kFty + 2 * H, z = kFtz + 2 * H, n = x * y * z;
-    static constexpr Stride32 st{x, x * y, n - 1};
-    __device__ static int at(int wx, int wy, int wz) { return wx + x * (wy + y * wz); }
-};
-
-struct FineArgs {
-    int n[3];         // fine level
+struct LevelArgs {
+    int64_t nc;
+    Stride st;
     float ih2[3];
-    int c[3];         // coarse level
-    int tiles[3];     // tile grid
-    int64_t padded;   // tile count padded for the XCD remap
+    int64_t chunk, padded;
 };
 
-// Tile coordinates of this workgroup (consecutive tiles share an XCD) and
-// whether it maps past the last tile.
-__device__ inline bool tile_of(const FineArgs& a, int& tx, int& ty, int& tz) {
-    const int64_t t = xcd_remap(blockIdx.x, a.padded);
-    if (t >= (int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2]) return false;
-    tx = (int)(t % a.tiles[0]);
-    ty = (int)((t / a.tiles[0]) % a.tiles[1]);
-    tz = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
-    return true;
-}
-
-__device__ inline bool fine_inside(const FineArgs& a, int i, int j, int k) {
-    return i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
-}
-
-__device__ inline int64_t fine_index(const FineArgs& a, int i, int j, int k) {
-    return (int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k);
-}
-
-__global__ __launch_bounds__(kBlock) void k_mg_down(FineArgs a, const float* __restrict__ b,
-                                                    const uint16_t* __restrict__ code, float* __restrict__ x2out,
-                                                    const uint16_t* __restrict__ ccode, float* __restrict__ bc,
-                                                    const SolverScalars* __restrict__ st) {
+// x_out = x_in + w (b - A x_in).  With prz != nullptr (the last level-0 sweep)
+// also leaves partials of b . x_out, which is r . z of the CG.
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
+                                                      const float* __restrict__ b, const float* __restrict__ xin,
+                                                      float* __restrict__ xout, const SolverScalars* __restrict__ st,
+                                                      double* __restrict__ prz) {
     if (st->done) return;  // uniform: the solve has stopped
-    int tx, ty, tz;
-    if (!tile_of(a, tx, ty, tz)) return;
-    __shared__ float s1[Win<3>::n];  // x1
-    __shared__ float s2[Win<2>::n];  // x2
-    __shared__ float s3[Win<1>::n];  // r
-    const int o[3] = {tx * kFtx, ty * kFty, tz * kFtz};
-    const int tid = threadIdx.x;
-    // x1 = w b on the tile + 3
-    for (int e = tid; e < Win<3>::n; e += kBlock) {
-        const int wx = e % Win<3>::x, wy = (e / Win<3>::x) % Win<3>::y, wz = e / (Win<3>::x * Win<3>::y);
-        const int i = o[0] - 3 + wx, j = o[1] - 3 + wy, k = o[2] - 3 + wz;
-        float v = 0.0f;
-        if (fine_inside(a, i, j, k)) {
-            const int64_t f = fine_index(a, i, j, k);
-            v = mg_weight(code[f], a.ih2) * b[f];
-        }
-        s1[e] = v;
+    const Span sp = block_span(L.nc, L.chunk, L.padded);
+    const Stride stv = L.st;
+    double v[1] = {0.0};
+    auto X = [&](int64_t n) { return xin[n]; };
+#pragma unroll 2
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+        const uint16_t cd = code[c];
+        const float dgn = code_center(cd, L.ih2);
+        const float w = mg_weight(cd, L.ih2);
+        const float xc = xin[c];
+        const float bc = b[c];
+        const float xo = xc + w * (bc - code_apply(cd, dgn, c, stv, L.ih2, xc, X));
+        xout[c] = xo;
+        v[0] += (double)bc * (double)xo;
+    }
+    if (prz) write_partials<1>(v, prz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint16_t* __restrict__ code,
+                                                        const float* __restrict__ b, const float* __restrict__ x,
+                                                        float* __restrict__ r, const SolverScalars* __restrict__ st) {
+    if (st->done) return;
+    const Span sp = block_span(L.nc, L.chunk, L.padded);
+    const Stride stv = L.st;
+    auto X = [&](int64_t n) { return x[n]; };
+#pragma unroll 2
+    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+        const uint16_t cd = code[c];
+        float rc = 0.0f;
+        if (cd & kFluidBit) rc = b[c] - code_apply(cd, code_center(cd, L.ih2), c, stv, L.ih2, x[c], X);
+        r[c] = rc;
+    }
+}
+
+// b_c = R r_f on coarse FLUID cells (0 elsewhere); with x1 != nullptr also the
+// coarse level's first sweep from zero, x1 = w b_c.  A workgroup covers an
+// 8x8x4 coarse tile and first stages the 18x18x10 fine window it reads in LDS.
+constexpr int kRtx = 8, kRty = 8, kRtz = 4;
+constexpr int kRwx = 2 * kRtx + 2, kRwy = 2 * kRty + 2, kRwz = 2 * kRtz + 2;
+static_assert(kRtx * kRty * kRtz == kBlock, "one coarse cell per thread");
+
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict__ rf, int fx, int fy, int fz,
+                                                        const uint16_t* __restrict__ ccode, float* __restrict__ bc,
+                                                        float* __restrict__ x1, int cx, int cy, int cz, float cihx2,
+                                                        float cihy2, float cihz2,
+                                                        const SolverScalars* __restrict__ st) {
+    if (st->done) return;
+    __shared__ float win[kRwz][kRwy][kRwx];
+    const int I0 = blockIdx.x * kRtx, J0 = blockIdx.y * kRty, K0 = blockIdx.z * kRtz;
+    const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1, fk0 = 2 * K0 - 1;
+    const int tx = threadIdx.x % kRtx, ty = (threadIdx.x / kRtx) % kRty, tz = threadIdx.x / (kRtx * kRty);
+    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
+    const bool inside = I < cx && J < cy && K < cz;
+    const int64_t c = (int64_t)I + (int64_t)cx * ((int64_t)J + (int64_t)cy * K);
+    const uint16_t cd = inside ? ccode[c] : (uint16_t)0;  // issued with the window loads
+    for (int e = threadIdx.x; e < kRwx * kRwy * kRwz; e += kBlock) {
+        const int wi = e % kRwx, wj = (e / kRwx) % kRwy, wk = e / (kRwx * kRwy);
+        const int fi = min(max(fi0 + wi, 0), fx - 1);
+        const int fj = min(max(fj0 + wj, 0), fy - 1);
+        const int fk = min(max(fk0 + wk, 0), fz - 1);
+        win[wk][wj][wi] = rf[(int64_t)fi + (int64_t)fx * ((int64_t)fj + (int64_t)fy * fk)];
     }
     __syncthreads();
-    // x2 on the tile + 2
-    for (int e = tid; e < Win<2>::n; e += kBlock) {
-        const int wx = e % Win<2>::x, wy = (e / Win<2>::x) % Win<2>::y, wz = e / (Win<2>::x * Win<2>::y);
-        const int i = o[0] - 2 + wx, j = o[1] - 2 + wy, k = o[2] - 2 + wz;
-        float v = 0.0f;
-        if (fine_inside(a, i, j, k)) {
-            const int64_t f = fine_index(a, i, j, k);
-            const uint16_t cd = code[f];
-            const float dgn = code_center(cd, a.ih2);
-            const int c3 = Win<3>::at(wx + 1, wy + 1, wz + 1);
-            const float xc = s1[c3];
-            v = xc + mg_weight(cd, a.ih2) *
-                         (b[f] - code_apply(cd, dgn, c3, Win<3>::st, a.ih2, xc, [&](int n) { return s1[n]; }));
-            if (wx >= 2 && wx < 2 + kFtx && wy >= 2 && wy < 2 + kFty && wz >= 2 && wz < 2 + kFtz) x2out[f] = v;
-        }
-        s2[e] = v;
-    }
-    __syncthreads();
-    // r = b - A x2 on FLUID cells of the tile + 1
-    for (int e = tid; e < Win<1>::n; e += kBlock) {
-        const int wx = e % Win<1>::x, wy = (e / Win<1>::x) % Win<1>::y, wz = e / (Win<1>::x * Win<1>::y);
-        const int i = o[0] - 1 + wx, j = o[1] - 1 + wy, k = o[2] - 1 + wz;
-        float v = 0.0f;
-        if (fine_inside(a, i, j, k)) {
-            const int64_t f = fine_index(a, i, j, k);
-            const uint16_t cd = code[f];
-            if (cd & kFluidBit) {
-                const int c2 = Win<2>::at(wx + 1, wy + 1, wz + 1);
-                v = b[f] - code_apply(cd, code_center(cd, a.ih2), c2, Win<2>::st, a.ih2, s2[c2],
-                                      [&](int n) { return s2[n]; });
-            }
-        }
-        s3[e] = v;
-    }
-    __syncthreads();
-    // restriction onto the coarse tile: fine index 2I-1+t sits at window
-    // coordinate 2*(I - I0) + t (clamped taps carry weight 0)
-    const int lx = tid % kRtx, ly = (tid / kRtx) % kRty, lz = tid / (kRtx * kRty);
-    const int I = tx * kRtx + lx, J = ty * kRty + ly, K = tz * kRtz + lz;
-    if (I >= a.c[0] || J >= a.c[1] || K >= a.c[2]) return;
-    const int64_t cidx = (int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K);
+    if (!inside) return;
     float v = 0.0f;
-    if (ccode[cidx] & kFluidBit) {
+    if (cd & kFluidBit) {
+        // window coordinates of fine index 2I-1+t are 2*tx+t (clamped taps carry weight 0)
         int ix[4], iy[4], iz[4];
         float wx[4], wy[4], wz[4];
-        restrict_taps(I, a.n[0], a.c[0], ix, wx);
-        restrict_taps(J, a.n[1], a.c[1], iy, wy);
-        restrict_taps(K, a.n[2], a.c[2], iz, wz);
+        restrict_taps(I, fx, cx, ix, wx);
+        restrict_taps(J, fy, cy, iy, wy);
+        restrict_taps(K, fz, cz, iz, wz);
         float acc = 0.0f;
 #pragma unroll
         for (int tk = 0; tk < 4; ++tk) {
@@ -291,75 +256,30 @@ __global__ __launch_bounds__(kBlock) void k_mg_down(FineArgs a, const float* __r
             for (int tj = 0; tj < 4; ++tj) {
                 const float wjk = wy[tj] * wz[tk];
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti)
-                    acc += (wx[ti] * wjk) * s3[Win<1>::at(2 * lx + ti, 2 * ly + tj, 2 * lz + tk)];
+                for (int ti = 0; ti < 4; ++ti) acc += (wx[ti] * wjk) * win[2 * tz + tk][2 * ty + tj][2 * tx + ti];
             }
         }
         v = acc * 0.125f;
     }
-    bc[cidx] = v;
+    bc[c] = v;
+    if (x1) {
+        const float ih2[3] = {cihx2, cihy2, cihz2};
+        x1[c] = mg_weight(cd, ih2) * v;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mg_up(FineArgs a, const float* __restrict__ x2, const float* __restrict__ xc,
-                                                  const float* __restrict__ b, const uint16_t* __restrict__ code,
-                                                  float* __restrict__ out, double* __restrict__ prz,
-                                                  const SolverScalars* __restrict__ st) {
+// x_f += P x_c on FLUID fine cells.
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, int cx, int cy, int cz,
+                                                       const uint16_t* __restrict__ fcode, float* __restrict__ xf,
+                                                       int fx, int fy, int fz, const SolverScalars* __restrict__ st) {
     if (st->done) return;
-    int tx, ty, tz;
-    const bool real = tile_of(a, tx, ty, tz);
-    __shared__ float sp[Win<2>::n];  // x2 + P xc
-    __shared__ float s1[Win<1>::n];  // first sweep
-    double v[1] = {0.0};
-    if (real) {
-        const int o[3] = {tx * kFtx, ty * kFty, tz * kFtz};
-        const int tid = threadIdx.x;
-        for (int e = tid; e < Win<2>::n; e += kBlock) {
-            const int wx = e % Win<2>::x, wy = (e / Win<2>::x) % Win<2>::y, wz = e / (Win<2>::x * Win<2>::y);
-            const int i = o[0] - 2 + wx, j = o[1] - 2 + wy, k = o[2] - 2 + wz;
-            float val = 0.0f;
-            if (fine_inside(a, i, j, k)) {
-                const int64_t f = fine_index(a, i, j, k);
-                val = x2[f];
-                if (code[f] & kFluidBit)
-                    val += prolong_at(i, j, k, a.c[0], a.c[1], a.c[2], [&](int64_t n) { return xc[n]; });
-            }
-            sp[e] = val;
-        }
-        __syncthreads();
-        for (int e = tid; e < Win<1>::n; e += kBlock) {
-            const int wx = e % Win<1>::x, wy = (e / Win<1>::x) % Win<1>::y, wz = e / (Win<1>::x * Win<1>::y);
-            const int i = o[0] - 1 + wx, j = o[1] - 1 + wy, k = o[2] - 1 + wz;
-            float val = 0.0f;
-            if (fine_inside(a, i, j, k)) {
-                const int64_t f = fine_index(a, i, j, k);
-                const uint16_t cd = code[f];
-                const float dgn = code_center(cd, a.ih2);
-                const int c2 = Win<2>::at(wx + 1, wy + 1, wz + 1);
-                const float xv = sp[c2];
-                val = xv + mg_weight(cd, a.ih2) *
-                               (b[f] - code_apply(cd, dgn, c2, Win<2>::st, a.ih2, xv, [&](int n) { return sp[n]; }));
-            }
-            s1[e] = val;
-        }
-        __syncthreads();
-        for (int e = tid; e < Win<0>::n; e += kBlock) {
-            const int wx = e % Win<0>::x, wy = (e / Win<0>::x) % Win<0>::y, wz = e / (Win<0>::x * Win<0>::y);
-            const int i = o[0] + wx, j = o[1] + wy, k = o[2] + wz;
-            if (!fine_inside(a, i, j, k)) continue;
-            const int64_t f = fine_index(a, i, j, k);
-            const uint16_t cd = code[f];
-            const float dgn = code_center(cd, a.ih2);
-            const int c1 = Win<1>::at(wx + 1, wy + 1, wz + 1);
-            const float xv = s1[c1];
-            const float bf = b[f];
-            const float val =
-                xv + mg_weight(cd, a.ih2) *
-                         (bf - code_apply(cd, dgn, c1, Win<1>::st, a.ih2, xv, [&](int n) { return s1[n]; }));
-            out[f] = val;
-            v[0] += (double)bf * (double)val;
-        }
-    }
-    if (prz) write_partials<1>(v, prz);  // every workgroup, real or padding, leaves its partial
+    const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nf = (int64_t)fx * fy * fz;
+    if (f >= nf) return;
+    if (!(fcode[f] & kFluidBit)) return;
+    int i, j, k;
+    cell_ijk(f, fx, fy, i, j, k);
+    xf[f] += prolong_at(i, j, k, cx, cy, cz, [&](int64_t n) { return xc[n]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -500,17 +420,14 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
 // ---------------------------------------------------------------------------
 static inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-static FineArgs fine_args(const MGLevel& l, const MGLevel& c) {
-    FineArgs a;
-    for (int q = 0; q < 3; ++q) {
-        a.n[q] = l.n[q];
-        a.ih2[q] = l.ih2[q];
-        a.c[q] = c.n[q];
-    }
-    a.tiles[0] = (c.n[0] + kRtx - 1) / kRtx;
-    a.tiles[1] = (c.n[1] + kRty - 1) / kRty;
-    a.tiles[2] = (c.n[2] + kRtz - 1) / kRtz;
-    a.padded = pad_blocks((int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2]);
+static LevelArgs level_args(const MGLevel& l) {
+    const SolverShape sh = solver_shape_n(l.cells);
+    LevelArgs a;
+    a.nc = l.cells;
+    a.st = stride_of(l.n[0], l.n[1], l.n[2]);
+    for (int q = 0; q < 3; ++q) a.ih2[q] = l.ih2[q];
+    a.chunk = sh.chunk;
+    a.padded = sh.blocks;
     return a;
 }
 
@@ -563,9 +480,8 @@ static hipError_t mg_alloc(fe_scene* s) {
         }
         if ((e = hipMalloc((void**)&l.x, n * sizeof(float))) != hipSuccess) break;
         if ((e = hipMalloc((void**)&l.x2, n * sizeof(float))) != hipSuccess) break;
+        if ((e = hipMalloc((void**)&l.r, n * sizeof(float))) != hipSuccess) break;
     }
-    s->mg_npz = fine_args(lv[0], lv[1]).padded;
-    if (e == hipSuccess) e = hipMalloc((void**)&s->mg_pz, (size_t)s->mg_npz * sizeof(double));
     s->mg = lv;
     s->mg_tail = tail_start(lv);
     if (e != hipSuccess) mg_free(s);  // leaves the hierarchy empty: the next solve retries
@@ -581,11 +497,9 @@ void mg_free(fe_scene* s) {
         }
         (void)hipFree(l.x);
         (void)hipFree(l.x2);
+        (void)hipFree(l.r);
     }
     s->mg.clear();
-    (void)hipFree(s->mg_pz);
-    s->mg_pz = nullptr;
-    s->mg_npz = 0;
 }
 
 // Per solve: coarse markers and row codes from the current level-0 markers
@@ -606,24 +520,34 @@ hipError_t mg_setup(fe_scene* s) {
     return hipGetLastError();
 }
 
-// z = M^-1 r with one V-cycle; leaves s->mg_npz partials of r.z in prz.
-// Every kernel returns at once when `st` says the solve has stopped.
+float* mg_first_sweep_target(fe_scene* s) { return s->mg.empty() ? nullptr : s->mg[0].x; }
+
+// z = M^-1 r with one V-cycle.  The caller has already written level 0's
+// first sweep (x1 = w r) into mg[0].x.  Leaves partials of r.z in prz.  Every
+// kernel returns at once when `st` says the solve has stopped.
 hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz, int tag) {
     std::vector<MGLevel>& lv = s->mg;
     const int L = (int)lv.size();
     const int T = s->mg_tail;
     lv[0].b = const_cast<float*>(r);
     for (int q = 0; q < T; ++q) {
-        const MGLevel& l = lv[q];
+        MGLevel& l = lv[q];
         MGLevel& c = lv[q + 1];
-        const FineArgs a = fine_args(l, c);
+        const LevelArgs a = level_args(l);
         const dim3 grid((unsigned)a.padded);
         if (q == 0)
-            launch_timed(s, FE_KERNEL_MG_DOWN, tag, k_mg_down, grid, dim3(kBlock), a, (const float*)l.b,
-                         (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b, st);
+            launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
+                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
         else
-            hipLaunchKernelGGL(k_mg_down, grid, dim3(kBlock), 0, s->stream, a, (const float*)l.b,
-                               (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b, st);
+            hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+        hipLaunchKernelGGL(k_mg_residual, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                           (const float*)l.b, (const float*)l.x2, l.r, st);
+        const dim3 rgrid((unsigned)((c.n[0] + kRtx - 1) / kRtx), (unsigned)((c.n[1] + kRty - 1) / kRty),
+                         (unsigned)((c.n[2] + kRtz - 1) / kRtz));
+        hipLaunchKernelGGL(k_mg_restrict, rgrid, dim3(kBlock), 0, s->stream, (const float*)l.r, l.n[0],
+                           l.n[1], l.n[2], (const uint16_t*)c.code, c.b, q + 1 < T ? c.x : (float*)nullptr, c.n[0],
+                           c.n[1], c.n[2], c.ih2[0], c.ih2[1], c.ih2[2], st);
     }
     {
         TailArgs ta{};
@@ -639,21 +563,27 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* 
             ta.off[t] = off;
             off += (int)l.cells;
         }
-        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s->stream, ta, (const float*)lv[T].b, lv[T].x,
-                           st);
+        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s->stream, ta, (const float*)lv[T].b,
+                           lv[T].x2, st);
     }
-    // the solution of every level ends in its x
     for (int q = T - 1; q >= 0; --q) {
         MGLevel& l = lv[q];
         const MGLevel& c = lv[q + 1];
-        const FineArgs a = fine_args(l, c);
+        const LevelArgs a = level_args(l);
         const dim3 grid((unsigned)a.padded);
-        if (q == 0)
-            launch_timed(s, FE_KERNEL_MG_UP, tag, k_mg_up, grid, dim3(kBlock), a, (const float*)l.x2,
-                         (const float*)c.x, (const float*)l.b, (const uint16_t*)l.code, z, prz, st);
-        else
-            hipLaunchKernelGGL(k_mg_up, grid, dim3(kBlock), 0, s->stream, a, (const float*)l.x2, (const float*)c.x,
-                               (const float*)l.b, (const uint16_t*)l.code, l.x, (double*)nullptr, st);
+        hipLaunchKernelGGL(k_mg_prolong, grid1(l.cells), dim3(kBlock), 0, s->stream, (const float*)c.x2, c.n[0],
+                           c.n[1], c.n[2], (const uint16_t*)l.code, l.x2, l.n[0], l.n[1], l.n[2], st);
+        if (q == 0) {
+            launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
+                         (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr);
+            launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
+                         (const float*)l.b, (const float*)l.x, z, st, prz);
+        } else {
+            hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                               (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr);
+            hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
+                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+        }
     }
     return hipGetLastError();
 }

@@ -192,244 +192,116 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ o
 // P2G: TransferParticles2Grid as a deterministic gather.
 // PIC: src/pic_simulator.cpp:356-437; FLIP: src/flip_simulator.cpp:25-119;
 // APIC: src/apic_simulator.cpp:15-118.
-// A face F of component C receives the particles whose stencil (the
-// reference's GetCooridnatesAndWeights with the scheme's stencil size) has F
-// among its 8 nodes; they lie in cells [F-1, F+1] on the transverse axes and
-// [F-1, F] on the normal axis (plus cell n-1 for face n-2 when PIC/APIC clamp
-// the normal axis at n-2).  The face sums w*v and w over them in ascending
-// particle order and is marked even for a zero weight.
-//
-// One workgroup per 8x8x4 tile of cells computes the three faces at each of
-// its cells (and the last face n of an axis on the grid's far side).  The
-// particles of the tile's cells and a one-cell halo are staged in LDS in
-// chunks, each with its stencil computed once: per axis the normal-axis base
-// and fraction (used by the component along that axis) and the transverse
-// base and fraction (used by the other two; both share the data origin
-// o + h/2 and the cell count as stencil size).  Cells are x-fastest and
-// particles sorted by cell, so every (j,k) row of the halo region is one
-// contiguous particle range; chunks walk the rows in order, so each face still
-// visits its candidates in ascending particle order.
+// One thread per face of component C.  The particles whose stencil can touch
+// face F lie in cells [F-1, F+1] on the transverse axes and [F-1, F] on the
+// normal axis (plus cell n-1 for face n-2 when PIC/APIC clamp the normal axis
+// at n-2).  Cells are x-fastest and particles sorted by cell, so each (j,k)
+// row of candidate cells is one contiguous particle range.  For every
+// candidate the reference's own stencil (GetCooridnatesAndWeights with the
+// scheme's stencil size) is evaluated and the particle contributes iff the face
+// is one of its 8 nodes; the face is marked even for a zero weight.
 // ---------------------------------------------------------------------------
-constexpr int kPtx = 8, kPty = 8, kPtz = 4;  // tile (one cell per thread)
-static_assert(kPtx * kPty * kPtz == kBlock, "one cell per thread");
-constexpr int kPrx = kPtx + 2, kPry = kPty + 2, kPrz = kPtz + 2;  // halo region
-constexpr int kPRows = kPry * kPrz;
-constexpr int kPStarts = kPRows * (kPrx + 1);
-
-template <int METHOD>
-struct P2GChunk {
-    static constexpr int kSize = METHOD == FE_APIC ? 512 : 1024;
-};
-
-// Particle index of concatenated halo slot `slot`: the last row whose first
-// slot is <= slot holds it (empty rows share their successor's first slot).
-__device__ inline int64_t halo_particle(const int* seg_off, const int* seg_g0, int nrows, int slot) {
-    int lo = 0, hi = nrows - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (seg_off[mid] <= slot) lo = mid;
-        else hi = mid - 1;
-    }
-    return (int64_t)seg_g0[lo] + (slot - seg_off[lo]);
-}
-
-// Stencil of one particle on one axis: base relative to (region origin - 1)
-// and fraction, for the normal and the transverse data origin.
-__device__ inline void p2g_axis(const Geom& g, int a, float p, int rel0, uint32_t& packed, float& frn, float& frt) {
-    int i0;
-    stencil_axis(g, a, a, p, stencil_size(g, a, a), i0, frn);
-    packed |= (uint32_t)min(max(i0 - rel0, 0), 31) << (5 * a);
-    const int ct = a == 0 ? 1 : 0;  // a component for which axis a is transverse
-    stencil_axis(g, ct, a, p, stencil_size(g, ct, a), i0, frt);
-    packed |= (uint32_t)min(max(i0 - rel0, 0), 31) << (15 + 5 * a);
-}
-
-template <int METHOD>
-__global__ __launch_bounds__(kBlock) void k_p2g_tile(Geom g, const float* __restrict__ X, const float* __restrict__ V,
-                                                     const float* __restrict__ Cm, int64_t cap,
-                                                     const int32_t* __restrict__ start, FaceSet out, FaceSet snap,
-                                                     uint8_t* __restrict__ mk0, uint8_t* __restrict__ mk1,
-                                                     uint8_t* __restrict__ mk2) {
-    constexpr int CH = P2GChunk<METHOD>::kSize;
-    constexpr bool kApic = METHOD == FE_APIC;
-    __shared__ int seg_off[kPRows + 1];  // concatenated slot of each halo row
-    __shared__ int seg_g0[kPRows];       // particle index of each row's first particle
-    __shared__ int cst[kPStarts];        // cell starts of the halo region (one extra per row)
-    __shared__ uint32_t s_pk[CH];
-    __shared__ float s_fn[3][CH], s_ft[3][CH], s_v[3][CH];
-    __shared__ float s_x[kApic ? 3 : 1][kApic ? CH : 1], s_c[kApic ? 9 : 1][kApic ? CH : 1];
-
-    const int t0[3] = {(int)blockIdx.x * kPtx, (int)blockIdx.y * kPty, (int)blockIdx.z * kPtz};
-    int r0[3], r1[3];
+template <int C, int METHOD>
+__global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ X, const float* __restrict__ V,
+                                                const float* __restrict__ Cm, int64_t cap,
+                                                const int32_t* __restrict__ start, float* __restrict__ out,
+                                                uint8_t* __restrict__ mark, float* __restrict__ snap,
+                                                int64_t padded) {
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
+    const int64_t nf = (int64_t)sx * sy * sz;
+    const int64_t f = blk * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    const int I = (int)(f % sx);
+    const int64_t t = f / sx;
+    const int J = (int)(t % sy);
+    const int K = (int)(t / sy);
+    const int F[3] = {I, J, K};
+    int lo[3], hi[3], S[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        r0[a] = max(t0[a] - 1, 0);
-        r1[a] = min(t0[a] + (a == 0 ? kPtx : (a == 1 ? kPty : kPtz)), g.n[a] - 1);
-    }
-    const int nrx = r1[0] - r0[0] + 1, nry = r1[1] - r0[1] + 1, nrz = r1[2] - r0[2] + 1;
-    const int nrows = nry * nrz;
-    const int tid = threadIdx.x;
-    // halo rows and cell starts
-    for (int e = tid; e < nrows * (nrx + 1); e += kBlock) {
-        const int r = e / (nrx + 1), x = e % (nrx + 1);
-        const int jj = r0[1] + r % nry, kk = r0[2] + r / nry;
-        cst[r * (kPrx + 1) + x] = start[(int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk) + r0[0] + x];
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int r = 0; r < nrows; ++r) {
-            seg_off[r] = acc;
-            seg_g0[r] = cst[r * (kPrx + 1)];
-            acc += cst[r * (kPrx + 1) + nrx] - cst[r * (kPrx + 1)];
+        if (a == C) {
+            lo[a] = F[a] - 1;
+            hi[a] = F[a];
+            if (METHOD != FE_FLIP && F[a] == g.n[a] - 2) hi[a] = F[a] + 1;
+        } else {
+            lo[a] = F[a] - 1;
+            hi[a] = F[a] + 1;
         }
-        seg_off[nrows] = acc;
+        lo[a] = max(lo[a], 0);
+        hi[a] = min(hi[a], g.n[a] - 1);
+        S[a] = stencil_size(g, C, a);
     }
-    __syncthreads();
-    const int total = seg_off[nrows];
-
-    // this thread's cell and faces
-    const int li = tid % kPtx, lj = (tid / kPtx) % kPty, lk = tid / (kPtx * kPty);
-    const int cell[3] = {t0[0] + li, t0[1] + lj, t0[2] + lk};
-    const bool inside = cell[0] < g.n[0] && cell[1] < g.n[1] && cell[2] < g.n[2];
-    const int rel0[3] = {r0[0] - 1, r0[1] - 1, r0[2] - 1};
-    // face q: component q % 3 at the cell, or (q >= 3) the far-side face n of that axis
-    bool has[6];
-    int F[6][3];
-    float num[6], den[6];
-    int marked[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int C = q % 3;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) F[q][a] = cell[a] + ((q >= 3 && a == C) ? 1 : 0);
-        has[q] = inside && (q < 3 || cell[C] == g.n[C] - 1);
-        num[q] = 0.0f;
-        den[q] = 0.0f;
-        marked[q] = 0;
-    }
-    // APIC: face positions GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
+    // APIC: face position GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
     // and the clamp box [o + h/2, o + L - h/2] of the transverse axes (apic_simulator.cpp:54-63).
-    float clo[3], chi[3];
+    float gp[3], clo[3], chi[3];
+    if (METHOD == FE_APIC) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        clo[a] = g.o[a] + 0.5f * g.h[a];
-        chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
+        for (int a = 0; a < 3; ++a) {
+            gp[a] = g.co[C][a] + g.h[a] * (float)F[a];
+            clo[a] = g.o[a] + 0.5f * g.h[a];
+            chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
+        }
     }
-
-    for (int c0 = 0; c0 < total; c0 += CH) {
-        const int cn = min(CH, total - c0);
-        // ---- stage the chunk: all loads of a thread's particles issued together
-        constexpr int kPer = CH / kBlock;
-        int64_t gi[kPer];
-        float p[kPer][3], vv[kPer][3];
+    float num = 0.0f, den = 0.0f;
+    int marked = 0;
+    // one candidate particle, in ascending particle order
+    auto consider = [&](int s, float px, float py, float pz, float val) {
+        int i0;
+        float fr;
+        stencil_axis(g, C, 0, px, S[0], i0, fr);
+        float wx;
+        if (I == i0) wx = 1.0f - fr;
+        else if (I == i0 + 1) wx = fr;
+        else return;
+        stencil_axis(g, C, 1, py, S[1], i0, fr);
+        float wy;
+        if (J == i0) wy = 1.0f - fr;
+        else if (J == i0 + 1) wy = fr;
+        else return;
+        stencil_axis(g, C, 2, pz, S[2], i0, fr);
+        float wz;
+        if (K == i0) wz = 1.0f - fr;
+        else if (K == i0 + 1) wz = fr;
+        else return;
+        marked = 1;
+        const float w = (wx * wy) * wz;
+        if (METHOD == FE_APIC) {
+            float pc[3] = {px, py, pz};
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int q = tid + u * kBlock;
-            gi[u] = q < cn ? halo_particle(seg_off, seg_g0, nrows, c0 + q) : -1;
+            for (int a = 0; a < 3; ++a)
+                if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
+            const float* cr = Cm + (int64_t)(3 * C) * cap + s;
+            val += cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]);
         }
+        num += val * w;
+        den += w;
+    };
+    constexpr int kBatch = 4;  // candidates whose loads are issued together
+    for (int kc = lo[2]; kc <= hi[2]; ++kc) {
+        for (int jc = lo[1]; jc <= hi[1]; ++jc) {
+            const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
+            const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
+            int s = s0;
+            for (; s + kBatch <= s1; s += kBatch) {
+                float px[kBatch], py[kBatch], pz[kBatch], pv[kBatch];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int64_t i = gi[u] < 0 ? 0 : gi[u];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                p[u][a] = X[a * cap + i];
-                vv[u][a] = V[a * cap + i];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int q = tid + u * kBlock;
-            if (gi[u] < 0) continue;
-            uint32_t pk = 0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                p2g_axis(g, a, p[u][a], rel0[a], pk, s_fn[a][q], s_ft[a][q]);
-                s_v[a][q] = vv[u][a];
-            }
-            s_pk[q] = pk;
-            if (kApic) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) s_x[a][q] = p[u][a];
-#pragma unroll
-                for (int e = 0; e < 9; ++e) s_c[e][q] = Cm[e * cap + gi[u]];
-            }
-        }
-        __syncthreads();
-        // ---- every face of this thread over its candidates in the chunk
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            if (!has[q]) continue;
-            const int C = q % 3;
-            int lo[3], hi[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = max(F[q][a] - 1, 0);
-                hi[a] = a == C ? F[q][a] : F[q][a] + 1;
-                if (a == C && METHOD != FE_FLIP && F[q][a] == g.n[a] - 2) hi[a] = F[q][a] + 1;
-                hi[a] = min(hi[a], g.n[a] - 1);
-            }
-            const int fr[3] = {F[q][0] - rel0[0], F[q][1] - rel0[1], F[q][2] - rel0[2]};
-            float gp[3];
-            if (kApic) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) gp[a] = g.co[C][a] + g.h[a] * (float)F[q][a];
-            }
-            for (int kc = lo[2]; kc <= hi[2]; ++kc) {
-                for (int jc = lo[1]; jc <= hi[1]; ++jc) {
-                    const int r = (kc - r0[2]) * nry + (jc - r0[1]);
-                    const int* cs = cst + r * (kPrx + 1);
-                    const int g0 = cs[lo[0] - r0[0]], g1 = cs[hi[0] + 1 - r0[0]];
-                    const int base = seg_off[r] - seg_g0[r];
-                    const int q0 = max(base + g0, c0) - c0, q1 = min(base + g1, c0 + cn) - c0;
-                    for (int sl = q0; sl < q1; ++sl) {
-                        const uint32_t pk = s_pk[sl];
-                        int d[3];
-                        bool hit = true;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const int sh = a == C ? 5 * a : 15 + 5 * a;
-                            d[a] = fr[a] - (int)((pk >> sh) & 31u);
-                            hit = hit && (d[a] == 0 || d[a] == 1);
-                        }
-                        if (!hit) continue;
-                        float w = 1.0f;
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            const float f = a == C ? s_fn[a][sl] : s_ft[a][sl];
-                            const float wa = d[a] == 0 ? 1.0f - f : f;
-                            w = a == 0 ? wa : w * wa;  // (wx * wy) * wz
-                        }
-                        marked[q] = 1;
-                        float val = s_v[C][sl];
-                        if (kApic) {
-                            float pc[3] = {s_x[0][sl], s_x[1][sl], s_x[2][sl]};
-#pragma unroll
-                            for (int a = 0; a < 3; ++a)
-                                if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
-                            val += s_c[3 * C][sl] * (gp[0] - pc[0]) + s_c[3 * C + 1][sl] * (gp[1] - pc[1]) +
-                                   s_c[3 * C + 2][sl] * (gp[2] - pc[2]);
-                        }
-                        num[q] += val * w;
-                        den[q] += w;
-                    }
+                for (int q = 0; q < kBatch; ++q) {
+                    px[q] = X[s + q];
+                    py[q] = X[cap + s + q];
+                    pz[q] = X[2 * cap + s + q];
+                    pv[q] = V[C * cap + s + q];
                 }
-            }
-        }
-        __syncthreads();
-    }
-    uint8_t* mk[3] = {mk0, mk1, mk2};
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        if (!has[q]) continue;
-        const int C = q % 3;
-        const int64_t f = (int64_t)F[q][0] +
-                          (int64_t)face_dim(g, C, 0) * ((int64_t)F[q][1] + (int64_t)face_dim(g, C, 1) * F[q][2]);
-        const float res = den[q] > 0.0f ? num[q] / den[q] : 0.0f;
-        out.a[C][f] = res;
-        mk[C][f] = (uint8_t)marked[q];
-        if (METHOD == FE_FLIP) snap.a[C][f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+                for (int q = 0; q < kBatch; ++q) consider(s + q, px[q], py[q], pz[q], pv[q]);
+            }
+            for (; s < s1; ++s) consider(s, X[s], X[cap + s], X[2 * cap + s], V[C * cap + s]);
+        }
     }
+    const float res = den > 0.0f ? num / den : 0.0f;
+    out[f] = res;
+    mark[f] = (uint8_t)marked;
+    if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
 }
 
 // ---------------------------------------------------------------------------
@@ -657,13 +529,14 @@ template <int METHOD>
 static hipError_t p2g_method(fe_scene* s) {
     const Particles& P = s->P[s->cur];
     const Geom& g = s->g;
-    const dim3 grid((unsigned)((g.n[0] + kPtx - 1) / kPtx), (unsigned)((g.n[1] + kPty - 1) / kPty),
-                    (unsigned)((g.n[2] + kPtz - 1) / kPtz));
-    FaceSet out{{s->u[0], s->u[1], s->u[2]}};
-    FaceSet snap{{s->snap[0], s->snap[1], s->snap[2]}};
-    launch_timed(s, FE_KERNEL_P2G, 0, k_p2g_tile<METHOD>, grid, dim3(kBlock), g, (const float*)P.x,
-                 (const float*)P.v, (const float*)P.C, s->cap, (const int32_t*)s->cell_start, out, snap, s->mark[0],
-                 s->mark[1], s->mark[2]);
+    for (int c = 0; c < 3; ++c) {
+        const int64_t nf = face_count(g, c);
+        const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
+        auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
+        launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.x,
+                     (const float*)P.v, (const float*)P.C, s->cap, (const int32_t*)s->cell_start, s->u[c],
+                     s->mark[c], s->snap[c], padded);
+    }
     return hipGetLastError();
 }
 

@@ -184,7 +184,7 @@ constexpr double kReliableDrop2 = 1e-4;  // (100x in ||r||)^2
 __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __restrict__ code,
                                                     const float* __restrict__ b, const double* __restrict__ x,
                                                     float* __restrict__ r, float* __restrict__ z,
-                                                    float* __restrict__ d0, SolverScalars* sc,
+                                                    float* __restrict__ d0, float* __restrict__ mgx, SolverScalars* sc,
                                                     const double* __restrict__ pbb, double* __restrict__ pout,
                                                     int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
                                                     int64_t padded) {
@@ -208,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
+        if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
         d0[c] = 0.0f;
         if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
                                                     float* __restrict__ dnew, float* __restrict__ q,
                                                     const SolverScalars* __restrict__ in,
                                                     SolverScalars* __restrict__ out, const double* __restrict__ pin,
-                                                    const double* __restrict__ pz, int npz, double* __restrict__ pout,
+                                                    const double* __restrict__ pz, double* __restrict__ pout,
                                                     int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
                                                     int64_t padded) {
     if (in->done) {  // uniform: an earlier spmv stopped the solve
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     sum_partials<2>(pin, nparts, t);
     if (pz) {
         double tz[1];
-        sum_partials<1>(pz, npz, tz);
+        sum_partials<1>(pz, nparts, tz);
         t[0] = tz[0];
     }
     const int it_prev = in->it;
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
                                                       const float* __restrict__ b, const double* __restrict__ xo,
                                                       double* __restrict__ xn, const float* __restrict__ d,
                                                       const float* __restrict__ q, float* __restrict__ r,
-                                                      float* __restrict__ z, SolverScalars* sc,
+                                                      float* __restrict__ z, float* __restrict__ mgx, SolverScalars* sc,
                                                       const double* __restrict__ pin, double* __restrict__ pout,
                                                       int nparts, float ihx2, float ihy2, float ihz2,
                                                       int64_t chunk, int64_t padded) {
@@ -337,6 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
+        if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
         if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
     }
@@ -425,7 +427,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     init.relative = d->relative;
     init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
     const bool mg = init.precond == 2;
-
+    double* PZ = s->partials + 4 * sh.blocks;  // r.z of the V-cycle, 1 per block
     init.max_it = d->max_iterations;
     init.it = -1;
     if ((e = hipMemcpyAsync(s->sc, &init, sizeof(SolverScalars), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
@@ -476,10 +478,9 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
 
     // CG / PCG
     if (mg && (e = mg_setup(s)) != hipSuccess) return e;
-    double* PZ = mg ? s->mg_pz : nullptr;  // r.z of the V-cycle
-    const int npz = mg ? (int)s->mg_npz : 0;
+    float* mgx = mg ? mg_first_sweep_target(s) : nullptr;
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
-                       s->r, s->z, s->d[0], s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+                       s->r, s->z, s->d[0], mgx, s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
     if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ, -1)) != hipSuccess) return e;
     double* xa = s->p;
     double* xb = s->p2;
@@ -498,7 +499,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         dcur = 1 - dcur;
         launch_timed(s, FE_KERNEL_CG_UPDATE, kk, k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                      (const uint16_t*)s->code, (const float*)s->b, (const double*)xa, xb, (const float*)s->d[dcur],
-                     (const float*)s->q, s->r, s->z, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
+                     (const float*)s->q, s->r, s->z, mgx, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
                      ih2[1], ih2[2], sh.chunk, sh.blocks);
         std::swap(xa, xb);
         if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk);
@@ -508,7 +509,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         for (int t = 0; t < nb; ++t) {
             launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                          (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
-                         (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, pz, npz,
+                         (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, pz,
                          PA, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
             if (t == nb - 1) break;
             update(k);

@@ -203,8 +203,7 @@ enum fe_kernel_id {
     FE_KERNEL_G2P_MOVE = 4,   /* fused G2P + advection */
     FE_KERNEL_SDF = 5,        /* fluid SDF */
     FE_KERNEL_JACOBI = 6,     /* one Jacobi sweep */
-    FE_KERNEL_MG_DOWN = 7,    /* finest-level V-cycle descent: 2 sweeps, residual, restriction */
-    FE_KERNEL_MG_UP = 8       /* finest-level V-cycle ascent: prolongation, 2 sweeps, r.z */
+    FE_KERNEL_MG_SMOOTH = 7   /* multigrid smoothing sweep on the finest level (3 per V-cycle) */
 };
 int fe_kernel_timer(fe_scene* scene, int32_t kernel_id);
 /* Total device ms and launch count since the last reset (reset != 0 clears). */
