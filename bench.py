@@ -67,10 +67,84 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--summarize-trace", metavar="STATS_CSV",
+                    help="write profiles/flip128_kernel_trace.md from a rocprofv3 --stats kernel summary and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/flip128_hbm_traffic.{json,md} from FETCH_SIZE/WRITE_SIZE passes and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (k_copy_probe), the reference kernel "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
+                  "g2p_move": "k_g2p_move", "sdf": "k_sdf"}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
+TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_kt -o kt -- "
+             "python3 bench.py --steps 20 --warmup 5 --no-cpu-baseline")
+PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{counter} -o pmc -- "
+           "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --copy-calibration 256")
+
+
+def short_name(kernel):
+    """'fe::k_cg_update(fe::Geom, ...)' -> 'k_cg_update'; template arguments dropped."""
+    name = kernel.split("(")[0].replace("void ", "").strip()
+    name = name.split("::")[-1]
+    return name.split("<")[0]
+
+
+def summarize_trace(stats_csv):
+    """Markdown table of a rocprofv3 --kernel-trace --stats summary."""
+    import csv
+
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = ["# FLIP 128^3 dam break: kernel trace (rocprofv3)", "",
+             "Command (repo root on the MI355X box; 5 warm-up + 20 timed substeps, 4,021,248 particles, "
+             "multigrid-PCG to 1e-6 relative residual, warm start):", "", "```", TRACE_CMD, "```", "",
+             f"Source: `{os.path.basename(stats_csv)}`, summarised by `python3 bench.py --summarize-trace <csv>`. "
+             "25 substeps in total (warm-up included).", "",
+             "| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append(f"| `{r['Name'].split('(')[0]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                     f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+    return "\n".join(lines) + "\n"
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    """HBM bytes per launch per kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
+    each scaled by what the counter reads for k_copy_probe (a plain copy of known size)."""
+    import csv
+
+    def load(path, counter):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            acc.setdefault(short_name(r["Kernel_Name"]), []).append(float(r["Counter_Value"]) * 1024.0)
+        return acc
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_bytes = 256 * 1024 * 1024
+    f_scale = copy_bytes / fetch["k_copy_probe"][0]
+    w_scale = copy_bytes / write["k_copy_probe"][0]
+    out = {"fetch_scale": f_scale, "write_scale": w_scale, "calibration": "k_copy_probe, 256 MiB read + 256 MiB "
+           "written", "commands": [PMC_CMD.format(counter=c) for c in ("FETCH_SIZE", "WRITE_SIZE")], "kernels": {}}
+    for k in sorted(fetch):
+        if k == "k_copy_probe" or k not in write:
+            continue
+        fb = sum(fetch[k]) / len(fetch[k]) * f_scale
+        wb = sum(write[k]) / len(write[k]) * w_scale
+        out["kernels"][k] = {"launches": len(fetch[k]), "read_bytes": round(fb), "write_bytes": round(wb),
+                             "bytes_per_launch": round(fb + wb)}
+    return out
+
+
+def traffic_for(kernel):
+    try:
+        with open(TRAFFIC_JSON) as f:
+            return json.load(f)["kernels"][KERNEL_SYMBOLS[kernel]]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def copy_calibration(ctx, mib):
@@ -81,6 +155,32 @@ def copy_calibration(ctx, mib):
 
 def main():
     args = parse()
+    if args.summarize_trace:
+        md = summarize_trace(args.summarize_trace)
+        with open(os.path.join(ROOT, "profiles", "flip128_kernel_trace.md"), "w") as f:
+            f.write(md)
+        print(md)
+        return
+    if args.summarize_pmc:
+        t = summarize_pmc(*args.summarize_pmc)
+        with open(TRAFFIC_JSON, "w") as f:
+            json.dump(t, f, indent=1)
+        lines = ["# FLIP 128^3 dam break: HBM traffic per kernel launch (rocprofv3 PMC)", "",
+                 "Commands (repo root on the MI355X box), one counter per run:", "", "```"] + t["commands"] + [
+                 "```", "", f"Scale factors from `k_copy_probe` (256 MiB read + 256 MiB written): FETCH_SIZE x "
+                 f"{t['fetch_scale']:.4f}, WRITE_SIZE x {t['write_scale']:.4f} (KiB counters). Summarised by "
+                 "`python3 bench.py --summarize-pmc <fetch_csv> <write_csv>`; machine-readable copy in "
+                 "`flip128_hbm_traffic.json` (read by bench.py for `roofline.traffic`).", "",
+                 "| kernel | launches | read MB/launch | write MB/launch | total MB/launch |",
+                 "|---|---:|---:|---:|---:|"]
+        for k, v in sorted(t["kernels"].items(), key=lambda kv: -kv[1]["bytes_per_launch"]):
+            lines.append(f"| `{k}` | {v['launches']} | {v['read_bytes'] / 1e6:.2f} | {v['write_bytes'] / 1e6:.2f} | "
+                         f"{v['bytes_per_launch'] / 1e6:.2f} |")
+        md = "\n".join(lines) + "\n"
+        with open(os.path.join(ROOT, "profiles", "flip128_hbm_traffic.md"), "w") as f:
+            f.write(md)
+        print(md)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a design only "
               "(z-slabs with face/pressure halos) -- run with --gpus 1", file=sys.stderr)
@@ -137,7 +237,8 @@ def main():
         bpl = kernel_bytes_per_cell(args.kernel, args.solver) * ncells
         achieved = bpl / (kavg_ms * 1e-3) / 1e9
         roof = {"kernel": args.kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_for(args.kernel) if (args.n == 128 and args.solver == "mgpcg") else None,
                 "avg_launch_us": round(kavg_ms * 1e3, 3), "launches": klaunch,
                 "bytes_per_launch": bpl}
 
