@@ -27,7 +27,7 @@ EXPORTED = [
     "fe_scene_sync", "fe_particles_upload", "fe_particles_download", "fe_particles_count", "fe_substep",
     "fe_advance_frame", "fe_cfl", "fe_run_stage", "fe_grid_download", "fe_grid_upload", "fe_grid_bytes",
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
-    "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe",
+    "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -101,6 +101,7 @@ def load():
     L.fe_last_solve_stats.argtypes = [vp, C.POINTER(SolveStats)]
     L.fe_kernel_timer_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
     L.fe_copy_probe.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
+    L.fe_scene_set_params.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double]
     L.fe_version.restype = C.c_char_p
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name in ("fe_last_error", "fe_version") else C.c_int

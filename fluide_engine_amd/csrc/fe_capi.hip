@@ -374,6 +374,24 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     return FE_OK;
 }
 
+int fe_scene_set_params(fe_scene* s, double blend, double max_cfl, const double gravity[3], double viscosity,
+                        double density) {
+    if (!s || !gravity) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (viscosity != 0.0)
+        return set_err(ctx, FE_E_UNSUPPORTED, "viscosity > 0 (BackwardEulerDiffusionSolver) is not on this backend");
+    if (!(max_cfl >= 0.0) || max_cfl > 64) return set_err(ctx, FE_E_INVALID, "max_cfl out of range");
+    if (!(blend >= 0.0 && blend <= 1.0)) return set_err(ctx, FE_E_INVALID, "blend must be in [0, 1]");
+    s->desc.blend = blend;
+    s->desc.max_cfl = max_cfl;
+    for (int a = 0; a < 3; ++a) s->desc.gravity[a] = gravity[a];
+    s->desc.viscosity = viscosity;
+    s->desc.density = density;
+    s->depth = (int)std::ceil(max_cfl);                        // ceil(maxCfl), src/pic_simulator.cpp:491
+    s->move_substeps = (int)(unsigned)std::max(max_cfl, 1.0);  // src/pic_simulator.cpp:255
+    return FE_OK;
+}
+
 int fe_scene_destroy(fe_scene* s) {
     if (!s) return FE_OK;
     if (s->ctx) (void)hipSetDevice(s->ctx->device);

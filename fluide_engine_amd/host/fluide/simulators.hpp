@@ -1,0 +1,245 @@
+// simulators.hpp -- the reference's simulator classes over the gfx950 backend.
+//
+// HipPICSimulator / HipFLIPSimulator / HipAPICSimulator keep the method set of
+// PhysicsAnimation (src/physics_animation.hpp:6-38), HybridSimulator
+// (src/hybrid_simulator.hpp:12-52), PICSimulator (src/pic_simulator.hpp:29-43)
+// and FLIPSimulator (src/flip_simulator.hpp:14-16), so code written against
+// CudaPICSimulator / CudaFLIPSimulator / CudaAPICSimulator
+// (src/cuda_pic_simulator.hpp:29-58) compiles against these through the
+// Cuda* aliases at the end.  The state lives on the GPU in one fe_scene
+// (include/fluide_hip.h); host copies are made only by the explicit
+// particle/grid accessors.
+//
+// Not on this backend (each throws Unsupported): triangle-mesh scenes
+// (InitializeFromTriangleMesh, GetSurface), colliders, emitters, viscosity > 0,
+// and external forces other than a uniform DirectionalField.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fluide_hip.h"
+
+namespace fluide {
+
+// src/common/vector3.hpp (the members the simulators use)
+template <typename T>
+struct Vector3 {
+    T x, y, z;
+    Vector3(T v = T(0)) : x(v), y(v), z(v) {}
+    Vector3(T x_, T y_, T z_) : x(x_), y(y_), z(z_) {}
+};
+
+// src/3d/bounding_box_3d.hpp
+class BoundingBox3D {
+  public:
+    BoundingBox3D(Vector3<double> origin = 0, Vector3<double> size = 1) : _origin(origin), _size(size) {}
+    bool IsInside(Vector3<double> p) const;
+    Vector3<double> Corner(size_t idx) const;
+    Vector3<double> GetOrigin() const { return _origin; }
+    Vector3<double> GetSize() const { return _size; }
+    void SetOrigin(Vector3<double> origin) { _origin = origin; }
+    void SetSize(Vector3<double> size) { _size = size; }
+
+  private:
+    Vector3<double> _origin;
+    Vector3<double> _size;
+};
+
+// src/common/frame.hpp: frame index and interval; time = index * interval.
+class Frame {
+  public:
+    Frame() = default;
+    explicit Frame(double timeIntervalInSeconds) : _timeIntervalInSeconds(timeIntervalInSeconds) {}
+    void Advance() { ++_index; }
+    void Advance(int numberOfFrames) { _index += numberOfFrames; }
+    Frame& operator++() {
+        Advance();
+        return *this;
+    }
+    Frame operator++(int) {
+        Frame r = *this;
+        Advance();
+        return r;
+    }
+    unsigned long GetIndex() const { return _index; }
+    double GetTimeIntervalInSeconds() const { return _timeIntervalInSeconds; }
+    double GetTimeInSeconds() const { return _index * _timeIntervalInSeconds; }
+
+  private:
+    unsigned long _index = 0;
+    double _timeIntervalInSeconds = 1 / 60.0f;
+};
+
+// src/forces/external_force.hpp, src/forces/directional_field.hpp
+class ExternalForce {
+  public:
+    virtual ~ExternalForce() = default;
+    virtual Vector3<double> Sample(const Vector3<double>& position) const = 0;
+};
+
+class DirectionalField : public ExternalForce {
+  public:
+    explicit DirectionalField(Vector3<double> strength) : _strength(strength) {}
+    Vector3<double> Sample(const Vector3<double>&) const override { return _strength; }
+    void SetStrength(Vector3<double> strength) { _strength = strength; }
+
+  private:
+    Vector3<double> _strength;
+};
+
+// Host copy of a cell-centred scalar grid (src/3d/scalar_grid3d.hpp): x-fastest.
+struct ScalarGrid3D {
+    Vector3<size_t> size;
+    Vector3<double> origin, spacing;
+    std::vector<double> data;
+    double operator()(size_t i, size_t j, size_t k) const { return data[i + size.x * (j + size.y * k)]; }
+    Vector3<size_t> GetSize() const { return size; }
+    Vector3<double> GetOrigin() const { return origin; }
+    Vector3<double> GetGridSpacing() const { return spacing; }
+};
+
+// Pressure-solver selection (SetPressureSolver / SetLinearSystemSolver,
+// src/fluid_solvers/single_phase_pressure_solver.hpp:21).  The default is the
+// reference's CudaJacobiIterationSolver(1000, 5, 1e-14)
+// (src/fluid_solvers/single_phase_pressure_solver.cpp:11).
+struct PressureSolverSettings {
+    int kind = FE_SOLVER_JACOBI;  // fe_solver_kind
+    int maxIterations = 1000;
+    int checkInterval = 5;
+    double tolerance = 1e-14;
+    bool relative = false;
+    bool warmStart = true;
+};
+
+class Error : public std::runtime_error {
+  public:
+    Error(const std::string& what, int code) : std::runtime_error(what), code(code) {}
+    int code;
+};
+
+class Unsupported : public std::logic_error {
+  public:
+    using std::logic_error::logic_error;
+};
+
+// One fe_ctx per process and device, shared by the simulators on it.
+class Device;
+
+class HipHybridSimulator {
+  public:
+    HipHybridSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain, int method);
+    virtual ~HipHybridSimulator();
+    HipHybridSimulator(const HipHybridSimulator&) = delete;
+    HipHybridSimulator& operator=(const HipHybridSimulator&) = delete;
+
+    // PhysicsAnimation (src/physics_animation.cpp): frame -> AdvanceTimeStep ->
+    // CFL-limited substeps (fe_advance_frame).
+    void AdvanceSingleFrame();
+    void Update(const Frame& frame);
+    Frame GetCurrentFrame() const { return _frame; }
+    double GetCurrentTimeInSeconds() const { return _time; }
+    unsigned int GetNumberOfSubTimeSteps() const { return _substeps; }
+    void SetCurrentFrame(const Frame& frame) { _frame = frame; }
+
+    // HybridSimulator
+    double Cfl(double timeIntervalInSeconds) const;
+    void AddExternalForce(std::shared_ptr<ExternalForce> force);
+    double MaxCfl() const { return _maxCfl; }
+    double GetViscosity() const { return _viscosity; }
+    double GetDensity() const { return _density; }
+    size_t GetParticlesPerBlock() const { return _particlesPerBlock; }
+    Vector3<double> GetOrigin() const { return _domain.GetOrigin(); }
+    Vector3<size_t> GetResolution() const { return _resolution; }
+    Vector3<double> GetGridSpacing() const;
+    const ScalarGrid3D& GetFluidSdf() const;  // downloads the current SDF
+    size_t GetNumberOfParticles() const;
+    void SetMaxClf(double maxCfl);
+    void SetViscosity(double viscosity);
+    void SetDensity(double density);
+    void SetParticlesPerBlock(size_t particlesPerBlock) { _particlesPerBlock = particlesPerBlock; }
+
+    // PICSimulator
+    void SetPressureSolver(const PressureSolverSettings& settings);
+    PressureSolverSettings GetPressureSolver() const { return _solver; }
+
+    // Not on this backend.
+    void AddCollider(const void*) { throw Unsupported("colliders are not on the GPU backend"); }
+    void AddEmitter(const void*) { throw Unsupported("emitters are not on the GPU backend"); }
+    void InitializeFromTriangleMesh(const void*) {
+        throw Unsupported("triangle-mesh scenes are not on the GPU backend; use InitializeParticles");
+    }
+    void GetSurface(void*) { throw Unsupported("surface meshing is not on the GPU backend"); }
+
+    // Added for the GPU backend: particle and grid transfer, analytic scenes.
+    void InitializeParticles(const std::vector<fe_seed_shape>& shapes, uint64_t seed);
+    void SetParticles(const float* x, const float* v, const float* C, int64_t count);  // xyz-interleaved
+    void GetParticles(std::vector<float>* x, std::vector<float>* v, std::vector<float>* C) const;
+    size_t GridBytes(int field) const;
+    void GetGrid(int field, void* dst, size_t bytes) const;
+    void SetGrid(int field, const void* src, size_t bytes);
+    fe_solve_stats LastSolveStats() const;
+    int Method() const { return _method; }
+    fe_scene* Handle() const { return _scene; }
+
+  protected:
+    void SetBlend(double blend);
+    double Blend() const { return _blend; }
+
+  private:
+    void EnsureScene(int64_t capacity) const;
+    void PushParams() const;
+    fe_scene_desc Desc(int64_t capacity) const;
+    void Check(int rc, const char* what) const;
+
+    Vector3<size_t> _resolution;
+    BoundingBox3D _domain;
+    int _method;
+    double _maxCfl = 3;  // src/hybrid_simulator.cpp:21
+    double _viscosity = 0;
+    double _density = 1;
+    size_t _particlesPerBlock = 16;
+    double _blend = 0.0;
+    Vector3<double> _gravity = 0;
+    PressureSolverSettings _solver;
+    Frame _frame;
+    double _time = 0.0;
+    unsigned int _substeps = 0;
+    std::shared_ptr<Device> _device;
+    mutable fe_scene* _scene = nullptr;
+    mutable int64_t _capacity = 0;
+    mutable ScalarGrid3D _sdf;
+};
+
+class HipPICSimulator : public HipHybridSimulator {
+  public:
+    HipPICSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
+        : HipHybridSimulator(gridSize, domain, FE_PIC) {}
+};
+
+class HipFLIPSimulator : public HipHybridSimulator {
+  public:
+    HipFLIPSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
+        : HipHybridSimulator(gridSize, domain, FE_FLIP) {
+        SetBlend(0.05);  // _blendingFactor(0.05), src/flip_simulator.cpp:5
+    }
+    void SetBlendingFactor(double factor) { SetBlend(factor); }
+    double GetBlendingFactor() const { return Blend(); }
+};
+
+class HipAPICSimulator : public HipHybridSimulator {
+  public:
+    HipAPICSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
+        : HipHybridSimulator(gridSize, domain, FE_APIC) {}
+};
+
+// The drop-in names of the reference's CUDA simulators.
+using CudaPICSimulator = HipPICSimulator;
+using CudaFLIPSimulator = HipFLIPSimulator;
+using CudaAPICSimulator = HipAPICSimulator;
+
+}  // namespace fluide

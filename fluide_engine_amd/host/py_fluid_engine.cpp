@@ -1,0 +1,213 @@
+// py_fluid_engine.cpp -- module PyFluidEngine with the class and method names
+// of the reference's pybind11 wrapper (src/python_wrapper/main.cpp:20-33,
+// vector3_wrapper.cpp, bounding_box_3d_wrapper.cpp, frame_wrapper.cpp,
+// exteranl_forces_wrapper.cpp, simulators_wrapper.cu:36-85), backed by the
+// gfx950 simulators.  Added: particle/grid transfer as numpy arrays, analytic
+// scene initialisation and the pressure-solver selection.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "fluide/simulators.hpp"
+
+namespace py = pybind11;
+using namespace fluide;
+
+namespace {
+
+using FArr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+fe_seed_shape box(std::vector<double> lo, std::vector<double> hi) {
+    if (lo.size() != 3 || hi.size() != 3) throw py::value_error("box corners need 3 coordinates");
+    fe_seed_shape s{};
+    s.kind = 0;
+    for (int a = 0; a < 3; ++a) {
+        s.lo[a] = lo[a];
+        s.hi[a] = hi[a];
+    }
+    return s;
+}
+
+fe_seed_shape sphere(std::vector<double> centre, double radius) {
+    if (centre.size() != 3) throw py::value_error("sphere centre needs 3 coordinates");
+    fe_seed_shape s{};
+    s.kind = 1;
+    for (int a = 0; a < 3; ++a) s.lo[a] = centre[a];
+    s.hi[0] = radius;
+    return s;
+}
+
+void set_particles(HipHybridSimulator& sim, FArr x, FArr v, py::object C) {
+    if (x.ndim() != 2 || x.shape(1) != 3) throw py::value_error("x must be (n, 3)");
+    if (v.ndim() != 2 || v.shape(1) != 3 || v.shape(0) != x.shape(0)) throw py::value_error("v must be (n, 3)");
+    FArr c;
+    const float* cp = nullptr;
+    if (!C.is_none()) {
+        c = C.cast<FArr>();
+        if (c.ndim() != 2 || c.shape(1) != 9 || c.shape(0) != x.shape(0)) throw py::value_error("C must be (n, 9)");
+        cp = c.data();
+    }
+    sim.SetParticles(x.data(), v.data(), cp, (int64_t)x.shape(0));
+}
+
+py::tuple get_particles(const HipHybridSimulator& sim) {
+    std::vector<float> x, v, C;
+    sim.GetParticles(&x, &v, &C);
+    const py::ssize_t n = (py::ssize_t)(x.size() / 3);
+    FArr ax({n, (py::ssize_t)3}), av({n, (py::ssize_t)3});
+    std::copy(x.begin(), x.end(), ax.mutable_data());
+    std::copy(v.begin(), v.end(), av.mutable_data());
+    if (sim.Method() != FE_APIC) return py::make_tuple(ax, av, py::none());
+    FArr ac({n, (py::ssize_t)9});
+    std::copy(C.begin(), C.end(), ac.mutable_data());
+    return py::make_tuple(ax, av, ac);
+}
+
+// Grid field as a numpy array shaped (nz, ny, nx) of the field's own dims.
+py::array get_grid(const HipHybridSimulator& sim, int field) {
+    const Vector3<size_t> n = sim.GetResolution();
+    size_t d[3] = {n.x, n.y, n.z};
+    if (field >= FE_GRID_U && field <= FE_GRID_W) d[field] += 1;
+    if (field >= FE_GRID_MU && field <= FE_GRID_MW) d[field - FE_GRID_MU] += 1;
+    if (field >= FE_GRID_SU && field <= FE_GRID_SW) d[field - FE_GRID_SU] += 1;
+    const std::vector<py::ssize_t> shape = {(py::ssize_t)d[2], (py::ssize_t)d[1], (py::ssize_t)d[0]};
+    py::array out;
+    if (field == FE_GRID_SDF || field == FE_GRID_PRESSURE) out = py::array_t<double>(shape);
+    else if ((field >= FE_GRID_MU && field <= FE_GRID_MW) || field == FE_GRID_FLAGS) out = py::array_t<uint8_t>(shape);
+    else out = py::array_t<float>(shape);
+    sim.GetGrid(field, out.mutable_data(), (size_t)out.nbytes());
+    return out;
+}
+
+py::dict stats(const HipHybridSimulator& sim) {
+    const fe_solve_stats s = sim.LastSolveStats();
+    py::dict d;
+    d["iterations"] = s.iterations;
+    d["converged"] = s.converged != 0;
+    d["rel_residual"] = s.rel_residual;
+    d["abs_residual"] = s.abs_residual;
+    d["ms"] = s.ms;
+    return d;
+}
+
+void set_solver(HipHybridSimulator& sim, const std::string& kind, int max_iterations, double tolerance, bool relative,
+                bool warm_start, int check_interval) {
+    PressureSolverSettings s;
+    if (kind == "jacobi") s.kind = FE_SOLVER_JACOBI;
+    else if (kind == "cg") s.kind = FE_SOLVER_CG;
+    else if (kind == "pcg") s.kind = FE_SOLVER_PCG_JACOBI;
+    else if (kind == "mgpcg") s.kind = FE_SOLVER_MGPCG;
+    else throw py::value_error("solver kind must be one of jacobi, cg, pcg, mgpcg");
+    s.maxIterations = max_iterations;
+    s.tolerance = tolerance;
+    s.relative = relative;
+    s.warmStart = warm_start;
+    s.checkInterval = check_interval;
+    sim.SetPressureSolver(s);
+}
+
+template <typename Sim, typename Base>
+py::class_<Sim, std::shared_ptr<Sim>, Base> add_sim(py::module& m, const char* name) {
+    auto c = py::class_<Sim, std::shared_ptr<Sim>, Base>(m, name)
+                 .def(py::init<Vector3<size_t>, BoundingBox3D>())
+                 .def("AdvanceSingleFrame", &Sim::AdvanceSingleFrame)
+                 .def("InitializeFromTriangleMesh",
+                      [](Sim& s, py::object) { s.InitializeFromTriangleMesh(nullptr); })
+                 .def("AddExternalForce", &Sim::AddExternalForce)
+                 .def("AddCollider", [](Sim& s, py::object) { s.AddCollider(nullptr); })
+                 .def("AddEmitter", [](Sim& s, py::object) { s.AddEmitter(nullptr); })
+                 .def("SetCurrentFrame", &Sim::SetCurrentFrame)
+                 .def("SetViscosity", &Sim::SetViscosity)
+                 .def("SetMaxClf", &Sim::SetMaxClf)
+                 .def("GetSurface", [](Sim& s, py::object) { s.GetSurface(nullptr); });
+    return c;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(PyFluidEngine, m) {
+    m.doc() = "Fluide_Engine hybrid particle-grid simulators on the gfx950 (MI355X) backend";
+    py::register_exception<Unsupported>(m, "Unsupported", PyExc_NotImplementedError);
+    py::register_exception<Error>(m, "FluideError", PyExc_RuntimeError);
+
+    py::class_<Frame>(m, "Frame")
+        .def(py::init<double>())
+        .def(py::init<>())
+        .def("GetIndex", &Frame::GetIndex)
+        .def("GetTimeIntervalInSeconds", &Frame::GetTimeIntervalInSeconds)
+        .def("GetTimeInSeconds", &Frame::GetTimeInSeconds)
+        .def("Advance", py::overload_cast<>(&Frame::Advance));
+
+    py::class_<Vector3<double>>(m, "Vector3D")
+        .def(py::init<double, double, double>())
+        .def_readwrite("x", &Vector3<double>::x)
+        .def_readwrite("y", &Vector3<double>::y)
+        .def_readwrite("z", &Vector3<double>::z);
+    py::class_<Vector3<size_t>>(m, "Vector3S")
+        .def(py::init<size_t, size_t, size_t>())
+        .def_readwrite("x", &Vector3<size_t>::x)
+        .def_readwrite("y", &Vector3<size_t>::y)
+        .def_readwrite("z", &Vector3<size_t>::z);
+
+    py::class_<BoundingBox3D>(m, "BoundingBox3D")
+        .def(py::init<Vector3<double>, Vector3<double>>())
+        .def("GetOrigin", &BoundingBox3D::GetOrigin)
+        .def("GetSize", &BoundingBox3D::GetSize);
+
+    py::class_<ExternalForce, std::shared_ptr<ExternalForce>>(m, "ExternalForce");
+    py::class_<DirectionalField, std::shared_ptr<DirectionalField>, ExternalForce>(m, "DirectionalField")
+        .def(py::init<Vector3<double>>());
+
+    py::class_<HipHybridSimulator, std::shared_ptr<HipHybridSimulator>>(m, "HybridSimulator")
+        .def("MaxCfl", &HipHybridSimulator::MaxCfl)
+        .def("GetViscosity", &HipHybridSimulator::GetViscosity)
+        .def("GetDensity", &HipHybridSimulator::GetDensity)
+        .def("GetParticlesPerBlock", &HipHybridSimulator::GetParticlesPerBlock)
+        .def("GetOrigin", &HipHybridSimulator::GetOrigin)
+        .def("GetResolution", &HipHybridSimulator::GetResolution)
+        .def("GetGridSpacing", &HipHybridSimulator::GetGridSpacing)
+        .def("GetNumberOfParticles", &HipHybridSimulator::GetNumberOfParticles)
+        .def("GetCurrentFrame", &HipHybridSimulator::GetCurrentFrame)
+        .def("GetCurrentTimeInSeconds", &HipHybridSimulator::GetCurrentTimeInSeconds)
+        .def("GetNumberOfSubTimeSteps", &HipHybridSimulator::GetNumberOfSubTimeSteps)
+        .def("Cfl", &HipHybridSimulator::Cfl)
+        .def("SetDensity", &HipHybridSimulator::SetDensity)
+        .def("SetParticlesPerBlock", &HipHybridSimulator::SetParticlesPerBlock)
+        .def("GetFluidSdf",
+             [](const HipHybridSimulator& s) {
+                 const ScalarGrid3D& g = s.GetFluidSdf();
+                 py::array_t<double> a({(py::ssize_t)g.size.z, (py::ssize_t)g.size.y, (py::ssize_t)g.size.x});
+                 std::copy(g.data.begin(), g.data.end(), a.mutable_data());
+                 return a;
+             })
+        // GPU-backend additions
+        .def("InitializeParticles", &HipHybridSimulator::InitializeParticles, py::arg("shapes"), py::arg("seed") = 0)
+        .def("SetParticles", &set_particles, py::arg("x"), py::arg("v"), py::arg("C") = py::none())
+        .def("GetParticles", &get_particles)
+        .def("GetGrid", &get_grid)
+        .def("LastSolveStats", &stats)
+        .def("SetPressureSolver", &set_solver, py::arg("kind"), py::arg("max_iterations") = 1000,
+             py::arg("tolerance") = 1e-14, py::arg("relative") = false, py::arg("warm_start") = true,
+             py::arg("check_interval") = 5);
+
+    py::class_<fe_seed_shape>(m, "SeedShape");
+    m.def("Box", &box, py::arg("lo"), py::arg("hi"), "Particle-seeding box [lo, hi)");
+    m.def("Sphere", &sphere, py::arg("centre"), py::arg("radius"), "Particle-seeding sphere");
+
+    auto pic = add_sim<HipPICSimulator, HipHybridSimulator>(m, "PICSimulator");
+    add_sim<HipAPICSimulator, HipHybridSimulator>(m, "APICSimulator");
+    add_sim<HipFLIPSimulator, HipHybridSimulator>(m, "FLIPSimulator")
+        .def("SetBlendingFactor", &HipFLIPSimulator::SetBlendingFactor)
+        .def("GetBlendingFactor", &HipFLIPSimulator::GetBlendingFactor);
+    (void)pic;
+    m.attr("CudaPICSimulator") = m.attr("PICSimulator");
+    m.attr("CudaFLIPSimulator") = m.attr("FLIPSimulator");
+    m.attr("CudaAPICSimulator") = m.attr("APICSimulator");
+
+    m.attr("GRID_U") = (int)FE_GRID_U;
+    m.attr("GRID_V") = (int)FE_GRID_V;
+    m.attr("GRID_W") = (int)FE_GRID_W;
+    m.attr("GRID_SDF") = (int)FE_GRID_SDF;
+    m.attr("GRID_FLAGS") = (int)FE_GRID_FLAGS;
+    m.attr("GRID_PRESSURE") = (int)FE_GRID_PRESSURE;
+}

@@ -1,0 +1,243 @@
+// simulators.cpp -- host classes of fluide/simulators.hpp over the C ABI.
+#include "fluide/simulators.hpp"
+
+#include <cmath>
+#include <map>
+#include <mutex>
+
+namespace fluide {
+
+bool BoundingBox3D::IsInside(Vector3<double> p) const {
+    return p.x >= _origin.x && p.x <= _origin.x + _size.x && p.y >= _origin.y && p.y <= _origin.y + _size.y &&
+           p.z >= _origin.z && p.z <= _origin.z + _size.z;
+}
+
+Vector3<double> BoundingBox3D::Corner(size_t idx) const {
+    return Vector3<double>(_origin.x + ((idx & 1) ? _size.x : 0.0), _origin.y + ((idx & 2) ? _size.y : 0.0),
+                           _origin.z + ((idx & 4) ? _size.z : 0.0));
+}
+
+class Device {
+  public:
+    static std::shared_ptr<Device> Get(int device) {
+        static std::mutex m;
+        static std::map<int, std::weak_ptr<Device>> all;
+        std::lock_guard<std::mutex> lock(m);
+        std::shared_ptr<Device> d = all[device].lock();
+        if (!d) {
+            d = std::shared_ptr<Device>(new Device(device));
+            all[device] = d;
+        }
+        return d;
+    }
+    ~Device() {
+        if (ctx) fe_ctx_destroy(ctx);
+    }
+    fe_ctx* Context() {
+        if (!ctx) {
+            const int rc = fe_ctx_create(id, &ctx);
+            if (rc != FE_OK) throw Error(std::string("fe_ctx_create: ") + fe_last_error(nullptr), rc);
+        }
+        return ctx;
+    }
+
+  private:
+    explicit Device(int device) : id(device) {}
+    int id;
+    fe_ctx* ctx = nullptr;
+};
+
+HipHybridSimulator::HipHybridSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain, int method)
+    : _resolution(gridSize), _domain(domain), _method(method), _device(Device::Get(0)) {}
+
+HipHybridSimulator::~HipHybridSimulator() {
+    if (_scene) fe_scene_destroy(_scene);
+}
+
+void HipHybridSimulator::Check(int rc, const char* what) const {
+    if (rc == FE_OK) return;
+    const char* msg = fe_last_error(_device->Context());
+    throw Error(std::string(what) + ": " + (msg ? msg : "error"), rc);
+}
+
+fe_scene_desc HipHybridSimulator::Desc(int64_t capacity) const {
+    fe_scene_desc d{};
+    const size_t n[3] = {_resolution.x, _resolution.y, _resolution.z};
+    const Vector3<double> o = _domain.GetOrigin(), L = _domain.GetSize();
+    for (int a = 0; a < 3; ++a) d.n[a] = (int32_t)n[a];
+    d.method = _method;
+    d.origin[0] = o.x;
+    d.origin[1] = o.y;
+    d.origin[2] = o.z;
+    d.size[0] = L.x;
+    d.size[1] = L.y;
+    d.size[2] = L.z;
+    d.blend = _blend;
+    d.max_cfl = _maxCfl;
+    d.gravity[0] = _gravity.x;
+    d.gravity[1] = _gravity.y;
+    d.gravity[2] = _gravity.z;
+    d.viscosity = _viscosity;
+    d.density = _density;
+    d.max_particles = capacity;
+    return d;
+}
+
+void HipHybridSimulator::EnsureScene(int64_t capacity) const {
+    if (_scene && _capacity >= capacity) return;
+    fe_ctx* ctx = _device->Context();
+    if (_scene) {
+        fe_scene_destroy(_scene);
+        _scene = nullptr;
+    }
+    const fe_scene_desc d = Desc(std::max<int64_t>(capacity, 1));
+    Check(fe_scene_create(ctx, &d, &_scene), "fe_scene_create");
+    _capacity = d.max_particles;
+}
+
+void HipHybridSimulator::PushParams() const {
+    if (!_scene) return;
+    const double g[3] = {_gravity.x, _gravity.y, _gravity.z};
+    Check(fe_scene_set_params(_scene, _blend, _maxCfl, g, _viscosity, _density), "fe_scene_set_params");
+}
+
+// PhysicsAnimation::OnUpdate / AdvanceTimeStep (src/physics_animation.cpp:45-92).
+void HipHybridSimulator::Update(const Frame& frame) {
+    if (frame.GetIndex() <= _frame.GetIndex()) return;
+    EnsureScene(0);
+    const unsigned long frames = frame.GetIndex() - _frame.GetIndex();
+    const fe_solver_desc sd{_solver.kind,       _solver.maxIterations, _solver.checkInterval, _solver.relative ? 1 : 0,
+                            _solver.warmStart ? 1 : 0, 0,              _solver.tolerance};
+    for (unsigned long f = 0; f < frames; ++f) {
+        _time = _frame.GetTimeInSeconds();
+        int32_t sub = 0;
+        Check(fe_advance_frame(_scene, frame.GetTimeIntervalInSeconds(), &sd, &sub), "fe_advance_frame");
+        _substeps = (unsigned int)sub;
+        _time += frame.GetTimeIntervalInSeconds();
+    }
+    _frame = frame;
+}
+
+void HipHybridSimulator::AdvanceSingleFrame() {
+    Frame f = _frame;
+    Update(++f);
+}
+
+double HipHybridSimulator::Cfl(double dt) const {
+    EnsureScene(0);
+    double c = 0.0;
+    Check(fe_cfl(_scene, dt, &c), "fe_cfl");
+    return c;
+}
+
+// PICSimulator keeps a force list and applies each to every face
+// (src/forces/external_force.cpp:4-30); a uniform field sums into gravity.
+void HipHybridSimulator::AddExternalForce(std::shared_ptr<ExternalForce> force) {
+    if (!force) return;
+    const Vector3<double> s = force->Sample(_domain.GetOrigin());
+    for (size_t c = 1; c < 8; ++c) {
+        const Vector3<double> t = force->Sample(_domain.Corner(c));
+        if (t.x != s.x || t.y != s.y || t.z != s.z)
+            throw Unsupported("only uniform (DirectionalField) forces are on the GPU backend");
+    }
+    _gravity = Vector3<double>(_gravity.x + s.x, _gravity.y + s.y, _gravity.z + s.z);
+    PushParams();
+}
+
+Vector3<double> HipHybridSimulator::GetGridSpacing() const {
+    const Vector3<double> L = _domain.GetSize();
+    return Vector3<double>(L.x / _resolution.x, L.y / _resolution.y, L.z / _resolution.z);  // src/hybrid_simulator.cpp:7
+}
+
+const ScalarGrid3D& HipHybridSimulator::GetFluidSdf() const {
+    EnsureScene(0);
+    const Vector3<double> h = GetGridSpacing(), o = _domain.GetOrigin();
+    _sdf.size = _resolution;
+    _sdf.spacing = h;
+    _sdf.origin = Vector3<double>(o.x + h.x / 2.0, o.y + h.y / 2.0, o.z + h.z / 2.0);  // src/hybrid_simulator.cpp:15
+    _sdf.data.resize(_resolution.x * _resolution.y * _resolution.z);
+    Check(fe_grid_download(_scene, FE_GRID_SDF, _sdf.data.data(), _sdf.data.size() * sizeof(double)),
+          "fe_grid_download");
+    return _sdf;
+}
+
+size_t HipHybridSimulator::GetNumberOfParticles() const {
+    if (!_scene) return 0;
+    int64_t n = 0;
+    Check(fe_particles_count(_scene, &n), "fe_particles_count");
+    return (size_t)n;
+}
+
+void HipHybridSimulator::SetMaxClf(double maxCfl) {
+    _maxCfl = maxCfl;
+    PushParams();
+}
+
+void HipHybridSimulator::SetViscosity(double viscosity) {
+    if (viscosity != 0.0) throw Unsupported("viscosity > 0 (BackwardEulerDiffusionSolver) is not on the GPU backend");
+    _viscosity = viscosity;
+    PushParams();
+}
+
+void HipHybridSimulator::SetDensity(double density) {
+    _density = density;
+    PushParams();
+}
+
+void HipHybridSimulator::SetBlend(double blend) {
+    _blend = blend;
+    PushParams();
+}
+
+void HipHybridSimulator::SetPressureSolver(const PressureSolverSettings& settings) { _solver = settings; }
+
+void HipHybridSimulator::InitializeParticles(const std::vector<fe_seed_shape>& shapes, uint64_t seed) {
+    const fe_scene_desc d = Desc(0);
+    int64_t n = 0;
+    Check(fe_seed_particles(&d, shapes.data(), (int32_t)shapes.size(), seed, nullptr, 0, &n), "fe_seed_particles");
+    std::vector<float> x((size_t)n * 3), v((size_t)n * 3, 0.0f), C;
+    Check(fe_seed_particles(&d, shapes.data(), (int32_t)shapes.size(), seed, x.data(), n, &n), "fe_seed_particles");
+    if (_method == FE_APIC) C.assign((size_t)n * 9, 0.0f);
+    SetParticles(x.data(), v.data(), C.empty() ? nullptr : C.data(), n);
+}
+
+void HipHybridSimulator::SetParticles(const float* x, const float* v, const float* C, int64_t count) {
+    EnsureScene(count);
+    Check(fe_particles_upload(_scene, x, v, C, count), "fe_particles_upload");
+}
+
+void HipHybridSimulator::GetParticles(std::vector<float>* x, std::vector<float>* v, std::vector<float>* C) const {
+    const size_t n = GetNumberOfParticles();
+    if (x) x->resize(3 * n);
+    if (v) v->resize(3 * n);
+    if (C) C->resize(_method == FE_APIC ? 9 * n : 0);
+    if (n == 0) return;
+    Check(fe_particles_download(_scene, x ? x->data() : nullptr, v ? v->data() : nullptr,
+                                (C && _method == FE_APIC) ? C->data() : nullptr, (int64_t)n),
+          "fe_particles_download");
+}
+
+size_t HipHybridSimulator::GridBytes(int field) const {
+    EnsureScene(0);
+    size_t b = 0;
+    Check(fe_grid_bytes(_scene, field, &b), "fe_grid_bytes");
+    return b;
+}
+
+void HipHybridSimulator::GetGrid(int field, void* dst, size_t bytes) const {
+    EnsureScene(0);
+    Check(fe_grid_download(_scene, field, dst, bytes), "fe_grid_download");
+}
+
+void HipHybridSimulator::SetGrid(int field, const void* src, size_t bytes) {
+    EnsureScene(0);
+    Check(fe_grid_upload(_scene, field, src, bytes), "fe_grid_upload");
+}
+
+fe_solve_stats HipHybridSimulator::LastSolveStats() const {
+    fe_solve_stats st{};
+    if (_scene) Check(fe_last_solve_stats(_scene, &st), "fe_last_solve_stats");
+    return st;
+}
+
+}  // namespace fluide

@@ -137,6 +137,13 @@ int fe_ctx_device_name(const fe_ctx* ctx, char* buf, size_t len);
 int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* desc, fe_scene** out);
 int fe_scene_destroy(fe_scene* scene);
 int fe_scene_sync(fe_scene* scene);
+/* Simulator settings changed after construction: SetBlendingFactor
+ * (src/flip_simulator.hpp:14-16), SetMaxClf, SetViscosity, SetDensity
+ * (src/hybrid_simulator.hpp:37-40) and the DirectionalField forces of
+ * AddExternalForce, summed (src/pic_simulator.cpp:41-44).  viscosity != 0 is
+ * FE_E_UNSUPPORTED. */
+int fe_scene_set_params(fe_scene* scene, double blend, double max_cfl, const double gravity[3], double viscosity,
+                        double density);
 
 /* Particles: ParticleSystem "postion"/"velocity" attributes
  * (src/hybrid_simulator.hpp:50-51) and the APIC affine rows _cX/_cY/_cZ

@@ -1,0 +1,84 @@
+"""The C++ simulators (host_api_demo), the PyFluidEngine module and the C ABI
+driven directly run the same scene to bit-identical particles; the trajectory
+agrees with the FP64 oracle.  Frame semantics follow PhysicsAnimation
+(src/physics_animation.cpp:45-92)."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from fluide_engine_amd import capi  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fluide_engine_amd", "lib")
+N, FRAMES, DT = 24, 4, 1.0 / 240.0
+
+
+def _py_module():
+    if LIB not in sys.path:
+        sys.path.insert(0, LIB)
+    import PyFluidEngine
+
+    return PyFluidEngine
+
+
+def test_cpp_python_capi_identical():
+    print("[gpu] host API: C++ demo, PyFluidEngine, C ABI", flush=True)
+    out = subprocess.run([os.path.join(LIB, "host_api_demo"), "flip", str(N), str(FRAMES), repr(DT), "jacobi"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    demo = json.loads(out.stdout.strip().splitlines()[-1])
+
+    P = _py_module()
+    sim = P.CudaFLIPSimulator(P.Vector3S(N, N, N), P.BoundingBox3D(P.Vector3D(0, 0, 0), P.Vector3D(1, 1, 1)))
+    sim.AddExternalForce(P.DirectionalField(P.Vector3D(0, -9.81, 0)))
+    sim.SetPressureSolver("jacobi", 200, 1e-14)
+    sim.InitializeParticles([P.Box([-1, -1, -1], [0.4, 0.6, 2.0])], 0)
+    sim.SetCurrentFrame(P.Frame(DT))
+    subs = 0
+    for _ in range(FRAMES):
+        sim.AdvanceSingleFrame()
+        subs += sim.GetNumberOfSubTimeSteps()
+    px, pv, _ = sim.GetParticles()
+    assert sim.GetCurrentFrame().GetIndex() == FRAMES
+    assert sim.GetNumberOfParticles() == demo["particles"] == px.shape[0]
+    assert subs == demo["substeps"]
+    c = px.astype(np.float64).mean(0)
+    assert np.allclose(c, demo["centroid"], rtol=0, atol=1e-8), (c, demo["centroid"])
+
+    desc = capi.scene_desc((N, N, N), method=capi.FLIP, blend=0.05, max_particles=px.shape[0])
+    x0 = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.4, 0.6, 2.0))], seed=0)
+    ctx = capi.Context(0)
+    sc = capi.Scene(ctx, desc)
+    sc.upload_particles(x0, np.zeros_like(x0))
+    solv = capi.solver(capi.SOLVER_JACOBI, 200, 5, 1e-14)
+    for _ in range(FRAMES):
+        sc.advance_frame(DT, solv)
+    cx, cv, _ = sc.download_particles()
+    assert np.array_equal(cx, px) and np.array_equal(cv, pv)  # same kernels, same inputs
+    sc.close()
+
+    # and the trajectory against the FP64 oracle (same substep count, drift bar of the 32^3 run test)
+    o = orc.Scene((N, N, N), method=orc.FLIP, blend=0.05)
+    o.set_particles(x0.astype(np.float64), np.zeros(x0.shape))
+    for _ in range(FRAMES):
+        o.advance_frame(DT, orc.solver(orc.JACOBI, 200, 5, 1e-14))
+    ox, _, _ = o.particles()
+    assert np.abs(ox.mean(0) - c).max() < 1e-4
+    ctx.close()
+
+
+def test_cpp_demo_mgpcg_apic():
+    print("[gpu] host API: APIC with MGPCG from C++", flush=True)
+    out = subprocess.run([os.path.join(LIB, "host_api_demo"), "apic", "32", "3", repr(1.0 / 480.0), "mgpcg"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert r["particles"] == 63232 and r["substeps"] == 3
+    assert r["last_rel_residual"] <= 1.05e-6 and 0 < r["last_iterations"] < 60

@@ -1,0 +1,77 @@
+"""Host C++ simulators and the PyFluidEngine module (CPU side: names, defaults
+and the settings that need no device).  The reference's surface:
+src/python_wrapper/simulators_wrapper.cu:36-85, vector3_wrapper.cpp,
+bounding_box_3d_wrapper.cpp, frame_wrapper.cpp, exteranl_forces_wrapper.cpp;
+defaults from src/hybrid_simulator.cpp:20-24 and src/flip_simulator.cpp:5."""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fluide_engine_amd", "lib")
+
+
+@pytest.fixture(scope="module")
+def P():
+    if LIB not in sys.path:
+        sys.path.insert(0, LIB)
+    import PyFluidEngine
+
+    return PyFluidEngine
+
+
+def box(P):
+    return P.BoundingBox3D(P.Vector3D(0, 0, 0), P.Vector3D(1, 2, 4))
+
+
+def test_module_surface(P):
+    for name in ("Frame", "Vector3D", "Vector3S", "BoundingBox3D", "ExternalForce", "DirectionalField",
+                 "HybridSimulator", "PICSimulator", "FLIPSimulator", "APICSimulator"):
+        assert hasattr(P, name), name
+    assert P.CudaFLIPSimulator is P.FLIPSimulator
+    assert P.CudaPICSimulator is P.PICSimulator and P.CudaAPICSimulator is P.APICSimulator
+    for cls in (P.PICSimulator, P.FLIPSimulator, P.APICSimulator):
+        for m in ("AdvanceSingleFrame", "InitializeFromTriangleMesh", "AddExternalForce", "AddCollider",
+                  "AddEmitter", "SetCurrentFrame", "SetViscosity", "SetMaxClf", "GetSurface", "MaxCfl",
+                  "GetViscosity", "GetParticlesPerBlock", "GetOrigin", "GetResolution", "GetGridSpacing",
+                  "GetNumberOfParticles"):
+            assert hasattr(cls, m), (cls, m)
+
+
+def test_defaults_and_geometry(P):
+    s = P.FLIPSimulator(P.Vector3S(16, 8, 4), box(P))
+    assert s.MaxCfl() == 3.0 and s.GetViscosity() == 0.0 and s.GetDensity() == 1.0
+    assert s.GetParticlesPerBlock() == 16
+    assert s.GetBlendingFactor() == 0.05
+    h = s.GetGridSpacing()
+    assert (h.x, h.y, h.z) == (1 / 16, 2 / 8, 4 / 4)
+    r = s.GetResolution()
+    assert (r.x, r.y, r.z) == (16, 8, 4)
+    assert s.GetNumberOfParticles() == 0
+    s.SetBlendingFactor(0.03)
+    s.SetMaxClf(2.0)
+    assert s.GetBlendingFactor() == 0.03 and s.MaxCfl() == 2.0
+    p = P.PICSimulator(P.Vector3S(4, 4, 4), box(P))
+    assert not hasattr(p, "SetBlendingFactor")
+
+
+def test_frame(P):
+    f = P.Frame(0.25)
+    assert f.GetIndex() == 0 and f.GetTimeInSeconds() == 0.0
+    f.Advance()
+    f.Advance()
+    assert f.GetIndex() == 2 and f.GetTimeInSeconds() == 0.5
+    assert abs(P.Frame().GetTimeIntervalInSeconds() - 1 / 60.0) < 1e-7  # 1/60.0f in the reference
+
+
+def test_unsupported_paths(P):
+    s = P.APICSimulator(P.Vector3S(8, 8, 8), box(P))
+    for call in (lambda: s.AddCollider(None), lambda: s.AddEmitter(None), lambda: s.GetSurface(None),
+                 lambda: s.InitializeFromTriangleMesh(None), lambda: s.SetViscosity(0.1)):
+        with pytest.raises(NotImplementedError):
+            call()
+    s.SetViscosity(0.0)
+    with pytest.raises(ValueError):
+        s.SetPressureSolver("gauss-seidel")
+    s.SetPressureSolver("mgpcg", 2000, 1e-6, True)
