@@ -265,6 +265,7 @@ struct fe_scene {
     // particles (HBM, SoA, ping-pong for the reorder)
     int64_t np = 0, cap = 0;
     fe::Particles P[2];
+    float4* pos4 = nullptr;  // positions of the sorted particles as (x, y, z, 0): one load per gathered candidate
     int cur = 0;
     uint32_t* key = nullptr;
     int32_t* cell_count = nullptr;   // n^3 + 1

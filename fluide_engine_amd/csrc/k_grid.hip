@@ -45,11 +45,17 @@ hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n) {
 // Positive values may differ from the reference's where its buckets reach
 // particles farther than r that this scan does not (they only feed the
 // surface mesher).
+// Every candidate's squared distance is first formed in FP32 (one 16-byte
+// load of its position); the FP64 distance is computed only when that value
+// is within `slack` of the running FP32 minimum.  slack is twice a bound E on
+// |d2_fp32 - d2_fp64| for candidates within 2*band, so the candidate with the
+// smallest FP64 distance is never skipped (its FP32 value is at most the FP32
+// minimum + 2E) and the result is the same FP64 minimum.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float* __restrict__ X, int64_t cap,
+__global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict__ pos4,
                                                 const int32_t* __restrict__ start, double* __restrict__ sdf,
                                                 int ex, int ey, int ez, double radius, double band, double qr2,
-                                                int64_t padded) {
+                                                float slack, int64_t padded) {
 #pragma clang fp contract(off)
     const int64_t blk = xcd_remap(blockIdx.x, padded);
     const int64_t c = blk * kBlock + threadIdx.x;
@@ -65,22 +71,38 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float* __restrict_
     const int i0 = max(i - ex, 0), i1 = min(i + ex, g.n[0] - 1);
     const int j0 = max(j - ey, 0), j1 = min(j + ey, g.n[1] - 1);
     const int k0 = max(k - ez, 0), k1 = min(k + ez, g.n[2] - 1);
+    const float gx32 = (float)gx, gy32 = (float)gy, gz32 = (float)gz;
+    float best32 = __builtin_inff();
     double best = qr2;
     bool found = false;
     for (int kk = k0; kk <= k1; ++kk) {
         for (int jj = j0; jj <= j1; ++jj) {
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk);
             const int s0 = start[row + i0], s1 = start[row + i1 + 1];
-            for (int s = s0; s < s1; ++s) {
-                const double dx = (double)X[s] - gx;
-                const double dy = (double)X[cap + s] - gy;
-                const double dz = (double)X[2 * cap + s] - gz;
+            auto consider = [&](const float4 p) {
+                const float fx = p.x - gx32, fy = p.y - gy32, fz = p.z - gz32;
+                const float d2f = fx * fx + fy * fy + fz * fz;
+                if (d2f > best32 + slack) return;
+                best32 = fminf(best32, d2f);
+                const double dx = (double)p.x - gx;
+                const double dy = (double)p.y - gy;
+                const double dz = (double)p.z - gz;
                 const double d2 = dx * dx + dy * dy + dz * dz;
                 if (d2 <= qr2 && (!found || d2 < best)) {
                     best = d2;
                     found = true;
                 }
+            };
+            constexpr int kB = 8;  // candidate loads issued together
+            int s = s0;
+            for (; s + kB <= s1; s += kB) {
+                float4 p[kB];
+#pragma unroll
+                for (int q = 0; q < kB; ++q) p[q] = pos4[s + q];
+#pragma unroll
+                for (int q = 0; q < kB; ++q) consider(p[q]);
             }
+            for (; s < s1; ++s) consider(pos4[s]);
         }
     }
     double minD = band;
@@ -96,10 +118,20 @@ hipError_t launch_sdf(fe_scene* s) {
     const double qr2 = band * band;
     int ext[3];
     for (int a = 0; a < 3; ++a) ext[a] = (int)std::floor(radius / g.hd[a] + 0.5);
+    // Bound E on |d2_fp32 - d2_fp64| for candidates within D = 2*band of the
+    // centre: each FP32 difference is within delta = eps*(G + D) + eps^2*G of
+    // the exact one (G bounds every coordinate), and the squares and sums add
+    // at most 3*eps of 3*D^2 more.  slack = 2E, doubled again for margin.
+    const double eps = std::ldexp(1.0, -24), D = 2.0 * band;
+    double G = 0.0;
+    for (int a = 0; a < 3; ++a) G = fmax(G, fabs(g.od[a]) + g.Ld[a] + g.hd[a]);
+    const double delta = eps * (G + D) + eps * eps * G;
+    const double E = 3.0 * (2.0 * D * delta + delta * delta) + 3.0 * eps * 3.0 * D * D;
+    const float slack = (float)(4.0 * E);
     const int64_t nc = cell_count(g);
     const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-    launch_timed(s, FE_KERNEL_SDF, 0, k_sdf, dim3((unsigned)padded), dim3(kBlock), g, (const float*)s->P[s->cur].x,
-                 s->cap, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, padded);
+    launch_timed(s, FE_KERNEL_SDF, 0, k_sdf, dim3((unsigned)padded), dim3(kBlock), g, (const float4*)s->pos4,
+                 (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, slack, padded);
     return hipGetLastError();
 }
 

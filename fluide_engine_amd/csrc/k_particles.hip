@@ -172,15 +172,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict_
 }
 
 __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ order, int64_t np, int64_t cap,
-                                                   Particles src, Particles dst, int with_c) {
+                                                   Particles src, Particles dst, int with_c,
+                                                   float4* __restrict__ pos4) {
     int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= np) return;
     const int64_t o = order[s];
+    float p[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        dst.x[a * cap + s] = src.x[a * cap + o];
+        p[a] = src.x[a * cap + o];
+        dst.x[a * cap + s] = p[a];
         dst.v[a * cap + s] = src.v[a * cap + o];
     }
+    pos4[s] = make_float4(p[0], p[1], p[2], 0.0f);
     if (with_c) {
 #pragma unroll
         for (int a = 0; a < 9; ++a) dst.C[a * cap + s] = src.C[a * cap + o];
@@ -203,8 +207,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ o
 // ---------------------------------------------------------------------------
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ X, const float* __restrict__ V,
-                                                const float* __restrict__ Cm, int64_t cap,
-                                                const int32_t* __restrict__ start, float* __restrict__ out,
+                                                const float* __restrict__ Cm, const float4* __restrict__ pos4,
+                                                int64_t cap, const int32_t* __restrict__ start, float* __restrict__ out,
                                                 uint8_t* __restrict__ mark, float* __restrict__ snap,
                                                 int64_t padded) {
     const int64_t blk = xcd_remap(blockIdx.x, padded);
@@ -277,25 +281,27 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         num += val * w;
         den += w;
     };
-    constexpr int kBatch = 4;  // candidates whose loads are issued together
+    constexpr int kBatch = 8;  // candidates whose loads are issued together
     for (int kc = lo[2]; kc <= hi[2]; ++kc) {
         for (int jc = lo[1]; jc <= hi[1]; ++jc) {
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
             int s = s0;
             for (; s + kBatch <= s1; s += kBatch) {
-                float px[kBatch], py[kBatch], pz[kBatch], pv[kBatch];
+                float4 pp[kBatch];
+                float pv[kBatch];
 #pragma unroll
                 for (int q = 0; q < kBatch; ++q) {
-                    px[q] = X[s + q];
-                    py[q] = X[cap + s + q];
-                    pz[q] = X[2 * cap + s + q];
+                    pp[q] = pos4[s + q];
                     pv[q] = V[C * cap + s + q];
                 }
 #pragma unroll
-                for (int q = 0; q < kBatch; ++q) consider(s + q, px[q], py[q], pz[q], pv[q]);
+                for (int q = 0; q < kBatch; ++q) consider(s + q, pp[q].x, pp[q].y, pp[q].z, pv[q]);
             }
-            for (; s < s1; ++s) consider(s, X[s], X[cap + s], X[2 * cap + s], V[C * cap + s]);
+            for (; s < s1; ++s) {
+                const float4 pp = pos4[s];
+                consider(s, pp.x, pp.y, pp.z, V[C * cap + s]);
+            }
         }
     }
     const float res = den > 0.0f ? num / den : 0.0f;
@@ -517,7 +523,7 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
                            s->cursor, s->order);
         hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order);
         hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->order, s->np, s->cap,
-                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0);
+                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         s->cur = 1 - s->cur;
     }
@@ -534,7 +540,7 @@ static hipError_t p2g_method(fe_scene* s) {
         const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
         auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
         launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.x,
-                     (const float*)P.v, (const float*)P.C, s->cap, (const int32_t*)s->cell_start, s->u[c],
+                     (const float*)P.v, (const float*)P.C, (const float4*)s->pos4, s->cap, (const int32_t*)s->cell_start, s->u[c],
                      s->mark[c], s->snap[c], padded);
     }
     return hipGetLastError();
