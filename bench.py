@@ -55,13 +55,39 @@ def kernel_bytes_per_cell(kernel, solver):
     return b
 
 
+def kernel_bytes_per_launch(kernel, solver, n, npart):
+    """Algorithmic HBM bytes of one launch of a timed kernel on the n^3 FLIP scene."""
+    cells = n ** 3
+    if kernel in KERNEL_BYTES_PER_CELL:
+        return kernel_bytes_per_cell(kernel, solver) * cells
+    if kernel == "p2g":  # one component: float4 position (16) + v_c (4) per particle, cell start (4) per cell,
+        return 20 * npart + 4 * cells + 9 * (n + 1) * n * n  # write u + marker + FLIP snapshot (9) per face
+    if kernel == "g2p_move":  # read x, v (24), write x, v (24), bin key (4) per particle; read u, snapshot per face
+        return 52 * npart + 8 * 3 * (n + 1) * n * n
+    if kernel == "sdf":  # read every particle position (12), cell starts (4) and write the SDF (8) per cell
+        return 12 * npart + 12 * cells
+    return None
+
+
+def roofline(kernel, solver, n, npart, kms, launches, traffic):
+    bpl = kernel_bytes_per_launch(kernel, solver, n, npart)
+    if bpl is None or launches <= 0:
+        return None
+    avg_ms = kms / launches
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches, "bytes_per_launch": bpl}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--kernel", default="cg_update", choices=sorted(KERNEL_IDS))
+    ap.add_argument("--kernel", default="p2g", choices=sorted(KERNEL_IDS),
+                    help="kernel of the roofline object (default: P2G, the largest share of the substep)")
     ap.add_argument("--solver", default="mgpcg", choices=sorted(SOLVERS))
     ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -227,20 +253,23 @@ def main():
                   file=sys.stderr, flush=True)
     scene.sync()
     t1 = time.perf_counter()
-    kms, klaunch = scene.kernel_timer_read()
+    kms, klaunch = scene.kernel_timer_read(reset=True)
     wall = t1 - t0
     value = npart * sub_total / wall / 1e6
-    ncells = n ** 3
-    kavg_ms = kms / max(klaunch, 1)
-    roof = None
-    if args.kernel in KERNEL_BYTES_PER_CELL and klaunch > 0:
-        bpl = kernel_bytes_per_cell(args.kernel, args.solver) * ncells
-        achieved = bpl / (kavg_ms * 1e-3) / 1e9
-        roof = {"kernel": args.kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_for(args.kernel) if (args.n == 128 and args.solver == "mgpcg") else None,
-                "avg_launch_us": round(kavg_ms * 1e3, 3), "launches": klaunch,
-                "bytes_per_launch": bpl}
+    on_profile = args.n == 128 and args.solver == "mgpcg"
+    roof = roofline(args.kernel, args.solver, n, npart, kms, klaunch,
+                    traffic_for(args.kernel) if on_profile else None)
+    # the CG update's roofline as well, over a few more substeps after the timed region
+    roof_cg = None
+    if args.solver != "jacobi" and args.kernel not in ("cg_update", "none"):
+        scene.kernel_timer(KERNEL_IDS["cg_update"])
+        for i in range(min(20, max(args.steps, 1))):
+            scene.advance_frame(dt, solver)
+        scene.sync()
+        cms, clau = scene.kernel_timer_read(reset=True)
+        roof_cg = roofline("cg_update", args.solver, n, npart, cms, clau,
+                           traffic_for("cg_update") if on_profile else None)
+    scene.kernel_timer(KERNEL_IDS["none"])
 
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
@@ -270,6 +299,7 @@ def main():
         "pressure_rel_residual_max": max(resid),
         "stage_ms_per_step": {k: round(val / max(args.steps, 1), 4) for k, val in stage_acc.items()},
         "roofline": roof,
+        "roofline_cg_update": roof_cg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

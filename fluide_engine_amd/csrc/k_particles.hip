@@ -171,9 +171,26 @@ __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict_
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ order, int64_t np, int64_t cap,
-                                                   Particles src, Particles dst, int with_c,
-                                                   float4* __restrict__ pos4) {
+// Besides the reorder, k_gather writes what P2G reads per candidate: the
+// position as one float4 and, per face component c, the stencil record
+// st4[c*cap + s] = (fx, fy, fz, bits(i0 & 255 | (j0 & 255) << 8 | (k0 & 255) << 16))
+// of GetCooridnatesAndWeights (the same stencil_axis values P2G would compute).
+// Only the low 8 bits of each base index are kept: every candidate a face F
+// scans lies in a cell within 1 of F on each axis and its clamped base is
+// within 2 of its cell, so F - base is in [-3, 3] and is 0 or 1 exactly when
+// (F - base) mod 256 is.
+__device__ inline float4 stencil_record(const Geom& g, int c, const float p[3]) {
+    int i[3];
+    float f[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) stencil_axis(g, c, a, p[a], stencil_size(g, c, a), i[a], f[a]);
+    const uint32_t b = ((uint32_t)i[0] & 255u) | (((uint32_t)i[1] & 255u) << 8) | (((uint32_t)i[2] & 255u) << 16);
+    return make_float4(f[0], f[1], f[2], __uint_as_float(b));
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __restrict__ order, int64_t np,
+                                                   int64_t cap, Particles src, Particles dst, int with_c,
+                                                   float4* __restrict__ pos4, float4* __restrict__ st4) {
     int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s >= np) return;
     const int64_t o = order[s];
@@ -185,6 +202,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ o
         dst.v[a * cap + s] = src.v[a * cap + o];
     }
     pos4[s] = make_float4(p[0], p[1], p[2], 0.0f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) st4[c * cap + s] = stencil_record(g, c, p);
     if (with_c) {
 #pragma unroll
         for (int a = 0; a < 9; ++a) dst.C[a * cap + s] = src.C[a * cap + o];
@@ -202,12 +221,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ o
 // at n-2).  Cells are x-fastest and particles sorted by cell, so each (j,k)
 // row of candidate cells is one contiguous particle range.  For every
 // candidate the reference's own stencil (GetCooridnatesAndWeights with the
-// scheme's stencil size) is evaluated and the particle contributes iff the face
-// is one of its 8 nodes; the face is marked even for a zero weight.
+// scheme's stencil size, precomputed per particle by k_gather) decides: the
+// particle contributes iff the face is one of its 8 nodes; the face is marked
+// even for a zero weight.
 // ---------------------------------------------------------------------------
 template <int C, int METHOD>
-__global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ X, const float* __restrict__ V,
-                                                const float* __restrict__ Cm, const float4* __restrict__ pos4,
+__global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
+                                                const float4* __restrict__ pos4, const float4* __restrict__ st4,
                                                 int64_t cap, const int32_t* __restrict__ start, float* __restrict__ out,
                                                 uint8_t* __restrict__ mark, float* __restrict__ snap,
                                                 int64_t padded) {
@@ -221,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
     const int J = (int)(t % sy);
     const int K = (int)(t / sy);
     const int F[3] = {I, J, K};
-    int lo[3], hi[3], S[3];
+    int lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (a == C) {
@@ -234,7 +254,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         }
         lo[a] = max(lo[a], 0);
         hi[a] = min(hi[a], g.n[a] - 1);
-        S[a] = stencil_size(g, C, a);
     }
     // APIC: face position GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
     // and the clamp box [o + h/2, o + L - h/2] of the transverse axes (apic_simulator.cpp:54-63).
@@ -248,39 +267,42 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         }
     }
     float num = 0.0f, den = 0.0f;
-    int marked = 0;
-    // one candidate particle, in ascending particle order
-    auto consider = [&](int s, float px, float py, float pz, float val) {
-        int i0;
-        float fr;
-        stencil_axis(g, C, 0, px, S[0], i0, fr);
-        float wx;
-        if (I == i0) wx = 1.0f - fr;
-        else if (I == i0 + 1) wx = fr;
-        else return;
-        stencil_axis(g, C, 1, py, S[1], i0, fr);
-        float wy;
-        if (J == i0) wy = 1.0f - fr;
-        else if (J == i0 + 1) wy = fr;
-        else return;
-        stencil_axis(g, C, 2, pz, S[2], i0, fr);
-        float wz;
-        if (K == i0) wz = 1.0f - fr;
-        else if (K == i0 + 1) wz = fr;
-        else return;
-        marked = 1;
-        const float w = (wx * wy) * wz;
+    uint32_t marked = 0;
+    // One candidate particle, in ascending particle order, from its stencil
+    // record (fx, fy, fz, packed low bytes of the base indices): the face is
+    // one of its 8 nodes iff F - base is 0 or 1 on every axis.  PIC/FLIP take
+    // the rejected candidates branch-free with zero weight and value, which
+    // leaves num and den bit for bit unchanged (num never holds -0).
+    auto consider = [&](int s, float4 r, float val) {
+        const uint32_t b = __float_as_uint(r.w);
+        const uint32_t dx = ((uint32_t)I - b) & 255u;
+        const uint32_t dy = ((uint32_t)J - (b >> 8)) & 255u;
+        const uint32_t dz = ((uint32_t)K - (b >> 16)) & 255u;
+        const bool ok = (dx | dy | dz) <= 1u;
+        const float wx = dx ? r.x : 1.0f - r.x;
+        const float wy = dy ? r.y : 1.0f - r.y;
+        const float wz = dz ? r.z : 1.0f - r.z;
+        float w = (wx * wy) * wz;
         if (METHOD == FE_APIC) {
-            float pc[3] = {px, py, pz};
+            if (!ok) return;
+            const float4 pp = pos4[s];
+            float pc[3] = {pp.x, pp.y, pp.z};
 #pragma unroll
             for (int a = 0; a < 3; ++a)
                 if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
             val += cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]);
+            marked = 1;
+        } else {
+            marked |= ok ? 1u : 0u;
+            w = ok ? w : 0.0f;
+            val = ok ? val : 0.0f;
         }
         num += val * w;
         den += w;
     };
+    const float4* __restrict__ st = st4 + (int64_t)C * cap;
+    const float* __restrict__ vc = V + (int64_t)C * cap;
     constexpr int kBatch = 8;  // candidates whose loads are issued together
     for (int kc = lo[2]; kc <= hi[2]; ++kc) {
         for (int jc = lo[1]; jc <= hi[1]; ++jc) {
@@ -288,20 +310,17 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
             int s = s0;
             for (; s + kBatch <= s1; s += kBatch) {
-                float4 pp[kBatch];
+                float4 rr[kBatch];
                 float pv[kBatch];
 #pragma unroll
                 for (int q = 0; q < kBatch; ++q) {
-                    pp[q] = pos4[s + q];
-                    pv[q] = V[C * cap + s + q];
+                    rr[q] = st[s + q];
+                    pv[q] = vc[s + q];
                 }
 #pragma unroll
-                for (int q = 0; q < kBatch; ++q) consider(s + q, pp[q].x, pp[q].y, pp[q].z, pv[q]);
+                for (int q = 0; q < kBatch; ++q) consider(s + q, rr[q], pv[q]);
             }
-            for (; s < s1; ++s) {
-                const float4 pp = pos4[s];
-                consider(s, pp.x, pp.y, pp.z, V[C * cap + s]);
-            }
+            for (; s < s1; ++s) consider(s, st[s], vc[s]);
         }
     }
     const float res = den > 0.0f ? num / den : 0.0f;
@@ -522,8 +541,8 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
         hipLaunchKernelGGL(k_scatter, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->np, s->cell_start,
                            s->cursor, s->order);
         hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order);
-        hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->order, s->np, s->cap,
-                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4);
+        hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, s->order, s->np, s->cap,
+                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4, s->st4);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         s->cur = 1 - s->cur;
     }
@@ -539,9 +558,9 @@ static hipError_t p2g_method(fe_scene* s) {
         const int64_t nf = face_count(g, c);
         const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
         auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
-        launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.x,
-                     (const float*)P.v, (const float*)P.C, (const float4*)s->pos4, s->cap, (const int32_t*)s->cell_start, s->u[c],
-                     s->mark[c], s->snap[c], padded);
+        launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
+                     (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
+                     (const int32_t*)s->cell_start, s->u[c], s->mark[c], s->snap[c], padded);
     }
     return hipGetLastError();
 }
