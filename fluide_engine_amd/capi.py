@@ -17,10 +17,11 @@ PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
 SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI, SOLVER_MGPCG = 0, 1, 2, 3
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
- GRID_PRESSURE, GRID_RHS) = range(13)
+ GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW) = range(16)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
- STAGE_MOVE, STAGE_MARKERS) = range(10)
-STAGE_NAMES = ["sort", "p2g", "sdf", "extrapolate", "bc", "gravity", "pressure", "g2p_move", "move", "markers"]
+ STAGE_MOVE, STAGE_MARKERS, STAGE_DIFFUSION) = range(11)
+STAGE_NAMES = ["sort", "p2g", "sdf", "extrapolate", "bc", "gravity", "pressure", "g2p_move", "move", "markers",
+               "diffusion"]
 
 EXPORTED = [
     "fe_ctx_create", "fe_ctx_destroy", "fe_last_error", "fe_ctx_device_name", "fe_scene_create", "fe_scene_destroy",
@@ -28,6 +29,7 @@ EXPORTED = [
     "fe_advance_frame", "fe_cfl", "fe_run_stage", "fe_grid_download", "fe_grid_upload", "fe_grid_bytes",
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
+    "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -102,6 +104,8 @@ def load():
     L.fe_kernel_timer_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
     L.fe_copy_probe.argtypes = [vp, C.c_int64, C.POINTER(C.c_double)]
     L.fe_scene_set_params.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double]
+    L.fe_scene_set_diffusion_solver.argtypes = [vp, C.POINTER(SolverDesc)]
+    L.fe_last_diffusion_stats.argtypes = [vp, C.POINTER(SolveStats)]
     L.fe_version.restype = C.c_char_p
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name in ("fe_last_error", "fe_version") else C.c_int
@@ -223,17 +227,17 @@ class Scene:
 
     def shape(self, field):
         nx, ny, nz = self.n
-        if field in (GRID_U, GRID_MU, GRID_SU):
+        if field in (GRID_U, GRID_MU, GRID_SU, GRID_DMU):
             return (nz, ny, nx + 1)
-        if field in (GRID_V, GRID_MV, GRID_SV):
+        if field in (GRID_V, GRID_MV, GRID_SV, GRID_DMV):
             return (nz, ny + 1, nx)
-        if field in (GRID_W, GRID_MW, GRID_SW):
+        if field in (GRID_W, GRID_MW, GRID_SW, GRID_DMW):
             return (nz + 1, ny, nx)
         return (nz, ny, nx)
 
     @staticmethod
     def dtype(field):
-        if field in (GRID_MU, GRID_MV, GRID_MW, GRID_FLAGS):
+        if field in (GRID_MU, GRID_MV, GRID_MW, GRID_FLAGS, GRID_DMU, GRID_DMV, GRID_DMW):
             return np.uint8
         if field in (GRID_SDF, GRID_PRESSURE):
             return np.float64
@@ -297,8 +301,9 @@ class Scene:
 
     def run_stage(self, stage, dt=0.0, desc=None):
         st = SolveStats()
-        d = desc or solver()
-        self.ctx.check(self.L.fe_run_stage(self._h, stage, dt, C.byref(d), C.byref(st)),
+        # the diffusion stage takes the scene's own diffusion solver when desc is None
+        d = None if (desc is None and stage == STAGE_DIFFUSION) else C.byref(desc or solver())
+        self.ctx.check(self.L.fe_run_stage(self._h, stage, dt, d, C.byref(st)),
                        f"fe_run_stage({STAGE_NAMES[stage] if 0 <= stage < len(STAGE_NAMES) else stage})")
         return st
 
@@ -314,6 +319,14 @@ class Scene:
         st = SolveStats()
         self.ctx.check(self.L.fe_last_solve_stats(self._h, C.byref(st)), "fe_last_solve_stats")
         return st
+
+    def set_diffusion_solver(self, desc):
+        self.ctx.check(self.L.fe_scene_set_diffusion_solver(self._h, C.byref(desc)), "fe_scene_set_diffusion_solver")
+
+    def last_diffusion_stats(self):
+        st = (SolveStats * 3)()
+        self.ctx.check(self.L.fe_last_diffusion_stats(self._h, st), "fe_last_diffusion_stats")
+        return list(st)
 
     def kernel_timer(self, kernel_id):
         """Time every launch of one kernel with HIP events on the scene's stream."""

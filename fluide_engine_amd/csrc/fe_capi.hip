@@ -74,6 +74,9 @@ size_t field_bytes(const fe_scene* s, int32_t field) {
         case FE_GRID_SDF: case FE_GRID_PRESSURE: return cell_count(g) * sizeof(double);
         case FE_GRID_FLAGS: return cell_count(g);
         case FE_GRID_RHS: return cell_count(g) * sizeof(float);
+        case FE_GRID_DMU: return face_count(g, 0);
+        case FE_GRID_DMV: return face_count(g, 1);
+        case FE_GRID_DMW: return face_count(g, 2);
         default: return 0;
     }
 }
@@ -93,8 +96,31 @@ void* field_ptr(fe_scene* s, int32_t field) {
         case FE_GRID_FLAGS: return s->flags;
         case FE_GRID_PRESSURE: return s->p;
         case FE_GRID_RHS: return s->b;
+        case FE_GRID_DMU: return s->dmark[0];
+        case FE_GRID_DMV: return s->dmark[1];
+        case FE_GRID_DMW: return s->dmark[2];
         default: return nullptr;
     }
+}
+
+bool diffusion_field(int32_t field) { return field == FE_GRID_DMU || field == FE_GRID_DMV || field == FE_GRID_DMW; }
+
+int check_viscosity(fe_ctx* ctx, double viscosity) {
+    if (!(viscosity >= 0.0) || std::isinf(viscosity)) return set_err(ctx, FE_E_INVALID, "viscosity must be finite and >= 0");
+    return FE_OK;
+}
+
+// BackwardEulerDiffusionSolver's linear solver: Jacobi only (its default,
+// src/fluid_solvers/backward_euler_diffusion_solver.cpp:10).
+int check_diffusion_solver(fe_ctx* ctx, const fe_solver_desc* d) {
+    if (!d) return set_err(ctx, FE_E_INVALID, "solver descriptor is NULL");
+    if (d->kind != FE_SOLVER_JACOBI)
+        return set_err(ctx, FE_E_UNSUPPORTED, "the diffusion solve runs the Jacobi solver only");
+    if (d->max_iterations < 0 || d->max_iterations > 1000000)
+        return set_err(ctx, FE_E_INVALID, "max_iterations out of range");
+    if (d->check_interval < 0) return set_err(ctx, FE_E_INVALID, "check_interval must be >= 0");
+    if (!(d->tolerance >= 0.0)) return set_err(ctx, FE_E_INVALID, "tolerance must be >= 0");
+    return FE_OK;
 }
 
 int check_solver(fe_ctx* ctx, const fe_solver_desc* d) {
@@ -180,7 +206,13 @@ int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st)
     // ComputeExternalForces (:203-212) = forces, then ApplyBoundryCondition
     clk.begin(FE_STAGE_GRAVITY);
     HIPCHK(ctx, launch_gravity(s, dt, true));
-    // ComputeDiffusion is skipped at viscosity 0 (:214-222); ComputePressure (:224-229)
+    // ComputeDiffusion (:214-222) when viscosity > 0, then ApplyBoundryCondition
+    if (s->desc.viscosity > 0.0) {
+        clk.begin(FE_STAGE_DIFFUSION);
+        HIPCHK(ctx, launch_diffusion(s, dt, &s->diff_desc));
+        HIPCHK(ctx, launch_walls(s));
+    }
+    // ComputePressure (:224-229)
     const int pev = clk.n;
     clk.begin(FE_STAGE_PRESSURE);
     int rc = pressure(s, d, true, st, pev);
@@ -286,8 +318,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     if ((int64_t)d->n[0] * d->n[1] * d->n[2] > (int64_t)1 << 30)
         return set_err(ctx, FE_E_INVALID, "grid has more than 2^30 cells");
     if (d->method < FE_PIC || d->method > FE_APIC) return set_err(ctx, FE_E_INVALID, "unknown method");
-    if (d->viscosity != 0.0)
-        return set_err(ctx, FE_E_UNSUPPORTED, "viscosity > 0 (BackwardEulerDiffusionSolver) is not on this backend");
+    if (check_viscosity(ctx, d->viscosity) != FE_OK) return FE_E_INVALID;
     if (d->max_particles < 0 || d->max_particles > ((int64_t)1 << 31) - 1)
         return set_err(ctx, FE_E_INVALID, "max_particles out of range");
     if (!(d->max_cfl >= 0.0) || d->max_cfl > 64) return set_err(ctx, FE_E_INVALID, "max_cfl out of range");
@@ -380,8 +411,7 @@ int fe_scene_set_params(fe_scene* s, double blend, double max_cfl, const double 
                         double density) {
     if (!s || !gravity) return FE_E_INVALID;
     fe_ctx* ctx = s->ctx;
-    if (viscosity != 0.0)
-        return set_err(ctx, FE_E_UNSUPPORTED, "viscosity > 0 (BackwardEulerDiffusionSolver) is not on this backend");
+    if (check_viscosity(ctx, viscosity) != FE_OK) return FE_E_INVALID;
     if (!(max_cfl >= 0.0) || max_cfl > 64) return set_err(ctx, FE_E_INVALID, "max_cfl out of range");
     if (!(blend >= 0.0 && blend <= 1.0)) return set_err(ctx, FE_E_INVALID, "blend must be in [0, 1]");
     s->desc.blend = blend;
@@ -410,6 +440,7 @@ int fe_scene_destroy(fe_scene* s) {
     F(s->sdf); F(s->flags); F(s->code); F(s->p); F(s->b); F(s->r); F(s->z); F(s->p2);
     F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->sc); F(s->cflmax);
     mg_free(s);
+    diffusion_free(s);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
@@ -599,6 +630,12 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
         case FE_STAGE_G2P:
             HIPCHK(ctx, launch_g2p_move(s, dt, true, false, false));
             break;
+        case FE_STAGE_DIFFUSION: {
+            const fe_solver_desc* dd = d ? d : &s->diff_desc;
+            if ((rc = check_diffusion_solver(ctx, dd)) != FE_OK) return rc;
+            HIPCHK(ctx, launch_diffusion(s, dt, dd));
+            break;
+        }
         case FE_STAGE_MOVE:
             HIPCHK(ctx, launch_g2p_move(s, dt, false, true, false));
             break;
@@ -616,6 +653,20 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
 int fe_last_solve_stats(const fe_scene* s, fe_solve_stats* st) {
     if (!s || !st) return FE_E_INVALID;
     *st = s->last_stats;
+    return FE_OK;
+}
+
+int fe_scene_set_diffusion_solver(fe_scene* s, const fe_solver_desc* d) {
+    if (!s) return FE_E_INVALID;
+    int rc = check_diffusion_solver(s->ctx, d);
+    if (rc != FE_OK) return rc;
+    s->diff_desc = *d;
+    return FE_OK;
+}
+
+int fe_last_diffusion_stats(const fe_scene* s, fe_solve_stats stats[3]) {
+    if (!s || !stats) return FE_E_INVALID;
+    for (int c = 0; c < 3; ++c) stats[c] = s->diff_stats[c];
     return FE_OK;
 }
 
@@ -665,6 +716,7 @@ int fe_grid_download(fe_scene* s, int32_t field, void* dst, size_t bytes) {
     const size_t need = field_bytes(s, field);
     if (need == 0) return set_err(ctx, FE_E_INVALID, "unknown grid field " + std::to_string(field));
     if (bytes != need) return set_err(ctx, FE_E_INVALID, "byte count mismatch: need " + std::to_string(need));
+    if (!field_ptr(s, field)) return set_err(ctx, FE_E_INVALID, "no diffusion solve has run on this scene");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(dst, field_ptr(s, field), need, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
@@ -677,6 +729,7 @@ int fe_grid_upload(fe_scene* s, int32_t field, const void* src, size_t bytes) {
     const size_t need = field_bytes(s, field);
     if (need == 0) return set_err(ctx, FE_E_INVALID, "unknown grid field " + std::to_string(field));
     if (bytes != need) return set_err(ctx, FE_E_INVALID, "byte count mismatch: need " + std::to_string(need));
+    if (diffusion_field(field)) return set_err(ctx, FE_E_INVALID, "diffusion markers are download-only");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(field_ptr(s, field), src, need, hipMemcpyHostToDevice, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));

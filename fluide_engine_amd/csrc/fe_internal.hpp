@@ -113,6 +113,51 @@ __device__ inline int cell_axis(const Geom& g, int a, float p) {
     return s < 0 ? 0 : (s > g.n[a] - 1 ? g.n[a] - 1 : s);
 }
 
+// GetBarycentric(x, 0, size-1) in FP64 (src/common/math_utils.hpp:48-73).
+__device__ inline void bary64(double x, int size, int& i, double& f) {
+#pragma clang fp contract(off)
+    const double s = floor(x);
+    i = (int)fmin(fmax(s, -1.0e9), 1.0e9);
+    if (size <= 1) {
+        i = 0;
+        f = 0.0;
+    } else if (i < 0) {
+        i = 0;
+        f = 0.0;
+    } else if (i > size - 2) {
+        i = size - 2;
+        f = 1.0;
+    } else {
+        f = x - s;
+    }
+}
+
+__device__ inline double lerp64(double a, double b, double f) {
+#pragma clang fp contract(off)
+    return (1 - f) * a + f * b;
+}
+
+// ScalarGrid3D::Sample (src/3d/scalar_grid3d.cpp:33-58) of the fluid SDF
+// (cell centred, origin o + h/2) at p, in the reference's FP64 arithmetic.
+__device__ inline double sample_sdf64(const Geom& g, const double* __restrict__ sdf, const double p[3]) {
+#pragma clang fp contract(off)
+    int ii[3];
+    double ff[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double so = g.od[a] + g.hd[a] / 2.0;
+        bary64((p[a] - so) / g.hd[a], g.n[a], ii[a], ff[a]);
+    }
+    const int ip1 = min(ii[0] + 1, g.n[0] - 1), jp1 = min(ii[1] + 1, g.n[1] - 1), kp1 = min(ii[2] + 1, g.n[2] - 1);
+    auto S = [&](int a, int b, int cc) {
+        return sdf[(int64_t)a + (int64_t)g.n[0] * ((int64_t)b + (int64_t)g.n[1] * cc)];
+    };
+    const int i = ii[0], j = ii[1], k = ii[2];
+    return lerp64(lerp64(lerp64(S(i, j, k), S(ip1, j, k), ff[0]), lerp64(S(i, jp1, k), S(ip1, jp1, k), ff[0]), ff[1]),
+                  lerp64(lerp64(S(i, j, kp1), S(ip1, j, kp1), ff[0]), lerp64(S(i, jp1, kp1), S(ip1, jp1, kp1), ff[0]), ff[1]),
+                  ff[2]);
+}
+
 // FP32 fast coordinates for interpolation (values, not parity-critical indices).
 __device__ inline void fast_axis(float p, float o, float ih, int size, int& i, float& f) {
     const float x = (p - o) * ih;
@@ -283,6 +328,15 @@ struct fe_scene {
     fe_solve_stats last_stats{};
     std::vector<fe::MGLevel> mg;
     int mg_tail = 0;  // first level handled by the one-workgroup tail kernel  // multigrid hierarchy, built on first use
+
+    // BackwardEulerDiffusionSolver: its linear solver (default Jacobi(1000, 5, 1e-14)),
+    // buffers allocated on first use (FP64 x ping-pong, row codes, face markers
+    // per component) and the stats of the last solve per component
+    fe_solver_desc diff_desc{FE_SOLVER_JACOBI, 1000, 5, 0, 0, 0, 1e-14};
+    double* dx[2] = {nullptr, nullptr};
+    uint16_t* dcode = nullptr;
+    uint8_t* dmark[3] = {nullptr, nullptr, nullptr};
+    fe_solve_stats diff_stats[3] = {};
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

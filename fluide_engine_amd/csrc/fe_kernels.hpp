@@ -43,4 +43,8 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* 
 float* mg_first_sweep_target(fe_scene* s);  // where the CG kernels write level 0's first sweep
 void mg_free(fe_scene* s);
 
+// k_diffusion.hip: BackwardEulerDiffusionSolver (without the trailing boundary condition)
+hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d);
+void diffusion_free(fe_scene* s);
+
 }  // namespace fe

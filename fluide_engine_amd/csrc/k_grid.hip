@@ -142,29 +142,6 @@ hipError_t launch_sdf(fe_scene* s) {
 // Without colliders the collider SDF is positive everywhere, so the other
 // cells are AIR.
 // ---------------------------------------------------------------------------
-__device__ inline void bary64(double x, int size, int& i, double& f) {
-#pragma clang fp contract(off)
-    const double s = floor(x);
-    i = (int)fmin(fmax(s, -1.0e9), 1.0e9);
-    if (size <= 1) {
-        i = 0;
-        f = 0.0;
-    } else if (i < 0) {
-        i = 0;
-        f = 0.0;
-    } else if (i > size - 2) {
-        i = size - 2;
-        f = 1.0;
-    } else {
-        f = x - s;
-    }
-}
-
-__device__ inline double lerp64(double a, double b, double f) {
-#pragma clang fp contract(off)
-    return (1 - f) * a + f * b;
-}
-
 __global__ __launch_bounds__(kBlock) void k_markers(Geom g, const double* __restrict__ sdf,
                                                     uint8_t* __restrict__ flags, int64_t padded) {
 #pragma clang fp contract(off)
@@ -173,23 +150,10 @@ __global__ __launch_bounds__(kBlock) void k_markers(Geom g, const double* __rest
     const int64_t nc = cell_count(g);
     if (c >= nc) return;
     const int idx[3] = {(int)(c % g.n[0]), (int)((c / g.n[0]) % g.n[1]), (int)(c / ((int64_t)g.n[0] * g.n[1]))};
-    int ii[3];
-    double ff[3];
+    double cc[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const double cc = (g.od[a] + 0.5 * g.hd[a]) + g.hd[a] * (double)idx[a];  // GetCellCenterPos
-        const double so = g.od[a] + g.hd[a] / 2.0;
-        bary64((cc - so) / g.hd[a], g.n[a], ii[a], ff[a]);
-    }
-    const int ip1 = min(ii[0] + 1, g.n[0] - 1), jp1 = min(ii[1] + 1, g.n[1] - 1), kp1 = min(ii[2] + 1, g.n[2] - 1);
-    auto S = [&](int a, int b, int cc) {
-        return sdf[(int64_t)a + (int64_t)g.n[0] * ((int64_t)b + (int64_t)g.n[1] * cc)];
-    };
-    const int i = ii[0], j = ii[1], k = ii[2];
-    const double v = lerp64(lerp64(lerp64(S(i, j, k), S(ip1, j, k), ff[0]), lerp64(S(i, jp1, k), S(ip1, jp1, k), ff[0]), ff[1]),
-                            lerp64(lerp64(S(i, j, kp1), S(ip1, j, kp1), ff[0]), lerp64(S(i, jp1, kp1), S(ip1, jp1, kp1), ff[0]), ff[1]),
-                            ff[2]);
-    flags[c] = v < 0 ? (uint8_t)FE_FLUID : (uint8_t)FE_AIR;
+    for (int a = 0; a < 3; ++a) cc[a] = (g.od[a] + 0.5 * g.hd[a]) + g.hd[a] * (double)idx[a];  // GetCellCenterPos
+    flags[c] = sample_sdf64(g, sdf, cc) < 0 ? (uint8_t)FE_FLUID : (uint8_t)FE_AIR;
 }
 
 hipError_t launch_markers(fe_scene* s) {

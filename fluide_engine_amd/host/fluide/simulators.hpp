@@ -11,8 +11,8 @@
 // particle/grid accessors.
 //
 // Not on this backend (each throws Unsupported): triangle-mesh scenes
-// (InitializeFromTriangleMesh, GetSurface), colliders, emitters, viscosity > 0,
-// and external forces other than a uniform DirectionalField.
+// (InitializeFromTriangleMesh, GetSurface), colliders, emitters and external
+// forces other than a uniform DirectionalField.
 #pragma once
 
 #include <cstddef>
@@ -116,6 +116,15 @@ struct PressureSolverSettings {
     bool warmStart = true;
 };
 
+// The linear solver of BackwardEulerDiffusionSolver, which the reference fixes
+// to CudaJacobiIterationSolver(1000, 5, 1e-14)
+// (src/fluid_solvers/backward_euler_diffusion_solver.cpp:10): Jacobi only.
+struct DiffusionSolverSettings {
+    int maxIterations = 1000;
+    int checkInterval = 5;
+    double tolerance = 1e-14;
+};
+
 class Error : public std::runtime_error {
   public:
     Error(const std::string& what, int code) : std::runtime_error(what), code(code) {}
@@ -166,6 +175,10 @@ class HipHybridSimulator {
     // PICSimulator
     void SetPressureSolver(const PressureSolverSettings& settings);
     PressureSolverSettings GetPressureSolver() const { return _solver; }
+    // SetDiffusionSolver (src/pic_simulator.hpp:36) for the reference's one
+    // diffusion solver, BackwardEulerDiffusionSolver; it runs when viscosity > 0.
+    void SetDiffusionSolver(const DiffusionSolverSettings& settings);
+    DiffusionSolverSettings GetDiffusionSolver() const { return _diffusion; }
 
     // Not on this backend.
     void AddCollider(const void*) { throw Unsupported("colliders are not on the GPU backend"); }
@@ -183,6 +196,7 @@ class HipHybridSimulator {
     void GetGrid(int field, void* dst, size_t bytes) const;
     void SetGrid(int field, const void* src, size_t bytes);
     fe_solve_stats LastSolveStats() const;
+    std::vector<fe_solve_stats> LastDiffusionStats() const;  // u, v, w of the last diffusion solve
     int Method() const { return _method; }
     fe_scene* Handle() const { return _scene; }
 
@@ -193,6 +207,7 @@ class HipHybridSimulator {
   private:
     void EnsureScene(int64_t capacity) const;
     void PushParams() const;
+    void PushDiffusionSolver() const;
     fe_scene_desc Desc(int64_t capacity) const;
     void Check(int rc, const char* what) const;
 
@@ -206,6 +221,7 @@ class HipHybridSimulator {
     double _blend = 0.0;
     Vector3<double> _gravity = 0;
     PressureSolverSettings _solver;
+    DiffusionSolverSettings _diffusion;
     Frame _frame;
     double _time = 0.0;
     unsigned int _substeps = 0;

@@ -70,17 +70,19 @@ py::array get_grid(const HipHybridSimulator& sim, int field) {
     if (field >= FE_GRID_U && field <= FE_GRID_W) d[field] += 1;
     if (field >= FE_GRID_MU && field <= FE_GRID_MW) d[field - FE_GRID_MU] += 1;
     if (field >= FE_GRID_SU && field <= FE_GRID_SW) d[field - FE_GRID_SU] += 1;
+    if (field >= FE_GRID_DMU && field <= FE_GRID_DMW) d[field - FE_GRID_DMU] += 1;
     const std::vector<py::ssize_t> shape = {(py::ssize_t)d[2], (py::ssize_t)d[1], (py::ssize_t)d[0]};
     py::array out;
     if (field == FE_GRID_SDF || field == FE_GRID_PRESSURE) out = py::array_t<double>(shape);
-    else if ((field >= FE_GRID_MU && field <= FE_GRID_MW) || field == FE_GRID_FLAGS) out = py::array_t<uint8_t>(shape);
+    else if ((field >= FE_GRID_MU && field <= FE_GRID_MW) || field == FE_GRID_FLAGS ||
+             (field >= FE_GRID_DMU && field <= FE_GRID_DMW))
+        out = py::array_t<uint8_t>(shape);
     else out = py::array_t<float>(shape);
     sim.GetGrid(field, out.mutable_data(), (size_t)out.nbytes());
     return out;
 }
 
-py::dict stats(const HipHybridSimulator& sim) {
-    const fe_solve_stats s = sim.LastSolveStats();
+py::dict stats_dict(const fe_solve_stats& s) {
     py::dict d;
     d["iterations"] = s.iterations;
     d["converged"] = s.converged != 0;
@@ -88,6 +90,22 @@ py::dict stats(const HipHybridSimulator& sim) {
     d["abs_residual"] = s.abs_residual;
     d["ms"] = s.ms;
     return d;
+}
+
+py::dict stats(const HipHybridSimulator& sim) { return stats_dict(sim.LastSolveStats()); }
+
+py::list diffusion_stats(const HipHybridSimulator& sim) {
+    py::list out;
+    for (const fe_solve_stats& s : sim.LastDiffusionStats()) out.append(stats_dict(s));
+    return out;
+}
+
+void set_diffusion_solver(HipHybridSimulator& sim, int max_iterations, int check_interval, double tolerance) {
+    DiffusionSolverSettings s;
+    s.maxIterations = max_iterations;
+    s.checkInterval = check_interval;
+    s.tolerance = tolerance;
+    sim.SetDiffusionSolver(s);
 }
 
 void set_solver(HipHybridSimulator& sim, const std::string& kind, int max_iterations, double tolerance, bool relative,
@@ -188,7 +206,10 @@ PYBIND11_MODULE(PyFluidEngine, m) {
         .def("LastSolveStats", &stats)
         .def("SetPressureSolver", &set_solver, py::arg("kind"), py::arg("max_iterations") = 1000,
              py::arg("tolerance") = 1e-14, py::arg("relative") = false, py::arg("warm_start") = true,
-             py::arg("check_interval") = 5);
+             py::arg("check_interval") = 5)
+        .def("SetDiffusionSolver", &set_diffusion_solver, py::arg("max_iterations") = 1000,
+             py::arg("check_interval") = 5, py::arg("tolerance") = 1e-14)
+        .def("LastDiffusionStats", &diffusion_stats);
 
     py::class_<fe_seed_shape>(m, "SeedShape");
     m.def("Box", &box, py::arg("lo"), py::arg("hi"), "Particle-seeding box [lo, hi)");
@@ -210,4 +231,7 @@ PYBIND11_MODULE(PyFluidEngine, m) {
     m.attr("GRID_SDF") = (int)FE_GRID_SDF;
     m.attr("GRID_FLAGS") = (int)FE_GRID_FLAGS;
     m.attr("GRID_PRESSURE") = (int)FE_GRID_PRESSURE;
+    m.attr("GRID_DMU") = (int)FE_GRID_DMU;
+    m.attr("GRID_DMV") = (int)FE_GRID_DMV;
+    m.attr("GRID_DMW") = (int)FE_GRID_DMW;
 }

@@ -93,6 +93,14 @@ void HipHybridSimulator::EnsureScene(int64_t capacity) const {
     const fe_scene_desc d = Desc(std::max<int64_t>(capacity, 1));
     Check(fe_scene_create(ctx, &d, &_scene), "fe_scene_create");
     _capacity = d.max_particles;
+    PushDiffusionSolver();
+}
+
+void HipHybridSimulator::PushDiffusionSolver() const {
+    if (!_scene) return;
+    const fe_solver_desc sd{FE_SOLVER_JACOBI, _diffusion.maxIterations, _diffusion.checkInterval, 0, 0, 0,
+                            _diffusion.tolerance};
+    Check(fe_scene_set_diffusion_solver(_scene, &sd), "fe_scene_set_diffusion_solver");
 }
 
 void HipHybridSimulator::PushParams() const {
@@ -174,7 +182,7 @@ void HipHybridSimulator::SetMaxClf(double maxCfl) {
 }
 
 void HipHybridSimulator::SetViscosity(double viscosity) {
-    if (viscosity != 0.0) throw Unsupported("viscosity > 0 (BackwardEulerDiffusionSolver) is not on the GPU backend");
+    if (!(viscosity >= 0.0)) throw std::invalid_argument("viscosity must be >= 0");
     _viscosity = viscosity;
     PushParams();
 }
@@ -190,6 +198,19 @@ void HipHybridSimulator::SetBlend(double blend) {
 }
 
 void HipHybridSimulator::SetPressureSolver(const PressureSolverSettings& settings) { _solver = settings; }
+
+void HipHybridSimulator::SetDiffusionSolver(const DiffusionSolverSettings& settings) {
+    if (settings.maxIterations < 0 || settings.checkInterval < 0 || !(settings.tolerance >= 0.0))
+        throw std::invalid_argument("diffusion solver: maxIterations, checkInterval and tolerance must be >= 0");
+    _diffusion = settings;
+    PushDiffusionSolver();
+}
+
+std::vector<fe_solve_stats> HipHybridSimulator::LastDiffusionStats() const {
+    std::vector<fe_solve_stats> st(3);
+    if (_scene) Check(fe_last_diffusion_stats(_scene, st.data()), "fe_last_diffusion_stats");
+    return st;
+}
 
 void HipHybridSimulator::InitializeParticles(const std::vector<fe_seed_shape>& shapes, uint64_t seed) {
     const fe_scene_desc d = Desc(0);

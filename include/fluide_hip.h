@@ -86,7 +86,7 @@ typedef struct fe_scene_desc {
     double blend;         /* FLIP PIC weight (_blendingFactor, src/flip_simulator.cpp:5) */
     double max_cfl;       /* _maxCfl (src/hybrid_simulator.cpp:21) */
     double gravity[3];    /* DirectionalField strength (src/forces/directional_field.cpp:13-16) */
-    double viscosity;     /* must be 0: BackwardEulerDiffusionSolver is not on this backend */
+    double viscosity;     /* >= 0; > 0 adds BackwardEulerDiffusionSolver to the substep */
     double density;       /* unused by the reference's pressure solve (kept for the API) */
     int64_t max_particles;
 } fe_scene_desc;
@@ -105,7 +105,11 @@ enum fe_grid_field {
     FE_GRID_SDF = 9,     /* double, fluid SDF at cell centres (Fluid3::sdf) */
     FE_GRID_FLAGS = 10,  /* uint8 cell markers (fe_marker) */
     FE_GRID_PRESSURE = 11, /* double, pressure (LinearSystem::x) */
-    FE_GRID_RHS = 12     /* float, divergence right-hand side (LinearSystem::b) */
+    FE_GRID_RHS = 12,    /* float, divergence right-hand side (LinearSystem::b) */
+    FE_GRID_DMU = 13,    /* uint8 fe_marker of the u faces in the last diffusion solve
+                            (BackwardEulerDiffusionSolver::_fluidMarkers); download only */
+    FE_GRID_DMV = 14,
+    FE_GRID_DMW = 15
 };
 
 /* Stages of one substep, for fe_run_stage (PICSimulator::OnAdvanceTimeStep,
@@ -120,7 +124,9 @@ enum fe_stage {
     FE_STAGE_PRESSURE = 6,      /* SinglePhasePressureSolver::Solve */
     FE_STAGE_G2P = 7,           /* TransferGrid2Particles */
     FE_STAGE_MOVE = 8,          /* MoveParticles */
-    FE_STAGE_MARKERS = 9        /* SinglePhasePressureSolver::BuildMarkers only */
+    FE_STAGE_MARKERS = 9,       /* SinglePhasePressureSolver::BuildMarkers only */
+    FE_STAGE_DIFFUSION = 10     /* BackwardEulerDiffusionSolver::Solve (ComputeDiffusion
+                                   without its boundary condition); solver NULL = the scene's */
 };
 
 typedef struct fe_ctx fe_ctx;
@@ -140,10 +146,21 @@ int fe_scene_sync(fe_scene* scene);
 /* Simulator settings changed after construction: SetBlendingFactor
  * (src/flip_simulator.hpp:14-16), SetMaxClf, SetViscosity, SetDensity
  * (src/hybrid_simulator.hpp:37-40) and the DirectionalField forces of
- * AddExternalForce, summed (src/pic_simulator.cpp:41-44).  viscosity != 0 is
- * FE_E_UNSUPPORTED. */
+ * AddExternalForce, summed (src/pic_simulator.cpp:41-44).  viscosity > 0 runs
+ * ComputeDiffusion in every substep (src/pic_simulator.cpp:214-222). */
 int fe_scene_set_params(fe_scene* scene, double blend, double max_cfl, const double gravity[3], double viscosity,
                         double density);
+
+/* The linear solver of the scene's BackwardEulerDiffusionSolver, fixed in the
+ * reference to CudaJacobiIterationSolver(1000, 5, 1e-14)
+ * (src/fluid_solvers/backward_euler_diffusion_solver.cpp:10), which is the
+ * default here; kind must be FE_SOLVER_JACOBI (else FE_E_UNSUPPORTED).
+ * Replaces PICSimulator::SetDiffusionSolver (src/pic_simulator.hpp:36) for the
+ * one diffusion solver the reference has. */
+int fe_scene_set_diffusion_solver(fe_scene* scene, const fe_solver_desc* solver);
+/* Per component (u, v, w) of the last diffusion solve: Jacobi sweeps run,
+ * converged flag and the last checked ||b - A x||_2 (-1 when none was checked). */
+int fe_last_diffusion_stats(const fe_scene* scene, fe_solve_stats stats[3]);
 
 /* Particles: ParticleSystem "postion"/"velocity" attributes
  * (src/hybrid_simulator.hpp:50-51) and the APIC affine rows _cX/_cY/_cZ

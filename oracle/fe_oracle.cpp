@@ -292,6 +292,14 @@ struct orc_scene {
     std::vector<double> x, vel, C;  // particles (3N, 3N, 9N)
     int64_t np = 0;
     orc_solve_stats last{};
+    // BackwardEulerDiffusionSolver state: viscosity (Fluid3::viscosity), its
+    // linear solver (CudaJacobiIterationSolver(1000, 5, 1e-14),
+    // src/fluid_solvers/backward_euler_diffusion_solver.cpp:10), the face
+    // markers of the last solve per component (_fluidMarkers) and its stats.
+    double viscosity = 0.0;
+    orc_solver_desc diff{ORC_SOLVER_JACOBI, 1000, 5, 0, 0, 0, 1e-14};
+    std::vector<uint8_t> dmark[3];
+    orc_solve_stats diff_last[3]{};
 };
 
 namespace {
@@ -913,6 +921,106 @@ void pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
 
 void apply_bc(orc_scene* s) { constrain_velocity(s, static_cast<int>(std::ceil(s->maxCfl))); }
 
+// ---------------------------------------------------------------------------
+// BackwardEulerDiffusionSolver::Solve (src/fluid_solvers/backward_euler_diffusion_solver.cpp:20-49),
+// as PICSimulator::ComputeDiffusion calls it (src/pic_simulator.cpp:214-222).
+// The call passes (dt, viscosity) into parameters named (viscosity, dt); the
+// coefficient c = (dt * viscosity) / (h * h) is the same product either way.
+// ---------------------------------------------------------------------------
+// BuildX/Y/ZMarkers (:71-128): FLUID iff fluidSdf.Sample(face position) < 0,
+// else BOUNDRY iff colliderSdf.Sample(face position) < 0, else AIR; face
+// position GetX/Y/ZPos = data origin + h * (i, j, k) (face_centered_grid3d.cpp:303-316).
+void diffusion_markers(orc_scene* s, int c) {
+    const Arr3& a = comp_arr(s, c);
+    const double* fo = comp_origin(s, c);
+    const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
+    std::vector<uint8_t>& m = s->dmark[c];
+    m.assign(a.size(), ORC_AIR);
+    par_for(s->nthreads, a.sz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < a.sy; ++j)
+                for (int i = 0; i < a.sx; ++i) {
+                    const double pos[3] = {fo[0] + s->h[0] * (double)i, fo[1] + s->h[1] * (double)j,
+                                           fo[2] + s->h[2] * (double)k};
+                    uint8_t f = ORC_AIR;
+                    if (sample_array(s->sdf, so, s->h, pos) < 0)
+                        f = ORC_FLUID;
+                    else if (sample_collider(s, pos) < 0)
+                        f = ORC_BOUNDRY;
+                    m[a.at(i, j, k)] = f;
+                }
+    });
+}
+
+// BuildMatrix (:136-198) and BuildVectors (:200-242) into a row system on the
+// component's face grid: a FLUID row has centre 1 + c_a per in-range non-AIR
+// neighbour (+ neighbour first, then -, axes x, y, z), -c_a to each FLUID +
+// neighbour (right/up/front), and b = u + c_a * u(neighbour) per BOUNDRY
+// neighbour; every other row is the identity with b = u.  x starts at u.
+void diffusion_system(orc_scene* s, int c, const double cc[3], orc_scene& sys) {
+    const Arr3& a = comp_arr(s, c);
+    const std::vector<uint8_t>& m = s->dmark[c];
+    const int nx = a.sx, ny = a.sy, nz = a.sz;
+    sys.n[0] = nx; sys.n[1] = ny; sys.n[2] = nz;
+    sys.nthreads = s->nthreads;
+    sys.Ac.resize(nx, ny, nz, 0.0); sys.Ar.resize(nx, ny, nz, 0.0);
+    sys.Au.resize(nx, ny, nz, 0.0); sys.Af.resize(nx, ny, nz, 0.0);
+    sys.b.resize(nx, ny, nz, 0.0); sys.p.resize(nx, ny, nz, 0.0);
+    auto M = [&](int i, int j, int k) { return m[a.at(i, j, k)]; };
+    par_for(s->nthreads, nz, [&](int64_t k0, int64_t k1) {
+        for (int k = (int)k0; k < (int)k1; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    double center = 1.0, right = 0.0, up = 0.0, front = 0.0;
+                    double b = a(i, j, k);
+                    if (M(i, j, k) == ORC_FLUID) {
+                        if (i + 1 < nx) {
+                            if (M(i + 1, j, k) != ORC_AIR) center += cc[0];
+                            if (M(i + 1, j, k) == ORC_FLUID) right -= cc[0];
+                        }
+                        if (i > 0 && M(i - 1, j, k) != ORC_AIR) center += cc[0];
+                        if (j + 1 < ny) {
+                            if (M(i, j + 1, k) != ORC_AIR) center += cc[1];
+                            if (M(i, j + 1, k) == ORC_FLUID) up -= cc[1];
+                        }
+                        if (j > 0 && M(i, j - 1, k) != ORC_AIR) center += cc[1];
+                        if (k + 1 < nz) {
+                            if (M(i, j, k + 1) != ORC_AIR) center += cc[2];
+                            if (M(i, j, k + 1) == ORC_FLUID) front -= cc[2];
+                        }
+                        if (k > 0 && M(i, j, k - 1) != ORC_AIR) center += cc[2];
+                        if (i + 1 < nx && M(i + 1, j, k) == ORC_BOUNDRY) b += cc[0] * a(i + 1, j, k);
+                        if (i > 0 && M(i - 1, j, k) == ORC_BOUNDRY) b += cc[0] * a(i - 1, j, k);
+                        if (j + 1 < ny && M(i, j + 1, k) == ORC_BOUNDRY) b += cc[1] * a(i, j + 1, k);
+                        if (j > 0 && M(i, j - 1, k) == ORC_BOUNDRY) b += cc[1] * a(i, j - 1, k);
+                        if (k + 1 < nz && M(i, j, k + 1) == ORC_BOUNDRY) b += cc[2] * a(i, j, k + 1);
+                        if (k > 0 && M(i, j, k - 1) == ORC_BOUNDRY) b += cc[2] * a(i, j, k - 1);
+                    }
+                    sys.Ac(i, j, k) = center; sys.Ar(i, j, k) = right;
+                    sys.Au(i, j, k) = up; sys.Af(i, j, k) = front;
+                    sys.b(i, j, k) = b;
+                    sys.p(i, j, k) = a(i, j, k);
+                }
+    });
+}
+
+void diffusion(orc_scene* s, double dt) {
+    const double cc[3] = {(dt * s->viscosity) / (s->h[0] * s->h[0]), (dt * s->viscosity) / (s->h[1] * s->h[1]),
+                          (dt * s->viscosity) / (s->h[2] * s->h[2])};
+    for (int c = 0; c < 3; ++c) {
+        diffusion_markers(s, c);
+        orc_scene sys{};
+        diffusion_system(s, c, cc, sys);
+        orc_solve_stats st{};
+        if (s->diff.kind == ORC_SOLVER_JACOBI)
+            jacobi(&sys, &s->diff, &st);
+        else
+            cg(&sys, &s->diff, &st);
+        comp_arr(s, c).d = sys.p.d;  // output->x/y/z(i, j, k) = _system.x(i, j, k)
+        s->diff_last[c] = st;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1049,6 +1157,19 @@ void orc_move_particles(orc_scene* s, double dt) { move_particles(s, dt); }
 
 void orc_pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) { pressure(s, d, st); }
 
+void orc_set_viscosity(orc_scene* s, double viscosity) { s->viscosity = viscosity; }
+void orc_set_diffusion_solver(orc_scene* s, const orc_solver_desc* d) { s->diff = *d; }
+void orc_diffusion(orc_scene* s, double dt, orc_solve_stats st[3]) {
+    diffusion(s, dt);
+    if (st)
+        for (int c = 0; c < 3; ++c) st[c] = s->diff_last[c];
+}
+int orc_diffusion_markers(const orc_scene* s, int c, uint8_t* out) {
+    if (c < 0 || c > 2) return -1;
+    std::memcpy(out, s->dmark[c].data(), s->dmark[c].size());
+    return (int)s->dmark[c].size();
+}
+
 // The substep: PICSimulator::OnAdvanceTimeStep (src/pic_simulator.cpp:111-154) with
 // OnBeginAdvanceTimeStep (:156-196), ComputeExternalForces (:203-212), ComputeDiffusion
 // (:214-222, skipped at viscosity 0), ComputePressure (:224-229), ComputeAdvection (:231-237).
@@ -1059,6 +1180,10 @@ void orc_substep(orc_scene* s, double dt, const orc_solver_desc* d, orc_solve_st
     apply_bc(s);
     apply_gravity(s, dt);
     apply_bc(s);
+    if (s->viscosity > 0.0) {
+        diffusion(s, dt);
+        apply_bc(s);
+    }
     pressure(s, d, st);
     apply_bc(s);
     extrapolate_to_air(s);

@@ -63,6 +63,13 @@ void orc_g2p(orc_scene* s);
 void orc_move_particles(orc_scene* s, double dt);
 void orc_pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st);
 void orc_substep(orc_scene* s, double dt, const orc_solver_desc* d, orc_solve_stats* st);
+// BackwardEulerDiffusionSolver: viscosity > 0 adds ComputeDiffusion to orc_substep;
+// orc_diffusion runs it alone (no boundary condition after it); markers of the
+// last solve per component, uint8 FluidMarker values, face-array sized.
+void orc_set_viscosity(orc_scene* s, double viscosity);
+void orc_set_diffusion_solver(orc_scene* s, const orc_solver_desc* d);
+void orc_diffusion(orc_scene* s, double dt, orc_solve_stats st[3]);
+int orc_diffusion_markers(const orc_scene* s, int c, uint8_t* out);
 double orc_cfl(const orc_scene* s, double dt);
 unsigned orc_num_substeps(const orc_scene* s, double dt);
 int orc_advance_frame(orc_scene* s, double frame_dt, const orc_solver_desc* d);

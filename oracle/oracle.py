@@ -101,6 +101,10 @@ def lib():
         L.orc_solve_rows.argtypes = [dp, dp, dp, dp, dp, dp, ip, C.POINTER(SolverDesc), C.POINTER(SolveStats)]
         L.orc_residual_rows.argtypes = [dp, dp, dp, dp, dp, dp, ip, dp]
         L.orc_spmv_rows.argtypes = [dp, dp, dp, dp, dp, ip, dp]
+        L.orc_set_viscosity.argtypes = [C.c_void_p, C.c_double]
+        L.orc_set_diffusion_solver.argtypes = [C.c_void_p, C.POINTER(SolverDesc)]
+        L.orc_diffusion.argtypes = [C.c_void_p, C.c_double, C.POINTER(SolveStats)]
+        L.orc_diffusion_markers.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         _lib = L
     return _lib
 
@@ -233,6 +237,23 @@ class Scene:
         st = SolveStats()
         lib().orc_substep(self._s, dt, C.byref(desc), C.byref(st))
         return st
+
+    def set_viscosity(self, nu):
+        lib().orc_set_viscosity(self._s, nu)
+
+    def set_diffusion_solver(self, desc):
+        lib().orc_set_diffusion_solver(self._s, C.byref(desc))
+
+    def diffusion(self, dt):
+        """BackwardEulerDiffusionSolver::Solve on u, v, w in place; stats per component."""
+        st = (SolveStats * 3)()
+        lib().orc_diffusion(self._s, dt, st)
+        return list(st)
+
+    def diffusion_markers(self, c):
+        out = np.empty(self.shape((U, V, W)[c]), dtype=np.uint8)
+        lib().orc_diffusion_markers(self._s, c, out.ctypes.data_as(C.c_void_p))
+        return out
 
     def cfl(self, dt):
         return lib().orc_cfl(self._s, dt)

@@ -82,3 +82,36 @@ def test_cpp_demo_mgpcg_apic():
     r = json.loads(out.stdout.strip().splitlines()[-1])
     assert r["particles"] == 63232 and r["substeps"] == 3
     assert r["last_rel_residual"] <= 1.05e-6 and 0 < r["last_iterations"] < 60
+
+
+def test_python_viscosity_matches_oracle():
+    """SetViscosity > 0 through PyFluidEngine runs BackwardEulerDiffusionSolver
+    in every substep; the frame agrees with the oracle run the same way."""
+    print("[gpu] host API: viscosity", flush=True)
+    P = _py_module()
+    n, nu = 16, 0.05
+    sim = P.FLIPSimulator(P.Vector3S(n, n, n), P.BoundingBox3D(P.Vector3D(0, 0, 0), P.Vector3D(1, 1, 1)))
+    sim.AddExternalForce(P.DirectionalField(P.Vector3D(0, -9.81, 0)))
+    sim.SetPressureSolver("jacobi", 200, 1e-14)
+    sim.SetViscosity(nu)
+    sim.SetDiffusionSolver(1000, 5, 1e-14)
+    sim.InitializeParticles([P.Box([-1, -1, -1], [0.4, 0.6, 2.0])], 0)
+    sim.SetCurrentFrame(P.Frame(DT))
+    for _ in range(2):
+        sim.AdvanceSingleFrame()
+    st = sim.LastDiffusionStats()
+    assert len(st) == 3 and all(s["iterations"] > 5 and s["converged"] for s in st)
+    dmu = sim.GetGrid(P.GRID_DMU)
+    assert dmu.shape == (n, n, n + 1) and (dmu == 1).any()
+    px, pv, _ = sim.GetParticles()
+
+    desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.05, max_particles=px.shape[0])
+    x0 = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.4, 0.6, 2.0))], seed=0)
+    o = orc.Scene((n, n, n), method=orc.FLIP, blend=0.05)
+    o.set_viscosity(nu)
+    o.set_particles(x0.astype(np.float64), np.zeros(x0.shape))
+    for _ in range(2):
+        o.advance_frame(DT, orc.solver(orc.JACOBI, 200, 5, 1e-14))
+    ox, ov, _ = o.particles()
+    assert np.abs(pv - ov).max() <= 2e-4 * max(1.0, float(np.abs(ov).max()))
+    assert np.array_equal(dmu, o.diffusion_markers(0))

@@ -68,10 +68,17 @@ def test_frame(P):
 def test_unsupported_paths(P):
     s = P.APICSimulator(P.Vector3S(8, 8, 8), box(P))
     for call in (lambda: s.AddCollider(None), lambda: s.AddEmitter(None), lambda: s.GetSurface(None),
-                 lambda: s.InitializeFromTriangleMesh(None), lambda: s.SetViscosity(0.1)):
+                 lambda: s.InitializeFromTriangleMesh(None)):
         with pytest.raises(NotImplementedError):
             call()
+    s.SetViscosity(0.1)  # BackwardEulerDiffusionSolver runs on the GPU from the next substep
+    assert s.GetViscosity() == 0.1
     s.SetViscosity(0.0)
+    with pytest.raises(ValueError):
+        s.SetViscosity(-1.0)
+    s.SetDiffusionSolver(500, 5, 1e-12)
+    with pytest.raises(ValueError):
+        s.SetDiffusionSolver(-1)
     with pytest.raises(ValueError):
         s.SetPressureSolver("gauss-seidel")
     s.SetPressureSolver("mgpcg", 2000, 1e-6, True)
