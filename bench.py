@@ -271,6 +271,8 @@ def main():
                            traffic_for("cg_update") if on_profile else None)
     scene.kernel_timer(KERNEL_IDS["none"])
 
+    solve_cmp = None if args.solver == "jacobi" else solver_comparison(scene, capi, dt)
+
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
     cpu = None
@@ -300,11 +302,44 @@ def main():
         "stage_ms_per_step": {k: round(val / max(args.steps, 1), 4) for k, val in stage_acc.items()},
         "roofline": roof,
         "roofline_cg_update": roof_cg,
+        "pressure_solver_comparison": solve_cmp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     scene.close()
     ctx.close()
+
+
+def solver_comparison(scene, capi, dt):
+    """One pressure projection (build + solve + apply, device ms) on the input a
+    substep gives it -- the current particles through P2G, SDF, extrapolation,
+    walls and gravity -- by plain CG (FP32 vectors, FP64 dots) from a zero
+    guess, multigrid-PCG from a zero guess and multigrid-PCG warm-started from
+    the previous pressure, all to 1e-6 relative residual.  The grid and the
+    pressure are restored before each run."""
+    for stage in (capi.STAGE_P2G, capi.STAGE_SDF, capi.STAGE_EXTRAPOLATE, capi.STAGE_BC, capi.STAGE_GRAVITY,
+                  capi.STAGE_BC):
+        scene.run_stage(stage, dt)
+    fields = (capi.GRID_U, capi.GRID_V, capi.GRID_W, capi.GRID_PRESSURE)
+    saved = {f: scene.get(f) for f in fields}
+    out = {}
+    for name, kind, warm in (("cg_zero_guess", capi.SOLVER_CG, False),
+                             ("mgpcg_zero_guess", capi.SOLVER_MGPCG, False),
+                             ("mgpcg_warm_start", capi.SOLVER_MGPCG, True)):
+        best = None
+        for _ in range(3):
+            for f, a in saved.items():
+                scene.set(f, a)
+            st = scene.run_stage(capi.STAGE_PRESSURE, dt,
+                                 capi.solver(kind, 5000, 0, 1e-6, relative=True, warm_start=warm))
+            if best is None or st.ms < best.ms:
+                best = st
+        out[name] = {"ms": round(best.ms, 4), "iterations": best.iterations,
+                     "rel_residual": float(f"{best.rel_residual:.3e}")}
+    for f, a in saved.items():
+        scene.set(f, a)
+    out["mgpcg_warm_over_cg_time"] = round(out["mgpcg_warm_start"]["ms"] / out["cg_zero_guess"]["ms"], 4)
+    return out
 
 
 def cpu_baseline(scene, capi, desc, dt, args):
