@@ -92,6 +92,10 @@ def parse():
     ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-frames", type=int, default=10,
+                    help="frames of the 96-scene batch workload after the headline run (0 skips it)")
+    ap.add_argument("--batch-workers", type=int, nargs="+", default=[4],
+                    help="host worker threads of the batch (several values: one measurement each)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--summarize-trace", metavar="STATS_CSV",
                     help="write profiles/flip128_kernel_trace.md from a rocprofv3 --stats kernel summary and exit")
@@ -272,6 +276,9 @@ def main():
     scene.kernel_timer(KERNEL_IDS["none"])
 
     solve_cmp = None if args.solver == "jacobi" else solver_comparison(scene, capi, dt)
+    batch = None
+    if args.batch_frames > 0:
+        batch = [batch_workload(capi, ctx, args.batch_frames, w) for w in args.batch_workers]
 
     if args.copy_calibration:
         copy_calibration(ctx, args.copy_calibration)
@@ -303,6 +310,7 @@ def main():
         "roofline": roof,
         "roofline_cg_update": roof_cg,
         "pressure_solver_comparison": solve_cmp,
+        "batch_96_scenes": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -340,6 +348,47 @@ def solver_comparison(scene, capi, dt):
         scene.set(f, a)
     out["mgpcg_warm_over_cg_time"] = round(out["mgpcg_warm_start"]["ms"] / out["cg_zero_guess"]["ms"], 4)
     return out
+
+
+def batch_workload(capi, ctx, frames, workers, warmup=2):
+    """96 independent FLIP dam breaks on cubic grids n ~ U{24,32,48,64,96}
+    (seed 1), each one fluid box with extents ~ U[0.2, 0.7], 8 particles per
+    fluid cell, multigrid-PCG to 1e-6, frame dt = 1/480, advanced together by
+    fe_batch_advance_frame.  Reports scenes/s (scene-frames per second), the
+    whole batch's particle-substeps/s and the per-scene substep wall time."""
+    specs = capi.batch_scene_specs()
+    scenes, npart = [], 0
+    for n, lo, hi in specs:
+        desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.03, max_cfl=3.0, gravity=(0.0, -9.81, 0.0),
+                               max_particles=8 * n ** 3)
+        x = capi.seed_particles(desc, [capi.box(lo, hi)], seed=n)
+        desc.max_particles = max(1, x.shape[0])
+        sc = capi.Scene(ctx, desc)
+        sc.upload_particles(x, np.zeros_like(x))
+        scenes.append((sc, x.shape[0]))
+        npart += x.shape[0]
+    solver = capi.solver(capi.SOLVER_MGPCG, 2000, 0, 1e-6, relative=True, warm_start=True)
+    b = capi.Batch(ctx, [sc for sc, _ in scenes], workers)
+    dt = 1.0 / 480.0
+    for _ in range(warmup):
+        b.advance_frame(dt, solver)
+    per_substep, psub = [], 0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sub, ms = b.advance_frame(dt, solver)
+        psub += sum(cnt * s for (_, cnt), s in zip(scenes, sub))
+        per_substep.extend(m / max(s, 1) for m, s in zip(ms, sub))
+    wall = time.perf_counter() - t0
+    b.close()
+    for sc, _ in scenes:
+        sc.close()
+    per_substep.sort()
+    q = lambda f: per_substep[min(len(per_substep) - 1, int(f * len(per_substep)))]
+    return {"scenes": len(specs), "particles": npart, "frames": frames, "workers": workers,
+            "scenes_per_s": round(len(specs) * frames / wall, 2),
+            "mparticle_substeps_per_s": round(psub / wall / 1e6, 2),
+            "scene_substep_ms_p50": round(q(0.5), 4), "scene_substep_ms_p99": round(q(0.99), 4),
+            "wall_s": round(wall, 3)}
 
 
 def cpu_baseline(scene, capi, desc, dt, args):

@@ -29,7 +29,8 @@ EXPORTED = [
     "fe_advance_frame", "fe_cfl", "fe_run_stage", "fe_grid_download", "fe_grid_upload", "fe_grid_bytes",
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
-    "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats",
+    "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats", "fe_batch_create", "fe_batch_destroy",
+    "fe_batch_advance_frame",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -106,6 +107,10 @@ def load():
     L.fe_scene_set_params.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double), C.c_double, C.c_double]
     L.fe_scene_set_diffusion_solver.argtypes = [vp, C.POINTER(SolverDesc)]
     L.fe_last_diffusion_stats.argtypes = [vp, C.POINTER(SolveStats)]
+    L.fe_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.fe_batch_destroy.argtypes = [vp]
+    L.fe_batch_advance_frame.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_double)]
     L.fe_version.restype = C.c_char_p
     for name in EXPORTED:
         getattr(L, name).restype = C.c_char_p if name in ("fe_last_error", "fe_version") else C.c_int
@@ -150,6 +155,21 @@ def sphere(centre, radius):
     s.lo[:] = list(centre)
     s.hi[:] = [radius, 0.0, 0.0]
     return s
+
+
+def batch_scene_specs(count=96, seed=1, sizes=(24, 32, 48, 64, 96), extent=(0.2, 0.7)):
+    """The batched-scenes workload: `count` cubic grids with n drawn uniformly
+    from `sizes`, each with one axis-aligned fluid box of extents drawn from
+    U[extent] per axis, placed uniformly inside the unit cube; returns a list
+    of (n, lo, hi)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(sizes))
+        ext = rng.uniform(extent[0], extent[1], 3)
+        lo = rng.uniform(0.0, 1.0 - ext)
+        out.append((n, tuple(float(a) for a in lo), tuple(float(a) for a in lo + ext)))
+    return out
 
 
 def seed_particles(desc, shapes, seed=0):
@@ -338,3 +358,34 @@ class Scene:
         self.ctx.check(self.L.fe_kernel_timer_read(self._h, C.byref(ms), C.byref(n), int(reset)),
                        "fe_kernel_timer_read")
         return ms.value, n.value
+
+
+class Batch:
+    """fe_batch: independent scenes of one context advanced together by host
+    worker threads, each scene on its own stream."""
+
+    def __init__(self, ctx: Context, scenes, workers=4):
+        self.ctx, self.L = ctx, ctx.L
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[sc._h for sc in self.scenes])
+        h = C.c_void_p()
+        ctx.check(self.L.fe_batch_create(ctx._h, arr, len(self.scenes), workers, C.byref(h)), "fe_batch_create")
+        self._h = h
+
+    def advance_frame(self, frame_dt, desc=None):
+        """One frame of every scene; returns (substeps, wall ms) per scene."""
+        n = len(self.scenes)
+        sub = (C.c_int32 * n)()
+        ms = (C.c_double * n)()
+        desc = desc or solver()
+        self.ctx.check(self.L.fe_batch_advance_frame(self._h, frame_dt, C.byref(desc), sub, ms),
+                       "fe_batch_advance_frame")
+        return list(sub), list(ms)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fe_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

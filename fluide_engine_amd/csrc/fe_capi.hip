@@ -2,9 +2,13 @@
 // transfer and the substep orchestration of PICSimulator::OnAdvanceTimeStep
 // (src/pic_simulator.cpp:111-237) over the gfx950 kernels.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "fe_stencil.hpp"
 
@@ -15,9 +19,10 @@ namespace {
 thread_local std::string g_err_no_ctx;
 
 int set_err(fe_ctx* ctx, int code, const std::string& msg) {
-    if (ctx)
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->err_mu);
         ctx->err = msg;
-    else
+    } else
         g_err_no_ctx = msg;
     return code;
 }
@@ -588,6 +593,82 @@ int fe_advance_frame(fe_scene* s, double frame_dt, const fe_solver_desc* d, int3
         ++count;
     }
     if (substeps_out) *substeps_out = count;
+    return FE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent scenes.  Every scene owns a stream; host worker
+// threads take scenes largest first from a shared counter and advance each by
+// one frame, so the kernels of different scenes overlap on the device's
+// hardware queues and the host-side waits of one scene (solver checks, CFL)
+// hide behind the work of the others.
+// ---------------------------------------------------------------------------
+int fe_batch_create(fe_ctx* ctx, fe_scene* const* scenes, int32_t count, int32_t workers, fe_batch** out) {
+    if (!ctx || !out || (count > 0 && !scenes) || count < 0) return set_err(ctx, FE_E_INVALID, "NULL argument");
+    *out = nullptr;
+    if (workers < 1 || workers > 64) return set_err(ctx, FE_E_INVALID, "workers must be in [1, 64]");
+    auto* b = new fe_batch();
+    b->ctx = ctx;
+    b->workers = workers;
+    for (int32_t i = 0; i < count; ++i) {
+        if (!scenes[i] || scenes[i]->ctx != ctx) {
+            delete b;
+            return set_err(ctx, FE_E_INVALID, "batch scene " + std::to_string(i) + " is NULL or of another context");
+        }
+        for (int32_t j = 0; j < i; ++j)
+            if (scenes[j] == scenes[i]) {
+                delete b;
+                return set_err(ctx, FE_E_INVALID, "scene listed twice in a batch");
+            }
+        b->scenes.push_back(scenes[i]);
+    }
+    // largest scenes first: the longest frames start early (LPT order)
+    b->order.resize(b->scenes.size());
+    for (size_t i = 0; i < b->order.size(); ++i) b->order[i] = (int32_t)i;
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int32_t x, int32_t y) {
+        const fe_scene* a = b->scenes[x];
+        const fe_scene* c = b->scenes[y];
+        return cell_count(a->g) + a->np > cell_count(c->g) + c->np;
+    });
+    *out = b;
+    return FE_OK;
+}
+
+int fe_batch_destroy(fe_batch* b) {
+    delete b;
+    return FE_OK;
+}
+
+int fe_batch_advance_frame(fe_batch* b, double frame_dt, const fe_solver_desc* d, int32_t* substeps, double* ms) {
+    if (!b) return FE_E_INVALID;
+    fe_ctx* ctx = b->ctx;
+    int rc = check_solver(ctx, d);
+    if (rc != FE_OK) return rc;
+    const int n = (int)b->scenes.size();
+    std::atomic<int> next{0};
+    std::vector<int> rcs(n, FE_OK);
+    auto work = [&]() {
+        if (hipSetDevice(ctx->device) != hipSuccess) return;
+        for (;;) {
+            const int t = next.fetch_add(1);
+            if (t >= n) return;
+            const int i = b->order[t];
+            const auto t0 = std::chrono::steady_clock::now();
+            int32_t sub = 0;
+            rcs[i] = fe_advance_frame(b->scenes[i], frame_dt, d, &sub);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (substeps) substeps[i] = sub;
+            if (ms) ms[i] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        }
+    };
+    const int nw = std::min(b->workers, std::max(n, 1));
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nw; ++w) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+    if (next.load() < n) return set_err(ctx, FE_E_HIP, "hipSetDevice failed on a batch worker thread");
+    for (int i = 0; i < n; ++i)
+        if (rcs[i] != FE_OK) return rcs[i];
     return FE_OK;
 }
 

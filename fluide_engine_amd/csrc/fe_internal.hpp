@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -111,6 +112,17 @@ __device__ inline int cell_axis(const Geom& g, int a, float p) {
     float fr;
     axis_floor(p, g.o[a], g.ih[a], g.od[a], g.hd[a], s, fr);
     return s < 0 ? 0 : (s > g.n[a] - 1 ? g.n[a] - 1 : s);
+}
+
+// (i, j, k) of the linear index f of an x-fastest sx * sy * sz array.  Every
+// array here holds fewer than 2^31 elements (a scene has at most 2^30 cells),
+// so 32-bit division suffices (64-bit division is a long software sequence).
+__device__ inline void split_index(int64_t f, int sx, int sy, int& i, int& j, int& k) {
+    const uint32_t u = (uint32_t)f, ux = (uint32_t)sx, uy = (uint32_t)sy;
+    const uint32_t t = u / ux;
+    i = (int)(u - t * ux);
+    j = (int)(t % uy);
+    k = (int)(t / uy);
 }
 
 // GetBarycentric(x, 0, size-1) in FP64 (src/common/math_utils.hpp:48-73).
@@ -275,6 +287,7 @@ struct MGLevel {
 struct fe_ctx {
     int device = 0;
     std::string err;
+    std::mutex err_mu;  // scenes of one context may run on several host threads (fe_batch)
     char name[64] = {0};
 };
 
@@ -326,6 +339,7 @@ struct fe_scene {
     hipEvent_t ev[fe::kMaxStages + 1] = {};
     double stage_ms[fe::kMaxStages] = {};
     fe_solve_stats last_stats{};
+    int last_cg_iters[3] = {0, 0, 0};  // iterations of the last CG solve per preconditioner (none, diagonal, multigrid)
     std::vector<fe::MGLevel> mg;
     int mg_tail = 0;  // first level handled by the one-workgroup tail kernel  // multigrid hierarchy, built on first use
 
@@ -345,6 +359,13 @@ struct fe_scene {
     int kt_used = 0;
     double kt_ms = 0.0;
     int64_t kt_launches = 0;
+};
+
+struct fe_batch {
+    fe_ctx* ctx = nullptr;
+    std::vector<fe_scene*> scenes;  // not owned
+    std::vector<int32_t> order;     // scene indices, largest first
+    int workers = 1;
 };
 
 namespace fe {

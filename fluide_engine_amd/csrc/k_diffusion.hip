@@ -34,10 +34,7 @@ struct FaceDims {
 };
 
 __device__ inline void face_ijk(const FaceDims& d, int64_t f, int& i, int& j, int& k) {
-    i = (int)(f % d.nx);
-    const int64_t t = f / d.nx;
-    j = (int)(t % d.ny);
-    k = (int)(t / d.ny);
+    split_index(f, d.nx, d.ny, i, j, k);
 }
 
 // Markers and row codes of component C, and x0 = u.

@@ -61,9 +61,8 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict
     const int64_t c = blk * kBlock + threadIdx.x;
     const int64_t nc = cell_count(g);
     if (c >= nc) return;
-    const int i = (int)(c % g.n[0]);
-    const int j = (int)((c / g.n[0]) % g.n[1]);
-    const int k = (int)(c / ((int64_t)g.n[0] * g.n[1]));
+    int i, j, k;
+    split_index(c, g.n[0], g.n[1], i, j, k);
     // Grid3D::GridIndexToPosition on the SDF grid of origin o + h/2 (src/hybrid_simulator.cpp:13-15)
     const double gx = (g.od[0] + g.hd[0] / 2.0) + (double)i * g.hd[0];
     const double gy = (g.od[1] + g.hd[1] / 2.0) + (double)j * g.hd[1];
@@ -149,7 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_markers(Geom g, const double* __rest
     const int64_t c = blk * kBlock + threadIdx.x;
     const int64_t nc = cell_count(g);
     if (c >= nc) return;
-    const int idx[3] = {(int)(c % g.n[0]), (int)((c / g.n[0]) % g.n[1]), (int)(c / ((int64_t)g.n[0] * g.n[1]))};
+    int idx[3];
+    split_index(c, g.n[0], g.n[1], idx[0], idx[1], idx[2]);
     double cc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) cc[a] = (g.od[a] + 0.5 * g.hd[a]) + g.hd[a] * (double)idx[a];  // GetCellCenterPos
@@ -174,54 +174,71 @@ struct ValidSet {
     uint8_t* out[3];
 };
 
-__global__ __launch_bounds__(kBlock) void k_extrapolate(Geom g, FaceSet a, ValidSet vs, int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    int64_t f = blk * kBlock + threadIdx.x;
-    int c = 0;
-    int64_t nf = face_count(g, 0);
-    if (f >= nf) {
-        f -= nf;
-        c = 1;
-        nf = face_count(g, 1);
-        if (f >= nf) {
-            f -= nf;
-            c = 2;
-            nf = face_count(g, 2);
-            if (f >= nf) return;
-        }
-    }
+// Each block covers kBlock * kExtraPer faces of one component (blocks
+// [0, b0) u, [b0, b1) v, [b1, ...) w), so the component and its arrays are
+// uniform per block.  A thread takes kExtraPer faces a block apart and issues
+// all their validity loads (own + six neighbours) before it uses any: the
+// sweep is bound by load latency, not bytes.
+constexpr int kExtraPer = 4;
+
+__global__ __launch_bounds__(kBlock) void k_extrapolate(Geom g, FaceSet a, ValidSet vs, int64_t b0, int64_t b1) {
+    const int64_t blk = blockIdx.x;
+    const int c = blk < b0 ? 0 : (blk < b1 ? 1 : 2);
+    const int64_t lb = blk - (c == 0 ? 0 : (c == 1 ? b0 : b1));
+    const uint8_t* __restrict__ vin = c == 0 ? vs.in[0] : (c == 1 ? vs.in[1] : vs.in[2]);
+    uint8_t* __restrict__ vout = c == 0 ? vs.out[0] : (c == 1 ? vs.out[1] : vs.out[2]);
+    float* __restrict__ A = c == 0 ? a.a[0] : (c == 1 ? a.a[1] : a.a[2]);
     const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
-    const uint8_t* vin = vs.in[c];
-    if (vin[f]) {
-        vs.out[c][f] = 1;
-        return;
+    const int64_t nf = (int64_t)sx * sy * sz;
+    const int64_t st[3] = {1, sx, (int64_t)sx * sy};
+    int64_t F[kExtraPer];
+    uint8_t self[kExtraPer], nb[kExtraPer][6];
+#pragma unroll
+    for (int q = 0; q < kExtraPer; ++q) {
+        const int64_t f = lb * (kBlock * kExtraPer) + q * kBlock + threadIdx.x;
+        F[q] = f;
+        self[q] = 1;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) nb[q][d] = 0;
+        if (f >= nf) continue;
+        int i, j, k;
+        split_index(f, sx, sy, i, j, k);
+        self[q] = vin[f];
+        // neighbour order of ExtrapolateToRegion: i+1, i-1, j+1, j-1, k+1, k-1
+        const bool in[6] = {i + 1 < sx, i > 0, j + 1 < sy, j > 0, k + 1 < sz, k > 0};
+#pragma unroll
+        for (int d = 0; d < 6; ++d) nb[q][d] = in[d] ? vin[(d & 1) ? f - st[d >> 1] : f + st[d >> 1]] : (uint8_t)0;
     }
-    const int i = (int)(f % sx);
-    const int64_t t = f / sx;
-    const int j = (int)(t % sy);
-    const int k = (int)(t / sy);
-    const int64_t dy = sx, dz = (int64_t)sx * sy;
-    float* A = a.a[c];
-    float sum = 0.0f;
-    unsigned count = 0;
-    if (i + 1 < sx && vin[f + 1]) { sum += A[f + 1]; ++count; }
-    if (i > 0 && vin[f - 1]) { sum += A[f - 1]; ++count; }
-    if (j + 1 < sy && vin[f + dy]) { sum += A[f + dy]; ++count; }
-    if (j > 0 && vin[f - dy]) { sum += A[f - dy]; ++count; }
-    if (k + 1 < sz && vin[f + dz]) { sum += A[f + dz]; ++count; }
-    if (k > 0 && vin[f - dz]) { sum += A[f - dz]; ++count; }
-    if (count > 0) {
-        A[f] = sum / (float)count;
-        vs.out[c][f] = 1;
-    } else {
-        vs.out[c][f] = 0;
+#pragma unroll
+    for (int q = 0; q < kExtraPer; ++q) {
+        const int64_t f = F[q];
+        if (f >= nf) continue;
+        if (self[q]) {
+            vout[f] = 1;
+            continue;
+        }
+        float sum = 0.0f;
+        unsigned count = 0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            if (!nb[q][d]) continue;
+            sum += A[(d & 1) ? f - st[d >> 1] : f + st[d >> 1]];
+            ++count;
+        }
+        if (count > 0) {
+            A[f] = sum / (float)count;
+            vout[f] = 1;
+        } else {
+            vout[f] = 0;
+        }
     }
 }
 
 hipError_t launch_extrapolate(fe_scene* s) {
     const Geom& g = s->g;
-    const int64_t tot = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
-    const int64_t padded = pad_blocks((tot + kBlock - 1) / kBlock);
+    const int64_t per = kBlock * kExtraPer;
+    const int64_t nb0 = (face_count(g, 0) + per - 1) / per, nb1 = (face_count(g, 1) + per - 1) / per,
+                  nb2 = (face_count(g, 2) + per - 1) / per;
     FaceSet a{{s->u[0], s->u[1], s->u[2]}};
     const int64_t off[3] = {0, face_count(g, 0), face_count(g, 0) + face_count(g, 1)};
     for (int it = 0; it < s->depth; ++it) {
@@ -230,7 +247,8 @@ hipError_t launch_extrapolate(fe_scene* s) {
             vs.in[c] = it == 0 ? s->mark[c] : s->valid[(it - 1) & 1] + off[c];
             vs.out[c] = s->valid[it & 1] + off[c];
         }
-        hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, a, vs, padded);
+        hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)(nb0 + nb1 + nb2)), dim3(kBlock), 0, s->stream, g, a, vs, nb0,
+                           nb0 + nb1);
     }
     return hipGetLastError();
 }
@@ -309,9 +327,8 @@ __global__ __launch_bounds__(kBlock) void k_gravity(Geom g, FaceSet a, float gx,
     float v = a.a[c][f] + add;
     if (walls) {
         const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
-        const int i = (int)(f % sx);
-        const int64_t t = f / sx;
-        const int j = (int)(t % sy), k = (int)(t / sy);
+        int i, j, k;
+        split_index(f, sx, sy, i, j, k);
         if (is_wall(g, c, i, j, k)) v = 0.0f;
     }
     a.a[c][f] = v;
@@ -352,9 +369,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_pressure(Geom g, FaceSet a, co
         }
     }
     const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
-    const int i = (int)(f % sx);
-    const int64_t t = f / sx;
-    const int j = (int)(t % sy), k = (int)(t / sy);
+    int i, j, k;
+    split_index(f, sx, sy, i, j, k);
     const int q = c == 0 ? i : (c == 1 ? j : k);
     float v = a.a[c][f];
     if (q >= 1 && q <= g.n[c] - 1) {
@@ -385,9 +401,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, un
     double m = 0.0;
     const int64_t nc = cell_count(g);
     for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        const int i = (int)(c % g.n[0]);
-        const int j = (int)((c / g.n[0]) % g.n[1]);
-        const int k = (int)(c / ((int64_t)g.n[0] * g.n[1]));
+        int i, j, k;
+        split_index(c, g.n[0], g.n[1], i, j, k);
         const int64_t iu = (int64_t)i + (int64_t)(g.n[0] + 1) * ((int64_t)j + (int64_t)g.n[1] * k);
         const int64_t iv = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)(g.n[1] + 1) * k);
         const int64_t iw = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);

@@ -236,10 +236,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
     const int64_t nf = (int64_t)sx * sy * sz;
     const int64_t f = blk * kBlock + threadIdx.x;
     if (f >= nf) return;
-    const int I = (int)(f % sx);
-    const int64_t t = f / sx;
-    const int J = (int)(t % sy);
-    const int K = (int)(t / sy);
+    int I, J, K;
+    split_index(f, sx, sy, I, J, K);
     const int F[3] = {I, J, K};
     int lo[3], hi[3];
 #pragma unroll

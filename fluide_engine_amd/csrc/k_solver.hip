@@ -485,12 +485,15 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     double* xa = s->p;
     double* xb = s->p2;
     int dcur = 0;
-    // iterations per host check: fewer when each carries a V-cycle
+    // iterations per host check: fewer when each carries a V-cycle; the first
+    // check comes where the previous solve of this kind stopped (warm-started
+    // solves of consecutive substeps take similar counts), so a typical solve
+    // waits for the device once
     const int batch = mg ? 4 : 16;
+    const int prev = s->last_cg_iters[init.precond];
+    int first = prev > 0 ? std::min(prev + 1, 64) : batch;
     const float* zr = init.precond ? s->z : s->r;
     const double* pz = mg ? PZ : nullptr;
-    if ((e = read_scalars(s, 0)) != hipSuccess) return e;
-    kt_flush(s);  // pairs of earlier kernels are complete after the sync
     // spmv k decides the stop of update k-1, so every batch ends with an spmv
     // whose state slot the host reads; launches after the stop exit at once
     // and are left out of the kernel timer (tags >= the stopping iteration).
@@ -505,7 +508,8 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk);
     };
     for (;;) {
-        const int nb = std::min(batch, d->max_iterations + 1 - k);
+        const int nb = std::min(first, d->max_iterations + 1 - k);
+        first = mg ? 2 : batch;
         for (int t = 0; t < nb; ++t) {
             launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                          (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
@@ -526,6 +530,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         s->p2 = s->p;
         s->p = xa;
     }
+    s->last_cg_iters[init.precond] = s->sc_host->done ? s->sc_host->it : d->max_iterations;
     return hipSuccess;
 }
 

@@ -178,6 +178,19 @@ int fe_substep(fe_scene* scene, double dt, const fe_solver_desc* solver, fe_solv
  * CFL-adaptive substeps; substeps_out may be NULL. */
 int fe_advance_frame(fe_scene* scene, double frame_dt, const fe_solver_desc* solver, int32_t* substeps_out);
 
+/* Batches of independent scenes (no reference counterpart; the reference runs
+ * one simulator per process).  A batch lists scenes of one context (not owned;
+ * they must outlive it); fe_batch_advance_frame advances every scene by one
+ * frame (fe_advance_frame) using `workers` host threads, largest scenes first,
+ * each scene on its own stream.  substeps and ms (may be NULL) receive per
+ * scene, in list order, the substep count and the wall time of its frame.
+ * Returns the first failing scene's code. */
+typedef struct fe_batch fe_batch;
+int fe_batch_create(fe_ctx* ctx, fe_scene* const* scenes, int32_t count, int32_t workers, fe_batch** out);
+int fe_batch_destroy(fe_batch* batch);
+int fe_batch_advance_frame(fe_batch* batch, double frame_dt, const fe_solver_desc* solver, int32_t* substeps,
+                           double* ms);
+
 /* PICSimulator::Cfl (src/pic_simulator.cpp:80-97). */
 int fe_cfl(fe_scene* scene, double dt, double* cfl_out);
 
