@@ -30,7 +30,7 @@ EXPORTED = [
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
     "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats", "fe_batch_create", "fe_batch_destroy",
-    "fe_batch_advance_frame",
+    "fe_batch_advance_frame", "fe_surface_build", "fe_surface_download",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -109,6 +109,8 @@ def load():
     L.fe_last_diffusion_stats.argtypes = [vp, C.POINTER(SolveStats)]
     L.fe_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)]
     L.fe_batch_destroy.argtypes = [vp]
+    L.fe_surface_build.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.fe_surface_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.fe_batch_advance_frame.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_double)]
     L.fe_version.restype = C.c_char_p
@@ -339,6 +341,18 @@ class Scene:
         st = SolveStats()
         self.ctx.check(self.L.fe_last_solve_stats(self._h, C.byref(st)), "fe_last_solve_stats")
         return st
+
+    def surface(self):
+        """Fluid surface of the current SDF: (vertices (V,3), normals (V,3), triangles (T,3) int32)."""
+        nv, nt = C.c_int64(), C.c_int64()
+        self.ctx.check(self.L.fe_surface_build(self._h, C.byref(nv), C.byref(nt)), "fe_surface_build")
+        v = np.empty((nv.value, 3), np.float64)
+        n = np.empty((nv.value, 3), np.float64)
+        t = np.empty((nt.value, 3), np.int32)
+        dp = C.POINTER(C.c_double)
+        self.ctx.check(self.L.fe_surface_download(self._h, v.ctypes.data_as(dp), n.ctypes.data_as(dp),
+                                                  t.ctypes.data_as(C.POINTER(C.c_int32))), "fe_surface_download")
+        return v, n, t
 
     def set_diffusion_solver(self, desc):
         self.ctx.check(self.L.fe_scene_set_diffusion_solver(self._h, C.byref(desc)), "fe_scene_set_diffusion_solver")

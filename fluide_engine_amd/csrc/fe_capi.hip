@@ -446,6 +446,7 @@ int fe_scene_destroy(fe_scene* s) {
     F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->sc); F(s->cflmax);
     mg_free(s);
     diffusion_free(s);
+    surface_free(s);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
@@ -734,6 +735,31 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
 int fe_last_solve_stats(const fe_scene* s, fe_solve_stats* st) {
     if (!s || !st) return FE_E_INVALID;
     *st = s->last_stats;
+    return FE_OK;
+}
+
+int fe_surface_build(fe_scene* s, int64_t* vertex_count, int64_t* triangle_count) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    const Geom& g = s->g;
+    const int64_t nedges = (int64_t)(g.n[0] - 1) * g.n[1] * g.n[2] + (int64_t)g.n[0] * (g.n[1] - 1) * g.n[2] +
+                           (int64_t)g.n[0] * g.n[1] * (g.n[2] - 1);
+    if (nedges >= ((int64_t)1 << 31) - 1)
+        return set_err(ctx, FE_E_UNSUPPORTED, "surface extraction needs fewer than 2^31 node edges");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, surface_build(s));
+    const McState& m = s->mc;
+    if (vertex_count) *vertex_count = m.nv3 + (int64_t)m.cap_pos.size() / 3;
+    if (triangle_count) *triangle_count = m.nt3 + (int64_t)m.cap_tri.size() / 3;
+    return FE_OK;
+}
+
+int fe_surface_download(fe_scene* s, double* vertices, double* normals, int32_t* triangles) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (!s->mc.built) return set_err(ctx, FE_E_INVALID, "no surface has been built on this scene");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, surface_download(s, vertices, normals, triangles));
     return FE_OK;
 }
 

@@ -284,6 +284,26 @@ struct MGLevel {
 
 }  // namespace fe
 
+namespace fe {
+// Surface extraction buffers (allocated on first use): crossed-edge flags and
+// vertex ids, per-cube triangle counts and offsets, scan tile sums, the case
+// tables, the six boundary planes, the cube part of the mesh on the device
+// and the caps on the host.
+struct McState {
+    int32_t *flag = nullptr, *vid = nullptr, *cnt = nullptr, *toff = nullptr, *sums = nullptr;
+    void* tables = nullptr;
+    double* planes = nullptr;
+    int64_t plane_stride = 0;
+    double *pos = nullptr, *nrm = nullptr;
+    int32_t* tri = nullptr;
+    int64_t vcap = 0, ncap = 0, tcap = 0;
+    int64_t nv3 = 0, nt3 = 0;  // cube-part vertices and triangles of the last build
+    std::vector<double> cap_pos, cap_nrm;
+    std::vector<int32_t> cap_tri;
+    bool built = false;
+};
+}  // namespace fe
+
 struct fe_ctx {
     int device = 0;
     std::string err;
@@ -351,6 +371,8 @@ struct fe_scene {
     uint16_t* dcode = nullptr;
     uint8_t* dmark[3] = {nullptr, nullptr, nullptr};
     fe_solve_stats diff_stats[3] = {};
+
+    fe::McState mc;  // surface extraction
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

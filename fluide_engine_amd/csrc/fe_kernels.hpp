@@ -13,6 +13,10 @@ struct FaceSet {
 hipError_t launch_fill_i32(hipStream_t st, int32_t* a, int32_t v, int64_t n);
 hipError_t launch_keys(fe_scene* s, bool histogram);
 hipError_t launch_scan(fe_scene* s, const int32_t* in, int32_t* out, int64_t n);
+// exclusive scan of n values into out[0..n] with a caller's tile-sum buffer
+// of scan_tiles(n) + 1 entries
+int64_t scan_tiles(int64_t n);
+hipError_t launch_scan_with(hipStream_t st, const int32_t* in, int32_t* out, int64_t n, int32_t* sums);
 hipError_t launch_sort(fe_scene* s, bool keys_ready);
 hipError_t launch_p2g(fe_scene* s);
 hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bool emit_keys);
@@ -42,6 +46,11 @@ hipError_t mg_setup(fe_scene* s);  // coarse markers and row codes from s->flags
 hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz, int tag);
 float* mg_first_sweep_target(fe_scene* s);  // where the CG kernels write level 0's first sweep
 void mg_free(fe_scene* s);
+
+// k_surface.hip: MarchingCubesSolver::BuildSurface on the fluid SDF
+hipError_t surface_build(fe_scene* s);  // counts in s->mc (nv3 + caps, nt3 + caps)
+hipError_t surface_download(fe_scene* s, double* pos, double* nrm, int32_t* tri);
+void surface_free(fe_scene* s);
 
 // k_diffusion.hip: BackwardEulerDiffusionSolver (without the trailing boundary condition)
 hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d);

@@ -516,12 +516,18 @@ hipError_t launch_keys(fe_scene* s, bool histogram) {
     return hipGetLastError();
 }
 
-hipError_t launch_scan(fe_scene* s, const int32_t* in, int32_t* out, int64_t n) {
-    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, s->stream, in, n, s->scan_sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s->stream, s->scan_sums, tiles);
-    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)tiles), dim3(kBlock), 0, s->stream, in, n, s->scan_sums, out);
+int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+hipError_t launch_scan_with(hipStream_t st, const int32_t* in, int32_t* out, int64_t n, int32_t* sums) {
+    const int64_t tiles = scan_tiles(n);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, st, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, st, sums, tiles);
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)tiles), dim3(kBlock), 0, st, in, n, sums, out);
     return hipGetLastError();
+}
+
+hipError_t launch_scan(fe_scene* s, const int32_t* in, int32_t* out, int64_t n) {
+    return launch_scan_with(s->stream, in, out, n, s->scan_sums);
 }
 
 // Bins the particles of P[cur] by cell into P[1-cur]; keys and counts must be

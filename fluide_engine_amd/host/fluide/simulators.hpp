@@ -11,8 +11,9 @@
 // particle/grid accessors.
 //
 // Not on this backend (each throws Unsupported): triangle-mesh scenes
-// (InitializeFromTriangleMesh, GetSurface), colliders, emitters and external
-// forces other than a uniform DirectionalField.
+// (InitializeFromTriangleMesh), colliders, emitters and external forces other
+// than a uniform DirectionalField.  GetSurface extracts the surface on the GPU;
+// OBJManager and SimulationRunner complete the reference's frame loop.
 #pragma once
 
 #include <cstddef>
@@ -103,6 +104,49 @@ struct ScalarGrid3D {
     Vector3<double> GetGridSpacing() const { return spacing; }
 };
 
+// src/3d/triangle_3d.hpp
+struct Triangle3D {
+    size_t point1Idx = 0, point2Idx = 1, point3Idx = 2, normalIdx = 0;
+    Triangle3D() = default;
+    Triangle3D(size_t p1, size_t p2, size_t p3, size_t n) : point1Idx(p1), point2Idx(p2), point3Idx(p3), normalIdx(n) {}
+};
+using Triangle3D_t = Triangle3D;
+
+// src/3d/triangle_mesh.hpp (the members the surface and OBJ paths use)
+class TriangleMesh {
+  public:
+    void Clear() {
+        _vertices.clear();
+        _normals.clear();
+        _triangles.clear();
+        _objectName.clear();
+    }
+    void AddVertex(const Vector3<double>& v) { _vertices.push_back(v); }
+    void AddNormal(const Vector3<double>& n) { _normals.push_back(n); }
+    void AddTriangle(const Triangle3D_t& t) { _triangles.push_back(t); }
+    const std::vector<Vector3<double>>& GetVerticies() const { return _vertices; }
+    const std::vector<Vector3<double>>& GetNormals() const { return _normals; }
+    const std::vector<Triangle3D_t>& GetTriangles() const { return _triangles; }
+    std::vector<Vector3<double>>& GetVerticies() { return _vertices; }
+    std::vector<Vector3<double>>& GetNormals() { return _normals; }
+    std::vector<Triangle3D_t>& GetTriangles() { return _triangles; }
+    std::string GetObjectName() const { return _objectName; }
+    void SetObjectName(const std::string& name) { _objectName = name; }
+
+  private:
+    std::string _objectName;
+    std::vector<Vector3<double>> _vertices, _normals;
+    std::vector<Triangle3D_t> _triangles;
+};
+
+// OBJ files in the reference's layout (src/3d/obj_manager.cpp): header,
+// object name, "v", "vn", "s off", "f a//n b//n c//n" (1-based).
+class OBJManager {
+  public:
+    void Save(const std::string& path, const TriangleMesh& mesh) const;
+    void Load(const std::string& path, TriangleMesh& mesh) const;
+};
+
 // Pressure-solver selection (SetPressureSolver / SetLinearSystemSolver,
 // src/fluid_solvers/single_phase_pressure_solver.hpp:21).  The default is the
 // reference's CudaJacobiIterationSolver(1000, 5, 1e-14)
@@ -186,7 +230,10 @@ class HipHybridSimulator {
     void InitializeFromTriangleMesh(const void*) {
         throw Unsupported("triangle-mesh scenes are not on the GPU backend; use InitializeParticles");
     }
-    void GetSurface(void*) { throw Unsupported("surface meshing is not on the GPU backend"); }
+    // HybridSimulator::GetSurface: MarchingCubesSolver::BuildSurface on the
+    // current fluid SDF (extracted on the GPU), appended to mesh; each
+    // triangle's normal index is its first vertex's (per-vertex normals).
+    void GetSurface(TriangleMesh& mesh) const;
 
     // Added for the GPU backend: particle and grid transfer, analytic scenes.
     void InitializeParticles(const std::vector<fe_seed_shape>& shapes, uint64_t seed);
@@ -251,6 +298,34 @@ class HipAPICSimulator : public HipHybridSimulator {
   public:
     HipAPICSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
         : HipHybridSimulator(gridSize, domain, FE_APIC) {}
+};
+
+// src/simulation_stats.hpp
+struct SimulationStats {
+    std::string simulatorType;
+    size_t numberOfIterations = 0;
+    Vector3<size_t> gridSize = 0;
+    size_t numberOfParticles = 0;
+    double timeIntervalInSeconds = 0;
+    double simulationTimeInSeconds = 0;
+    std::vector<double> iterationTimeInSeconds;
+    std::vector<double> cflPerIteration;
+    void PrintStats() const;
+    void Clear() { *this = SimulationStats(); }
+};
+
+// SimulationRunner::RunSimulation (src/simulation_runner.cpp:17-55): per frame
+// AdvanceSingleFrame, GetSurface and an OBJ file
+// <cache>/simulated_frames/<name>_<i>.obj; the log goes to <cache>/log.txt.
+class SimulationRunner {
+  public:
+    void RunSimulation(const std::shared_ptr<HipHybridSimulator>& simulator, double timeIntervalInSeconds,
+                       size_t numberOfIterations, const std::string& simulationOutputName,
+                       const std::string& cacheFolderPath);
+    SimulationStats GetStats() const { return _stats; }
+
+  private:
+    SimulationStats _stats;
 };
 
 // The drop-in names of the reference's CUDA simulators.

@@ -63,6 +63,17 @@ py::tuple get_particles(const HipHybridSimulator& sim) {
     return py::make_tuple(ax, av, ac);
 }
 
+py::array_t<double> vec3_array(const std::vector<Vector3<double>>& v) {
+    py::array_t<double> a({(py::ssize_t)v.size(), (py::ssize_t)3});
+    auto* d = a.mutable_data();
+    for (const auto& p : v) {
+        *d++ = p.x;
+        *d++ = p.y;
+        *d++ = p.z;
+    }
+    return a;
+}
+
 // Grid field as a numpy array shaped (nz, ny, nx) of the field's own dims.
 py::array get_grid(const HipHybridSimulator& sim, int field) {
     const Vector3<size_t> n = sim.GetResolution();
@@ -136,8 +147,7 @@ py::class_<Sim, std::shared_ptr<Sim>, Base> add_sim(py::module& m, const char* n
                  .def("AddEmitter", [](Sim& s, py::object) { s.AddEmitter(nullptr); })
                  .def("SetCurrentFrame", &Sim::SetCurrentFrame)
                  .def("SetViscosity", &Sim::SetViscosity)
-                 .def("SetMaxClf", &Sim::SetMaxClf)
-                 .def("GetSurface", [](Sim& s, py::object) { s.GetSurface(nullptr); });
+                 .def("SetMaxClf", &Sim::SetMaxClf);
     return c;
 }
 
@@ -176,7 +186,45 @@ PYBIND11_MODULE(PyFluidEngine, m) {
     py::class_<DirectionalField, std::shared_ptr<DirectionalField>, ExternalForce>(m, "DirectionalField")
         .def(py::init<Vector3<double>>());
 
+    // src/python_wrapper/triangle_mesh_wrapper.cpp, obj_manager_wrapper.cpp,
+    // simulator_runner_wrapper.cpp; the mesh also hands its arrays to numpy
+    py::class_<TriangleMesh>(m, "TriangleMesh")
+        .def(py::init<>())
+        .def("Clear", &TriangleMesh::Clear)
+        .def("GetObjectName", &TriangleMesh::GetObjectName)
+        .def("SetObjectName", &TriangleMesh::SetObjectName)
+        .def("GetVerticies", [](const TriangleMesh& t) { return vec3_array(t.GetVerticies()); })
+        .def("GetNormals", [](const TriangleMesh& t) { return vec3_array(t.GetNormals()); })
+        .def("GetTriangles", [](const TriangleMesh& t) {
+            py::array_t<int64_t> a({(py::ssize_t)t.GetTriangles().size(), (py::ssize_t)4});
+            auto* d = a.mutable_data();
+            for (const auto& tr : t.GetTriangles()) {
+                *d++ = (int64_t)tr.point1Idx;
+                *d++ = (int64_t)tr.point2Idx;
+                *d++ = (int64_t)tr.point3Idx;
+                *d++ = (int64_t)tr.normalIdx;
+            }
+            return a;
+        });
+    py::class_<OBJManager>(m, "OBJManager")
+        .def(py::init<>())
+        .def("Save", &OBJManager::Save)
+        .def("Load", &OBJManager::Load);
+    py::class_<SimulationStats>(m, "SimulationStats")
+        .def(py::init<>())
+        .def("PrintStats", &SimulationStats::PrintStats)
+        .def("Clear", &SimulationStats::Clear)
+        .def_readwrite("simulatorType", &SimulationStats::simulatorType)
+        .def_readwrite("numberOfIterations", &SimulationStats::numberOfIterations)
+        .def_readwrite("gridSize", &SimulationStats::gridSize)
+        .def_readwrite("numberOfParticles", &SimulationStats::numberOfParticles)
+        .def_readwrite("timeIntervalInSeconds", &SimulationStats::timeIntervalInSeconds)
+        .def_readwrite("simulationTimeInSeconds", &SimulationStats::simulationTimeInSeconds)
+        .def_readwrite("iterationTimeInSeconds", &SimulationStats::iterationTimeInSeconds)
+        .def_readwrite("cflPerIteration", &SimulationStats::cflPerIteration);
+
     py::class_<HipHybridSimulator, std::shared_ptr<HipHybridSimulator>>(m, "HybridSimulator")
+        .def("GetSurface", &HipHybridSimulator::GetSurface)
         .def("MaxCfl", &HipHybridSimulator::MaxCfl)
         .def("GetViscosity", &HipHybridSimulator::GetViscosity)
         .def("GetDensity", &HipHybridSimulator::GetDensity)
@@ -221,6 +269,11 @@ PYBIND11_MODULE(PyFluidEngine, m) {
         .def("SetBlendingFactor", &HipFLIPSimulator::SetBlendingFactor)
         .def("GetBlendingFactor", &HipFLIPSimulator::GetBlendingFactor);
     (void)pic;
+    py::class_<SimulationRunner>(m, "SimulationRunner")
+        .def(py::init<>())
+        .def("RunSimulation", &SimulationRunner::RunSimulation)
+        .def("GetStats", &SimulationRunner::GetStats);
+
     m.attr("CudaPICSimulator") = m.attr("PICSimulator");
     m.attr("CudaFLIPSimulator") = m.attr("FLIPSimulator");
     m.attr("CudaAPICSimulator") = m.attr("APICSimulator");

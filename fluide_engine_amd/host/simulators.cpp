@@ -1,9 +1,18 @@
 // simulators.cpp -- host classes of fluide/simulators.hpp over the C ABI.
 #include "fluide/simulators.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <limits>
 #include <map>
 #include <mutex>
+#include <numeric>
+#include <sstream>
+#include <typeinfo>
 
 namespace fluide {
 
@@ -259,6 +268,147 @@ fe_solve_stats HipHybridSimulator::LastSolveStats() const {
     fe_solve_stats st{};
     if (_scene) Check(fe_last_solve_stats(_scene, &st), "fe_last_solve_stats");
     return st;
+}
+
+}  // namespace fluide
+
+// ---------------------------------------------------------------------------
+// Surface, OBJ files and the frame loop
+// ---------------------------------------------------------------------------
+namespace fluide {
+
+void HipHybridSimulator::GetSurface(TriangleMesh& mesh) const {
+    EnsureScene(0);
+    int64_t nv = 0, nt = 0;
+    Check(fe_surface_build(_scene, &nv, &nt), "fe_surface_build");
+    std::vector<double> v((size_t)nv * 3), n((size_t)nv * 3);
+    std::vector<int32_t> t((size_t)nt * 3);
+    Check(fe_surface_download(_scene, v.data(), n.data(), t.data()), "fe_surface_download");
+    const size_t base = mesh.GetVerticies().size();
+    for (int64_t i = 0; i < nv; ++i) {
+        mesh.AddVertex(Vector3<double>(v[3 * i], v[3 * i + 1], v[3 * i + 2]));
+        mesh.AddNormal(Vector3<double>(n[3 * i], n[3 * i + 1], n[3 * i + 2]));
+    }
+    for (int64_t i = 0; i < nt; ++i)
+        mesh.AddTriangle(Triangle3D_t(base + t[3 * i], base + t[3 * i + 1], base + t[3 * i + 2], base + t[3 * i]));
+}
+
+void OBJManager::Save(const std::string& path, const TriangleMesh& mesh) const {
+    std::ofstream f(path.c_str());
+    if (!f) throw std::runtime_error("cannot open " + path);
+    f << "mtllib test.mtl\n";
+    f << "o " << mesh.GetObjectName() << "\n" << std::endl;
+    for (const auto& v : mesh.GetVerticies())
+        f << "v " << std::to_string(v.x) << " " << std::to_string(v.y) << " " << std::to_string(v.z) << "\n";
+    f << std::endl;
+    for (const auto& n : mesh.GetNormals())
+        f << "vn " << std::to_string(n.x) << " " << std::to_string(n.y) << " " << std::to_string(n.z) << "\n";
+    f << std::endl;
+    f << "s off\n" << std::endl;
+    for (const auto& t : mesh.GetTriangles()) {
+        f << "f " << t.point1Idx + 1 << "//" << t.normalIdx + 1 << " " << t.point2Idx + 1 << "//" << t.normalIdx + 1
+          << " " << t.point3Idx + 1 << "//" << t.normalIdx + 1 << "\n";
+    }
+    f << std::endl;
+    if (!f) throw std::runtime_error("write error on " + path);
+}
+
+void OBJManager::Load(const std::string& path, TriangleMesh& mesh) const {
+    std::ifstream f(path.c_str());
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream in(line);
+        std::string tag;
+        if (!(in >> tag)) continue;
+        if (tag == "o") {
+            std::string name;
+            if (in >> name) mesh.SetObjectName(name);
+        } else if (tag == "v" || tag == "vn") {
+            double x, y, z;
+            if (in >> x >> y >> z) (tag == "v" ? mesh.GetVerticies() : mesh.GetNormals()).emplace_back(x, y, z);
+        } else if (tag == "f") {
+            size_t p[3], nidx = 0;
+            std::string tok;
+            int k = 0;
+            while (k < 3 && in >> tok) {
+                const size_t slash = tok.find('/');
+                p[k++] = std::stoul(tok.substr(0, slash)) - 1;
+                const size_t last = tok.rfind('/');
+                if (last != std::string::npos && last + 1 < tok.size()) nidx = std::stoul(tok.substr(last + 1)) - 1;
+            }
+            if (k == 3) mesh.AddTriangle(Triangle3D_t(p[0], p[1], p[2], nidx));
+        }
+    }
+}
+
+static double median_of(std::vector<double> v) {
+    if (v.empty()) return std::numeric_limits<double>::quiet_NaN();
+    std::sort(v.begin(), v.end());
+    const size_t n = v.size();
+    return n % 2 ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+
+static double mean_of(const std::vector<double>& v) {
+    if (v.empty()) return std::numeric_limits<double>::quiet_NaN();
+    return std::accumulate(v.begin(), v.end(), 0.0) / (double)v.size();
+}
+
+// SimulationStats::PrintStats (src/simulation_stats.cpp:19-33), with its
+// labels (the reference prints the mean under "Median CFL" and the median
+// under "Mean CFL"; kept).
+void SimulationStats::PrintStats() const {
+    std::cout << "================ SIMULATION STATS ================\n";
+    std::cout << "Simulator type: " << simulatorType << "\n";
+    std::cout << "Grid resolution: " << gridSize.x << " x " << gridSize.y << " x " << gridSize.z << "\n";
+    std::cout << "Number of particles: " << numberOfParticles << "\n";
+    std::cout << "Number of Iterations: " << numberOfIterations << "\n";
+    std::cout << "Time interval: " << timeIntervalInSeconds << " [s]\n";
+    std::cout << "Median CFL: " << mean_of(cflPerIteration) << "\n";
+    std::cout << "Mean CFL: " << median_of(cflPerIteration) << "\n";
+    std::cout << "Median frame: " << median_of(iterationTimeInSeconds) << " [s]\n";
+    std::cout << "Mean frame: " << mean_of(iterationTimeInSeconds) << " [s]\n";
+    std::cout << "Simulation time: " << simulationTimeInSeconds << " [s]\n";
+    std::cout << "==================================================\n";
+}
+
+void SimulationRunner::RunSimulation(const std::shared_ptr<HipHybridSimulator>& simulator,
+                                     double timeIntervalInSeconds, size_t numberOfIterations,
+                                     const std::string& simulationOutputName, const std::string& cacheFolderPath) {
+    if (!simulator) throw std::invalid_argument("simulator is null");
+    OBJManager obj;
+    TriangleMesh surface;
+    simulator->SetCurrentFrame(Frame(timeIntervalInSeconds));
+    std::filesystem::create_directories(cacheFolderPath + "/simulated_frames");
+    std::ofstream log(cacheFolderPath + "/log.txt");
+    std::streambuf* saved = std::cout.rdbuf(log.rdbuf());
+    try {
+        for (size_t i = 0; i < numberOfIterations; ++i) {
+            std::cout << "Iteration: " << i << "\n";
+            const auto t0 = std::chrono::steady_clock::now();
+            simulator->AdvanceSingleFrame();
+            simulator->GetSurface(surface);
+            obj.Save(cacheFolderPath + "/simulated_frames/" + simulationOutputName + "_" + std::to_string(i) + ".obj",
+                     surface);
+            const auto t1 = std::chrono::steady_clock::now();
+            const double sec = std::chrono::duration<double>(t1 - t0).count();
+            _stats.simulationTimeInSeconds += sec;
+            _stats.iterationTimeInSeconds.push_back(sec);
+            _stats.cflPerIteration.push_back(simulator->Cfl(timeIntervalInSeconds));
+            surface.Clear();
+            std::cout << std::endl;
+        }
+        _stats.numberOfIterations = numberOfIterations;
+        _stats.timeIntervalInSeconds = timeIntervalInSeconds;
+        _stats.simulatorType = typeid(*simulator).name();
+        _stats.numberOfParticles = simulator->GetNumberOfParticles();
+        _stats.gridSize = simulator->GetResolution();
+        _stats.PrintStats();
+    } catch (...) {
+        std::cout.rdbuf(saved);
+        throw;
+    }
+    std::cout.rdbuf(saved);
 }
 
 }  // namespace fluide

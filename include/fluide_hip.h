@@ -191,6 +191,16 @@ int fe_batch_destroy(fe_batch* batch);
 int fe_batch_advance_frame(fe_batch* batch, double frame_dt, const fe_solver_desc* solver, int32_t* substeps,
                            double* ms);
 
+/* Fluid surface: HybridSimulator::GetSurface -> MarchingCubesSolver::BuildSurface
+ * (src/3d/marching_cubes_solver.cpp:25-51) on the current fluid SDF, iso value 0.
+ * fe_surface_build extracts the mesh on the device and returns its counts;
+ * fe_surface_download copies it out: vertices and per-vertex normals as
+ * 3 doubles each, triangles as 3 vertex indices (counter-clockwise seen from
+ * outside the fluid).  Cube vertices come first, then the caps on the six
+ * boundary node planes.  Grids need fewer than 2^31 node edges. */
+int fe_surface_build(fe_scene* scene, int64_t* vertex_count, int64_t* triangle_count);
+int fe_surface_download(fe_scene* scene, double* vertices, double* normals, int32_t* triangles);
+
 /* PICSimulator::Cfl (src/pic_simulator.cpp:80-97). */
 int fe_cfl(fe_scene* scene, double dt, double* cfl_out);
 

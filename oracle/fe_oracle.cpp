@@ -33,11 +33,14 @@
 // Build: oracle/Makefile -> oracle/_build/libfe_oracle.so (C ABI, loaded by ctypes).
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -1021,6 +1024,318 @@ void diffusion(orc_scene* s, double dt) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// MarchingCubesSolver::BuildSurface (src/3d/marching_cubes_solver.cpp:25-51):
+// cubes in the reference's i, j, k loop order with an edge-keyed vertex map
+// (SolveSingleCube, :172-270), then caps on the boundary node planes, one
+// vertex map per plane (SolveXY / SolveYZ / SolveXZ / SolveSingleSquare,
+// :272-589).  The triangulation of a cube follows one face rule (restated
+// here by tracing each cube's loops; the reference's 256-entry table is not
+// used): on every cube face the crossed edges are joined in pairs, cutting
+// diagonally opposite inside corners off on their own; the loops are wound
+// counter-clockwise seen from outside and fanned from the first position none
+// of whose triangles lies in one cube face.  The same rule caps the planes.
+// ---------------------------------------------------------------------------
+struct McMesh {
+    std::vector<double> v, n;
+    std::vector<int32_t> t;
+};
+
+// Cube corner c has offsets (c & 1, c >> 1 & 1, c >> 2 & 1); the corner
+// pairs of the 12 edges, lower corner first, x edges then y then z.
+void mc_edge_pair(int e, int& a, int& b) {
+    static const int pairs[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                                     {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    a = pairs[e][0];
+    b = pairs[e][1];
+}
+
+int mc_edge_between(int a, int b) {
+    for (int e = 0; e < 12; ++e) {
+        int p, q;
+        mc_edge_pair(e, p, q);
+        if ((p == a && q == b) || (p == b && q == a)) return e;
+    }
+    return -1;
+}
+
+// The four corners of cube face f (axis f/2, side f%2) in cyclic order in the
+// plane of the two other axes, lower axis first.
+void mc_face(int f, int q[4]) {
+    const int axis = f / 2, side = f % 2;
+    const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
+    const int cyc[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    for (int t = 0; t < 4; ++t) q[t] = (side << axis) | (cyc[t][0] << u) | (cyc[t][1] << w);
+}
+
+bool mc_in_face(int f, int e) {
+    int q[4];
+    mc_face(f, q);
+    for (int t = 0; t < 4; ++t)
+        if (mc_edge_between(q[t], q[(t + 1) % 4]) == e) return true;
+    return false;
+}
+
+// Loops of the crossed edges of a cube with inside flags in[8], each as a
+// triangle fan (edge indices, counter-clockwise from outside).
+std::vector<int> mc_cube_triangles(const bool in[8]) {
+    std::vector<int> link[12];
+    for (int f = 0; f < 6; ++f) {
+        int q[4];
+        mc_face(f, q);
+        int crossed[4], nc = 0;
+        for (int t = 0; t < 4; ++t)
+            if (in[q[t]] != in[q[(t + 1) % 4]]) crossed[nc++] = t;
+        auto join = [&](int ta, int tb) {
+            const int ea = mc_edge_between(q[ta], q[(ta + 1) % 4]), eb = mc_edge_between(q[tb], q[(tb + 1) % 4]);
+            link[ea].push_back(eb);
+            link[eb].push_back(ea);
+        };
+        if (nc == 2) join(crossed[0], crossed[1]);
+        if (nc == 4)
+            for (int m = 0; m < 4; ++m)
+                if (in[q[m]]) join((m + 3) % 4, m);  // the inside corner's two edges
+    }
+    std::vector<int> tris;
+    bool seen[12] = {false};
+    for (int e0 = 0; e0 < 12; ++e0) {
+        if (link[e0].size() != 2 || seen[e0]) continue;
+        std::vector<int> loop;
+        int prev = -1, cur = e0;
+        do {
+            loop.push_back(cur);
+            seen[cur] = true;
+            const int nx = link[cur][0] != prev ? link[cur][0] : link[cur][1];
+            prev = cur;
+            cur = nx;
+        } while (cur != e0);
+        const int L = (int)loop.size();
+        double N[3] = {0, 0, 0}, D[3] = {0, 0, 0};
+        std::vector<std::array<double, 3>> mid(L);
+        for (int t = 0; t < L; ++t) {
+            int a, b;
+            mc_edge_pair(loop[t], a, b);
+            const int ci = in[a] ? a : b, co = in[a] ? b : a;
+            for (int x = 0; x < 3; ++x) {
+                mid[t][x] = 0.5 * (((a >> x) & 1) + ((b >> x) & 1));
+                D[x] += ((co >> x) & 1) - ((ci >> x) & 1);
+            }
+        }
+        for (int t = 0; t < L; ++t) {  // Newell's normal
+            const auto& p = mid[t];
+            const auto& r = mid[(t + 1) % L];
+            N[0] += (p[1] - r[1]) * (p[2] + r[2]);
+            N[1] += (p[2] - r[2]) * (p[0] + r[0]);
+            N[2] += (p[0] - r[0]) * (p[1] + r[1]);
+        }
+        if (N[0] * D[0] + N[1] * D[1] + N[2] * D[2] < 0) std::reverse(loop.begin() + 1, loop.end());
+        int start = 0;
+        for (int s0 = 0; s0 < L; ++s0) {
+            bool ok = true;
+            for (int t = 1; t + 1 < L && ok; ++t)
+                for (int f = 0; f < 6 && ok; ++f)
+                    ok = !(mc_in_face(f, loop[s0]) && mc_in_face(f, loop[(s0 + t) % L]) &&
+                           mc_in_face(f, loop[(s0 + t + 1) % L]));
+            if (ok) {
+                start = s0;
+                break;
+            }
+        }
+        for (int t = 1; t + 1 < L; ++t) {
+            tris.push_back(loop[start]);
+            tris.push_back(loop[(start + t) % L]);
+            tris.push_back(loop[(start + t + 1) % L]);
+        }
+    }
+    return tris;
+}
+
+// MarchingCubesSolver::Gradient (:116-160)
+void mc_gradient(const Arr3& sdf, const double ih[3], int i, int j, int k, double g[3]) {
+    const int idx[3] = {i, j, k}, dim[3] = {sdf.sx, sdf.sy, sdf.sz};
+    for (int a = 0; a < 3; ++a) {
+        int p[3] = {i, j, k}, m[3] = {i, j, k};
+        p[a] = idx[a] + 1;
+        m[a] = idx[a] - 1;
+        if (idx[a] > dim[a] - 2)
+            p[a] = idx[a];
+        else if (idx[a] == 0)
+            m[a] = 0;
+        g[a] = 0.5 * ih[a] * (sdf(p[0], p[1], p[2]) - sdf(m[0], m[1], m[2]));
+    }
+}
+
+int surface(orc_scene* s, double** vout, double** nout, int32_t** tout, int64_t* nv, int64_t* nt) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
+    const double ih[3] = {1.0 / s->h[0], 1.0 / s->h[1], 1.0 / s->h[2]};
+    // sdf = fluid SDF, replaced by -collider where both are negative (:29-37)
+    Arr3 sdf = s->sdf;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                if (s->sdf(i, j, k) < 0 && s->csdf(i, j, k) < 0) sdf(i, j, k) = s->csdf(i, j, k) * (-1);
+    auto node = [&](int i, int j, int k, int a) {  // GridIndexToPosition
+        const int idx[3] = {i, j, k};
+        return so[a] + idx[a] * s->h[a];
+    };
+    McMesh M;
+    std::map<int64_t, int32_t> vmap;
+    auto key3 = [&](int axis, int i, int j, int k) {
+        return (int64_t)axis + 3 * ((int64_t)i + (int64_t)(nx + 1) * ((int64_t)j + (int64_t)(ny + 1) * k));
+    };
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 0; j < ny - 1; ++j)
+            for (int k = 0; k < nz - 1; ++k) {
+                bool in[8];
+                double val[8];
+                for (int c = 0; c < 8; ++c) {
+                    val[c] = sdf(i + (c & 1), j + ((c >> 1) & 1), k + ((c >> 2) & 1));
+                    in[c] = val[c] <= 0.0;
+                }
+                const std::vector<int> tris = mc_cube_triangles(in);
+                if (tris.empty()) continue;
+                const double oc[3] = {node(i, j, k, 0), node(i, j, k, 1), node(i, j, k, 2)};  // bbox origin
+                for (size_t q = 0; q < tris.size(); q += 3) {
+                    int32_t face[3];
+                    for (int r = 0; r < 3; ++r) {
+                        const int e = tris[q + r];
+                        int a, b;
+                        mc_edge_pair(e, a, b);
+                        const int axis = e / 4;
+                        const int64_t key = key3(axis, i + (a & 1), j + ((a >> 1) & 1), k + ((a >> 2) & 1));
+                        auto it = vmap.find(key);
+                        if (it != vmap.end()) {
+                            face[r] = it->second;
+                            continue;
+                        }
+                        // DistanceToZeroLevelSet (:591-601), clamps (:263-268)
+                        const double phi0 = val[a], phi1 = val[b];
+                        double alpha = std::fabs(phi0) + std::fabs(phi1) > 0.000000001
+                                           ? std::fabs(phi0) / (std::fabs(phi0) + std::fabs(phi1))
+                                           : 0.5;
+                        if (alpha < 0.000001) alpha = 0.000001;
+                        if (alpha > 0.999999) alpha = 0.999999;
+                        double g0[3], g1[3], nrm[3];
+                        mc_gradient(sdf, ih, i + (a & 1), j + ((a >> 1) & 1), k + ((a >> 2) & 1), g0);
+                        mc_gradient(sdf, ih, i + (b & 1), j + ((b >> 1) & 1), k + ((b >> 2) & 1), g1);
+                        for (int x = 0; x < 3; ++x) {
+                            // BoundingBox3D::Corner (src/3d/bounding_box_3d.cpp:23-31)
+                            const double p0 = (s->h[x] * (((a >> x) & 1) ? 0.5 : -0.5) + oc[x]) + 0.5 * s->h[x];
+                            const double p1 = (s->h[x] * (((b >> x) & 1) ? 0.5 : -0.5) + oc[x]) + 0.5 * s->h[x];
+                            M.v.push_back((1.0 - alpha) * p0 + alpha * p1);
+                            nrm[x] = (1.0 - alpha) * g0[x] + alpha * g1[x];
+                        }
+                        const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+                        for (int x = 0; x < 3; ++x) M.n.push_back(len * len > 0.0 ? nrm[x] / len : nrm[x]);
+                        face[r] = (int32_t)(M.v.size() / 3 - 1);
+                        vmap[key] = face[r];
+                    }
+                    M.t.insert(M.t.end(), face, face + 3);
+                }
+            }
+    // caps: planes z (back, front), x (left, right), y (down, up)
+    const int planes[6][2] = {{2, 0}, {2, 1}, {0, 0}, {0, 1}, {1, 0}, {1, 1}};
+    for (const auto& pl : planes) {
+        const int axis = pl[0], side = pl[1];
+        const int u = axis == 0 ? 1 : 0, w = axis == 2 ? 1 : 2;
+        const int dim[3] = {nx, ny, nz};
+        const int fixed = side ? dim[axis] - 1 : 0;
+        const double outward = side ? 1.0 : -1.0;
+        std::map<int64_t, int32_t> pmap;
+        auto nodeval = [&](int iu, int iw) {
+            int idx[3];
+            idx[axis] = fixed;
+            idx[u] = iu;
+            idx[w] = iw;
+            return sdf(idx[0], idx[1], idx[2]);
+        };
+        auto nodepos = [&](int iu, int iw, double p[3]) {
+            int idx[3];
+            idx[axis] = fixed;
+            idx[u] = iu;
+            idx[w] = iw;
+            for (int x = 0; x < 3; ++x) p[x] = node(idx[0], idx[1], idx[2], x);
+        };
+        auto add = [&](const double p[3]) {
+            for (int x = 0; x < 3; ++x) {
+                M.v.push_back(p[x]);
+                M.n.push_back(x == axis ? outward : 0.0);
+            }
+            return (int32_t)(M.v.size() / 3 - 1);
+        };
+        for (int iw = 0; iw + 1 < dim[w]; ++iw)
+            for (int iu = 0; iu + 1 < dim[u]; ++iu) {
+                const int cu[4] = {iu, iu + 1, iu + 1, iu}, cw[4] = {iw, iw, iw + 1, iw + 1};
+                double phi[4];
+                bool in[4];
+                for (int t = 0; t < 4; ++t) {
+                    phi[t] = nodeval(cu[t], cw[t]);
+                    in[t] = phi[t] <= 0.0;
+                }
+                // cap polygons: each inside corner alone when the four square
+                // edges are crossed, else one run of inside corners and crossings
+                std::vector<std::vector<int>> polys;  // slots: 0-3 corners, 4-7 crossings
+                int ncross = 0;
+                for (int t = 0; t < 4; ++t) ncross += in[t] != in[(t + 1) % 4];
+                if (ncross == 4) {
+                    for (int m = 0; m < 4; ++m)
+                        if (in[m]) polys.push_back({4 + (m + 3) % 4, m, 4 + m});
+                } else if (in[0] || in[1] || in[2] || in[3]) {
+                    std::vector<int> p;
+                    for (int t = 0; t < 4; ++t) {
+                        if (in[t]) p.push_back(t);
+                        if (in[t] != in[(t + 1) % 4]) p.push_back(4 + t);
+                    }
+                    polys.push_back(p);
+                }
+                for (const auto& poly : polys) {
+                    std::vector<int32_t> ids;
+                    for (int slot : poly) {
+                        int64_t key;
+                        double p[3];
+                        if (slot < 4) {
+                            key = 3 * ((int64_t)cu[slot] + (int64_t)(dim[u] + 1) * cw[slot]);
+                            nodepos(cu[slot], cw[slot], p);
+                        } else {
+                            const int t = slot - 4, t1 = (t + 1) % 4;
+                            const int lo = (cu[t] + cw[t] <= cu[t1] + cw[t1]) ? t : t1, hi = lo == t ? t1 : t;
+                            key = 3 * ((int64_t)cu[lo] + (int64_t)(dim[u] + 1) * cw[lo]) + (cw[lo] == cw[hi] ? 1 : 2);
+                            // SolveSingleSquare alpha (:536-549)
+                            const double f0 = std::fabs(phi[lo]), f1 = std::fabs(phi[hi]);
+                            double alpha = f0 + f1 > 1e-12 ? f0 / (f0 + f1) : 0.5f;
+                            if (alpha < 0.000001f) alpha = 0.000001f;
+                            if (alpha > 0.999999f) alpha = 0.999999f;
+                            double p0[3], p1[3];
+                            nodepos(cu[lo], cw[lo], p0);
+                            nodepos(cu[hi], cw[hi], p1);
+                            for (int x = 0; x < 3; ++x) p[x] = (1.0 - alpha) * p0[x] + alpha * p1[x];
+                        }
+                        auto it = pmap.find(key);
+                        if (it == pmap.end()) it = pmap.emplace(key, add(p)).first;
+                        ids.push_back(it->second);
+                    }
+                    // the cyclic (u, w) order turns about +x, -y, +z for x, y, z planes
+                    const bool flip = (axis == 1 ? -1.0 : 1.0) != outward;
+                    for (size_t t = 1; t + 1 < ids.size(); ++t) {
+                        M.t.push_back(ids[0]);
+                        M.t.push_back(flip ? ids[t + 1] : ids[t]);
+                        M.t.push_back(flip ? ids[t] : ids[t + 1]);
+                    }
+                }
+            }
+    }
+    *nv = (int64_t)M.v.size() / 3;
+    *nt = (int64_t)M.t.size() / 3;
+    *vout = (double*)std::malloc(sizeof(double) * std::max<size_t>(M.v.size(), 1));
+    *nout = (double*)std::malloc(sizeof(double) * std::max<size_t>(M.n.size(), 1));
+    *tout = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(M.t.size(), 1));
+    std::copy(M.v.begin(), M.v.end(), *vout);
+    std::copy(M.n.begin(), M.n.end(), *nout);
+    std::copy(M.t.begin(), M.t.end(), *tout);
+    return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1156,6 +1471,11 @@ void orc_g2p(orc_scene* s) { g2p(s); }
 void orc_move_particles(orc_scene* s, double dt) { move_particles(s, dt); }
 
 void orc_pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) { pressure(s, d, st); }
+
+int orc_surface(orc_scene* s, double** vertices, double** normals, int32_t** triangles, int64_t* nv, int64_t* nt) {
+    return surface(s, vertices, normals, triangles, nv, nt);
+}
+void orc_free(void* p) { std::free(p); }
 
 void orc_set_viscosity(orc_scene* s, double viscosity) { s->viscosity = viscosity; }
 void orc_set_diffusion_solver(orc_scene* s, const orc_solver_desc* d) { s->diff = *d; }

@@ -70,6 +70,10 @@ void orc_set_viscosity(orc_scene* s, double viscosity);
 void orc_set_diffusion_solver(orc_scene* s, const orc_solver_desc* d);
 void orc_diffusion(orc_scene* s, double dt, orc_solve_stats st[3]);
 int orc_diffusion_markers(const orc_scene* s, int c, uint8_t* out);
+// MarchingCubesSolver::BuildSurface on the fluid SDF: malloc'ed vertices,
+// normals (3 doubles each) and triangles (3 indices); free with orc_free.
+int orc_surface(orc_scene* s, double** vertices, double** normals, int32_t** triangles, int64_t* nv, int64_t* nt);
+void orc_free(void* p);
 double orc_cfl(const orc_scene* s, double dt);
 unsigned orc_num_substeps(const orc_scene* s, double dt);
 int orc_advance_frame(orc_scene* s, double frame_dt, const orc_solver_desc* d);

@@ -105,6 +105,9 @@ def lib():
         L.orc_set_diffusion_solver.argtypes = [C.c_void_p, C.POINTER(SolverDesc)]
         L.orc_diffusion.argtypes = [C.c_void_p, C.c_double, C.POINTER(SolveStats)]
         L.orc_diffusion_markers.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_surface.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),
+                                  C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -254,6 +257,21 @@ class Scene:
         out = np.empty(self.shape((U, V, W)[c]), dtype=np.uint8)
         lib().orc_diffusion_markers(self._s, c, out.ctypes.data_as(C.c_void_p))
         return out
+
+    def surface(self):
+        """MarchingCubesSolver::BuildSurface: (vertices (V,3), normals (V,3), triangles (T,3))."""
+        L = lib()
+        vp, np_, tp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)(), C.POINTER(C.c_int32)()
+        nv, nt = C.c_int64(), C.c_int64()
+        L.orc_surface(self._s, C.byref(vp), C.byref(np_), C.byref(tp), C.byref(nv), C.byref(nt))
+        try:
+            v = np.ctypeslib.as_array(vp, shape=(max(nv.value, 1) * 3,))[: nv.value * 3].reshape(-1, 3).copy()
+            n = np.ctypeslib.as_array(np_, shape=(max(nv.value, 1) * 3,))[: nv.value * 3].reshape(-1, 3).copy()
+            t = np.ctypeslib.as_array(tp, shape=(max(nt.value, 1) * 3,))[: nt.value * 3].reshape(-1, 3).copy()
+        finally:
+            for ptr in (vp, np_, tp):
+                L.orc_free(C.cast(ptr, C.c_void_p))
+        return v, n, t
 
     def cfl(self, dt):
         return lib().orc_cfl(self._s, dt)

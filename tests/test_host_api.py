@@ -27,8 +27,15 @@ def box(P):
 
 def test_module_surface(P):
     for name in ("Frame", "Vector3D", "Vector3S", "BoundingBox3D", "ExternalForce", "DirectionalField",
-                 "HybridSimulator", "PICSimulator", "FLIPSimulator", "APICSimulator"):
+                 "HybridSimulator", "PICSimulator", "FLIPSimulator", "APICSimulator", "TriangleMesh", "OBJManager",
+                 "SimulationRunner", "SimulationStats"):
         assert hasattr(P, name), name
+    m = P.TriangleMesh()
+    m.Clear()
+    assert m.GetVerticies().shape == (0, 3) and m.GetTriangles().shape == (0, 4)
+    st = P.SimulationStats()
+    st.numberOfIterations = 3
+    assert st.numberOfIterations == 3
     assert P.CudaFLIPSimulator is P.FLIPSimulator
     assert P.CudaPICSimulator is P.PICSimulator and P.CudaAPICSimulator is P.APICSimulator
     for cls in (P.PICSimulator, P.FLIPSimulator, P.APICSimulator):
@@ -67,7 +74,7 @@ def test_frame(P):
 
 def test_unsupported_paths(P):
     s = P.APICSimulator(P.Vector3S(8, 8, 8), box(P))
-    for call in (lambda: s.AddCollider(None), lambda: s.AddEmitter(None), lambda: s.GetSurface(None),
+    for call in (lambda: s.AddCollider(None), lambda: s.AddEmitter(None),
                  lambda: s.InitializeFromTriangleMesh(None)):
         with pytest.raises(NotImplementedError):
             call()
