@@ -92,6 +92,8 @@ def parse():
     ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solver-comparison", action="store_true",
+                    help="skip the plain-CG / multigrid-PCG comparison after the timed run")
     ap.add_argument("--batch-frames", type=int, default=10,
                     help="frames of the 96-scene batch workload after the headline run (0 skips it)")
     ap.add_argument("--batch-workers", type=int, nargs="+", default=[4],
@@ -111,9 +113,11 @@ KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": 
                   "g2p_move": "k_g2p_move", "sdf": "k_sdf"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_kt -o kt -- "
-             "python3 bench.py --steps 20 --warmup 5 --no-cpu-baseline")
+             "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
+             "--batch-frames 0")
 PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{counter} -o pmc -- "
-           "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --copy-calibration 256")
+           "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --no-solver-comparison "
+           "--batch-frames 0 --copy-calibration 256")
 
 
 def short_name(kernel):
@@ -275,7 +279,7 @@ def main():
                            traffic_for("cg_update") if on_profile else None)
     scene.kernel_timer(KERNEL_IDS["none"])
 
-    solve_cmp = None if args.solver == "jacobi" else solver_comparison(scene, capi, dt)
+    solve_cmp = None if (args.solver == "jacobi" or args.no_solver_comparison) else solver_comparison(scene, capi, dt)
     batch = None
     if args.batch_frames > 0:
         batch = [batch_workload(capi, ctx, args.batch_frames, w) for w in args.batch_workers]
