@@ -16,8 +16,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libfluide_hip.so")
 PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
 SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI, SOLVER_MGPCG = 0, 1, 2, 3
+SDF_SEARCH_CELLS, SDF_SEARCH_BUCKETS = 0, 1
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
- GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW) = range(16)
+ GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW, GRID_COLLIDER_SDF) = range(17)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
  STAGE_MOVE, STAGE_MARKERS, STAGE_DIFFUSION) = range(11)
 STAGE_NAMES = ["sort", "p2g", "sdf", "extrapolate", "bc", "gravity", "pressure", "g2p_move", "move", "markers",
@@ -30,7 +31,8 @@ EXPORTED = [
     "fe_particle_cells", "fe_seed_particles", "fe_stage_times", "fe_version", "fe_kernel_timer",
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
     "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats", "fe_batch_create", "fe_batch_destroy",
-    "fe_batch_advance_frame", "fe_surface_build", "fe_surface_download",
+    "fe_batch_advance_frame", "fe_surface_build", "fe_surface_download", "fe_scene_clear_colliders",
+    "fe_scene_set_sdf_search",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -110,6 +112,8 @@ def load():
     L.fe_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(vp)]
     L.fe_batch_destroy.argtypes = [vp]
     L.fe_surface_build.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.fe_scene_clear_colliders.argtypes = [vp]
+    L.fe_scene_set_sdf_search.argtypes = [vp, C.c_int32]
     L.fe_surface_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.fe_batch_advance_frame.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_double)]
@@ -261,7 +265,7 @@ class Scene:
     def dtype(field):
         if field in (GRID_MU, GRID_MV, GRID_MW, GRID_FLAGS, GRID_DMU, GRID_DMV, GRID_DMW):
             return np.uint8
-        if field in (GRID_SDF, GRID_PRESSURE):
+        if field in (GRID_SDF, GRID_PRESSURE, GRID_COLLIDER_SDF):
             return np.float64
         return np.float32
 
@@ -353,6 +357,16 @@ class Scene:
         self.ctx.check(self.L.fe_surface_download(self._h, v.ctypes.data_as(dp), n.ctypes.data_as(dp),
                                                   t.ctypes.data_as(C.POINTER(C.c_int32))), "fe_surface_download")
         return v, n, t
+
+    def set_collider_sdf(self, phi):
+        """Upload the collider SDF (nz, ny, nx FP64 at the nodes origin + h*(i,j,k))."""
+        self.set(GRID_COLLIDER_SDF, np.ascontiguousarray(phi, dtype=np.float64))
+
+    def set_sdf_search(self, mode):
+        self.ctx.check(self.L.fe_scene_set_sdf_search(self._h, mode), "fe_scene_set_sdf_search")
+
+    def clear_colliders(self):
+        self.ctx.check(self.L.fe_scene_clear_colliders(self._h), "fe_scene_clear_colliders")
 
     def set_diffusion_solver(self, desc):
         self.ctx.check(self.L.fe_scene_set_diffusion_solver(self._h, C.byref(desc)), "fe_scene_set_diffusion_solver")

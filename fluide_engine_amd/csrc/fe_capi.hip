@@ -82,6 +82,7 @@ size_t field_bytes(const fe_scene* s, int32_t field) {
         case FE_GRID_DMU: return face_count(g, 0);
         case FE_GRID_DMV: return face_count(g, 1);
         case FE_GRID_DMW: return face_count(g, 2);
+        case FE_GRID_COLLIDER_SDF: return cell_count(g) * sizeof(double);
         default: return 0;
     }
 }
@@ -104,6 +105,7 @@ void* field_ptr(fe_scene* s, int32_t field) {
         case FE_GRID_DMU: return s->dmark[0];
         case FE_GRID_DMV: return s->dmark[1];
         case FE_GRID_DMW: return s->dmark[2];
+        case FE_GRID_COLLIDER_SDF: return s->csdf;
         default: return nullptr;
     }
 }
@@ -205,30 +207,37 @@ int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st)
     clk.begin(FE_STAGE_SDF);
     HIPCHK(ctx, launch_sdf(s));
     HIPCHK(ctx, launch_markers(s));  // BuildMarkers reads only the SDF: done once here
+    // ApplyBoundryCondition is the wall zeroing without a collider (fused into
+    // the gravity and pressure-gradient kernels) and the full
+    // ConstrainVelocity with one
+    const bool col = s->has_collider;
     clk.begin(FE_STAGE_EXTRAPOLATE);
     HIPCHK(ctx, launch_extrapolate(s));
-    HIPCHK(ctx, launch_walls(s));
+    HIPCHK(ctx, launch_bc(s));
     // ComputeExternalForces (:203-212) = forces, then ApplyBoundryCondition
     clk.begin(FE_STAGE_GRAVITY);
-    HIPCHK(ctx, launch_gravity(s, dt, true));
+    HIPCHK(ctx, launch_gravity(s, dt, !col));
+    if (col) HIPCHK(ctx, launch_bc(s));
     // ComputeDiffusion (:214-222) when viscosity > 0, then ApplyBoundryCondition
     if (s->desc.viscosity > 0.0) {
         clk.begin(FE_STAGE_DIFFUSION);
         HIPCHK(ctx, launch_diffusion(s, dt, &s->diff_desc));
-        HIPCHK(ctx, launch_walls(s));
+        HIPCHK(ctx, launch_bc(s));
     }
     // ComputePressure (:224-229)
     const int pev = clk.n;
     clk.begin(FE_STAGE_PRESSURE);
-    int rc = pressure(s, d, true, st, pev);
+    int rc = pressure(s, d, !col, st, pev);
     if (rc != FE_OK) return rc;
+    if (col) HIPCHK(ctx, launch_bc(s));
     // ComputeAdvection (:231-237)
     clk.begin(FE_STAGE_EXTRAPOLATE);
     HIPCHK(ctx, launch_extrapolate(s));
-    HIPCHK(ctx, launch_walls(s));
+    HIPCHK(ctx, launch_bc(s));
     clk.begin(FE_STAGE_G2P);
-    HIPCHK(ctx, launch_g2p_move(s, dt, true, true, true));
-    s->keys_ready = true;
+    HIPCHK(ctx, launch_g2p_move(s, dt, true, true, !col));
+    HIPCHK(ctx, launch_collide(s));  // moved particles inside the collider (keys recomputed by the sort)
+    s->keys_ready = !col;
     clk.end();
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
     clk.collect();
@@ -447,6 +456,7 @@ int fe_scene_destroy(fe_scene* s) {
     mg_free(s);
     diffusion_free(s);
     surface_free(s);
+    collider_free(s);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
@@ -700,7 +710,7 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
             HIPCHK(ctx, launch_extrapolate(s));
             break;
         case FE_STAGE_BC:
-            HIPCHK(ctx, launch_walls(s));
+            HIPCHK(ctx, launch_bc(s));
             break;
         case FE_STAGE_GRAVITY:
             HIPCHK(ctx, launch_gravity(s, dt, false));
@@ -720,6 +730,7 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
         }
         case FE_STAGE_MOVE:
             HIPCHK(ctx, launch_g2p_move(s, dt, false, true, false));
+            HIPCHK(ctx, launch_collide(s));
             break;
         default:
             return set_err(ctx, FE_E_INVALID, "unknown stage " + std::to_string(stage));
@@ -735,6 +746,22 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
 int fe_last_solve_stats(const fe_scene* s, fe_solve_stats* st) {
     if (!s || !st) return FE_E_INVALID;
     *st = s->last_stats;
+    return FE_OK;
+}
+
+int fe_scene_clear_colliders(fe_scene* s) {
+    if (!s) return FE_E_INVALID;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+    collider_free(s);
+    return FE_OK;
+}
+
+int fe_scene_set_sdf_search(fe_scene* s, int32_t mode) {
+    if (!s) return FE_E_INVALID;
+    if (mode != FE_SDF_SEARCH_CELLS && mode != FE_SDF_SEARCH_BUCKETS)
+        return set_err(s->ctx, FE_E_INVALID, "unknown SDF search mode " + std::to_string(mode));
+    s->sdf_buckets = mode == FE_SDF_SEARCH_BUCKETS;
     return FE_OK;
 }
 
@@ -823,7 +850,9 @@ int fe_grid_download(fe_scene* s, int32_t field, void* dst, size_t bytes) {
     const size_t need = field_bytes(s, field);
     if (need == 0) return set_err(ctx, FE_E_INVALID, "unknown grid field " + std::to_string(field));
     if (bytes != need) return set_err(ctx, FE_E_INVALID, "byte count mismatch: need " + std::to_string(need));
-    if (!field_ptr(s, field)) return set_err(ctx, FE_E_INVALID, "no diffusion solve has run on this scene");
+    if (!field_ptr(s, field))
+        return set_err(ctx, FE_E_INVALID, field == FE_GRID_COLLIDER_SDF ? "the scene has no collider"
+                                                                     : "no diffusion solve has run on this scene");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(dst, field_ptr(s, field), need, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
@@ -838,6 +867,10 @@ int fe_grid_upload(fe_scene* s, int32_t field, const void* src, size_t bytes) {
     if (bytes != need) return set_err(ctx, FE_E_INVALID, "byte count mismatch: need " + std::to_string(need));
     if (diffusion_field(field)) return set_err(ctx, FE_E_INVALID, "diffusion markers are download-only");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (field == FE_GRID_COLLIDER_SDF && !s->csdf) {  // first collider: ColliderCollection storage
+        HIPCHK(ctx, hipMalloc((void**)&s->csdf, need));
+        s->has_collider = true;
+    }
     HIPCHK(ctx, hipMemcpyAsync(field_ptr(s, field), src, need, hipMemcpyHostToDevice, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
     return FE_OK;

@@ -170,6 +170,47 @@ __device__ inline double sample_sdf64(const Geom& g, const double* __restrict__ 
                   ff[2]);
 }
 
+// ScalarGrid3D::Sample of the collider SDF (node centred at the domain origin,
+// src/3d/collider_collection.cpp:5-8) at p, in FP64.
+__device__ inline double sample_collider64(const Geom& g, const double* __restrict__ c, const double p[3]) {
+#pragma clang fp contract(off)
+    int ii[3];
+    double ff[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bary64((p[a] - g.od[a]) / g.hd[a], g.n[a], ii[a], ff[a]);
+    const int ip1 = min(ii[0] + 1, g.n[0] - 1), jp1 = min(ii[1] + 1, g.n[1] - 1), kp1 = min(ii[2] + 1, g.n[2] - 1);
+    auto S = [&](int a, int b, int cc) { return c[(int64_t)a + (int64_t)g.n[0] * ((int64_t)b + (int64_t)g.n[1] * cc)]; };
+    const int i = ii[0], j = ii[1], k = ii[2];
+    return lerp64(lerp64(lerp64(S(i, j, k), S(ip1, j, k), ff[0]), lerp64(S(i, jp1, k), S(ip1, jp1, k), ff[0]), ff[1]),
+                  lerp64(lerp64(S(i, j, kp1), S(ip1, j, kp1), ff[0]), lerp64(S(i, jp1, kp1), S(ip1, jp1, kp1), ff[0]), ff[1]),
+                  ff[2]);
+}
+
+// ScalarGrid3D::Gradient (src/3d/scalar_grid3d.cpp:67-90) of the collider SDF:
+// the trilinear weights at p times clamped central differences at the nodes.
+__device__ inline void collider_gradient64(const Geom& g, const double* __restrict__ c, const double p[3],
+                                           double out[3]) {
+#pragma clang fp contract(off)
+    int ii[3];
+    double ff[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) bary64((p[a] - g.od[a]) / g.hd[a], g.n[a], ii[a], ff[a]);
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+    auto S = [&](int a, int b, int cc) { return c[(int64_t)a + (int64_t)n0 * ((int64_t)b + (int64_t)n1 * cc)]; };
+    out[0] = out[1] = out[2] = 0.0;
+    for (int q = 0; q < 8; ++q) {
+        const int i = min(ii[0] + (q & 1), n0 - 1), j = min(ii[1] + ((q >> 1) & 1), n1 - 1),
+                  k = min(ii[2] + ((q >> 2) & 1), n2 - 1);
+        const double wx = (q & 1) ? ff[0] : 1.00 - ff[0], wy = ((q >> 1) & 1) ? ff[1] : 1.00 - ff[1],
+                     wz = ((q >> 2) & 1) ? ff[2] : 1.00 - ff[2];
+        const double w = wx * wy * wz;  // GetCooridnatesAndWeights' products
+        const double d[3] = {S(i + 1 < n0 ? i + 1 : i, j, k) - S(i > 0 ? i - 1 : i, j, k),
+                             S(i, j + 1 < n1 ? j + 1 : j, k) - S(i, j > 0 ? j - 1 : j, k),
+                             S(i, j, k + 1 < n2 ? k + 1 : k) - S(i, j, k > 0 ? k - 1 : k)};
+        for (int a = 0; a < 3; ++a) out[a] += (w * 0.5 * d[a]) / g.hd[a];
+    }
+}
+
 // FP32 fast coordinates for interpolation (values, not parity-critical indices).
 __device__ inline void fast_axis(float p, float o, float ih, int size, int& i, float& f) {
     const float x = (p - o) * ih;
@@ -296,6 +337,7 @@ struct McState {
     int64_t plane_stride = 0;
     double *pos = nullptr, *nrm = nullptr;
     int32_t* tri = nullptr;
+    double* comb = nullptr;  // fluid SDF combined with the collider's
     int64_t vcap = 0, ncap = 0, tcap = 0;
     int64_t nv3 = 0, nt3 = 0;  // cube-part vertices and triangles of the last build
     std::vector<double> cap_pos, cap_nrm;
@@ -369,10 +411,18 @@ struct fe_scene {
     fe_solver_desc diff_desc{FE_SOLVER_JACOBI, 1000, 5, 0, 0, 0, 1e-14};
     double* dx[2] = {nullptr, nullptr};
     uint16_t* dcode = nullptr;
+    double* db = nullptr;
     uint8_t* dmark[3] = {nullptr, nullptr, nullptr};
     fe_solve_stats diff_stats[3] = {};
 
     fe::McState mc;  // surface extraction
+
+    // colliders: the ColliderCollection SDF (node centred at the origin, n^3
+    // FP64) and scratch for the tangential pass of the boundary condition
+    double* csdf = nullptr;
+    bool has_collider = false;
+    bool sdf_buckets = false;  // FE_SDF_SEARCH_BUCKETS
+    float* bc_tmp[3] = {nullptr, nullptr, nullptr};
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

@@ -29,6 +29,7 @@ hipError_t launch_cells_by_pid(fe_scene* s, int32_t* out);
 hipError_t launch_sdf(fe_scene* s);
 hipError_t launch_markers(fe_scene* s);
 hipError_t launch_extrapolate(fe_scene* s);               // ExtrapolateVelocityToAir, depth iterations
+hipError_t launch_extrapolate_from(fe_scene* s, const uint8_t* const init[3]);  // from other validity
 hipError_t launch_walls(fe_scene* s);                      // wall layers of ConstrainVelocity
 hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after);
 hipError_t launch_apply_pressure(fe_scene* s, bool walls_after);
@@ -51,6 +52,13 @@ void mg_free(fe_scene* s);
 hipError_t surface_build(fe_scene* s);  // counts in s->mc (nv3 + caps, nt3 + caps)
 hipError_t surface_download(fe_scene* s, double* pos, double* nrm, int32_t* tri);
 void surface_free(fe_scene* s);
+
+// k_collider.hip: ApplyBoundryCondition (walls only without a collider) and
+// the collision response of moved particles
+hipError_t launch_bc(fe_scene* s);
+hipError_t launch_collide(fe_scene* s);
+hipError_t launch_sdf_into_collider(fe_scene* s);  // ExtrapolateIntoCollider on the fluid SDF
+void collider_free(fe_scene* s);
 
 // k_diffusion.hip: BackwardEulerDiffusionSolver (without the trailing boundary condition)
 hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d);

@@ -4,18 +4,18 @@
 // PICSimulator::ComputeDiffusion (src/pic_simulator.cpp:214-222) when the
 // viscosity is positive.  Per face component:
 //   * face markers: FLUID iff the fluid SDF sampled at the face position is
-//     negative (BuildX/Y/ZMarkers, :71-128).  Without colliders the collider
-//     SDF is positive everywhere, so every other face is AIR.
+//     negative, else BOUNDRY iff the collider SDF is, else AIR (BuildX/Y/ZMarkers,
+//     :71-128);
 //   * system (BuildMatrix :136-198, BuildVectors :200-242): a FLUID row has
 //     centre 1 + c_a per in-range non-AIR neighbour and -c_a to every FLUID
-//     neighbour; other rows are the identity; b = x0 = u (no BOUNDRY faces).
-//     c_a = dt * viscosity / h_a^2.
+//     neighbour, b = u + c_a * u(neighbour) per BOUNDRY neighbour; other rows
+//     are the identity with b = u; x0 = u.  c_a = dt * viscosity / h_a^2.
 //   * solver: the reference's CudaJacobiIterationSolver(1000, 5, 1e-14) (:10)
 //     semantics: sweep, and after sweep it (it > 0, it % interval == 0) stop
 //     when ||b - A x||_2 < tolerance.
-// Everything is FP64 (x, the row arithmetic in the reference's operation order
-// with contraction off, the residual); the face velocities stay FP32 and are
-// exact as FP64 right-hand sides.  The 16-bit row code has the pressure
+// Everything is FP64 (x, b, the row arithmetic in the reference's operation
+// order with contraction off, the residual); the face velocities stay FP32
+// and are exact in FP64.  The 16-bit row code has the pressure
 // solver's layout: bits 0-5 coupled (FLUID) neighbours in the order i-1, i+1,
 // j-1, j+1, k-1, k+1; bits 6-11 in-range non-AIR neighbours per axis; bit 12
 // the row is FLUID.
@@ -39,7 +39,8 @@ __device__ inline void face_ijk(const FaceDims& d, int64_t f, int& i, int& j, in
 
 // Markers and row codes of component C, and x0 = u.
 __global__ __launch_bounds__(kBlock) void k_diff_markers(Geom g, int C, FaceDims d, const double* __restrict__ sdf,
-                                                         uint8_t* __restrict__ mark, int64_t padded) {
+                                                         const double* __restrict__ csdf, uint8_t* __restrict__ mark,
+                                                         int64_t padded) {
 #pragma clang fp contract(off)
     const int64_t f = xcd_remap(blockIdx.x, padded) * kBlock + threadIdx.x;
     const int64_t nf = (int64_t)d.nx * d.ny * d.nz;
@@ -49,16 +50,24 @@ __global__ __launch_bounds__(kBlock) void k_diff_markers(Geom g, int C, FaceDims
     double p[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) p[a] = g.cod[C][a] + g.hd[a] * (double)idx[a];  // GetX/Y/ZPos
-    mark[f] = sample_sdf64(g, sdf, p) < 0 ? (uint8_t)FE_FLUID : (uint8_t)FE_AIR;
+    uint8_t m = FE_AIR;
+    if (sample_sdf64(g, sdf, p) < 0)
+        m = FE_FLUID;
+    else if (csdf && sample_collider64(g, csdf, p) < 0)
+        m = FE_BOUNDRY;
+    mark[f] = m;
 }
 
 __global__ __launch_bounds__(kBlock) void k_diff_build(FaceDims d, const uint8_t* __restrict__ mark,
                                                        const float* __restrict__ u, uint16_t* __restrict__ code,
-                                                       double* __restrict__ x0, int64_t padded) {
+                                                       double* __restrict__ x0, double* __restrict__ b, double cx,
+                                                       double cy, double cz, int64_t padded) {
+#pragma clang fp contract(off)
     const int64_t f = xcd_remap(blockIdx.x, padded) * kBlock + threadIdx.x;
     const int64_t nf = (int64_t)d.nx * d.ny * d.nz;
     if (f >= nf) return;
     uint16_t cd = 0;
+    double bb = (double)u[f];
     if (mark[f] == FE_FLUID) {
         int i, j, k;
         face_ijk(d, f, i, j, k);
@@ -74,9 +83,18 @@ __global__ __launch_bounds__(kBlock) void k_diff_build(FaceDims d, const uint8_t
             cd = (uint16_t)(cd + (1u << (6 + 2 * (q >> 1))));
             if (m == FE_FLUID) cd |= (uint16_t)(1u << q);
         }
+        // BuildVectors' BOUNDRY terms in its order: i+1, i-1, j+1, j-1, k+1, k-1
+        const double c[3] = {cx, cy, cz};
+        const int order[6] = {1, 0, 3, 2, 5, 4};
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            const int q = order[t];
+            if (in[q] && mark[nb[q]] == FE_BOUNDRY) bb += c[q >> 1] * (double)u[nb[q]];
+        }
     }
     code[f] = cd;
     x0[f] = (double)u[f];
+    b[f] = bb;
 }
 
 // Centre 1 + c_a per non-AIR neighbour, summed in BuildMatrix's order (x, y, z).
@@ -106,7 +124,7 @@ __device__ inline double diff_offdiag(uint16_t cd, int64_t f, int64_t sy, int64_
 }
 
 __global__ __launch_bounds__(kBlock) void k_diff_jacobi(FaceDims d, const uint16_t* __restrict__ code,
-                                                        const float* __restrict__ b, const double* __restrict__ xo,
+                                                        const double* __restrict__ b, const double* __restrict__ xo,
                                                         double* __restrict__ xn, double cx, double cy, double cz,
                                                         const SolverScalars* sc, int64_t padded) {
 #pragma clang fp contract(off)
@@ -115,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_jacobi(FaceDims d, const uint16
     const int64_t nf = (int64_t)d.nx * d.ny * d.nz;
     if (f >= nf) return;
     const uint16_t cd = code[f];
-    const double bb = (double)b[f];
+    const double bb = b[f];
     if (!(cd & kFluidBit)) {
         xn[f] = bb;  // identity row: (b - 0) / 1
         return;
@@ -127,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_jacobi(FaceDims d, const uint16
 
 // Partials of ||b - A x||^2 (BLAS::Residual, src/linear_system/blas.cpp:21-37).
 __global__ __launch_bounds__(kBlock) void k_diff_residual(FaceDims d, const uint16_t* __restrict__ code,
-                                                          const float* __restrict__ b,
+                                                          const double* __restrict__ b,
                                                           const double* __restrict__ x, double cx, double cy,
                                                           double cz, const SolverScalars* sc,
                                                           double* __restrict__ part, int64_t chunk, int64_t padded) {
@@ -139,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_residual(FaceDims d, const uint
     if (!sc->done) {
         for (int64_t f = sp.c0 + threadIdx.x; f < sp.c1; f += kBlock) {
             const uint16_t cd = code[f];
-            double r = (double)b[f];
+            double r = b[f];
             if (cd & kFluidBit)
                 r = (r - diff_center(cd, c) * x[f]) - diff_offdiag(cd, f, d.nx, (int64_t)d.nx * d.ny, c, x);
             else
@@ -184,6 +202,7 @@ hipError_t diffusion_alloc(fe_scene* s) {
     for (int b = 0; b < 2; ++b)
         if ((e = hipMalloc((void**)&s->dx[b], sizeof(double) * nf)) != hipSuccess) return e;
     if ((e = hipMalloc((void**)&s->dcode, sizeof(uint16_t) * nf)) != hipSuccess) return e;
+    if ((e = hipMalloc((void**)&s->db, sizeof(double) * nf)) != hipSuccess) return e;
     for (int c = 0; c < 3; ++c)
         if ((e = hipMalloc((void**)&s->dmark[c], face_count(s->g, c))) != hipSuccess) return e;
     return hipSuccess;
@@ -196,6 +215,8 @@ void diffusion_free(fe_scene* s) {
     }
     if (s->dcode) (void)hipFree(s->dcode);
     s->dcode = nullptr;
+    if (s->db) (void)hipFree(s->db);
+    s->db = nullptr;
     for (int c = 0; c < 3; ++c) {
         if (s->dmark[c]) (void)hipFree(s->dmark[c]);
         s->dmark[c] = nullptr;
@@ -219,10 +240,12 @@ hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d) {
         const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
         const SolverShape sh = solver_shape_n(nf);
         if (sh.blocks > s->npartials) return hipErrorInvalidValue;  // partial sums must fit the scene's buffer
-        hipLaunchKernelGGL(k_diff_markers, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, comp, fd, s->sdf,
+        hipLaunchKernelGGL(k_diff_markers, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, comp, fd,
+                           (const double*)s->sdf, (const double*)(s->has_collider ? s->csdf : nullptr),
                            s->dmark[comp], padded);
         hipLaunchKernelGGL(k_diff_build, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, fd,
-                           (const uint8_t*)s->dmark[comp], (const float*)s->u[comp], s->dcode, s->dx[0], padded);
+                           (const uint8_t*)s->dmark[comp], (const float*)s->u[comp], s->dcode, s->dx[0], s->db, c[0],
+                           c[1], c[2], padded);
         SolverScalars init{};
         init.tol = d->tolerance;
         init.max_it = d->max_iterations;
@@ -235,11 +258,11 @@ hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d) {
             const int end = std::min(d->max_iterations, it + kBatch);
             for (; it < end; ++it) {
                 hipLaunchKernelGGL(k_diff_jacobi, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, fd,
-                                   (const uint16_t*)s->dcode, (const float*)s->u[comp], (const double*)s->dx[it & 1],
+                                   (const uint16_t*)s->dcode, (const double*)s->db, (const double*)s->dx[it & 1],
                                    s->dx[(it + 1) & 1], c[0], c[1], c[2], (const SolverScalars*)s->sc, padded);
                 if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
                     hipLaunchKernelGGL(k_diff_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, fd,
-                                       (const uint16_t*)s->dcode, (const float*)s->u[comp],
+                                       (const uint16_t*)s->dcode, (const double*)s->db,
                                        (const double*)s->dx[(it + 1) & 1], c[0], c[1], c[2],
                                        (const SolverScalars*)s->sc, s->partials, sh.chunk, sh.blocks);
                     hipLaunchKernelGGL(k_diff_check, dim3(1), dim3(kBlock), 0, s->stream, (const double*)s->partials,

@@ -51,7 +51,16 @@ hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n) {
 // |d2_fp32 - d2_fp64| for candidates within 2*band, so the candidate with the
 // smallest FP64 distance is never skipped (its FP32 value is at most the FP32
 // minimum + 2E) and the result is the same FP64 minimum.
+//
+// Buckets = true restates the reference's candidate set exactly, so that every
+// value (positive ones included) is the reference's: a particle is a candidate
+// iff its bucket floor(p / 2r) lies, per axis, in {o, o + d} with o the bucket
+// of the centre and d = +1 when (o + 0.5f) * 2r <= centre else -1
+// (hash_grid_particle_searcher.cpp:154-198; the 64-bucket wrap aliases only
+// buckets farther than 2r, which the distance test drops).  The scan covers the
+// cells overlapping that bucket block within 2r of the centre.
 // ---------------------------------------------------------------------------
+template <bool Buckets>
 __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict__ pos4,
                                                 const int32_t* __restrict__ start, double* __restrict__ sdf,
                                                 int ex, int ey, int ez, double radius, double band, double qr2,
@@ -67,9 +76,27 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict
     const double gx = (g.od[0] + g.hd[0] / 2.0) + (double)i * g.hd[0];
     const double gy = (g.od[1] + g.hd[1] / 2.0) + (double)j * g.hd[1];
     const double gz = (g.od[2] + g.hd[2] / 2.0) + (double)k * g.hd[2];
-    const int i0 = max(i - ex, 0), i1 = min(i + ex, g.n[0] - 1);
-    const int j0 = max(j - ey, 0), j1 = min(j + ey, g.n[1] - 1);
-    const int k0 = max(k - ez, 0), k1 = min(k + ez, g.n[2] - 1);
+    int i0 = max(i - ex, 0), i1 = min(i + ex, g.n[0] - 1);
+    int j0 = max(j - ey, 0), j1 = min(j + ey, g.n[1] - 1);
+    int k0 = max(k - ez, 0), k1 = min(k + ez, g.n[2] - 1);
+    const double sp = 2.0 * radius;
+    int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+    if (Buckets) {
+        const double gp[3] = {gx, gy, gz};
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            const int o = (int)floor(gp[a] / sp);
+            const int d = ((double)((float)o + 0.5f) * sp <= gp[a]) ? 1 : -1;
+            blo[a] = min(o, o + d);
+            bhi[a] = max(o, o + d);
+            // cells overlapping [blo*sp, (bhi+1)*sp) and [gp - band, gp + band], one cell of margin
+            const double x0 = fmax((double)blo[a] * sp, gp[a] - band);
+            const double x1 = fmin((double)(bhi[a] + 1) * sp, gp[a] + band);
+            lo[a] = max((int)floor((x0 - g.od[a]) / g.hd[a]) - 1, 0);
+            hi[a] = min((int)floor((x1 - g.od[a]) / g.hd[a]) + 1, g.n[a] - 1);
+        }
+        i0 = lo[0], i1 = hi[0], j0 = lo[1], j1 = hi[1], k0 = lo[2], k1 = hi[2];
+    }
     const float gx32 = (float)gx, gy32 = (float)gy, gz32 = (float)gz;
     float best32 = __builtin_inff();
     double best = qr2;
@@ -82,6 +109,12 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict
                 const float fx = p.x - gx32, fy = p.y - gy32, fz = p.z - gz32;
                 const float d2f = fx * fx + fy * fy + fz * fz;
                 if (d2f > best32 + slack) return;
+                if (Buckets) {
+                    const int bx = (int)floor((double)p.x / sp), by = (int)floor((double)p.y / sp),
+                              bz = (int)floor((double)p.z / sp);
+                    if (bx < blo[0] || bx > bhi[0] || by < blo[1] || by > bhi[1] || bz < blo[2] || bz > bhi[2])
+                        return;
+                }
                 best32 = fminf(best32, d2f);
                 const double dx = (double)p.x - gx;
                 const double dy = (double)p.y - gy;
@@ -129,20 +162,29 @@ hipError_t launch_sdf(fe_scene* s) {
     const float slack = (float)(4.0 * E);
     const int64_t nc = cell_count(g);
     const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-    launch_timed(s, FE_KERNEL_SDF, 0, k_sdf, dim3((unsigned)padded), dim3(kBlock), g, (const float4*)s->pos4,
-                 (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius, band, qr2, slack, padded);
-    return hipGetLastError();
+    if (s->has_collider || s->sdf_buckets)
+        launch_timed(s, FE_KERNEL_SDF, 0, k_sdf<true>, dim3((unsigned)padded), dim3(kBlock), g,
+                     (const float4*)s->pos4, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius,
+                     band, qr2, slack, padded);
+    else
+        launch_timed(s, FE_KERNEL_SDF, 0, k_sdf<false>, dim3((unsigned)padded), dim3(kBlock), g,
+                     (const float4*)s->pos4, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius,
+                     band, qr2, slack, padded);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_sdf_into_collider(s);
 }
 
 // ---------------------------------------------------------------------------
 // SinglePhasePressureSolver::BuildMarkers (single_phase_pressure_solver.cpp:30-48):
 // FLUID iff fluidSdf.Sample(GetCellCenterPos(i,j,k)) < 0, evaluated with the
-// reference's FP64 arithmetic (ScalarGrid3D::Sample, src/3d/scalar_grid3d.cpp:33-58).
-// Without colliders the collider SDF is positive everywhere, so the other
-// cells are AIR.
+// reference's FP64 arithmetic (ScalarGrid3D::Sample, src/3d/scalar_grid3d.cpp:33-58);
+// else BOUNDRY iff the collider SDF samples negative there, else AIR.  Without
+// a collider the collider SDF is positive everywhere and the other cells are AIR.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_markers(Geom g, const double* __restrict__ sdf,
-                                                    uint8_t* __restrict__ flags, int64_t padded) {
+                                                    const double* __restrict__ csdf, uint8_t* __restrict__ flags,
+                                                    int64_t padded) {
 #pragma clang fp contract(off)
     const int64_t blk = xcd_remap(blockIdx.x, padded);
     const int64_t c = blk * kBlock + threadIdx.x;
@@ -153,13 +195,19 @@ __global__ __launch_bounds__(kBlock) void k_markers(Geom g, const double* __rest
     double cc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) cc[a] = (g.od[a] + 0.5 * g.hd[a]) + g.hd[a] * (double)idx[a];  // GetCellCenterPos
-    flags[c] = sample_sdf64(g, sdf, cc) < 0 ? (uint8_t)FE_FLUID : (uint8_t)FE_AIR;
+    uint8_t f = FE_AIR;
+    if (sample_sdf64(g, sdf, cc) < 0)
+        f = FE_FLUID;
+    else if (csdf && sample_collider64(g, csdf, cc) < 0)
+        f = FE_BOUNDRY;
+    flags[c] = f;
 }
 
 hipError_t launch_markers(fe_scene* s) {
     const int64_t nc = cell_count(s->g);
     const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_markers, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, s->g, s->sdf, s->flags, padded);
+    hipLaunchKernelGGL(k_markers, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, s->g, (const double*)s->sdf,
+                       (const double*)(s->has_collider ? s->csdf : nullptr), s->flags, padded);
     return hipGetLastError();
 }
 
@@ -235,6 +283,13 @@ __global__ __launch_bounds__(kBlock) void k_extrapolate(Geom g, FaceSet a, Valid
 }
 
 hipError_t launch_extrapolate(fe_scene* s) {
+    const uint8_t* init[3] = {s->mark[0], s->mark[1], s->mark[2]};
+    return launch_extrapolate_from(s, init);
+}
+
+// depth sweeps from the validity init[c]; the sweeps ping-pong through
+// s->valid[0] and s->valid[1] (init may be s->valid[1])
+hipError_t launch_extrapolate_from(fe_scene* s, const uint8_t* const init[3]) {
     const Geom& g = s->g;
     const int64_t per = kBlock * kExtraPer;
     const int64_t nb0 = (face_count(g, 0) + per - 1) / per, nb1 = (face_count(g, 1) + per - 1) / per,
@@ -244,7 +299,7 @@ hipError_t launch_extrapolate(fe_scene* s) {
     for (int it = 0; it < s->depth; ++it) {
         ValidSet vs;
         for (int c = 0; c < 3; ++c) {
-            vs.in[c] = it == 0 ? s->mark[c] : s->valid[(it - 1) & 1] + off[c];
+            vs.in[c] = it == 0 ? init[c] : s->valid[(it - 1) & 1] + off[c];
             vs.out[c] = s->valid[it & 1] + off[c];
         }
         hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)(nb0 + nb1 + nb2)), dim3(kBlock), 0, s->stream, g, a, vs, nb0,

@@ -302,6 +302,17 @@ void plane_caps(const NodeGrid& G, int p, const double* val, std::vector<double>
         }
 }
 
+// The surface SDF with a collider: the fluid SDF, replaced by minus the
+// collider SDF where both are negative (GetSurface, src/hybrid_simulator.cpp:29-37).
+__global__ __launch_bounds__(kBlock) void k_mc_combine(const double* __restrict__ sdf,
+                                                       const double* __restrict__ csdf, double* __restrict__ out,
+                                                       int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const double a = sdf[c], b = csdf[c];
+    out[c] = (a < 0 && b < 0) ? -b : a;
+}
+
 }  // namespace
 
 void surface_free(fe_scene* s) {
@@ -320,6 +331,7 @@ void surface_free(fe_scene* s) {
     F(m.pos);
     F(m.nrm);
     F(m.tri);
+    F(m.comb);
     m = McState{};
 }
 
@@ -343,13 +355,21 @@ hipError_t surface_build(fe_scene* s) {
         if ((e = hipMalloc((void**)&m.planes, sizeof(double) * 6 * m.plane_stride)) != hipSuccess) return e;
     }
     hipStream_t st = s->stream;
+    const double* sdf = s->sdf;
+    if (s->has_collider) {
+        const int64_t nc = cell_count(s->g);
+        if (!m.comb && (e = hipMalloc((void**)&m.comb, sizeof(double) * nc)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_mc_combine, grid_of(nc), dim3(kBlock), 0, st, sdf,
+                           (const double*)s->csdf, m.comb, nc);
+        sdf = m.comb;
+    }
     // cubes: crossed edges -> vertex ids; per-cube triangle counts -> offsets
-    hipLaunchKernelGGL(k_mc_edges, grid_of(nedges), dim3(kBlock), 0, st, G, (const double*)s->sdf, m.flag, nedges);
+    hipLaunchKernelGGL(k_mc_edges, grid_of(nedges), dim3(kBlock), 0, st, G, sdf, m.flag, nedges);
     if ((e = launch_scan_with(st, m.flag, m.vid, nedges, m.sums)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mc_count, grid_of(ncubes), dim3(kBlock), 0, st, G, (const double*)s->sdf,
+    hipLaunchKernelGGL(k_mc_count, grid_of(ncubes), dim3(kBlock), 0, st, G, sdf,
                        (const McTables*)m.tables, m.cnt, ncubes);
     if ((e = launch_scan_with(st, m.cnt, m.toff, ncubes, m.sums)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mc_planes, grid_of(6 * m.plane_stride), dim3(kBlock), 0, st, G, (const double*)s->sdf,
+    hipLaunchKernelGGL(k_mc_planes, grid_of(6 * m.plane_stride), dim3(kBlock), 0, st, G, sdf,
                        m.planes, m.plane_stride);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     int32_t nv = 0, nt = 0;
@@ -364,10 +384,10 @@ hipError_t surface_build(fe_scene* s) {
     if ((e = grow(&m.nrm, &m.ncap, 3 * (int64_t)nv)) != hipSuccess) return e;
     if ((e = grow(&m.tri, &m.tcap, 3 * (int64_t)nt)) != hipSuccess) return e;
     if (nv > 0)
-        hipLaunchKernelGGL(k_mc_vertices, grid_of(nedges), dim3(kBlock), 0, st, G, (const double*)s->sdf,
+        hipLaunchKernelGGL(k_mc_vertices, grid_of(nedges), dim3(kBlock), 0, st, G, sdf,
                            (const int32_t*)m.vid, m.pos, m.nrm, nedges);
     if (nt > 0)
-        hipLaunchKernelGGL(k_mc_emit, grid_of(ncubes), dim3(kBlock), 0, st, G, (const double*)s->sdf,
+        hipLaunchKernelGGL(k_mc_emit, grid_of(ncubes), dim3(kBlock), 0, st, G, sdf,
                            (const McTables*)m.tables, (const int32_t*)m.toff, (const int32_t*)m.vid, m.tri, ncubes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // caps on the host while the device writes the cube part

@@ -109,7 +109,10 @@ enum fe_grid_field {
     FE_GRID_DMU = 13,    /* uint8 fe_marker of the u faces in the last diffusion solve
                             (BackwardEulerDiffusionSolver::_fluidMarkers); download only */
     FE_GRID_DMV = 14,
-    FE_GRID_DMW = 15
+    FE_GRID_DMW = 15,
+    FE_GRID_COLLIDER_SDF = 16 /* double, nx*ny*nz: the collider SDF at the nodes origin + h*(i,j,k)
+                                 (ColliderCollection::BuildSdf, src/3d/collider_collection.cpp:33-50);
+                                 uploading it adds the collider to the scene */
 };
 
 /* Stages of one substep, for fe_run_stage (PICSimulator::OnAdvanceTimeStep,
@@ -190,6 +193,28 @@ int fe_batch_create(fe_ctx* ctx, fe_scene* const* scenes, int32_t count, int32_t
 int fe_batch_destroy(fe_batch* batch);
 int fe_batch_advance_frame(fe_batch* batch, double frame_dt, const fe_solver_desc* solver, int32_t* substeps,
                            double* ms);
+
+/* Colliders (AddCollider, src/pic_simulator.cpp:46-49): upload the collision
+ * SDF with fe_grid_upload(FE_GRID_COLLIDER_SDF).  With a collider, cells outside
+ * the fluid whose centre samples it negative are BOUNDRY, the boundary
+ * condition runs the collider branches of ConstrainVelocity
+ * (src/fluid_solvers/blocked_boundry_condition_solver.cpp:18-291) and moved
+ * particles inside it are pushed to its zero level set along its gradient with
+ * the inward normal velocity removed.  fe_scene_clear_colliders removes it. */
+int fe_scene_clear_colliders(fe_scene* scene);
+
+/* Candidate search of the fluid SDF (BuildSignedDistanceField,
+ * src/pic_simulator.cpp:452-475).  FE_SDF_SEARCH_CELLS (the default) scans the
+ * cells within the particle radius r of each centre: every negative value is
+ * the reference's, positive ones may be smaller where the reference's buckets
+ * miss particles between r and 2r.  FE_SDF_SEARCH_BUCKETS takes exactly the
+ * reference's candidates (8 hash buckets of width 2r,
+ * src/particle_systems/hash_grid_particle_searcher.cpp:154-204), so every
+ * value is the reference's, at about three times the cost; it is used
+ * whenever the scene has a collider (the extrapolation into the collider
+ * averages positive values). */
+enum fe_sdf_search { FE_SDF_SEARCH_CELLS = 0, FE_SDF_SEARCH_BUCKETS = 1 };
+int fe_scene_set_sdf_search(fe_scene* scene, int32_t mode);
 
 /* Fluid surface: HybridSimulator::GetSurface -> MarchingCubesSolver::BuildSurface
  * (src/3d/marching_cubes_solver.cpp:25-51) on the current fluid SDF, iso value 0.

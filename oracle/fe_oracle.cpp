@@ -2,8 +2,9 @@
 //
 // Clean-room FP64 CPU restatement of the Fluide_Engine hybrid particle-grid
 // substep (PIC / FLIP / APIC transfers, fluid SDF + cell marking, velocity
-// extrapolation, blocked boundary conditions without colliders, gravity,
-// SinglePhasePressureSolver with Jacobi / CG, G2P and midpoint advection).
+// extrapolation, blocked boundary conditions with an SDF collider, gravity,
+// backward-Euler viscosity, SinglePhasePressureSolver with Jacobi / CG, G2P
+// and midpoint advection with collider collision, marching-cubes surface).
 // It is the checker for the HIP library and the CPU baseline timed beside it.
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
 //
@@ -332,6 +333,24 @@ inline void sample_vel(const orc_scene* s, const Arr3& u, const Arr3& v, const A
 // ScalarGrid3D::Sample on the collider SDF (node centred at the domain origin)
 inline double sample_collider(const orc_scene* s, const double p[3]) { return sample_array(s->csdf, s->origin, s->h, p); }
 
+// ScalarGrid3D::Gradient (src/3d/scalar_grid3d.cpp:67-90) on the collider SDF:
+// the trilinear weights of the position times clamped central differences.
+void collider_gradient(const orc_scene* s, const double p[3], double g[3]) {
+    const int size[3] = {s->csdf.sx, s->csdf.sy, s->csdf.sz};
+    Stencil st;
+    coords_weights(size, s->origin, s->h, p, &st);
+    g[0] = g[1] = g[2] = 0.0;
+    for (int c = 0; c < 8; ++c) {
+        const int i = st.idx[c][0], j = st.idx[c][1], k = st.idx[c][2];
+        const double left = s->csdf(i > 0 ? i - 1 : i, j, k), right = s->csdf(i + 1 < size[0] ? i + 1 : i, j, k);
+        const double down = s->csdf(i, j > 0 ? j - 1 : j, k), up = s->csdf(i, j + 1 < size[1] ? j + 1 : j, k);
+        const double back = s->csdf(i, j, k > 0 ? k - 1 : k), front = s->csdf(i, j, k + 1 < size[2] ? k + 1 : k);
+        const double d[3] = {right - left, up - down, front - back};
+        for (int a = 0; a < 3; ++a) g[a] += (st.w[c] * 0.5 * d[a]) / s->h[a];
+    }
+}
+
+
 // ---------------------------------------------------------------------------
 // P2G.  PIC: src/pic_simulator.cpp:356-437.  FLIP: src/flip_simulator.cpp:25-119.
 // APIC: src/apic_simulator.cpp:15-118.
@@ -461,8 +480,8 @@ void g2p(orc_scene* s) {
 }
 
 // ---------------------------------------------------------------------------
-// MoveParticles, src/pic_simulator.cpp:239-312.  Collider resolution (:297-304)
-// is only reached with colliders, which this restatement does not model.
+// MoveParticles, src/pic_simulator.cpp:239-312, with the collider resolution
+// (:297-304) restated on the collider SDF (below).
 // ---------------------------------------------------------------------------
 void move_particles(orc_scene* s, double dt_total) {
     const int64_t N = s->np;
@@ -489,6 +508,29 @@ void move_particles(orc_scene* s, double dt_total) {
                 if (pt1[a] >= s->origin[a] + s->size[a]) {
                     pt1[a] = s->origin[a] + s->size[a];
                     vel[a] = 0.0;
+                }
+            }
+            // Particles that end inside a collider: the reference resolves them
+            // against the collider mesh, indexing the particle by the
+            // collider's loop index (src/pic_simulator.cpp:297-304, a shadowed
+            // i).  Restated for an SDF collider on the particle itself: move it
+            // to the zero level set along the normalised collider gradient,
+            // x -= phi n, and drop an inward normal velocity, v -= min(v.n, 0) n
+            // (restitution 0, friction 0, as the reference's ResolveCollision).
+            if (s->has_collider && sample_collider(s, pt1) < 0) {
+                double g[3];
+                collider_gradient(s, pt1, g);
+                const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+                if (len * len > 0.0) {
+                    const double n[3] = {g[0] / len, g[1] / len, g[2] / len};
+                    const double phi = sample_collider(s, pt1);
+                    for (int a = 0; a < 3; ++a) {
+                        pt1[a] = pt1[a] - phi * n[a];
+                        pt1[a] = std::min(std::max(pt1[a], s->origin[a]), s->origin[a] + s->size[a]);
+                    }
+                    const double vn = vel[0] * n[0] + vel[1] * n[1] + vel[2] * n[2];
+                    if (vn < 0.0)
+                        for (int a = 0; a < 3; ++a) vel[a] = vel[a] - vn * n[a];
                 }
             }
             pos[0] = pt1[0]; pos[1] = pt1[1]; pos[2] = pt1[2];
@@ -604,12 +646,20 @@ double fraction_inside(double phi0, double phi1) {
 }
 
 // BlockedBoundryConditionSolver::ConstrainVelocity,
-// src/fluid_solvers/blocked_boundry_condition_solver.cpp:18-291.  With no
-// collider inside the domain (collider SDF >= 0 everywhere) the face-fraction
-// pass marks every face valid, the extrapolation is the identity and the
-// friction pass copies; what remains is the wall zeroing (:198-244).  The
-// collider branches are restated for an SDF that is >= 0 only; a negative
-// collider SDF is rejected by the caller.
+// src/fluid_solvers/blocked_boundry_condition_solver.cpp:18-291, collider
+// velocity 0 and friction 0 as in the reference:
+//  (1) faces whose collider fraction FractionInsideSdf(phi(x - h/2), phi(x + h/2))
+//      is 1 are zeroed and marked invalid (:38-96);
+//  (2) ExtrapolateToRegion from the valid faces, depth iterations (:98-103);
+//  (3) faces inside the collider take the tangential part of the sampled
+//      velocity, normal = normalised collider gradient, or 0 where the
+//      gradient vanishes (:105-183; ApplyFriction with friction 0 is the
+//      projection v - (v.n) n, src/common/vector3.hpp:160-163);
+//  (4) the wall layers are zeroed (:198-244);
+//  (5) faces between a collider node (colliderSdf(i,j,k) < 0, read at the
+//      index) and a non-collider node are zeroed (:246-290).
+// With no collider inside the domain (collider SDF >= 0 everywhere) steps 1,
+// 2, 3 and 5 are identities and only the walls remain.
 void constrain_velocity(orc_scene* s, int depth) {
     for (int c = 0; c < 3; ++c) {
         Arr3& a = comp_arr(s, c);
@@ -637,6 +687,35 @@ void constrain_velocity(orc_scene* s, int depth) {
             extrapolate_to_region(prev, marker, depth, a, s->nthreads);
         }
     }
+    if (s->has_collider) {
+        Arr3 tmp[3] = {s->u, s->v, s->w};
+        for (int c = 0; c < 3; ++c) {
+            const Arr3& a = comp_arr(s, c);
+            const double* co = comp_origin(s, c);
+            par_for(s->nthreads, a.sz, [&](int64_t k0, int64_t k1) {
+                for (int k = (int)k0; k < (int)k1; ++k)
+                    for (int j = 0; j < a.sy; ++j)
+                        for (int i = 0; i < a.sx; ++i) {
+                            const double pos[3] = {co[0] + s->h[0] * i, co[1] + s->h[1] * j, co[2] + s->h[2] * k};
+                            if (!(sample_collider(s, pos) < 0)) continue;
+                            double vel[3], g[3];
+                            sample_vel(s, s->u, s->v, s->w, pos, vel);
+                            collider_gradient(s, pos, g);
+                            const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+                            double out = 0.0;
+                            if (len * len > 0.0) {
+                                const double n[3] = {g[0] / len, g[1] / len, g[2] / len};
+                                const double d = vel[0] * n[0] + vel[1] * n[1] + vel[2] * n[2];
+                                out = vel[c] - n[c] * d;
+                            }
+                            tmp[c](i, j, k) = out;
+                        }
+            });
+        }
+        s->u = tmp[0];
+        s->v = tmp[1];
+        s->w = tmp[2];
+    }
     // walls (:198-244)
     for (int k = 0; k < s->u.sz; ++k)
         for (int j = 0; j < s->u.sy; ++j) { s->u(0, j, k) = 0; s->u(s->u.sx - 1, j, k) = 0; }
@@ -644,6 +723,21 @@ void constrain_velocity(orc_scene* s, int depth) {
         for (int i = 0; i < s->v.sx; ++i) { s->v(i, 0, k) = 0; s->v(i, s->v.sy - 1, k) = 0; }
     for (int j = 0; j < s->w.sy; ++j)
         for (int i = 0; i < s->w.sx; ++i) { s->w(i, j, 0) = 0; s->w(i, j, s->w.sz - 1) = 0; }
+    if (s->has_collider) {  // (:246-290)
+        const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+        auto solid = [&](int i, int j, int k) { return s->csdf(i, j, k) < 0; };
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    if (!solid(i, j, k)) continue;
+                    if (i > 0 && !solid(i - 1, j, k)) s->u(i, j, k) = 0;
+                    if (i < nx - 1 && !solid(i + 1, j, k)) s->u(i + 1, j, k) = 0;
+                    if (j > 0 && !solid(i, j - 1, k)) s->v(i, j, k) = 0;
+                    if (j < ny - 1 && !solid(i, j + 1, k)) s->v(i, j + 1, k) = 0;
+                    if (k > 0 && !solid(i, j, k - 1)) s->w(i, j, k) = 0;
+                    if (k < nz - 1 && !solid(i, j, k + 1)) s->w(i, j, k + 1) = 0;
+                }
+    }
 }
 
 // ExternalForce::ApplyExternalForce with a DirectionalField (src/forces/external_force.cpp:4-30,
@@ -1446,6 +1540,7 @@ int orc_get_grid(orc_scene* s, int what, void* out) {
 int orc_set_grid(orc_scene* s, int what, const void* in) {
     if (Arr3* a = grid_field(s, what)) {
         std::memcpy(a->d.data(), in, sizeof(double) * a->size());
+        if (what == ORC_COLLIDER_SDF) s->has_collider = 1;  // ColliderCollection::BuildSdf output
         return 0;
     }
     if (Arr3i* m = grid_marker(s, what)) {
