@@ -32,7 +32,7 @@ EXPORTED = [
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
     "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats", "fe_batch_create", "fe_batch_destroy",
     "fe_batch_advance_frame", "fe_surface_build", "fe_surface_download", "fe_scene_clear_colliders",
-    "fe_scene_set_sdf_search",
+    "fe_scene_set_sdf_search", "fe_scene_add_mesh_collider", "fe_mesh_sdf",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -114,6 +114,10 @@ def load():
     L.fe_surface_build.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.fe_scene_clear_colliders.argtypes = [vp]
     L.fe_scene_set_sdf_search.argtypes = [vp, C.c_int32]
+    i32p = C.POINTER(C.c_int32)
+    dp = C.POINTER(C.c_double)
+    L.fe_scene_add_mesh_collider.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64]
+    L.fe_mesh_sdf.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64, dp]
     L.fe_surface_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.fe_batch_advance_frame.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_double)]
@@ -196,6 +200,33 @@ def seed_particles(desc, shapes, seed=0):
 
 def _fptr(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _mesh_args(size, origin, spacing, vertices, triangles):
+    """ctypes arguments (size, origin, spacing, vertices, nv, triangles, nt) of a
+    mesh on a node grid; the arrays are kept alive on the returned tuple."""
+    v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(triangles, dtype=np.int32).reshape(-1, 3))
+    sz = (C.c_int32 * 3)(*[int(q) for q in size])
+    o = (C.c_double * 3)(*[float(q) for q in origin])
+    h = (C.c_double * 3)(*[float(q) for q in spacing])
+    args = _MeshArgs((sz, o, h, v.ctypes.data_as(C.POINTER(C.c_double)), v.shape[0],
+                      t.ctypes.data_as(C.POINTER(C.c_int32)), t.shape[0]))
+    args.keep = (v, t)
+    return args
+
+
+class _MeshArgs(tuple):
+    pass
+
+
+def mesh_sdf(ctx, size, origin, spacing, vertices, triangles):
+    """Mesh2SDF::Build on the device: the SDF of a triangle mesh on a node grid
+    of size (nx, ny, nz), returned as (nz, ny, nx) float64."""
+    out = np.empty((int(size[2]), int(size[1]), int(size[0])), np.float64)
+    args = _mesh_args(size, origin, spacing, vertices, triangles)
+    ctx.check(ctx.L.fe_mesh_sdf(ctx._h, *args, out.ctypes.data_as(C.POINTER(C.c_double))), "fe_mesh_sdf")
+    return out
 
 
 class Context:
@@ -364,6 +395,11 @@ class Scene:
 
     def set_sdf_search(self, mode):
         self.ctx.check(self.L.fe_scene_set_sdf_search(self._h, mode), "fe_scene_set_sdf_search")
+
+    def add_mesh_collider(self, size, origin, spacing, vertices, triangles):
+        """AddCollider(TriangleMeshCollider(size, origin, spacing, mesh)); size is (nx, ny, nz)."""
+        args = _mesh_args(size, origin, spacing, vertices, triangles)
+        self.ctx.check(self.L.fe_scene_add_mesh_collider(self._h, *args), "fe_scene_add_mesh_collider")
 
     def clear_colliders(self):
         self.ctx.check(self.L.fe_scene_clear_colliders(self._h), "fe_scene_clear_colliders")

@@ -245,6 +245,58 @@ int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st)
     return FE_OK;
 }
 
+// Mesh2SDF of one triangle mesh on a node grid, into a new device buffer
+// *out (freed by the caller); synchronous on st.
+int mesh_sdf_device(fe_ctx* ctx, hipStream_t st, const int32_t size[3], const double origin[3],
+                    const double spacing[3], const double* vertices, int64_t nv, const int32_t* triangles,
+                    int64_t nt, NodeGrid3& G, double** out) {
+    *out = nullptr;
+    if (!size || !origin || !spacing || nv < 0 || nt < 0 || (nv > 0 && !vertices) || (nt > 0 && !triangles))
+        return set_err(ctx, FE_E_INVALID, "mesh collider: null pointer or negative count");
+    for (int a = 0; a < 3; ++a) {
+        if (size[a] < 1 || !(spacing[a] > 0.0) || !std::isfinite(spacing[a]) || !std::isfinite(origin[a]))
+            return set_err(ctx, FE_E_INVALID, "mesh collider: grid sizes must be >= 1, spacings finite and > 0");
+        G.n[a] = size[a];
+        G.o[a] = origin[a];
+        G.h[a] = spacing[a];
+    }
+    const int64_t nn = (int64_t)size[0] * size[1] * size[2];
+    std::vector<double> tv;
+    std::vector<int32_t> tb;
+    double fill = 0.0;
+    if (!mesh_prepare(G, vertices, nv, triangles, nt, tv, tb, fill))
+        return set_err(ctx, FE_E_INVALID,
+                       "mesh collider: a triangle's vertex index is out of range, or a coordinate / h.x does not "
+                       "fit an int");
+    double* tvd = nullptr;
+    int32_t* tbd = nullptr;
+    double* o = nullptr;
+    auto cleanup = [&]() {
+        if (tvd) (void)hipFree(tvd);
+        if (tbd) (void)hipFree(tbd);
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void**)&tvd, sizeof(double) * std::max<size_t>(tv.size(), 1))) != hipSuccess ||
+        (e = hipMalloc((void**)&tbd, sizeof(int32_t) * std::max<size_t>(tb.size(), 1))) != hipSuccess ||
+        (e = hipMalloc((void**)&o, sizeof(double) * nn)) != hipSuccess) {
+        cleanup();
+        if (o) (void)hipFree(o);
+        return hip_fail(ctx, e, "mesh collider allocation");
+    }
+    if ((nt > 0 && ((e = hipMemcpyAsync(tvd, tv.data(), sizeof(double) * tv.size(), hipMemcpyHostToDevice, st)) !=
+                        hipSuccess ||
+                    (e = hipMemcpyAsync(tbd, tb.data(), sizeof(int32_t) * tb.size(), hipMemcpyHostToDevice, st)) !=
+                        hipSuccess)) ||
+        (e = launch_mesh_sdf(st, G, tvd, tbd, nt, fill, o)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) {
+        cleanup();
+        (void)hipFree(o);
+        return hip_fail(ctx, e, "mesh collider SDF");
+    }
+    cleanup();
+    *out = o;
+    return FE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -747,6 +799,45 @@ int fe_last_solve_stats(const fe_scene* s, fe_solve_stats* st) {
     if (!s || !st) return FE_E_INVALID;
     *st = s->last_stats;
     return FE_OK;
+}
+
+int fe_scene_add_mesh_collider(fe_scene* s, const int32_t grid_size[3], const double grid_origin[3],
+                               const double grid_spacing[3], const double* vertices, int64_t vertex_count,
+                               const int32_t* triangles, int64_t triangle_count) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    NodeGrid3 G{};
+    double* c = nullptr;
+    int rc = mesh_sdf_device(ctx, s->stream, grid_size, grid_origin, grid_spacing, vertices, vertex_count, triangles,
+                             triangle_count, G, &c);
+    if (rc != FE_OK) return rc;
+    hipError_t e = collider_merge(s, G, c);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(c);
+    if (e != hipSuccess) return hip_fail(ctx, e, "collider_merge");
+    return FE_OK;
+}
+
+int fe_mesh_sdf(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origin[3], const double grid_spacing[3],
+                const double* vertices, int64_t vertex_count, const int32_t* triangles, int64_t triangle_count,
+                double* sdf) {
+    if (!ctx || !sdf) return FE_E_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = nullptr;
+    HIPCHK(ctx, hipStreamCreate(&st));
+    NodeGrid3 G{};
+    double* c = nullptr;
+    int rc = mesh_sdf_device(ctx, st, grid_size, grid_origin, grid_spacing, vertices, vertex_count, triangles,
+                             triangle_count, G, &c);
+    if (rc == FE_OK) {
+        const size_t bytes = sizeof(double) * (size_t)G.n[0] * G.n[1] * G.n[2];
+        hipError_t e = hipMemcpy(sdf, c, bytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "fe_mesh_sdf download");
+    }
+    if (c) (void)hipFree(c);
+    (void)hipStreamDestroy(st);
+    return rc;
 }
 
 int fe_scene_clear_colliders(fe_scene* s) {

@@ -186,6 +186,29 @@ __device__ inline double sample_collider64(const Geom& g, const double* __restri
                   ff[2]);
 }
 
+// A node grid of its own (a TriangleMeshCollider's ScalarGrid3D): size,
+// origin, spacing; values x fastest.
+struct NodeGrid3 {
+    int n[3];
+    double o[3];
+    double h[3];
+};
+
+// ScalarGrid3D::Sample on such a grid, in FP64.
+__device__ inline double sample_grid64(const NodeGrid3& G, const double* __restrict__ a, const double p[3]) {
+#pragma clang fp contract(off)
+    int ii[3];
+    double ff[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) bary64((p[q] - G.o[q]) / G.h[q], G.n[q], ii[q], ff[q]);
+    const int ip1 = min(ii[0] + 1, G.n[0] - 1), jp1 = min(ii[1] + 1, G.n[1] - 1), kp1 = min(ii[2] + 1, G.n[2] - 1);
+    auto S = [&](int x, int y, int z) { return a[(int64_t)x + (int64_t)G.n[0] * ((int64_t)y + (int64_t)G.n[1] * z)]; };
+    const int i = ii[0], j = ii[1], k = ii[2];
+    return lerp64(lerp64(lerp64(S(i, j, k), S(ip1, j, k), ff[0]), lerp64(S(i, jp1, k), S(ip1, jp1, k), ff[0]), ff[1]),
+                  lerp64(lerp64(S(i, j, kp1), S(ip1, j, kp1), ff[0]), lerp64(S(i, jp1, kp1), S(ip1, jp1, kp1), ff[0]), ff[1]),
+                  ff[2]);
+}
+
 // ScalarGrid3D::Gradient (src/3d/scalar_grid3d.cpp:67-90) of the collider SDF:
 // the trilinear weights at p times clamped central differences at the nodes.
 __device__ inline void collider_gradient64(const Geom& g, const double* __restrict__ c, const double p[3],

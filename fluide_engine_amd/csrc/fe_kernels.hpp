@@ -59,6 +59,15 @@ hipError_t launch_bc(fe_scene* s);
 hipError_t launch_collide(fe_scene* s);
 hipError_t launch_sdf_into_collider(fe_scene* s);  // ExtrapolateIntoCollider on the fluid SDF
 void collider_free(fe_scene* s);
+// Mesh2SDF on the device: mesh_prepare packs the triangles and their index
+// boxes on the host (false on a bad index or coordinate), launch_mesh_sdf
+// writes the collider grid's SDF, collider_merge folds it into the scene's
+// collider SDF (allocated at MAX_DISTANCE on first use).
+bool mesh_prepare(const NodeGrid3& G, const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                  std::vector<double>& tv, std::vector<int32_t>& tb, double& fill);
+hipError_t launch_mesh_sdf(hipStream_t st, const NodeGrid3& G, const double* tv_dev, const int32_t* tb_dev,
+                           int64_t nt, double fill, double* out);
+hipError_t collider_merge(fe_scene* s, const NodeGrid3& G, const double* c_dev);
 
 // k_diffusion.hip: BackwardEulerDiffusionSolver (without the trailing boundary condition)
 hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d);

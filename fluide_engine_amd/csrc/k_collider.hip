@@ -17,6 +17,8 @@
 //   (5) faces between a collider node and a free node are zeroed.
 // Without a collider (1)-(3) and (5) are identities and only (4) runs.
 #include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "fe_kernels.hpp"
 
@@ -212,9 +214,231 @@ __global__ __launch_bounds__(kBlock) void k_sdf_extrapolate(Geom g, double* __re
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mesh colliders: Mesh2SDF::Build (src/3d/mesh_2_sdf.cpp:14-95) of a
+// TriangleMeshCollider (src/3d/triangle_mesh_collider.cpp:5-9) on the
+// collider's own node grid, one thread per node looping over all triangles
+// (the loop index is uniform, so each triangle's vertices and index box are
+// scalar loads shared by the wave):
+//   * distance: the minimum of DistanceToTriangle over the triangles whose
+//     index box holds the node, starting from max(h) * (nx + ny + nz);
+//   * sign: negated where TriangleMesh::IsInside (src/3d/triangle_mesh.cpp:120-163)
+//     holds, i.e. the summed half solid angles reach 2 pi - 0.001 in magnitude.
+// FP64 in the reference's operation order with contraction off, so the
+// distances are the reference's bit for bit; the angle sum goes through the
+// device atan2, which only feeds the threshold.
+// ---------------------------------------------------------------------------
+struct D3 {
+    double x, y, z;
+};
+__device__ inline D3 dsub(const D3& a, const D3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline double ddot(const D3& a, const D3& b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ inline double dlen(const D3& a) {
+#pragma clang fp contract(off)
+    return sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+
+// Collisions::ClossestPointOnTriangle (src/3d/collisions.cpp:21-213): D. Eberly's
+// point-triangle distance (Geometric Tools, "Distance Between Point and
+// Triangle in 3D"): the minimiser (s, t) of |p0 + s e0 + t e1 - x|^2 over the
+// triangle, by the region of the (s, t) plane holding the free minimum.
+__device__ inline double triangle_distance(const D3& x, const D3& p0, const D3& p1, const D3& p2) {
+#pragma clang fp contract(off)
+    const D3 d = dsub(p0, x), e0 = dsub(p1, p0), e1 = dsub(p2, p0);
+    const double a00 = ddot(e0, e0), a01 = ddot(e0, e1), a11 = ddot(e1, e1);
+    const double b0 = ddot(d, e0), b1 = ddot(d, e1);
+    const double det = fmax(a00 * a11 - a01 * a01, 0.0);
+    double s = a01 * b1 - a11 * b0;
+    double t = a01 * b0 - a00 * b1;
+    // on an edge from the (s, t) origin: 0 for a non-negative slope b, 1 once
+    // -b reaches the curvature a, else -b / a
+    auto along = [](double b, double a) { return b >= 0 ? 0.0 : (-b >= a ? 1.0 : -b / a); };
+    if (s + t <= det) {
+        if (s < 0) {
+            if (t < 0 && b0 < 0) {  // region 4, towards the s edge
+                t = 0;
+                s = along(b0, a00);
+            } else {  // region 4 towards the t edge, and region 3
+                s = 0;
+                t = along(b1, a11);
+            }
+        } else if (t < 0) {  // region 5
+            t = 0;
+            s = along(b0, a00);
+        } else {  // region 0
+            s /= det;
+            t /= det;
+        }
+    } else {
+        const double denom = a00 - 2 * a01 + a11;  // curvature along the edge s + t = 1
+        if (s < 0 || t < 0) {  // regions 2 and 6: the roles of (s, b0, a00) and (t, b1, a11) swap
+            const bool r2 = s < 0;
+            const double bu = r2 ? b0 : b1, bw = r2 ? b1 : b0, aw = r2 ? a11 : a00;
+            const double tmp0 = a01 + bu, tmp1 = aw + bw;
+            double u, w;  // u: the parameter that is negative, w: the other
+            if (tmp1 > tmp0) {
+                const double numer = tmp1 - tmp0;
+                if (numer >= denom) {
+                    u = 1;
+                    w = 0;
+                } else {
+                    u = numer / denom;
+                    w = 1 - u;
+                }
+            } else {
+                u = 0;
+                w = tmp1 <= 0 ? 1.0 : (bw >= 0 ? 0.0 : -bw / aw);
+            }
+            s = r2 ? u : w;
+            t = r2 ? w : u;
+        } else {  // region 1
+            const double numer = a11 + b1 - a01 - b0;
+            if (numer <= 0) {
+                s = 0;
+                t = 1;
+            } else if (numer >= denom) {
+                s = 1;
+                t = 0;
+            } else {
+                s = numer / denom;
+                t = 1 - s;
+            }
+        }
+    }
+    const D3 c{p0.x + s * e0.x + t * e1.x, p0.y + s * e0.y + t * e1.y, p0.z + s * e0.z + t * e1.z};
+    return dlen(dsub(c, x));
+}
+
+// One triangle's term of TriangleMesh::IsInside: atan2(det[A B C],
+// |A||B||C| + |C| A.B + |A| B.C + |B| C.A), A, B, C = vertices - x, the
+// determinant in ADet's term order (:154-163).
+__device__ inline double half_solid_angle(const D3& x, const D3& p0, const D3& p1, const D3& p2) {
+#pragma clang fp contract(off)
+    const D3 A = dsub(p0, x), B = dsub(p1, x), C = dsub(p2, x);
+    double omega = A.x * B.y * C.z;
+    omega += B.x * C.y * A.z;
+    omega += C.x * A.y * B.z;
+    omega -= C.x * B.y * A.z;
+    omega -= B.x * A.y * C.z;
+    omega -= A.x * C.y * B.z;
+    const double nA = dlen(A), nB = dlen(B), nC = dlen(C);
+    double k = nA * nB * nC;
+    k += nC * ddot(A, B);
+    k += nA * ddot(B, C);
+    k += nB * ddot(C, A);
+    return atan2(omega, k);
+}
+
+constexpr double kPi = 3.14159265358979323846;  // src/common/math_utils.hpp:11
+
+// tv: 9 doubles per triangle (its three vertices); tb: 6 ints per triangle,
+// the index box i0, j0, k0, i1, j1, k1 (inclusive).
+__global__ __launch_bounds__(kBlock) void k_mesh_sdf(NodeGrid3 G, const double* __restrict__ tv,
+                                                     const int32_t* __restrict__ tb, int64_t nt, double fill,
+                                                     double* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nn = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    if (c >= nn) return;
+    int i, j, k;
+    split_index(c, G.n[0], G.n[1], i, j, k);
+    const D3 x{G.o[0] + (double)i * G.h[0], G.o[1] + (double)j * G.h[1], G.o[2] + (double)k * G.h[2]};
+    double d = fill, ret = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        const double* P = tv + 9 * t;
+        const int32_t* B = tb + 6 * t;
+        const D3 p0{P[0], P[1], P[2]}, p1{P[3], P[4], P[5]}, p2{P[6], P[7], P[8]};
+        if (i >= B[0] && j >= B[1] && k >= B[2] && i <= B[3] && j <= B[4] && k <= B[5]) {
+            const double dist = triangle_distance(x, p0, p1, p2);
+            if (dist < d) d = dist;
+        }
+        ret += half_solid_angle(x, p0, p1, p2);
+    }
+    if (fabs(ret) >= 2.0 * kPi - 0.001) d = d * -1.0;
+    out[c] = d;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f64(double* __restrict__ a, double v, int64_t n) {
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) a[c] = v;
+}
+
+// ColliderCollection::BuildSdf with one more collider: the running minimum of
+// the collider's SDF sampled at the scene's collider nodes origin + h*(i,j,k).
+__global__ __launch_bounds__(kBlock) void k_collider_merge(Geom g, NodeGrid3 G, const double* __restrict__ c,
+                                                           double* __restrict__ csdf) {
+#pragma clang fp contract(off)
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= cell_count(g)) return;
+    int idx[3];
+    split_index(q, g.n[0], g.n[1], idx[0], idx[1], idx[2]);
+    double p[3];
+    for (int a = 0; a < 3; ++a) p[a] = g.od[a] + (double)idx[a] * g.hd[a];  // GridIndexToPosition
+    const double d = sample_grid64(G, c, p);
+    if (d < csdf[q]) csdf[q] = d;
+}
+
 inline dim3 grid_n(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 }  // namespace
+
+// Host part of Mesh2SDF::ComputeExactBandDistanceField (:20-80): each
+// triangle's index box from its vertices divided by h.x on every axis, no
+// origin subtracted, widened by 1 below and 2 above, clamped to the grid.
+// Returns false when a quotient does not fit an int (the reference's int()
+// conversion is undefined there) or a vertex index is out of range.
+bool mesh_prepare(const NodeGrid3& G, const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                  std::vector<double>& tv, std::vector<int32_t>& tb, double& fill) {
+    const double hx = G.h[0], hy = G.h[1], hz = G.h[2];
+    const double hmax = (hx >= hy && hx >= hz) ? hx : ((hy >= hx && hy >= hz) ? hy : hz);  // Vector3::Max
+    fill = hmax * (double)((size_t)G.n[0] + (size_t)G.n[1] + (size_t)G.n[2]);
+    tv.assign((size_t)(9 * nt), 0.0);
+    tb.assign((size_t)(6 * nt), 0);
+    const double inv = 1.0 / hx;
+    auto clampi = [](int v, int hi, int lo) { return v < lo ? lo : (v > hi ? hi : v); };
+    for (int64_t t = 0; t < nt; ++t) {
+        double f[3][3];  // f[axis][vertex]
+        for (int q = 0; q < 3; ++q) {
+            const int32_t vi = tris[3 * t + q];
+            if (vi < 0 || vi >= nv) return false;
+            for (int a = 0; a < 3; ++a) {
+                tv[(size_t)(9 * t + 3 * q + a)] = verts[3 * (int64_t)vi + a];
+                f[a][q] = verts[3 * (int64_t)vi + a] * inv;
+                if (!(std::fabs(f[a][q]) < 2.0e9)) return false;
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            const double lo = std::fmin(f[a][0], std::fmin(f[a][1], f[a][2]));
+            const double hi = std::fmax(f[a][0], std::fmax(f[a][1], f[a][2]));
+            tb[(size_t)(6 * t + a)] = clampi(int(lo) - 1, G.n[a] - 1, 0);
+            tb[(size_t)(6 * t + 3 + a)] = clampi(int(hi) + 1 + 1, G.n[a] - 1, 0);
+        }
+    }
+    return true;
+}
+
+hipError_t launch_mesh_sdf(hipStream_t st, const NodeGrid3& G, const double* tv_dev, const int32_t* tb_dev,
+                           int64_t nt, double fill, double* out) {
+    const int64_t nn = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    hipLaunchKernelGGL(k_mesh_sdf, grid_n(nn), dim3(kBlock), 0, st, G, tv_dev, tb_dev, nt, fill, out);
+    return hipGetLastError();
+}
+
+hipError_t collider_merge(fe_scene* s, const NodeGrid3& G, const double* c_dev) {
+    const int64_t nc = cell_count(s->g);
+    hipError_t e;
+    if (!s->csdf) {  // ColliderCollection's SDF before any collider: MAX_DISTANCE (collider_collection.cpp:3)
+        if ((e = hipMalloc((void**)&s->csdf, sizeof(double) * nc)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_fill_f64, grid_n(nc), dim3(kBlock), 0, s->stream, s->csdf, 1.7976931348623157E+30, nc);
+    }
+    hipLaunchKernelGGL(k_collider_merge, grid_n(nc), dim3(kBlock), 0, s->stream, s->g, G, c_dev, s->csdf);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    s->has_collider = true;
+    return hipSuccess;
+}
 
 hipError_t launch_sdf_into_collider(fe_scene* s) {
     if (!s->has_collider) return hipSuccess;

@@ -181,6 +181,27 @@ class Unsupported : public std::logic_error {
 };
 
 // One fe_ctx per process and device, shared by the simulators on it.
+// TriangleMeshCollider (src/3d/triangle_mesh_collider.hpp:11-30): a closed
+// triangle mesh whose SDF is built by Mesh2SDF on the node grid (gridSize,
+// gridOrigin, gridSpacing).  The SDF is computed on the device when the
+// collider is added to a simulator (fe_scene_add_mesh_collider).
+class TriangleMeshCollider {
+  public:
+    TriangleMeshCollider(const Vector3<size_t>& gridSize, const Vector3<double>& gridOrigin,
+                         const Vector3<double>& gridSpacing, const TriangleMesh& mesh);
+    const Vector3<size_t>& GetGridSize() const { return _size; }
+    const Vector3<double>& GetGridOrigin() const { return _origin; }
+    const Vector3<double>& GetGridSpacing() const { return _spacing; }
+    const std::vector<double>& Vertices() const { return _vertices; }    // 3 per vertex
+    const std::vector<int32_t>& Triangles() const { return _triangles; }  // 3 vertex indices per triangle
+
+  private:
+    Vector3<size_t> _size;
+    Vector3<double> _origin, _spacing;
+    std::vector<double> _vertices;
+    std::vector<int32_t> _triangles;
+};
+
 class Device;
 
 class HipHybridSimulator {
@@ -224,8 +245,13 @@ class HipHybridSimulator {
     void SetDiffusionSolver(const DiffusionSolverSettings& settings);
     DiffusionSolverSettings GetDiffusionSolver() const { return _diffusion; }
 
+    // HybridSimulator::AddCollider (src/hybrid_simulator.hpp:35,
+    // src/pic_simulator.cpp:66-69): the scene's collider SDF becomes the
+    // minimum over its colliders (ColliderCollection::BuildSdf).
+    void AddCollider(std::shared_ptr<TriangleMeshCollider> collider);
+    size_t NumberOfColliders() const { return _colliders.size(); }
+
     // Not on this backend.
-    void AddCollider(const void*) { throw Unsupported("colliders are not on the GPU backend"); }
     void AddEmitter(const void*) { throw Unsupported("emitters are not on the GPU backend"); }
     void InitializeFromTriangleMesh(const void*) {
         throw Unsupported("triangle-mesh scenes are not on the GPU backend; use InitializeParticles");
@@ -255,6 +281,7 @@ class HipHybridSimulator {
     void EnsureScene(int64_t capacity) const;
     void PushParams() const;
     void PushDiffusionSolver() const;
+    void PushCollider(const TriangleMeshCollider& c) const;
     fe_scene_desc Desc(int64_t capacity) const;
     void Check(int rc, const char* what) const;
 
@@ -269,6 +296,7 @@ class HipHybridSimulator {
     Vector3<double> _gravity = 0;
     PressureSolverSettings _solver;
     DiffusionSolverSettings _diffusion;
+    std::vector<std::shared_ptr<TriangleMeshCollider>> _colliders;
     Frame _frame;
     double _time = 0.0;
     unsigned int _substeps = 0;

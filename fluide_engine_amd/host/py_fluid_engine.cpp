@@ -84,7 +84,8 @@ py::array get_grid(const HipHybridSimulator& sim, int field) {
     if (field >= FE_GRID_DMU && field <= FE_GRID_DMW) d[field - FE_GRID_DMU] += 1;
     const std::vector<py::ssize_t> shape = {(py::ssize_t)d[2], (py::ssize_t)d[1], (py::ssize_t)d[0]};
     py::array out;
-    if (field == FE_GRID_SDF || field == FE_GRID_PRESSURE) out = py::array_t<double>(shape);
+    if (field == FE_GRID_SDF || field == FE_GRID_PRESSURE || field == FE_GRID_COLLIDER_SDF)
+        out = py::array_t<double>(shape);
     else if ((field >= FE_GRID_MU && field <= FE_GRID_MW) || field == FE_GRID_FLAGS ||
              (field >= FE_GRID_DMU && field <= FE_GRID_DMW))
         out = py::array_t<uint8_t>(shape);
@@ -143,7 +144,7 @@ py::class_<Sim, std::shared_ptr<Sim>, Base> add_sim(py::module& m, const char* n
                  .def("InitializeFromTriangleMesh",
                       [](Sim& s, py::object) { s.InitializeFromTriangleMesh(nullptr); })
                  .def("AddExternalForce", &Sim::AddExternalForce)
-                 .def("AddCollider", [](Sim& s, py::object) { s.AddCollider(nullptr); })
+                 .def("AddCollider", &Sim::AddCollider)
                  .def("AddEmitter", [](Sim& s, py::object) { s.AddEmitter(nullptr); })
                  .def("SetCurrentFrame", &Sim::SetCurrentFrame)
                  .def("SetViscosity", &Sim::SetViscosity)
@@ -206,6 +207,9 @@ PYBIND11_MODULE(PyFluidEngine, m) {
             }
             return a;
         });
+    // src/python_wrapper/colliders_wrapper.cpp:14-17
+    py::class_<TriangleMeshCollider, std::shared_ptr<TriangleMeshCollider>>(m, "TriangleMeshCollider")
+        .def(py::init<const Vector3<size_t>&, const Vector3<double>&, const Vector3<double>&, const TriangleMesh&>());
     py::class_<OBJManager>(m, "OBJManager")
         .def(py::init<>())
         .def("Save", &OBJManager::Save)
@@ -287,4 +291,5 @@ PYBIND11_MODULE(PyFluidEngine, m) {
     m.attr("GRID_DMU") = (int)FE_GRID_DMU;
     m.attr("GRID_DMV") = (int)FE_GRID_DMV;
     m.attr("GRID_DMW") = (int)FE_GRID_DMW;
+    m.attr("GRID_COLLIDER_SDF") = (int)FE_GRID_COLLIDER_SDF;
 }

@@ -103,6 +103,45 @@ void HipHybridSimulator::EnsureScene(int64_t capacity) const {
     Check(fe_scene_create(ctx, &d, &_scene), "fe_scene_create");
     _capacity = d.max_particles;
     PushDiffusionSolver();
+    for (const auto& c : _colliders) PushCollider(*c);
+}
+
+void HipHybridSimulator::PushCollider(const TriangleMeshCollider& c) const {
+    const Vector3<size_t>& n = c.GetGridSize();
+    const int32_t size[3] = {(int32_t)n.x, (int32_t)n.y, (int32_t)n.z};
+    const double o[3] = {c.GetGridOrigin().x, c.GetGridOrigin().y, c.GetGridOrigin().z};
+    const double h[3] = {c.GetGridSpacing().x, c.GetGridSpacing().y, c.GetGridSpacing().z};
+    Check(fe_scene_add_mesh_collider(_scene, size, o, h, c.Vertices().data(), (int64_t)c.Vertices().size() / 3,
+                                     c.Triangles().data(), (int64_t)c.Triangles().size() / 3),
+          "fe_scene_add_mesh_collider");
+}
+
+void HipHybridSimulator::AddCollider(std::shared_ptr<TriangleMeshCollider> collider) {
+    if (!collider) throw std::invalid_argument("AddCollider: null collider");
+    if (_scene) PushCollider(*collider);  // a scene created later receives every collider
+    _colliders.push_back(std::move(collider));
+}
+
+TriangleMeshCollider::TriangleMeshCollider(const Vector3<size_t>& gridSize, const Vector3<double>& gridOrigin,
+                                           const Vector3<double>& gridSpacing, const TriangleMesh& mesh)
+    : _size(gridSize), _origin(gridOrigin), _spacing(gridSpacing) {
+    const auto& v = mesh.GetVerticies();
+    const auto& t = mesh.GetTriangles();
+    if (gridSize.x > INT32_MAX || gridSize.y > INT32_MAX || gridSize.z > INT32_MAX)
+        throw std::invalid_argument("TriangleMeshCollider: grid size beyond int32");
+    _vertices.reserve(3 * v.size());
+    for (const auto& p : v) {
+        _vertices.push_back(p.x);
+        _vertices.push_back(p.y);
+        _vertices.push_back(p.z);
+    }
+    _triangles.reserve(3 * t.size());
+    for (const auto& tr : t) {
+        for (size_t q : {tr.point1Idx, tr.point2Idx, tr.point3Idx}) {
+            if (q >= v.size()) throw std::invalid_argument("TriangleMeshCollider: vertex index out of range");
+            _triangles.push_back((int32_t)q);
+        }
+    }
 }
 
 void HipHybridSimulator::PushDiffusionSolver() const {

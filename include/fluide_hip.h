@@ -203,6 +203,25 @@ int fe_batch_advance_frame(fe_batch* batch, double frame_dt, const fe_solver_des
  * the inward normal velocity removed.  fe_scene_clear_colliders removes it. */
 int fe_scene_clear_colliders(fe_scene* scene);
 
+/* AddCollider(TriangleMeshCollider(gridSize, gridOrigin, gridSpacing, mesh))
+ * (src/pic_simulator.cpp:66-69, src/3d/triangle_mesh_collider.cpp:5-9): the
+ * collider's SDF is Mesh2SDF::Build (src/3d/mesh_2_sdf.cpp:14-95) of the mesh
+ * on its own node grid (grid_size nodes, x fastest; computed on the device),
+ * and the scene's collider SDF becomes the running minimum of the colliders'
+ * SDFs sampled at its nodes (ColliderCollection::BuildSdf,
+ * src/3d/collider_collection.cpp:33-50).  vertices: 3 doubles each;
+ * triangles: 3 vertex indices each (the reference's point1Idx..point3Idx).
+ * Synchronous.  FE_E_INVALID for an index out of range, a grid size < 1, a
+ * spacing <= 0, or a vertex coordinate / spacing[0] beyond the int range. */
+int fe_scene_add_mesh_collider(fe_scene* scene, const int32_t grid_size[3], const double grid_origin[3],
+                               const double grid_spacing[3], const double* vertices, int64_t vertex_count,
+                               const int32_t* triangles, int64_t triangle_count);
+/* Mesh2SDF::Build alone: the mesh's SDF on a node grid into sdf (host memory,
+ * grid_size[0]*grid_size[1]*grid_size[2] doubles, x fastest).  Synchronous. */
+int fe_mesh_sdf(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origin[3], const double grid_spacing[3],
+                const double* vertices, int64_t vertex_count, const int32_t* triangles, int64_t triangle_count,
+                double* sdf);
+
 /* Candidate search of the fluid SDF (BuildSignedDistanceField,
  * src/pic_simulator.cpp:452-475).  FE_SDF_SEARCH_CELLS (the default) scans the
  * cells within the particle radius r of each centre: every negative value is

@@ -1430,6 +1430,197 @@ int surface(orc_scene* s, double** vout, double** nout, int32_t** tout, int64_t*
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Mesh colliders: TriangleMeshCollider (src/3d/triangle_mesh_collider.cpp:5-9)
+// builds its SDF on its own grid with Mesh2SDF::Build (src/3d/mesh_2_sdf.cpp:14-95),
+// and ColliderCollection::BuildSdf (src/3d/collider_collection.cpp:33-50) takes,
+// at every node of the scene's collider grid, the minimum over colliders of
+// that SDF sampled there.
+// ---------------------------------------------------------------------------
+struct V3 {
+    double x, y, z;
+};
+inline V3 sub3(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double dot3(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }  // Vector3::Dot
+inline double len3(const V3& a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }  // GetLength
+
+// Collisions::ClossestPointOnTriangle (src/3d/collisions.cpp:21-213), which is
+// D. Eberly's point-triangle distance ("Distance Between Point and Triangle in
+// 3D", Geometric Tools): the closest point is p0 + s e0 + t e1 with (s, t)
+// minimising the quadratic a00 s^2 + 2 a01 s t + a11 t^2 + 2 b0 s + 2 b1 t over
+// the triangle; the unconstrained minimum's region of the (s, t) plane picks
+// the edge or vertex it clamps to.  Same coefficients and branch tests as the
+// reference, so the result is bit-identical.
+V3 closest_on_triangle(const V3& x, const V3& p0, const V3& p1, const V3& p2) {
+    const V3 d = sub3(p0, x), e0 = sub3(p1, p0), e1 = sub3(p2, p0);
+    const double a00 = dot3(e0, e0), a01 = dot3(e0, e1), a11 = dot3(e1, e1);
+    const double b0 = dot3(d, e0), b1 = dot3(d, e1);
+    const double det = std::max(a00 * a11 - a01 * a01, 0.0);
+    double s = a01 * b1 - a11 * b0;
+    double t = a01 * b0 - a00 * b1;
+    // minimiser along an edge that starts at the origin of (s, t): 0 when the
+    // slope b is non-negative, 1 when -b reaches the curvature a, else -b / a
+    auto along = [](double b, double a) { return b >= 0 ? 0.0 : (-b >= a ? 1.0 : -b / a); };
+    // the edge s + t = 1 seen from the side where parameter u is 0:
+    // tmp0 = a01 + bu, tmp1 = aw + bw with w the other parameter
+    const double denom_h = a00 - 2 * a01 + a11;
+    if (s + t <= det) {
+        if (s < 0) {
+            if (t < 0) {  // region 4
+                if (b0 < 0) {
+                    t = 0;
+                    s = (-b0 >= a00) ? 1.0 : -b0 / a00;
+                } else {
+                    s = 0;
+                    t = along(b1, a11);
+                }
+            } else {  // region 3
+                s = 0;
+                t = along(b1, a11);
+            }
+        } else if (t < 0) {  // region 5
+            t = 0;
+            s = along(b0, a00);
+        } else {  // region 0: interior
+            s /= det;
+            t /= det;
+        }
+    } else if (s < 0) {  // region 2
+        const double tmp0 = a01 + b0, tmp1 = a11 + b1;
+        if (tmp1 > tmp0) {
+            const double numer = tmp1 - tmp0;
+            if (numer >= denom_h) {
+                s = 1;
+                t = 0;
+            } else {
+                s = numer / denom_h;
+                t = 1 - s;
+            }
+        } else {
+            s = 0;
+            t = tmp1 <= 0 ? 1.0 : (b1 >= 0 ? 0.0 : -b1 / a11);
+        }
+    } else if (t < 0) {  // region 6
+        const double tmp0 = a01 + b1, tmp1 = a00 + b0;
+        if (tmp1 > tmp0) {
+            const double numer = tmp1 - tmp0;
+            if (numer >= denom_h) {
+                t = 1;
+                s = 0;
+            } else {
+                t = numer / denom_h;
+                s = 1 - t;
+            }
+        } else {
+            t = 0;
+            s = tmp1 <= 0 ? 1.0 : (b0 >= 0 ? 0.0 : -b0 / a00);
+        }
+    } else {  // region 1
+        const double numer = a11 + b1 - a01 - b0;
+        if (numer <= 0) {
+            s = 0;
+            t = 1;
+        } else if (numer >= denom_h) {
+            s = 1;
+            t = 0;
+        } else {
+            s = numer / denom_h;
+            t = 1 - s;
+        }
+    }
+    return {p0.x + s * e0.x + t * e1.x, p0.y + s * e0.y + t * e1.y, p0.z + s * e0.z + t * e1.z};
+}
+
+// Collisions::DistanceToTriangle (:215-218) = |closest - x|
+inline double distance_to_triangle(const V3& x, const V3& p0, const V3& p1, const V3& p2) {
+    return len3(sub3(closest_on_triangle(x, p0, p1, p2), x));
+}
+
+// TriangleMesh::IsInside (src/3d/triangle_mesh.cpp:120-152): the summed
+// half solid angles atan2(det[A B C], |A||B||C| + |C| A.B + |A| B.C + |B| C.A)
+// of the triangles seen from x (A, B, C = vertices - x; Van Oosterom and
+// Strackee) reach 2 pi - 0.001 in magnitude.  det in ADet's term order (:154-163).
+constexpr double kPi = 3.14159265358979323846;  // src/common/math_utils.hpp:11
+bool mesh_inside(const std::vector<V3>& verts, const std::vector<int32_t>& tris, const V3& x) {
+    double ret = 0;
+    const size_t nt = tris.size() / 3;
+    for (size_t t = 0; t < nt; ++t) {
+        const V3 A = sub3(verts[tris[3 * t]], x), B = sub3(verts[tris[3 * t + 1]], x),
+                 C = sub3(verts[tris[3 * t + 2]], x);
+        double omega = A.x * B.y * C.z;
+        omega += B.x * C.y * A.z;
+        omega += C.x * A.y * B.z;
+        omega -= C.x * B.y * A.z;
+        omega -= B.x * A.y * C.z;
+        omega -= A.x * C.y * B.z;
+        const double nA = len3(A), nB = len3(B), nC = len3(C);
+        double k = nA * nB * nC;
+        k += nC * dot3(A, B);  // (A * B).Sum() sums x, y, z in Dot's order
+        k += nA * dot3(B, C);
+        k += nB * dot3(C, A);
+        ret += std::atan2(omega, k);
+    }
+    return std::abs(ret) >= 2.0 * kPi - 0.001;
+}
+
+// Mesh2SDF::Build = ComputeExactBandDistanceField(bandwidth 1) + ComputeSigns.
+// Every node starts at max(h) * (nx + ny + nz); a triangle updates the nodes in
+// its index box, whose bounds divide the vertex coordinates by h.x on every
+// axis without subtracting the grid origin (:33-56, as the reference), widened
+// by 1 below and 2 above and clamped to the grid; nodes inside the mesh are
+// negated.
+Arr3 mesh_to_sdf(const int size[3], const double origin[3], const double h[3], const std::vector<V3>& verts,
+                 const std::vector<int32_t>& tris) {
+    Arr3 sdf;
+    const double hmax = (h[0] >= h[1] && h[0] >= h[2]) ? h[0] : ((h[1] >= h[0] && h[1] >= h[2]) ? h[1] : h[2]);
+    sdf.resize(size[0], size[1], size[2], hmax * (double)((size_t)size[0] + (size_t)size[1] + (size_t)size[2]));
+    auto node = [&](int i, int j, int k) {  // Grid3D::GridIndexToPosition
+        return V3{origin[0] + i * h[0], origin[1] + j * h[1], origin[2] + k * h[2]};
+    };
+    auto clampi = [](int v, int hi, int lo) { return v < lo ? lo : (v > hi ? hi : v); };
+    const size_t nt = tris.size() / 3;
+    if (!verts.empty()) {
+        const double inv = 1.0 / h[0];
+        for (size_t t = 0; t < nt; ++t) {
+            const V3 &p = verts[tris[3 * t]], &q = verts[tris[3 * t + 1]], &r = verts[tris[3 * t + 2]];
+            const double fi[3] = {p.x * inv, q.x * inv, r.x * inv}, fj[3] = {p.y * inv, q.y * inv, r.y * inv},
+                         fk[3] = {p.z * inv, q.z * inv, r.z * inv};
+            const int i0 = clampi(int(std::fmin(fi[0], std::fmin(fi[1], fi[2]))) - 1, size[0] - 1, 0);
+            const int j0 = clampi(int(std::fmin(fj[0], std::fmin(fj[1], fj[2]))) - 1, size[1] - 1, 0);
+            const int k0 = clampi(int(std::fmin(fk[0], std::fmin(fk[1], fk[2]))) - 1, size[2] - 1, 0);
+            const int i1 = clampi(int(std::fmax(fi[0], std::fmax(fi[1], fi[2]))) + 1 + 1, size[0] - 1, 0);
+            const int j1 = clampi(int(std::fmax(fj[0], std::fmax(fj[1], fj[2]))) + 1 + 1, size[1] - 1, 0);
+            const int k1 = clampi(int(std::fmax(fk[0], std::fmax(fk[1], fk[2]))) + 1 + 1, size[2] - 1, 0);
+            for (int k = k0; k <= k1; ++k)
+                for (int j = j0; j <= j1; ++j)
+                    for (int i = i0; i <= i1; ++i) {
+                        const double d = distance_to_triangle(node(i, j, k), p, q, r);
+                        if (d < sdf(i, j, k)) sdf(i, j, k) = d;
+                    }
+        }
+    }
+    for (int k = 0; k < size[2]; ++k)
+        for (int j = 0; j < size[1]; ++j)
+            for (int i = 0; i < size[0]; ++i)
+                if (mesh_inside(verts, tris, node(i, j, k))) sdf(i, j, k) = sdf(i, j, k) * -1.0;
+    return sdf;
+}
+
+// ColliderCollection::BuildSdf with one more collider: the running minimum of
+// the collider SDFs sampled at the scene's collider nodes origin + h*(i,j,k).
+void add_mesh_collider(orc_scene* s, const int size[3], const double origin[3], const double h[3],
+                       const std::vector<V3>& verts, const std::vector<int32_t>& tris) {
+    const Arr3 c = mesh_to_sdf(size, origin, h, verts, tris);
+    for (int k = 0; k < s->csdf.sz; ++k)
+        for (int j = 0; j < s->csdf.sy; ++j)
+            for (int i = 0; i < s->csdf.sx; ++i) {
+                const double p[3] = {s->origin[0] + i * s->h[0], s->origin[1] + j * s->h[1], s->origin[2] + k * s->h[2]};
+                const double d = sample_array(c, origin, h, p);
+                if (d < s->csdf(i, j, k)) s->csdf(i, j, k) = d;
+            }
+    s->has_collider = 1;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1571,6 +1762,45 @@ int orc_surface(orc_scene* s, double** vertices, double** normals, int32_t** tri
     return surface(s, vertices, normals, triangles, nv, nt);
 }
 void orc_free(void* p) { std::free(p); }
+
+static std::vector<V3> to_v3(const double* v, int64_t nv) {
+    std::vector<V3> out((size_t)nv);
+    for (int64_t i = 0; i < nv; ++i) out[(size_t)i] = V3{v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+    return out;
+}
+
+int orc_mesh_to_sdf(const int size[3], const double origin[3], const double h[3], const double* vertices, int64_t nv,
+                    const int32_t* triangles, int64_t nt, double* out) {
+    const std::vector<int32_t> tris(triangles, triangles + 3 * nt);
+    for (int32_t t : tris)
+        if (t < 0 || t >= nv) return -1;
+    const Arr3 a = mesh_to_sdf(size, origin, h, to_v3(vertices, nv), tris);
+    std::copy(a.d.begin(), a.d.end(), out);
+    return 0;
+}
+
+int orc_add_mesh_collider(orc_scene* s, const int size[3], const double origin[3], const double h[3],
+                          const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt) {
+    const std::vector<int32_t> tris(triangles, triangles + 3 * nt);
+    for (int32_t t : tris)
+        if (t < 0 || t >= nv) return -1;
+    add_mesh_collider(s, size, origin, h, to_v3(vertices, nv), tris);
+    return 0;
+}
+
+void orc_closest_on_triangle(const double x[3], const double p0[3], const double p1[3], const double p2[3],
+                             double out[3]) {
+    const V3 c = closest_on_triangle(V3{x[0], x[1], x[2]}, V3{p0[0], p0[1], p0[2]}, V3{p1[0], p1[1], p1[2]},
+                                     V3{p2[0], p2[1], p2[2]});
+    out[0] = c.x;
+    out[1] = c.y;
+    out[2] = c.z;
+}
+
+int orc_mesh_inside(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, const double x[3]) {
+    const std::vector<int32_t> tris(triangles, triangles + 3 * nt);
+    return mesh_inside(to_v3(vertices, nv), tris, V3{x[0], x[1], x[2]}) ? 1 : 0;
+}
 
 void orc_set_viscosity(orc_scene* s, double viscosity) { s->viscosity = viscosity; }
 void orc_set_diffusion_solver(orc_scene* s, const orc_solver_desc* d) { s->diff = *d; }

@@ -74,6 +74,17 @@ int orc_diffusion_markers(const orc_scene* s, int c, uint8_t* out);
 // normals (3 doubles each) and triangles (3 indices); free with orc_free.
 int orc_surface(orc_scene* s, double** vertices, double** normals, int32_t** triangles, int64_t* nv, int64_t* nt);
 void orc_free(void* p);
+// Mesh colliders: Mesh2SDF::Build of a triangle mesh on a node grid (x fastest;
+// triangles as 3 vertex indices), and AddCollider of such a TriangleMeshCollider
+// (the scene's collider SDF becomes the running minimum of the sampled collider
+// SDFs).  Both return -1 for a vertex index out of range.
+int orc_mesh_to_sdf(const int size[3], const double origin[3], const double h[3], const double* vertices, int64_t nv,
+                    const int32_t* triangles, int64_t nt, double* out);
+int orc_add_mesh_collider(orc_scene* s, const int size[3], const double origin[3], const double h[3],
+                          const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt);
+void orc_closest_on_triangle(const double x[3], const double p0[3], const double p1[3], const double p2[3],
+                             double out[3]);
+int orc_mesh_inside(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, const double x[3]);
 double orc_cfl(const orc_scene* s, double dt);
 unsigned orc_num_substeps(const orc_scene* s, double dt);
 int orc_advance_frame(orc_scene* s, double frame_dt, const orc_solver_desc* d);

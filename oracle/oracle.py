@@ -108,6 +108,11 @@ def lib():
         L.orc_surface.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double)),
                                   C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.orc_free.argtypes = [C.c_void_p]
+        i32p = C.POINTER(C.c_int32)
+        L.orc_mesh_to_sdf.argtypes = [ip, dp, dp, dp, C.c_int64, i32p, C.c_int64, dp]
+        L.orc_add_mesh_collider.argtypes = [C.c_void_p, ip, dp, dp, dp, C.c_int64, i32p, C.c_int64]
+        L.orc_closest_on_triangle.argtypes = [dp, dp, dp, dp, dp]
+        L.orc_mesh_inside.argtypes = [dp, C.c_int64, i32p, C.c_int64, dp]
         _lib = L
     return _lib
 
@@ -258,6 +263,18 @@ class Scene:
         lib().orc_diffusion_markers(self._s, c, out.ctypes.data_as(C.c_void_p))
         return out
 
+    def add_mesh_collider(self, size, origin, h, vertices, triangles):
+        """AddCollider(TriangleMeshCollider(size, origin, h, mesh)); size is (nx, ny, nz)."""
+        v = _f64(np.asarray(vertices).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(triangles).reshape(-1, 3), dtype=np.int32)
+        sz = (C.c_int * 3)(*[int(q) for q in size])
+        o = (C.c_double * 3)(*origin)
+        hh = (C.c_double * 3)(*h)
+        rc = lib().orc_add_mesh_collider(self._s, sz, o, hh, _dp(v), v.shape[0],
+                                         t.ctypes.data_as(C.POINTER(C.c_int32)), t.shape[0])
+        if rc != 0:
+            raise ValueError("triangle vertex index out of range")
+
     def surface(self):
         """MarchingCubesSolver::BuildSurface: (vertices (V,3), normals (V,3), triangles (T,3))."""
         L = lib()
@@ -362,3 +379,33 @@ def spmv_rows(Ac, Ar, Au, Af, v):
     out = np.empty_like(arrs[0])
     lib().orc_spmv_rows(*[_dp(a) for a in arrs], _ip(dims), _dp(out))
     return out
+
+
+def mesh_to_sdf(size, origin, h, vertices, triangles):
+    """Mesh2SDF::Build on a node grid of size (nx, ny, nz): array (nz, ny, nx)."""
+    v = _f64(np.asarray(vertices).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(triangles).reshape(-1, 3), dtype=np.int32)
+    out = np.empty((size[2], size[1], size[0]), np.float64)
+    sz = (C.c_int * 3)(*[int(q) for q in size])
+    rc = lib().orc_mesh_to_sdf(sz, (C.c_double * 3)(*origin), (C.c_double * 3)(*h), _dp(v), v.shape[0],
+                               t.ctypes.data_as(C.POINTER(C.c_int32)), t.shape[0], _dp(out))
+    if rc != 0:
+        raise ValueError("triangle vertex index out of range")
+    return out
+
+
+def closest_on_triangle(x, p0, p1, p2):
+    """Collisions::ClossestPointOnTriangle."""
+    out = np.empty(3)
+    args = [_f64(np.asarray(a, dtype=np.float64)) for a in (x, p0, p1, p2)]
+    lib().orc_closest_on_triangle(*[_dp(a) for a in args], _dp(out))
+    return out
+
+
+def mesh_inside(vertices, triangles, x):
+    """TriangleMesh::IsInside."""
+    v = _f64(np.asarray(vertices).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(triangles).reshape(-1, 3), dtype=np.int32)
+    xx = _f64(np.asarray(x, dtype=np.float64))
+    return bool(lib().orc_mesh_inside(_dp(v), v.shape[0], t.ctypes.data_as(C.POINTER(C.c_int32)), t.shape[0],
+                                      _dp(xx)))

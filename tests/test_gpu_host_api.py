@@ -143,3 +143,48 @@ def test_simulation_runner_writes_obj_frames(tmp_path):
     assert loaded.GetVerticies().shape == v.shape and loaded.GetTriangles().shape == t.shape
     assert np.abs(loaded.GetVerticies() - v).max() < 1e-6  # the OBJ holds 6 decimals
     assert np.array_equal(loaded.GetTriangles(), t)
+
+
+def test_python_mesh_collider_matches_capi_and_oracle(tmp_path):
+    """AddCollider(TriangleMeshCollider(...)) from an OBJ through PyFluidEngine:
+    the collider SDF is the oracle's bit for bit, and the frames are those of
+    the C ABI scene given the same mesh."""
+    print("[gpu] host API: mesh collider", flush=True)
+    from test_host_api import write_box_obj
+
+    P = _py_module()
+    n = 20
+    lo, hi = (0.55, 0.0, 0.25), (0.85, 0.35, 0.75)
+    write_box_obj(tmp_path / "block.obj", lo, hi)
+    mesh = P.TriangleMesh()
+    P.OBJManager().Load(str(tmp_path / "block.obj"), mesh)
+    h = 1.0 / n
+    sim = P.FLIPSimulator(P.Vector3S(n, n, n), P.BoundingBox3D(P.Vector3D(0, 0, 0), P.Vector3D(1, 1, 1)))
+    sim.AddExternalForce(P.DirectionalField(P.Vector3D(0, -9.81, 0)))
+    sim.SetPressureSolver("jacobi", 200, 1e-14)
+    sim.AddCollider(P.TriangleMeshCollider(P.Vector3S(n, n, n), P.Vector3D(0, 0, 0), P.Vector3D(h, h, h), mesh))
+    sim.InitializeParticles([P.Box([-1, -1, -1], [0.45, 0.6, 2.0])], 0)  # recreates the scene: collider re-added
+    sim.SetCurrentFrame(P.Frame(DT))
+    for _ in range(2):
+        sim.AdvanceSingleFrame()
+    px, pv, _ = sim.GetParticles()
+    csdf = sim.GetGrid(P.GRID_COLLIDER_SDF)
+    verts = mesh.GetVerticies()
+    tris = mesh.GetTriangles()[:, :3].astype(np.int32)
+    o = orc.Scene((n, n, n))
+    o.add_mesh_collider((n, n, n), (0.0, 0.0, 0.0), (h, h, h), verts, tris)
+    assert np.array_equal(csdf, o.get(orc.COLLIDER_SDF))
+    assert (sim.GetGrid(P.GRID_FLAGS) == capi.BOUNDRY).any()
+
+    desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.05, max_particles=px.shape[0])
+    x0 = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.45, 0.6, 2.0))], seed=0)
+    ctx = capi.Context(0)
+    sc = capi.Scene(ctx, desc)
+    sc.upload_particles(x0, np.zeros_like(x0))
+    sc.add_mesh_collider((n, n, n), (0.0, 0.0, 0.0), (h, h, h), verts, tris)
+    for _ in range(2):
+        sc.advance_frame(DT, capi.solver(capi.SOLVER_JACOBI, 200, 5, 1e-14))
+    cx, cv, _ = sc.download_particles()
+    assert np.array_equal(cx, px) and np.array_equal(cv, pv)
+    sc.close()
+    ctx.close()

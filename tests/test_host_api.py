@@ -74,10 +74,11 @@ def test_frame(P):
 
 def test_unsupported_paths(P):
     s = P.APICSimulator(P.Vector3S(8, 8, 8), box(P))
-    for call in (lambda: s.AddCollider(None), lambda: s.AddEmitter(None),
-                 lambda: s.InitializeFromTriangleMesh(None)):
+    for call in (lambda: s.AddEmitter(None), lambda: s.InitializeFromTriangleMesh(None)):
         with pytest.raises(NotImplementedError):
             call()
+    with pytest.raises((ValueError, TypeError)):
+        s.AddCollider(None)
     s.SetViscosity(0.1)  # BackwardEulerDiffusionSolver runs on the GPU from the next substep
     assert s.GetViscosity() == 0.1
     s.SetViscosity(0.0)
@@ -89,3 +90,34 @@ def test_unsupported_paths(P):
     with pytest.raises(ValueError):
         s.SetPressureSolver("gauss-seidel")
     s.SetPressureSolver("mgpcg", 2000, 1e-6, True)
+
+
+def write_box_obj(path, lo, hi, bad_index=False):
+    """A closed box in the reference's OBJ layout (v, vn, f a//n b//n c//n)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from meshes import box as box_mesh
+
+    v, t = box_mesh(lo, hi)
+    lines = ["o box"] + [f"v {x:.6f} {y:.6f} {z:.6f}" for x, y, z in v] + ["vn 0 0 1", "s off"]
+    for a, b, c in t:
+        c = c + 40 if bad_index else c
+        lines.append(f"f {a + 1}//1 {b + 1}//1 {c + 1}//1")
+    path.write_text("\n".join(lines) + "\n")
+
+
+def test_triangle_mesh_collider_from_obj(P, tmp_path):
+    """TriangleMeshCollider(gridSize, gridOrigin, gridSpacing, mesh)
+    (src/python_wrapper/colliders_wrapper.cpp:16-17) from an OBJ; a triangle
+    naming a missing vertex is rejected when the collider is made."""
+    write_box_obj(tmp_path / "box.obj", (0.3, 0.1, 0.3), (0.6, 0.4, 0.7))
+    mesh = P.TriangleMesh()
+    P.OBJManager().Load(str(tmp_path / "box.obj"), mesh)
+    assert mesh.GetVerticies().shape == (8, 3) and mesh.GetTriangles().shape == (12, 4)
+    c = P.TriangleMeshCollider(P.Vector3S(16, 16, 16), P.Vector3D(0, 0, 0), P.Vector3D(1 / 16, 1 / 16, 1 / 16), mesh)
+    assert c is not None
+    write_box_obj(tmp_path / "bad.obj", (0.3, 0.1, 0.3), (0.6, 0.4, 0.7), bad_index=True)
+    bad = P.TriangleMesh()
+    P.OBJManager().Load(str(tmp_path / "bad.obj"), bad)
+    with pytest.raises(ValueError):
+        P.TriangleMeshCollider(P.Vector3S(8, 8, 8), P.Vector3D(0, 0, 0), P.Vector3D(0.125, 0.125, 0.125), bad)
