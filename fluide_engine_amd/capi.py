@@ -20,9 +20,9 @@ SDF_SEARCH_CELLS, SDF_SEARCH_BUCKETS = 0, 1
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
  GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW, GRID_COLLIDER_SDF) = range(17)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
- STAGE_MOVE, STAGE_MARKERS, STAGE_DIFFUSION) = range(11)
+ STAGE_MOVE, STAGE_MARKERS, STAGE_DIFFUSION, STAGE_EMIT) = range(12)
 STAGE_NAMES = ["sort", "p2g", "sdf", "extrapolate", "bc", "gravity", "pressure", "g2p_move", "move", "markers",
-               "diffusion"]
+               "diffusion", "emit"]
 
 EXPORTED = [
     "fe_ctx_create", "fe_ctx_destroy", "fe_last_error", "fe_ctx_device_name", "fe_scene_create", "fe_scene_destroy",
@@ -32,7 +32,9 @@ EXPORTED = [
     "fe_kernel_timer_read", "fe_last_solve_stats", "fe_copy_probe", "fe_scene_set_params",
     "fe_scene_set_diffusion_solver", "fe_last_diffusion_stats", "fe_batch_create", "fe_batch_destroy",
     "fe_batch_advance_frame", "fe_surface_build", "fe_surface_download", "fe_scene_clear_colliders",
-    "fe_scene_set_sdf_search", "fe_scene_add_mesh_collider", "fe_mesh_sdf",
+    "fe_scene_set_sdf_search", "fe_scene_add_mesh_collider", "fe_mesh_sdf", "fe_scene_initialize_from_mesh",
+    "fe_scene_add_volume_emitter", "fe_scene_clear_emitters", "fe_scene_reserve_particles",
+    "fe_scene_particle_capacity", "fe_scene_add_volume_emitter_sdf",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -118,6 +120,13 @@ def load():
     dp = C.POINTER(C.c_double)
     L.fe_scene_add_mesh_collider.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64]
     L.fe_mesh_sdf.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64, dp]
+    L.fe_scene_initialize_from_mesh.argtypes = [vp, dp, C.c_int64, i32p, C.c_int64, C.c_int32, C.c_uint64]
+    L.fe_scene_add_volume_emitter.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64, C.c_int32, dp,
+                                              C.c_uint64]
+    L.fe_scene_clear_emitters.argtypes = [vp]
+    L.fe_scene_add_volume_emitter_sdf.argtypes = [vp, i32p, dp, dp, dp, C.c_int32, dp, C.c_uint64]
+    L.fe_scene_reserve_particles.argtypes = [vp, C.c_int64]
+    L.fe_scene_particle_capacity.argtypes = [vp, C.POINTER(C.c_int64)]
     L.fe_surface_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.fe_batch_advance_frame.argtypes = [vp, C.c_double, C.POINTER(SolverDesc), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_double)]
@@ -400,6 +409,31 @@ class Scene:
         """AddCollider(TriangleMeshCollider(size, origin, spacing, mesh)); size is (nx, ny, nz)."""
         args = _mesh_args(size, origin, spacing, vertices, triangles)
         self.ctx.check(self.L.fe_scene_add_mesh_collider(self._h, *args), "fe_scene_add_mesh_collider")
+
+    def initialize_from_mesh(self, vertices, triangles, particles_per_block, seed=0):
+        """InitializeFromTriangleMesh: Mesh2SDF into the fluid SDF, then particles appended in its cells."""
+        a = _mesh_args((1, 1, 1), (0, 0, 0), (1, 1, 1), vertices, triangles)
+        self.ctx.check(self.L.fe_scene_initialize_from_mesh(self._h, a[3], a[4], a[5], a[6], int(particles_per_block),
+                                                            int(seed)), "fe_scene_initialize_from_mesh")
+
+    def add_volume_emitter(self, resolution, domain_origin, domain_size, vertices, triangles, particles_per_block,
+                           velocity, seed=0):
+        """AddEmitter(VolumeEmitter(mesh, resolution, domain, particlesPerBlock, velocity, variance))."""
+        a = _mesh_args(resolution, domain_origin, domain_size, vertices, triangles)
+        vel = (C.c_double * 3)(*[float(q) for q in velocity])
+        self.ctx.check(self.L.fe_scene_add_volume_emitter(self._h, *a, int(particles_per_block), vel, int(seed)),
+                       "fe_scene_add_volume_emitter")
+
+    def reserve_particles(self, capacity):
+        self.ctx.check(self.L.fe_scene_reserve_particles(self._h, int(capacity)), "fe_scene_reserve_particles")
+
+    def capacity(self):
+        c = C.c_int64()
+        self.ctx.check(self.L.fe_scene_particle_capacity(self._h, C.byref(c)), "fe_scene_particle_capacity")
+        return c.value
+
+    def clear_emitters(self):
+        self.ctx.check(self.L.fe_scene_clear_emitters(self._h), "fe_scene_clear_emitters")
 
     def clear_colliders(self):
         self.ctx.check(self.L.fe_scene_clear_colliders(self._h), "fe_scene_clear_colliders")

@@ -194,8 +194,24 @@ int pressure(fe_scene* s, const fe_solver_desc* d, bool walls_after, fe_solve_st
     return FE_OK;
 }
 
+int emit(fe_scene* s) {
+    bool over = false;
+    hipError_t e = launch_emit(s, &over);
+    if (over)
+        return set_err(s->ctx, FE_E_INVALID, "emitted particles exceed 2^31 - 1 particles");
+    HIPCHK(s->ctx, e);
+    return FE_OK;
+}
+
 int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st) {
     fe_ctx* ctx = s->ctx;
+    // Emitter::Emitt (OnBeginAdvanceTimeStep, src/pic_simulator.cpp:158-163),
+    // before BuildCollider: the first substep sees a collider SDF of 0
+    if (!s->emitters.empty()) {
+        const int rc = emit(s);
+        if (rc != FE_OK) return rc;
+    }
+    ++s->substeps_run;
     StageClock clk(s);
     // particles binned by cell (keys come fused from the previous advection)
     clk.begin(FE_STAGE_SORT);
@@ -509,6 +525,7 @@ int fe_scene_destroy(fe_scene* s) {
     diffusion_free(s);
     surface_free(s);
     collider_free(s);
+    sources_free(s);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
@@ -784,6 +801,9 @@ int fe_run_stage(fe_scene* s, int32_t stage, double dt, const fe_solver_desc* d,
             HIPCHK(ctx, launch_g2p_move(s, dt, false, true, false));
             HIPCHK(ctx, launch_collide(s));
             break;
+        case FE_STAGE_EMIT:
+            if ((rc = emit(s)) != FE_OK) return rc;
+            break;
         default:
             return set_err(ctx, FE_E_INVALID, "unknown stage " + std::to_string(stage));
     }
@@ -838,6 +858,124 @@ int fe_mesh_sdf(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origi
     if (c) (void)hipFree(c);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+int fe_scene_initialize_from_mesh(fe_scene* s, const double* vertices, int64_t vertex_count,
+                                  const int32_t* triangles, int64_t triangle_count, int32_t particles_per_block,
+                                  uint64_t seed) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (particles_per_block < 0) return set_err(ctx, FE_E_INVALID, "particles_per_block must be >= 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Geom& g = s->g;
+    const int32_t size[3] = {g.n[0], g.n[1], g.n[2]};
+    // the fluid SDF grid: origin o + h/2 (src/hybrid_simulator.cpp:13-15)
+    const double so[3] = {g.od[0] + g.hd[0] / 2.0, g.od[1] + g.hd[1] / 2.0, g.od[2] + g.hd[2] / 2.0};
+    NodeGrid3 G{};
+    double* c = nullptr;
+    int rc = mesh_sdf_device(ctx, s->stream, size, so, g.hd, vertices, vertex_count, triangles, triangle_count, G, &c);
+    if (rc != FE_OK) return rc;
+    hipError_t e = hipMemcpyAsync(s->sdf, c, sizeof(double) * cell_count(g), hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(c);
+    if (e != hipSuccess) return hip_fail(ctx, e, "fe_scene_initialize_from_mesh");
+    bool over = false;
+    e = launch_initialize_from_sdf(s, particles_per_block, seed, &over);
+    if (over)
+        return set_err(ctx, FE_E_INVALID, "particles from the mesh exceed 2^31 - 1 particles");
+    HIPCHK(ctx, e);
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    return FE_OK;
+}
+
+int fe_scene_add_volume_emitter(fe_scene* s, const int32_t resolution[3], const double domain_origin[3],
+                                const double domain_size[3], const double* vertices, int64_t vertex_count,
+                                const int32_t* triangles, int64_t triangle_count, int32_t particles_per_block,
+                                const double velocity[3], uint64_t seed) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (!resolution || !domain_origin || !domain_size || !velocity)
+        return set_err(ctx, FE_E_INVALID, "volume emitter: null argument");
+    for (int a = 0; a < 3; ++a) {
+        if (resolution[a] != s->g.n[a])
+            return set_err(ctx, FE_E_INVALID,
+                           "volume emitter: its resolution must equal the scene's (Emitt reads the fluid and collider "
+                           "SDFs at the emitter's indices)");
+        if (!std::isfinite(velocity[a])) return set_err(ctx, FE_E_INVALID, "volume emitter: velocity must be finite");
+    }
+    if (particles_per_block < 0) return set_err(ctx, FE_E_INVALID, "particles_per_block must be >= 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // src/emmiter/volume_emmiter.cpp:15-19: spacing = domain size / resolution, origin = domain origin
+    const double h[3] = {domain_size[0] / resolution[0], domain_size[1] / resolution[1],
+                         domain_size[2] / resolution[2]};
+    Emitter em;
+    int rc = mesh_sdf_device(ctx, s->stream, resolution, domain_origin, h, vertices, vertex_count, triangles,
+                             triangle_count, em.grid, &em.sdf);
+    if (rc != FE_OK) return rc;
+    em.ppb = particles_per_block;
+    for (int a = 0; a < 3; ++a) em.vel[a] = velocity[a];
+    em.seed = seed;
+    s->emitters.push_back(em);
+    return FE_OK;
+}
+
+int fe_scene_reserve_particles(fe_scene* s, int64_t capacity) {
+    if (!s) return FE_E_INVALID;
+    if (capacity < 0 || capacity > INT32_MAX)
+        return set_err(s->ctx, FE_E_INVALID, "capacity must be in [0, 2^31 - 1]");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, reserve_particles(s, capacity));
+    return FE_OK;
+}
+
+int fe_scene_particle_capacity(const fe_scene* s, int64_t* capacity) {
+    if (!s || !capacity) return FE_E_INVALID;
+    *capacity = s->cap;
+    return FE_OK;
+}
+
+int fe_scene_add_volume_emitter_sdf(fe_scene* s, const int32_t resolution[3], const double domain_origin[3],
+                                    const double domain_size[3], const double* sdf, int32_t particles_per_block,
+                                    const double velocity[3], uint64_t seed) {
+    if (!s) return FE_E_INVALID;
+    fe_ctx* ctx = s->ctx;
+    if (!resolution || !domain_origin || !domain_size || !velocity || !sdf)
+        return set_err(ctx, FE_E_INVALID, "volume emitter: null argument");
+    for (int a = 0; a < 3; ++a) {
+        if (resolution[a] != s->g.n[a])
+            return set_err(ctx, FE_E_INVALID, "volume emitter: its resolution must equal the scene's");
+        if (!std::isfinite(velocity[a])) return set_err(ctx, FE_E_INVALID, "volume emitter: velocity must be finite");
+    }
+    if (particles_per_block < 0) return set_err(ctx, FE_E_INVALID, "particles_per_block must be >= 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    Emitter em;
+    for (int a = 0; a < 3; ++a) {
+        em.grid.n[a] = resolution[a];
+        em.grid.o[a] = domain_origin[a];
+        em.grid.h[a] = domain_size[a] / resolution[a];
+        em.vel[a] = velocity[a];
+    }
+    const size_t bytes = sizeof(double) * (size_t)cell_count(s->g);
+    HIPCHK(ctx, hipMalloc((void**)&em.sdf, bytes));
+    hipError_t e = hipMemcpy(em.sdf, sdf, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(em.sdf);
+        return hip_fail(ctx, e, "volume emitter SDF upload");
+    }
+    em.ppb = particles_per_block;
+    em.seed = seed;
+    s->emitters.push_back(em);
+    return FE_OK;
+}
+
+int fe_scene_clear_emitters(fe_scene* s) {
+    if (!s) return FE_E_INVALID;
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+    for (Emitter& em : s->emitters)
+        if (em.sdf) (void)hipFree(em.sdf);
+    s->emitters.clear();
+    return FE_OK;
 }
 
 int fe_scene_clear_colliders(fe_scene* s) {

@@ -369,6 +369,18 @@ struct McState {
 };
 }  // namespace fe
 
+namespace fe {
+// A VolumeEmitter: its Mesh2SDF on its node grid, particles per cell, start
+// velocity, generator seed and Emitt call count.
+struct Emitter {
+    double* sdf = nullptr;
+    NodeGrid3 grid{};
+    int ppb = 0;
+    double vel[3] = {0, 0, 0};
+    uint64_t seed = 0, calls = 0;
+};
+}  // namespace fe
+
 struct fe_ctx {
     int device = 0;
     std::string err;
@@ -446,6 +458,12 @@ struct fe_scene {
     bool has_collider = false;
     bool sdf_buckets = false;  // FE_SDF_SEARCH_BUCKETS
     float* bc_tmp[3] = {nullptr, nullptr, nullptr};
+
+    // scene sources: emitters, substeps run (the collider SDF is built from
+    // the first one on), and count / offset scratch of the seeding passes
+    std::vector<fe::Emitter> emitters;
+    int64_t substeps_run = 0;
+    int32_t *src_cnt = nullptr, *src_off = nullptr, *src_sums = nullptr;
 
     // per-kernel timer: HIP events around every launch of one selected kernel
     int32_t kt_id = 0;

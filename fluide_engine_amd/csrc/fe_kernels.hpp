@@ -69,6 +69,14 @@ hipError_t launch_mesh_sdf(hipStream_t st, const NodeGrid3& G, const double* tv_
                            int64_t nt, double fill, double* out);
 hipError_t collider_merge(fe_scene* s, const NodeGrid3& G, const double* c_dev);
 
+// k_sources.hip: Emitter::Emitt of every emitter and InitializeParticles from
+// the fluid SDF; *over is set (and an error returned) when the particles would
+// exceed the capacity
+hipError_t launch_emit(fe_scene* s, bool* over);
+hipError_t reserve_particles(fe_scene* s, int64_t capacity);
+hipError_t launch_initialize_from_sdf(fe_scene* s, int ppb, uint64_t seed, bool* over);
+void sources_free(fe_scene* s);
+
 // k_diffusion.hip: BackwardEulerDiffusionSolver (without the trailing boundary condition)
 hipError_t launch_diffusion(fe_scene* s, double dt, const fe_solver_desc* d);
 void diffusion_free(fe_scene* s);

@@ -10,10 +10,11 @@
 // (include/fluide_hip.h); host copies are made only by the explicit
 // particle/grid accessors.
 //
-// Not on this backend (each throws Unsupported): triangle-mesh scenes
-// (InitializeFromTriangleMesh), colliders, emitters and external forces other
-// than a uniform DirectionalField.  GetSurface extracts the surface on the GPU;
-// OBJManager and SimulationRunner complete the reference's frame loop.
+// Triangle-mesh scenes (InitializeFromTriangleMesh), TriangleMeshColliders and
+// VolumeEmitters run on the device; external forces other than a uniform
+// DirectionalField are not on this backend (Unsupported).  GetSurface extracts
+// the surface on the GPU; OBJManager and SimulationRunner complete the
+// reference's frame loop.
 #pragma once
 
 #include <cstddef>
@@ -202,6 +203,41 @@ class TriangleMeshCollider {
     std::vector<int32_t> _triangles;
 };
 
+// VolumeEmitter (src/emmiter/volume_emmiter.hpp:13-36): particles per block
+// with a start velocity in the nodes of the domain grid at `resolution` where
+// the emitter's SDF is <= 0 (Mesh2SDF of the mesh, or given directly by
+// InitializeFromSdf).  velocityVariance is kept but unused, as in the
+// reference.  The seed keys the particle generator (no reference counterpart).
+class VolumeEmitter {
+  public:
+    VolumeEmitter(const TriangleMesh& mesh, Vector3<size_t> resolution, BoundingBox3D domain,
+                  const size_t& particlesPerBlock, const Vector3<double>& velocity,
+                  const Vector3<double>& velocityVariance);
+    void InitializeFromTriangleMesh(const TriangleMesh& mesh);
+    void InitializeFromSdf(const ScalarGrid3D& sdf);
+    void SetSeed(uint64_t seed) { _seed = seed; }
+
+    // read by the simulator
+    const Vector3<size_t>& Resolution() const { return _resolution; }
+    const BoundingBox3D& Domain() const { return _domain; }
+    size_t ParticlesPerBlock() const { return _particlesPerBlock; }
+    const Vector3<double>& Velocity() const { return _velocity; }
+    uint64_t Seed() const { return _seed; }
+    bool HasSdf() const { return !_sdf.empty(); }
+    const std::vector<double>& Sdf() const { return _sdf; }  // x fastest
+    const std::vector<double>& Vertices() const { return _vertices; }
+    const std::vector<int32_t>& Triangles() const { return _triangles; }
+
+  private:
+    Vector3<size_t> _resolution;
+    BoundingBox3D _domain;
+    size_t _particlesPerBlock;
+    Vector3<double> _velocity, _velocityVariance;
+    uint64_t _seed = 0;
+    std::vector<double> _vertices, _sdf;
+    std::vector<int32_t> _triangles;
+};
+
 class Device;
 
 class HipHybridSimulator {
@@ -251,11 +287,14 @@ class HipHybridSimulator {
     void AddCollider(std::shared_ptr<TriangleMeshCollider> collider);
     size_t NumberOfColliders() const { return _colliders.size(); }
 
-    // Not on this backend.
-    void AddEmitter(const void*) { throw Unsupported("emitters are not on the GPU backend"); }
-    void InitializeFromTriangleMesh(const void*) {
-        throw Unsupported("triangle-mesh scenes are not on the GPU backend; use InitializeParticles");
-    }
+    // PICSimulator::AddEmitter (src/pic_simulator.cpp:46-49): emits at the
+    // start of every substep from the second on.
+    void AddEmitter(std::shared_ptr<VolumeEmitter> emitter);
+    size_t NumberOfEmitters() const { return _emitters.size(); }
+    // PICSimulator::InitializeFromTriangleMesh (src/pic_simulator.cpp:33-39):
+    // Mesh2SDF of the mesh into the fluid SDF, then GetParticlesPerBlock()
+    // particles in each cell whose corner samples it negative (appended).
+    void InitializeFromTriangleMesh(const TriangleMesh& mesh, uint64_t seed = 0);
     // HybridSimulator::GetSurface: MarchingCubesSolver::BuildSurface on the
     // current fluid SDF (extracted on the GPU), appended to mesh; each
     // triangle's normal index is its first vertex's (per-vertex normals).
@@ -282,6 +321,7 @@ class HipHybridSimulator {
     void PushParams() const;
     void PushDiffusionSolver() const;
     void PushCollider(const TriangleMeshCollider& c) const;
+    void PushEmitter(const VolumeEmitter& e) const;
     fe_scene_desc Desc(int64_t capacity) const;
     void Check(int rc, const char* what) const;
 
@@ -297,12 +337,12 @@ class HipHybridSimulator {
     PressureSolverSettings _solver;
     DiffusionSolverSettings _diffusion;
     std::vector<std::shared_ptr<TriangleMeshCollider>> _colliders;
+    std::vector<std::shared_ptr<VolumeEmitter>> _emitters;
     Frame _frame;
     double _time = 0.0;
     unsigned int _substeps = 0;
     std::shared_ptr<Device> _device;
     mutable fe_scene* _scene = nullptr;
-    mutable int64_t _capacity = 0;
     mutable ScalarGrid3D _sdf;
 };
 

@@ -141,11 +141,11 @@ py::class_<Sim, std::shared_ptr<Sim>, Base> add_sim(py::module& m, const char* n
     auto c = py::class_<Sim, std::shared_ptr<Sim>, Base>(m, name)
                  .def(py::init<Vector3<size_t>, BoundingBox3D>())
                  .def("AdvanceSingleFrame", &Sim::AdvanceSingleFrame)
-                 .def("InitializeFromTriangleMesh",
-                      [](Sim& s, py::object) { s.InitializeFromTriangleMesh(nullptr); })
+                 .def("InitializeFromTriangleMesh", &Sim::InitializeFromTriangleMesh, py::arg("mesh"),
+                      py::arg("seed") = 0)
                  .def("AddExternalForce", &Sim::AddExternalForce)
                  .def("AddCollider", &Sim::AddCollider)
-                 .def("AddEmitter", [](Sim& s, py::object) { s.AddEmitter(nullptr); })
+                 .def("AddEmitter", &Sim::AddEmitter)
                  .def("SetCurrentFrame", &Sim::SetCurrentFrame)
                  .def("SetViscosity", &Sim::SetViscosity)
                  .def("SetMaxClf", &Sim::SetMaxClf);
@@ -207,6 +207,21 @@ PYBIND11_MODULE(PyFluidEngine, m) {
             }
             return a;
         });
+    // src/python_wrapper/emitters_wrapper.cpp:15-23; InitializeFromSdf takes the
+    // emitter grid's values as a (nz, ny, nx) array
+    py::class_<VolumeEmitter, std::shared_ptr<VolumeEmitter>>(m, "VolumeEmitter")
+        .def(py::init<const TriangleMesh&, Vector3<size_t>, BoundingBox3D, const size_t&, const Vector3<double>&,
+                      const Vector3<double>&>())
+        .def("InitializeFromTriangleMesh", &VolumeEmitter::InitializeFromTriangleMesh)
+        .def("InitializeFromSdf",
+             [](VolumeEmitter& e, py::array_t<double, py::array::c_style | py::array::forcecast> a) {
+                 if (a.ndim() != 3) throw py::value_error("InitializeFromSdf: a (nz, ny, nx) array");
+                 ScalarGrid3D g;
+                 g.size = Vector3<size_t>((size_t)a.shape(2), (size_t)a.shape(1), (size_t)a.shape(0));
+                 g.data.assign(a.data(), a.data() + a.size());
+                 e.InitializeFromSdf(g);
+             })
+        .def("SetSeed", &VolumeEmitter::SetSeed);
     // src/python_wrapper/colliders_wrapper.cpp:14-17
     py::class_<TriangleMeshCollider, std::shared_ptr<TriangleMeshCollider>>(m, "TriangleMeshCollider")
         .def(py::init<const Vector3<size_t>&, const Vector3<double>&, const Vector3<double>&, const TriangleMesh&>());

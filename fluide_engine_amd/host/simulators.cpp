@@ -92,18 +92,94 @@ fe_scene_desc HipHybridSimulator::Desc(int64_t capacity) const {
     return d;
 }
 
+static void flatten_mesh(const TriangleMesh& mesh, std::vector<double>& v, std::vector<int32_t>& t,
+                         const char* who) {
+    v.clear();
+    t.clear();
+    for (const auto& p : mesh.GetVerticies()) {
+        v.push_back(p.x);
+        v.push_back(p.y);
+        v.push_back(p.z);
+    }
+    for (const auto& tr : mesh.GetTriangles())
+        for (size_t q : {tr.point1Idx, tr.point2Idx, tr.point3Idx}) {
+            if (q >= mesh.GetVerticies().size()) throw std::invalid_argument(std::string(who) + ": vertex index out of range");
+            t.push_back((int32_t)q);
+        }
+}
+
+// The scene is made on first use and then kept: more particles grow its
+// storage in place, so emitters, colliders and the fluid state stay.
 void HipHybridSimulator::EnsureScene(int64_t capacity) const {
-    if (_scene && _capacity >= capacity) return;
-    fe_ctx* ctx = _device->Context();
     if (_scene) {
-        fe_scene_destroy(_scene);
-        _scene = nullptr;
+        Check(fe_scene_reserve_particles(_scene, capacity), "fe_scene_reserve_particles");
+        return;
     }
     const fe_scene_desc d = Desc(std::max<int64_t>(capacity, 1));
-    Check(fe_scene_create(ctx, &d, &_scene), "fe_scene_create");
-    _capacity = d.max_particles;
+    Check(fe_scene_create(_device->Context(), &d, &_scene), "fe_scene_create");
     PushDiffusionSolver();
     for (const auto& c : _colliders) PushCollider(*c);
+    for (const auto& e : _emitters) PushEmitter(*e);
+}
+
+void HipHybridSimulator::PushEmitter(const VolumeEmitter& e) const {
+    const Vector3<size_t>& r = e.Resolution();
+    const int32_t res[3] = {(int32_t)r.x, (int32_t)r.y, (int32_t)r.z};
+    const Vector3<double> o = e.Domain().GetOrigin(), L = e.Domain().GetSize();
+    const double origin[3] = {o.x, o.y, o.z}, size[3] = {L.x, L.y, L.z};
+    const double vel[3] = {e.Velocity().x, e.Velocity().y, e.Velocity().z};
+    const int32_t ppb = (int32_t)e.ParticlesPerBlock();
+    if (e.HasSdf())
+        Check(fe_scene_add_volume_emitter_sdf(_scene, res, origin, size, e.Sdf().data(), ppb, vel, e.Seed()),
+              "fe_scene_add_volume_emitter_sdf");
+    else
+        Check(fe_scene_add_volume_emitter(_scene, res, origin, size, e.Vertices().data(),
+                                          (int64_t)e.Vertices().size() / 3, e.Triangles().data(),
+                                          (int64_t)e.Triangles().size() / 3, ppb, vel, e.Seed()),
+              "fe_scene_add_volume_emitter");
+}
+
+void HipHybridSimulator::AddEmitter(std::shared_ptr<VolumeEmitter> emitter) {
+    if (!emitter) throw std::invalid_argument("AddEmitter: null emitter");
+    if (_scene) PushEmitter(*emitter);
+    _emitters.push_back(std::move(emitter));
+}
+
+void HipHybridSimulator::InitializeFromTriangleMesh(const TriangleMesh& mesh, uint64_t seed) {
+    std::vector<double> v;
+    std::vector<int32_t> t;
+    flatten_mesh(mesh, v, t, "InitializeFromTriangleMesh");
+    EnsureScene(0);
+    Check(fe_scene_initialize_from_mesh(_scene, v.data(), (int64_t)v.size() / 3, t.data(), (int64_t)t.size() / 3,
+                                        (int32_t)_particlesPerBlock, seed),
+          "fe_scene_initialize_from_mesh");
+}
+
+VolumeEmitter::VolumeEmitter(const TriangleMesh& mesh, Vector3<size_t> resolution, BoundingBox3D domain,
+                             const size_t& particlesPerBlock, const Vector3<double>& velocity,
+                             const Vector3<double>& velocityVariance)
+    : _resolution(resolution), _domain(domain), _particlesPerBlock(particlesPerBlock), _velocity(velocity),
+      _velocityVariance(velocityVariance) {
+    if (resolution.x > INT32_MAX || resolution.y > INT32_MAX || resolution.z > INT32_MAX)
+        throw std::invalid_argument("VolumeEmitter: resolution beyond int32");
+    if (particlesPerBlock > INT32_MAX) throw std::invalid_argument("VolumeEmitter: particlesPerBlock beyond int32");
+    InitializeFromTriangleMesh(mesh);
+}
+
+void VolumeEmitter::InitializeFromTriangleMesh(const TriangleMesh& mesh) {
+    flatten_mesh(mesh, _vertices, _triangles, "VolumeEmitter");
+    _sdf.clear();
+}
+
+void VolumeEmitter::InitializeFromSdf(const ScalarGrid3D& sdf) {
+    // src/emmiter/volume_emmiter.cpp:50-56: the emitter grid's values copied index by index
+    const size_t nx = _resolution.x, ny = _resolution.y, nz = _resolution.z;
+    if (sdf.size.x < nx || sdf.size.y < ny || sdf.size.z < nz)
+        throw std::invalid_argument("VolumeEmitter::InitializeFromSdf: the grid is smaller than the emitter's");
+    _sdf.assign(nx * ny * nz, 0.0);
+    for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) _sdf[i + nx * (j + ny * k)] = sdf(i, j, k);
 }
 
 void HipHybridSimulator::PushCollider(const TriangleMeshCollider& c) const {
@@ -125,23 +201,9 @@ void HipHybridSimulator::AddCollider(std::shared_ptr<TriangleMeshCollider> colli
 TriangleMeshCollider::TriangleMeshCollider(const Vector3<size_t>& gridSize, const Vector3<double>& gridOrigin,
                                            const Vector3<double>& gridSpacing, const TriangleMesh& mesh)
     : _size(gridSize), _origin(gridOrigin), _spacing(gridSpacing) {
-    const auto& v = mesh.GetVerticies();
-    const auto& t = mesh.GetTriangles();
     if (gridSize.x > INT32_MAX || gridSize.y > INT32_MAX || gridSize.z > INT32_MAX)
         throw std::invalid_argument("TriangleMeshCollider: grid size beyond int32");
-    _vertices.reserve(3 * v.size());
-    for (const auto& p : v) {
-        _vertices.push_back(p.x);
-        _vertices.push_back(p.y);
-        _vertices.push_back(p.z);
-    }
-    _triangles.reserve(3 * t.size());
-    for (const auto& tr : t) {
-        for (size_t q : {tr.point1Idx, tr.point2Idx, tr.point3Idx}) {
-            if (q >= v.size()) throw std::invalid_argument("TriangleMeshCollider: vertex index out of range");
-            _triangles.push_back((int32_t)q);
-        }
-    }
+    flatten_mesh(mesh, _vertices, _triangles, "TriangleMeshCollider");
 }
 
 void HipHybridSimulator::PushDiffusionSolver() const {

@@ -128,8 +128,9 @@ enum fe_stage {
     FE_STAGE_G2P = 7,           /* TransferGrid2Particles */
     FE_STAGE_MOVE = 8,          /* MoveParticles */
     FE_STAGE_MARKERS = 9,       /* SinglePhasePressureSolver::BuildMarkers only */
-    FE_STAGE_DIFFUSION = 10     /* BackwardEulerDiffusionSolver::Solve (ComputeDiffusion
+    FE_STAGE_DIFFUSION = 10,    /* BackwardEulerDiffusionSolver::Solve (ComputeDiffusion
                                    without its boundary condition); solver NULL = the scene's */
+    FE_STAGE_EMIT = 11          /* Emitter::Emitt of every emitter (first step of a substep) */
 };
 
 typedef struct fe_ctx fe_ctx;
@@ -234,6 +235,48 @@ int fe_mesh_sdf(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origi
  * averages positive values). */
 enum fe_sdf_search { FE_SDF_SEARCH_CELLS = 0, FE_SDF_SEARCH_BUCKETS = 1 };
 int fe_scene_set_sdf_search(fe_scene* scene, int32_t mode);
+
+/* Scene sources (PICSimulator::InitializeFromTriangleMesh and AddEmitter,
+ * src/pic_simulator.cpp:33-49).  Both put particles_per_block particles into
+ * each picked cell at pos + h * u per axis, pos = the cell's corner, u
+ * uniform; u comes from a counter-based generator (SplitMix64) keyed by
+ * (seed, call, particle, axis) in place of the reference's std::rand(), so
+ * runs repeat.  Particles are appended in the reference's ForEachIndex order
+ * (i outer, k inner).  The particle storage grows past max_particles when they
+ * need it (the reference's particle system grows without bound).
+ *
+ * fe_scene_initialize_from_mesh: Mesh2SDF of the mesh into the fluid SDF
+ * (cell-centred grid), then InitializeParticles (src/pic_simulator.cpp:521-548):
+ * a cell is picked when that SDF sampled at its corner is negative; the new
+ * particles are at rest.
+ *
+ * fe_scene_add_volume_emitter: VolumeEmitter(mesh, resolution, domain,
+ * particlesPerBlock, velocity, variance) (src/emmiter/volume_emmiter.cpp:12-22;
+ * the variance is unused there too).  Its SDF is Mesh2SDF on the node grid of
+ * the domain at that resolution, which must equal the scene's (Emitt reads the
+ * fluid and collider SDFs at the emitter's indices).  At the start of every
+ * substep (src/pic_simulator.cpp:156-163) a node emits when emitterSdf <= 0,
+ * fluidSdf >= 0 and colliderSdf > 0 there, the new particles moving with
+ * velocity.  Emitt runs before BuildCollider, whose SDF starts at 0: nothing is
+ * emitted in a scene's first substep. */
+int fe_scene_initialize_from_mesh(fe_scene* scene, const double* vertices, int64_t vertex_count,
+                                  const int32_t* triangles, int64_t triangle_count, int32_t particles_per_block,
+                                  uint64_t seed);
+int fe_scene_add_volume_emitter(fe_scene* scene, const int32_t resolution[3], const double domain_origin[3],
+                                const double domain_size[3], const double* vertices, int64_t vertex_count,
+                                const int32_t* triangles, int64_t triangle_count, int32_t particles_per_block,
+                                const double velocity[3], uint64_t seed);
+/* VolumeEmitter::InitializeFromSdf (src/emmiter/volume_emmiter.cpp:50-56): the
+ * emitter's SDF given directly (resolution[0]*[1]*[2] doubles, x fastest). */
+int fe_scene_add_volume_emitter_sdf(fe_scene* scene, const int32_t resolution[3], const double domain_origin[3],
+                                    const double domain_size[3], const double* sdf, int32_t particles_per_block,
+                                    const double velocity[3], uint64_t seed);
+int fe_scene_clear_emitters(fe_scene* scene);
+/* Particle storage: grow it to at least capacity particles (the current ones
+ * are kept; no counterpart in the reference, whose particle system grows by
+ * itself), and read the current capacity. */
+int fe_scene_reserve_particles(fe_scene* scene, int64_t capacity);
+int fe_scene_particle_capacity(const fe_scene* scene, int64_t* capacity);
 
 /* Fluid surface: HybridSimulator::GetSurface -> MarchingCubesSolver::BuildSurface
  * (src/3d/marching_cubes_solver.cpp:25-51) on the current fluid SDF, iso value 0.

@@ -304,6 +304,18 @@ struct orc_scene {
     orc_solver_desc diff{ORC_SOLVER_JACOBI, 1000, 5, 0, 0, 0, 1e-14};
     std::vector<uint8_t> dmark[3];
     orc_solve_stats diff_last[3]{};
+    // VolumeEmitters (src/emmiter/volume_emmiter.cpp:12-85): the emitter's SDF
+    // on its node grid, particles per cell, start velocity, and the seed and
+    // call count of the generator that stands in for std::rand
+    struct Emitter {
+        Arr3 sdf;
+        double origin[3], h[3];
+        int ppb;
+        double vel[3];
+        uint64_t seed, calls;
+    };
+    std::vector<Emitter> emitters;
+    int64_t substeps_run = 0;  // the collider SDF exists once a substep has run (Q19 below)
 };
 
 namespace {
@@ -1606,6 +1618,84 @@ Arr3 mesh_to_sdf(const int size[3], const double origin[3], const double h[3], c
     return sdf;
 }
 
+// ---------------------------------------------------------------------------
+// Scene sources: PICSimulator::InitializeFromTriangleMesh + InitializeParticles
+// (src/pic_simulator.cpp:33-39,521-548) and VolumeEmitter::Emitt
+// (src/emmiter/volume_emmiter.cpp:58-85).  Both put particlesPerBlock particles
+// at pos + h * u (u uniform per axis) into each picked cell, pos the cell's
+// GridIndexToPosition.  The reference draws u from std::rand() / RAND_MAX on
+// racing threads, which no run reproduces; here u comes from SplitMix64
+// (G. Steele, D. Lea, C. Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014) keyed by (seed, call, particle, axis), in [0, 1),
+// and the particles are appended in ForEachIndex order (i outer, k inner).
+// ---------------------------------------------------------------------------
+inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform01(uint64_t seed, uint64_t call, uint64_t n, int axis) {
+    const uint64_t r = splitmix64(splitmix64(splitmix64(seed) ^ call) ^ (3 * n + (uint64_t)axis));
+    return (double)(r >> 11) * 0x1.0p-53;
+}
+
+template <class Pick>
+int64_t seed_cells(orc_scene* s, Pick pick, const double o[3], const double h[3], int ppb, uint64_t seed,
+                   uint64_t call, const double vel[3]) {
+    uint64_t n = 0;
+    for (int i = 0; i < s->n[0]; ++i)
+        for (int j = 0; j < s->n[1]; ++j)
+            for (int k = 0; k < s->n[2]; ++k) {
+                if (!pick(i, j, k)) continue;
+                const double pos[3] = {o[0] + i * h[0], o[1] + j * h[1], o[2] + k * h[2]};
+                for (int p = 0; p < ppb; ++p, ++n) {
+                    for (int a = 0; a < 3; ++a) s->x.push_back(h[a] * uniform01(seed, call, n, a) + pos[a]);
+                    for (int a = 0; a < 3; ++a) s->vel.push_back(vel[a]);
+                    for (int a = 0; a < 9; ++a) s->C.push_back(0.0);
+                }
+            }
+    s->np += (int64_t)n;
+    return (int64_t)n;
+}
+
+// VolumeEmitter::Emitt of every emitter, at the start of a substep
+// (OnBeginAdvanceTimeStep, src/pic_simulator.cpp:156-163): a node of the
+// emitter grid emits when emitterSdf <= 0, fluidSdf >= 0 and colliderSdf > 0
+// there, the last two read at the same indices.  Emitt runs before
+// BuildCollider, so in the first substep the collider SDF still holds its
+// initial 0 and nothing is emitted.
+void emit(orc_scene* s) {
+    for (auto& e : s->emitters) {
+        seed_cells(
+            s,
+            [&](int i, int j, int k) {
+                const double c = s->substeps_run == 0 ? 0.0 : s->csdf(i, j, k);
+                return e.sdf(i, j, k) <= 0 && s->sdf(i, j, k) >= 0 && c > 0;
+            },
+            e.origin, e.h, e.ppb, e.seed, e.calls, e.vel);
+        ++e.calls;
+    }
+}
+
+// InitializeFromTriangleMesh: Mesh2SDF of the mesh into the fluid SDF (cell
+// centred grid, origin o + h/2), then InitializeParticles: a cell is picked
+// when that SDF sampled at the cell's corner o + h*(i,j,k) is negative; the
+// new particles have zero velocity.
+void initialize_from_mesh(orc_scene* s, const std::vector<V3>& verts, const std::vector<int32_t>& tris, int ppb,
+                          uint64_t seed) {
+    const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
+    s->sdf = mesh_to_sdf(s->n, so, s->h, verts, tris);
+    const double zero[3] = {0.0, 0.0, 0.0};
+    seed_cells(
+        s,
+        [&](int i, int j, int k) {
+            const double p[3] = {s->origin[0] + i * s->h[0], s->origin[1] + j * s->h[1], s->origin[2] + k * s->h[2]};
+            return sample_array(s->sdf, so, s->h, p) < 0;
+        },
+        s->origin, s->h, ppb, seed, 0, zero);
+}
+
 // ColliderCollection::BuildSdf with one more collider: the running minimum of
 // the collider SDFs sampled at the scene's collider nodes origin + h*(i,j,k).
 void add_mesh_collider(orc_scene* s, const int size[3], const double origin[3], const double h[3],
@@ -1797,6 +1887,41 @@ void orc_closest_on_triangle(const double x[3], const double p0[3], const double
     out[2] = c.z;
 }
 
+int orc_add_volume_emitter(orc_scene* s, const int size[3], const double origin[3], const double h[3],
+                           const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, int ppb,
+                           const double velocity[3], uint64_t seed) {
+    const std::vector<int32_t> tris(triangles, triangles + 3 * nt);
+    for (int32_t t : tris)
+        if (t < 0 || t >= nv) return -1;
+    for (int a = 0; a < 3; ++a)
+        if (size[a] != s->n[a]) return -2;
+    orc_scene::Emitter e;
+    e.sdf = mesh_to_sdf(size, origin, h, to_v3(vertices, nv), tris);
+    for (int a = 0; a < 3; ++a) {
+        e.origin[a] = origin[a];
+        e.h[a] = h[a];
+        e.vel[a] = velocity[a];
+    }
+    e.ppb = ppb;
+    e.seed = seed;
+    e.calls = 0;
+    s->emitters.push_back(std::move(e));
+    return 0;
+}
+
+int orc_initialize_from_mesh(orc_scene* s, const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt,
+                             int ppb, uint64_t seed) {
+    const std::vector<int32_t> tris(triangles, triangles + 3 * nt);
+    for (int32_t t : tris)
+        if (t < 0 || t >= nv) return -1;
+    initialize_from_mesh(s, to_v3(vertices, nv), tris, ppb, seed);
+    return 0;
+}
+
+double orc_uniform01(uint64_t seed, uint64_t call, uint64_t n, int axis) { return uniform01(seed, call, n, axis); }
+
+void orc_emit(orc_scene* s) { emit(s); }
+
 int orc_mesh_inside(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, const double x[3]) {
     const std::vector<int32_t> tris(triangles, triangles + 3 * nt);
     return mesh_inside(to_v3(vertices, nv), tris, V3{x[0], x[1], x[2]}) ? 1 : 0;
@@ -1819,6 +1944,8 @@ int orc_diffusion_markers(const orc_scene* s, int c, uint8_t* out) {
 // OnBeginAdvanceTimeStep (:156-196), ComputeExternalForces (:203-212), ComputeDiffusion
 // (:214-222, skipped at viscosity 0), ComputePressure (:224-229), ComputeAdvection (:231-237).
 void orc_substep(orc_scene* s, double dt, const orc_solver_desc* d, orc_solve_stats* st) {
+    emit(s);
+    ++s->substeps_run;
     p2g(s);
     build_sdf(s);
     extrapolate_to_air(s);

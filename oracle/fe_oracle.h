@@ -85,6 +85,17 @@ int orc_add_mesh_collider(orc_scene* s, const int size[3], const double origin[3
 void orc_closest_on_triangle(const double x[3], const double p0[3], const double p1[3], const double p2[3],
                              double out[3]);
 int orc_mesh_inside(const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, const double x[3]);
+// Scene sources.  A VolumeEmitter (its grid must have the scene's size; -2
+// otherwise) emits at the start of every orc_substep; InitializeFromTriangleMesh
+// writes the mesh SDF into the fluid SDF and appends particles.  The generator
+// standing in for std::rand is orc_uniform01.
+int orc_add_volume_emitter(orc_scene* s, const int size[3], const double origin[3], const double h[3],
+                           const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt, int ppb,
+                           const double velocity[3], uint64_t seed);
+int orc_initialize_from_mesh(orc_scene* s, const double* vertices, int64_t nv, const int32_t* triangles, int64_t nt,
+                             int ppb, uint64_t seed);
+double orc_uniform01(uint64_t seed, uint64_t call, uint64_t n, int axis);
+void orc_emit(orc_scene* s);  // the emitters' Emitt alone (the first step of orc_substep)
 double orc_cfl(const orc_scene* s, double dt);
 unsigned orc_num_substeps(const orc_scene* s, double dt);
 int orc_advance_frame(orc_scene* s, double frame_dt, const orc_solver_desc* d);

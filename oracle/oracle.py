@@ -113,6 +113,12 @@ def lib():
         L.orc_add_mesh_collider.argtypes = [C.c_void_p, ip, dp, dp, dp, C.c_int64, i32p, C.c_int64]
         L.orc_closest_on_triangle.argtypes = [dp, dp, dp, dp, dp]
         L.orc_mesh_inside.argtypes = [dp, C.c_int64, i32p, C.c_int64, dp]
+        L.orc_add_volume_emitter.argtypes = [C.c_void_p, ip, dp, dp, dp, C.c_int64, i32p, C.c_int64, C.c_int, dp,
+                                             C.c_uint64]
+        L.orc_initialize_from_mesh.argtypes = [C.c_void_p, dp, C.c_int64, i32p, C.c_int64, C.c_int, C.c_uint64]
+        L.orc_uniform01.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]
+        L.orc_uniform01.restype = C.c_double
+        L.orc_emit.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -275,6 +281,30 @@ class Scene:
         if rc != 0:
             raise ValueError("triangle vertex index out of range")
 
+    def add_volume_emitter(self, size, origin, h, vertices, triangles, particles_per_block, velocity, seed):
+        """AddEmitter(VolumeEmitter(mesh, resolution, domain, particlesPerBlock, velocity, 0)); the emitter
+        grid (size, origin, h) must have the scene's size."""
+        v = _f64(np.asarray(vertices).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(triangles).reshape(-1, 3), dtype=np.int32)
+        rc = lib().orc_add_volume_emitter(self._s, (C.c_int * 3)(*[int(q) for q in size]), (C.c_double * 3)(*origin),
+                                          (C.c_double * 3)(*h), _dp(v), v.shape[0],
+                                          t.ctypes.data_as(C.POINTER(C.c_int32)), t.shape[0],
+                                          int(particles_per_block), (C.c_double * 3)(*velocity), int(seed))
+        if rc != 0:
+            raise ValueError("bad emitter (vertex index out of range or grid size differs from the scene's)")
+
+    def emit(self):
+        lib().orc_emit(self._s)
+
+    def initialize_from_mesh(self, vertices, triangles, particles_per_block, seed):
+        """InitializeFromTriangleMesh: Mesh2SDF into the fluid SDF, then InitializeParticles (appends)."""
+        v = _f64(np.asarray(vertices).reshape(-1, 3))
+        t = np.ascontiguousarray(np.asarray(triangles).reshape(-1, 3), dtype=np.int32)
+        rc = lib().orc_initialize_from_mesh(self._s, _dp(v), v.shape[0], t.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            t.shape[0], int(particles_per_block), int(seed))
+        if rc != 0:
+            raise ValueError("triangle vertex index out of range")
+
     def surface(self):
         """MarchingCubesSolver::BuildSurface: (vertices (V,3), normals (V,3), triangles (T,3))."""
         L = lib()
@@ -409,3 +439,8 @@ def mesh_inside(vertices, triangles, x):
     xx = _f64(np.asarray(x, dtype=np.float64))
     return bool(lib().orc_mesh_inside(_dp(v), v.shape[0], t.ctypes.data_as(C.POINTER(C.c_int32)), t.shape[0],
                                       _dp(xx)))
+
+
+def uniform01(seed, call, n, axis):
+    """The generator standing in for std::rand() / RAND_MAX in particle seeding."""
+    return lib().orc_uniform01(int(seed), int(call), int(n), int(axis))

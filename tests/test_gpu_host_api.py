@@ -188,3 +188,49 @@ def test_python_mesh_collider_matches_capi_and_oracle(tmp_path):
     assert np.array_equal(cx, px) and np.array_equal(cv, pv)
     sc.close()
     ctx.close()
+
+
+def test_python_mesh_scene_and_emitter_match_capi(tmp_path):
+    """InitializeFromTriangleMesh and AddEmitter(VolumeEmitter(...)) through
+    PyFluidEngine give the particles of the C ABI scene built the same way."""
+    print("[gpu] host API: mesh scene and emitter", flush=True)
+    from test_host_api import write_box_obj
+
+    P = _py_module()
+    n = 20
+    write_box_obj(tmp_path / "water.obj", (0.05, 0.05, 0.1), (0.45, 0.5, 0.9))
+    write_box_obj(tmp_path / "source.obj", (0.6, 0.6, 0.35), (0.8, 0.8, 0.65))
+    water, source = P.TriangleMesh(), P.TriangleMesh()
+    P.OBJManager().Load(str(tmp_path / "water.obj"), water)
+    P.OBJManager().Load(str(tmp_path / "source.obj"), source)
+    dom = P.BoundingBox3D(P.Vector3D(0, 0, 0), P.Vector3D(1, 1, 1))
+    sim = P.FLIPSimulator(P.Vector3S(n, n, n), dom)
+    sim.AddExternalForce(P.DirectionalField(P.Vector3D(0, -9.81, 0)))
+    sim.SetPressureSolver("jacobi", 200, 1e-14)
+    sim.SetParticlesPerBlock(8)
+    em = P.VolumeEmitter(source, P.Vector3S(n, n, n), dom, 4, P.Vector3D(0, -1.0, 0), P.Vector3D(0, 0, 0))
+    em.SetSeed(7)
+    sim.AddEmitter(em)
+    sim.InitializeFromTriangleMesh(water, 3)
+    n0 = sim.GetNumberOfParticles()
+    sim.SetCurrentFrame(P.Frame(DT))
+    for _ in range(2):
+        sim.AdvanceSingleFrame()
+    px, pv, _ = sim.GetParticles()
+    assert px.shape[0] > n0 > 1000  # the emitter added particles after the first substep
+
+    def tri(mesh):
+        return mesh.GetVerticies(), mesh.GetTriangles()[:, :3].astype(np.int32)
+
+    desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.05, max_particles=1)
+    ctx = capi.Context(0)
+    sc = capi.Scene(ctx, desc)
+    sc.add_volume_emitter((n, n, n), (0, 0, 0), (1, 1, 1), *tri(source), 4, (0.0, -1.0, 0.0), seed=7)
+    sc.initialize_from_mesh(*tri(water), 8, seed=3)
+    assert sc.count() == n0
+    for _ in range(2):
+        sc.advance_frame(DT, capi.solver(capi.SOLVER_JACOBI, 200, 5, 1e-14))
+    cx, cv, _ = sc.download_particles()
+    assert np.array_equal(cx, px) and np.array_equal(cv, pv)
+    sc.close()
+    ctx.close()

@@ -74,11 +74,10 @@ def test_frame(P):
 
 def test_unsupported_paths(P):
     s = P.APICSimulator(P.Vector3S(8, 8, 8), box(P))
-    for call in (lambda: s.AddEmitter(None), lambda: s.InitializeFromTriangleMesh(None)):
-        with pytest.raises(NotImplementedError):
+    for call in (lambda: s.AddCollider(None), lambda: s.AddEmitter(None),
+                 lambda: s.InitializeFromTriangleMesh(None)):
+        with pytest.raises((ValueError, TypeError)):
             call()
-    with pytest.raises((ValueError, TypeError)):
-        s.AddCollider(None)
     s.SetViscosity(0.1)  # BackwardEulerDiffusionSolver runs on the GPU from the next substep
     assert s.GetViscosity() == 0.1
     s.SetViscosity(0.0)
@@ -121,3 +120,14 @@ def test_triangle_mesh_collider_from_obj(P, tmp_path):
     P.OBJManager().Load(str(tmp_path / "bad.obj"), bad)
     with pytest.raises(ValueError):
         P.TriangleMeshCollider(P.Vector3S(8, 8, 8), P.Vector3D(0, 0, 0), P.Vector3D(0.125, 0.125, 0.125), bad)
+    # VolumeEmitter(mesh, resolution, domain, particlesPerBlock, velocity, variance)
+    # (src/python_wrapper/emitters_wrapper.cpp:19-22)
+    dom = P.BoundingBox3D(P.Vector3D(0, 0, 0), P.Vector3D(1, 1, 1))
+    e = P.VolumeEmitter(mesh, P.Vector3S(16, 16, 16), dom, 12, P.Vector3D(0, 6, 0), P.Vector3D(0, 0, 0))
+    e.InitializeFromTriangleMesh(mesh)
+    import numpy as np
+    e.InitializeFromSdf(np.ones((16, 16, 16)))
+    with pytest.raises(ValueError):
+        e.InitializeFromSdf(np.ones((8, 16, 16)))
+    with pytest.raises(ValueError):
+        P.VolumeEmitter(bad, P.Vector3S(8, 8, 8), dom, 4, P.Vector3D(0, 0, 0), P.Vector3D(0, 0, 0))
