@@ -122,7 +122,7 @@ enum fe_stage {
     FE_STAGE_P2G = 1,           /* TransferParticles2Grid */
     FE_STAGE_SDF = 2,           /* BuildSignedDistanceField (+ cell markers) */
     FE_STAGE_EXTRAPOLATE = 3,   /* ExtrapolateVelocityToAir */
-    FE_STAGE_BC = 4,            /* ApplyBoundryCondition (walls) */
+    FE_STAGE_BC = 4,            /* ApplyBoundryCondition (walls; the collider branches with a collider) */
     FE_STAGE_GRAVITY = 5,       /* ExternalForce::ApplyExternalForce */
     FE_STAGE_PRESSURE = 6,      /* SinglePhasePressureSolver::Solve */
     FE_STAGE_G2P = 7,           /* TransferGrid2Particles */
