@@ -98,6 +98,8 @@ def parse():
                     help="frames of the 96-scene batch workload after the headline run (0 skips it)")
     ap.add_argument("--batch-workers", type=int, nargs="+", default=[4],
                     help="host worker threads of the batch (several values: one measurement each)")
+    ap.add_argument("--learned-steps", type=int, default=1500,
+                    help="training steps of the learned warm start (0 skips that workload)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--summarize-trace", metavar="STATS_CSV",
                     help="write profiles/flip128_kernel_trace.md from a rocprofv3 --stats kernel summary and exit")
@@ -114,10 +116,10 @@ KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
-             "--batch-frames 0")
+             "--batch-frames 0 --learned-steps 0")
 PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{counter} -o pmc -- "
            "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --no-solver-comparison "
-           "--batch-frames 0 --copy-calibration 256")
+           "--batch-frames 0 --learned-steps 0 --copy-calibration 256")
 
 
 def short_name(kernel):
@@ -238,9 +240,13 @@ def main():
     scene.upload_particles(x, v)
     print(f"bench: {ctx.device_name} FLIP {n}^3 {npart} particles, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
+    fluid_cells = []  # FLUID cells (the fluid volume in cells) after the first frame and around the timed region
     for i in range(args.warmup):
         scene.advance_frame(dt, solver)
+        if i == 0:
+            fluid_cells.append(int((scene.get(capi.GRID_FLAGS) == capi.FLUID).sum()))
     scene.sync()
+    fluid_cells.append(int((scene.get(capi.GRID_FLAGS) == capi.FLUID).sum()))
 
     scene.kernel_timer(KERNEL_IDS[args.kernel])
     pressure_ms, iters, resid, sub_total = [], [], [], 0
@@ -262,6 +268,7 @@ def main():
     scene.sync()
     t1 = time.perf_counter()
     kms, klaunch = scene.kernel_timer_read(reset=True)
+    fluid_cells.append(int((scene.get(capi.GRID_FLAGS) == capi.FLUID).sum()))
     wall = t1 - t0
     value = npart * sub_total / wall / 1e6
     on_profile = args.n == 128 and args.solver == "mgpcg"
@@ -280,6 +287,9 @@ def main():
     scene.kernel_timer(KERNEL_IDS["none"])
 
     solve_cmp = None if (args.solver == "jacobi" or args.no_solver_comparison) else solver_comparison(scene, capi, dt)
+    learned = None
+    if args.learned_steps > 0 and args.solver != "jacobi":
+        learned = learned_warm_start(capi, ctx, scene, n, dt, args.learned_steps)
     batch = None
     if args.batch_frames > 0:
         batch = [batch_workload(capi, ctx, args.batch_frames, w) for w in args.batch_workers]
@@ -310,16 +320,26 @@ def main():
         "pressure_ms_median": round(statistics.median(pressure_ms), 4),
         "pressure_iterations_median": statistics.median(iters),
         "pressure_rel_residual_max": max(resid),
+        "fluid_volume_cells": {"seeded": seeded_cells(n), "after_first_frame": fluid_cells[0],
+                               "before_timed": fluid_cells[-2], "after_timed": fluid_cells[-1],
+                               "drift_over_run": round((fluid_cells[-1] - fluid_cells[0]) / max(fluid_cells[0], 1), 5)},
         "stage_ms_per_step": {k: round(val / max(args.steps, 1), 4) for k, val in stage_acc.items()},
         "roofline": roof,
         "roofline_cg_update": roof_cg,
         "pressure_solver_comparison": solve_cmp,
+        "learned_warm_start": learned,
         "batch_96_scenes": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     scene.close()
     ctx.close()
+
+
+def seeded_cells(n):
+    """Cells of the n^3 dam-break box x < 0.4, y < 0.6 (cell centres inside)."""
+    c = (np.arange(n) + 0.5) / n
+    return int((c < 0.4).sum() * (c < 0.6).sum() * n)
 
 
 def solver_comparison(scene, capi, dt):
@@ -351,6 +371,74 @@ def solver_comparison(scene, capi, dt):
     for f, a in saved.items():
         scene.set(f, a)
     out["mgpcg_warm_over_cg_time"] = round(out["mgpcg_warm_start"]["ms"] / out["cg_zero_guess"]["ms"], 4)
+    return out
+
+
+def learned_warm_start(capi, ctx, scene, n, dt, steps):
+    """The learned warm start at n^3: samples (pressure systems and solutions)
+    from 20 randomised scenes (seeds 100-119; even: a dam break x < U[0.25,
+    0.6], y < U[0.4, 0.8]; odd: a pool y < U[0.1, 0.3] and a sphere of radius
+    U[0.1, 0.2] above it), one after every 30 of 120 substeps; `steps` Adam
+    steps on the device; then
+    plain CG and multigrid-PCG to 1e-6 from zero and from the network's guess
+    on the dam-break input of the timed scene (build + guess + solve + apply,
+    device ms, best of 3)."""
+    net = capi.Net(ctx, seed=7)
+    t0 = time.perf_counter()
+    mg = capi.solver(capi.SOLVER_MGPCG, 2000, 0, 1e-6, relative=True, warm_start=True)
+    for seed in range(100, 120):
+        rng = np.random.default_rng(seed)
+        if seed % 2 == 0:
+            shapes = [capi.box((-1.0, -1.0, -1.0), (rng.uniform(0.25, 0.6), rng.uniform(0.4, 0.8), 2.0))]
+        else:
+            r = rng.uniform(0.1, 0.2)
+            shapes = [capi.box((-1.0, -1.0, -1.0), (2.0, rng.uniform(0.1, 0.3), 2.0)),
+                      capi.sphere((rng.uniform(0.3, 0.7), rng.uniform(0.55, 0.75), rng.uniform(0.3, 0.7)), r)]
+        desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.03, max_particles=8 * n ** 3)
+        x = capi.seed_particles(desc, shapes, seed=seed)
+        s = capi.Scene(ctx, desc)
+        s.upload_particles(x, np.zeros_like(x))
+        for k in range(1, 121):
+            s.substep(dt, mg)
+            if k % 30 == 0:
+                net.add_sample(s)
+        s.close()
+    t_gen = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    losses = net.train(steps, learning_rate=2e-3, seed=1)
+    t_train = time.perf_counter() - t1
+    print(f"bench: learned warm start: {net.sample_count()} samples in {t_gen:.1f} s, {steps} steps in "
+          f"{t_train:.1f} s, loss {losses[:20].mean():.3f} -> {losses[-50:].mean():.4f}", file=sys.stderr, flush=True)
+
+    for stage in (capi.STAGE_P2G, capi.STAGE_SDF, capi.STAGE_EXTRAPOLATE, capi.STAGE_BC, capi.STAGE_GRAVITY,
+                  capi.STAGE_BC):
+        scene.run_stage(stage, dt)
+    fields = (capi.GRID_U, capi.GRID_V, capi.GRID_W, capi.GRID_PRESSURE)
+    saved = {f: scene.get(f) for f in fields}
+    scene.set_warm_start_net(net)
+    out = {"samples": net.sample_count(), "train_steps": steps, "train_s": round(t_train, 2),
+           "train_ms_per_step": round(t_train * 1e3 / steps, 3),
+           "train_loss_first20": round(float(losses[:20].mean()), 5),
+           "train_loss_last50": round(float(losses[-50:].mean()), 5),
+           "loss": "relative A-norm error^2 after the best scaling"}
+    for name, kind, warm in (("cg_zero_guess", capi.SOLVER_CG, capi.WARM_START_ZERO),
+                             ("cg_learned_guess", capi.SOLVER_CG, capi.WARM_START_LEARNED),
+                             ("mgpcg_zero_guess", capi.SOLVER_MGPCG, capi.WARM_START_ZERO),
+                             ("mgpcg_learned_guess", capi.SOLVER_MGPCG, capi.WARM_START_LEARNED)):
+        best = None
+        for _ in range(3):
+            for f, a in saved.items():
+                scene.set(f, a)
+            st = scene.run_stage(capi.STAGE_PRESSURE, dt,
+                                 capi.solver(kind, 5000, 0, 1e-6, relative=True, warm_start=warm))
+            if best is None or st.ms < best.ms:
+                best = st
+        out[name] = {"ms": round(best.ms, 4), "iterations": best.iterations,
+                     "rel_residual": float(f"{best.rel_residual:.3e}")}
+    for f, a in saved.items():
+        scene.set(f, a)
+    scene.set_warm_start_net(None)
+    net.close()
     return out
 
 

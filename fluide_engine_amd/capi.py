@@ -17,6 +17,9 @@ PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
 SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI, SOLVER_MGPCG = 0, 1, 2, 3
 SDF_SEARCH_CELLS, SDF_SEARCH_BUCKETS = 0, 1
+WARM_START_ZERO, WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1, 2
+NET_SCALE_RESIDUAL, NET_SCALE_ENERGY = 0, 1
+NET_LOSS_MSE, NET_LOSS_ENERGY = 0, 1
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
  GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW, GRID_COLLIDER_SDF) = range(17)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
@@ -34,7 +37,10 @@ EXPORTED = [
     "fe_batch_advance_frame", "fe_surface_build", "fe_surface_download", "fe_scene_clear_colliders",
     "fe_scene_set_sdf_search", "fe_scene_add_mesh_collider", "fe_mesh_sdf", "fe_scene_initialize_from_mesh",
     "fe_scene_add_volume_emitter", "fe_scene_clear_emitters", "fe_scene_reserve_particles",
-    "fe_scene_particle_capacity", "fe_scene_add_volume_emitter_sdf",
+    "fe_scene_particle_capacity", "fe_scene_add_volume_emitter_sdf", "fe_net_create", "fe_net_destroy",
+    "fe_net_param_count", "fe_net_get_params", "fe_net_set_params", "fe_net_add_sample", "fe_net_sample_count",
+    "fe_net_clear_samples", "fe_net_train", "fe_net_loss_and_grads", "fe_scene_set_warm_start_net",
+    "fe_scene_warm_start_scale", "fe_net_set_scale_mode", "fe_net_set_loss",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -125,6 +131,21 @@ def load():
                                               C.c_uint64]
     L.fe_scene_clear_emitters.argtypes = [vp]
     L.fe_scene_add_volume_emitter_sdf.argtypes = [vp, i32p, dp, dp, dp, C.c_int32, dp, C.c_uint64]
+    fp = C.POINTER(C.c_float)
+    L.fe_net_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.fe_net_destroy.argtypes = [vp]
+    L.fe_net_param_count.argtypes = [C.POINTER(C.c_int64)]
+    L.fe_net_get_params.argtypes = [vp, fp, C.c_int64]
+    L.fe_net_set_params.argtypes = [vp, fp, C.c_int64]
+    L.fe_net_add_sample.argtypes = [vp, vp]
+    L.fe_net_sample_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.fe_net_clear_samples.argtypes = [vp]
+    L.fe_net_train.argtypes = [vp, C.c_int32, C.c_double, C.c_uint64, fp]
+    L.fe_net_loss_and_grads.argtypes = [vp, i32p, dp, fp, C.POINTER(C.c_uint8), fp, dp, fp, fp]
+    L.fe_net_set_loss.argtypes = [vp, C.c_int32]
+    L.fe_scene_set_warm_start_net.argtypes = [vp, vp]
+    L.fe_scene_warm_start_scale.argtypes = [vp, dp]
+    L.fe_net_set_scale_mode.argtypes = [vp, C.c_int32]
     L.fe_scene_reserve_particles.argtypes = [vp, C.c_int64]
     L.fe_scene_particle_capacity.argtypes = [vp, C.POINTER(C.c_int64)]
     L.fe_surface_download.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
@@ -432,6 +453,15 @@ class Scene:
         self.ctx.check(self.L.fe_scene_particle_capacity(self._h, C.byref(c)), "fe_scene_particle_capacity")
         return c.value
 
+    def set_warm_start_net(self, net):
+        self.ctx.check(self.L.fe_scene_set_warm_start_net(self._h, net._h if net is not None else None),
+                       "fe_scene_set_warm_start_net")
+
+    def warm_start_scale(self):
+        a = C.c_double()
+        self.ctx.check(self.L.fe_scene_warm_start_scale(self._h, C.byref(a)), "fe_scene_warm_start_scale")
+        return a.value
+
     def clear_emitters(self):
         self.ctx.check(self.L.fe_scene_clear_emitters(self._h), "fe_scene_clear_emitters")
 
@@ -487,3 +517,101 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+class Net:
+    """The learned pressure warm start (fe_net_*)."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx, self.L = ctx, ctx.L
+        h = C.c_void_p()
+        ctx.check(self.L.fe_net_create(ctx._h, int(seed), C.byref(h)), "fe_net_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fe_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def param_count():
+        c = C.c_int64()
+        load().fe_net_param_count(C.byref(c))
+        return c.value
+
+    def params(self):
+        out = np.empty(self.param_count(), np.float32)
+        self.ctx.check(self.L.fe_net_get_params(self._h, _fptr(out), out.size), "fe_net_get_params")
+        return out
+
+    def set_params(self, flat):
+        a = np.ascontiguousarray(flat, dtype=np.float32)
+        self.ctx.check(self.L.fe_net_set_params(self._h, _fptr(a), a.size), "fe_net_set_params")
+
+    def set_scale_mode(self, mode):
+        self.ctx.check(self.L.fe_net_set_scale_mode(self._h, int(mode)), "fe_net_set_scale_mode")
+
+    def add_sample(self, scene):
+        self.ctx.check(self.L.fe_net_add_sample(self._h, scene._h), "fe_net_add_sample")
+
+    def sample_count(self):
+        c = C.c_int64()
+        self.ctx.check(self.L.fe_net_sample_count(self._h, C.byref(c)), "fe_net_sample_count")
+        return c.value
+
+    def clear_samples(self):
+        self.ctx.check(self.L.fe_net_clear_samples(self._h), "fe_net_clear_samples")
+
+    def train(self, steps, learning_rate=1e-3, seed=0):
+        losses = np.empty(int(steps), np.float32)
+        self.ctx.check(self.L.fe_net_train(self._h, int(steps), float(learning_rate), int(seed), _fptr(losses)),
+                       "fe_net_train")
+        return losses
+
+    def set_loss(self, mode):
+        self.ctx.check(self.L.fe_net_set_loss(self._h, int(mode)), "fe_net_set_loss")
+
+    def loss_and_grads(self, b, flags, target, h=None):
+        """b, flags, target as (nz, ny, nx) arrays, h the spacing per axis (x, y, z)
+        or None for 1/n; returns (loss, flat gradients, prediction)."""
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        n = (C.c_int32 * 3)(b.shape[2], b.shape[1], b.shape[0])
+        loss = C.c_double()
+        g = np.empty(self.param_count(), np.float32)
+        pred = np.empty(b.shape, np.float32)
+        hh = None if h is None else (C.c_double * 3)(*[float(q) for q in h])
+        self.ctx.check(self.L.fe_net_loss_and_grads(self._h, n, hh, _fptr(b), f.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                    _fptr(t), C.byref(loss), _fptr(g), _fptr(pred)),
+                       "fe_net_loss_and_grads")
+        return loss.value, g, pred
+
+
+NET_LAYERS = [("enc0", 16, 16), ("enc1", 16, 32), ("enc2", 32, 48), ("enc3", 48, 64),
+              ("dec2", 112, 48), ("dec1", 80, 32), ("dec0", 48, 16)]
+
+
+def net_unflatten(flat):
+    """Flat parameters -> {layer: (W (27, ci, co), b (co,)), 'head': (hw (16,), hb)}."""
+    out, o = {}, 0
+    for name, ci, co in NET_LAYERS:
+        w = flat[o:o + 27 * ci * co].reshape(27, ci, co)
+        o += 27 * ci * co
+        out[name] = (w, flat[o:o + co])
+        o += co
+    out["head"] = (flat[o:o + 16], flat[o + 16])
+    return out
+
+
+def net_flatten(params):
+    parts = []
+    for name, ci, co in NET_LAYERS:
+        w, b = params[name]
+        parts += [np.asarray(w, np.float32).reshape(-1), np.asarray(b, np.float32).reshape(-1)]
+    hw, hb = params["head"]
+    parts += [np.asarray(hw, np.float32).reshape(-1), np.asarray([hb], np.float32)]
+    return np.concatenate(parts)

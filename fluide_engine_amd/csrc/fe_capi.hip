@@ -10,6 +10,7 @@
 #include <thread>
 #include <vector>
 
+#include "fe_learn.hpp"
 #include "fe_stencil.hpp"
 
 using namespace fe;
@@ -137,6 +138,8 @@ int check_solver(fe_ctx* ctx, const fe_solver_desc* d) {
     if (d->max_iterations < 0 || d->max_iterations > 1000000)
         return set_err(ctx, FE_E_INVALID, "max_iterations out of range");
     if (!(d->tolerance >= 0.0)) return set_err(ctx, FE_E_INVALID, "tolerance must be >= 0");
+    if (d->warm_start < FE_WARM_START_ZERO || d->warm_start > FE_WARM_START_LEARNED)
+        return set_err(ctx, FE_E_INVALID, "unknown warm_start " + std::to_string(d->warm_start));
     return FE_OK;
 }
 
@@ -172,6 +175,8 @@ int ensure_sorted(fe_scene* s) {
 // Pressure stage: SinglePhasePressureSolver::Solve (single_phase_pressure_solver.cpp:21-28)
 // minus the markers (built with the SDF), plus the FP64 residual for the stats.
 int pressure(fe_scene* s, const fe_solver_desc* d, bool walls_after, fe_solve_stats* st, int ev_start) {
+    if (d->warm_start == FE_WARM_START_LEARNED && !s->net)
+        return set_err(s->ctx, FE_E_INVALID, "FE_WARM_START_LEARNED needs a network (fe_scene_set_warm_start_net)");
     HIPCHK(s->ctx, launch_pressure_solve(s, d));
     const SolverScalars& h = *s->sc_host;
     const int iters = d->kind == FE_SOLVER_JACOBI ? (h.done ? h.it : d->max_iterations) : h.it;
@@ -975,6 +980,208 @@ int fe_scene_clear_emitters(fe_scene* s) {
     for (Emitter& em : s->emitters)
         if (em.sdf) (void)hipFree(em.sdf);
     s->emitters.clear();
+    return FE_OK;
+}
+
+int fe_net_create(fe_ctx* ctx, uint64_t seed, fe_net** out) {
+    if (!ctx || !out) return FE_E_INVALID;
+    *out = nullptr;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    fe_net* n = new fe_net();
+    n->ctx = ctx;
+    hipError_t e = net_alloc(n, seed);
+    if (e != hipSuccess) {
+        net_free(n);
+        delete n;
+        return hip_fail(ctx, e, "fe_net_create");
+    }
+    *out = n;
+    return FE_OK;
+}
+
+int fe_net_destroy(fe_net* n) {
+    if (!n) return FE_OK;
+    (void)hipSetDevice(n->ctx->device);
+    net_free(n);
+    delete n;
+    return FE_OK;
+}
+
+int fe_net_param_count(int64_t* count) {
+    if (!count) return FE_E_INVALID;
+    *count = net_param_count();
+    return FE_OK;
+}
+
+int fe_net_get_params(fe_net* n, float* params, int64_t count) {
+    if (!n || !params) return FE_E_INVALID;
+    if (count != net_param_count()) return set_err(n->ctx, FE_E_INVALID, "count must be fe_net_param_count");
+    HIPCHK(n->ctx, hipSetDevice(n->ctx->device));
+    HIPCHK(n->ctx, hipStreamSynchronize(n->stream));
+    HIPCHK(n->ctx, net_get_params(n, params));
+    return FE_OK;
+}
+
+int fe_net_set_params(fe_net* n, const float* params, int64_t count) {
+    if (!n || !params) return FE_E_INVALID;
+    if (count != net_param_count()) return set_err(n->ctx, FE_E_INVALID, "count must be fe_net_param_count");
+    HIPCHK(n->ctx, hipSetDevice(n->ctx->device));
+    HIPCHK(n->ctx, hipStreamSynchronize(n->stream));
+    HIPCHK(n->ctx, net_set_params(n, params));
+    return FE_OK;
+}
+
+static int net_grid_ok(fe_ctx* ctx, const int n[3]) {
+    for (int a = 0; a < 3; ++a)
+        if (n[a] < 8 || n[a] % 8 != 0)
+            return set_err(ctx, FE_E_INVALID, "the warm-start network needs every grid axis a multiple of 8");
+    return FE_OK;
+}
+
+int fe_net_add_sample(fe_net* n, fe_scene* s) {
+    if (!n || !s) return FE_E_INVALID;
+    int rc = net_grid_ok(n->ctx, s->g.n);
+    if (rc != FE_OK) return rc;
+    HIPCHK(n->ctx, hipSetDevice(n->ctx->device));
+    hipError_t e = net_add_sample(n, s);
+    if (e == hipErrorInvalidValue)
+        return set_err(n->ctx, FE_E_INVALID, "the scene has no solved pressure system with FLUID cells to sample");
+    HIPCHK(n->ctx, e);
+    return FE_OK;
+}
+
+int fe_net_sample_count(const fe_net* n, int64_t* count) {
+    if (!n || !count) return FE_E_INVALID;
+    *count = (int64_t)n->samples.size();
+    return FE_OK;
+}
+
+int fe_net_clear_samples(fe_net* n) {
+    if (!n) return FE_E_INVALID;
+    HIPCHK(n->ctx, hipSetDevice(n->ctx->device));
+    HIPCHK(n->ctx, hipStreamSynchronize(n->stream));
+    for (auto& smp : n->samples) {
+        (void)hipFree(smp.b);
+        (void)hipFree(smp.flags);
+        (void)hipFree(smp.target);
+        (void)hipFree(smp.code);
+    }
+    n->samples.clear();
+    return FE_OK;
+}
+
+int fe_net_train(fe_net* n, int32_t steps, double learning_rate, uint64_t seed, float* losses) {
+    if (!n) return FE_E_INVALID;
+    if (steps <= 0) return set_err(n->ctx, FE_E_INVALID, "steps must be > 0");
+    if (n->samples.empty()) return set_err(n->ctx, FE_E_INVALID, "the network has no samples");
+    if (!(learning_rate > 0.0) || !std::isfinite(learning_rate))
+        return set_err(n->ctx, FE_E_INVALID, "learning_rate must be finite and > 0");
+    HIPCHK(n->ctx, hipSetDevice(n->ctx->device));
+    HIPCHK(n->ctx, net_train(n, steps, (float)learning_rate, seed));
+    if (losses) HIPCHK(n->ctx, hipMemcpy(losses, n->losses, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return FE_OK;
+}
+
+int fe_net_set_loss(fe_net* n, int32_t mode) {
+    if (!n) return FE_E_INVALID;
+    if (mode != FE_NET_LOSS_MSE && mode != FE_NET_LOSS_ENERGY)
+        return set_err(n->ctx, FE_E_INVALID, "unknown loss " + std::to_string(mode));
+    n->loss_mode = mode;
+    return FE_OK;
+}
+
+int fe_net_loss_and_grads(fe_net* n, const int32_t dims[3], const double h[3], const float* b, const uint8_t* flags,
+                          const float* target, double* loss, float* grads, float* prediction) {
+    if (!n || !dims || !b || !flags || !target) return FE_E_INVALID;
+    int rc = net_grid_ok(n->ctx, dims);
+    if (rc != FE_OK) return rc;
+    fe_ctx* ctx = n->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int64_t V = (int64_t)dims[0] * dims[1] * dims[2];
+    float *db = nullptr, *dt = nullptr, *dl = nullptr;
+    uint8_t* df = nullptr;
+    uint16_t* dc = nullptr;
+    auto cleanup = [&]() {
+        if (db) (void)hipFree(db);
+        if (dt) (void)hipFree(dt);
+        if (dl) (void)hipFree(dl);
+        if (df) (void)hipFree(df);
+        if (dc) (void)hipFree(dc);
+    };
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc((void**)&db, sizeof(float) * V)) != hipSuccess ||
+        (e = hipMalloc((void**)&dt, sizeof(float) * V)) != hipSuccess ||
+        (e = hipMalloc((void**)&dl, sizeof(float))) != hipSuccess || (e = hipMalloc((void**)&df, V)) != hipSuccess ||
+        (e = hipMalloc((void**)&dc, sizeof(uint16_t) * V)) != hipSuccess ||
+        (e = hipMemcpy(db, b, sizeof(float) * V, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dt, target, sizeof(float) * V, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(df, flags, V, hipMemcpyHostToDevice)) != hipSuccess) {
+        cleanup();
+        return hip_fail(ctx, e, "fe_net_loss_and_grads upload");
+    }
+    double ss = 0.0;
+    for (int64_t v = 0; v < V; ++v) ss += (double)b[v] * b[v];
+    const double rms = std::sqrt(ss / (double)V);
+    LossIn in;
+    double energy = 0.0;
+    for (int64_t v = 0; v < V; ++v)
+        if (flags[v] == FE_FLUID) {
+            ++in.nfluid;
+            energy += (double)b[v] * target[v];
+        }
+    in.target = dt;
+    in.flags = df;
+    in.code = dc;
+    in.b = db;
+    in.energy = energy;
+    for (int a = 0; a < 3; ++a) {
+        const double ha = h ? h[a] : 1.0 / dims[a];
+        in.ih2[a] = 1.0 / (ha * ha);
+    }
+    if ((e = net_ensure(n, dims)) == hipSuccess && (e = net_codes(n, dims, df, dc)) == hipSuccess &&
+        (e = net_forward(n, db, df, rms > 0 ? rms : 1.0)) == hipSuccess && (e = net_backward(n, in, dl)) == hipSuccess)
+        e = hipStreamSynchronize(n->stream);
+    if (e == hipSuccess && loss) e = hipMemcpy(loss, n->scal, sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && prediction) e = hipMemcpy(prediction, n->out, sizeof(float) * V, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && grads) {
+        float* g = grads;
+        for (auto& L : n->L) {
+            const int64_t wsz = 27LL * L.cin * L.cout;
+            if (e == hipSuccess) e = hipMemcpy(g, L.gw, sizeof(float) * wsz, hipMemcpyDeviceToHost);
+            g += wsz;
+            if (e == hipSuccess) e = hipMemcpy(g, L.gb, sizeof(float) * L.cout, hipMemcpyDeviceToHost);
+            g += L.cout;
+        }
+        if (e == hipSuccess) e = hipMemcpy(g, n->ghead, sizeof(float) * 17, hipMemcpyDeviceToHost);
+    }
+    cleanup();
+    HIPCHK(ctx, e);
+    return FE_OK;
+}
+
+int fe_scene_set_warm_start_net(fe_scene* s, fe_net* n) {
+    if (!s) return FE_E_INVALID;
+    if (n) {
+        int rc = net_grid_ok(s->ctx, s->g.n);
+        if (rc != FE_OK) return rc;
+        if (n->ctx->device != s->ctx->device)
+            return set_err(s->ctx, FE_E_INVALID, "the network and the scene are on different devices");
+    }
+    s->net = n;
+    return FE_OK;
+}
+
+int fe_net_set_scale_mode(fe_net* n, int32_t mode) {
+    if (!n) return FE_E_INVALID;
+    if (mode != FE_NET_SCALE_RESIDUAL && mode != FE_NET_SCALE_ENERGY)
+        return set_err(n->ctx, FE_E_INVALID, "unknown scale mode " + std::to_string(mode));
+    n->scale_mode = mode;
+    return FE_OK;
+}
+
+int fe_scene_warm_start_scale(const fe_scene* s, double* alpha) {
+    if (!s || !alpha) return FE_E_INVALID;
+    *alpha = s->net_alpha;
     return FE_OK;
 }
 

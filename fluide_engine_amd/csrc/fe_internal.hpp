@@ -381,6 +381,8 @@ struct Emitter {
 };
 }  // namespace fe
 
+struct fe_net;  // the warm-start network (fe_learn.hpp)
+
 struct fe_ctx {
     int device = 0;
     std::string err;
@@ -463,6 +465,10 @@ struct fe_scene {
     // the first one on), and count / offset scratch of the seeding passes
     std::vector<fe::Emitter> emitters;
     int64_t substeps_run = 0;
+
+    // learned warm start: the attached network and the scale of its last guess
+    fe_net* net = nullptr;
+    double net_alpha = 0.0;
     int32_t *src_cnt = nullptr, *src_off = nullptr, *src_sums = nullptr;
 
     // per-kernel timer: HIP events around every launch of one selected kernel

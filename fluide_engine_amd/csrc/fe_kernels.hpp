@@ -74,6 +74,9 @@ hipError_t collider_merge(fe_scene* s, const NodeGrid3& G, const double* c_dev);
 // exceed the capacity
 hipError_t launch_emit(fe_scene* s, bool* over);
 hipError_t reserve_particles(fe_scene* s, int64_t capacity);
+
+// k_learn.hip: x0 = alpha * the network's prediction in the scene's pressure
+hipError_t net_warm_start(fe_net* net, fe_scene* s);
 hipError_t launch_initialize_from_sdf(fe_scene* s, int ppb, uint64_t seed, bool* over);
 void sources_free(fe_scene* s);
 

@@ -435,7 +435,12 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     FaceSet u{{s->u[0], s->u[1], s->u[2]}};
     hipLaunchKernelGGL(k_build_system, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->code,
                        s->b, PA, sh.chunk, sh.blocks);
-    if (!d->warm_start) hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p, nc);
+    if (d->warm_start == FE_WARM_START_LEARNED) {
+        if (!s->net) return hipErrorInvalidValue;
+        if ((e = net_warm_start(s->net, s)) != hipSuccess) return e;
+    } else if (!d->warm_start) {
+        hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p, nc);
+    }
 
     if (d->kind == FE_SOLVER_JACOBI) {
         // The reference's CudaJacobiIterationSolver uploads the previous x (warm start).

@@ -63,7 +63,7 @@ typedef struct fe_solver_desc {
     int32_t max_iterations;
     int32_t check_interval; /* Jacobi residual check interval (reference default 5) */
     int32_t relative;       /* CG: relative stopping rule */
-    int32_t warm_start;     /* start from the previous pressure (reference behaviour) */
+    int32_t warm_start;     /* FE_WARM_START_*: 1 = from the previous pressure (reference behaviour) */
     int32_t reserved;
     double tolerance;
 } fe_solver_desc;
@@ -131,6 +131,14 @@ enum fe_stage {
     FE_STAGE_DIFFUSION = 10,    /* BackwardEulerDiffusionSolver::Solve (ComputeDiffusion
                                    without its boundary condition); solver NULL = the scene's */
     FE_STAGE_EMIT = 11          /* Emitter::Emitt of every emitter (first step of a substep) */
+};
+
+/* Initial guess of a pressure solve (fe_solver_desc.warm_start). */
+enum fe_warm_start {
+    FE_WARM_START_ZERO = 0,
+    FE_WARM_START_PREVIOUS = 1, /* the previous solve's pressure (Array3::Resize keeps it, src/common/array3.hpp:189-197) */
+    FE_WARM_START_LEARNED = 2   /* alpha * the warm-start network's prediction from (b, markers), alpha minimising
+                                   ||b - alpha A p||; needs fe_scene_set_warm_start_net.  No reference counterpart. */
 };
 
 typedef struct fe_ctx fe_ctx;
@@ -277,6 +285,55 @@ int fe_scene_clear_emitters(fe_scene* scene);
  * itself), and read the current capacity. */
 int fe_scene_reserve_particles(fe_scene* scene, int64_t capacity);
 int fe_scene_particle_capacity(const fe_scene* scene, int64_t* capacity);
+
+/* Learned warm start (no reference counterpart).  A 4-level convolutional
+ * encoder-decoder (3x3x3 kernels, 16-64 channels, BF16 weights and activations
+ * with FP32 accumulation on the matrix cores) maps a pressure system's
+ * right-hand side and cell markers to a pressure shape; a solve with
+ * warm_start = FE_WARM_START_LEARNED starts CG from alpha times it, alpha
+ * minimising ||b - alpha A p||, so the initial residual never exceeds ||b||,
+ * and stops at the usual tolerance.  The network is trained on the device
+ * (Adam on FP32 master weights) on samples taken from solved scenes: each
+ * sample is the scene's last system (b, markers) and its solution divided by
+ * its RMS over FLUID cells; the loss is the mean squared error over FLUID
+ * cells.  Grids need every axis a multiple of 8.
+ * Parameters are FP32 in this order: for each convolution (enc0 16->16,
+ * enc1 16->32, enc2 32->48, enc3 48->64, dec2 112->48, dec1 80->32,
+ * dec0 48->16) W[t][ci][co] (t = (dz+1)*9 + (dy+1)*3 + (dx+1)) then its bias,
+ * then the 1x1 head's 16 weights and bias. */
+typedef struct fe_net fe_net;
+int fe_net_create(fe_ctx* ctx, uint64_t seed, fe_net** net);
+int fe_net_destroy(fe_net* net);
+int fe_net_param_count(int64_t* count);
+int fe_net_get_params(fe_net* net, float* params, int64_t count);
+int fe_net_set_params(fe_net* net, const float* params, int64_t count);
+int fe_net_add_sample(fe_net* net, fe_scene* scene); /* the scene's last pressure system and solution */
+int fe_net_sample_count(const fe_net* net, int64_t* count);
+int fe_net_clear_samples(fe_net* net);
+/* steps Adam steps, one sample each (picked by seed); losses (steps floats) may be NULL */
+int fe_net_train(fe_net* net, int32_t steps, double learning_rate, uint64_t seed, float* losses);
+/* Training loss.  FE_NET_LOSS_ENERGY (default): the A-norm error left after
+ * the best scaling of the prediction p, relative to the solution x's,
+ * 1 - (b.p)^2 / ((p.A.p)(b.x)) over FLUID cells -- what a CG warm start needs;
+ * FE_NET_LOSS_MSE: the mean over FLUID cells of (p - x / rms(x))^2. */
+enum fe_net_loss { FE_NET_LOSS_MSE = 0, FE_NET_LOSS_ENERGY = 1 };
+int fe_net_set_loss(fe_net* net, int32_t mode);
+/* One forward and backward pass on host arrays (x fastest): b, markers and the
+ * target (the solution x for the energy loss, the normalised one for MSE) on a
+ * grid of spacing h (NULL: 1/n per axis); the loss, the gradient of every
+ * parameter (fe_net_param_count floats, may be NULL) and the prediction
+ * (n[0]*n[1]*n[2] floats, may be NULL). */
+int fe_net_loss_and_grads(fe_net* net, const int32_t n[3], const double h[3], const float* b, const uint8_t* flags,
+                          const float* target, double* loss, float* grads, float* prediction);
+/* The network used by solves with FE_WARM_START_LEARNED (NULL detaches); the
+ * scale alpha of the scene's last learned warm start. */
+int fe_scene_set_warm_start_net(fe_scene* scene, fe_net* net);
+/* How alpha is chosen: FE_NET_SCALE_ENERGY (default) minimises the A-norm of
+ * the initial error (alpha = b.p / p.A.p, the quantity CG reduces);
+ * FE_NET_SCALE_RESIDUAL minimises ||b - alpha A p||. */
+enum fe_net_scale { FE_NET_SCALE_RESIDUAL = 0, FE_NET_SCALE_ENERGY = 1 };
+int fe_net_set_scale_mode(fe_net* net, int32_t mode);
+int fe_scene_warm_start_scale(const fe_scene* scene, double* alpha);
 
 /* Fluid surface: HybridSimulator::GetSurface -> MarchingCubesSolver::BuildSurface
  * (src/3d/marching_cubes_solver.cpp:25-51) on the current fluid SDF, iso value 0.
