@@ -326,6 +326,9 @@ int fe_net_set_loss(fe_net* net, int32_t mode);
 int fe_net_loss_and_grads(fe_net* net, const int32_t n[3], const double h[3], const float* b, const uint8_t* flags,
                           const float* target, double* loss, float* grads, float* prediction);
 /* The network used by solves with FE_WARM_START_LEARNED (NULL detaches); the
+ * scene keeps a pointer, so the network must outlive the scenes that use it
+ * (or be detached first).  Scenes advanced on several host threads may share
+ * one network (its warm start is serialised).  fe_scene_warm_start_scale: the
  * scale alpha of the scene's last learned warm start. */
 int fe_scene_set_warm_start_net(fe_scene* scene, fe_net* net);
 /* How alpha is chosen: FE_NET_SCALE_ENERGY (default) minimises the A-norm of
