@@ -1,7 +1,7 @@
 // k_particles.hip -- particle-side kernels of the gfx950 substep:
 //   * counting sort of particles by cell (deterministic: bins hold their
 //     particles in ascending previous order),
-//   * P2G as a per-face gather over the sorted cell bins (no float atomics),
+//   * P2G as a gather over 2x2x2 tiles of faces from the sorted cell bins (no float atomics),
 //   * fused G2P + midpoint advection (+ the next substep's bin keys).
 #include "fe_kernels.hpp"
 
@@ -215,16 +215,23 @@ __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __rest
 // P2G: TransferParticles2Grid as a deterministic gather.
 // PIC: src/pic_simulator.cpp:356-437; FLIP: src/flip_simulator.cpp:25-119;
 // APIC: src/apic_simulator.cpp:15-118.
-// One thread per face of component C.  The particles whose stencil can touch
-// face F lie in cells [F-1, F+1] on the transverse axes and [F-1, F] on the
-// normal axis (plus cell n-1 for face n-2 when PIC/APIC clamp the normal axis
-// at n-2).  Cells are x-fastest and particles sorted by cell, so each (j,k)
+// The particles whose stencil can touch face F lie in cells [F-1, F+1] on the
+// transverse axes and [F-1, F] on the normal axis (plus cell n-1 for face n-2
+// when PIC/APIC clamp the normal axis at n-2).  Faces are taken in 2x2x2
+// tiles; a tile's candidate cells are the union of its faces' (4 x 4 x 3
+// cells, against 3 x 3 x 2 per face), so each candidate is read once for 8
+// faces: 6 cells read per face instead of 18.  Four lanes share a tile, lane q
+// scanning the candidate plane kc = klo + q; their partial sums are added in
+// a fixed order (q0 + q1) + (q2 + q3), so the result does not depend on
+// timing.  Cells are x-fastest and particles sorted by cell, so each (j, k)
 // row of candidate cells is one contiguous particle range.  For every
 // candidate the reference's own stencil (GetCooridnatesAndWeights with the
 // scheme's stencil size, precomputed per particle by k_gather) decides: the
-// particle contributes iff the face is one of its 8 nodes; the face is marked
-// even for a zero weight.
+// particle contributes to a face iff the face is one of its 8 nodes; the face
+// is marked even for a zero weight.
 // ---------------------------------------------------------------------------
+constexpr int kP2GLanes = 4;  // lanes per tile (one candidate plane each)
+
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
                                                 const float4* __restrict__ pos4, const float4* __restrict__ st4,
@@ -233,76 +240,106 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
                                                 int64_t padded) {
     const int64_t blk = xcd_remap(blockIdx.x, padded);
     const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
-    const int64_t nf = (int64_t)sx * sy * sz;
-    const int64_t f = blk * kBlock + threadIdx.x;
-    if (f >= nf) return;
-    int I, J, K;
-    split_index(f, sx, sy, I, J, K);
-    const int F[3] = {I, J, K};
+    const int tx = (sx + 1) >> 1, ty = (sy + 1) >> 1, tz = (sz + 1) >> 1;
+    const int64_t t = blk * (kBlock / kP2GLanes) + (threadIdx.x / kP2GLanes);
+    const int q = threadIdx.x % kP2GLanes;
+    // Whole tiles (four aligned lanes) leave together, so the lane exchanges
+    // below only ever read lanes of the same tile.
+    if (t >= (int64_t)tx * ty * tz) return;
+    int ti, tj, tk;
+    split_index(t, tx, ty, ti, tj, tk);
+    const int F0[3] = {2 * ti, 2 * tj, 2 * tk};
     int lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
+        lo[a] = max(F0[a] - 1, 0);
         if (a == C) {
-            lo[a] = F[a] - 1;
-            hi[a] = F[a];
-            if (METHOD != FE_FLIP && F[a] == g.n[a] - 2) hi[a] = F[a] + 1;
+            hi[a] = F0[a] + 1;
+            if (METHOD != FE_FLIP && F0[a] + 1 == g.n[a] - 2) hi[a] = F0[a] + 2;
         } else {
-            lo[a] = F[a] - 1;
-            hi[a] = F[a] + 1;
+            hi[a] = F0[a] + 2;
         }
-        lo[a] = max(lo[a], 0);
         hi[a] = min(hi[a], g.n[a] - 1);
     }
-    // APIC: face position GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
-    // and the clamp box [o + h/2, o + L - h/2] of the transverse axes (apic_simulator.cpp:54-63).
-    float gp[3], clo[3], chi[3];
+    // APIC: face positions GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
+    // of the tile's first face and the clamp box [o + h/2, o + L - h/2] of the
+    // transverse axes (apic_simulator.cpp:54-63).
+    float gp0[3], gp1[3], clo[3], chi[3];
     if (METHOD == FE_APIC) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            gp[a] = g.co[C][a] + g.h[a] * (float)F[a];
+            gp0[a] = g.co[C][a] + g.h[a] * (float)F0[a];
+            gp1[a] = g.co[C][a] + g.h[a] * (float)(F0[a] + 1);
             clo[a] = g.o[a] + 0.5f * g.h[a];
             chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
         }
     }
-    float num = 0.0f, den = 0.0f;
+    // Face e = ex + 2 ey + 4 ez of the tile is (F0x + ex, F0y + ey, F0z + ez).
+    float num[8], den[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) num[e] = den[e] = 0.0f;
     uint32_t marked = 0;
-    // One candidate particle, in ascending particle order, from its stencil
-    // record (fx, fy, fz, packed low bytes of the base indices): the face is
-    // one of its 8 nodes iff F - base is 0 or 1 on every axis.  PIC/FLIP take
-    // the rejected candidates branch-free with zero weight and value, which
-    // leaves num and den bit for bit unchanged (num never holds -0).
+    // One candidate, from its stencil record (fx, fy, fz, packed low bytes of
+    // the base indices): face F takes it iff F - base is 0 or 1 on every axis,
+    // with weight factor 1 - f at offset 0 and f at offset 1 (0 when rejected,
+    // so a rejected candidate adds exact zeros).
     auto consider = [&](int s, float4 r, float val) {
         const uint32_t b = __float_as_uint(r.w);
-        const uint32_t dx = ((uint32_t)I - b) & 255u;
-        const uint32_t dy = ((uint32_t)J - (b >> 8)) & 255u;
-        const uint32_t dz = ((uint32_t)K - (b >> 16)) & 255u;
-        const bool ok = (dx | dy | dz) <= 1u;
-        const float wx = dx ? r.x : 1.0f - r.x;
-        const float wy = dy ? r.y : 1.0f - r.y;
-        const float wz = dz ? r.z : 1.0f - r.z;
-        float w = (wx * wy) * wz;
+        const float fr[3] = {r.x, r.y, r.z};
+        float w0[3], w1[3];
+        uint32_t ok = 0;  // bit 2a: face offset 0 on axis a accepts; bit 2a+1: offset 1
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t d0 = ((uint32_t)F0[a] - (b >> (8 * a))) & 255u;
+            const uint32_t d1 = (d0 + 1u) & 255u;
+            const bool a0 = d0 <= 1u, a1 = d1 <= 1u;
+            w0[a] = a0 ? (d0 ? fr[a] : 1.0f - fr[a]) : 0.0f;
+            w1[a] = a1 ? (d1 ? fr[a] : 1.0f - fr[a]) : 0.0f;
+            ok |= (a0 ? 1u : 0u) << (2 * a);
+            ok |= (a1 ? 1u : 0u) << (2 * a + 1);
+        }
+        // faces accepted on all three axes, as an 8-bit mask over e
+        const uint32_t mx = (ok & 3u) * 0x55u;
+        const uint32_t my = ((ok & 4u) ? 0x33u : 0u) | ((ok & 8u) ? 0xCCu : 0u);
+        const uint32_t mz = ((ok & 16u) ? 0x0Fu : 0u) | ((ok & 32u) ? 0xF0u : 0u);
+        const uint32_t m = mx & my & mz;
         if (METHOD == FE_APIC) {
-            if (!ok) return;
+            if (!m) return;
             const float4 pp = pos4[s];
             float pc[3] = {pp.x, pp.y, pp.z};
 #pragma unroll
             for (int a = 0; a < 3; ++a)
                 if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
-            val += cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]);
-            marked = 1;
+            const float c0 = cr[0], c1 = cr[cap], c2 = cr[2 * cap];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (!((m >> e) & 1u)) continue;
+                const float gx = (e & 1) ? gp1[0] : gp0[0];
+                const float gy = (e & 2) ? gp1[1] : gp0[1];
+                const float gz = (e & 4) ? gp1[2] : gp0[2];
+                const float ve = val + (c0 * (gx - pc[0]) + c1 * (gy - pc[1]) + c2 * (gz - pc[2]));
+                const float w = (((e & 1) ? w1[0] : w0[0]) * ((e & 2) ? w1[1] : w0[1])) * ((e & 4) ? w1[2] : w0[2]);
+                num[e] += ve * w;
+                den[e] += w;
+            }
         } else {
-            marked |= ok ? 1u : 0u;
-            w = ok ? w : 0.0f;
-            val = ok ? val : 0.0f;
+            const float vm = m ? val : 0.0f;
+            const float wxy[4] = {w0[0] * w0[1], w1[0] * w0[1], w0[0] * w1[1], w1[0] * w1[1]};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float w = wxy[e & 3] * ((e & 4) ? w1[2] : w0[2]);
+                num[e] += vm * w;
+                den[e] += w;
+            }
         }
-        num += val * w;
-        den += w;
+        marked |= m;
     };
-    const float4* __restrict__ st = st4 + (int64_t)C * cap;
-    const float* __restrict__ vc = V + (int64_t)C * cap;
-    constexpr int kBatch = 8;  // candidates whose loads are issued together
-    for (int kc = lo[2]; kc <= hi[2]; ++kc) {
+    const int kc = lo[2] + q;
+    if (kc <= hi[2]) {
+        const float4* __restrict__ st = st4 + (int64_t)C * cap;
+        const float* __restrict__ vc = V + (int64_t)C * cap;
+        constexpr int kBatch = 8;  // candidates whose loads are issued together
         for (int jc = lo[1]; jc <= hi[1]; ++jc) {
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
@@ -311,20 +348,47 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
                 float4 rr[kBatch];
                 float pv[kBatch];
 #pragma unroll
-                for (int q = 0; q < kBatch; ++q) {
-                    rr[q] = st[s + q];
-                    pv[q] = vc[s + q];
+                for (int u = 0; u < kBatch; ++u) {
+                    rr[u] = st[s + u];
+                    pv[u] = vc[s + u];
                 }
 #pragma unroll
-                for (int q = 0; q < kBatch; ++q) consider(s + q, rr[q], pv[q]);
+                for (int u = 0; u < kBatch; ++u) consider(s + u, rr[u], pv[u]);
             }
             for (; s < s1; ++s) consider(s, st[s], vc[s]);
         }
     }
-    const float res = den > 0.0f ? num / den : 0.0f;
-    out[f] = res;
-    mark[f] = (uint8_t)marked;
-    if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+    // (q0 + q1) + (q2 + q3): the same value on all four lanes.
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        num[e] += __shfl_xor(num[e], 1, kP2GLanes);
+        den[e] += __shfl_xor(den[e], 1, kP2GLanes);
+        num[e] += __shfl_xor(num[e], 2, kP2GLanes);
+        den[e] += __shfl_xor(den[e], 2, kP2GLanes);
+    }
+    marked |= __shfl_xor(marked, 1, kP2GLanes);
+    marked |= __shfl_xor(marked, 2, kP2GLanes);
+    // Lane q writes faces e = 2q and 2q + 1 (ey = q & 1, ez = q >> 1).
+    const int J = F0[1] + (q & 1), K = F0[2] + (q >> 1);
+    if (J >= sy || K >= sz) return;
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex) {
+        const int I = F0[0] + ex;
+        if (I >= sx) continue;
+        const int e = ex + 2 * q;
+        const int64_t f = (int64_t)I + (int64_t)sx * ((int64_t)J + (int64_t)sy * K);
+        float ne = 0.0f, de = 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (u == e) {
+                ne = num[u];
+                de = den[u];
+            }
+        const float res = de > 0.0f ? ne / de : 0.0f;
+        out[f] = res;
+        mark[f] = (uint8_t)((marked >> e) & 1u);
+        if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -559,8 +623,10 @@ static hipError_t p2g_method(fe_scene* s) {
     const Particles& P = s->P[s->cur];
     const Geom& g = s->g;
     for (int c = 0; c < 3; ++c) {
-        const int64_t nf = face_count(g, c);
-        const int64_t padded = pad_blocks((nf + kBlock - 1) / kBlock);
+        const int64_t tiles = (int64_t)((face_dim(g, c, 0) + 1) / 2) * ((face_dim(g, c, 1) + 1) / 2) *
+                              ((face_dim(g, c, 2) + 1) / 2);
+        const int64_t per_block = kBlock / kP2GLanes;
+        const int64_t padded = pad_blocks((tiles + per_block - 1) / per_block);
         auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
         launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
                      (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
