@@ -39,6 +39,7 @@ constexpr int kTailMax = 4096;        // levels of at most this many cells run i
 constexpr int kTailLds = 4608;        // cells of all tail levels together (3 floats + a row code each in LDS)
 constexpr int kTailThreads = 1024;
 constexpr int kMaxTail = 8;
+constexpr int kTailPer = (kTailMax + kTailThreads - 1) / kTailThreads;  // cells per thread on a tail level
 
 // ---------------------------------------------------------------------------
 // Level construction
@@ -232,12 +233,27 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
     const bool inside = I < cx && J < cy && K < cz;
     const int64_t c = (int64_t)I + (int64_t)cx * ((int64_t)J + (int64_t)cy * K);
     const uint16_t cd = inside ? ccode[c] : (uint16_t)0;  // issued with the window loads
-    for (int e = threadIdx.x; e < kRwx * kRwy * kRwz; e += kBlock) {
-        const int wi = e % kRwx, wj = (e / kRwx) % kRwy, wk = e / (kRwx * kRwy);
-        const int fi = min(max(fi0 + wi, 0), fx - 1);
-        const int fj = min(max(fj0 + wj, 0), fy - 1);
-        const int fk = min(max(fk0 + wk, 0), fz - 1);
-        win[wk][wj][wi] = rf[(int64_t)fi + (int64_t)fx * ((int64_t)fj + (int64_t)fy * fk)];
+    // All of a thread's window loads are issued before the first LDS store, so
+    // the window costs one memory latency rather than one per element.
+    constexpr int kWin = kRwx * kRwy * kRwz;
+    constexpr int kPer = (kWin + kBlock - 1) / kBlock;
+    float wv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + u * kBlock;
+        wv[u] = 0.0f;
+        if (e < kWin) {
+            const int wi = e % kRwx, wj = (e / kRwx) % kRwy, wk = e / (kRwx * kRwy);
+            const int fi = min(max(fi0 + wi, 0), fx - 1);
+            const int fj = min(max(fj0 + wj, 0), fy - 1);
+            const int fk = min(max(fk0 + wk, 0), fz - 1);
+            wv[u] = rf[(int64_t)fi + (int64_t)fx * ((int64_t)fj + (int64_t)fy * fk)];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + u * kBlock;
+        if (e < kWin) (&win[0][0][0])[e] = wv[u];
     }
     __syncthreads();
     if (!inside) return;
@@ -315,7 +331,12 @@ __device__ inline TailLevel tail_level(const TailArgs& a, int t) {
 // One Jacobi sweep: out = in + w (b - A in), or out = w b when `in` is null.
 __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, const float* in, float* out,
                                   const float* b, float omega) {
-    for (int c = threadIdx.x; c < l.nc; c += kTailThreads) {
+    // a tail level holds at most kTailMax cells: a fixed trip count lets the
+    // LDS reads of a thread's cells overlap
+#pragma unroll
+    for (int u = 0; u < kTailPer; ++u) {
+        const int c = threadIdx.x + u * kTailThreads;
+        if (c >= l.nc) break;
         const uint16_t cd = code[c];
         const float dgn = code_center(cd, l.ih2);
         const float w = mg_weight(cd, l.ih2, omega);
@@ -340,12 +361,42 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
     if (st->done) return;
     __shared__ float X[kTailLds], Tm[kTailLds], B[kTailLds];
     __shared__ uint16_t Cd[kTailLds];
-    for (int t = 0; t < a.levels; ++t) {
-        const int nc = a.n[t][0] * a.n[t][1] * a.n[t][2];
-        for (int c = threadIdx.x; c < nc; c += kTailThreads) Cd[a.off[t] + c] = a.code[t][c];
-    }
+    // Row codes and the right-hand side: every load is issued before the
+    // first LDS store (one memory latency in all).
+    constexpr int kPer = kTailPer;
     const int n0 = a.n[0][0] * a.n[0][1] * a.n[0][2];
-    for (int c = threadIdx.x; c < n0; c += kTailThreads) B[c] = bT[c];
+    {
+        float bv[kPer];
+        uint16_t cv[kMaxTail][kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int c = threadIdx.x + u * kTailThreads;
+            bv[u] = c < n0 ? bT[c] : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < kMaxTail; ++t) {
+            const int nc = t < a.levels ? a.n[t][0] * a.n[t][1] * a.n[t][2] : 0;
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int c = threadIdx.x + u * kTailThreads;
+                cv[t][u] = c < nc ? a.code[t][c] : (uint16_t)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int c = threadIdx.x + u * kTailThreads;
+            if (c < n0) B[c] = bv[u];
+        }
+#pragma unroll
+        for (int t = 0; t < kMaxTail; ++t) {
+            const int nc = t < a.levels ? a.n[t][0] * a.n[t][1] * a.n[t][2] : 0;
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int c = threadIdx.x + u * kTailThreads;
+                if (c < nc) Cd[a.off[t] + c] = cv[t][u];
+            }
+        }
+    }
     __syncthreads();
     // down: the solution of every level ends in Tm after its pre-smoothing
     for (int t = 0; t < a.levels - 1; ++t) {
@@ -357,7 +408,10 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         // residual into X (free now)
         {
             const float* xs = Tm + o;
-            for (int c = threadIdx.x; c < l.nc; c += kTailThreads) {
+#pragma unroll
+            for (int u = 0; u < kTailPer; ++u) {
+                const int c = threadIdx.x + u * kTailThreads;
+                if (c >= l.nc) break;
                 const uint16_t cd = code[c];
                 float rc = 0.0f;
                 if (cd & kFluidBit)
@@ -370,7 +424,10 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         // restriction into the next level's B
         {
             const float* rs = X + o;
-            for (int c = threadIdx.x; c < lc.nc; c += kTailThreads) {
+#pragma unroll
+            for (int u = 0; u < kTailPer; ++u) {
+                const int c = threadIdx.x + u * kTailThreads;
+                if (c >= lc.nc) break;
                 float v = 0.0f;
                 if (Cd[lc.o + c] & kFluidBit) {
                     int I, J, K;
@@ -402,7 +459,10 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         const int o = l.o;
         const uint16_t* code = Cd + o;
         const float* xc = Tm + lc.o;
-        for (int f = threadIdx.x; f < l.nc; f += kTailThreads) {
+#pragma unroll
+        for (int u = 0; u < kTailPer; ++u) {
+            const int f = threadIdx.x + u * kTailThreads;
+            if (f >= l.nc) break;
             if (!(code[f] & kFluidBit)) continue;
             int i, j, k;
             cell_ijk(f, l.nx, l.ny, i, j, k);
@@ -412,7 +472,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         tail_sweep(l, code, Tm + o, X + o, B + o, kMgOmega);
         tail_sweep(l, code, X + o, Tm + o, B + o, kMgOmega);
     }
-    for (int c = threadIdx.x; c < n0; c += kTailThreads) xT[c] = Tm[c];
+#pragma unroll
+    for (int u = 0; u < kTailPer; ++u) {
+        const int c = threadIdx.x + u * kTailThreads;
+        if (c < n0) xT[c] = Tm[c];
+    }
 }
 
 // ---------------------------------------------------------------------------
