@@ -76,19 +76,52 @@ __device__ inline double code_center64(uint16_t code, const double ih2[3]) {
 // row code is consulted, so the loads never wait on it; an uncoupled
 // neighbour contributes fma(0, v, acc) = acc (every vector holds finite values).
 template <typename I, typename F>
+__device__ inline void code_neighbours(I c, const StrideT<I>& st, F v, float (&nb)[6]) {
+    nb[0] = v(c > 0 ? c - 1 : 0);
+    nb[1] = v(c < st.last ? c + 1 : st.last);
+    nb[2] = v(c >= st.sy ? c - st.sy : 0);
+    nb[3] = v(c + st.sy <= st.last ? c + st.sy : st.last);
+    nb[4] = v(c >= st.sz ? c - st.sz : 0);
+    nb[5] = v(c + st.sz <= st.last ? c + st.sz : st.last);
+}
+
+__device__ inline float code_apply_nb(uint16_t code, float center, const float ih2[3], float vc, const float (&nb)[6]) {
+    float acc = center * vc;
+    acc = fmaf((code & 1) ? -ih2[0] : 0.0f, nb[0], acc);
+    acc = fmaf((code & 2) ? -ih2[0] : 0.0f, nb[1], acc);
+    acc = fmaf((code & 4) ? -ih2[1] : 0.0f, nb[2], acc);
+    acc = fmaf((code & 8) ? -ih2[1] : 0.0f, nb[3], acc);
+    acc = fmaf((code & 16) ? -ih2[2] : 0.0f, nb[4], acc);
+    acc = fmaf((code & 32) ? -ih2[2] : 0.0f, nb[5], acc);
+    return acc;
+}
+
+template <typename I, typename F>
 __device__ inline float code_apply(uint16_t code, float center, I c, const StrideT<I>& st, const float ih2[3],
                                    float vc, F v) {
-    const float vxm = v(c > 0 ? c - 1 : 0), vxp = v(c < st.last ? c + 1 : st.last);
-    const float vym = v(c >= st.sy ? c - st.sy : 0), vyp = v(c + st.sy <= st.last ? c + st.sy : st.last);
-    const float vzm = v(c >= st.sz ? c - st.sz : 0), vzp = v(c + st.sz <= st.last ? c + st.sz : st.last);
-    float acc = center * vc;
-    acc = fmaf((code & 1) ? -ih2[0] : 0.0f, vxm, acc);
-    acc = fmaf((code & 2) ? -ih2[0] : 0.0f, vxp, acc);
-    acc = fmaf((code & 4) ? -ih2[1] : 0.0f, vym, acc);
-    acc = fmaf((code & 8) ? -ih2[1] : 0.0f, vyp, acc);
-    acc = fmaf((code & 16) ? -ih2[2] : 0.0f, vzm, acc);
-    acc = fmaf((code & 32) ? -ih2[2] : 0.0f, vzp, acc);
-    return acc;
+    float nb[6];
+    code_neighbours(c, st, v, nb);
+    return code_apply_nb(code, center, ih2, vc, nb);
+}
+
+// A thread's cells of a block span (c0 + threadIdx.x, then every kBlock-th)
+// in groups of G: load(c) fetches what cell c needs into a T and use(c, t)
+// consumes it.  A full group issues all G loads before the first use, so the
+// group costs one memory latency instead of G; a partial group at the end of
+// the span runs cell by cell.  Cells are used in ascending order either way.
+template <int G, typename T, typename Load, typename Use>
+__device__ inline void span_groups(int64_t c0, int64_t c1, Load load, Use use) {
+    for (int64_t g0 = c0 + threadIdx.x; g0 < c1; g0 += (int64_t)G * kBlock) {
+        if (g0 + (int64_t)(G - 1) * kBlock < c1) {
+            T t[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) t[u] = load(g0 + (int64_t)u * kBlock);
+#pragma unroll
+            for (int u = 0; u < G; ++u) use(g0 + (int64_t)u * kBlock, t[u]);
+        } else {
+            for (int64_t c = g0; c < c1; c += kBlock) use(c, load(c));
+        }
+    }
 }
 
 template <typename F>

@@ -171,6 +171,23 @@ struct LevelArgs {
     int64_t chunk, padded;
 };
 
+// What a smoothing sweep or a residual reads for cell c.
+struct StencilIn {
+    uint16_t cd;
+    float xc, bc, nb[6];
+};
+
+template <typename F>
+__device__ inline StencilIn stencil_in(int64_t c, const uint16_t* __restrict__ code, const float* __restrict__ x,
+                                       const float* __restrict__ b, const Stride& st, F X) {
+    StencilIn t;
+    t.cd = code[c];
+    t.xc = x[c];
+    t.bc = b[c];
+    code_neighbours(c, st, X, t.nb);
+    return t;
+}
+
 // x_out = x_in + w (b - A x_in).  With prz != nullptr (the last level-0 sweep)
 // also leaves partials of b . x_out, which is r . z of the CG.
 __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
@@ -182,17 +199,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
     const Stride stv = L.st;
     double v[1] = {0.0};
     auto X = [&](int64_t n) { return xin[n]; };
-#pragma unroll 2
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const uint16_t cd = code[c];
-        const float dgn = code_center(cd, L.ih2);
-        const float w = mg_weight(cd, L.ih2);
-        const float xc = xin[c];
-        const float bc = b[c];
-        const float xo = xc + w * (bc - code_apply(cd, dgn, c, stv, L.ih2, xc, X));
-        xout[c] = xo;
-        v[0] += (double)bc * (double)xo;
-    }
+    span_groups<8, StencilIn>(
+        sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, xin, b, stv, X); },
+        [&](int64_t c, const StencilIn& t) {
+            const float dgn = code_center(t.cd, L.ih2);
+            const float w = mg_weight(t.cd, L.ih2);
+            const float xo = t.xc + w * (t.bc - code_apply_nb(t.cd, dgn, L.ih2, t.xc, t.nb));
+            xout[c] = xo;
+            v[0] += (double)t.bc * (double)xo;
+        });
     if (prz) write_partials<1>(v, prz);
 }
 
@@ -203,13 +218,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint1
     const Span sp = block_span(L.nc, L.chunk, L.padded);
     const Stride stv = L.st;
     auto X = [&](int64_t n) { return x[n]; };
-#pragma unroll 2
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const uint16_t cd = code[c];
-        float rc = 0.0f;
-        if (cd & kFluidBit) rc = b[c] - code_apply(cd, code_center(cd, L.ih2), c, stv, L.ih2, x[c], X);
-        r[c] = rc;
-    }
+    span_groups<8, StencilIn>(
+        sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, x, b, stv, X); },
+        [&](int64_t c, const StencilIn& t) {
+            float rc = 0.0f;
+            if (t.cd & kFluidBit) rc = t.bc - code_apply_nb(t.cd, code_center(t.cd, L.ih2), L.ih2, t.xc, t.nb);
+            r[c] = rc;
+        });
 }
 
 // b_c = R r_f on coarse FLUID cells (0 elsewhere); with x1 != nullptr also the

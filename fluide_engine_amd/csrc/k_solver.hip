@@ -287,14 +287,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     const float fb = (float)beta;
     auto D = [&](int64_t n) { return fb * dold[n] + z[n]; };  // AXpY(beta, d, s, &d)
     double v[1] = {0.0};
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const uint16_t cd = code[c];
-        const float dc = D(c);
-        const float qc = code_apply(cd, code_center(cd, ih2), c, st, ih2, dc, D);
-        dnew[c] = dc;
-        q[c] = qc;
-        v[0] += (double)dc * (double)qc;
-    }
+    struct In {
+        uint16_t cd;
+        float dc, nb[6];
+    };
+    span_groups<4, In>(
+        sp.c0, sp.c1,
+        [&](int64_t c) {
+            In t;
+            t.cd = code[c];
+            t.dc = D(c);
+            code_neighbours(c, st, D, t.nb);
+            return t;
+        },
+        [&](int64_t c, const In& t) {
+            const float qc = code_apply_nb(t.cd, code_center(t.cd, ih2), ih2, t.dc, t.nb);
+            dnew[c] = t.dc;
+            q[c] = qc;
+            v[0] += (double)t.dc * (double)qc;
+        });
     write_partials<1>(v, pout);
 }
 
@@ -327,11 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
     const float fa = (float)alpha;
     auto X = [&](int64_t n) { return alpha * (double)d[n] + xo[n]; };
     double v[2] = {0.0, 0.0};
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const double xc = X(c);
-        xn[c] = xc;
-        const uint16_t cd = (replace || pre) ? code[c] : 0;
-        const float rc = replace ? (float)((double)b[c] - code_apply64(cd, c, st, ih2d, xc, X)) : -fa * q[c] + r[c];
+    auto finish = [&](int64_t c, uint16_t cd, float rc) {
         r[c] = rc;
         float zc = rc;
         if (pre == 1) {
@@ -341,6 +348,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
         if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
         if (pre != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
+    };
+    if (replace) {  // r = b - A x recomputed in FP64
+        for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
+            const double xc = X(c);
+            xn[c] = xc;
+            const uint16_t cd = code[c];
+            finish(c, cd, (float)((double)b[c] - code_apply64(cd, c, st, ih2d, xc, X)));
+        }
+    } else {
+        struct In {
+            double x;
+            float d, q, r;
+            uint16_t cd;
+        };
+        span_groups<4, In>(
+            sp.c0, sp.c1,
+            [&](int64_t c) {
+                In t;
+                t.x = xo[c];
+                t.d = d[c];
+                t.q = q[c];
+                t.r = r[c];
+                t.cd = pre ? code[c] : (uint16_t)0;
+                return t;
+            },
+            [&](int64_t c, const In& t) {
+                xn[c] = alpha * (double)t.d + t.x;
+                finish(c, t.cd, -fa * t.q + t.r);
+            });
     }
     write_partials<2>(v, pout);
 }
