@@ -320,6 +320,7 @@ struct SolverScalars {
     int32_t max_it;
     int32_t replace;   // the next (or last) update recomputes r = b - A x
     int32_t converged; // the stopping rule (not the iteration cap) ended the solve
+    int32_t confirm;   // a relative stop met on the recurrence awaits the recomputed residual
 };
 
 // ---------------------------------------------------------------------------

@@ -44,7 +44,11 @@ hipError_t read_final_residual(fe_scene* s, double* rr, double* bb);  // waits f
 
 // k_mg.hip: multigrid V-cycle preconditioner of the pressure CG
 hipError_t mg_setup(fe_scene* s);  // coarse markers and row codes from s->flags / s->code
-hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz, int tag);
+// pconf (nparts blocks of (r.z, r.r), or null): the CG update's partials, with
+// which a V-cycle that follows a confirming residual recomputation ends the
+// solve at once when the recomputed residual meets a relative stop.
+hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, double* prz, int tag,
+                    const double* pconf, int nparts);
 float* mg_first_sweep_target(fe_scene* s);  // where the CG kernels write level 0's first sweep
 void mg_free(fe_scene* s);
 

@@ -147,6 +147,12 @@ __device__ inline float mg_weight(uint16_t code, const float ih2[3], float omega
     return dg > 0.0f ? omega * __builtin_amdgcn_rcpf(dg) : 0.0f;  // hardware reciprocal: a smoother weight
 }
 
+// CG stopping rule: relative ||r|| <= tol ||b||, else the reference's sigma = r.z <= tol^2.
+__device__ inline bool stop_rule(const SolverScalars* sc, double bnorm2, double sigma, double rr) {
+    if (sc->relative) return sqrt(rr) <= sc->tol * sqrt(bnorm2);
+    return !(sigma > sc->tol * sc->tol);
+}
+
 // ---------------------------------------------------------------------------
 // Launch shape and block-level FP64 reductions.
 // ---------------------------------------------------------------------------

@@ -15,7 +15,7 @@
 //     correction (weights 3/4, 1/4 per axis; out-of-range coarse neighbours
 //     clamp onto the boundary cell, which is the Neumann wall);
 //   * restriction: the transpose of that prolongation divided by 8;
-//   * coarsest level (<= 512 cells): 16 damped Jacobi sweeps (omega = 0.9)
+//   * coarsest level (<= 512 cells): 8 damped Jacobi sweeps (omega = 0.9)
 //     from zero, a fixed polynomial in the operator.
 // Non-FLUID cells carry no residual into the coarse levels and take no
 // correction; on level 0 they are smoothed like the identity rows they are.
@@ -33,7 +33,7 @@
 namespace fe {
 
 constexpr float kOmegaCoarse = 0.9f;  // coarsest-level Jacobi damping
-constexpr int kCoarseSweeps = 16;
+constexpr int kCoarseSweeps = 8;
 constexpr int kCoarseMax = 512;       // cells of the coarsest level
 constexpr int kTailMax = 4096;        // levels of at most this many cells run in k_mg_tail
 constexpr int kTailLds = 4608;        // cells of all tail levels together (3 floats + a row code each in LDS)
@@ -189,12 +189,31 @@ __device__ inline StencilIn stencil_in(int64_t c, const uint16_t* __restrict__ c
 }
 
 // x_out = x_in + w (b - A x_in).  With prz != nullptr (the last level-0 sweep)
-// also leaves partials of b . x_out, which is r . z of the CG.
+// also leaves partials of b . x_out, which is r . z of the CG.  With pconf !=
+// nullptr (the V-cycle's first kernel) and a pending confirmation, the
+// residual the CG update just recomputed as b - A x is tested first: when it
+// meets the relative stop the solve ends here (iteration count of that
+// update), without this V-cycle and the next spmv.
 __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
                                                       const float* __restrict__ b, const float* __restrict__ xin,
-                                                      float* __restrict__ xout, const SolverScalars* __restrict__ st,
-                                                      double* __restrict__ prz) {
+                                                      float* __restrict__ xout, const SolverScalars* st,
+                                                      double* __restrict__ prz, const double* __restrict__ pconf,
+                                                      int nparts) {
     if (st->done) return;  // uniform: the solve has stopped
+    if (pconf && st->confirm) {
+        double t[2];
+        sum_partials<2>(pconf, nparts, t);
+        if (stop_rule(st, st->bnorm2, t[0], t[1])) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                SolverScalars* w = const_cast<SolverScalars*>(st);
+                w->rr = t[1];
+                w->it = st->it + 1;
+                w->converged = 1;
+                w->done = 1;
+            }
+            return;
+        }
+    }
     const Span sp = block_span(L.nc, L.chunk, L.padded);
     const Stride stv = L.st;
     double v[1] = {0.0};
@@ -604,7 +623,8 @@ float* mg_first_sweep_target(fe_scene* s) { return s->mg.empty() ? nullptr : s->
 // z = M^-1 r with one V-cycle.  The caller has already written level 0's
 // first sweep (x1 = w r) into mg[0].x.  Leaves partials of r.z in prz.  Every
 // kernel returns at once when `st` says the solve has stopped.
-hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* st, double* prz, int tag) {
+hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, double* prz, int tag,
+                    const double* pconf, int nparts) {
     std::vector<MGLevel>& lv = s->mg;
     const int L = (int)lv.size();
     const int T = s->mg_tail;
@@ -616,10 +636,11 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* 
         const dim3 grid((unsigned)a.padded);
         if (q == 0)
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, pconf, nparts);
         else
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
-                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+                               (const float*)l.b, (const float*)l.x, l.x2, (const SolverScalars*)st, (double*)nullptr,
+                               (const double*)nullptr, 0);
         hipLaunchKernelGGL(k_mg_residual, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                            (const float*)l.b, (const float*)l.x2, l.r, st);
         const dim3 rgrid((unsigned)((c.n[0] + kRtx - 1) / kRtx), (unsigned)((c.n[1] + kRty - 1) / kRty),
@@ -654,14 +675,14 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, const SolverScalars* 
                            c.n[1], c.n[2], (const uint16_t*)l.code, l.x2, l.n[0], l.n[1], l.n[2], st);
         if (q == 0) {
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr);
+                         (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0);
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x, z, st, prz);
+                         (const float*)l.b, (const float*)l.x, z, st, prz, (const double*)nullptr, 0);
         } else {
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
-                               (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr);
+                               (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0);
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
-                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr);
+                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, (const double*)nullptr, 0);
         }
     }
     return hipGetLastError();

@@ -24,10 +24,6 @@
 namespace fe {
 
 
-__device__ inline bool stop_rule(const SolverScalars* sc, double bnorm2, double sigma, double rr) {
-    if (sc->relative) return sqrt(rr) <= sc->tol * sqrt(bnorm2);
-    return !(sigma > sc->tol * sc->tol);
-}
 
 // ---------------------------------------------------------------------------
 // BuildSystem (single_phase_pressure_solver.cpp:99-164): the row codes and the
@@ -241,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     }
     const int it_prev = in->it;
     double beta, rr_ref;
-    int it, trig, done, conv, repl;
+    int it, trig, done, conv, repl, confirm = 0;
     if (it_prev < 0) {  // first iteration: partials come from k_cg_init
         it = 0;
         beta = 0.0;
@@ -259,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         if (stop && in->relative && !in->replace) {  // confirm on the true residual
             stop = false;
             trig = 1;
+            confirm = 1;
         }
         conv = stop ? 1 : 0;
         done = (stop || it >= in->max_it) ? 1 : 0;
@@ -276,6 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         o.done = done;
         o.replace = repl;
         o.rr_ref = rr_ref;
+        o.confirm = confirm;
         *out = o;
     }
     if (done) return;
@@ -522,7 +520,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     float* mgx = mg ? mg_first_sweep_target(s) : nullptr;
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
                        s->r, s->z, s->d[0], mgx, s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
-    if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ, -1)) != hipSuccess) return e;
+    if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ, -1, nullptr, 0)) != hipSuccess) return e;
     double* xa = s->p;
     double* xb = s->p2;
     int dcur = 0;
@@ -546,7 +544,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
                      (const float*)s->q, s->r, s->z, mgx, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
                      ih2[1], ih2[2], sh.chunk, sh.blocks);
         std::swap(xa, xb);
-        if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk);
+        if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk, PB, nparts);
     };
     for (;;) {
         const int nb = std::min(first, d->max_iterations + 1 - k);
