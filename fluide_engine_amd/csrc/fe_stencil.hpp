@@ -75,8 +75,8 @@ __device__ inline double code_center64(uint16_t code, const double ih2[3]) {
 // neighbour values are fetched (at indices clamped into the grid) before the
 // row code is consulted, so the loads never wait on it; an uncoupled
 // neighbour contributes fma(0, v, acc) = acc (every vector holds finite values).
-template <typename I, typename F>
-__device__ inline void code_neighbours(I c, const StrideT<I>& st, F v, float (&nb)[6]) {
+template <typename I, typename F, typename T>
+__device__ inline void code_neighbours(I c, const StrideT<I>& st, F v, T (&nb)[6]) {
     nb[0] = v(c > 0 ? c - 1 : 0);
     nb[1] = v(c < st.last ? c + 1 : st.last);
     nb[2] = v(c >= st.sy ? c - st.sy : 0);
@@ -122,6 +122,20 @@ __device__ inline void span_groups(int64_t c0, int64_t c1, Load load, Use use) {
             for (int64_t c = g0; c < c1; c += kBlock) use(c, load(c));
         }
     }
+}
+
+// code_apply64 on neighbour values fetched beforehand (clamped indices, as
+// code_neighbours), same arithmetic in the same order.
+__device__ inline double code_apply64_nb(uint16_t code, const double ih2[3], double vc, const double (&nb)[6]) {
+    if (!(code & kFluidBit)) return vc;
+    double off = 0.0;
+    if (code & 1) off -= ih2[0] * nb[0];
+    if (code & 2) off -= ih2[0] * nb[1];
+    if (code & 4) off -= ih2[1] * nb[2];
+    if (code & 8) off -= ih2[1] * nb[3];
+    if (code & 16) off -= ih2[2] * nb[4];
+    if (code & 32) off -= ih2[2] * nb[5];
+    return code_center64(code, ih2) * vc + off;
 }
 
 template <typename F>

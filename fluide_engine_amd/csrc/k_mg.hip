@@ -363,8 +363,9 @@ __device__ inline TailLevel tail_level(const TailArgs& a, int t) {
 }
 
 // One Jacobi sweep: out = in + w (b - A in), or out = w b when `in` is null.
-__device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, const float* in, float* out,
-                                  const float* b, float omega) {
+// dg, wt: the level's diagonal and smoother weight per cell (precomputed).
+__device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, const float* dg, const float* wt,
+                                  const float* in, float* out, const float* b) {
     // a tail level holds at most kTailMax cells: a fixed trip count lets the
     // LDS reads of a thread's cells overlap
 #pragma unroll
@@ -372,12 +373,11 @@ __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, cons
         const int c = threadIdx.x + u * kTailThreads;
         if (c >= l.nc) break;
         const uint16_t cd = code[c];
-        const float dgn = code_center(cd, l.ih2);
-        const float w = mg_weight(cd, l.ih2, omega);
+        const float w = wt[c];
         float xo;
         if (in) {
             const float xc = in[c];
-            xo = xc + w * (b[c] - code_apply(cd, dgn, c, l.st, l.ih2, xc, [&](int n) { return in[n]; }));
+            xo = xc + w * (b[c] - code_apply(cd, dg[c], c, l.st, l.ih2, xc, [&](int n) { return in[n]; }));
         } else {
             xo = w * b[c];
         }
@@ -387,13 +387,14 @@ __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, cons
 }
 
 // Solves level T's right-hand side bT approximately with the V-cycle of the
-// tail levels; the result goes to xT.  Everything lives in LDS: three values
-// and the row code per cell of every tail level (at most 4608 cells, 63 KiB).
+// tail levels; the result goes to xT.  Everything lives in LDS: five values
+// (two iterates, right-hand side, diagonal, smoother weight) and the row code
+// per cell of every tail level (at most 4608 cells, 99 KiB).
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bT,
                                                           float* __restrict__ xT,
                                                           const SolverScalars* __restrict__ st) {
     if (st->done) return;
-    __shared__ float X[kTailLds], Tm[kTailLds], B[kTailLds];
+    __shared__ float X[kTailLds], Tm[kTailLds], B[kTailLds], Dg[kTailLds], Wt[kTailLds];
     __shared__ uint16_t Cd[kTailLds];
     // Row codes and the right-hand side: every load is issued before the
     // first LDS store (one memory latency in all).
@@ -432,13 +433,28 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         }
     }
     __syncthreads();
+    // per cell: the diagonal and the damped-Jacobi weight (the coarsest level
+    // with its own damping)
+    for (int t = 0; t < a.levels; ++t) {
+        const TailLevel l = tail_level(a, t);
+        const float omega = t == a.levels - 1 ? kOmegaCoarse : kMgOmega;
+#pragma unroll
+        for (int u = 0; u < kTailPer; ++u) {
+            const int c = threadIdx.x + u * kTailThreads;
+            if (c >= l.nc) break;
+            const uint16_t cd = Cd[l.o + c];
+            Dg[l.o + c] = code_center(cd, l.ih2);
+            Wt[l.o + c] = mg_weight(cd, l.ih2, omega);
+        }
+    }
+    __syncthreads();
     // down: the solution of every level ends in Tm after its pre-smoothing
     for (int t = 0; t < a.levels - 1; ++t) {
         const TailLevel l = tail_level(a, t), lc = tail_level(a, t + 1);
         const int o = l.o;
         const uint16_t* code = Cd + o;
-        tail_sweep(l, code, nullptr, X + o, B + o, kMgOmega);
-        tail_sweep(l, code, X + o, Tm + o, B + o, kMgOmega);
+        tail_sweep(l, code, Dg + o, Wt + o, nullptr, X + o, B + o);
+        tail_sweep(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o);
         // residual into X (free now)
         {
             const float* xs = Tm + o;
@@ -449,7 +465,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
                 const uint16_t cd = code[c];
                 float rc = 0.0f;
                 if (cd & kFluidBit)
-                    rc = B[o + c] - code_apply(cd, code_center(cd, l.ih2), c, l.st, l.ih2, xs[c],
+                    rc = B[o + c] - code_apply(cd, Dg[o + c], c, l.st, l.ih2, xs[c],
                                                [&](int n) { return xs[n]; });
                 X[o + c] = rc;
             }
@@ -480,11 +496,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         const int o = l.o;
         const uint16_t* code = Cd + o;
         static_assert(kCoarseSweeps % 2 == 0, "the sweeps ping-pong X -> Tm");
-        tail_sweep(l, code, nullptr, X + o, B + o, kOmegaCoarse);
-        tail_sweep(l, code, X + o, Tm + o, B + o, kOmegaCoarse);
+        tail_sweep(l, code, Dg + o, Wt + o, nullptr, X + o, B + o);
+        tail_sweep(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o);
         for (int s = 2; s < kCoarseSweeps; s += 2) {
-            tail_sweep(l, code, Tm + o, X + o, B + o, kOmegaCoarse);
-            tail_sweep(l, code, X + o, Tm + o, B + o, kOmegaCoarse);
+            tail_sweep(l, code, Dg + o, Wt + o, Tm + o, X + o, B + o);
+            tail_sweep(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o);
         }
     }
     // up: prolongation, two sweeps Tm -> X -> Tm
@@ -503,8 +519,8 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
             Tm[o + f] += prolong_at(i, j, k, lc.nx, lc.ny, lc.nz, [&](int64_t n) { return xc[n]; });
         }
         __syncthreads();
-        tail_sweep(l, code, Tm + o, X + o, B + o, kMgOmega);
-        tail_sweep(l, code, X + o, Tm + o, B + o, kMgOmega);
+        tail_sweep(l, code, Dg + o, Wt + o, Tm + o, X + o, B + o);
+        tail_sweep(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o);
     }
 #pragma unroll
     for (int u = 0; u < kTailPer; ++u) {

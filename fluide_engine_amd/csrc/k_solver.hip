@@ -157,6 +157,24 @@ __global__ __launch_bounds__(kBlock) void k_zero_f64(double* __restrict__ x, int
     if (c < n) x[c] = 0.0;
 }
 
+// What a residual b - A x on the FP64 solution reads for cell c.
+struct Stencil64In {
+    uint16_t cd;
+    float bc;
+    double xc, nb[6];
+};
+
+__device__ inline Stencil64In stencil64_in(int64_t c, const uint16_t* __restrict__ code,
+                                           const double* __restrict__ x, const float* __restrict__ b,
+                                           const Stride& st) {
+    Stencil64In t;
+    t.cd = code[c];
+    t.bc = b[c];
+    t.xc = x[c];
+    code_neighbours(c, st, [&](int64_t n) { return x[n]; }, t.nb);
+    return t;
+}
+
 // ---------------------------------------------------------------------------
 // Conjugate gradient (ConjugateGradientSolver::Solve, src/linear_system/conjugate_gradient_solver.cpp:14-55),
 // two kernels per iteration:
@@ -169,7 +187,9 @@ __global__ __launch_bounds__(kBlock) void k_zero_f64(double* __restrict__ x, int
 //           neighbours is formed on the fly from x and d); z = M^-1 r;
 //           partials of r.z and r.r
 // With the relative rule a stop is only accepted on a freshly replaced
-// residual, so the decision is taken on b - A x, not on the recurrence.
+// residual, so the decision is taken on b - A x, not on the recurrence (with
+// the multigrid preconditioner the next V-cycle's first kernel tests the
+// replaced residual, so a confirmed stop costs no V-cycle and no spmv).
 // The FP32 recurrence drifts from b - A x by about eps32 * ||A|| * sum|alpha d|,
 // which grows with the residual at the last recomputation: the residual is
 // also recomputed whenever ||r|| has fallen 100x below that value ("reliable
@@ -194,21 +214,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
     const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     const int pre = sc->precond;  // 0 none, 1 diagonal, 2 multigrid (z and r.z come later)
     double v[2] = {0.0, 0.0};
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const uint16_t cd = code[c];
-        const double ax = code_apply64(cd, c, st, ih2d, x[c], [&](int64_t n) { return x[n]; });
-        const float rc = (float)((double)b[c] - ax);
-        r[c] = rc;
-        float zc = rc;
-        if (pre == 1) {
-            zc = rc / code_center(cd, ih2);
-            z[c] = zc;
-        }
-        if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
-        d0[c] = 0.0f;
-        if (pre != 2) v[0] += (double)rc * (double)zc;
-        v[1] += (double)rc * (double)rc;
-    }
+    span_groups<4, Stencil64In>(
+        sp.c0, sp.c1, [&](int64_t c) { return stencil64_in(c, code, x, b, st); },
+        [&](int64_t c, const Stencil64In& t) {
+            const float rc = (float)((double)t.bc - code_apply64_nb(t.cd, ih2d, t.xc, t.nb));
+            r[c] = rc;
+            float zc = rc;
+            if (pre == 1) {
+                zc = rc / code_center(t.cd, ih2);
+                z[c] = zc;
+            }
+            if (mgx) mgx[c] = mg_weight(t.cd, ih2) * rc;  // the V-cycle's first sweep from zero
+            d0[c] = 0.0f;
+            if (pre != 2) v[0] += (double)rc * (double)zc;
+            v[1] += (double)rc * (double)rc;
+        });
     write_partials<2>(v, pout);
 }
 
@@ -390,12 +410,13 @@ __global__ __launch_bounds__(kBlock) void k_final_residual(Geom g, const uint16_
     const double ih2[3] = {ihx2, ihy2, ihz2};
     const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     double v[2] = {0.0, 0.0};
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        const double ax = code_apply64(code[c], c, st, ih2, x[c], [&](int64_t n) { return x[n]; });
-        const double res = (double)b[c] - ax;
-        v[0] += res * res;
-        v[1] += (double)b[c] * (double)b[c];
-    }
+    span_groups<4, Stencil64In>(
+        sp.c0, sp.c1, [&](int64_t c) { return stencil64_in(c, code, x, b, st); },
+        [&](int64_t c, const Stencil64In& t) {
+            const double res = (double)t.bc - code_apply64_nb(t.cd, ih2, t.xc, t.nb);
+            v[0] += res * res;
+            v[1] += (double)t.bc * (double)t.bc;
+        });
     write_partials<2>(v, pout);
 }
 
