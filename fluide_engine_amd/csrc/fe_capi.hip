@@ -427,6 +427,8 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     for (int c = 0; c < 3; ++c) {
         chk(dalloc(&s->u[c], nf[c]));
         chk(dalloc(&s->snap[c], nf[c]));
+        chk(dalloc(&s->quad[c], nf[c]));
+        if (d->method == FE_FLIP) chk(dalloc(&s->squad[c], nf[c]));
         chk(dalloc(&s->mark[c], nf[c]));
     }
     chk(dalloc(&s->valid[0], nf[0] + nf[1] + nf[2]));
@@ -521,6 +523,8 @@ int fe_scene_destroy(fe_scene* s) {
     for (int c = 0; c < 3; ++c) {
         F(s->u[c]);
         F(s->snap[c]);
+        F(s->quad[c]);
+        F(s->squad[c]);
         F(s->mark[c]);
     }
     F(s->valid[0]); F(s->valid[1]);

@@ -286,6 +286,22 @@ __device__ inline float sample_face(const float* __restrict__ a, const Geom& g, 
     return trilerp_pairs(load_pair(a + t.r00), load_pair(a + t.r10), load_pair(a + t.r01), load_pair(a + t.r11), t);
 }
 
+// The quad layout of a face array: Q[i + sx (j + sy k)] = (a(i, j, k),
+// a(i1, j, k), a(i, j1, k), a(i1, j1, k)) with i1 = min(i+1, sx-1) and
+// j1 = min(j+1, sy-1), the clamps of the stencil above.  A sample then takes
+// two 16-byte loads (planes k and k+1) instead of four 8-byte ones, and
+// interpolates the same values in the same order, so it is bit-identical.
+__device__ inline float trilerp_quads(float4 q0, float4 q1, const FaceStencil& t) {
+    return trilerp_pairs(make_float2(q0.x, q0.y), make_float2(q0.z, q0.w), make_float2(q1.x, q1.y),
+                         make_float2(q1.z, q1.w), t);
+}
+
+__device__ inline float sample_quad(const float4* __restrict__ q, const Geom& g, int c, float px, float py,
+                                    float pz) {
+    const FaceStencil t = face_stencil(g, c, px, py, pz);
+    return trilerp_quads(q[t.r00], q[t.r01], t);
+}
+
 // XCD-aware block remap: consecutive logical tiles land on the same XCD (and
 // its L2) instead of being dealt round-robin.  nblocks is padded to a multiple
 // of 8 by the launcher; blocks mapped past the real count exit.
@@ -402,6 +418,8 @@ struct fe_scene {
     // grid (HBM)
     float* u[3] = {nullptr, nullptr, nullptr};       // face velocities
     float* snap[3] = {nullptr, nullptr, nullptr};    // FLIP snapshot
+    float4* quad[3] = {nullptr, nullptr, nullptr};   // face values as 2x2 (x, y) quads for G2P sampling
+    float4* squad[3] = {nullptr, nullptr, nullptr};  // the FLIP snapshot as quads
     uint8_t* mark[3] = {nullptr, nullptr, nullptr};  // P2G face markers
     uint8_t* valid[2] = {nullptr, nullptr};          // extrapolation validity ping-pong (all 3 comps)
     double* sdf = nullptr;

@@ -8,6 +8,9 @@ namespace fe {
 struct FaceSet {
     float* a[3];
 };
+struct QuadSet {
+    const float4* a[3];
+};
 
 // k_particles.hip
 hipError_t launch_fill_i32(hipStream_t st, int32_t* a, int32_t v, int64_t n);
@@ -20,6 +23,7 @@ hipError_t launch_scan_with(hipStream_t st, const int32_t* in, int32_t* out, int
 hipError_t launch_sort(fe_scene* s, bool keys_ready);
 hipError_t launch_p2g(fe_scene* s);
 hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bool emit_keys);
+hipError_t launch_quads(fe_scene* s, float* const a[3], float4* const q[3]);
 hipError_t launch_unpermute(fe_scene* s, const float* src, int ncomp, float* dst);
 hipError_t launch_deinterleave(fe_scene* s, const float* src, int ncomp, float* dst);
 hipError_t launch_iota(fe_scene* s, int32_t* a, int64_t n);

@@ -398,13 +398,14 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 // ---------------------------------------------------------------------------
 // PIC sample of a and the FLIP delta sample of a - snapshot (node by node,
 // flip_simulator.cpp:121-161) on one stencil: 8 pair loads instead of 24 loads.
-__device__ inline void sample_pic_delta(const float* __restrict__ a, const float* __restrict__ sn, const Geom& g,
+__device__ inline void sample_pic_delta(const float4* __restrict__ qa, const float4* __restrict__ qs, const Geom& g,
                                         int c, float px, float py, float pz, float& pic, float& delta) {
     const FaceStencil t = face_stencil(g, c, px, py, pz);
-    const float2 a00 = load_pair(a + t.r00), a10 = load_pair(a + t.r10), a01 = load_pair(a + t.r01),
-                 a11 = load_pair(a + t.r11);
-    const float2 s00 = load_pair(sn + t.r00), s10 = load_pair(sn + t.r10), s01 = load_pair(sn + t.r01),
-                 s11 = load_pair(sn + t.r11);
+    const float4 qa0 = qa[t.r00], qa1 = qa[t.r01], qs0 = qs[t.r00], qs1 = qs[t.r01];
+    const float2 a00 = make_float2(qa0.x, qa0.y), a10 = make_float2(qa0.z, qa0.w), a01 = make_float2(qa1.x, qa1.y),
+                 a11 = make_float2(qa1.z, qa1.w);
+    const float2 s00 = make_float2(qs0.x, qs0.y), s10 = make_float2(qs0.z, qs0.w), s01 = make_float2(qs1.x, qs1.y),
+                 s11 = make_float2(qs1.z, qs1.w);
     pic = trilerp_pairs(a00, a10, a01, a11, t);
     auto d = [](float2 x, float2 y) { return make_float2(x.x - y.x, x.y - y.y); };
     delta = trilerp_pairs(d(a00, s00), d(a10, s10), d(a01, s01), d(a11, s11), t);
@@ -438,8 +439,8 @@ __device__ inline void grad_contract(const float* __restrict__ a, const Geom& g,
 
 template <int METHOD>
 __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_t cap, int64_t np, FaceSet grid,
-                                                     FaceSet snapset, float blend, float dt, int nsub, int do_g2p,
-                                                     int do_move, uint32_t* __restrict__ key,
+                                                     QuadSet gq, QuadSet sq, float blend, float dt, int nsub,
+                                                     int do_g2p, int do_move, uint32_t* __restrict__ key,
                                                      int32_t* __restrict__ count) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = s < np;
@@ -451,19 +452,19 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
             float pic[3];
             if (METHOD == FE_PIC) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = sample_face(grid.a[c], g, c, x, y, z);
+                for (int c = 0; c < 3; ++c) v[c] = sample_quad(gq.a[c], g, c, x, y, z);
             } else if (METHOD == FE_FLIP) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float delta;
-                    sample_pic_delta(grid.a[c], snapset.a[c], g, c, x, y, z, pic[c], delta);
+                    sample_pic_delta(gq.a[c], sq.a[c], g, c, x, y, z, pic[c], delta);
                     float fl = v[c] + delta;
                     if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
                     v[c] = fl;
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) pic[c] = sample_face(grid.a[c], g, c, x, y, z);
+                for (int c = 0; c < 3; ++c) pic[c] = sample_quad(gq.a[c], g, c, x, y, z);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = pic[c];
 #pragma unroll
@@ -489,11 +490,11 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
             for (int t = 0; t < nsub; ++t) {
                 float v0[3], mid[3], mv[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v0[c] = sample_face(grid.a[c], g, c, p0[0], p0[1], p0[2]);
+                for (int c = 0; c < 3; ++c) v0[c] = sample_quad(gq.a[c], g, c, p0[0], p0[1], p0[2]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) mid[a] = p0[a] + 0.5f * sdt * v0[a];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) mv[c] = sample_face(grid.a[c], g, c, mid[0], mid[1], mid[2]);
+                for (int c = 0; c < 3; ++c) mv[c] = sample_quad(gq.a[c], g, c, mid[0], mid[1], mid[2]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     p1[a] = p0[a] + sdt * mv[a];
@@ -643,12 +644,43 @@ hipError_t launch_p2g(fe_scene* s) {
     }
 }
 
+// Quad layout of the three face arrays a[c] into q[c] (see sample_quad).
+__global__ __launch_bounds__(kBlock) void k_quads(Geom g, const float* __restrict__ a0, const float* __restrict__ a1,
+                                                  const float* __restrict__ a2, float4* __restrict__ q0,
+                                                  float4* __restrict__ q1, float4* __restrict__ q2, int64_t b0,
+                                                  int64_t b1) {
+    const int64_t blk = blockIdx.x;
+    const int c = blk < b0 ? 0 : (blk < b1 ? 1 : 2);
+    const int64_t f = (blk - (c == 0 ? 0 : (c == 1 ? b0 : b1))) * kBlock + threadIdx.x;
+    const float* __restrict__ a = c == 0 ? a0 : (c == 1 ? a1 : a2);
+    float4* __restrict__ q = c == 0 ? q0 : (c == 1 ? q1 : q2);
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
+    if (f >= (int64_t)sx * sy * sz) return;
+    int i, j, k;
+    split_index(f, sx, sy, i, j, k);
+    const int64_t di = i + 1 < sx ? 1 : 0, dj = j + 1 < sy ? sx : 0;
+    q[f] = make_float4(a[f], a[f + di], a[f + dj], a[f + dj + di]);
+}
+
+hipError_t launch_quads(fe_scene* s, float* const a[3], float4* const q[3]) {
+    const Geom& g = s->g;
+    const int64_t nb0 = (face_count(g, 0) + kBlock - 1) / kBlock, nb1 = (face_count(g, 1) + kBlock - 1) / kBlock,
+                  nb2 = (face_count(g, 2) + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_quads, dim3((unsigned)(nb0 + nb1 + nb2)), dim3(kBlock), 0, s->stream, g, (const float*)a[0],
+                       (const float*)a[1], (const float*)a[2], q[0], q[1], q[2], nb0, nb0 + nb1);
+    return hipGetLastError();
+}
+
 hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bool emit_keys) {
     if (s->np == 0) return hipSuccess;
     const Particles& P = s->P[s->cur];
     FaceSet grid{{s->u[0], s->u[1], s->u[2]}};
-    FaceSet snaps{{s->snap[0], s->snap[1], s->snap[2]}};
+    QuadSet gq{{s->quad[0], s->quad[1], s->quad[2]}};
+    QuadSet sq{{s->squad[0], s->squad[1], s->squad[2]}};
+    const bool flip_g2p = do_g2p && s->g.method == FE_FLIP;
     hipError_t e;
+    if ((e = launch_quads(s, s->u, s->quad)) != hipSuccess) return e;
+    if (flip_g2p && (e = launch_quads(s, s->snap, s->squad)) != hipSuccess) return e;
     if (emit_keys) {
         if ((e = launch_fill_i32(s->stream, s->cell_count, 0, cell_count(s->g) + 1)) != hipSuccess) return e;
     }
@@ -657,7 +689,7 @@ hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bo
     const float blend = (float)s->desc.blend;
     auto kern = s->g.method == FE_PIC ? k_g2p_move<FE_PIC>
                                       : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP> : k_g2p_move<FE_APIC>);
-    launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid, snaps,
+    launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid, gq, sq,
                  blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key, cnt);
     if (do_move) s->sorted = false;
     return hipGetLastError();
