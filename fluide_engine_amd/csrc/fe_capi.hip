@@ -38,11 +38,13 @@ int hip_fail(fe_ctx* ctx, hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_fail(ctx, _e, #call); \
     } while (0)
 
+// Device arrays carry 64 bytes of slack past their end, so kernels may read
+// whole aligned words that straddle an array's last element.
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
-    return hipMalloc((void**)p, count * sizeof(T));
+    return hipMalloc((void**)p, count * sizeof(T) + 64);
 }
 
 void build_geom(fe_scene* s) {
