@@ -464,23 +464,41 @@ def batch_workload(capi, ctx, frames, workers, warmup=2):
     dt = 1.0 / 480.0
     for _ in range(warmup):
         b.advance_frame(dt, solver)
-    per_substep, psub = [], 0
+
+    def timed():
+        per_substep, psub = [], 0
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            sub, ms = b.advance_frame(dt, solver)
+            psub += sum(cnt * s for (_, cnt), s in zip(scenes, sub))
+            per_substep.extend(m / max(s, 1) for m, s in zip(ms, sub))
+        wall = time.perf_counter() - t0
+        per_substep.sort()
+        q = lambda f: per_substep[min(len(per_substep) - 1, int(f * len(per_substep)))]
+        return {"scenes_per_s": round(len(specs) * frames / wall, 2),
+                "mparticle_substeps_per_s": round(psub / wall / 1e6, 2),
+                "scene_substep_ms_p50": round(q(0.5), 4), "scene_substep_ms_p99": round(q(0.99), 4),
+                "wall_s": round(wall, 3)}
+
+    out = {"scenes": len(specs), "particles": npart, "frames": frames, "workers": workers}
+    out.update(timed())  # default kernel variants (P2G tiles, G2P quads)
+    # the autotuner picks per scene between bit-identical kernel variants
     t0 = time.perf_counter()
-    for _ in range(frames):
-        sub, ms = b.advance_frame(dt, solver)
-        psub += sum(cnt * s for (_, cnt), s in zip(scenes, sub))
-        per_substep.extend(m / max(s, 1) for m, s in zip(ms, sub))
-    wall = time.perf_counter() - t0
+    picks = {}
+    for (sc, _), (n, _, _) in zip(scenes, specs):
+        (p2g, g2p), _ = sc.autotune(reps=3)
+        key = f"{n}^3"
+        picks.setdefault(key, {"p2g_faces": 0, "g2p_pairs": 0, "scenes": 0})
+        picks[key]["scenes"] += 1
+        picks[key]["p2g_faces"] += int(p2g == capi.P2G_FACES)
+        picks[key]["g2p_pairs"] += int(g2p == capi.G2P_PAIRS)
+    out["autotune_s"] = round(time.perf_counter() - t0, 3)
+    out["autotune_picks"] = picks
+    out["autotuned"] = timed()
     b.close()
     for sc, _ in scenes:
         sc.close()
-    per_substep.sort()
-    q = lambda f: per_substep[min(len(per_substep) - 1, int(f * len(per_substep)))]
-    return {"scenes": len(specs), "particles": npart, "frames": frames, "workers": workers,
-            "scenes_per_s": round(len(specs) * frames / wall, 2),
-            "mparticle_substeps_per_s": round(psub / wall / 1e6, 2),
-            "scene_substep_ms_p50": round(q(0.5), 4), "scene_substep_ms_p99": round(q(0.99), 4),
-            "wall_s": round(wall, 3)}
+    return out
 
 
 def cpu_baseline(scene, capi, desc, dt, args):

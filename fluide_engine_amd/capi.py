@@ -20,6 +20,8 @@ SDF_SEARCH_CELLS, SDF_SEARCH_BUCKETS = 0, 1
 WARM_START_ZERO, WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1, 2
 NET_SCALE_RESIDUAL, NET_SCALE_ENERGY = 0, 1
 NET_LOSS_MSE, NET_LOSS_ENERGY = 0, 1
+P2G_TILES, P2G_FACES = 0, 1
+G2P_QUADS, G2P_PAIRS = 0, 1
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
  GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW, GRID_COLLIDER_SDF) = range(17)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
@@ -40,7 +42,8 @@ EXPORTED = [
     "fe_scene_particle_capacity", "fe_scene_add_volume_emitter_sdf", "fe_net_create", "fe_net_destroy",
     "fe_net_param_count", "fe_net_get_params", "fe_net_set_params", "fe_net_add_sample", "fe_net_sample_count",
     "fe_net_clear_samples", "fe_net_train", "fe_net_loss_and_grads", "fe_scene_set_warm_start_net",
-    "fe_scene_warm_start_scale", "fe_net_set_scale_mode", "fe_net_set_loss",
+    "fe_scene_warm_start_scale", "fe_net_set_scale_mode", "fe_net_set_loss", "fe_scene_get_tuning",
+    "fe_scene_set_tuning", "fe_scene_autotune",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -50,6 +53,14 @@ class SolverDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("max_iterations", C.c_int32), ("check_interval", C.c_int32),
                 ("relative", C.c_int32), ("warm_start", C.c_int32), ("reserved", C.c_int32),
                 ("tolerance", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("p2g", C.c_int32), ("g2p", C.c_int32)]
+
+
+class TuningTimes(C.Structure):
+    _fields_ = [("p2g_ms", C.c_double * 2), ("g2p_ms", C.c_double * 2)]
 
 
 class SolveStats(C.Structure):
@@ -461,6 +472,23 @@ class Scene:
         a = C.c_double()
         self.ctx.check(self.L.fe_scene_warm_start_scale(self._h, C.byref(a)), "fe_scene_warm_start_scale")
         return a.value
+
+    def tuning(self):
+        """(p2g variant, g2p variant) in use."""
+        t = Tuning()
+        self.ctx.check(self.L.fe_scene_get_tuning(self._h, C.byref(t)), "fe_scene_get_tuning")
+        return t.p2g, t.g2p
+
+    def set_tuning(self, p2g, g2p):
+        t = Tuning(p2g, g2p)
+        self.ctx.check(self.L.fe_scene_set_tuning(self._h, C.byref(t)), "fe_scene_set_tuning")
+
+    def autotune(self, reps=5):
+        """Times the kernel variants on the current particles and keeps the
+        fastest; returns ((p2g, g2p), {"p2g_ms": [tiles, faces], "g2p_ms": [quads, pairs]})."""
+        t, tm = Tuning(), TuningTimes()
+        self.ctx.check(self.L.fe_scene_autotune(self._h, int(reps), C.byref(t), C.byref(tm)), "fe_scene_autotune")
+        return (t.p2g, t.g2p), {"p2g_ms": list(tm.p2g_ms), "g2p_ms": list(tm.g2p_ms)}
 
     def clear_emitters(self):
         self.ctx.check(self.L.fe_scene_clear_emitters(self._h), "fe_scene_clear_emitters")

@@ -930,6 +930,29 @@ int fe_scene_add_volume_emitter(fe_scene* s, const int32_t resolution[3], const 
     return FE_OK;
 }
 
+int fe_scene_get_tuning(const fe_scene* s, fe_tuning* out) {
+    if (!s || !out) return FE_E_INVALID;
+    *out = s->tune;
+    return FE_OK;
+}
+
+int fe_scene_set_tuning(fe_scene* s, const fe_tuning* t) {
+    if (!s || !t) return FE_E_INVALID;
+    if (t->p2g < FE_P2G_TILES || t->p2g > FE_P2G_FACES || t->g2p < FE_G2P_QUADS || t->g2p > FE_G2P_PAIRS)
+        return set_err(s->ctx, FE_E_INVALID, "unknown kernel variant");
+    s->tune = *t;
+    return FE_OK;
+}
+
+int fe_scene_autotune(fe_scene* s, int32_t reps, fe_tuning* out, fe_tuning_times* times) {
+    if (!s) return FE_E_INVALID;
+    if (reps < 1 || reps > 1000) return set_err(s->ctx, FE_E_INVALID, "reps must be in [1, 1000]");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, autotune(s, reps, times));
+    if (out) *out = s->tune;
+    return FE_OK;
+}
+
 int fe_scene_reserve_particles(fe_scene* s, int64_t capacity) {
     if (!s) return FE_E_INVALID;
     if (capacity < 0 || capacity > INT32_MAX)

@@ -414,6 +414,7 @@ struct fe_scene {
     hipStream_t stream = nullptr;
     int depth = 3;            // ceil(maxCfl): extrapolation / BC depth
     int move_substeps = 3;    // (unsigned)max(maxCfl, 1)
+    fe_tuning tune{FE_P2G_TILES, FE_G2P_QUADS};  // kernel variants (bit-identical results)
 
     // grid (HBM)
     float* u[3] = {nullptr, nullptr, nullptr};       // face velocities

@@ -24,6 +24,8 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready);
 hipError_t launch_p2g(fe_scene* s);
 hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bool emit_keys);
 hipError_t launch_quads(fe_scene* s, float* const a[3], float4* const q[3]);
+// times the P2G and G2P variants on the scene's particles; keeps the fastest
+hipError_t autotune(fe_scene* s, int reps, fe_tuning_times* times);
 hipError_t launch_unpermute(fe_scene* s, const float* src, int ncomp, float* dst);
 hipError_t launch_deinterleave(fe_scene* s, const float* src, int ncomp, float* dst);
 hipError_t launch_iota(fe_scene* s, int32_t* a, int64_t n);

@@ -3,6 +3,10 @@
 //     particles in ascending previous order),
 //   * P2G as a gather over 2x2x2 tiles of faces from the sorted cell bins (no float atomics),
 //   * fused G2P + midpoint advection (+ the next substep's bin keys).
+#include <algorithm>
+#include <functional>
+#include <vector>
+
 #include "fe_kernels.hpp"
 
 namespace fe {
@@ -391,6 +395,113 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
     }
 }
 
+// The same transfer with one thread per face (more threads, each face's
+// candidates read by the face itself): the variant that wins on small grids.
+// Its sums are bit-identical to the tile kernel's: a face's candidates form
+// per-plane partials in the tile kernel's order (the tile's extra candidates
+// add exact zeros there), and the partials of the tile's planes
+// lo, lo+1, lo+2, lo+3 (lo = max((K & ~1) - 1, 0)) are added as
+// (p0 + p1) + (p2 + p3), planes the face does not read contributing +0.
+template <int C, int METHOD>
+__global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
+                                                     const float4* __restrict__ pos4, const float4* __restrict__ st4,
+                                                     int64_t cap, const int32_t* __restrict__ start,
+                                                     float* __restrict__ out, uint8_t* __restrict__ mark,
+                                                     float* __restrict__ snap, int64_t padded) {
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
+    const int64_t f = blk * kBlock + threadIdx.x;
+    if (f >= (int64_t)sx * sy * sz) return;
+    int I, J, K;
+    split_index(f, sx, sy, I, J, K);
+    const int F[3] = {I, J, K};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = max(F[a] - 1, 0);
+        if (a == C) {
+            hi[a] = F[a];
+            if (METHOD != FE_FLIP && F[a] == g.n[a] - 2) hi[a] = F[a] + 1;
+        } else {
+            hi[a] = F[a] + 1;
+        }
+        hi[a] = min(hi[a], g.n[a] - 1);
+    }
+    float gp[3], clo[3], chi[3];
+    if (METHOD == FE_APIC) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            gp[a] = g.co[C][a] + g.h[a] * (float)F[a];
+            clo[a] = g.o[a] + 0.5f * g.h[a];
+            chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
+        }
+    }
+    float num = 0.0f, den = 0.0f;
+    uint32_t marked = 0;
+    auto consider = [&](int s, float4 r, float val) {
+        const uint32_t b = __float_as_uint(r.w);
+        const uint32_t dx = ((uint32_t)I - b) & 255u;
+        const uint32_t dy = ((uint32_t)J - (b >> 8)) & 255u;
+        const uint32_t dz = ((uint32_t)K - (b >> 16)) & 255u;
+        if ((dx | dy | dz) > 1u) return;
+        const float wx = dx ? r.x : 1.0f - r.x;
+        const float wy = dy ? r.y : 1.0f - r.y;
+        const float wz = dz ? r.z : 1.0f - r.z;
+        const float w = (wx * wy) * wz;
+        if (METHOD == FE_APIC) {
+            const float4 pp = pos4[s];
+            float pc[3] = {pp.x, pp.y, pp.z};
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
+            const float* cr = Cm + (int64_t)(3 * C) * cap + s;
+            val = val + (cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]));
+        }
+        marked = 1;
+        num += val * w;
+        den += w;
+    };
+    const float4* __restrict__ st = st4 + (int64_t)C * cap;
+    const float* __restrict__ vc = V + (int64_t)C * cap;
+    const int pl = max((K & ~1) - 1, 0);
+    float pn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    constexpr int kBatch = 8;  // candidates whose loads are issued together
+    for (int kc = lo[2]; kc <= hi[2]; ++kc) {
+        num = 0.0f;
+        den = 0.0f;
+        for (int jc = lo[1]; jc <= hi[1]; ++jc) {
+            const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
+            const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
+            int s = s0;
+            for (; s + kBatch <= s1; s += kBatch) {
+                float4 rr[kBatch];
+                float pv[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    rr[u] = st[s + u];
+                    pv[u] = vc[s + u];
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) consider(s + u, rr[u], pv[u]);
+            }
+            for (; s < s1; ++s) consider(s, st[s], vc[s]);
+        }
+        const int q = kc - pl;  // 0..3
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u == q) {
+                pn[u] = num;
+                pd[u] = den;
+            }
+    }
+    const float nt = (pn[0] + pn[1]) + (pn[2] + pn[3]);
+    const float dt = (pd[0] + pd[1]) + (pd[2] + pd[3]);
+    const float res = dt > 0.0f ? nt / dt : 0.0f;
+    out[f] = res;
+    mark[f] = (uint8_t)marked;
+    if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+}
+
 // ---------------------------------------------------------------------------
 // G2P (PIC: src/pic_simulator.cpp:439-450; FLIP: src/flip_simulator.cpp:121-161;
 // APIC: src/apic_simulator.cpp:120-176) fused with MoveParticles
@@ -398,14 +509,42 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 // ---------------------------------------------------------------------------
 // PIC sample of a and the FLIP delta sample of a - snapshot (node by node,
 // flip_simulator.cpp:121-161) on one stencil: 8 pair loads instead of 24 loads.
-__device__ inline void sample_pic_delta(const float4* __restrict__ qa, const float4* __restrict__ qs, const Geom& g,
-                                        int c, float px, float py, float pz, float& pic, float& delta) {
+// Face values for G2P: the plain arrays (four 8-byte pair loads per sample)
+// or their quad layouts (two 16-byte loads); the same values either way.
+template <bool Quads>
+struct GridSrc {
+    FaceSet a;   // plain arrays
+    QuadSet q;   // quad layouts (Quads only)
+    __device__ float sample(const Geom& g, int c, float px, float py, float pz) const {
+        if (Quads) return sample_quad(q.a[c], g, c, px, py, pz);
+        return sample_face(a.a[c], g, c, px, py, pz);
+    }
+    __device__ void pairs(const FaceStencil& t, int c, float2 (&v)[4]) const {
+        if (Quads) {
+            const float4 q0 = q.a[c][t.r00], q1 = q.a[c][t.r01];
+            v[0] = make_float2(q0.x, q0.y);
+            v[1] = make_float2(q0.z, q0.w);
+            v[2] = make_float2(q1.x, q1.y);
+            v[3] = make_float2(q1.z, q1.w);
+        } else {
+            const float* p = a.a[c];
+            v[0] = load_pair(p + t.r00);
+            v[1] = load_pair(p + t.r10);
+            v[2] = load_pair(p + t.r01);
+            v[3] = load_pair(p + t.r11);
+        }
+    }
+};
+
+template <bool Quads>
+__device__ inline void sample_pic_delta(const GridSrc<Quads>& ga, const GridSrc<Quads>& gs, const Geom& g, int c,
+                                        float px, float py, float pz, float& pic, float& delta) {
     const FaceStencil t = face_stencil(g, c, px, py, pz);
-    const float4 qa0 = qa[t.r00], qa1 = qa[t.r01], qs0 = qs[t.r00], qs1 = qs[t.r01];
-    const float2 a00 = make_float2(qa0.x, qa0.y), a10 = make_float2(qa0.z, qa0.w), a01 = make_float2(qa1.x, qa1.y),
-                 a11 = make_float2(qa1.z, qa1.w);
-    const float2 s00 = make_float2(qs0.x, qs0.y), s10 = make_float2(qs0.z, qs0.w), s01 = make_float2(qs1.x, qs1.y),
-                 s11 = make_float2(qs1.z, qs1.w);
+    float2 av[4], sv[4];
+    ga.pairs(t, c, av);
+    gs.pairs(t, c, sv);
+    const float2 a00 = av[0], a10 = av[1], a01 = av[2], a11 = av[3];
+    const float2 s00 = sv[0], s10 = sv[1], s01 = sv[2], s11 = sv[3];
     pic = trilerp_pairs(a00, a10, a01, a11, t);
     auto d = [](float2 x, float2 y) { return make_float2(x.x - y.x, x.y - y.y); };
     delta = trilerp_pairs(d(a00, s00), d(a10, s10), d(a01, s01), d(a11, s11), t);
@@ -437,10 +576,10 @@ __device__ inline void grad_contract(const float* __restrict__ a, const Geom& g,
              ((1 - fx) * fy * iz) * v6 + (fx * fy * iz) * v7;
 }
 
-template <int METHOD>
+template <int METHOD, bool Quads>
 __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_t cap, int64_t np, FaceSet grid,
-                                                     QuadSet gq, QuadSet sq, float blend, float dt, int nsub,
-                                                     int do_g2p, int do_move, uint32_t* __restrict__ key,
+                                                     GridSrc<Quads> gsrc, GridSrc<Quads> ssrc, float blend, float dt,
+                                                     int nsub, int do_g2p, int do_move, uint32_t* __restrict__ key,
                                                      int32_t* __restrict__ count) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = s < np;
@@ -452,19 +591,19 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
             float pic[3];
             if (METHOD == FE_PIC) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = sample_quad(gq.a[c], g, c, x, y, z);
+                for (int c = 0; c < 3; ++c) v[c] = gsrc.sample(g, c, x, y, z);
             } else if (METHOD == FE_FLIP) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float delta;
-                    sample_pic_delta(gq.a[c], sq.a[c], g, c, x, y, z, pic[c], delta);
+                    sample_pic_delta(gsrc, ssrc, g, c, x, y, z, pic[c], delta);
                     float fl = v[c] + delta;
                     if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
                     v[c] = fl;
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) pic[c] = sample_quad(gq.a[c], g, c, x, y, z);
+                for (int c = 0; c < 3; ++c) pic[c] = gsrc.sample(g, c, x, y, z);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = pic[c];
 #pragma unroll
@@ -490,11 +629,11 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
             for (int t = 0; t < nsub; ++t) {
                 float v0[3], mid[3], mv[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v0[c] = sample_quad(gq.a[c], g, c, p0[0], p0[1], p0[2]);
+                for (int c = 0; c < 3; ++c) v0[c] = gsrc.sample(g, c, p0[0], p0[1], p0[2]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) mid[a] = p0[a] + 0.5f * sdt * v0[a];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) mv[c] = sample_quad(gq.a[c], g, c, mid[0], mid[1], mid[2]);
+                for (int c = 0; c < 3; ++c) mv[c] = gsrc.sample(g, c, mid[0], mid[1], mid[2]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     p1[a] = p0[a] + sdt * mv[a];
@@ -624,6 +763,14 @@ static hipError_t p2g_method(fe_scene* s) {
     const Particles& P = s->P[s->cur];
     const Geom& g = s->g;
     for (int c = 0; c < 3; ++c) {
+        if (s->tune.p2g == FE_P2G_FACES) {
+            const int64_t padded = pad_blocks((face_count(g, c) + kBlock - 1) / kBlock);
+            auto kern = c == 0 ? k_p2g_face<0, METHOD> : (c == 1 ? k_p2g_face<1, METHOD> : k_p2g_face<2, METHOD>);
+            launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
+                         (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
+                         (const int32_t*)s->cell_start, s->u[c], s->mark[c], s->snap[c], padded);
+            continue;
+        }
         const int64_t tiles = (int64_t)((face_dim(g, c, 0) + 1) / 2) * ((face_dim(g, c, 1) + 1) / 2) *
                               ((face_dim(g, c, 2) + 1) / 2);
         const int64_t per_block = kBlock / kP2GLanes;
@@ -675,22 +822,34 @@ hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bo
     if (s->np == 0) return hipSuccess;
     const Particles& P = s->P[s->cur];
     FaceSet grid{{s->u[0], s->u[1], s->u[2]}};
+    FaceSet snaps{{s->snap[0], s->snap[1], s->snap[2]}};
     QuadSet gq{{s->quad[0], s->quad[1], s->quad[2]}};
     QuadSet sq{{s->squad[0], s->squad[1], s->squad[2]}};
+    const bool quads = s->tune.g2p == FE_G2P_QUADS;
     const bool flip_g2p = do_g2p && s->g.method == FE_FLIP;
     hipError_t e;
-    if ((e = launch_quads(s, s->u, s->quad)) != hipSuccess) return e;
-    if (flip_g2p && (e = launch_quads(s, s->snap, s->squad)) != hipSuccess) return e;
+    if (quads && (e = launch_quads(s, s->u, s->quad)) != hipSuccess) return e;
+    if (quads && flip_g2p && (e = launch_quads(s, s->snap, s->squad)) != hipSuccess) return e;
     if (emit_keys) {
         if ((e = launch_fill_i32(s->stream, s->cell_count, 0, cell_count(s->g) + 1)) != hipSuccess) return e;
     }
     uint32_t* key = emit_keys ? s->key : nullptr;
     int32_t* cnt = emit_keys ? s->cell_count : nullptr;
     const float blend = (float)s->desc.blend;
-    auto kern = s->g.method == FE_PIC ? k_g2p_move<FE_PIC>
-                                      : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP> : k_g2p_move<FE_APIC>);
-    launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid, gq, sq,
-                 blend, (float)dt, s->move_substeps, (int)do_g2p, (int)do_move, key, cnt);
+    if (quads) {
+        auto kern = s->g.method == FE_PIC ? k_g2p_move<FE_PIC, true>
+                                          : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP, true> : k_g2p_move<FE_APIC, true>);
+        launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid,
+                     GridSrc<true>{grid, gq}, GridSrc<true>{snaps, sq}, blend, (float)dt, s->move_substeps,
+                     (int)do_g2p, (int)do_move, key, cnt);
+    } else {
+        auto kern = s->g.method == FE_PIC
+                        ? k_g2p_move<FE_PIC, false>
+                        : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP, false> : k_g2p_move<FE_APIC, false>);
+        launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid,
+                     GridSrc<false>{grid, gq}, GridSrc<false>{snaps, sq}, blend, (float)dt, s->move_substeps,
+                     (int)do_g2p, (int)do_move, key, cnt);
+    }
     if (do_move) s->sorted = false;
     return hipGetLastError();
 }
@@ -721,6 +880,87 @@ hipError_t launch_cells_by_pid(fe_scene* s, int32_t* out) {
     hipLaunchKernelGGL(k_cells_by_pid, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->P[s->cur].pid, s->np,
                        out);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the kernel variants.  Each variant runs `reps` times between
+// HIP events on the scene's stream and the median decides.  P2G runs on the
+// sorted particles as the substep would; G2P and advection run on a copy of
+// the particles in the second particle buffer (free between sorts), so the
+// scene's particles are not changed.  The variants of a stage are
+// bit-identical, so the choice never changes results.
+// ---------------------------------------------------------------------------
+static hipError_t time_median(fe_scene* s, int reps, double* out, const std::function<hipError_t()>& prep,
+                              const std::function<hipError_t()>& run) {
+    std::vector<double> ms;
+    hipEvent_t a, b;
+    hipError_t e;
+    if ((e = hipEventCreate(&a)) != hipSuccess) return e;
+    if ((e = hipEventCreate(&b)) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return e;
+    }
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        if (prep && (e = prep()) != hipSuccess) break;
+        if ((e = hipEventRecord(a, s->stream)) != hipSuccess) break;
+        if ((e = run()) != hipSuccess) break;
+        if ((e = hipEventRecord(b, s->stream)) != hipSuccess) break;
+        if ((e = hipEventSynchronize(b)) != hipSuccess) break;
+        float t = 0.0f;
+        if ((e = hipEventElapsedTime(&t, a, b)) != hipSuccess) break;
+        ms.push_back(t);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return e;
+    std::sort(ms.begin(), ms.end());
+    *out = ms.empty() ? 0.0 : ms[ms.size() / 2];
+    return hipSuccess;
+}
+
+hipError_t autotune(fe_scene* s, int reps, fe_tuning_times* times) {
+    fe_tuning_times t{};
+    hipError_t e;
+    if (s->np == 0) {
+        if (times) *times = t;
+        return hipSuccess;
+    }
+    if (!s->sorted && (e = launch_sort(s, false)) != hipSuccess) return e;
+    const fe_tuning keep = s->tune;
+    for (int v = 0; v < 2; ++v) {
+        s->tune.p2g = v;
+        if ((e = time_median(s, reps, &t.p2g_ms[v], nullptr, [&] { return launch_p2g(s); })) != hipSuccess) break;
+    }
+    // G2P + advection on a copy of the particles in the other buffer
+    const int cur = s->cur;
+    const Particles& src = s->P[cur];
+    const Particles& dst = s->P[1 - cur];
+    auto copy = [&]() -> hipError_t {
+        hipError_t r = hipMemcpyAsync(dst.x, src.x, sizeof(float) * 3 * s->cap, hipMemcpyDeviceToDevice, s->stream);
+        if (r == hipSuccess)
+            r = hipMemcpyAsync(dst.v, src.v, sizeof(float) * 3 * s->cap, hipMemcpyDeviceToDevice, s->stream);
+        if (r == hipSuccess && src.C)
+            r = hipMemcpyAsync(dst.C, src.C, sizeof(float) * 9 * s->cap, hipMemcpyDeviceToDevice, s->stream);
+        return r;
+    };
+    const bool sorted = s->sorted;
+    s->cur = 1 - cur;
+    for (int v = 0; v < 2 && e == hipSuccess; ++v) {
+        s->tune.g2p = v;
+        e = time_median(s, reps, &t.g2p_ms[v], copy, [&] { return launch_g2p_move(s, 1.0 / 480.0, true, true, false); });
+    }
+    s->cur = cur;
+    s->sorted = sorted;
+    if (e != hipSuccess) {
+        s->tune = keep;
+        return e;
+    }
+    s->tune.p2g = t.p2g_ms[FE_P2G_FACES] < t.p2g_ms[FE_P2G_TILES] ? FE_P2G_FACES : FE_P2G_TILES;
+    s->tune.g2p = t.g2p_ms[FE_G2P_PAIRS] < t.g2p_ms[FE_G2P_QUADS] ? FE_G2P_PAIRS : FE_G2P_QUADS;
+    // leave the grid holding the chosen P2G of the current particles
+    if ((e = launch_p2g(s)) != hipSuccess) return e;
+    if (times) *times = t;
+    return hipStreamSynchronize(s->stream);
 }
 
 }  // namespace fe

@@ -286,6 +286,31 @@ int fe_scene_clear_emitters(fe_scene* scene);
 int fe_scene_reserve_particles(fe_scene* scene, int64_t capacity);
 int fe_scene_particle_capacity(const fe_scene* scene, int64_t* capacity);
 
+/* Kernel variants and their autotuner (no reference counterpart).  P2G runs
+ * as 2x2x2 face tiles (FE_P2G_TILES, four lanes per tile) or one thread per
+ * face (FE_P2G_FACES); G2P and advection sample the face arrays directly
+ * (FE_G2P_PAIRS) or from a quad layout built first (FE_G2P_QUADS).  The
+ * variants of each stage give bit-identical results, so the choice only
+ * changes speed.  fe_scene_autotune times every variant `reps` times on the
+ * scene's current particles (it sorts them if needed and leaves the face
+ * velocities, markers and FLIP snapshot holding their P2G, which the next
+ * substep recomputes anyway; the particles are untouched), keeps the fastest
+ * of each stage and reports the choice and the median times in ms (times may
+ * be NULL: p2g_ms[variant], g2p_ms[variant]).  Defaults: TILES, QUADS. */
+enum fe_p2g_variant { FE_P2G_TILES = 0, FE_P2G_FACES = 1 };
+enum fe_g2p_variant { FE_G2P_QUADS = 0, FE_G2P_PAIRS = 1 };
+typedef struct fe_tuning {
+    int32_t p2g; /* fe_p2g_variant */
+    int32_t g2p; /* fe_g2p_variant */
+} fe_tuning;
+typedef struct fe_tuning_times {
+    double p2g_ms[2];
+    double g2p_ms[2];
+} fe_tuning_times;
+int fe_scene_get_tuning(const fe_scene* scene, fe_tuning* out);
+int fe_scene_set_tuning(fe_scene* scene, const fe_tuning* tuning);
+int fe_scene_autotune(fe_scene* scene, int32_t reps, fe_tuning* out, fe_tuning_times* times);
+
 /* Learned warm start (no reference counterpart).  A 4-level convolutional
  * encoder-decoder (3x3x3 kernels, 16-64 channels, BF16 weights and activations
  * with FP32 accumulation on the matrix cores) maps a pressure system's
