@@ -225,9 +225,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __rest
 // tiles; a tile's candidate cells are the union of its faces' (4 x 4 x 3
 // cells, against 3 x 3 x 2 per face), so each candidate is read once for 8
 // faces: 6 cells read per face instead of 18.  Four lanes share a tile, lane q
-// scanning the candidate plane kc = klo + q; their partial sums are added in
-// a fixed order (q0 + q1) + (q2 + q3), so the result does not depend on
-// timing.  Cells are x-fastest and particles sorted by cell, so each (j, k)
+// scanning the candidate cells of one z plane (one y row when z is the normal
+// axis); their partial sums are added in a fixed order (q0 + q1) + (q2 + q3),
+// so the result does not depend on timing.  Cells are x-fastest and particles sorted by cell, so each (j, k)
 // row of candidate cells is one contiguous particle range.  For every
 // candidate the reference's own stencil (GetCooridnatesAndWeights with the
 // scheme's stencil size, precomputed per particle by k_gather) decides: the
@@ -339,12 +339,17 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         }
         marked |= m;
     };
-    const int kc = lo[2] + q;
-    if (kc <= hi[2]) {
+    // Lane q takes the candidate cells with coordinate lo[LA] + q on the lane
+    // axis LA: the z planes (4 of them) when z is transverse, the y rows when
+    // z is the normal axis (C = 2: 3 planes but 4 rows), so all four lanes work.
+    constexpr int LA = C == 2 ? 1 : 2, OA = 3 - LA;
+    const int lc = lo[LA] + q;
+    if (lc <= hi[LA]) {
         const float4* __restrict__ st = st4 + (int64_t)C * cap;
         const float* __restrict__ vc = V + (int64_t)C * cap;
         constexpr int kBatch = 8;  // candidates whose loads are issued together
-        for (int jc = lo[1]; jc <= hi[1]; ++jc) {
+        for (int oc = lo[OA]; oc <= hi[OA]; ++oc) {
+            const int jc = LA == 1 ? lc : oc, kc = LA == 2 ? lc : oc;
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
             int s = s0;
@@ -398,10 +403,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 // The same transfer with one thread per face (more threads, each face's
 // candidates read by the face itself): the variant that wins on small grids.
 // Its sums are bit-identical to the tile kernel's: a face's candidates form
-// per-plane partials in the tile kernel's order (the tile's extra candidates
-// add exact zeros there), and the partials of the tile's planes
-// lo, lo+1, lo+2, lo+3 (lo = max((K & ~1) - 1, 0)) are added as
-// (p0 + p1) + (p2 + p3), planes the face does not read contributing +0.
+// one partial per lane coordinate (z plane, or y row for C = 2) in the tile
+// kernel's order (the tile's extra candidates add exact zeros there), and the
+// partials of the tile's lanes lo, lo+1, lo+2, lo+3 (lo = max((F & ~1) - 1, 0)
+// on the lane axis) are added as (p0 + p1) + (p2 + p3), lanes the face does
+// not read contributing +0.
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
                                                      const float4* __restrict__ pos4, const float4* __restrict__ st4,
@@ -463,13 +469,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
     };
     const float4* __restrict__ st = st4 + (int64_t)C * cap;
     const float* __restrict__ vc = V + (int64_t)C * cap;
-    const int pl = max((K & ~1) - 1, 0);
+    constexpr int LA = C == 2 ? 1 : 2, OA = 3 - LA;  // the tile kernel's lane axis
+    const int pl = max((F[LA] & ~1) - 1, 0);
     float pn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     constexpr int kBatch = 8;  // candidates whose loads are issued together
-    for (int kc = lo[2]; kc <= hi[2]; ++kc) {
+    for (int lc = lo[LA]; lc <= hi[LA]; ++lc) {
         num = 0.0f;
         den = 0.0f;
-        for (int jc = lo[1]; jc <= hi[1]; ++jc) {
+        for (int oc = lo[OA]; oc <= hi[OA]; ++oc) {
+            const int jc = LA == 1 ? lc : oc, kc = LA == 2 ? lc : oc;
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
             int s = s0;
@@ -486,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
             }
             for (; s < s1; ++s) consider(s, st[s], vc[s]);
         }
-        const int q = kc - pl;  // 0..3
+        const int q = lc - pl;  // 0..3
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (u == q) {
