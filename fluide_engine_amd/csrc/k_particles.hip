@@ -157,6 +157,29 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__
     if (active) order[base + ((int)(threadIdx.x & 63) - r.first)] = (int32_t)p;
 }
 
+// Bins of at most N entries sort in registers: all loads first, then an
+// odd-even transposition network (N rounds of compare-exchanges at fixed
+// positions, so no register array is indexed at run time); padding entries
+// hold INT32_MAX and sort to the end.
+template <int N>
+__device__ inline void sort_bin_regs(int32_t* __restrict__ order, int s0, int n) {
+    int32_t a[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) a[u] = u < n ? order[s0 + u] : INT32_MAX;
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+#pragma unroll
+        for (int j = r & 1; j + 1 < N; j += 2) {
+            const int32_t lo = min(a[j], a[j + 1]), hi = max(a[j], a[j + 1]);
+            a[j] = lo;
+            a[j + 1] = hi;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+        if (u < n) order[s0 + u] = a[u];
+}
+
 // Atomic slots give each bin its particles in arbitrary order; sorting every
 // bin by previous index restores a deterministic, stable order.
 __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict__ start, int64_t ncells,
@@ -164,6 +187,16 @@ __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict_
     int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncells) return;
     const int s0 = start[c], s1 = start[c + 1];
+    const int n = s1 - s0;
+    if (n <= 1) return;
+    if (n <= 8) {
+        sort_bin_regs<8>(order, s0, n);
+        return;
+    }
+    if (n <= 16) {
+        sort_bin_regs<16>(order, s0, n);
+        return;
+    }
     for (int a = s0 + 1; a < s1; ++a) {
         const int32_t v = order[a];
         int b = a - 1;
