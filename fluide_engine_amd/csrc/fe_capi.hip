@@ -449,6 +449,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     // CG: 2 + 2 partials per block, multigrid: 1 more (r.z of the V-cycle)
     s->npartials = solver_shape_n(nc).blocks * 6;
     chk(dalloc(&s->partials, s->npartials));
+    chk(dalloc(&s->chunk_fluid, solver_shape_n(nc).blocks));
     chk(dalloc(&s->sc, 2));  // two state slots (CG alternates them)
     chk(hipHostMalloc((void**)&s->sc_host, sizeof(SolverScalars), hipHostMallocDefault));
     chk(dalloc(&s->cflmax, 1));
@@ -531,7 +532,7 @@ int fe_scene_destroy(fe_scene* s) {
     }
     F(s->valid[0]); F(s->valid[1]);
     F(s->sdf); F(s->flags); F(s->code); F(s->p); F(s->b); F(s->r); F(s->z); F(s->p2);
-    F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->sc); F(s->cflmax);
+    F(s->d[0]); F(s->d[1]); F(s->q); F(s->partials); F(s->chunk_fluid); F(s->sc); F(s->cflmax);
     mg_free(s);
     diffusion_free(s);
     surface_free(s);

@@ -434,6 +434,7 @@ struct fe_scene {
     float* d[2] = {nullptr, nullptr};
     float* q = nullptr;
     double* partials = nullptr;   // per-block FP64 partial sums (2 per block)
+    uint8_t* chunk_fluid = nullptr;  // per solver block: its cells include a FLUID cell (from BuildSystem)
     int64_t npartials = 0;
     fe::SolverScalars* sc = nullptr;
     fe::SolverScalars* sc_host = nullptr;  // pinned mirror

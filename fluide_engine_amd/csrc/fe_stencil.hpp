@@ -172,6 +172,7 @@ __device__ inline bool stop_rule(const SolverScalars* sc, double bnorm2, double 
 // ---------------------------------------------------------------------------
 struct Span {
     int64_t c0, c1;
+    int64_t lb;  // the block's chunk index
 };
 
 __device__ inline Span block_span(int64_t nc, int64_t chunk, int64_t padded) {
@@ -179,6 +180,7 @@ __device__ inline Span block_span(int64_t nc, int64_t chunk, int64_t padded) {
     Span s;
     s.c0 = lb * chunk;
     s.c1 = min(s.c0 + chunk, nc);
+    s.lb = lb;
     return s;
 }
 

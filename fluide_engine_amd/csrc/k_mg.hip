@@ -193,12 +193,13 @@ __device__ inline StencilIn stencil_in(int64_t c, const uint16_t* __restrict__ c
 // nullptr (the V-cycle's first kernel) and a pending confirmation, the
 // residual the CG update just recomputed as b - A x is tested first: when it
 // meets the relative stop the solve ends here (iteration count of that
-// update), without this V-cycle and the next spmv.
+// update), without this V-cycle and the next spmv.  cfl (level 0 only): the
+// solver chunks holding a FLUID cell; the others hold zeros and are skipped.
 __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
                                                       const float* __restrict__ b, const float* __restrict__ xin,
                                                       float* __restrict__ xout, const SolverScalars* st,
                                                       double* __restrict__ prz, const double* __restrict__ pconf,
-                                                      int nparts) {
+                                                      int nparts, const uint8_t* __restrict__ cfl) {
     if (st->done) return;  // uniform: the solve has stopped
     if (pconf && st->confirm) {
         double t[2];
@@ -215,6 +216,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
         }
     }
     const Span sp = block_span(L.nc, L.chunk, L.padded);
+    if (cfl && !cfl[sp.lb]) {  // level 0 chunk without FLUID cells: x stays 0
+        if (prz && threadIdx.x == 0) prz[blockIdx.x] = 0.0;
+        return;
+    }
     const Stride stv = L.st;
     double v[1] = {0.0};
     auto X = [&](int64_t n) { return xin[n]; };
@@ -232,9 +237,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
 
 __global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint16_t* __restrict__ code,
                                                         const float* __restrict__ b, const float* __restrict__ x,
-                                                        float* __restrict__ r, const SolverScalars* __restrict__ st) {
+                                                        float* __restrict__ r, const SolverScalars* __restrict__ st,
+                                                        const uint8_t* __restrict__ cfl) {
     if (st->done) return;
     const Span sp = block_span(L.nc, L.chunk, L.padded);
+    if (cfl && !cfl[sp.lb]) return;  // level 0 chunk without FLUID cells: r stays 0
     const Stride stv = L.st;
     auto X = [&](int64_t n) { return x[n]; };
     span_groups<8, StencilIn>(
@@ -645,6 +652,7 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
     const int L = (int)lv.size();
     const int T = s->mg_tail;
     lv[0].b = const_cast<float*>(r);
+    const uint8_t* cfl0 = s->chunk_fluid;
     for (int q = 0; q < T; ++q) {
         MGLevel& l = lv[q];
         MGLevel& c = lv[q + 1];
@@ -652,13 +660,13 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
         const dim3 grid((unsigned)a.padded);
         if (q == 0)
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, pconf, nparts);
+                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, pconf, nparts, cfl0);
         else
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x, l.x2, (const SolverScalars*)st, (double*)nullptr,
-                               (const double*)nullptr, 0);
+                               (const double*)nullptr, 0, (const uint8_t*)nullptr);
         hipLaunchKernelGGL(k_mg_residual, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
-                           (const float*)l.b, (const float*)l.x2, l.r, st);
+                           (const float*)l.b, (const float*)l.x2, l.r, st, q == 0 ? cfl0 : (const uint8_t*)nullptr);
         const dim3 rgrid((unsigned)((c.n[0] + kRtx - 1) / kRtx), (unsigned)((c.n[1] + kRty - 1) / kRty),
                          (unsigned)((c.n[2] + kRtz - 1) / kRtz));
         hipLaunchKernelGGL(k_mg_restrict, rgrid, dim3(kBlock), 0, s->stream, (const float*)l.r, l.n[0],
@@ -691,14 +699,17 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
                            c.n[1], c.n[2], (const uint16_t*)l.code, l.x2, l.n[0], l.n[1], l.n[2], st);
         if (q == 0) {
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0);
+                         (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0,
+                         cfl0);
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x, z, st, prz, (const double*)nullptr, 0);
+                         (const float*)l.b, (const float*)l.x, z, st, prz, (const double*)nullptr, 0, cfl0);
         } else {
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
-                               (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0);
+                               (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0,
+                               (const uint8_t*)nullptr);
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
-                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, (const double*)nullptr, 0);
+                               (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, (const double*)nullptr, 0,
+                               (const uint8_t*)nullptr);
         }
     }
     return hipGetLastError();

@@ -32,12 +32,14 @@ namespace fe {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_build_system(Geom g, FaceSet u, const uint8_t* __restrict__ flags,
                                                          uint16_t* __restrict__ code, float* __restrict__ b,
-                                                         double* __restrict__ pout, int64_t chunk, int64_t padded) {
+                                                         double* __restrict__ pout, uint8_t* __restrict__ cfl,
+                                                         int64_t chunk, int64_t padded) {
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1], nz = (uint32_t)g.n[2];
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double v[1] = {0.0};
+    int fluid = 0;
     for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
         float bb = 0.0f;
         uint16_t cd = 0;
@@ -47,6 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_build_system(Geom g, FaceSet u, cons
             const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
             const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
             cd = kFluidBit;
+            fluid = 1;
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 if (!in[q]) continue;
@@ -66,6 +69,10 @@ __global__ __launch_bounds__(kBlock) void k_build_system(Geom g, FaceSet u, cons
         b[c] = bb;
         v[0] += (double)bb * (double)bb;
     }
+    // chunks without FLUID cells hold zeros in every CG and level-0 multigrid
+    // vector throughout the solve, so those kernels skip them
+    fluid = __syncthreads_or(fluid);
+    if (threadIdx.x == 0) cfl[sp.lb] = (uint8_t)(fluid != 0);
     write_partials<1>(v, pout);
 }
 
@@ -198,7 +205,7 @@ __device__ inline Stencil64In stencil64_in(int64_t c, const uint16_t* __restrict
 // ---------------------------------------------------------------------------
 constexpr double kReliableDrop2 = 1e-4;  // (100x in ||r||)^2
 __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __restrict__ code,
-                                                    const float* __restrict__ b, const double* __restrict__ x,
+                                                    const float* __restrict__ b, double* __restrict__ x,
                                                     float* __restrict__ r, float* __restrict__ z,
                                                     float* __restrict__ d0, float* __restrict__ mgx, SolverScalars* sc,
                                                     const double* __restrict__ pbb, double* __restrict__ pout,
@@ -217,7 +224,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
     span_groups<4, Stencil64In>(
         sp.c0, sp.c1, [&](int64_t c) { return stencil64_in(c, code, x, b, st); },
         [&](int64_t c, const Stencil64In& t) {
-            const float rc = (float)((double)t.bc - code_apply64_nb(t.cd, ih2d, t.xc, t.nb));
+            // the identity rows of non-FLUID cells have the solution 0: the
+            // start takes it there (FLUID rows couple only to FLUID cells), so
+            // r is 0 outside FLUID from the start and stays so
+            const bool fl = (t.cd & kFluidBit) != 0;
+            if (!fl) x[c] = 0.0;
+            const float rc = fl ? (float)((double)t.bc - code_apply64_nb(t.cd, ih2d, t.xc, t.nb)) : 0.0f;
             r[c] = rc;
             float zc = rc;
             if (pre == 1) {
@@ -240,8 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
                                                     const SolverScalars* __restrict__ in,
                                                     SolverScalars* __restrict__ out, const double* __restrict__ pin,
                                                     const double* __restrict__ pz, double* __restrict__ pout,
-                                                    int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
-                                                    int64_t padded) {
+                                                    int nparts, float ihx2, float ihy2, float ihz2,
+                                                    const uint8_t* __restrict__ cfl, int64_t chunk, int64_t padded) {
     if (in->done) {  // uniform: an earlier spmv stopped the solve
         if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
         return;
@@ -298,6 +310,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     }
     if (done) return;
     // ---- body
+    if (cfl && !cfl[block_span(cell_count(g), chunk, padded).lb]) {  // no FLUID cell: d, q stay 0
+        if (threadIdx.x == 0) pout[blockIdx.x] = 0.0;
+        return;
+    }
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
@@ -334,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
                                                       float* __restrict__ z, float* __restrict__ mgx, SolverScalars* sc,
                                                       const double* __restrict__ pin, double* __restrict__ pout,
                                                       int nparts, float ihx2, float ihy2, float ihz2,
-                                                      int64_t chunk, int64_t padded) {
+                                                      const uint8_t* __restrict__ cfl, int64_t chunk, int64_t padded) {
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     if (sc->done) {  // uniform: keep the x rotation consistent
@@ -347,6 +363,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->dq = t[0];
         sc->alpha = alpha;
+    }
+    if (cfl && !cfl[sp.lb]) {  // no FLUID cell: x, r, z and the first sweep stay 0
+        if (threadIdx.x == 0) pout[2 * blockIdx.x] = pout[2 * blockIdx.x + 1] = 0.0;
+        return;
     }
     const bool replace = sc->replace != 0;
     const int pre = sc->precond;
@@ -489,7 +509,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         return e;
     FaceSet u{{s->u[0], s->u[1], s->u[2]}};
     hipLaunchKernelGGL(k_build_system, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->code,
-                       s->b, PA, sh.chunk, sh.blocks);
+                       s->b, PA, s->chunk_fluid, sh.chunk, sh.blocks);
     if (d->warm_start == FE_WARM_START_LEARNED) {
         if (!s->net) return hipErrorInvalidValue;
         if ((e = net_warm_start(s->net, s)) != hipSuccess) return e;
@@ -538,6 +558,17 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
 
     // CG / PCG
     if (mg && (e = mg_setup(s)) != hipSuccess) return e;
+    // The CG and level-0 multigrid kernels skip the chunks without FLUID cells
+    // (s->chunk_fluid): every vector they would write there starts at 0 and,
+    // FLUID rows coupling only to FLUID cells, stays 0 (k_cg_init writes r, d,
+    // the first sweep and the solution's zeros outside FLUID itself).
+    {
+        float* zero32[4] = {s->z, s->q, s->d[1], mg ? s->mg[0].x2 : nullptr};
+        for (float* a : zero32)
+            if (a && (e = launch_fill_f32(s->stream, a, 0.0f, nc)) != hipSuccess) return e;
+        if (mg && (e = launch_fill_f32(s->stream, s->mg[0].r, 0.0f, nc)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p2, nc);
+    }
     float* mgx = mg ? mg_first_sweep_target(s) : nullptr;
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
                        s->r, s->z, s->d[0], mgx, s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
@@ -563,7 +594,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         launch_timed(s, FE_KERNEL_CG_UPDATE, kk, k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                      (const uint16_t*)s->code, (const float*)s->b, (const double*)xa, xb, (const float*)s->d[dcur],
                      (const float*)s->q, s->r, s->z, mgx, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
-                     ih2[1], ih2[2], sh.chunk, sh.blocks);
+                     ih2[1], ih2[2], (const uint8_t*)s->chunk_fluid, sh.chunk, sh.blocks);
         std::swap(xa, xb);
         if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk, PB, nparts);
     };
@@ -574,7 +605,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
             launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                          (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
                          (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, pz,
-                         PA, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+                         PA, nparts, ih2[0], ih2[1], ih2[2], (const uint8_t*)s->chunk_fluid, sh.chunk, sh.blocks);
             if (t == nb - 1) break;
             update(k);
             ++k;
