@@ -455,7 +455,10 @@ __global__ __launch_bounds__(kBlock) void k_apply_pressure(Geom g, FaceSet a, co
         const int64_t cr = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);  // cell on the + side
         const int64_t step = c == 0 ? 1 : (c == 1 ? (int64_t)g.n[0] : (int64_t)g.n[0] * g.n[1]);
         const int64_t cl = cr - step;
-        if (flags[cl] == FE_FLUID && flags[cr] != FE_BOUNDRY) v = (float)((double)v + g.ihd[c] * (p[cr] - p[cl]));
+        // the pressures are read with the markers (both cells exist), not after them
+        const uint8_t fl = flags[cl], fr = flags[cr];
+        const double pr = p[cr], pl = p[cl];
+        if (fl == FE_FLUID && fr != FE_BOUNDRY) v = (float)((double)v + g.ihd[c] * (pr - pl));
     }
     if (walls && (q == 0 || q == g.n[c])) v = 0.0f;
     a.a[c][f] = v;
@@ -478,18 +481,37 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, un
     __shared__ double smem[kBlock / 64];
     double m = 0.0;
     const int64_t nc = cell_count(g);
-    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
-        int i, j, k;
-        split_index(c, g.n[0], g.n[1], i, j, k);
-        const int64_t iu = (int64_t)i + (int64_t)(g.n[0] + 1) * ((int64_t)j + (int64_t)g.n[1] * k);
-        const int64_t iv = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)(g.n[1] + 1) * k);
-        const int64_t iw = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);
-        const double cx = 0.5 * ((double)a.a[0][iu] + (double)a.a[0][iu + 1]);
-        const double cy = 0.5 * ((double)a.a[1][iv] + (double)a.a[1][iv + g.n[0]]);
-        const double cz = 0.5 * ((double)a.a[2][iw] + (double)a.a[2][iw + (int64_t)g.n[0] * g.n[1]]);
-        m = fmax(m, fabs(cx) + dt * 0.0);
-        m = fmax(m, fabs(cy) + dt * -9.8);
-        m = fmax(m, fabs(cz) + dt * 0.0);
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    // four cells per step, all their face loads issued before any is used
+    constexpr int kG = 4;
+    for (int64_t c0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; c0 < nc; c0 += kG * stride) {
+        float f[kG][6];
+#pragma unroll
+        for (int u = 0; u < kG; ++u) {
+            const int64_t c = c0 + u * stride;
+            if (c >= nc) continue;
+            int i, j, k;
+            split_index(c, g.n[0], g.n[1], i, j, k);
+            const int64_t iu = (int64_t)i + (int64_t)(g.n[0] + 1) * ((int64_t)j + (int64_t)g.n[1] * k);
+            const int64_t iv = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)(g.n[1] + 1) * k);
+            const int64_t iw = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);
+            f[u][0] = a.a[0][iu];
+            f[u][1] = a.a[0][iu + 1];
+            f[u][2] = a.a[1][iv];
+            f[u][3] = a.a[1][iv + g.n[0]];
+            f[u][4] = a.a[2][iw];
+            f[u][5] = a.a[2][iw + (int64_t)g.n[0] * g.n[1]];
+        }
+#pragma unroll
+        for (int u = 0; u < kG; ++u) {
+            if (c0 + u * stride >= nc) continue;
+            const double cx = 0.5 * ((double)f[u][0] + (double)f[u][1]);
+            const double cy = 0.5 * ((double)f[u][2] + (double)f[u][3]);
+            const double cz = 0.5 * ((double)f[u][4] + (double)f[u][5]);
+            m = fmax(m, fabs(cx) + dt * 0.0);
+            m = fmax(m, fabs(cy) + dt * -9.8);
+            m = fmax(m, fabs(cz) + dt * 0.0);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
     if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = m;
