@@ -114,10 +114,10 @@ def parse():
 KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
                   "g2p_move": "k_g2p_move", "sdf": "k_sdf"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
-TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_kt -o kt -- "
+TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d rocprof/kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
              "--batch-frames 0 --learned-steps 0")
-PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{counter} -o pmc -- "
+PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d rocprof/pmc_{counter} -o pmc -- "
            "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --no-solver-comparison "
            "--batch-frames 0 --learned-steps 0 --copy-calibration 256")
 
