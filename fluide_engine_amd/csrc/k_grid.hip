@@ -384,42 +384,57 @@ hipError_t launch_walls(fe_scene* s) {
 // ExternalForce::ApplyExternalForce with a DirectionalField (src/forces/external_force.cpp:4-30):
 // u += dt*g on every face, followed (optionally) by the wall layers of the
 // ApplyBoundryCondition that ComputeExternalForces runs next (src/pic_simulator.cpp:203-212).
+// Four consecutive faces of one component per thread, as one 16-byte vector
+// (face arrays are separate allocations, so f0 = 4t is 16-byte aligned).
+// Blocks [0, b0) take u, [b0, b1) v, [b1, ...) w.
 __global__ __launch_bounds__(kBlock) void k_gravity(Geom g, FaceSet a, float gx, float gy, float gz, int walls,
-                                                    int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    int64_t f = blk * kBlock + threadIdx.x;
-    int c = 0;
-    int64_t nf = face_count(g, 0);
-    if (f >= nf) {
-        f -= nf;
-        c = 1;
-        nf = face_count(g, 1);
-        if (f >= nf) {
-            f -= nf;
-            c = 2;
-            nf = face_count(g, 2);
-            if (f >= nf) return;
+                                                    int64_t b0, int64_t b1) {
+    const int64_t blk = blockIdx.x;
+    const int c = blk < b0 ? 0 : (blk < b1 ? 1 : 2);
+    const int64_t f0 = 4 * ((blk - (c == 0 ? 0 : (c == 1 ? b0 : b1))) * kBlock + threadIdx.x);
+    const int64_t nf = face_count(g, c);
+    if (f0 >= nf) return;
+    float* __restrict__ A = c == 0 ? a.a[0] : (c == 1 ? a.a[1] : a.a[2]);
+    const float add = c == 0 ? gx : (c == 1 ? gy : gz);
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
+    // index of face f on the component's normal axis (the wall layers are 0 and n)
+    auto normal_index = [&](int64_t f) -> int {
+        const uint32_t u = (uint32_t)f;
+        return c == 0 ? (int)(u % (uint32_t)sx)
+                      : (c == 1 ? (int)((u / (uint32_t)sx) % (uint32_t)sy) : (int)(u / ((uint32_t)sx * sy)));
+    };
+    if (f0 + 4 <= nf) {
+        float4 v = *(const float4*)(A + f0);
+        float r[4] = {v.x + add, v.y + add, v.z + add, v.w + add};
+        if (walls) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int nq = normal_index(f0 + q);
+                if (nq == 0 || nq == g.n[c]) r[q] = 0.0f;
+            }
+        }
+        *(float4*)(A + f0) = make_float4(r[0], r[1], r[2], r[3]);
+    } else {
+        for (int64_t f = f0; f < nf; ++f) {
+            float v = A[f] + add;
+            if (walls) {
+                const int nq = normal_index(f);
+                if (nq == 0 || nq == g.n[c]) v = 0.0f;
+            }
+            A[f] = v;
         }
     }
-    const float add = c == 0 ? gx : (c == 1 ? gy : gz);
-    float v = a.a[c][f] + add;
-    if (walls) {
-        const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
-        int i, j, k;
-        split_index(f, sx, sy, i, j, k);
-        if (is_wall(g, c, i, j, k)) v = 0.0f;
-    }
-    a.a[c][f] = v;
 }
 
 hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after) {
     const Geom& g = s->g;
-    const int64_t tot = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
-    const int64_t padded = pad_blocks((tot + kBlock - 1) / kBlock);
+    const int64_t per = 4 * (int64_t)kBlock;  // faces per block
+    const int64_t nb0 = (face_count(g, 0) + per - 1) / per, nb1 = (face_count(g, 1) + per - 1) / per,
+                  nb2 = (face_count(g, 2) + per - 1) / per;
     FaceSet a{{s->u[0], s->u[1], s->u[2]}};
-    hipLaunchKernelGGL(k_gravity, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, a,
+    hipLaunchKernelGGL(k_gravity, dim3((unsigned)(nb0 + nb1 + nb2)), dim3(kBlock), 0, s->stream, g, a,
                        (float)(dt * s->desc.gravity[0]), (float)(dt * s->desc.gravity[1]),
-                       (float)(dt * s->desc.gravity[2]), walls_after ? 1 : 0, padded);
+                       (float)(dt * s->desc.gravity[2]), walls_after ? 1 : 0, nb0, nb0 + nb1);
     return hipGetLastError();
 }
 
