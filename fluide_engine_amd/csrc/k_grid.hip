@@ -443,49 +443,52 @@ hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after) {
 // in range and not BOUNDRY: u(i+1) += (p(i+1) - p(i)) / h.  Optionally fused
 // with the wall layers of the ApplyBoundryCondition that follows
 // (src/pic_simulator.cpp:224-229).
+// Four consecutive faces of one component per thread (u read and written as
+// one 16-byte vector; face arrays are separate allocations).  Blocks [0, b0)
+// take u, [b0, b1) v, [b1, ...) w.
 __global__ __launch_bounds__(kBlock) void k_apply_pressure(Geom g, FaceSet a, const double* __restrict__ p,
-                                                           const uint8_t* __restrict__ flags, int walls,
-                                                           int64_t padded) {
-    const int64_t blk = xcd_remap(blockIdx.x, padded);
-    int64_t f = blk * kBlock + threadIdx.x;
-    int c = 0;
-    int64_t nf = face_count(g, 0);
-    if (f >= nf) {
-        f -= nf;
-        c = 1;
-        nf = face_count(g, 1);
-        if (f >= nf) {
-            f -= nf;
-            c = 2;
-            nf = face_count(g, 2);
-            if (f >= nf) return;
-        }
-    }
+                                                           const uint8_t* __restrict__ flags, int walls, int64_t b0,
+                                                           int64_t b1) {
+    const int64_t blk = blockIdx.x;
+    const int c = blk < b0 ? 0 : (blk < b1 ? 1 : 2);
+    const int64_t f0 = 4 * ((blk - (c == 0 ? 0 : (c == 1 ? b0 : b1))) * kBlock + threadIdx.x);
+    const int64_t nf = face_count(g, c);
+    if (f0 >= nf) return;
+    float* __restrict__ A = c == 0 ? a.a[0] : (c == 1 ? a.a[1] : a.a[2]);
     const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
-    int i, j, k;
-    split_index(f, sx, sy, i, j, k);
-    const int q = c == 0 ? i : (c == 1 ? j : k);
-    float v = a.a[c][f];
-    if (q >= 1 && q <= g.n[c] - 1) {
-        const int64_t cr = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);  // cell on the + side
-        const int64_t step = c == 0 ? 1 : (c == 1 ? (int64_t)g.n[0] : (int64_t)g.n[0] * g.n[1]);
-        const int64_t cl = cr - step;
-        // the pressures are read with the markers (both cells exist), not after them
-        const uint8_t fl = flags[cl], fr = flags[cr];
-        const double pr = p[cr], pl = p[cl];
-        if (fl == FE_FLUID && fr != FE_BOUNDRY) v = (float)((double)v + g.ihd[c] * (pr - pl));
+    const int64_t step = c == 0 ? 1 : (c == 1 ? (int64_t)g.n[0] : (int64_t)g.n[0] * g.n[1]);
+    const double ih = g.ihd[c];
+    auto face = [&](int64_t f, float v) {
+        int i, j, k;
+        split_index(f, sx, sy, i, j, k);
+        const int q = c == 0 ? i : (c == 1 ? j : k);
+        if (q >= 1 && q <= g.n[c] - 1) {
+            const int64_t cr = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);  // + side cell
+            const int64_t cl = cr - step;
+            // the pressures are read with the markers (both cells exist), not after them
+            const uint8_t fl = flags[cl], fr = flags[cr];
+            const double pr = p[cr], pl = p[cl];
+            if (fl == FE_FLUID && fr != FE_BOUNDRY) v = (float)((double)v + ih * (pr - pl));
+        }
+        if (walls && (q == 0 || q == g.n[c])) v = 0.0f;
+        return v;
+    };
+    if (f0 + 4 <= nf) {
+        const float4 v = *(const float4*)(A + f0);
+        *(float4*)(A + f0) = make_float4(face(f0, v.x), face(f0 + 1, v.y), face(f0 + 2, v.z), face(f0 + 3, v.w));
+    } else {
+        for (int64_t f = f0; f < nf; ++f) A[f] = face(f, A[f]);
     }
-    if (walls && (q == 0 || q == g.n[c])) v = 0.0f;
-    a.a[c][f] = v;
 }
 
 hipError_t launch_apply_pressure(fe_scene* s, bool walls_after) {
     const Geom& g = s->g;
-    const int64_t tot = face_count(g, 0) + face_count(g, 1) + face_count(g, 2);
-    const int64_t padded = pad_blocks((tot + kBlock - 1) / kBlock);
+    const int64_t per = 4 * (int64_t)kBlock;  // faces per block
+    const int64_t nb0 = (face_count(g, 0) + per - 1) / per, nb1 = (face_count(g, 1) + per - 1) / per,
+                  nb2 = (face_count(g, 2) + per - 1) / per;
     FaceSet a{{s->u[0], s->u[1], s->u[2]}};
-    hipLaunchKernelGGL(k_apply_pressure, dim3((unsigned)padded), dim3(kBlock), 0, s->stream, g, a, s->p, s->flags,
-                       walls_after ? 1 : 0, padded);
+    hipLaunchKernelGGL(k_apply_pressure, dim3((unsigned)(nb0 + nb1 + nb2)), dim3(kBlock), 0, s->stream, g, a, s->p,
+                       s->flags, walls_after ? 1 : 0, nb0, nb0 + nb1);
     return hipGetLastError();
 }
 
