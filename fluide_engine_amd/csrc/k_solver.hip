@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
             // start takes it there (FLUID rows couple only to FLUID cells), so
             // r is 0 outside FLUID from the start and stays so
             const bool fl = (t.cd & kFluidBit) != 0;
-            if (!fl) x[c] = 0.0;
+            if (!fl && t.xc != 0.0) x[c] = 0.0;  // written only where it is not 0 already
             const float rc = fl ? (float)((double)t.bc - code_apply64_nb(t.cd, ih2d, t.xc, t.nb)) : 0.0f;
             r[c] = rc;
             float zc = rc;
