@@ -200,7 +200,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
                                                       float* __restrict__ xout, const SolverScalars* st,
                                                       double* __restrict__ prz, const double* __restrict__ pconf,
                                                       int nparts, const uint8_t* __restrict__ cfl) {
-    if (st->done) return;  // uniform: the solve has stopped
+    // The stop flag is read with the data and tested before the first write,
+    // so its load adds no latency of its own (the loads before it are in bounds).
+    const int done = st->done;
+    if (pconf && done) return;
     if (pconf && st->confirm) {
         double t[2];
         sum_partials<2>(pconf, nparts, t);
@@ -226,12 +229,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
     span_groups<8, StencilIn>(
         sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, xin, b, stv, X); },
         [&](int64_t c, const StencilIn& t) {
+            if (done) return;  // uniform: the solve has stopped
             const float dgn = code_center(t.cd, L.ih2);
             const float w = mg_weight(t.cd, L.ih2);
             const float xo = t.xc + w * (t.bc - code_apply_nb(t.cd, dgn, L.ih2, t.xc, t.nb));
             xout[c] = xo;
             v[0] += (double)t.bc * (double)xo;
         });
+    if (done) return;
     if (prz) write_partials<1>(v, prz);
 }
 
@@ -239,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint1
                                                         const float* __restrict__ b, const float* __restrict__ x,
                                                         float* __restrict__ r, const SolverScalars* __restrict__ st,
                                                         const uint8_t* __restrict__ cfl) {
-    if (st->done) return;
+    const int done = st->done;  // tested before the first write (see k_mg_smooth)
     const Span sp = block_span(L.nc, L.chunk, L.padded);
     if (cfl && !cfl[sp.lb]) return;  // level 0 chunk without FLUID cells: r stays 0
     const Stride stv = L.st;
@@ -247,6 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint1
     span_groups<8, StencilIn>(
         sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, x, b, stv, X); },
         [&](int64_t c, const StencilIn& t) {
+            if (done) return;
             float rc = 0.0f;
             if (t.cd & kFluidBit) rc = t.bc - code_apply_nb(t.cd, code_center(t.cd, L.ih2), L.ih2, t.xc, t.nb);
             r[c] = rc;
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
                                                         float* __restrict__ x1, int cx, int cy, int cz, float cihx2,
                                                         float cihy2, float cihz2,
                                                         const SolverScalars* __restrict__ st) {
-    if (st->done) return;
+    const int done = st->done;  // tested before the first write (see k_mg_smooth)
     __shared__ float win[kRwz][kRwy][kRwx];
     const int I0 = blockIdx.x * kRtx, J0 = blockIdx.y * kRty, K0 = blockIdx.z * kRtz;
     const int fi0 = 2 * I0 - 1, fj0 = 2 * J0 - 1, fk0 = 2 * K0 - 1;
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
         if (e < kWin) (&win[0][0][0])[e] = wv[u];
     }
     __syncthreads();
-    if (!inside) return;
+    if (done || !inside) return;
     float v = 0.0f;
     if (cd & kFluidBit) {
         // window coordinates of fine index 2I-1+t are 2*tx+t (clamped taps carry weight 0)
@@ -329,14 +335,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
 __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__ xc, int cx, int cy, int cz,
                                                        const uint16_t* __restrict__ fcode, float* __restrict__ xf,
                                                        int fx, int fy, int fz, const SolverScalars* __restrict__ st) {
-    if (st->done) return;
+    const int done = st->done;  // tested before the write (see k_mg_smooth)
     const int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t nf = (int64_t)fx * fy * fz;
     if (f >= nf) return;
-    if (!(fcode[f] & kFluidBit)) return;
+    // the marker, the fine value and the coarse values are loaded together
+    const uint16_t cd = fcode[f];
+    const float x0 = xf[f];
     int i, j, k;
     cell_ijk(f, fx, fy, i, j, k);
-    xf[f] += prolong_at(i, j, k, cx, cy, cz, [&](int64_t n) { return xc[n]; });
+    const float add = prolong_at(i, j, k, cx, cy, cz, [&](int64_t n) { return xc[n]; });
+    if (done || !(cd & kFluidBit)) return;
+    xf[f] = x0 + add;
 }
 
 // ---------------------------------------------------------------------------
