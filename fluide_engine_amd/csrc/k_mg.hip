@@ -410,7 +410,7 @@ __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, cons
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bT,
                                                           float* __restrict__ xT,
                                                           const SolverScalars* __restrict__ st) {
-    if (st->done) return;
+    const int done = st->done;  // tested after the loads below (see k_mg_smooth)
     __shared__ float X[kTailLds], Tm[kTailLds], B[kTailLds], Dg[kTailLds], Wt[kTailLds];
     __shared__ uint16_t Cd[kTailLds];
     // Row codes and the right-hand side: every load is issued before the
@@ -450,6 +450,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
         }
     }
     __syncthreads();
+    if (done) return;
     // per cell: the diagonal and the damped-Jacobi weight (the coarsest level
     // with its own damping)
     for (int t = 0; t < a.levels; ++t) {
