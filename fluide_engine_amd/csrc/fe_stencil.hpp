@@ -229,6 +229,19 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nparts,
     block_reduce<NV>(tot, sm);
 }
 
+// Two sums in one pass: tot[0] over a[sa * b + oa], tot[1] over c[sc * b + oc]
+// (b < nparts), each in sum_partials' order, so the values are the same.
+__device__ inline void sum_partials_two(const double* __restrict__ a, int sa, int oa, const double* __restrict__ c,
+                                        int sc, int oc, int nparts, double (&tot)[2]) {
+    __shared__ double sm[2][kBlock / 64];
+    tot[0] = tot[1] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+        tot[0] += a[(int64_t)b * sa + oa];
+        tot[1] += c[(int64_t)b * sc + oc];
+    }
+    block_reduce<2>(tot, sm);
+}
+
 // Row code of cell (i,j,k) from the markers with BuildSystem's rules
 // (src/fluid_solvers/single_phase_pressure_solver.cpp:116-161).
 __device__ inline uint16_t row_code(const uint8_t* __restrict__ flags, int64_t c, uint32_t i, uint32_t j, uint32_t k,

@@ -261,11 +261,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     // ---- state update from the last update's r.z and r.r (every block alike);
     // with the multigrid preconditioner r.z comes from the V-cycle's last sweep
     double t[2];
-    sum_partials<2>(pin, nparts, t);
-    if (pz) {
-        double tz[1];
-        sum_partials<1>(pz, nparts, tz);
-        t[0] = tz[0];
+    if (pz) {  // r.z from the V-cycle's last sweep, r.r from the update: one pass
+        double u[2];
+        sum_partials_two(pz, 1, 0, pin, 2, 1, nparts, u);
+        t[0] = u[0];
+        t[1] = u[1];
+    } else {
+        sum_partials<2>(pin, nparts, t);
     }
     const int it_prev = in->it;
     double beta, rr_ref;
