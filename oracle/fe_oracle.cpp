@@ -283,6 +283,7 @@ struct orc_scene {
     double gravity[3];
     int nthreads = 1;
     int has_collider = 0;
+    int planted = 0;  // ORC_PLANT_*: a deliberate restatement mistake, set only by the pinning tests
     double xo[3], yo[3], zo[3];  // face data origins, src/grid_systems/face_centered_grid3d.cpp:324-332
 
     Arr3 u, v, w;         // face velocities
@@ -445,7 +446,10 @@ void g2p(orc_scene* s) {
         for (int c = 0; c < 3; ++c) {  // delta = new - snapshot (:138-150)
             Arr3& a = comp_arr(s, c);
             Arr3& r = comp_res(s, c);
-            for (size_t i = 0; i < a.size(); ++i) r.d[i] = a.d[i] - r.d[i];
+            if (s->planted == ORC_PLANT_FLIP_DELTA_SIGN)
+                for (size_t i = 0; i < a.size(); ++i) r.d[i] = r.d[i] - a.d[i];
+            else
+                for (size_t i = 0; i < a.size(); ++i) r.d[i] = a.d[i] - r.d[i];
         }
     }
     if (s->method == ORC_APIC) s->C.assign(9 * N, 0.0);  // :129-134
@@ -477,13 +481,18 @@ void g2p(orc_scene* s) {
                         if (ax != c)
                             pc[ax] = clamp_ref(pc[ax], s->origin[ax] + s->size[ax] - hh[ax], s->origin[ax] + hh[ax]);
                     const Arr3& a = comp_arr(s, c);
-                    coords_grad_weights(cell, comp_origin(s, c), s->h, pc, &gs);
+                    int dims[3] = {cell[0], cell[1], cell[2]};
+                    if (s->planted == ORC_PLANT_APIC_FACE_DIMS) dims[c] += 1;
+                    coords_grad_weights(dims, comp_origin(s, c), s->h, pc, &gs);
+                    // row c of C holds the gradient of component c (_cX, _cY, _cZ)
                     double* cr = &s->C[9 * pi + 3 * c];
+                    const int64_t st = s->planted == ORC_PLANT_APIC_TRANSPOSE ? 3 : 1;
+                    if (st == 3) cr = &s->C[9 * pi + c];
                     for (int q = 0; q < 8; ++q) {
                         double val = a(gs.idx[q][0], gs.idx[q][1], gs.idx[q][2]);
                         cr[0] += gs.g[q][0] * val;
-                        cr[1] += gs.g[q][1] * val;
-                        cr[2] += gs.g[q][2] * val;
+                        cr[st] += gs.g[q][1] * val;
+                        cr[2 * st] += gs.g[q][2] * val;
                     }
                 }
             }
@@ -509,7 +518,10 @@ void move_particles(orc_scene* s, double dt_total) {
                 sample_vel(s, s->u, s->v, s->w, pt0, v0);
                 for (int a = 0; a < 3; ++a) mid[a] = pt0[a] + 0.5 * dt * v0[a];
                 sample_vel(s, s->u, s->v, s->w, mid, mv);
-                for (int a = 0; a < 3; ++a) pt1[a] = pt0[a] + dt * mv[a];
+                if (s->planted == ORC_PLANT_EULER_ADVECTION)
+                    for (int a = 0; a < 3; ++a) pt1[a] = pt0[a] + dt * v0[a];
+                else
+                    for (int a = 0; a < 3; ++a) pt1[a] = pt0[a] + dt * mv[a];
                 for (int a = 0; a < 3; ++a) pt0[a] = pt1[a];
             }
             for (int a = 0; a < 3; ++a) {
@@ -1745,6 +1757,8 @@ orc_scene* orc_create(const int n[3], const double origin[3], const double size[
 
 void orc_destroy(orc_scene* s) { delete s; }
 void orc_set_threads(orc_scene* s, int nthreads) { s->nthreads = std::max(1, nthreads); }
+
+void orc_plant_mistake(orc_scene* s, int which) { s->planted = which; }
 
 void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np) {
     s->np = np;

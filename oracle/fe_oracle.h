@@ -44,6 +44,16 @@ orc_scene* orc_create(const int n[3], const double origin[3], const double size[
                       double max_cfl, const double gravity[3]);
 void orc_destroy(orc_scene* s);
 void orc_set_threads(orc_scene* s, int nthreads);
+// Pinning tests only: switch on one deliberate restatement mistake so that a
+// test can show its check fails on it.  0 (the default) is the oracle proper.
+enum {
+    ORC_PLANT_NONE = 0,
+    ORC_PLANT_APIC_TRANSPOSE = 1,    // G2P writes the affine matrix C transposed
+    ORC_PLANT_FLIP_DELTA_SIGN = 2,   // FLIP delta taken as snapshot - new
+    ORC_PLANT_EULER_ADVECTION = 3,   // MoveParticles steps with forward Euler
+    ORC_PLANT_APIC_FACE_DIMS = 4     // APIC gradient stencil sized by the face array on the normal axis
+};
+void orc_plant_mistake(orc_scene* s, int which);
 void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np);
 void orc_get_particles(const orc_scene* s, double* x, double* v, double* C);
 int64_t orc_num_particles(const orc_scene* s);

@@ -20,6 +20,8 @@ PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
 U, V, W, MU, MV, MW, RU, RV, RW, SDF, FLAGS, PRESSURE, RHS, COLLIDER_SDF = range(14)
 JACOBI, CG, PCG_JACOBI = 0, 1, 2
+# deliberate restatement mistakes for the pinning tests (fe_oracle.h ORC_PLANT_*)
+PLANT_NONE, PLANT_APIC_TRANSPOSE, PLANT_FLIP_DELTA_SIGN, PLANT_EULER_ADVECTION, PLANT_APIC_FACE_DIMS = range(5)
 
 
 class SolverDesc(C.Structure):
@@ -63,6 +65,7 @@ def lib():
         L.orc_create.argtypes = [ip, dp, dp, C.c_int, C.c_double, C.c_double, dp]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.orc_plant_mistake.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_particles.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
         L.orc_get_particles.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_num_particles.argtypes = [C.c_void_p]
@@ -172,6 +175,10 @@ class Scene:
 
     def set_threads(self, t):
         lib().orc_set_threads(self._s, t)
+
+    def plant_mistake(self, which):
+        """Pinning tests only: switch on one deliberate restatement mistake (PLANT_*)."""
+        lib().orc_plant_mistake(self._s, which)
 
     def set_particles(self, x, v, Cmat=None):
         x = _f64(x).reshape(-1, 3)

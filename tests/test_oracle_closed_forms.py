@@ -353,3 +353,207 @@ def test_cfl_closed_form():
     s.set(orc.V, np.full(s.shape(orc.V), 1.0))
     # Q13: the y component adds dt*(-9.8)
     assert s.cfl(dt) == pytest.approx((1.0 - 9.8 * dt) * dt / 0.25)
+
+
+# --- G2P and advection on multilinear fields --------------------------------
+# A field F(x) = k + A.x + B*x*y*z is multilinear, so trilinear interpolation
+# reproduces it exactly inside every cell and the gradient weights give its
+# exact gradient.  With the clamps of GetBarycentric the sample and gradient
+# are those of F at a clamped point q, which gives closed forms for G2P (PIC,
+# FLIP delta, APIC C) and for the midpoint rule of MoveParticles.  Each check
+# is shown to fail on a planted restatement mistake (Scene.plant_mistake).
+
+def _multilinear(n, k, A, B):
+    """Node values of F_c = k[c] + A[c].x + B[c]*x*y*z on the three face grids."""
+    h = 1.0 / n
+    out = []
+    for comp in range(3):
+        dims = [n, n, n]
+        dims[comp] += 1
+        off = np.array([0.5, 0.5, 0.5])
+        off[comp] = 0.0
+        K, J, I = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+        X, Y, Z = (I + off[0]) * h, (J + off[1]) * h, (K + off[2]) * h
+        out.append(k[comp] + A[comp, 0] * X + A[comp, 1] * Y + A[comp, 2] * Z + B[comp] * X * Y * Z)
+    return out
+
+
+def _clamped_point(n, comp, p, dims):
+    """The point whose value GetBarycentric samples: each axis clamped to the node range."""
+    h = 1.0 / n
+    o = np.array([0.5, 0.5, 0.5]) * h
+    o[comp] = 0.0
+    return o + h * np.clip((p - o) / h, 0.0, np.array(dims, dtype=np.float64) - 1.0)
+
+
+def _field_value(k, A, B, comp, q):
+    return k[comp] + q @ A[comp] + B[comp] * q[:, 0] * q[:, 1] * q[:, 2]
+
+
+def _field_gradient(A, B, comp, q):
+    g = np.tile(A[comp], (q.shape[0], 1))
+    g[:, 0] += B[comp] * q[:, 1] * q[:, 2]
+    g[:, 1] += B[comp] * q[:, 0] * q[:, 2]
+    g[:, 2] += B[comp] * q[:, 0] * q[:, 1]
+    return g
+
+
+def _set_fields(s, what3, fields):
+    for what, f in zip(what3, fields):
+        s.set(what, f)
+
+
+def _apic_c_expected(n, A, B, x):
+    """src/apic_simulator.cpp:147-175: row c of C is the gradient of component c,
+    taken with the cell-count stencil (GetSize(), not the face dims) at the
+    position clamped to [o + h/2, o + L - h/2] on the two transverse axes."""
+    h = 1.0 / n
+    C = np.empty((x.shape[0], 3, 3))
+    for comp in range(3):
+        pc = x.copy()
+        for ax in range(3):
+            if ax != comp:
+                pc[:, ax] = np.clip(pc[:, ax], h / 2, 1.0 - h / 2)
+        q = _clamped_point(n, comp, pc, (n, n, n))
+        C[:, comp, :] = _field_gradient(A, B, comp, q)
+    return C.reshape(-1, 9)
+
+
+def _apic_g2p(n, k, A, B, x, plant=orc.PLANT_NONE):
+    s = _box_scene(n, orc.APIC)
+    s.plant_mistake(plant)
+    _set_fields(s, (orc.U, orc.V, orc.W), _multilinear(n, k, A, B))
+    s.set_particles(x, np.zeros_like(x), np.zeros((x.shape[0], 9)))
+    s.g2p()
+    _, v, Cm = s.particles()
+    return v, Cm
+
+
+def test_apic_g2p_affine_matrix_is_field_gradient():
+    n = 8
+    lr = np.random.default_rng(77)
+    A = lr.normal(size=(3, 3))  # not symmetric: a transposed C cannot pass
+    B = lr.normal(size=3)
+    k = lr.normal(size=3)
+    x = lr.uniform(0.0, 1.0, (3000, 3))  # includes the clamped rims next to every wall
+    v, Cm = _apic_g2p(n, k, A, B, x)
+    # velocity: FaceCenteredGrid3D::Sample with the face dims, unclamped position
+    for comp in range(3):
+        dims = [n, n, n]
+        dims[comp] += 1
+        q = _clamped_point(n, comp, x, dims)
+        assert np.allclose(v[:, comp], _field_value(k, A, B, comp, q), atol=1e-12)
+    expect = _apic_c_expected(n, A, B, x)
+    assert np.allclose(Cm, expect, atol=1e-11)
+    # interior particles see the plain field gradient
+    inner = np.all((x > 1.0 / n) & (x < 1.0 - 1.0 / n), axis=1)
+    for comp in range(3):
+        assert np.allclose(Cm[inner].reshape(-1, 3, 3)[:, comp, :], _field_gradient(A, B, comp, x[inner]),
+                           atol=1e-11)
+
+
+@pytest.mark.parametrize("plant", [orc.PLANT_APIC_TRANSPOSE, orc.PLANT_APIC_FACE_DIMS])
+def test_apic_g2p_check_fails_on_planted_mistake(plant):
+    n = 8
+    lr = np.random.default_rng(77)
+    A = lr.normal(size=(3, 3))
+    B = lr.normal(size=3)
+    k = lr.normal(size=3)
+    x = lr.uniform(0.0, 1.0, (3000, 3))
+    _, Cm = _apic_g2p(n, k, A, B, x, plant)
+    assert not np.allclose(Cm, _apic_c_expected(n, A, B, x), atol=1e-11)
+
+
+def _flip_g2p(n, blend, snap, new, x, v, plant=orc.PLANT_NONE):
+    s = _box_scene(n, orc.FLIP, blend=blend)
+    s.plant_mistake(plant)
+    _set_fields(s, (orc.RU, orc.RV, orc.RW), snap)
+    _set_fields(s, (orc.U, orc.V, orc.W), new)
+    s.set_particles(x, v)
+    s.g2p()
+    return s.particles()[1]
+
+
+def _flip_expected(n, blend, f_snap, f_new, x, v):
+    """src/flip_simulator.cpp:121-161: v_flip = v_p + Sample(u_new - u_snapshot);
+    v = Lerp(v_flip, Sample(u_new), blend) = (1 - blend) v_flip + blend Sample(u_new)."""
+    out = np.empty_like(v)
+    for comp in range(3):
+        dims = [n, n, n]
+        dims[comp] += 1
+        q = _clamped_point(n, comp, x, dims)
+        new = _field_value(*f_new, comp, q)
+        flip = v[:, comp] + new - _field_value(*f_snap, comp, q)
+        out[:, comp] = flip if blend == 0.0 else (1 - blend) * flip + blend * new
+    return out
+
+
+def _flip_fields(seed):
+    lr = np.random.default_rng(seed)
+    f_snap = (lr.normal(size=3), lr.normal(size=(3, 3)), lr.normal(size=3))
+    f_new = (lr.normal(size=3), lr.normal(size=(3, 3)), lr.normal(size=3))
+    x = lr.uniform(0.0, 1.0, (1500, 3))
+    v = lr.normal(size=(1500, 3))
+    return f_snap, f_new, x, v
+
+
+@pytest.mark.parametrize("blend", [0.0, 0.03, 0.5])
+def test_flip_g2p_delta_on_changed_grid(blend):
+    n = 8
+    f_snap, f_new, x, v = _flip_fields(5)
+    out = _flip_g2p(n, blend, _multilinear(n, *f_snap), _multilinear(n, *f_new), x, v)
+    assert np.allclose(out, _flip_expected(n, blend, f_snap, f_new, x, v), atol=1e-12)
+
+
+def test_flip_g2p_check_fails_on_planted_delta_sign():
+    n = 8
+    f_snap, f_new, x, v = _flip_fields(5)
+    out = _flip_g2p(n, 0.03, _multilinear(n, *f_snap), _multilinear(n, *f_new), x, v, orc.PLANT_FLIP_DELTA_SIGN)
+    assert not np.allclose(out, _flip_expected(n, 0.03, f_snap, f_new, x, v), atol=1e-12)
+
+
+def _advect(n, A, c, x, dt, max_cfl, plant=orc.PLANT_NONE):
+    s = orc.Scene((n, n, n), method=orc.PIC, max_cfl=max_cfl)
+    s.plant_mistake(plant)
+    _set_fields(s, (orc.U, orc.V, orc.W), _multilinear(n, c, A, np.zeros(3)))
+    v = np.ones_like(x)
+    s.set_particles(x, v)
+    s.move_particles(dt)
+    xo, vo, _ = s.particles()
+    assert np.array_equal(vo, v)  # no wall reached: velocities untouched
+    return xo
+
+
+def _midpoint_closed_form(A, c, x, dt, m):
+    """m = max(maxCfl, 1) midpoint steps of d = dt/m on u(x) = A x + c
+    (src/pic_simulator.cpp:239-268): x <- (I + dA + d^2 A^2/2) x + (d I + d^2 A/2) c."""
+    d = dt / m
+    eye = np.eye(3)
+    M = eye + d * A + 0.5 * d * d * (A @ A)
+    t = (d * eye + 0.5 * d * d * A) @ c
+    for _ in range(m):
+        x = x @ M.T + t
+    return x
+
+
+@pytest.mark.parametrize("max_cfl,m", [(3.0, 3), (5.0, 5), (2.5, 2), (0.5, 1)])
+def test_move_particles_midpoint_on_linear_field(max_cfl, m):
+    n = 8
+    lr = np.random.default_rng(11)
+    A = lr.normal(size=(3, 3))
+    c = lr.normal(size=3)
+    x = lr.uniform(0.3, 0.7, (400, 3))
+    dt = 0.05
+    assert np.abs(x @ A.T + c).max() < 3.0  # with dt = 0.05 every path stays inside [h/2, 1 - h/2]
+    xo = _advect(n, A, c, x, dt, max_cfl)
+    assert np.allclose(xo, _midpoint_closed_form(A, c, x, dt, m), atol=1e-12)
+
+
+def test_move_particles_check_fails_on_planted_euler():
+    n = 8
+    lr = np.random.default_rng(11)
+    A = lr.normal(size=(3, 3))
+    c = lr.normal(size=3)
+    x = lr.uniform(0.3, 0.7, (400, 3))
+    xo = _advect(n, A, c, x, 0.05, 3.0, orc.PLANT_EULER_ADVECTION)
+    assert not np.allclose(xo, _midpoint_closed_form(A, c, x, 0.05, 3), atol=1e-12)
