@@ -60,12 +60,21 @@ def kernel_bytes_per_launch(kernel, solver, n, npart):
     cells = n ** 3
     if kernel in KERNEL_BYTES_PER_CELL:
         return kernel_bytes_per_cell(kernel, solver) * cells
-    if kernel == "p2g":  # one component: float4 position (16) + v_c (4) per particle, cell start (4) per cell,
-        return 20 * npart + 4 * cells + 9 * (n + 1) * n * n  # write u + marker + FLIP snapshot (9) per face
-    if kernel == "g2p_move":  # read x, v (24), write x, v (24), bin key (4) per particle; read u, snapshot per face
-        return 52 * npart + 8 * 3 * (n + 1) * n * n
+    faces = (n + 1) * n * n  # per component
+    if kernel == "p2g":  # one of three component launches: a third of the x, v read (8 of 24 per particle),
+        return 8 * npart + 8 * faces  # the face velocity and the FLIP snapshot written (8 per face)
+    if kernel == "g2p_move":  # read x, v (24) and write x, v (24) per particle; read u, v, w + snapshot (24 per face)
+        return 48 * npart + 8 * 3 * faces
     if kernel == "sdf":  # read every particle position (12), cell starts (4) and write the SDF (8) per cell
         return 12 * npart + 12 * cells
+    return None
+
+
+def kernel_impl_bytes_per_launch(kernel, n, npart):
+    """What this implementation's kernel reads and writes by design, where that
+    differs from the algorithmic bytes (P2G: the per-particle stencil records)."""
+    if kernel == "p2g":  # float4 stencil record (16) + v_c (4) per particle, bin start (4) per cell,
+        return 20 * npart + 4 * n ** 3 + 9 * (n + 1) * n * n  # u + marker + FLIP snapshot (9) per face
     return None
 
 
@@ -75,9 +84,13 @@ def roofline(kernel, solver, n, npart, kms, launches, traffic):
         return None
     avg_ms = kms / launches
     achieved = bpl / (avg_ms * 1e-3) / 1e9
-    return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches, "bytes_per_launch": bpl}
+    out = {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches, "bytes_per_launch": bpl}
+    impl = kernel_impl_bytes_per_launch(kernel, n, npart)
+    if impl is not None:
+        out["impl_bytes_per_launch"] = impl
+    return out
 
 
 def parse():
@@ -89,7 +102,7 @@ def parse():
     ap.add_argument("--kernel", default="p2g", choices=sorted(KERNEL_IDS),
                     help="kernel of the roofline object (default: P2G, the largest share of the substep)")
     ap.add_argument("--solver", default="mgpcg", choices=sorted(SOLVERS))
-    ap.add_argument("--cpu-substeps", type=int, default=1)
+    ap.add_argument("--cpu-substeps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solver-comparison", action="store_true",
@@ -112,7 +125,7 @@ def parse():
 
 
 KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
-                  "g2p_move": "k_g2p_move", "sdf": "k_sdf"}
+                  "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d rocprof/kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
@@ -274,16 +287,20 @@ def main():
     on_profile = args.n == 128 and args.solver == "mgpcg"
     roof = roofline(args.kernel, args.solver, n, npart, kms, klaunch,
                     traffic_for(args.kernel) if on_profile else None)
-    # the CG update's roofline as well, over a few more substeps after the timed region
-    roof_cg = None
-    if args.solver != "jacobi" and args.kernel not in ("cg_update", "none"):
-        scene.kernel_timer(KERNEL_IDS["cg_update"])
-        for i in range(min(20, max(args.steps, 1))):
-            scene.advance_frame(dt, solver)
-        scene.sync()
-        cms, clau = scene.kernel_timer_read(reset=True)
-        roof_cg = roofline("cg_update", args.solver, n, npart, cms, clau,
-                           traffic_for("cg_update") if on_profile else None)
+    # the other timed kernels' rooflines, each over a few more substeps after the timed region
+    rooflines = {}
+    if args.kernel != "none":
+        others = ["g2p_move", "sdf"] + ([] if args.solver == "jacobi" else ["cg_update", "cg_spmv", "mg_smooth"])
+        for kern in others:
+            if kern == args.kernel or (kern == "mg_smooth" and args.solver != "mgpcg"):
+                continue
+            scene.kernel_timer(KERNEL_IDS[kern])
+            for i in range(min(20, max(args.steps, 1))):
+                scene.advance_frame(dt, solver)
+            scene.sync()
+            cms, clau = scene.kernel_timer_read(reset=True)
+            rooflines[kern] = roofline(kern, args.solver, n, npart, cms, clau,
+                                       traffic_for(kern) if on_profile else None)
     scene.kernel_timer(KERNEL_IDS["none"])
 
     solve_cmp = None if (args.solver == "jacobi" or args.no_solver_comparison) else solver_comparison(scene, capi, dt)
@@ -325,7 +342,7 @@ def main():
                                "drift_over_run": round((fluid_cells[-1] - fluid_cells[0]) / max(fluid_cells[0], 1), 5)},
         "stage_ms_per_step": {k: round(val / max(args.steps, 1), 4) for k, val in stage_acc.items()},
         "roofline": roof,
-        "roofline_cg_update": roof_cg,
+        "rooflines_other": rooflines,
         "pressure_solver_comparison": solve_cmp,
         "learned_warm_start": learned,
         "batch_96_scenes": batch,
@@ -513,7 +530,9 @@ def cpu_baseline(scene, capi, desc, dt, args):
                   threads=threads)
     o.set_particles(gx.astype(np.float64), gv.astype(np.float64))
     o.set(orc.PRESSURE, scene.get(capi.GRID_PRESSURE))
-    solv = orc.solver(orc.PCG_JACOBI, 2000, 0, 1e-6, relative=True, warm_start=True)
+    # the reference's CPU ConjugateGradientSolver (src/linear_system/conjugate_gradient_solver.cpp:14-55),
+    # stopped at the metric's 1e-6 relative residual, warm-started from the previous pressure
+    solv = orc.solver(orc.CG, 4000, 0, 1e-6, relative=True, warm_start=True)
     print(f"bench: cpu baseline, {threads} threads, {args.cpu_substeps} substeps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.cpu_substeps):
@@ -522,7 +541,7 @@ def cpu_baseline(scene, capi, desc, dt, args):
     return {"value": round(gx.shape[0] * args.cpu_substeps / t / 1e6, 4), "unit": "Mparticle-substeps/s",
             "cores": threads, "kind": "port",
             "sample": f"{args.cpu_substeps} substeps of the same FLIP {n[0]}^3 scene from the GPU state after "
-                      f"warm-up ({gx.shape[0]} particles), FP64 oracle, {t:.1f} s"}
+                      f"warm-up ({gx.shape[0]} particles), FP64 oracle with CG to 1e-6, {t:.1f} s"}
 
 
 if __name__ == "__main__":

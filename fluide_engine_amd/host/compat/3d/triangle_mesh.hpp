@@ -1,0 +1,5 @@
+// 3d/triangle_mesh.hpp -- stands in for the reference's src/3d/triangle_mesh.hpp: its classes on the gfx950
+// backend (fluide/simulators.hpp), under their global names.
+#pragma once
+
+#include "fluide/global_names.hpp"

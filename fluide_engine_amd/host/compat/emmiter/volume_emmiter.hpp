@@ -1,0 +1,5 @@
+// emmiter/volume_emmiter.hpp -- stands in for the reference's src/emmiter/volume_emmiter.hpp: its classes on the gfx950
+// backend (fluide/simulators.hpp), under their global names.
+#pragma once
+
+#include "fluide/global_names.hpp"

@@ -1,0 +1,5 @@
+// fluid_solvers/forward_euler_diffusion_solver.hpp -- stands in for the reference's src/fluid_solvers/forward_euler_diffusion_solver.hpp: its classes on the gfx950
+// backend (fluide/simulators.hpp), under their global names.
+#pragma once
+
+#include "fluide/global_names.hpp"

@@ -1,0 +1,5 @@
+// forces/external_force.hpp -- stands in for the reference's src/forces/external_force.hpp: its classes on the gfx950
+// backend (fluide/simulators.hpp), under their global names.
+#pragma once
+
+#include "fluide/global_names.hpp"

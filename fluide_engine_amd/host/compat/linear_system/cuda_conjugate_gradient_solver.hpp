@@ -1,0 +1,5 @@
+// linear_system/cuda_conjugate_gradient_solver.hpp -- stands in for the reference's src/linear_system/cuda_conjugate_gradient_solver.hpp: its classes on the gfx950
+// backend (fluide/simulators.hpp), under their global names.
+#pragma once
+
+#include "fluide/global_names.hpp"

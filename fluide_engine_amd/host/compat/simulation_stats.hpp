@@ -1,0 +1,5 @@
+// simulation_stats.hpp -- stands in for the reference's src/simulation_stats.hpp: its classes on the gfx950
+// backend (fluide/simulators.hpp), under their global names.
+#pragma once
+
+#include "fluide/global_names.hpp"

@@ -94,6 +94,25 @@ class DirectionalField : public ExternalForce {
     Vector3<double> _strength;
 };
 
+// src/forces/point_field.hpp: strength / max(|x - origin|, 1) towards the
+// origin.  Not uniform, so AddExternalForce rejects it unless its strength is 0.
+class PointField : public ExternalForce {
+  public:
+    PointField(Vector3<double> origin = 0, double strength = 0, double strengthFallOff = 1)
+        : _origin(origin), _strength(strength), _strengthFallOff(strengthFallOff) {}
+    Vector3<double> Sample(const Vector3<double>& position) const override;
+    void SetPosition(Vector3<double> origin) { _origin = origin; }
+    void SetStrength(double strength) { _strength = strength; }
+    void SetStrengthFallOff(double strengthFallOff) { _strengthFallOff = strengthFallOff; }
+    Vector3<double> GetOrigin() const { return _origin; }
+    double GetStrength() const { return _strength; }
+    double GetStrengthFallOff() const { return _strengthFallOff; }
+
+  private:
+    Vector3<double> _origin;
+    double _strength, _strengthFallOff;
+};
+
 // Host copy of a cell-centred scalar grid (src/3d/scalar_grid3d.hpp): x-fastest.
 struct ScalarGrid3D {
     Vector3<size_t> size;
@@ -170,6 +189,94 @@ struct DiffusionSolverSettings {
     double tolerance = 1e-14;
 };
 
+// The reference's solver objects (SetPressureSolver / SetDiffusionSolver take
+// them as shared_ptrs, src/pic_simulator.hpp:41-43).  Each linear solver names
+// the device solve it stands for; the iteration itself runs in the scene.
+class LinearSystemSolver {
+  public:
+    virtual ~LinearSystemSolver() = default;
+    virtual PressureSolverSettings Settings() const = 0;
+};
+
+// src/linear_system/jacobi_iteration_solver.hpp: at most maxNumberOfIterations
+// sweeps, ||b - A x||_2 < tolerance checked every residualCheckInterval sweeps.
+class JacobiIterationSolver : public LinearSystemSolver {
+  public:
+    JacobiIterationSolver(size_t maxNumberOfIterations, size_t residualCheckInterval, double tolerance);
+    PressureSolverSettings Settings() const override { return _s; }
+
+  private:
+    PressureSolverSettings _s;
+};
+
+// src/linear_system/cuda_jacobi_iteration_solver.hpp: the same sweeps and stop.
+class CudaJacobiIterationSolver : public JacobiIterationSolver {
+  public:
+    using JacobiIterationSolver::JacobiIterationSolver;
+};
+
+// src/linear_system/conjugate_gradient_solver.hpp: unpreconditioned CG from the
+// previous solution, stopping when r.r <= tolerance^2 or after maxNumberOfIterations.
+class ConjugateGradientSolver : public LinearSystemSolver {
+  public:
+    ConjugateGradientSolver(size_t maxNumberOfIterations, double tolerance);
+    PressureSolverSettings Settings() const override { return _s; }
+
+  protected:
+    PressureSolverSettings _s;
+};
+
+// src/linear_system/cuda_conjugate_gradient_solver.hpp: the same iteration.
+class CudaConjugateGradientSolver : public ConjugateGradientSolver {
+  public:
+    using ConjugateGradientSolver::ConjugateGradientSolver;
+};
+
+// Added for the GPU backend (no reference class): CG preconditioned by one
+// geometric multigrid V-cycle, stopping on ||b - A x||_2 <= tolerance ||b||_2
+// when relative (the default) or r.r <= tolerance^2.
+class MultigridPCGSolver : public ConjugateGradientSolver {
+  public:
+    MultigridPCGSolver(size_t maxNumberOfIterations, double tolerance, bool relative = true);
+};
+
+// src/fluid_solvers/pressure_solver.hpp
+class PressureSolver {
+  public:
+    virtual ~PressureSolver() = default;
+};
+
+// src/fluid_solvers/single_phase_pressure_solver.hpp: its linear solver
+// defaults to CudaJacobiIterationSolver(1000, 5, 1e-14)
+// (src/fluid_solvers/single_phase_pressure_solver.cpp:11).
+class SinglePhasePressureSolver : public PressureSolver {
+  public:
+    SinglePhasePressureSolver();
+    void SetLinearSystemSolver(const std::shared_ptr<LinearSystemSolver>& solver);
+    const std::shared_ptr<LinearSystemSolver>& GetLinearSystemSolver() const { return _solver; }
+
+  private:
+    std::shared_ptr<LinearSystemSolver> _solver;
+};
+
+// src/fluid_solvers/diffusion_solver.hpp
+class DiffusionSolver {
+  public:
+    virtual ~DiffusionSolver() = default;
+};
+
+// src/fluid_solvers/backward_euler_diffusion_solver.hpp: implicit viscosity
+// with the fixed CudaJacobiIterationSolver(1000, 5, 1e-14)
+// (src/fluid_solvers/backward_euler_diffusion_solver.cpp:10).
+class BackwardEulerDiffusionSolver : public DiffusionSolver {
+  public:
+    DiffusionSolverSettings Settings() const { return DiffusionSolverSettings(); }
+};
+
+// src/fluid_solvers/forward_euler_diffusion_solver.hpp: not on this backend
+// (SetDiffusionSolver rejects it).
+class ForwardEulerDiffusionSolver : public DiffusionSolver {};
+
 class Error : public std::runtime_error {
   public:
     Error(const std::string& what, int code) : std::runtime_error(what), code(code) {}
@@ -186,7 +293,14 @@ class Unsupported : public std::logic_error {
 // triangle mesh whose SDF is built by Mesh2SDF on the node grid (gridSize,
 // gridOrigin, gridSpacing).  The SDF is computed on the device when the
 // collider is added to a simulator (fe_scene_add_mesh_collider).
-class TriangleMeshCollider {
+// src/3d/collider.hpp.  The colliders of this backend are sampled into the
+// scene's collider SDF on the device; their host-side queries are not kept.
+class Collider {
+  public:
+    virtual ~Collider() = default;
+};
+
+class TriangleMeshCollider : public Collider {
   public:
     TriangleMeshCollider(const Vector3<size_t>& gridSize, const Vector3<double>& gridOrigin,
                          const Vector3<double>& gridSpacing, const TriangleMesh& mesh);
@@ -208,7 +322,13 @@ class TriangleMeshCollider {
 // the emitter's SDF is <= 0 (Mesh2SDF of the mesh, or given directly by
 // InitializeFromSdf).  velocityVariance is kept but unused, as in the
 // reference.  The seed keys the particle generator (no reference counterpart).
-class VolumeEmitter {
+// src/emmiter/emmiter.hpp
+class Emitter {
+  public:
+    virtual ~Emitter() = default;
+};
+
+class VolumeEmitter : public Emitter {
   public:
     VolumeEmitter(const TriangleMesh& mesh, Vector3<size_t> resolution, BoundingBox3D domain,
                   const size_t& particlesPerBlock, const Vector3<double>& velocity,
@@ -236,6 +356,27 @@ class VolumeEmitter {
     uint64_t _seed = 0;
     std::vector<double> _vertices, _sdf;
     std::vector<int32_t> _triangles;
+};
+
+// src/3d/collider_collection.hpp: the colliders of a scene on the node grid
+// (size, gridSpacing, origin).  SetColliders hands them to a simulator, whose
+// device builds the minimum SDF over them (BuildSdf is that step, done there).
+class ColliderCollection {
+  public:
+    ColliderCollection(const Vector3<size_t>& size, const Vector3<double>& gridSpacing, const Vector3<double>& origin)
+        : _size(size), _gridSpacing(gridSpacing), _origin(origin) {}
+    void AddCollider(std::shared_ptr<Collider> collider);
+    void BuildSdf() {}
+    std::vector<std::shared_ptr<Collider>>& GetColliders() { return _colliders; }
+    const std::vector<std::shared_ptr<Collider>>& GetColliders() const { return _colliders; }
+    Vector3<size_t> GetSize() const { return _size; }
+    Vector3<double> GetGridSpacing() const { return _gridSpacing; }
+    Vector3<double> GetOrigin() const { return _origin; }
+
+  private:
+    std::vector<std::shared_ptr<Collider>> _colliders;
+    Vector3<size_t> _size;
+    Vector3<double> _gridSpacing, _origin;
 };
 
 class Device;
@@ -273,23 +414,31 @@ class HipHybridSimulator {
     void SetDensity(double density);
     void SetParticlesPerBlock(size_t particlesPerBlock) { _particlesPerBlock = particlesPerBlock; }
 
-    // PICSimulator
+    // PICSimulator::SetPressureSolver (src/pic_simulator.hpp:42): a
+    // SinglePhasePressureSolver and its linear solver, or the settings directly.
+    void SetPressureSolver(std::shared_ptr<PressureSolver> pressureSolver);
     void SetPressureSolver(const PressureSolverSettings& settings);
     PressureSolverSettings GetPressureSolver() const { return _solver; }
-    // SetDiffusionSolver (src/pic_simulator.hpp:36) for the reference's one
-    // diffusion solver, BackwardEulerDiffusionSolver; it runs when viscosity > 0.
+    // SetDiffusionSolver (src/pic_simulator.hpp:41) for the reference's
+    // BackwardEulerDiffusionSolver; it runs when viscosity > 0.
+    void SetDiffusionSolver(std::shared_ptr<DiffusionSolver> diffusionSolver);
     void SetDiffusionSolver(const DiffusionSolverSettings& settings);
     DiffusionSolverSettings GetDiffusionSolver() const { return _diffusion; }
 
     // HybridSimulator::AddCollider (src/hybrid_simulator.hpp:35,
     // src/pic_simulator.cpp:66-69): the scene's collider SDF becomes the
     // minimum over its colliders (ColliderCollection::BuildSdf).
-    void AddCollider(std::shared_ptr<TriangleMeshCollider> collider);
+    // TriangleMeshCollider is the collider type of this backend.
+    void AddCollider(std::shared_ptr<Collider> collider);
+    // PICSimulator::SetColliders (src/pic_simulator.hpp:43): the collection's
+    // colliders replace the simulator's.
+    void SetColliders(std::shared_ptr<ColliderCollection> colliders);
     size_t NumberOfColliders() const { return _colliders.size(); }
 
     // PICSimulator::AddEmitter (src/pic_simulator.cpp:46-49): emits at the
-    // start of every substep from the second on.
-    void AddEmitter(std::shared_ptr<VolumeEmitter> emitter);
+    // start of every substep from the second on.  VolumeEmitter is the emitter
+    // type of this backend.
+    void AddEmitter(std::shared_ptr<Emitter> emitter);
     size_t NumberOfEmitters() const { return _emitters.size(); }
     // PICSimulator::InitializeFromTriangleMesh (src/pic_simulator.cpp:33-39):
     // Mesh2SDF of the mesh into the fluid SDF, then GetParticlesPerBlock()
@@ -346,26 +495,33 @@ class HipHybridSimulator {
     mutable ScalarGrid3D _sdf;
 };
 
+// FLIPSimulator and APICSimulator derive from PICSimulator, as in the
+// reference (src/flip_simulator.hpp, src/apic_simulator.hpp), so a
+// shared_ptr<FLIPSimulator> passes where a shared_ptr<PICSimulator> is taken.
 class HipPICSimulator : public HipHybridSimulator {
   public:
     HipPICSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
         : HipHybridSimulator(gridSize, domain, FE_PIC) {}
+
+  protected:
+    HipPICSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain, int method)
+        : HipHybridSimulator(gridSize, domain, method) {}
 };
 
-class HipFLIPSimulator : public HipHybridSimulator {
+class HipFLIPSimulator : public HipPICSimulator {
   public:
     HipFLIPSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
-        : HipHybridSimulator(gridSize, domain, FE_FLIP) {
+        : HipPICSimulator(gridSize, domain, FE_FLIP) {
         SetBlend(0.05);  // _blendingFactor(0.05), src/flip_simulator.cpp:5
     }
     void SetBlendingFactor(double factor) { SetBlend(factor); }
     double GetBlendingFactor() const { return Blend(); }
 };
 
-class HipAPICSimulator : public HipHybridSimulator {
+class HipAPICSimulator : public HipPICSimulator {
   public:
     HipAPICSimulator(const Vector3<size_t>& gridSize, const BoundingBox3D& domain)
-        : HipHybridSimulator(gridSize, domain, FE_APIC) {}
+        : HipPICSimulator(gridSize, domain, FE_APIC) {}
 };
 
 // src/simulation_stats.hpp
@@ -396,7 +552,13 @@ class SimulationRunner {
     SimulationStats _stats;
 };
 
-// The drop-in names of the reference's CUDA simulators.
+// The drop-in names: the reference's CPU simulator classes
+// (src/hybrid_simulator.hpp, src/pic_simulator.hpp, src/flip_simulator.hpp,
+// src/apic_simulator.hpp) and its CUDA ones (src/cuda_pic_simulator.hpp).
+using HybridSimulator = HipHybridSimulator;
+using PICSimulator = HipPICSimulator;
+using FLIPSimulator = HipFLIPSimulator;
+using APICSimulator = HipAPICSimulator;
 using CudaPICSimulator = HipPICSimulator;
 using CudaFLIPSimulator = HipFLIPSimulator;
 using CudaAPICSimulator = HipAPICSimulator;

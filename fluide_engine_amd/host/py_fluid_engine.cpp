@@ -186,6 +186,8 @@ PYBIND11_MODULE(PyFluidEngine, m) {
     py::class_<ExternalForce, std::shared_ptr<ExternalForce>>(m, "ExternalForce");
     py::class_<DirectionalField, std::shared_ptr<DirectionalField>, ExternalForce>(m, "DirectionalField")
         .def(py::init<Vector3<double>>());
+    py::class_<PointField, std::shared_ptr<PointField>, ExternalForce>(m, "PointField")
+        .def(py::init<Vector3<double>, double, double>());
 
     // src/python_wrapper/triangle_mesh_wrapper.cpp, obj_manager_wrapper.cpp,
     // simulator_runner_wrapper.cpp; the mesh also hands its arrays to numpy
@@ -209,7 +211,8 @@ PYBIND11_MODULE(PyFluidEngine, m) {
         });
     // src/python_wrapper/emitters_wrapper.cpp:15-23; InitializeFromSdf takes the
     // emitter grid's values as a (nz, ny, nx) array
-    py::class_<VolumeEmitter, std::shared_ptr<VolumeEmitter>>(m, "VolumeEmitter")
+    py::class_<Emitter, std::shared_ptr<Emitter>>(m, "Emitter");
+    py::class_<VolumeEmitter, std::shared_ptr<VolumeEmitter>, Emitter>(m, "VolumeEmitter")
         .def(py::init<const TriangleMesh&, Vector3<size_t>, BoundingBox3D, const size_t&, const Vector3<double>&,
                       const Vector3<double>&>())
         .def("InitializeFromTriangleMesh", &VolumeEmitter::InitializeFromTriangleMesh)
@@ -223,7 +226,8 @@ PYBIND11_MODULE(PyFluidEngine, m) {
              })
         .def("SetSeed", &VolumeEmitter::SetSeed);
     // src/python_wrapper/colliders_wrapper.cpp:14-17
-    py::class_<TriangleMeshCollider, std::shared_ptr<TriangleMeshCollider>>(m, "TriangleMeshCollider")
+    py::class_<Collider, std::shared_ptr<Collider>>(m, "Collider");
+    py::class_<TriangleMeshCollider, std::shared_ptr<TriangleMeshCollider>, Collider>(m, "TriangleMeshCollider")
         .def(py::init<const Vector3<size_t>&, const Vector3<double>&, const Vector3<double>&, const TriangleMesh&>());
     py::class_<OBJManager>(m, "OBJManager")
         .def(py::init<>())
@@ -282,12 +286,12 @@ PYBIND11_MODULE(PyFluidEngine, m) {
     m.def("Box", &box, py::arg("lo"), py::arg("hi"), "Particle-seeding box [lo, hi)");
     m.def("Sphere", &sphere, py::arg("centre"), py::arg("radius"), "Particle-seeding sphere");
 
-    auto pic = add_sim<HipPICSimulator, HipHybridSimulator>(m, "PICSimulator");
-    add_sim<HipAPICSimulator, HipHybridSimulator>(m, "APICSimulator");
-    add_sim<HipFLIPSimulator, HipHybridSimulator>(m, "FLIPSimulator")
+    // src/python_wrapper/simulators_wrapper.cu:47-84: APIC and FLIP derive from PIC
+    add_sim<HipPICSimulator, HipHybridSimulator>(m, "PICSimulator");
+    add_sim<HipAPICSimulator, HipPICSimulator>(m, "APICSimulator");
+    add_sim<HipFLIPSimulator, HipPICSimulator>(m, "FLIPSimulator")
         .def("SetBlendingFactor", &HipFLIPSimulator::SetBlendingFactor)
         .def("GetBlendingFactor", &HipFLIPSimulator::GetBlendingFactor);
-    (void)pic;
     py::class_<SimulationRunner>(m, "SimulationRunner")
         .def(py::init<>())
         .def("RunSimulation", &SimulationRunner::RunSimulation)

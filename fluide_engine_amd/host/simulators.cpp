@@ -139,10 +139,12 @@ void HipHybridSimulator::PushEmitter(const VolumeEmitter& e) const {
               "fe_scene_add_volume_emitter");
 }
 
-void HipHybridSimulator::AddEmitter(std::shared_ptr<VolumeEmitter> emitter) {
+void HipHybridSimulator::AddEmitter(std::shared_ptr<Emitter> emitter) {
     if (!emitter) throw std::invalid_argument("AddEmitter: null emitter");
-    if (_scene) PushEmitter(*emitter);
-    _emitters.push_back(std::move(emitter));
+    auto ve = std::dynamic_pointer_cast<VolumeEmitter>(emitter);
+    if (!ve) throw Unsupported("AddEmitter: VolumeEmitter is the emitter type of the GPU backend");
+    if (_scene) PushEmitter(*ve);
+    _emitters.push_back(std::move(ve));
 }
 
 void HipHybridSimulator::InitializeFromTriangleMesh(const TriangleMesh& mesh, uint64_t seed) {
@@ -192,10 +194,90 @@ void HipHybridSimulator::PushCollider(const TriangleMeshCollider& c) const {
           "fe_scene_add_mesh_collider");
 }
 
-void HipHybridSimulator::AddCollider(std::shared_ptr<TriangleMeshCollider> collider) {
+void HipHybridSimulator::AddCollider(std::shared_ptr<Collider> collider) {
     if (!collider) throw std::invalid_argument("AddCollider: null collider");
-    if (_scene) PushCollider(*collider);  // a scene created later receives every collider
+    auto mc = std::dynamic_pointer_cast<TriangleMeshCollider>(collider);
+    if (!mc) throw Unsupported("AddCollider: TriangleMeshCollider is the collider type of the GPU backend");
+    if (_scene) PushCollider(*mc);  // a scene created later receives every collider
+    _colliders.push_back(std::move(mc));
+}
+
+void HipHybridSimulator::SetColliders(std::shared_ptr<ColliderCollection> colliders) {
+    if (!colliders) throw std::invalid_argument("SetColliders: null collection");
+    std::vector<std::shared_ptr<TriangleMeshCollider>> list;
+    for (const auto& c : colliders->GetColliders()) {
+        auto mc = std::dynamic_pointer_cast<TriangleMeshCollider>(c);
+        if (!mc) throw Unsupported("SetColliders: TriangleMeshCollider is the collider type of the GPU backend");
+        list.push_back(std::move(mc));
+    }
+    if (_scene) {
+        Check(fe_scene_clear_colliders(_scene), "fe_scene_clear_colliders");
+        for (const auto& c : list) PushCollider(*c);
+    }
+    _colliders = std::move(list);
+}
+
+void ColliderCollection::AddCollider(std::shared_ptr<Collider> collider) {
+    if (!collider) throw std::invalid_argument("ColliderCollection::AddCollider: null collider");
     _colliders.push_back(std::move(collider));
+}
+
+Vector3<double> PointField::Sample(const Vector3<double>& position) const {
+    // src/forces/point_field.cpp: strength * (1 / max(|x - o|, 1)) * (o - x) / |o - x|
+    const double dx = _origin.x - position.x, dy = _origin.y - position.y, dz = _origin.z - position.z;
+    const double len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    const double dist = len >= 1.0 ? len : 1.0;
+    const double f = _strength * (1.0 / dist);
+    return Vector3<double>(f * (dx / len), f * (dy / len), f * (dz / len));
+}
+
+JacobiIterationSolver::JacobiIterationSolver(size_t maxNumberOfIterations, size_t residualCheckInterval,
+                                             double tolerance) {
+    if (maxNumberOfIterations > 1000000 || residualCheckInterval > INT32_MAX)
+        throw std::invalid_argument("JacobiIterationSolver: iteration counts out of range");
+    _s.kind = FE_SOLVER_JACOBI;
+    _s.maxIterations = (int)maxNumberOfIterations;
+    _s.checkInterval = (int)residualCheckInterval;
+    _s.tolerance = tolerance;
+    _s.relative = false;
+    _s.warmStart = true;
+}
+
+ConjugateGradientSolver::ConjugateGradientSolver(size_t maxNumberOfIterations, double tolerance) {
+    if (maxNumberOfIterations > 1000000) throw std::invalid_argument("ConjugateGradientSolver: too many iterations");
+    _s.kind = FE_SOLVER_CG;
+    _s.maxIterations = (int)maxNumberOfIterations;
+    _s.checkInterval = 0;
+    _s.tolerance = tolerance;
+    _s.relative = false;
+    _s.warmStart = true;
+}
+
+MultigridPCGSolver::MultigridPCGSolver(size_t maxNumberOfIterations, double tolerance, bool relative)
+    : ConjugateGradientSolver(maxNumberOfIterations, tolerance) {
+    _s.kind = FE_SOLVER_MGPCG;
+    _s.relative = relative;
+}
+
+SinglePhasePressureSolver::SinglePhasePressureSolver()
+    : _solver(std::make_shared<CudaJacobiIterationSolver>(1000, 5, 1e-14)) {}
+
+void SinglePhasePressureSolver::SetLinearSystemSolver(const std::shared_ptr<LinearSystemSolver>& solver) {
+    if (!solver) throw std::invalid_argument("SetLinearSystemSolver: null solver");
+    _solver = solver;
+}
+
+void HipHybridSimulator::SetPressureSolver(std::shared_ptr<PressureSolver> pressureSolver) {
+    auto sp = std::dynamic_pointer_cast<SinglePhasePressureSolver>(pressureSolver);
+    if (!sp) throw Unsupported("SetPressureSolver: SinglePhasePressureSolver is the pressure solver of the GPU backend");
+    SetPressureSolver(sp->GetLinearSystemSolver()->Settings());
+}
+
+void HipHybridSimulator::SetDiffusionSolver(std::shared_ptr<DiffusionSolver> diffusionSolver) {
+    auto be = std::dynamic_pointer_cast<BackwardEulerDiffusionSolver>(diffusionSolver);
+    if (!be)
+        throw Unsupported("SetDiffusionSolver: BackwardEulerDiffusionSolver is the diffusion solver of the GPU backend");
+    SetDiffusionSolver(be->Settings());
 }
 
 TriangleMeshCollider::TriangleMeshCollider(const Vector3<size_t>& gridSize, const Vector3<double>& gridOrigin,

@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -283,6 +284,7 @@ struct orc_scene {
     double gravity[3];
     int nthreads = 1;
     int has_collider = 0;
+    int fast_dots = 0;  // orc_set_fast_dots: storage-order blocked dot products in the solvers
     int planted = 0;  // ORC_PLANT_*: a deliberate restatement mistake, set only by the pinning tests
     double xo[3], yo[3], zo[3];  // face data origins, src/grid_systems/face_centered_grid3d.cpp:324-332
 
@@ -905,6 +907,36 @@ double dot(const Arr3& a, const Arr3& b) {
     return r;
 }
 
+// The same sum in storage order over 64 fixed blocks added in block order:
+// deterministic for any thread count, cache-friendly.  Only for tests that
+// need a converged answer on large grids (orc_set_fast_dots); the default is
+// the reference's order above.
+double dot_blocked(const Arr3& a, const Arr3& b, int nthreads) {
+    constexpr int kBlocks = 64;
+    const int64_t n = (int64_t)a.size();
+    double part[kBlocks] = {0};
+    std::vector<std::thread> th;
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int q; (q = next.fetch_add(1)) < kBlocks;) {
+            const int64_t i0 = n * q / kBlocks, i1 = n * (q + 1) / kBlocks;
+            double r = 0;
+            for (int64_t i = i0; i < i1; ++i) r += a.d[i] * b.d[i];
+            part[q] = r;
+        }
+    };
+    for (int t = 1; t < std::max(1, nthreads); ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    double r = 0;
+    for (int q = 0; q < kBlocks; ++q) r += part[q];
+    return r;
+}
+
+double sdot(const orc_scene* s, const Arr3& a, const Arr3& b) {
+    return s->fast_dots ? dot_blocked(a, b, s->nthreads) : dot(a, b);
+}
+
 // JacobiIterationSolver (src/linear_system/jacobi_iteration_solver.cpp:14-76)
 void jacobi(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
     const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
@@ -932,7 +964,7 @@ void jacobi(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
         executed = it + 1;
         if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
             residual(s, s->p, s->b, res, s->nthreads);
-            double nrm = std::sqrt(dot(res, res));
+            double nrm = std::sqrt(sdot(s, res, res));
             if (nrm < d->tolerance) break;
         }
     }
@@ -966,11 +998,11 @@ void cg(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
     if (!d->warm_start) std::fill(s->p.d.begin(), s->p.d.end(), 0.0);
     residual(s, s->p, s->b, r, s->nthreads);
     apply_m(r, dd);
-    double sigmaNew = dot(r, dd);
-    const double bnorm = std::sqrt(dot(s->b, s->b));
+    double sigmaNew = sdot(s, r, dd);
+    const double bnorm = std::sqrt(sdot(s, s->b, s->b));
     auto done = [&](double sig, const Arr3& rr) {
         if (d->relative) {
-            double rn = precond ? std::sqrt(dot(rr, rr)) : std::sqrt(std::max(sig, 0.0));
+            double rn = precond ? std::sqrt(sdot(s, rr, rr)) : std::sqrt(std::max(sig, 0.0));
             return rn <= d->tolerance * bnorm;
         }
         return !(sig > d->tolerance * d->tolerance);
@@ -979,7 +1011,7 @@ void cg(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
     bool trigger = false;
     while (!done(sigmaNew, r) && it < d->max_iterations) {
         spmv(s, dd, q, s->nthreads);
-        double alpha = sigmaNew / dot(dd, q);
+        double alpha = sigmaNew / sdot(s, dd, q);
         pfor([&](size_t i) { s->p.d[i] = alpha * dd.d[i] + s->p.d[i]; });
         if (trigger || (it % 50 == 0 && it > 0)) {
             residual(s, s->p, s->b, r, s->nthreads);
@@ -989,7 +1021,7 @@ void cg(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
         }
         apply_m(r, z);
         double sigmaOld = sigmaNew;
-        sigmaNew = dot(r, z);
+        sigmaNew = sdot(s, r, z);
         if (sigmaNew > sigmaOld) trigger = true;
         double beta = sigmaNew / sigmaOld;
         pfor([&](size_t i) { dd.d[i] = beta * dd.d[i] + z.d[i]; });
@@ -1030,7 +1062,7 @@ void pressure(orc_scene* s, const orc_solver_desc* d, orc_solve_stats* st) {
     Arr3 r;
     r.resize(s->n[0], s->n[1], s->n[2]);
     residual(s, s->p, s->b, r, s->nthreads);
-    double rn = std::sqrt(dot(r, r)), bn = std::sqrt(dot(s->b, s->b));
+    double rn = std::sqrt(sdot(s, r, r)), bn = std::sqrt(sdot(s, s->b, s->b));
     apply_pressure(s);
     auto t1 = std::chrono::steady_clock::now();
     local.abs_residual = rn;
@@ -1759,6 +1791,8 @@ void orc_destroy(orc_scene* s) { delete s; }
 void orc_set_threads(orc_scene* s, int nthreads) { s->nthreads = std::max(1, nthreads); }
 
 void orc_plant_mistake(orc_scene* s, int which) { s->planted = which; }
+
+void orc_set_fast_dots(orc_scene* s, int on) { s->fast_dots = on ? 1 : 0; }
 
 void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np) {
     s->np = np;

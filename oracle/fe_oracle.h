@@ -54,6 +54,10 @@ enum {
     ORC_PLANT_APIC_FACE_DIMS = 4     // APIC gradient stencil sized by the face array on the normal axis
 };
 void orc_plant_mistake(orc_scene* s, int which);
+// Solver dot products in storage order over 64 fixed blocks (deterministic
+// for any thread count) instead of the reference's serial i-outer order: the
+// same converged answer, fast enough for tests on 128^3 grids.  Off by default.
+void orc_set_fast_dots(orc_scene* s, int on);
 void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np);
 void orc_get_particles(const orc_scene* s, double* x, double* v, double* C);
 int64_t orc_num_particles(const orc_scene* s);

@@ -66,6 +66,7 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.orc_plant_mistake.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_fast_dots.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_particles.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
         L.orc_get_particles.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_num_particles.argtypes = [C.c_void_p]
@@ -175,6 +176,10 @@ class Scene:
 
     def set_threads(self, t):
         lib().orc_set_threads(self._s, t)
+
+    def set_fast_dots(self, on=True):
+        """Solver dot products in storage order over fixed blocks (tests on large grids)."""
+        lib().orc_set_fast_dots(self._s, 1 if on else 0)
 
     def plant_mistake(self, which):
         """Pinning tests only: switch on one deliberate restatement mistake (PLANT_*)."""

@@ -1,0 +1,261 @@
+"""The FLIP dam break at its benchmark size -- 128^3, PIC weight 0.03,
+4,021,248 particles (2x2x2 jittered per cell of x < 0.4, y < 0.6, seed 0),
+dt = 1/480, multigrid-preconditioned CG to a 1e-6 relative residual -- against
+the FP64 oracle, plus the CFL substep rule and a per-step re-seeded 32^3 PIC
+run.
+
+Bars (stated per test):
+  * particle cell indices of the seeded state, cell markers, P2G face markers
+    and the negative fluid SDF: bit-exact;
+  * cell indices after a substep: identical, except for particles whose FP32
+    (GPU) and FP64 (oracle) positions straddle a cell face -- each such
+    particle lies within the position difference of that face;
+  * relative residual ||b - A p||_2 / ||b||_2 recomputed on the host in FP64
+    from the downloaded FP32 face velocities, cell markers and FP64 pressure
+    (the BuildSystem rule of src/fluid_solvers/single_phase_pressure_solver.cpp:99-164
+    written out in numpy, not the GPU's own residual kernel);
+  * particle velocities and positions after a substep: relative to the largest
+    oracle value (FP32 transfers, FP64 oracle, solver tolerance 1e-6 on the GPU
+    against 1e-8 in the oracle);
+  * CFL number: bit-identical to the oracle's on the same FP32 face values
+    (both take the cell-centre average and the maximum in FP64), and the
+    frame's substep count identical.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from fluide_engine_amd import capi  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+from pressure_residual import host_relative_residual  # noqa: E402
+
+N = 128
+DT = 1.0 / 480.0
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = capi.Context(0)
+    print(f"\n[gpu] device {c.device_name}", flush=True)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def dam128():
+    desc = capi.scene_desc((N, N, N), method=capi.FLIP, blend=0.03, max_particles=8 * N ** 3)
+    x = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.4, 0.6, 2.0))], seed=0)
+    assert x.shape[0] == 4021248
+    return desc, x
+
+
+def _cells(x, n):
+    idx = np.clip(np.floor(x.astype(np.float64) * n), 0, n - 1).astype(np.int64)
+    return idx[:, 0] + n * (idx[:, 1] + n * idx[:, 2])
+
+
+def test_c2_seeded_cells_bit_exact(ctx, dam128):
+    print("[gpu] C2 seeded cell indices", flush=True)
+    desc, x = dam128
+    g = capi.Scene(ctx, desc)
+    g.upload_particles(x, np.zeros_like(x))
+    assert np.array_equal(g.particle_cells().astype(np.int64), _cells(x, N))
+    g.close()
+
+
+def test_c2_first_substep_against_oracle(ctx, dam128):
+    print("[gpu] C2 first substep, GPU", flush=True)
+    desc, x = dam128
+    g = capi.Scene(ctx, desc)
+    g.upload_particles(x, np.zeros_like(x))
+    sg = g.substep(DT, capi.solver(capi.SOLVER_MGPCG, 200, 0, 1e-6, relative=True, warm_start=False))
+    assert sg.converged == 1 and sg.rel_residual <= 1e-6, (sg.iterations, sg.rel_residual)
+    gflags = g.get(capi.GRID_FLAGS)
+    gmarks = [g.get(m) for m in (capi.GRID_MU, capi.GRID_MV, capi.GRID_MW)]
+    gsdf = g.get(capi.GRID_SDF)
+    gx, gv, _ = g.download_particles()
+    gcells = g.particle_cells().astype(np.int64)
+    g.close()
+
+    print(f"[gpu] C2 first substep, oracle ({THREADS} threads)", flush=True)
+    o = orc.Scene((N, N, N), method=orc.FLIP, blend=0.03, threads=THREADS)
+    o.set_fast_dots()
+    o.set_particles(x.astype(np.float64), np.zeros((x.shape[0], 3)))
+    so = o.substep(DT, orc.solver(orc.CG, 5000, 0, 1e-8, relative=True, warm_start=False))
+    assert so.rel_residual <= 1e-8
+
+    assert np.array_equal(gflags, o.get(orc.FLAGS))
+    for gm, om in zip(gmarks, (orc.MU, orc.MV, orc.MW)):
+        assert np.array_equal(gm, o.get(om).astype(np.uint8))
+    osdf = o.get(orc.SDF)
+    neg = osdf < 0
+    assert np.array_equal(gsdf[neg], osdf[neg])
+
+    ox, ov, _ = o.particles()
+    ev = float(np.abs(gv - ov).max() / np.abs(ov).max())
+    ex = float(np.abs(gx - ox).max())
+    print(f"[gpu] C2 substep: |dv|/|v|max = {ev:.3e}, |dx|max = {ex:.3e}, "
+          f"GPU {sg.iterations} it rel {sg.rel_residual:.3e}, oracle {so.iterations} it", flush=True)
+    assert ev <= 2e-3, ev
+    assert ex <= 1e-6, ex
+
+    # cell indices of the advected particles
+    assert np.array_equal(gcells, _cells(gx, N))  # the GPU's own positions: bit-exact
+    ocells = _cells(ox, N)
+    diff = np.nonzero(gcells != ocells)[0]
+    print(f"[gpu] C2 cell indices differing from the oracle's: {diff.size} of {x.shape[0]}", flush=True)
+    assert diff.size <= 1e-4 * x.shape[0]
+    if diff.size:
+        # each one straddles a cell face: the face lies between the two positions
+        dist = np.abs(gx[diff].astype(np.float64) - ox[diff])
+        face = np.round(ox[diff] * N) / N
+        assert np.all(np.any(np.abs(ox[diff] - face) <= dist + 1e-12, axis=1))
+
+
+def test_c2_pressure_residual_host_fp64(ctx, dam128):
+    """Two pressure solves of the 128^3 dam break (the first from zero, the
+    second warm-started from the first's pressure one substep later), each
+    checked by the host FP64 residual."""
+    desc, x = dam128
+    g = capi.Scene(ctx, desc)
+    g.upload_particles(x, np.zeros_like(x))
+    h = (1.0 / N,) * 3
+    for sub, warm in ((0, False), (1, True)):
+        print(f"[gpu] C2 pressure solve {sub} (warm start {warm})", flush=True)
+        if sub:
+            for stage in (capi.STAGE_BC, capi.STAGE_EXTRAPOLATE, capi.STAGE_BC):
+                g.run_stage(stage)
+            g.run_stage(capi.STAGE_G2P, dt=DT)
+            g.run_stage(capi.STAGE_MOVE, dt=DT)
+        for stage in (capi.STAGE_P2G, capi.STAGE_SDF, capi.STAGE_EXTRAPOLATE, capi.STAGE_BC):
+            g.run_stage(stage)
+        g.run_stage(capi.STAGE_GRAVITY, dt=DT)
+        g.run_stage(capi.STAGE_BC)
+        u, v, w = g.get(capi.GRID_U), g.get(capi.GRID_V), g.get(capi.GRID_W)
+        st = g.run_stage(capi.STAGE_PRESSURE,
+                         desc=capi.solver(capi.SOLVER_MGPCG, 200, 0, 1e-6, relative=True, warm_start=warm))
+        flags = g.get(capi.GRID_FLAGS)
+        p = g.get(capi.GRID_PRESSURE)
+        assert p.dtype == np.float64
+        rel, b = host_relative_residual(flags, u, v, w, p, h)
+        print(f"[gpu]   {st.iterations} iterations, GPU rel {st.rel_residual:.4e}, host FP64 rel {rel:.4e}",
+              flush=True)
+        assert st.converged == 1
+        # the GPU solves with b rounded to FP32: that moves the residual by at
+        # most |b - fl32(b)| / |b| <= 2^-24 relative
+        assert rel <= 1e-6 + 1e-7, rel
+        assert abs(rel - st.rel_residual) <= 1e-7, (rel, st.rel_residual)
+        assert np.all(p[flags != capi.FLUID] == 0.0)
+        if warm:
+            assert st.iterations <= 13
+    g.close()
+
+
+def _orc_with_grid(g, n, max_cfl=3.0):
+    o = orc.Scene((n, n, n), method=orc.FLIP, max_cfl=max_cfl)
+    for gf, of in ((capi.GRID_U, orc.U), (capi.GRID_V, orc.V), (capi.GRID_W, orc.W)):
+        o.set(of, g.get(gf).astype(np.float64))
+    return o
+
+
+def test_cfl_bit_exact_and_boundaries(ctx):
+    """PICSimulator::Cfl (src/pic_simulator.cpp:80-97) on the same FP32 face
+    values: random fields, and fields that put CFL / maxCfl exactly on an
+    integer and one FP32 step above it, where ceil() changes the count."""
+    print("[gpu] CFL", flush=True)
+    n = 32
+    desc = capi.scene_desc((n, n, n), method=capi.FLIP, max_particles=16)
+    g = capi.Scene(ctx, desc)
+    rng = np.random.default_rng(21)
+    dt = 1.0 / 64.0
+    cases = []
+    for scale in (0.3, 3.0, 40.0):
+        cases.append([(rng.normal(size=g.shape(f)) * scale).astype(np.float32)
+                      for f in (capi.GRID_U, capi.GRID_V, capi.GRID_W)])
+    # u = 12 everywhere: CFL = 12 * (1/64) / (1/32) = 6 = 2 * maxCfl exactly
+    for val in (np.float32(12.0), np.nextafter(np.float32(12.0), np.float32(13.0))):
+        cases.append([np.full(g.shape(capi.GRID_U), val, np.float32),
+                      np.zeros(g.shape(capi.GRID_V), np.float32), np.zeros(g.shape(capi.GRID_W), np.float32)])
+    # v = -12: the y term is |v| - 9.8 dt = 11.85, CFL 5.92, two substeps
+    cases.append([np.zeros(g.shape(capi.GRID_U), np.float32),
+                  np.full(g.shape(capi.GRID_V), -12.0, np.float32), np.zeros(g.shape(capi.GRID_W), np.float32)])
+    counts = []
+    for fields in cases:
+        for f, a in zip((capi.GRID_U, capi.GRID_V, capi.GRID_W), fields):
+            g.set(f, a)
+        o = _orc_with_grid(g, n)
+        cg_, co = g.cfl(dt), o.cfl(dt)
+        assert cg_ == co, (cg_, co)
+        counts.append(int(max(np.ceil(cg_ / 3.0), 1.0)))
+    assert counts[3] == 2 and counts[4] == 3 and counts[5] == 2, counts
+    g.close()
+
+
+def test_advance_frame_substep_count(ctx):
+    """A 32^3 FLIP dam break thrown sideways at 10 m/s.  The first frame sees
+    the grid before any transfer (zero velocity: one substep, as in the
+    reference, where Cfl reads the grid velocity); the next frames need
+    several CFL substeps.  The GPU and the oracle take the same number in each
+    of three frames."""
+    print("[gpu] advance_frame substep counts", flush=True)
+    n = 32
+    desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.03, max_particles=8 * n ** 3)
+    x = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.4, 0.6, 2.0))], seed=0)
+    v = np.zeros_like(x)
+    v[:, 0] = 10.0
+    g = capi.Scene(ctx, desc)
+    g.upload_particles(x, v)
+    o = orc.Scene((n, n, n), method=orc.FLIP, blend=0.03, threads=THREADS)
+    o.set_particles(x.astype(np.float64), v.astype(np.float64))
+    sv_g = capi.solver(capi.SOLVER_CG, 2000, 0, 1e-7, relative=True)
+    sv_o = orc.solver(orc.CG, 2000, 0, 1e-9, relative=True)
+    counts = []
+    for frame in range(3):
+        cg_ = g.advance_frame(1.0 / 60.0, sv_g)
+        co = o.advance_frame(1.0 / 60.0, sv_o)
+        print(f"[gpu]   frame {frame}: GPU {cg_} substeps, oracle {co}", flush=True)
+        assert cg_ == co
+        counts.append(cg_)
+    assert counts[0] == 1 and max(counts) >= 2, counts
+    g.close()
+
+
+def test_dam_break_pic_32_reseeded_per_step(ctx):
+    """The 32^3 PIC dam break (dt = 1/240, Jacobi 200 sweeps, 50 substeps)
+    with the oracle re-seeded from the GPU state (particles and the Jacobi
+    warm-start pressure) before every substep, so each step is compared from
+    the same input: markers bit-exact, velocities within 2e-4 and positions
+    within 2e-6 of the largest oracle value, every step."""
+    print("[gpu] 32^3 PIC 50 substeps, re-seeded", flush=True)
+    n = 32
+    desc = capi.scene_desc((n, n, n), method=capi.PIC, max_particles=8 * n ** 3)
+    x = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.4, 0.6, 2.0))], seed=0)
+    assert x.shape[0] == 63232
+    g = capi.Scene(ctx, desc)
+    g.upload_particles(x, np.zeros_like(x))
+    o = orc.Scene((n, n, n), method=orc.PIC, threads=THREADS)
+    sv_g = capi.solver(capi.SOLVER_JACOBI, 200, 5, 1e-14)
+    sv_o = orc.solver(orc.JACOBI, 200, 5, 1e-14)
+    worst_v = worst_x = 0.0
+    for step in range(50):
+        gx, gv, _ = g.download_particles()
+        o.set_particles(gx.astype(np.float64), gv.astype(np.float64))
+        o.set(orc.PRESSURE, g.get(capi.GRID_PRESSURE).astype(np.float64))
+        g.substep(1.0 / 240, sv_g)
+        o.substep(1.0 / 240, sv_o)
+        assert np.array_equal(g.get(capi.GRID_FLAGS), o.get(orc.FLAGS)), step
+        for gm, om in ((capi.GRID_MU, orc.MU), (capi.GRID_MV, orc.MV), (capi.GRID_MW, orc.MW)):
+            assert np.array_equal(g.get(gm), o.get(om).astype(np.uint8)), step
+        gx, gv, _ = g.download_particles()
+        ox, ov, _ = o.particles()
+        ev = float(np.abs(gv - ov).max() / max(np.abs(ov).max(), 1e-30))
+        ex = float(np.abs(gx - ox).max())
+        worst_v, worst_x = max(worst_v, ev), max(worst_x, ex)
+        assert ev <= 2e-4, (step, ev)
+        assert ex <= 2e-6, (step, ex)
+    print(f"[gpu] worst per-step |dv|/|v|max = {worst_v:.3e}, |dx|max = {worst_x:.3e}", flush=True)
+    g.close()

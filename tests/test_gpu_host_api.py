@@ -234,3 +234,25 @@ def test_python_mesh_scene_and_emitter_match_capi(tmp_path):
     assert np.array_equal(cx, px) and np.array_equal(cv, pv)
     sc.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("method", ["pic", "flip", "apic"])
+def test_reference_pattern_demo(method, tmp_path):
+    """The reference's driver pattern (src/main.cpp:16-58) compiled against the
+    reference's own header paths: an OBJ emitter, gravity, viscosity 0.01,
+    maxCfl 5, SinglePhasePressureSolver with a CudaConjugateGradientSolver,
+    BackwardEulerDiffusionSolver and a ColliderCollection; SimulationRunner
+    writes one OBJ per frame."""
+    print(f"[gpu] reference pattern demo {method}", flush=True)
+    frames = 3
+    out = subprocess.run([os.path.join(LIB, "reference_pattern_demo"), method, "30", str(frames), str(tmp_path)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    assert res == {"simulator": f"{method}_out", "frames": frames, "particles": res["particles"], "colliders": 1,
+                   "max_cfl": 5, "viscosity": 0.01}
+    assert res["particles"] > 1000  # the emitter has filled its box
+    folder = tmp_path / f"{method}_out"
+    for i in range(frames):
+        assert (folder / "simulated_frames" / f"{method}_out_{i}.obj").exists()
+    assert "SIMULATION STATS" in (folder / "log.txt").read_text()
