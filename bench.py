@@ -102,7 +102,9 @@ def parse():
     ap.add_argument("--kernel", default="p2g", choices=sorted(KERNEL_IDS),
                     help="kernel of the roofline object (default: P2G, the largest share of the substep)")
     ap.add_argument("--solver", default="mgpcg", choices=sorted(SOLVERS))
-    ap.add_argument("--cpu-substeps", type=int, default=5)
+    # one substep (about 20 s on 16 host threads) keeps the CPU sample within
+    # half a minute; the rate is per substep, so more substeps only average it
+    ap.add_argument("--cpu-substeps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solver-comparison", action="store_true",
@@ -127,10 +129,10 @@ def parse():
 KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
                   "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
-TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d rocprof/kt -o kt -- "
+TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
              "--batch-frames 0 --learned-steps 0")
-PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d rocprof/pmc_{counter} -o pmc -- "
+PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{counter} -o pmc -- "
            "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --no-solver-comparison "
            "--batch-frames 0 --learned-steps 0 --copy-calibration 256")
 

@@ -14,9 +14,9 @@ Bars (stated per test):
     from the downloaded FP32 face velocities, cell markers and FP64 pressure
     (the BuildSystem rule of src/fluid_solvers/single_phase_pressure_solver.cpp:99-164
     written out in numpy, not the GPU's own residual kernel);
-  * particle velocities and positions after a substep: relative to the largest
-    oracle value (FP32 transfers, FP64 oracle, solver tolerance 1e-6 on the GPU
-    against 1e-8 in the oracle);
+  * particle velocities and positions after a substep: velocities within 2e-6
+    of the largest oracle velocity, positions within 5e-7 (FP32 transfers,
+    FP64 oracle, solver tolerance 1e-6 on the GPU against 1e-8 in the oracle);
   * CFL number: bit-identical to the oracle's on the same FP32 face values
     (both take the cell-centre average and the maximum in FP64), and the
     frame's substep count identical.
@@ -100,8 +100,9 @@ def test_c2_first_substep_against_oracle(ctx, dam128):
     ex = float(np.abs(gx - ox).max())
     print(f"[gpu] C2 substep: |dv|/|v|max = {ev:.3e}, |dx|max = {ex:.3e}, "
           f"GPU {sg.iterations} it rel {sg.rel_residual:.3e}, oracle {so.iterations} it", flush=True)
-    assert ev <= 2e-3, ev
-    assert ex <= 1e-6, ex
+    # measured on the MI355X: 1.5e-7 and 8.9e-8
+    assert ev <= 2e-6, ev
+    assert ex <= 5e-7, ex
 
     # cell indices of the advected particles
     assert np.array_equal(gcells, _cells(gx, N))  # the GPU's own positions: bit-exact
@@ -228,8 +229,9 @@ def test_dam_break_pic_32_reseeded_per_step(ctx):
     """The 32^3 PIC dam break (dt = 1/240, Jacobi 200 sweeps, 50 substeps)
     with the oracle re-seeded from the GPU state (particles and the Jacobi
     warm-start pressure) before every substep, so each step is compared from
-    the same input: markers bit-exact, velocities within 2e-4 and positions
-    within 2e-6 of the largest oracle value, every step."""
+    the same input: markers bit-exact, velocities within 1e-5 and positions
+    within 1e-6 of the largest oracle value, every step (measured worst on the
+    MI355X: 8.2e-7 and 8.9e-8)."""
     print("[gpu] 32^3 PIC 50 substeps, re-seeded", flush=True)
     n = 32
     desc = capi.scene_desc((n, n, n), method=capi.PIC, max_particles=8 * n ** 3)
@@ -255,7 +257,7 @@ def test_dam_break_pic_32_reseeded_per_step(ctx):
         ev = float(np.abs(gv - ov).max() / max(np.abs(ov).max(), 1e-30))
         ex = float(np.abs(gx - ox).max())
         worst_v, worst_x = max(worst_v, ev), max(worst_x, ex)
-        assert ev <= 2e-4, (step, ev)
-        assert ex <= 2e-6, (step, ex)
+        assert ev <= 1e-5, (step, ev)
+        assert ex <= 1e-6, (step, ex)
     print(f"[gpu] worst per-step |dv|/|v|max = {worst_v:.3e}, |dx|max = {worst_x:.3e}", flush=True)
     g.close()
