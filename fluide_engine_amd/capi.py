@@ -43,7 +43,7 @@ EXPORTED = [
     "fe_net_param_count", "fe_net_get_params", "fe_net_set_params", "fe_net_add_sample", "fe_net_sample_count",
     "fe_net_clear_samples", "fe_net_train", "fe_net_loss_and_grads", "fe_scene_set_warm_start_net",
     "fe_scene_warm_start_scale", "fe_net_set_scale_mode", "fe_net_set_loss", "fe_scene_get_tuning",
-    "fe_scene_set_tuning", "fe_scene_autotune",
+    "fe_scene_set_tuning", "fe_scene_autotune", "fe_mesh_sdf_noalloc",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH) = range(8)
@@ -137,6 +137,7 @@ def load():
     dp = C.POINTER(C.c_double)
     L.fe_scene_add_mesh_collider.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64]
     L.fe_mesh_sdf.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64, dp]
+    L.fe_mesh_sdf_noalloc.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64, dp]
     L.fe_scene_initialize_from_mesh.argtypes = [vp, dp, C.c_int64, i32p, C.c_int64, C.c_int32, C.c_uint64]
     L.fe_scene_add_volume_emitter.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64, C.c_int32, dp,
                                               C.c_uint64]
@@ -267,6 +268,16 @@ def mesh_sdf(ctx, size, origin, spacing, vertices, triangles):
     out = np.empty((int(size[2]), int(size[1]), int(size[0])), np.float64)
     args = _mesh_args(size, origin, spacing, vertices, triangles)
     ctx.check(ctx.L.fe_mesh_sdf(ctx._h, *args, out.ctypes.data_as(C.POINTER(C.c_double))), "fe_mesh_sdf")
+    return out
+
+
+def mesh_sdf_noalloc(ctx, size, origin, spacing, vertices, triangles):
+    """mesh_sdf without device allocation (the device's static scratch array;
+    at most 65536 vertices, 65536 triangles and 2^21 nodes per call)."""
+    out = np.empty((int(size[2]), int(size[1]), int(size[0])), np.float64)
+    args = _mesh_args(size, origin, spacing, vertices, triangles)
+    ctx.check(ctx.L.fe_mesh_sdf_noalloc(ctx._h, *args, out.ctypes.data_as(C.POINTER(C.c_double))),
+              "fe_mesh_sdf_noalloc")
     return out
 
 

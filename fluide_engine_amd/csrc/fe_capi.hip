@@ -377,6 +377,9 @@ int fe_ctx_create(int device, fe_ctx** out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_err(nullptr, FE_E_NO_DEVICE,
                        std::string("device is ") + prop.gcnArchName + "; this library is built for gfx950 only");
+    // the device address of the no-allocation entry point's scratch array
+    if ((e = hipSetDevice(device)) != hipSuccess || (e = noalloc_setup(device)) != hipSuccess)
+        return hip_fail(nullptr, e, "scratch set-up (hipGetSymbolAddress)");
     auto* ctx = new fe_ctx();
     ctx->device = device;
     std::snprintf(ctx->name, sizeof(ctx->name), "%s", prop.gcnArchName);
@@ -870,6 +873,59 @@ int fe_mesh_sdf(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origi
     if (c) (void)hipFree(c);
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+// fe_mesh_sdf with every device buffer in the device's scratch array
+// (fe_noalloc.hip): no device allocation, no stream object, synchronous.
+int fe_mesh_sdf_noalloc(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origin[3],
+                        const double grid_spacing[3], const double* vertices, int64_t vertex_count,
+                        const int32_t* triangles, int64_t triangle_count, double* sdf) {
+    if (!ctx || !sdf || !grid_size || !grid_origin || !grid_spacing || vertex_count < 0 || triangle_count < 0 ||
+        (vertex_count > 0 && !vertices) || (triangle_count > 0 && !triangles))
+        return set_err(ctx, FE_E_INVALID, "fe_mesh_sdf_noalloc: null pointer or negative count");
+    NodeGrid3 G{};
+    for (int a = 0; a < 3; ++a) {
+        if (grid_size[a] < 1 || !(grid_spacing[a] > 0.0) || !std::isfinite(grid_spacing[a]) ||
+            !std::isfinite(grid_origin[a]))
+            return set_err(ctx, FE_E_INVALID,
+                           "fe_mesh_sdf_noalloc: grid sizes must be >= 1, spacings finite and > 0");
+        G.n[a] = grid_size[a];
+        G.o[a] = grid_origin[a];
+        G.h[a] = grid_spacing[a];
+    }
+    const int64_t nn = (int64_t)grid_size[0] * grid_size[1] * grid_size[2];
+    if (nn > kNaMaxNodes || vertex_count > kNaMaxVerts || triangle_count > kNaMaxTris)
+        return set_err(ctx, FE_E_INVALID,
+                       "fe_mesh_sdf_noalloc: at most " + std::to_string(kNaMaxNodes) + " grid nodes, " +
+                           std::to_string(kNaMaxVerts) + " vertices and " + std::to_string(kNaMaxTris) +
+                           " triangles per call (fe_mesh_sdf has no such limit)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    NoallocSlices sl{};
+    HIPCHK(ctx, noalloc_slices(ctx->device, &sl));
+    // Mesh2SDF::Build's fill value (largest spacing times the summed sizes), as mesh_prepare
+    const double hx = G.h[0], hy = G.h[1], hz = G.h[2];
+    const double hmax = (hx >= hy && hx >= hz) ? hx : ((hy >= hx && hy >= hz) ? hy : hz);
+    const double fill = hmax * (double)((size_t)G.n[0] + (size_t)G.n[1] + (size_t)G.n[2]);
+    const hipStream_t st = nullptr;  // the null stream: every copy below is synchronous
+    if (triangle_count > 0) {
+        const int32_t zero = 0;
+        HIPCHK(ctx, hipMemcpy(sl.bad, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        if (vertex_count > 0)
+            HIPCHK(ctx, hipMemcpy(sl.verts, vertices, sizeof(double) * 3 * (size_t)vertex_count,
+                                  hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(sl.tris, triangles, sizeof(int32_t) * 3 * (size_t)triangle_count,
+                              hipMemcpyHostToDevice));
+        HIPCHK(ctx, launch_mesh_prepare(st, G, sl.verts, vertex_count, sl.tris, triangle_count, sl.tv, sl.tb, sl.bad));
+        int32_t flag = 0;
+        HIPCHK(ctx, hipMemcpy(&flag, sl.bad, sizeof(flag), hipMemcpyDeviceToHost));
+        if (flag)
+            return set_err(ctx, FE_E_INVALID,
+                           "mesh collider: a triangle's vertex index is out of range, or a coordinate / h.x does not "
+                           "fit an int");
+    }
+    HIPCHK(ctx, launch_mesh_sdf(st, G, sl.tv, sl.tb, triangle_count, fill, sl.sdf));
+    HIPCHK(ctx, hipMemcpy(sdf, sl.sdf, sizeof(double) * (size_t)nn, hipMemcpyDeviceToHost));
+    return FE_OK;
 }
 
 int fe_scene_initialize_from_mesh(fe_scene* s, const double* vertices, int64_t vertex_count,

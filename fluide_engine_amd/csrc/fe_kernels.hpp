@@ -69,6 +69,27 @@ hipError_t launch_bc(fe_scene* s);
 hipError_t launch_collide(fe_scene* s);
 hipError_t launch_sdf_into_collider(fe_scene* s);  // ExtrapolateIntoCollider on the fluid SDF
 void collider_free(fe_scene* s);
+// fe_mesh_sdf_noalloc (fe_noalloc.hip): the limits of one call and the slices
+// of the per-device scratch array that hold its buffers.  noalloc_setup runs
+// once per device (fe_ctx_create) with that device current.
+constexpr int64_t kNaMaxVerts = 65536;
+constexpr int64_t kNaMaxTris = 65536;
+constexpr int64_t kNaMaxNodes = int64_t(1) << 21;  // e.g. 128^3
+struct NoallocSlices {
+    double* verts;
+    int32_t* tris;
+    double* tv;
+    int32_t* tb;
+    int32_t* bad;
+    double* sdf;
+};
+hipError_t noalloc_setup(int device);
+hipError_t noalloc_slices(int device, NoallocSlices* out);
+// mesh_prepare on the device: raw vertices and triangles in, packed triangles
+// and index boxes out; *bad set on a bad index or coordinate.
+hipError_t launch_mesh_prepare(hipStream_t st, const NodeGrid3& G, const double* verts, int64_t nv,
+                               const int32_t* tris, int64_t nt, double* tv, int32_t* tb, int32_t* bad);
+
 // Mesh2SDF on the device: mesh_prepare packs the triangles and their index
 // boxes on the host (false on a bad index or coordinate), launch_mesh_sdf
 // writes the collider grid's SDF, collider_merge folds it into the scene's

@@ -230,6 +230,17 @@ int fe_scene_add_mesh_collider(fe_scene* scene, const int32_t grid_size[3], cons
 int fe_mesh_sdf(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origin[3], const double grid_spacing[3],
                 const double* vertices, int64_t vertex_count, const int32_t* triangles, int64_t triangle_count,
                 double* sdf);
+/* fe_mesh_sdf without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc), e.g. to rebuild a moving collider's SDF every frame.
+ * Same arguments and results as fe_mesh_sdf, bit for bit.  Its device buffers
+ * live in one static array per GPU, held until the process ends: at most
+ * 65536 vertices, 65536 triangles and 2^21 grid nodes per call (FE_E_INVALID
+ * beyond, before any device work).  Synchronous, on the null stream.  Calls
+ * that overlap on one GPU share that array and race: serialise them. */
+int fe_mesh_sdf_noalloc(fe_ctx* ctx, const int32_t grid_size[3], const double grid_origin[3],
+                        const double grid_spacing[3], const double* vertices, int64_t vertex_count,
+                        const int32_t* triangles, int64_t triangle_count, double* sdf);
 
 /* Candidate search of the fluid SDF (BuildSignedDistanceField,
  * src/pic_simulator.cpp:452-475).  FE_SDF_SEARCH_CELLS (the default) scans the
