@@ -39,27 +39,31 @@ KERNEL_BYTES_PER_CELL = {
     "jacobi": 14,
     # k_mg_smooth on the finest level: read x(4) b(4) row code(2), write x'(4)
     "mg_smooth": 14,
+    # k_mg_down on the finest level: read b(4) row code(2), write x2(4) and the coarse b (4/8)
+    "mg_down": 10.5,
+    # k_mg_up on the finest level: read x2(4) b(4) row code(2) and the coarse correction (4/8), write z(4)
+    "mg_up": 14.5,
 }
 KERNEL_IDS = {"none": 0, "cg_update": 2, "cg_spmv": 1, "jacobi": 6, "p2g": 3, "g2p_move": 4, "sdf": 5,
-              "mg_smooth": 7}
+              "mg_smooth": 7, "mg_down": 8, "mg_up": 9}
 SOLVERS = {"jacobi": 0, "cg": 1, "pcg": 2, "mgpcg": 3}
 SOLVER_TEXT = {"jacobi": "Jacobi sweeps", "cg": "CG", "pcg": "Jacobi-PCG", "mgpcg": "multigrid-PCG"}
 
 
-def kernel_bytes_per_cell(kernel, solver):
+def kernel_bytes_per_cell(kernel, solver, mg_split=False):
     b = KERNEL_BYTES_PER_CELL[kernel]
     if kernel == "cg_update" and solver == "pcg":
         b += 6
-    if kernel == "cg_update" and solver == "mgpcg":  # row code(2) + the V-cycle's first sweep(4)
+    if kernel == "cg_update" and solver == "mgpcg" and mg_split:  # row code(2) + the V-cycle's first sweep(4)
         b += 6
     return b
 
 
-def kernel_bytes_per_launch(kernel, solver, n, npart):
+def kernel_bytes_per_launch(kernel, solver, n, npart, mg_split=False):
     """Algorithmic HBM bytes of one launch of a timed kernel on the n^3 FLIP scene."""
     cells = n ** 3
     if kernel in KERNEL_BYTES_PER_CELL:
-        return kernel_bytes_per_cell(kernel, solver) * cells
+        return int(kernel_bytes_per_cell(kernel, solver, mg_split) * cells)
     faces = (n + 1) * n * n  # per component
     if kernel == "p2g":  # one of three component launches: a third of the x, v read (8 of 24 per particle),
         return 8 * npart + 8 * faces  # the face velocity and the FLIP snapshot written (8 per face)
@@ -78,8 +82,8 @@ def kernel_impl_bytes_per_launch(kernel, n, npart):
     return None
 
 
-def roofline(kernel, solver, n, npart, kms, launches, traffic):
-    bpl = kernel_bytes_per_launch(kernel, solver, n, npart)
+def roofline(kernel, solver, n, npart, kms, launches, traffic, mg_split=False):
+    bpl = kernel_bytes_per_launch(kernel, solver, n, npart, mg_split)
     if bpl is None or launches <= 0:
         return None
     avg_ms = kms / launches
@@ -102,6 +106,8 @@ def parse():
     ap.add_argument("--kernel", default="p2g", choices=sorted(KERNEL_IDS),
                     help="kernel of the roofline object (default: P2G, the largest share of the substep)")
     ap.add_argument("--solver", default="mgpcg", choices=sorted(SOLVERS))
+    ap.add_argument("--mg", default="split", choices=["fused", "split"],
+                    help="multigrid V-cycle kernels: one descent and one ascent per level, or one per step")
     # one substep (about 20 s on 16 host threads) keeps the CPU sample within
     # half a minute; the rate is per substep, so more substeps only average it
     ap.add_argument("--cpu-substeps", type=int, default=1)
@@ -127,7 +133,8 @@ def parse():
 
 
 KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
-                  "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth"}
+                  "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth", "mg_down": "k_mg_down",
+                  "mg_up": "k_mg_up"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
@@ -252,6 +259,7 @@ def main():
 
     ctx = capi.Context(0)
     scene = capi.Scene(ctx, desc)
+    scene.set_mg_variant(capi.MG_FUSED if args.mg == "fused" else capi.MG_SPLIT)
     scene.upload_particles(x, v)
     print(f"bench: {ctx.device_name} FLIP {n}^3 {npart} particles, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
@@ -288,13 +296,15 @@ def main():
     value = npart * sub_total / wall / 1e6
     on_profile = args.n == 128 and args.solver == "mgpcg"
     roof = roofline(args.kernel, args.solver, n, npart, kms, klaunch,
-                    traffic_for(args.kernel) if on_profile else None)
+                    traffic_for(args.kernel) if on_profile else None, scene.mg_variant() == capi.MG_SPLIT)
     # the other timed kernels' rooflines, each over a few more substeps after the timed region
     rooflines = {}
     if args.kernel != "none":
-        others = ["g2p_move", "sdf"] + ([] if args.solver == "jacobi" else ["cg_update", "cg_spmv", "mg_smooth"])
+        others = ["g2p_move", "sdf"] + ([] if args.solver == "jacobi" else ["cg_update", "cg_spmv"])
+        if args.solver == "mgpcg":
+            others += ["mg_down", "mg_up"] if scene.mg_variant() == capi.MG_FUSED else ["mg_smooth"]
         for kern in others:
-            if kern == args.kernel or (kern == "mg_smooth" and args.solver != "mgpcg"):
+            if kern == args.kernel:
                 continue
             scene.kernel_timer(KERNEL_IDS[kern])
             for i in range(min(20, max(args.steps, 1))):
@@ -302,7 +312,8 @@ def main():
             scene.sync()
             cms, clau = scene.kernel_timer_read(reset=True)
             rooflines[kern] = roofline(kern, args.solver, n, npart, cms, clau,
-                                       traffic_for(kern) if on_profile else None)
+                                       traffic_for(kern) if on_profile else None,
+                                       scene.mg_variant() == capi.MG_SPLIT)
     scene.kernel_timer(KERNEL_IDS["none"])
 
     solve_cmp = None if (args.solver == "jacobi" or args.no_solver_comparison) else solver_comparison(scene, capi, dt)
@@ -334,7 +345,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"FLIP (PIC weight 0.03) dam break {n}^3, {npart} particles, dt=1/480, "
                                f"{SOLVER_TEXT[args.solver]} to 1e-6 relative residual (FP64-checked), warm start",
-                   "solver": args.solver,
+                   "solver": args.solver, "mg_kernels": args.mg if args.solver == "mgpcg" else None,
                    "grid": [n, n, n], "particles": npart, "substeps": sub_total},
         "pressure_ms_median": round(statistics.median(pressure_ms), 4),
         "pressure_iterations_median": statistics.median(iters),

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfluide_hip.so")
 
 PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
+E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_NO_DEVICE, E_NONFINITE = -1, -2, -3, -4, -5, -6
 SOLVER_JACOBI, SOLVER_CG, SOLVER_PCG_JACOBI, SOLVER_MGPCG = 0, 1, 2, 3
 SDF_SEARCH_CELLS, SDF_SEARCH_BUCKETS = 0, 1
 WARM_START_ZERO, WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1, 2
@@ -22,6 +23,8 @@ NET_SCALE_RESIDUAL, NET_SCALE_ENERGY = 0, 1
 NET_LOSS_MSE, NET_LOSS_ENERGY = 0, 1
 P2G_TILES, P2G_FACES = 0, 1
 G2P_QUADS, G2P_PAIRS = 0, 1
+MG_FUSED, MG_SPLIT = 0, 1
+SURFACE_REFERENCE, SURFACE_CLOSED = 0, 1
 (GRID_U, GRID_V, GRID_W, GRID_MU, GRID_MV, GRID_MW, GRID_SU, GRID_SV, GRID_SW, GRID_SDF, GRID_FLAGS,
  GRID_PRESSURE, GRID_RHS, GRID_DMU, GRID_DMV, GRID_DMW, GRID_COLLIDER_SDF) = range(17)
 (STAGE_SORT, STAGE_P2G, STAGE_SDF, STAGE_EXTRAPOLATE, STAGE_BC, STAGE_GRAVITY, STAGE_PRESSURE, STAGE_G2P,
@@ -43,10 +46,10 @@ EXPORTED = [
     "fe_net_param_count", "fe_net_get_params", "fe_net_set_params", "fe_net_add_sample", "fe_net_sample_count",
     "fe_net_clear_samples", "fe_net_train", "fe_net_loss_and_grads", "fe_scene_set_warm_start_net",
     "fe_scene_warm_start_scale", "fe_net_set_scale_mode", "fe_net_set_loss", "fe_scene_get_tuning",
-    "fe_scene_set_tuning", "fe_scene_autotune", "fe_mesh_sdf_noalloc",
+    "fe_scene_set_tuning", "fe_scene_autotune", "fe_mesh_sdf_noalloc", "fe_scene_set_surface_mode",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
- KERNEL_MG_SMOOTH) = range(8)
+ KERNEL_MG_SMOOTH, KERNEL_MG_DOWN, KERNEL_MG_UP) = range(10)
 
 
 class SolverDesc(C.Structure):
@@ -56,7 +59,7 @@ class SolverDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("p2g", C.c_int32), ("g2p", C.c_int32)]
+    _fields_ = [("p2g", C.c_int32), ("g2p", C.c_int32), ("mg", C.c_int32)]
 
 
 class TuningTimes(C.Structure):
@@ -82,7 +85,9 @@ class SeedShape(C.Structure):
 
 
 class FluideError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -299,7 +304,7 @@ class Context:
 
     def check(self, rc, what):
         if rc != 0:
-            raise FluideError(f"{what} failed ({rc}): {self.L.fe_last_error(self._h).decode()}")
+            raise FluideError(f"{what} failed ({rc}): {self.L.fe_last_error(self._h).decode()}", rc)
 
     def copy_probe(self, nbytes):
         """One plain device copy of nbytes (the byte-counter calibration kernel); returns ms."""
@@ -445,6 +450,10 @@ class Scene:
         """Upload the collider SDF (nz, ny, nx FP64 at the nodes origin + h*(i,j,k))."""
         self.set(GRID_COLLIDER_SDF, np.ascontiguousarray(phi, dtype=np.float64))
 
+    def set_surface_mode(self, mode):
+        """SURFACE_REFERENCE (the reference's case tables, the default) or SURFACE_CLOSED."""
+        self.ctx.check(self.L.fe_scene_set_surface_mode(self._h, int(mode)), "fe_scene_set_surface_mode")
+
     def set_sdf_search(self, mode):
         self.ctx.check(self.L.fe_scene_set_sdf_search(self._h, mode), "fe_scene_set_sdf_search")
 
@@ -490,9 +499,22 @@ class Scene:
         self.ctx.check(self.L.fe_scene_get_tuning(self._h, C.byref(t)), "fe_scene_get_tuning")
         return t.p2g, t.g2p
 
-    def set_tuning(self, p2g, g2p):
-        t = Tuning(p2g, g2p)
+    def set_tuning(self, p2g, g2p, mg=None):
+        t = Tuning()
+        self.ctx.check(self.L.fe_scene_get_tuning(self._h, C.byref(t)), "fe_scene_get_tuning")
+        t.p2g, t.g2p = p2g, g2p
+        if mg is not None:
+            t.mg = mg
         self.ctx.check(self.L.fe_scene_set_tuning(self._h, C.byref(t)), "fe_scene_set_tuning")
+
+    def mg_variant(self):
+        t = Tuning()
+        self.ctx.check(self.L.fe_scene_get_tuning(self._h, C.byref(t)), "fe_scene_get_tuning")
+        return t.mg
+
+    def set_mg_variant(self, mg):
+        p2g, g2p = self.tuning()
+        self.set_tuning(p2g, g2p, mg)
 
     def autotune(self, reps=5):
         """Times the kernel variants on the current particles and keeps the

@@ -455,7 +455,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(dalloc(&s->chunk_fluid, solver_shape_n(nc).blocks));
     chk(dalloc(&s->sc, 2));  // two state slots (CG alternates them)
     chk(hipHostMalloc((void**)&s->sc_host, sizeof(SolverScalars), hipHostMallocDefault));
-    chk(dalloc(&s->cflmax, 1));
+    chk(dalloc(&s->cflmax, 4));
     const int64_t cap = std::max<int64_t>(d->max_particles, 1);
     s->cap = cap;
     for (int b = 0; b < 2; ++b) {
@@ -492,6 +492,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(launch_fill_u8(s->stream, s->flags, FE_AIR, nc));
     chk(hipMemsetAsync(s->sdf, 0, nc * sizeof(double), s->stream));
     chk(hipMemsetAsync(s->sc, 0, 2 * sizeof(SolverScalars), s->stream));
+    chk(hipMemsetAsync(s->cflmax, 0, 4 * sizeof(unsigned long long), s->stream));
     chk(hipStreamSynchronize(s->stream));
     if (e != hipSuccess) {
         int rc = hip_fail(ctx, e, "scene initialisation");
@@ -568,6 +569,14 @@ int fe_particles_upload(fe_scene* s, const float* x, const float* v, const float
         return set_err(ctx, FE_E_INVALID,
                        "particle count " + std::to_string(count) + " exceeds max_particles " + std::to_string(s->cap));
     if (count > 0 && (!x || !v)) return set_err(ctx, FE_E_INVALID, "x and v are required");
+    for (int64_t i = 0; i < 3 * count; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(v[i]))
+            return set_err(ctx, FE_E_NONFINITE, "particle " + std::to_string(i / 3) + " has a non-finite " +
+                                                    (std::isfinite(x[i]) ? "velocity" : "position"));
+    if (C && s->desc.method == FE_APIC)
+        for (int64_t i = 0; i < 9 * count; ++i)
+            if (!std::isfinite(C[i]))
+                return set_err(ctx, FE_E_NONFINITE, "particle " + std::to_string(i / 9) + " has a non-finite C");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     s->np = count;
     Particles& P = s->P[s->cur];
@@ -589,6 +598,7 @@ int fe_particles_upload(fe_scene* s, const float* x, const float* v, const float
     }
     s->sorted = false;
     s->keys_ready = false;
+    s->sdf_from_cells = 0;  // the bins no longer hold the particles the SDF came from
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
     return FE_OK;
 }
@@ -656,13 +666,17 @@ int fe_cfl(fe_scene* s, double dt, double* out) {
     fe_ctx* ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, launch_cfl(s, dt));
-    unsigned long long bits = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&bits, s->cflmax, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
+    unsigned long long bits[2] = {0, 0};
+    HIPCHK(ctx, hipMemcpyAsync(bits, s->cflmax + 2 * (s->cfl_slot & 1), sizeof(bits), hipMemcpyDeviceToHost,
+                               s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    ++s->cfl_slot;
     double maxv;
-    std::memcpy(&maxv, &bits, sizeof(double));
+    std::memcpy(&maxv, &bits[0], sizeof(double));
     const double minH = std::min(s->g.hd[0], std::min(s->g.hd[1], s->g.hd[2]));
     *out = maxv * dt / minH;
+    if (bits[1])
+        return set_err(ctx, FE_E_NONFINITE, "a face velocity is not finite (NaN or Inf): the state has diverged");
     return FE_OK;
 }
 
@@ -995,7 +1009,8 @@ int fe_scene_get_tuning(const fe_scene* s, fe_tuning* out) {
 
 int fe_scene_set_tuning(fe_scene* s, const fe_tuning* t) {
     if (!s || !t) return FE_E_INVALID;
-    if (t->p2g < FE_P2G_TILES || t->p2g > FE_P2G_FACES || t->g2p < FE_G2P_QUADS || t->g2p > FE_G2P_PAIRS)
+    if (t->p2g < FE_P2G_TILES || t->p2g > FE_P2G_FACES || t->g2p < FE_G2P_QUADS || t->g2p > FE_G2P_PAIRS ||
+        t->mg < FE_MG_FUSED || t->mg > FE_MG_SPLIT)
         return set_err(s->ctx, FE_E_INVALID, "unknown kernel variant");
     s->tune = *t;
     return FE_OK;
@@ -1303,6 +1318,14 @@ int fe_surface_build(fe_scene* s, int64_t* vertex_count, int64_t* triangle_count
     return FE_OK;
 }
 
+int fe_scene_set_surface_mode(fe_scene* s, int32_t mode) {
+    if (!s) return FE_E_INVALID;
+    if (mode != FE_SURFACE_REFERENCE && mode != FE_SURFACE_CLOSED)
+        return set_err(s->ctx, FE_E_INVALID, "unknown surface mode " + std::to_string(mode));
+    s->surface_mode = mode;
+    return FE_OK;
+}
+
 int fe_surface_download(fe_scene* s, double* vertices, double* normals, int32_t* triangles) {
     if (!s) return FE_E_INVALID;
     fe_ctx* ctx = s->ctx;
@@ -1329,7 +1352,7 @@ int fe_last_diffusion_stats(const fe_scene* s, fe_solve_stats stats[3]) {
 int fe_kernel_timer(fe_scene* s, int32_t kernel_id) {
     if (!s) return FE_E_INVALID;
     fe_ctx* ctx = s->ctx;
-    if (kernel_id < FE_KERNEL_NONE || kernel_id > FE_KERNEL_MG_SMOOTH)
+    if (kernel_id < FE_KERNEL_NONE || kernel_id > FE_KERNEL_MG_UP)
         return set_err(ctx, FE_E_INVALID, "unknown kernel id " + std::to_string(kernel_id));
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
@@ -1395,6 +1418,7 @@ int fe_grid_upload(fe_scene* s, int32_t field, const void* src, size_t bytes) {
     }
     HIPCHK(ctx, hipMemcpyAsync(field_ptr(s, field), src, need, hipMemcpyHostToDevice, s->stream));
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    if (field == FE_GRID_SDF) s->sdf_from_cells = 0;  // the caller's SDF is the surface's
     return FE_OK;
 }
 

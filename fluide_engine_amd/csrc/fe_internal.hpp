@@ -372,7 +372,8 @@ namespace fe {
 // and the caps on the host.
 struct McState {
     int32_t *flag = nullptr, *vid = nullptr, *cnt = nullptr, *toff = nullptr, *sums = nullptr;
-    void* tables = nullptr;
+    void* tables = nullptr;      // FE_SURFACE_CLOSED cube tables
+    void* tables_ref = nullptr;  // FE_SURFACE_REFERENCE cube tables
     double* planes = nullptr;
     int64_t plane_stride = 0;
     double *pos = nullptr, *nrm = nullptr;
@@ -414,7 +415,7 @@ struct fe_scene {
     hipStream_t stream = nullptr;
     int depth = 3;            // ceil(maxCfl): extrapolation / BC depth
     int move_substeps = 3;    // (unsigned)max(maxCfl, 1)
-    fe_tuning tune{FE_P2G_TILES, FE_G2P_QUADS};  // kernel variants (bit-identical results)
+    fe_tuning tune{FE_P2G_TILES, FE_G2P_QUADS, FE_MG_SPLIT};  // kernel variants
 
     // grid (HBM)
     float* u[3] = {nullptr, nullptr, nullptr};       // face velocities
@@ -438,7 +439,8 @@ struct fe_scene {
     int64_t npartials = 0;
     fe::SolverScalars* sc = nullptr;
     fe::SolverScalars* sc_host = nullptr;  // pinned mirror
-    unsigned long long* cflmax = nullptr;
+    unsigned long long* cflmax = nullptr;  // two slots of (max |u| as FP64 bits, non-finite flag)
+    int cfl_slot = 0;                      // the slot the next launch_cfl writes
 
     // particles (HBM, SoA, ping-pong for the reorder)
     int64_t np = 0, cap = 0;
@@ -460,8 +462,13 @@ struct fe_scene {
     double stage_ms[fe::kMaxStages] = {};
     fe_solve_stats last_stats{};
     int last_cg_iters[3] = {0, 0, 0};  // iterations of the last CG solve per preconditioner (none, diagonal, multigrid)
-    std::vector<fe::MGLevel> mg;
-    int mg_tail = 0;  // first level handled by the one-workgroup tail kernel  // multigrid hierarchy, built on first use
+    std::vector<fe::MGLevel> mg;  // multigrid hierarchy, built on first use
+    int mg_tail = 0;              // first level handled by the one-workgroup tail kernel
+    // fused V-cycle (FE_MG_FUSED): level 0's per-tile FLUID flags and the r.z
+    // partials of its ascent kernel (one per launched workgroup)
+    uint8_t* mg_tfl = nullptr;
+    double* mg_pz = nullptr;
+    int64_t mg_npz = 0;
 
     // BackwardEulerDiffusionSolver: its linear solver (default Jacobi(1000, 5, 1e-14)),
     // buffers allocated on first use (FP64 x ping-pong, row codes, face markers
@@ -480,6 +487,8 @@ struct fe_scene {
     double* csdf = nullptr;
     bool has_collider = false;
     bool sdf_buckets = false;  // FE_SDF_SEARCH_BUCKETS
+    int sdf_from_cells = 0;    // the fluid SDF in memory came from the cell scan of the current bins
+    int surface_mode = FE_SURFACE_REFERENCE;
     float* bc_tmp[3] = {nullptr, nullptr, nullptr};
 
     // scene sources: emitters, substeps run (the collider SDF is built from

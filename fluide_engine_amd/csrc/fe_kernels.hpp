@@ -32,14 +32,16 @@ hipError_t launch_iota(fe_scene* s, int32_t* a, int64_t n);
 hipError_t launch_cells_by_pid(fe_scene* s, int32_t* out);
 
 // k_grid.hip
-hipError_t launch_sdf(fe_scene* s);
+// the fluid SDF of the current bins; buckets forces the 8-bucket search (else the
+// scene's mode: buckets with a collider or FE_SDF_SEARCH_BUCKETS, the cell scan otherwise)
+hipError_t launch_sdf(fe_scene* s, bool buckets = false);
 hipError_t launch_markers(fe_scene* s);
 hipError_t launch_extrapolate(fe_scene* s);               // ExtrapolateVelocityToAir, depth iterations
 hipError_t launch_extrapolate_from(fe_scene* s, const uint8_t* const init[3]);  // from other validity
 hipError_t launch_walls(fe_scene* s);                      // wall layers of ConstrainVelocity
 hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after);
 hipError_t launch_apply_pressure(fe_scene* s, bool walls_after);
-hipError_t launch_cfl(fe_scene* s, double dt);             // result in s->cflmax (FP64 bits)
+hipError_t launch_cfl(fe_scene* s, double dt);  // result in s->cflmax slot s->cfl_slot: FP64 bits, non-finite flag
 hipError_t launch_fill_u8(hipStream_t st, uint8_t* a, uint8_t v, int64_t n);
 hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n);
 

@@ -21,6 +21,8 @@
 #include <cstdint>
 #include <cstring>
 
+#include "fe_mc_table.hpp"
+
 namespace fe {
 
 constexpr int kMcMaxTri = 5;  // triangles per case (a loop of n edges gives n - 2)
@@ -224,6 +226,33 @@ inline McTables make_mc_tables() {
             }
         }
         T.ntri[cs] = (uint8_t)nt;
+    }
+    return T;
+}
+
+// Corner bits (x | y << 1 | z << 2) of the reference's cube corners 0-7
+// (fe_mc_table.hpp).
+constexpr int kMcRefCorner[8] = {0, 1, 5, 4, 2, 3, 7, 6};
+
+inline int mc_hex(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
+
+// The reference's cube table (kMcRefCube) in this file's case and edge
+// encoding, triangles in the table's order and winding.
+inline McTables make_mc_tables_reference() {
+    McTables T;
+    std::memset(T.ntri, 0, sizeof(T.ntri));
+    std::memset(T.tri, -1, sizeof(T.tri));
+    for (int cs = 0; cs < 256; ++cs) {
+        int rc = 0;
+        for (int v = 0; v < 8; ++v)
+            if ((cs >> kMcRefCorner[v]) & 1) rc |= 1 << v;
+        const char* t = kMcRefCube[rc];
+        const int n = (int)std::strlen(t);
+        for (int q = 0; q < n; ++q) {
+            const int e = mc_hex(t[q]);
+            T.tri[cs][q] = (int8_t)mcr::edge_of(kMcRefCorner[kMcRefEdge[e][0]], kMcRefCorner[kMcRefEdge[e][1]]);
+        }
+        T.ntri[cs] = (uint8_t)(n / 3);
     }
     return T;
 }

@@ -229,15 +229,16 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nparts,
     block_reduce<NV>(tot, sm);
 }
 
-// Two sums in one pass: tot[0] over a[sa * b + oa], tot[1] over c[sc * b + oc]
-// (b < nparts), each in sum_partials' order, so the values are the same.
-__device__ inline void sum_partials_two(const double* __restrict__ a, int sa, int oa, const double* __restrict__ c,
-                                        int sc, int oc, int nparts, double (&tot)[2]) {
+// Two sums in one pass: tot[0] over a[sa * b + oa] (b < na), tot[1] over
+// c[sc * b + oc] (b < nc), each in sum_partials' order, so the values are the same.
+__device__ inline void sum_partials_two(const double* __restrict__ a, int sa, int oa, int na,
+                                        const double* __restrict__ c, int sc, int oc, int nc, double (&tot)[2]) {
     __shared__ double sm[2][kBlock / 64];
     tot[0] = tot[1] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) {
-        tot[0] += a[(int64_t)b * sa + oa];
-        tot[1] += c[(int64_t)b * sc + oc];
+    const int n = na > nc ? na : nc;
+    for (int b = threadIdx.x; b < n; b += kBlock) {
+        if (b < na) tot[0] += a[(int64_t)b * sa + oa];
+        if (b < nc) tot[1] += c[(int64_t)b * sc + oc];
     }
     block_reduce<2>(tot, sm);
 }

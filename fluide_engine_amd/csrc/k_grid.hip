@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict
     sdf[c] = minD - radius;
 }
 
-hipError_t launch_sdf(fe_scene* s) {
+hipError_t launch_sdf(fe_scene* s, bool buckets) {
     const Geom& g = s->g;
     const double maxH = fmax(g.hd[0], fmax(g.hd[1], g.hd[2]));
     const double radius = 1.2 * maxH / std::sqrt(2.0);  // src/pic_simulator.cpp:459
@@ -162,7 +162,8 @@ hipError_t launch_sdf(fe_scene* s) {
     const float slack = (float)(4.0 * E);
     const int64_t nc = cell_count(g);
     const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
-    if (s->has_collider || s->sdf_buckets)
+    buckets = buckets || s->has_collider || s->sdf_buckets;
+    if (buckets)
         launch_timed(s, FE_KERNEL_SDF, 0, k_sdf<true>, dim3((unsigned)padded), dim3(kBlock), g,
                      (const float4*)s->pos4, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius,
                      band, qr2, slack, padded);
@@ -170,6 +171,7 @@ hipError_t launch_sdf(fe_scene* s) {
         launch_timed(s, FE_KERNEL_SDF, 0, k_sdf<false>, dim3((unsigned)padded), dim3(kBlock), g,
                      (const float4*)s->pos4, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius,
                      band, qr2, slack, padded);
+    s->sdf_from_cells = buckets ? 0 : 1;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_sdf_into_collider(s);
@@ -495,9 +497,19 @@ hipError_t launch_apply_pressure(fe_scene* s, bool walls_after) {
 // PICSimulator::Cfl (src/pic_simulator.cpp:80-97): max over cells of
 // |ValueAtCellCenter| + dt*(0,-9.8,0), in FP64; max is order independent, so
 // an atomic max on the (non-negative) FP64 bit pattern is deterministic.
-__global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, unsigned long long* __restrict__ out) {
+// out[0]: the maximum as FP64 bits (atomicMax on non-negative doubles), out[1]:
+// set when a face value is not finite (NaN is otherwise ignored by fmax, as by
+// the reference's std::max).  The launch also clears the other slot, which the
+// next launch uses, so no memset is needed between launches.
+__global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, unsigned long long* __restrict__ out,
+                                                unsigned long long* __restrict__ next) {
     __shared__ double smem[kBlock / 64];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        next[0] = 0ull;
+        next[1] = 0ull;
+    }
     double m = 0.0;
+    int bad = 0;
     const int64_t nc = cell_count(g);
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     // four cells per step, all their face loads issued before any is used
@@ -523,6 +535,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, un
 #pragma unroll
         for (int u = 0; u < kG; ++u) {
             if (c0 + u * stride >= nc) continue;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) bad |= !isfinite(f[u][q]);
             const double cx = 0.5 * ((double)f[u][0] + (double)f[u][1]);
             const double cy = 0.5 * ((double)f[u][2] + (double)f[u][3]);
             const double cz = 0.5 * ((double)f[u][4] + (double)f[u][5]);
@@ -533,22 +547,24 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, un
     }
     for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
     if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = m;
-    __syncthreads();
+    bad = __syncthreads_or(bad);
     if (threadIdx.x == 0) {
         double b = smem[0];
         for (int w = 1; w < kBlock / 64; ++w) b = fmax(b, smem[w]);
         atomicMax(out, (unsigned long long)__double_as_longlong(b));
+        if (bad) atomicOr(out + 1, 1ull);
     }
 }
 
 hipError_t launch_cfl(fe_scene* s, double dt) {
     const int64_t nc = cell_count(s->g);
-    hipError_t e = hipMemsetAsync(s->cflmax, 0, sizeof(unsigned long long), s->stream);
-    if (e != hipSuccess) return e;
     FaceSet a{{s->u[0], s->u[1], s->u[2]}};
+    // slots of two words alternate between launches (the scene starts with both clear)
+    unsigned long long* out = s->cflmax + 2 * (s->cfl_slot & 1);
+    unsigned long long* next = s->cflmax + 2 * ((s->cfl_slot + 1) & 1);
     // few blocks with a grid-stride loop: one atomic per block on a single address
     const unsigned blocks = (unsigned)std::min<int64_t>((nc + kBlock - 1) / kBlock, 512);
-    hipLaunchKernelGGL(k_cfl, dim3(blocks), dim3(kBlock), 0, s->stream, s->g, a, dt, s->cflmax);
+    hipLaunchKernelGGL(k_cfl, dim3(blocks), dim3(kBlock), 0, s->stream, s->g, a, dt, out, next);
     return hipGetLastError();
 }
 

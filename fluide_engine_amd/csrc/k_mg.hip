@@ -148,17 +148,37 @@ __device__ inline float restrict_sum(int I, int J, int K, int fx, int fy, int fz
     return acc * 0.125f;
 }
 
-// Trilinear prolongation of the coarse correction at fine cell (i, j, k).
+// Trilinear prolongation of the coarse correction at fine cell (i, j, k);
+// V(a, b, c) is the coarse value at coarse index (a, b, c).
 template <typename F>
-__device__ inline float prolong_at(int i, int j, int k, int cx, int cy, int cz, F X) {
+__device__ inline float prolong_at3(int i, int j, int k, int cx, int cy, int cz, F V) {
     int ia, ib, ja, jb, ka, kb;
     float wia, wib, wja, wjb, wka, wkb;
     prolong_axis(i, cx, ia, wia, ib, wib);
     prolong_axis(j, cy, ja, wja, jb, wjb);
     prolong_axis(k, cz, ka, wka, kb, wkb);
-    auto V = [&](int a, int b, int c) { return X((int64_t)a + (int64_t)cx * ((int64_t)b + (int64_t)cy * c)); };
     return wka * (wja * (wia * V(ia, ja, ka) + wib * V(ib, ja, ka)) + wjb * (wia * V(ia, jb, ka) + wib * V(ib, jb, ka))) +
            wkb * (wja * (wia * V(ia, ja, kb) + wib * V(ib, ja, kb)) + wjb * (wia * V(ia, jb, kb) + wib * V(ib, jb, kb)));
+}
+
+// The same with X(n) the coarse value at linear index n.
+template <typename F>
+__device__ inline float prolong_at(int i, int j, int k, int cx, int cy, int cz, F X) {
+    return prolong_at3(i, j, k, cx, cy, cz,
+                       [&](int a, int b, int c) { return X((int64_t)a + (int64_t)cx * ((int64_t)b + (int64_t)cy * c)); });
+}
+
+// One damped-Jacobi update of a cell, x + w (b - A x): the smoother of every
+// level and kernel variant (the same expression, so the same rounding).
+__device__ inline float mg_sweep_cell(uint16_t cd, float xc, float bc, const float (&nb)[6], const float ih2[3]) {
+    const float dgn = code_center(cd, ih2);
+    const float w = mg_weight(cd, ih2);
+    return xc + w * (bc - code_apply_nb(cd, dgn, ih2, xc, nb));
+}
+
+// The residual b - A x of a FLUID cell, 0 elsewhere.
+__device__ inline float mg_residual_cell(uint16_t cd, float xc, float bc, const float (&nb)[6], const float ih2[3]) {
+    return (cd & kFluidBit) ? bc - code_apply_nb(cd, code_center(cd, ih2), ih2, xc, nb) : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -230,9 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
         sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, xin, b, stv, X); },
         [&](int64_t c, const StencilIn& t) {
             if (done) return;  // uniform: the solve has stopped
-            const float dgn = code_center(t.cd, L.ih2);
-            const float w = mg_weight(t.cd, L.ih2);
-            const float xo = t.xc + w * (t.bc - code_apply_nb(t.cd, dgn, L.ih2, t.xc, t.nb));
+            const float xo = mg_sweep_cell(t.cd, t.xc, t.bc, t.nb, L.ih2);
             xout[c] = xo;
             v[0] += (double)t.bc * (double)xo;
         });
@@ -253,9 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_residual(LevelArgs L, const uint1
         sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, x, b, stv, X); },
         [&](int64_t c, const StencilIn& t) {
             if (done) return;
-            float rc = 0.0f;
-            if (t.cd & kFluidBit) rc = t.bc - code_apply_nb(t.cd, code_center(t.cd, L.ih2), L.ih2, t.xc, t.nb);
-            r[c] = rc;
+            r[c] = mg_residual_cell(t.cd, t.xc, t.bc, t.nb, L.ih2);
         });
 }
 
@@ -347,6 +363,365 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
     const float add = prolong_at(i, j, k, cx, cy, cz, [&](int64_t n) { return xc[n]; });
     if (done || !(cd & kFluidBit)) return;
     xf[f] = x0 + add;
+}
+
+// ---------------------------------------------------------------------------
+// Large levels, fused (FE_MG_FUSED): one descent and one ascent kernel per
+// level.  A workgroup owns a coarse tile of kRtx x kRty x kRtz cells and the
+// fine tile of twice that size below it.  It stages what the tile's sweeps
+// read, with halos, in LDS and recomputes the halo cells of each sweep itself
+// instead of waiting for a neighbour tile, so a level takes two launches
+// instead of six.  Every value is the split kernels' (the same helpers in the
+// same order; only the source of a neighbour value differs, and a neighbour
+// outside the grid, which enters with weight 0, reads 0 here).
+//   descent:  x1 = w b (first sweep from zero), x2 = x1 + w (b - A x1) (kept
+//             for the ascent), r = b - A x2, coarse b = R r;
+//   ascent:   y = x2 + P xc, x' = y + w (b - A y), out = x' + w (b - A x'),
+//             with level 0's out = z and the partials of r.z.
+// Window of halo H around the fine tile: coordinates 0 .. kFt + 2H - 1 stand
+// for fine indices f0 - H ...
+// ---------------------------------------------------------------------------
+constexpr int kFtx = 2 * kRtx, kFty = 2 * kRty, kFtz = 2 * kRtz;
+template <int H>
+struct FWin {
+    static constexpr int x = kFtx + 2 * H, y = kFty + 2 * H, z = kFtz + 2 * H;
+    static constexpr int sy = x, sz = x * y, n = x * y * z;
+};
+// coarse window of the ascent: coarse indices I0 - 2 .. I0 + kRt + 1 per axis
+struct CWin {
+    static constexpr int x = kRtx + 4, y = kRty + 4, z = kRtz + 4;
+    static constexpr int sy = x, sz = x * y, n = x * y * z;
+};
+
+struct FusedArgs {
+    int n[3];        // the fine level
+    int c[3];        // the coarse level
+    float ih2[3];    // the fine level's 1/h^2
+    int tiles[3];    // coarse tiles per axis
+    int64_t padded;  // launched workgroups (XCD remap), >= the tile count
+};
+
+__device__ inline int64_t fused_tile(const FusedArgs& a) { return xcd_remap(blockIdx.x, a.padded); }
+
+__device__ inline bool fine_in(const FusedArgs& a, int i, int j, int k) {
+    return i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
+}
+
+// tfl (level 0 only): per tile, whether its fine tile holds a FLUID cell.  A
+// tile without one has every output 0 (its right-hand side and its coarse
+// cells' are 0), and those stay as the solve's start left them.
+constexpr int kFusedThreads = 1024;  // 16 waves per tile: the phases between barriers are LDS-latency bound
+
+// sum_partials for a kFusedThreads workgroup: every workgroup adds the NV-wide
+// partials in the same fixed order (threads strided over the partials, lanes
+// by the shuffle tree, waves in order), so all derive identical totals.
+template <int NV>
+__device__ inline void sum_partials_fused(const double* __restrict__ part, int nparts, double (&tot)[NV]) {
+    __shared__ double sm[NV][kFusedThreads / 64];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += kFusedThreads) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) tot[q] += part[(int64_t)b * NV + q];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double x = tot[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if ((threadIdx.x & 63) == 0) sm[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double t = 0.0;
+        for (int w = 0; w < kFusedThreads / 64; ++w) t += sm[q][w];
+        tot[q] = t;
+    }
+    __syncthreads();
+}
+
+template <bool L0>
+__global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const float* __restrict__ b,
+                                                    const uint16_t* __restrict__ code, float* __restrict__ x2out,
+                                                    const uint16_t* __restrict__ ccode, float* __restrict__ cb,
+                                                    const SolverScalars* st, const double* __restrict__ pconf,
+                                                    int nconf, const uint8_t* __restrict__ tfl) {
+    using W3 = FWin<3>;
+    using W2 = FWin<2>;
+    using W1 = FWin<1>;
+    __shared__ float sX1[W3::n];  // x1; later, over W1, the residual
+    __shared__ float sB[W2::n];
+    __shared__ uint16_t sC[W2::n];
+    __shared__ float sX2[W2::n];
+    const int done = st->done;  // read with the data, tested before the first write
+    if (L0) {
+        // the V-cycle's first kernel: a relative stop pending confirmation is
+        // tested on the residual the CG update just recomputed (as k_mg_smooth)
+        if (pconf && done) return;
+        if (pconf && st->confirm) {
+            double t[2];
+            sum_partials_fused<2>(pconf, nconf, t);
+            if (stop_rule(st, st->bnorm2, t[0], t[1])) {
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    SolverScalars* w = const_cast<SolverScalars*>(st);
+                    w->rr = t[1];
+                    w->it = st->it + 1;
+                    w->converged = 1;
+                    w->done = 1;
+                }
+                return;
+            }
+        }
+    }
+    const int64_t t = fused_tile(a);
+    if (t >= (int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2]) return;
+    if (L0 && tfl && !tfl[t]) return;
+    const int TX = (int)(t % a.tiles[0]), TY = (int)((t / a.tiles[0]) % a.tiles[1]),
+              TZ = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
+    const int I0 = TX * kRtx, J0 = TY * kRty, K0 = TZ * kRtz;
+    const int fx0 = 2 * I0, fy0 = 2 * J0, fz0 = 2 * K0;
+    // this thread's coarse cell (the restriction; the first kRtx*kRty*kRtz threads)
+    const int tx = threadIdx.x % kRtx, ty = (threadIdx.x / kRtx) % kRty, tz = threadIdx.x / (kRtx * kRty);
+    const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
+    const bool cin = threadIdx.x < kRtx * kRty * kRtz && I < a.c[0] && J < a.c[1] && K < a.c[2];
+    const int64_t ci = (int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K);
+    // b and the row codes on the tile + 3: every load issued before any LDS store
+    constexpr int kPer = (W3::n + kFusedThreads - 1) / kFusedThreads;
+    float vb[kPer];
+    uint16_t vc[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + u * kFusedThreads;
+        vb[u] = 0.0f;
+        vc[u] = 0;
+        if (e < W3::n) {
+            const int wx = e % W3::x, wy = (e / W3::x) % W3::y, wz = e / W3::sz;
+            const int i = fx0 - 3 + wx, j = fy0 - 3 + wy, k = fz0 - 3 + wz;
+            if (fine_in(a, i, j, k)) {
+                const int64_t f = (int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k);
+                vb[u] = b[f];
+                vc[u] = code[f];
+            }
+        }
+    }
+    const uint16_t ccd = cin ? ccode[ci] : (uint16_t)0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = threadIdx.x + u * kFusedThreads;
+        if (e < W3::n) {
+            sX1[e] = mg_weight(vc[u], a.ih2) * vb[u];  // the first sweep from zero
+            const int wx = e % W3::x, wy = (e / W3::x) % W3::y, wz = e / W3::sz;
+            if (wx >= 1 && wx < W3::x - 1 && wy >= 1 && wy < W3::y - 1 && wz >= 1 && wz < W3::z - 1) {
+                const int e2 = (wx - 1) + W2::sy * (wy - 1) + W2::sz * (wz - 1);
+                sB[e2] = vb[u];
+                sC[e2] = vc[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (done) return;  // uniform: the solve has stopped
+    // x2 on the tile + 2; the tile's own cells also go to global memory
+    for (int e = threadIdx.x; e < W2::n; e += kFusedThreads) {
+        const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
+        const int e3 = (wx + 1) + W3::sy * (wy + 1) + W3::sz * (wz + 1);
+        const float nb[6] = {sX1[e3 - 1],       sX1[e3 + 1],       sX1[e3 - W3::sy],
+                             sX1[e3 + W3::sy], sX1[e3 - W3::sz], sX1[e3 + W3::sz]};
+        const float x2 = mg_sweep_cell(sC[e], sX1[e3], sB[e], nb, a.ih2);
+        sX2[e] = x2;
+        if (wx >= 2 && wx < 2 + kFtx && wy >= 2 && wy < 2 + kFty && wz >= 2 && wz < 2 + kFtz) {
+            const int i = fx0 + wx - 2, j = fy0 + wy - 2, k = fz0 + wz - 2;
+            if (fine_in(a, i, j, k)) x2out[(int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k)] = x2;
+        }
+    }
+    __syncthreads();
+    // r on the tile + 1 (into sX1, whose x1 is no longer read)
+    float* sR = sX1;
+    for (int e = threadIdx.x; e < W1::n; e += kFusedThreads) {
+        const int wx = e % W1::x, wy = (e / W1::x) % W1::y, wz = e / W1::sz;
+        const int e2 = (wx + 1) + W2::sy * (wy + 1) + W2::sz * (wz + 1);
+        const float nb[6] = {sX2[e2 - 1],       sX2[e2 + 1],       sX2[e2 - W2::sy],
+                             sX2[e2 + W2::sy], sX2[e2 - W2::sz], sX2[e2 + W2::sz]};
+        sR[e] = mg_residual_cell(sC[e2], sX2[e2], sB[e2], nb, a.ih2);
+    }
+    __syncthreads();
+    // coarse b = R r: fine index 2I-1+t sits at window coordinate 2*(I - I0) + t
+    if (!cin) return;
+    float v = 0.0f;
+    if (ccd & kFluidBit) {
+        int ix[4], iy[4], iz[4];
+        float wx[4], wy[4], wz[4];
+        restrict_taps(I, a.n[0], a.c[0], ix, wx);
+        restrict_taps(J, a.n[1], a.c[1], iy, wy);
+        restrict_taps(K, a.n[2], a.c[2], iz, wz);
+        float acc = 0.0f;
+#pragma unroll
+        for (int tk = 0; tk < 4; ++tk) {
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj) {
+                const float wjk = wy[tj] * wz[tk];
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti)
+                    acc += (wx[ti] * wjk) * sR[(2 * tx + ti) + W1::sy * (2 * ty + tj) + W1::sz * (2 * tz + tk)];
+            }
+        }
+        v = acc * 0.125f;
+    }
+    cb[ci] = v;
+}
+
+// out = the level's corrected and post-smoothed solution.  With prz (level 0)
+// every launched workgroup leaves one partial of b . out (r.z).
+template <bool L0>
+__global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const float* __restrict__ x2,
+                                                  const float* __restrict__ xc, const float* __restrict__ b,
+                                                  const uint16_t* __restrict__ code, float* __restrict__ out,
+                                                  double* __restrict__ prz, const SolverScalars* __restrict__ st,
+                                                  const uint8_t* __restrict__ tfl) {
+    using W2 = FWin<2>;
+    using W1 = FWin<1>;
+    __shared__ float sY[W2::n];
+    __shared__ uint16_t sC[W2::n];
+    __shared__ float sB[W1::n];
+    __shared__ float sXp[W1::n];
+    __shared__ float sXc[CWin::n];
+    const int done = st->done;  // read with the data, tested before the first write
+    const int64_t t = fused_tile(a);
+    const bool real = t < (int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2] && !(L0 && tfl && !tfl[t]);
+    int fx0 = 0, fy0 = 0, fz0 = 0, cx0 = 0, cy0 = 0, cz0 = 0;
+    if (real) {
+        const int TX = (int)(t % a.tiles[0]), TY = (int)((t / a.tiles[0]) % a.tiles[1]),
+                  TZ = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
+        fx0 = 2 * TX * kRtx;
+        fy0 = 2 * TY * kRty;
+        fz0 = 2 * TZ * kRtz;
+        cx0 = TX * kRtx - 2;
+        cy0 = TY * kRty - 2;
+        cz0 = TZ * kRtz - 2;
+        // x2 and the row codes on the tile + 2, b on the tile + 1, the coarse
+        // correction on its window: every load issued before any LDS store
+        constexpr int kP2 = (W2::n + kFusedThreads - 1) / kFusedThreads;
+        constexpr int kPc = (CWin::n + kFusedThreads - 1) / kFusedThreads;
+        float vx[kP2], vb[kP2];
+        uint16_t vc[kP2];
+        float vxc[kPc];
+#pragma unroll
+        for (int u = 0; u < kP2; ++u) {
+            const int e = threadIdx.x + u * kFusedThreads;
+            vx[u] = vb[u] = 0.0f;
+            vc[u] = 0;
+            if (e < W2::n) {
+                const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
+                const int i = fx0 - 2 + wx, j = fy0 - 2 + wy, k = fz0 - 2 + wz;
+                if (fine_in(a, i, j, k)) {
+                    const int64_t f = (int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k);
+                    vx[u] = x2[f];
+                    vc[u] = code[f];
+                    const bool w1 = wx >= 1 && wx < W2::x - 1 && wy >= 1 && wy < W2::y - 1 && wz >= 1 && wz < W2::z - 1;
+                    if (w1) vb[u] = b[f];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPc; ++u) {
+            const int e = threadIdx.x + u * kFusedThreads;
+            vxc[u] = 0.0f;
+            if (e < CWin::n) {
+                const int wx = e % CWin::x, wy = (e / CWin::x) % CWin::y, wz = e / CWin::sz;
+                const int I = min(max(cx0 + wx, 0), a.c[0] - 1), J = min(max(cy0 + wy, 0), a.c[1] - 1),
+                          K = min(max(cz0 + wz, 0), a.c[2] - 1);
+                vxc[u] = xc[(int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kP2; ++u) {
+            const int e = threadIdx.x + u * kFusedThreads;
+            if (e < W2::n) {
+                sY[e] = vx[u];
+                sC[e] = vc[u];
+                const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
+                if (wx >= 1 && wx < W2::x - 1 && wy >= 1 && wy < W2::y - 1 && wz >= 1 && wz < W2::z - 1)
+                    sB[(wx - 1) + W1::sy * (wy - 1) + W1::sz * (wz - 1)] = vb[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPc; ++u) {
+            const int e = threadIdx.x + u * kFusedThreads;
+            if (e < CWin::n) sXc[e] = vxc[u];
+        }
+    }
+    __syncthreads();
+    if (done) return;  // uniform: the solve has stopped
+    double v[1] = {0.0};
+    if (real) {
+        // y = x2 + P xc on FLUID cells (each from its own x2: in place)
+        for (int e = threadIdx.x; e < W2::n; e += kFusedThreads) {
+            if (!(sC[e] & kFluidBit)) continue;
+            const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
+            const float add = prolong_at3(fx0 - 2 + wx, fy0 - 2 + wy, fz0 - 2 + wz, a.c[0], a.c[1], a.c[2],
+                                          [&](int I, int J, int K) {
+                                              return sXc[(I - cx0) + CWin::sy * (J - cy0) + CWin::sz * (K - cz0)];
+                                          });
+            sY[e] = sY[e] + add;
+        }
+    }
+    __syncthreads();
+    if (real) {
+        // x' on the tile + 1
+        for (int e = threadIdx.x; e < W1::n; e += kFusedThreads) {
+            const int wx = e % W1::x, wy = (e / W1::x) % W1::y, wz = e / W1::sz;
+            const int e2 = (wx + 1) + W2::sy * (wy + 1) + W2::sz * (wz + 1);
+            const float nb[6] = {sY[e2 - 1],       sY[e2 + 1],       sY[e2 - W2::sy],
+                                 sY[e2 + W2::sy], sY[e2 - W2::sz], sY[e2 + W2::sz]};
+            sXp[e] = mg_sweep_cell(sC[e2], sY[e2], sB[e], nb, a.ih2);
+        }
+    }
+    __syncthreads();
+    if (real) {
+        // out on the tile
+        constexpr int kTile = kFtx * kFty * kFtz;
+        for (int e = threadIdx.x; e < kTile; e += kFusedThreads) {
+            const int wx = e % kFtx, wy = (e / kFtx) % kFty, wz = e / (kFtx * kFty);
+            const int i = fx0 + wx, j = fy0 + wy, k = fz0 + wz;
+            if (!fine_in(a, i, j, k)) continue;
+            const int e1 = (wx + 1) + W1::sy * (wy + 1) + W1::sz * (wz + 1);
+            const int e2 = (wx + 2) + W2::sy * (wy + 2) + W2::sz * (wz + 2);
+            const float nb[6] = {sXp[e1 - 1],       sXp[e1 + 1],       sXp[e1 - W1::sy],
+                                 sXp[e1 + W1::sy], sXp[e1 - W1::sz], sXp[e1 + W1::sz]};
+            const float bc = sB[e1];
+            const float o = mg_sweep_cell(sC[e2], sXp[e1], bc, nb, a.ih2);
+            out[(int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k)] = o;
+            if (L0) v[0] += (double)bc * (double)o;
+        }
+    }
+    if (L0 && prz) {  // one partial per workgroup: waves in order, lanes by the shuffle tree
+        __shared__ double sm[kFusedThreads / 64];
+        double x = v[0];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kFusedThreads / 64; ++w) t += sm[w];
+            prz[blockIdx.x] = t;
+        }
+    }
+}
+
+// Level 0's tile flags: whether the fine tile of each coarse tile holds a FLUID cell.
+__global__ __launch_bounds__(kBlock) void k_mg_tile_fluid(const uint8_t* __restrict__ flags, int nx, int ny, int nz,
+                                                          int tx, int ty, int tz, uint8_t* __restrict__ tfl) {
+    const int64_t t = blockIdx.x;
+    const int TX = (int)(t % tx), TY = (int)((t / tx) % ty), TZ = (int)(t / ((int64_t)tx * ty));
+    int any = 0;
+    for (int e = threadIdx.x; e < kFtx * kFty * kFtz; e += kBlock) {
+        const int i = TX * kFtx + e % kFtx, j = TY * kFty + (e / kFtx) % kFty, k = TZ * kFtz + e / (kFtx * kFty);
+        if (i < nx && j < ny && k < nz && flags[(int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k)] == FE_FLUID)
+            any = 1;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) tfl[t] = (uint8_t)any;
 }
 
 // ---------------------------------------------------------------------------
@@ -563,6 +938,20 @@ static LevelArgs level_args(const MGLevel& l) {
     return a;
 }
 
+static FusedArgs fused_args(const MGLevel& l, const MGLevel& c) {
+    FusedArgs a;
+    for (int q = 0; q < 3; ++q) {
+        a.n[q] = l.n[q];
+        a.c[q] = c.n[q];
+        a.ih2[q] = l.ih2[q];
+    }
+    a.tiles[0] = (c.n[0] + kRtx - 1) / kRtx;
+    a.tiles[1] = (c.n[1] + kRty - 1) / kRty;
+    a.tiles[2] = (c.n[2] + kRtz - 1) / kRtz;
+    a.padded = pad_blocks((int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2]);
+    return a;
+}
+
 void mg_free(fe_scene* s);
 
 // First level of the tail: the first level below the finest with at most
@@ -614,6 +1003,11 @@ static hipError_t mg_alloc(fe_scene* s) {
         if ((e = hipMalloc((void**)&l.x2, n * sizeof(float))) != hipSuccess) break;
         if ((e = hipMalloc((void**)&l.r, n * sizeof(float))) != hipSuccess) break;
     }
+    // the fused V-cycle's level-0 tile flags and r.z partials
+    const FusedArgs fa = fused_args(lv[0], lv[1]);
+    s->mg_npz = fa.padded;
+    if (e == hipSuccess) e = hipMalloc((void**)&s->mg_tfl, (size_t)fa.padded);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->mg_pz, sizeof(double) * (size_t)fa.padded);
     s->mg = lv;
     s->mg_tail = tail_start(lv);
     if (e != hipSuccess) mg_free(s);  // leaves the hierarchy empty: the next solve retries
@@ -632,6 +1026,11 @@ void mg_free(fe_scene* s) {
         (void)hipFree(l.r);
     }
     s->mg.clear();
+    (void)hipFree(s->mg_tfl);
+    (void)hipFree(s->mg_pz);
+    s->mg_tfl = nullptr;
+    s->mg_pz = nullptr;
+    s->mg_npz = 0;
 }
 
 // Per solve: coarse markers and row codes from the current level-0 markers
@@ -649,6 +1048,12 @@ hipError_t mg_setup(fe_scene* s) {
         hipLaunchKernelGGL(k_mg_codes, grid1(c.cells), dim3(kBlock), 0, s->stream, (const uint8_t*)c.flags, c.n[0],
                            c.n[1], c.n[2], c.code);
     }
+    if (s->tune.mg == FE_MG_FUSED) {
+        const FusedArgs fa = fused_args(s->mg[0], s->mg[1]);
+        const int64_t nt = (int64_t)fa.tiles[0] * fa.tiles[1] * fa.tiles[2];
+        hipLaunchKernelGGL(k_mg_tile_fluid, dim3((unsigned)nt), dim3(kBlock), 0, s->stream, (const uint8_t*)s->flags,
+                           fa.n[0], fa.n[1], fa.n[2], fa.tiles[0], fa.tiles[1], fa.tiles[2], s->mg_tfl);
+    }
     return hipGetLastError();
 }
 
@@ -657,10 +1062,73 @@ float* mg_first_sweep_target(fe_scene* s) { return s->mg.empty() ? nullptr : s->
 // z = M^-1 r with one V-cycle.  The caller has already written level 0's
 // first sweep (x1 = w r) into mg[0].x.  Leaves partials of r.z in prz.  Every
 // kernel returns at once when `st` says the solve has stopped.
-hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, double* prz, int tag,
-                    const double* pconf, int nparts) {
+static void launch_tail(fe_scene* s, SolverScalars* st) {
     std::vector<MGLevel>& lv = s->mg;
     const int L = (int)lv.size();
+    const int T = s->mg_tail;
+    TailArgs ta{};
+    ta.levels = L - T;
+    int off = 0;
+    for (int t = 0; t < ta.levels; ++t) {
+        const MGLevel& l = lv[T + t];
+        for (int q = 0; q < 3; ++q) {
+            ta.n[t][q] = l.n[q];
+            ta.ih2[t][q] = l.ih2[q];
+        }
+        ta.code[t] = l.code;
+        ta.off[t] = off;
+        off += (int)l.cells;
+    }
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s->stream, ta, (const float*)lv[T].b, lv[T].x2, st);
+}
+
+// The fused V-cycle: level q's descent writes its pre-smoothed x2 and level
+// q+1's right-hand side; the tail leaves level T's result in lv[T].x2; level
+// q's ascent reads x2 and the coarse result and writes its own result to x
+// (level 0: z).  x is never read before the ascent writes it, so no level
+// holds a first sweep in memory.
+static hipError_t mg_apply_fused(fe_scene* s, const float* r, float* z, SolverScalars* st, double* prz, int tag,
+                                 const double* pconf, int nparts) {
+    std::vector<MGLevel>& lv = s->mg;
+    const int T = s->mg_tail;
+    lv[0].b = const_cast<float*>(r);
+    for (int q = 0; q < T; ++q) {
+        const MGLevel& l = lv[q];
+        const MGLevel& c = lv[q + 1];
+        const FusedArgs a = fused_args(l, c);
+        const dim3 grid((unsigned)a.padded);
+        if (q == 0)
+            launch_timed(s, FE_KERNEL_MG_DOWN, tag, k_mg_down<true>, grid, dim3(kFusedThreads), a, (const float*)l.b,
+                         (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b, (const SolverScalars*)st, pconf,
+                         nparts, (const uint8_t*)s->mg_tfl);
+        else
+            hipLaunchKernelGGL(k_mg_down<false>, grid, dim3(kFusedThreads), 0, s->stream, a, (const float*)l.b,
+                               (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b, (const SolverScalars*)st,
+                               (const double*)nullptr, 0, (const uint8_t*)nullptr);
+    }
+    launch_tail(s, st);
+    for (int q = T - 1; q >= 0; --q) {
+        const MGLevel& l = lv[q];
+        const MGLevel& c = lv[q + 1];
+        const FusedArgs a = fused_args(l, c);
+        const dim3 grid((unsigned)a.padded);
+        const float* xc = q + 1 == T ? c.x2 : c.x;
+        if (q == 0)
+            launch_timed(s, FE_KERNEL_MG_UP, tag, k_mg_up<true>, grid, dim3(kFusedThreads), a, (const float*)l.x2, xc,
+                         (const float*)l.b, (const uint16_t*)l.code, z, prz, (const SolverScalars*)st,
+                         (const uint8_t*)s->mg_tfl);
+        else
+            hipLaunchKernelGGL(k_mg_up<false>, grid, dim3(kFusedThreads), 0, s->stream, a, (const float*)l.x2, xc,
+                               (const float*)l.b, (const uint16_t*)l.code, l.x, (double*)nullptr,
+                               (const SolverScalars*)st, (const uint8_t*)nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, double* prz, int tag,
+                    const double* pconf, int nparts) {
+    if (s->tune.mg == FE_MG_FUSED) return mg_apply_fused(s, r, z, st, prz, tag, pconf, nparts);
+    std::vector<MGLevel>& lv = s->mg;
     const int T = s->mg_tail;
     lv[0].b = const_cast<float*>(r);
     const uint8_t* cfl0 = s->chunk_fluid;
@@ -684,23 +1152,7 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
                            l.n[1], l.n[2], (const uint16_t*)c.code, c.b, q + 1 < T ? c.x : (float*)nullptr, c.n[0],
                            c.n[1], c.n[2], c.ih2[0], c.ih2[1], c.ih2[2], st);
     }
-    {
-        TailArgs ta{};
-        ta.levels = L - T;
-        int off = 0;
-        for (int t = 0; t < ta.levels; ++t) {
-            const MGLevel& l = lv[T + t];
-            for (int q = 0; q < 3; ++q) {
-                ta.n[t][q] = l.n[q];
-                ta.ih2[t][q] = l.ih2[q];
-            }
-            ta.code[t] = l.code;
-            ta.off[t] = off;
-            off += (int)l.cells;
-        }
-        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), 0, s->stream, ta, (const float*)lv[T].b,
-                           lv[T].x2, st);
-    }
+    launch_tail(s, st);
     for (int q = T - 1; q >= 0; --q) {
         MGLevel& l = lv[q];
         const MGLevel& c = lv[q + 1];

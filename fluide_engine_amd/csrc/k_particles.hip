@@ -796,6 +796,7 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
         s->cur = 1 - s->cur;
     }
     s->sorted = true;
+    s->sdf_from_cells = 0;  // new bins: an SDF in memory no longer matches them
     return hipSuccess;
 }
 

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
                                                     float* __restrict__ dnew, float* __restrict__ q,
                                                     const SolverScalars* __restrict__ in,
                                                     SolverScalars* __restrict__ out, const double* __restrict__ pin,
-                                                    const double* __restrict__ pz, double* __restrict__ pout,
+                                                    const double* __restrict__ pz, int nz, double* __restrict__ pout,
                                                     int nparts, float ihx2, float ihy2, float ihz2,
                                                     const uint8_t* __restrict__ cfl, int64_t chunk, int64_t padded) {
     if (in->done) {  // uniform: an earlier spmv stopped the solve
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     double t[2];
     if (pz) {  // r.z from the V-cycle's last sweep, r.r from the update: one pass
         double u[2];
-        sum_partials_two(pz, 1, 0, pin, 2, 1, nparts, u);
+        sum_partials_two(pz, 1, 0, nz, pin, 2, 1, nparts, u);
         t[0] = u[0];
         t[1] = u[1];
     } else {
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
                 t.d = d[c];
                 t.q = q[c];
                 t.r = r[c];
-                t.cd = pre ? code[c] : (uint16_t)0;
+                t.cd = (pre == 1 || mgx) ? code[c] : (uint16_t)0;  // for z or the first sweep
                 return t;
             },
             [&](int64_t c, const In& t) {
@@ -560,6 +560,8 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
 
     // CG / PCG
     if (mg && (e = mg_setup(s)) != hipSuccess) return e;
+    const bool fused = mg && s->tune.mg == FE_MG_FUSED;
+    if (fused) PZ = s->mg_pz;  // one r.z partial per workgroup of the level-0 ascent
     // The CG and level-0 multigrid kernels skip the chunks without FLUID cells
     // (s->chunk_fluid): every vector they would write there starts at 0 and,
     // FLUID rows coupling only to FLUID cells, stays 0 (k_cg_init writes r, d,
@@ -568,10 +570,13 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
         float* zero32[4] = {s->z, s->q, s->d[1], mg ? s->mg[0].x2 : nullptr};
         for (float* a : zero32)
             if (a && (e = launch_fill_f32(s->stream, a, 0.0f, nc)) != hipSuccess) return e;
+        // the fused V-cycle skips level-0 tiles without FLUID cells: the coarse
+        // right-hand side below them stays 0
+        if (fused && (e = launch_fill_f32(s->stream, s->mg[1].b, 0.0f, s->mg[1].cells)) != hipSuccess) return e;
         if (mg && (e = launch_fill_f32(s->stream, s->mg[0].r, 0.0f, nc)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p2, nc);
     }
-    float* mgx = mg ? mg_first_sweep_target(s) : nullptr;
+    float* mgx = mg && !fused ? mg_first_sweep_target(s) : nullptr;  // the fused descent sweeps from b itself
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
                        s->r, s->z, s->d[0], mgx, s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
     if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ, -1, nullptr, 0)) != hipSuccess) return e;
@@ -587,6 +592,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
     int first = prev > 0 ? std::min(prev + 1, 64) : batch;
     const float* zr = init.precond ? s->z : s->r;
     const double* pz = mg ? PZ : nullptr;
+    const int nz = fused ? (int)s->mg_npz : nparts;
     // spmv k decides the stop of update k-1, so every batch ends with an spmv
     // whose state slot the host reads; launches after the stop exit at once
     // and are left out of the kernel timer (tags >= the stopping iteration).
@@ -607,7 +613,7 @@ hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
             launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
                          (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
                          (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, pz,
-                         PA, nparts, ih2[0], ih2[1], ih2[2], (const uint8_t*)s->chunk_fluid, sh.chunk, sh.blocks);
+                         nz, PA, nparts, ih2[0], ih2[1], ih2[2], (const uint8_t*)s->chunk_fluid, sh.chunk, sh.blocks);
             if (t == nb - 1) break;
             update(k);
             ++k;

@@ -302,6 +302,101 @@ void plane_caps(const NodeGrid& G, int p, const double* val, std::vector<double>
         }
 }
 
+// Caps of one boundary plane as the reference builds them (SolveXY / SolveYZ /
+// SolveXZ with SolveSingleSquare, :272-589, FE_SURFACE_REFERENCE): the
+// reference's square loop order and corner order per plane, its square table
+// (kMcRefSquare), a crossing placed from the square's own edge direction when
+// the plane's vertex map first meets it, the plane's outward axis normal for
+// every vertex.  Corner k of a square is node (cu, cv) + kRefSq[p][k] in the
+// plane's (u, v) coordinates, u the lower other axis.
+void plane_caps_reference(const NodeGrid& G, int p, const double* val, std::vector<double>& pos,
+                          std::vector<double>& nrm, std::vector<int32_t>& tri, int64_t vbase) {
+    // (du, dv) per corner; planes 2*axis + side: x (left, right), y (down, up), z (back, front)
+    static const int kRefSq[6][4][2] = {
+        {{0, 0}, {0, 1}, {1, 1}, {1, 0}},  // left  (u = y, v = z): (j,k) (j,k+1) (j+1,k+1) (j+1,k)
+        {{0, 1}, {0, 0}, {1, 0}, {1, 1}},  // right: (j,k+1) (j,k) (j+1,k) (j+1,k+1)
+        {{0, 0}, {1, 0}, {1, 1}, {0, 1}},  // down  (u = x, v = z): (i,k) (i+1,k) (i+1,k+1) (i,k+1)
+        {{1, 0}, {0, 0}, {0, 1}, {1, 1}},  // up:    (i+1,k) (i,k) (i,k+1) (i+1,k+1)
+        {{1, 0}, {0, 0}, {0, 1}, {1, 1}},  // back  (u = x, v = y): (i+1,j) (i,j) (i,j+1) (i+1,j+1)
+        {{0, 0}, {1, 0}, {1, 1}, {0, 1}},  // front: (i,j) (i+1,j) (i+1,j+1) (i,j+1)
+    };
+    const int axis = p >> 1, side = p & 1;
+    const int u = axis == 0 ? 1 : 0, v = axis == 2 ? 1 : 2;
+    const int nu = G.n[u], nv = G.n[v];
+    const double outward = side ? 1.0 : -1.0;
+    const int fixed = side ? G.n[axis] - 1 : 0;
+    const int64_t nnode = (int64_t)nu * nv, nue = (int64_t)(nu - 1) * nv;
+    std::vector<int32_t> id(nnode + nue + (int64_t)nu * (nv - 1), -1);
+    auto node_pos = [&](int a_u, int a_v, double out[3]) {
+        int idx[3];
+        idx[axis] = fixed;
+        idx[u] = a_u;
+        idx[v] = a_v;
+        for (int a = 0; a < 3; ++a) out[a] = G.o[a] + idx[a] * G.h[a];
+    };
+    auto add_vertex = [&](const double q[3]) {
+        const int64_t vi = vbase + (int64_t)pos.size() / 3;
+        for (int a = 0; a < 3; ++a) {
+            pos.push_back(q[a]);
+            nrm.push_back(a == axis ? outward : 0.0);
+        }
+        return (int32_t)vi;
+    };
+    auto value = [&](int a_u, int a_v) { return val[(int64_t)a_u + (int64_t)nu * a_v]; };
+    // the reference's loops: z planes j outer, i inner; x planes k outer, j
+    // inner; y planes k outer, i inner -- always v outer, u inner here
+    for (int jv = 0; jv + 1 < nv; ++jv)
+        for (int iu = 0; iu + 1 < nu; ++iu) {
+            int cu[4], cv[4];
+            double phi[4];
+            int cs = 0;
+            for (int t = 0; t < 4; ++t) {
+                cu[t] = iu + kRefSq[p][t][0];
+                cv[t] = jv + kRefSq[p][t][1];
+                phi[t] = value(cu[t], cv[t]);
+                if (phi[t] <= 0.0) cs |= 1 << t;
+            }
+            if (cs == 0) continue;
+            const char* tt = kMcRefSquare[cs];
+            const int n = (int)std::strlen(tt);
+            for (int q = 0; q + 2 < n; q += 3) {
+                int32_t f[3];
+                for (int r = 0; r < 3; ++r) {
+                    const int slot = mc_hex(tt[q + r]);
+                    if (slot < 4) {  // a corner node
+                        const int64_t key = (int64_t)cu[slot] + (int64_t)nu * cv[slot];
+                        if (id[key] < 0) {
+                            double pp[3];
+                            node_pos(cu[slot], cv[slot], pp);
+                            id[key] = add_vertex(pp);
+                        }
+                        f[r] = id[key];
+                    } else {  // the crossing on square edge slot-4, from corner t to t+1
+                        const int t = slot - 4, t1 = (t + 1) & 3;
+                        const int lo = (cu[t] + cv[t] <= cu[t1] + cv[t1]) ? t : t1;
+                        const bool along_u = cv[t] == cv[t1];
+                        const int64_t key = along_u ? nnode + (int64_t)cu[lo] + (int64_t)(nu - 1) * cv[lo]
+                                                    : nnode + nue + (int64_t)cu[lo] + (int64_t)nu * cv[lo];
+                        if (id[key] < 0) {
+                            // SolveSingleSquare's alpha and clamps (:536-549)
+                            const double f0 = std::fabs(phi[t]), f1 = std::fabs(phi[t1]);
+                            double alpha = f0 + f1 > 1e-12 ? f0 / (f0 + f1) : 0.5f;
+                            if (alpha < 0.000001f) alpha = 0.000001f;
+                            if (alpha > 0.999999f) alpha = 0.999999f;
+                            double p0[3], p1[3], pp[3];
+                            node_pos(cu[t], cv[t], p0);
+                            node_pos(cu[t1], cv[t1], p1);
+                            for (int a = 0; a < 3; ++a) pp[a] = (1.f - alpha) * p0[a] + alpha * p1[a];
+                            id[key] = add_vertex(pp);
+                        }
+                        f[r] = id[key];
+                    }
+                }
+                tri.insert(tri.end(), f, f + 3);
+            }
+        }
+}
+
 // The surface SDF with a collider: the fluid SDF, replaced by minus the
 // collider SDF where both are negative (GetSurface, src/hybrid_simulator.cpp:29-37).
 __global__ __launch_bounds__(kBlock) void k_mc_combine(const double* __restrict__ sdf,
@@ -327,6 +422,7 @@ void surface_free(fe_scene* s) {
     F(m.toff);
     F(m.sums);
     F(m.tables);
+    F(m.tables_ref);
     F(m.planes);
     F(m.pos);
     F(m.nrm);
@@ -341,10 +437,14 @@ hipError_t surface_build(fe_scene* s) {
     const int64_t nedges = G.ne[0] + G.ne[1] + G.ne[2];
     const int64_t ncubes = (int64_t)(G.n[0] - 1) * (G.n[1] - 1) * (G.n[2] - 1);
     hipError_t e;
+    const bool ref = s->surface_mode == FE_SURFACE_REFERENCE;
     if (!m.tables) {
         const McTables host = make_mc_tables();
         if ((e = hipMalloc((void**)&m.tables, sizeof(McTables))) != hipSuccess) return e;
         if ((e = hipMemcpy(m.tables, &host, sizeof(McTables), hipMemcpyHostToDevice)) != hipSuccess) return e;
+        const McTables href = make_mc_tables_reference();
+        if ((e = hipMalloc((void**)&m.tables_ref, sizeof(McTables))) != hipSuccess) return e;
+        if ((e = hipMemcpy(m.tables_ref, &href, sizeof(McTables), hipMemcpyHostToDevice)) != hipSuccess) return e;
         const int64_t tiles = scan_tiles(std::max(nedges, ncubes));
         if ((e = hipMalloc((void**)&m.flag, sizeof(int32_t) * (nedges + 1))) != hipSuccess) return e;
         if ((e = hipMalloc((void**)&m.vid, sizeof(int32_t) * (nedges + 1))) != hipSuccess) return e;
@@ -355,6 +455,11 @@ hipError_t surface_build(fe_scene* s) {
         if ((e = hipMalloc((void**)&m.planes, sizeof(double) * 6 * m.plane_stride)) != hipSuccess) return e;
     }
     hipStream_t st = s->stream;
+    // the reference's SDF values: the 8-bucket search over the same particles
+    // where the last SDF came from the cell scan (the negative values, and so
+    // the markers, are the same either way)
+    if (s->sdf_from_cells && (e = launch_sdf(s, true)) != hipSuccess) return e;
+    const McTables* tables = (const McTables*)(ref ? m.tables_ref : m.tables);
     const double* sdf = s->sdf;
     if (s->has_collider) {
         const int64_t nc = cell_count(s->g);
@@ -366,8 +471,7 @@ hipError_t surface_build(fe_scene* s) {
     // cubes: crossed edges -> vertex ids; per-cube triangle counts -> offsets
     hipLaunchKernelGGL(k_mc_edges, grid_of(nedges), dim3(kBlock), 0, st, G, sdf, m.flag, nedges);
     if ((e = launch_scan_with(st, m.flag, m.vid, nedges, m.sums)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_mc_count, grid_of(ncubes), dim3(kBlock), 0, st, G, sdf,
-                       (const McTables*)m.tables, m.cnt, ncubes);
+    hipLaunchKernelGGL(k_mc_count, grid_of(ncubes), dim3(kBlock), 0, st, G, sdf, tables, m.cnt, ncubes);
     if ((e = launch_scan_with(st, m.cnt, m.toff, ncubes, m.sums)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_mc_planes, grid_of(6 * m.plane_stride), dim3(kBlock), 0, st, G, sdf,
                        m.planes, m.plane_stride);
@@ -387,8 +491,8 @@ hipError_t surface_build(fe_scene* s) {
         hipLaunchKernelGGL(k_mc_vertices, grid_of(nedges), dim3(kBlock), 0, st, G, sdf,
                            (const int32_t*)m.vid, m.pos, m.nrm, nedges);
     if (nt > 0)
-        hipLaunchKernelGGL(k_mc_emit, grid_of(ncubes), dim3(kBlock), 0, st, G, sdf,
-                           (const McTables*)m.tables, (const int32_t*)m.toff, (const int32_t*)m.vid, m.tri, ncubes);
+        hipLaunchKernelGGL(k_mc_emit, grid_of(ncubes), dim3(kBlock), 0, st, G, sdf, tables,
+                           (const int32_t*)m.toff, (const int32_t*)m.vid, m.tri, ncubes);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // caps on the host while the device writes the cube part
     m.cap_pos.clear();
@@ -397,7 +501,11 @@ hipError_t surface_build(fe_scene* s) {
     const int order[6] = {4, 5, 0, 1, 2, 3};  // XY (z planes), YZ (x planes), XZ (y planes), as the reference
     for (int q = 0; q < 6; ++q) {
         const int p = order[q];
-        plane_caps(G, p, planes.data() + (size_t)p * m.plane_stride, m.cap_pos, m.cap_nrm, m.cap_tri, nv);
+        if (ref)
+            plane_caps_reference(G, p, planes.data() + (size_t)p * m.plane_stride, m.cap_pos, m.cap_nrm, m.cap_tri,
+                                 nv);
+        else
+            plane_caps(G, p, planes.data() + (size_t)p * m.plane_stride, m.cap_pos, m.cap_nrm, m.cap_tri, nv);
     }
     m.nv3 = nv;
     m.nt3 = nt;

@@ -33,7 +33,9 @@ enum fe_status {
     FE_E_HIP = -2,         /* a HIP runtime call failed (message names it) */
     FE_E_NOMEM = -3,       /* device or host allocation failed */
     FE_E_UNSUPPORTED = -4, /* a feature this backend does not implement */
-    FE_E_NO_DEVICE = -5    /* no usable gfx950 device */
+    FE_E_NO_DEVICE = -5,   /* no usable gfx950 device */
+    FE_E_NONFINITE = -6    /* NaN or Inf in the state: uploaded particles, or face velocities
+                              when a frame computes its CFL number (the state has diverged) */
 };
 
 /* Transfer scheme: PICSimulator (src/pic_simulator.hpp:27), FLIPSimulator
@@ -177,7 +179,8 @@ int fe_last_diffusion_stats(const fe_scene* scene, fe_solve_stats stats[3]);
 /* Particles: ParticleSystem "postion"/"velocity" attributes
  * (src/hybrid_simulator.hpp:50-51) and the APIC affine rows _cX/_cY/_cZ
  * (src/apic_simulator.hpp:21-23; C is 9 floats per particle: cx, cy, cz rows,
- * or NULL for zero).  Upload replaces all particles; download returns them in
+ * or NULL for zero).  Upload replaces all particles (FE_E_NONFINITE, and
+ * nothing replaced, if a value is NaN or Inf); download returns them in
  * upload order. */
 int fe_particles_upload(fe_scene* scene, const float* x, const float* v, const float* C, int64_t count);
 int fe_particles_download(fe_scene* scene, float* x, float* v, float* C, int64_t count);
@@ -187,7 +190,9 @@ int fe_particles_count(const fe_scene* scene, int64_t* count);
 int fe_substep(fe_scene* scene, double dt, const fe_solver_desc* solver, fe_solve_stats* stats);
 
 /* One frame: PhysicsAnimation::AdvanceTimeStep (src/physics_animation.cpp:64-92),
- * CFL-adaptive substeps; substeps_out may be NULL. */
+ * CFL-adaptive substeps; substeps_out may be NULL.  FE_E_NONFINITE when a
+ * substep's CFL evaluation finds a NaN or Inf face velocity (the reference has
+ * no such check and would go on with the diverged state). */
 int fe_advance_frame(fe_scene* scene, double frame_dt, const fe_solver_desc* solver, int32_t* substeps_out);
 
 /* Batches of independent scenes (no reference counterpart; the reference runs
@@ -310,9 +315,18 @@ int fe_scene_particle_capacity(const fe_scene* scene, int64_t* capacity);
  * be NULL: p2g_ms[variant], g2p_ms[variant]).  Defaults: TILES, QUADS. */
 enum fe_p2g_variant { FE_P2G_TILES = 0, FE_P2G_FACES = 1 };
 enum fe_g2p_variant { FE_G2P_QUADS = 0, FE_G2P_PAIRS = 1 };
+/* The multigrid V-cycle of FE_SOLVER_MGPCG on the levels above its
+ * one-workgroup tail: one descent and one ascent kernel per level over LDS
+ * tiles with recomputed halos (FE_MG_FUSED), or a kernel per sweep, residual,
+ * restriction and prolongation (FE_MG_SPLIT, the default).  Both apply the
+ * same preconditioner (the same values per cell); they sum r.z over
+ * different blocks, so FP64 roundings of the CG scalars may differ.  Not
+ * autotuned. */
+enum fe_mg_variant { FE_MG_FUSED = 0, FE_MG_SPLIT = 1 };
 typedef struct fe_tuning {
     int32_t p2g; /* fe_p2g_variant */
     int32_t g2p; /* fe_g2p_variant */
+    int32_t mg;  /* fe_mg_variant */
 } fe_tuning;
 typedef struct fe_tuning_times {
     double p2g_ms[2];
@@ -383,8 +397,22 @@ int fe_scene_warm_start_scale(const fe_scene* scene, double* alpha);
  * boundary node planes.  Grids need fewer than 2^31 node edges. */
 int fe_surface_build(fe_scene* scene, int64_t* vertex_count, int64_t* triangle_count);
 int fe_surface_download(fe_scene* scene, double* vertices, double* normals, int32_t* triangles);
+/* Triangulation of the surface: FE_SURFACE_REFERENCE (the default) uses the
+ * reference's case tables for cubes and boundary squares
+ * (src/3d/marching_cubes_solver.cpp:652-952), so every cube and cap square is
+ * triangulated as the reference does, ambiguous cases and their holes
+ * included; FE_SURFACE_CLOSED derives every case from one face rule, which
+ * closes the surface everywhere.  Both place the same vertices.  In both,
+ * when the scene's last fluid SDF came from the cell scan
+ * (FE_SDF_SEARCH_CELLS), fe_surface_build first rebuilds it from the same
+ * particles with the reference's 8-bucket search, so the positive values that
+ * place the vertices are the reference's (the markers do not change). */
+enum fe_surface_mode { FE_SURFACE_REFERENCE = 0, FE_SURFACE_CLOSED = 1 };
+int fe_scene_set_surface_mode(fe_scene* scene, int32_t mode);
 
-/* PICSimulator::Cfl (src/pic_simulator.cpp:80-97). */
+/* PICSimulator::Cfl (src/pic_simulator.cpp:80-97).  NaN face velocities are
+ * skipped by the maximum, as by the reference's std::max; any NaN or Inf face
+ * value makes the call return FE_E_NONFINITE (cfl_out still holds the value). */
 int fe_cfl(fe_scene* scene, double dt, double* cfl_out);
 
 /* Statistics of the most recent pressure solve (SinglePhasePressureSolver
@@ -433,7 +461,9 @@ enum fe_kernel_id {
     FE_KERNEL_G2P_MOVE = 4,   /* fused G2P + advection */
     FE_KERNEL_SDF = 5,        /* fluid SDF */
     FE_KERNEL_JACOBI = 6,     /* one Jacobi sweep */
-    FE_KERNEL_MG_SMOOTH = 7   /* multigrid smoothing sweep on the finest level (3 per V-cycle) */
+    FE_KERNEL_MG_SMOOTH = 7,  /* FE_MG_SPLIT: smoothing sweep on the finest level (3 per V-cycle) */
+    FE_KERNEL_MG_DOWN = 8,    /* FE_MG_FUSED: finest level's descent (sweep, residual, restriction) */
+    FE_KERNEL_MG_UP = 9       /* FE_MG_FUSED: finest level's ascent (prolongation, two sweeps, r.z) */
 };
 int fe_kernel_timer(fe_scene* scene, int32_t kernel_id);
 /* Total device ms and launch count since the last reset (reset != 0 clears). */

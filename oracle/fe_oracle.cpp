@@ -47,6 +47,7 @@
 #include <vector>
 
 #include "fe_oracle.h"
+#include "../fluide_engine_amd/csrc/fe_mc_table.hpp"
 
 namespace {
 
@@ -285,6 +286,7 @@ struct orc_scene {
     int nthreads = 1;
     int has_collider = 0;
     int fast_dots = 0;  // orc_set_fast_dots: storage-order blocked dot products in the solvers
+    int surface_mode = 0;  // ORC_SURFACE_*
     int planted = 0;  // ORC_PLANT_*: a deliberate restatement mistake, set only by the pinning tests
     double xo[3], yo[3], zo[3];  // face data origins, src/grid_systems/face_centered_grid3d.cpp:324-332
 
@@ -1315,7 +1317,191 @@ void mc_gradient(const Arr3& sdf, const double ih[3], int i, int j, int k, doubl
     }
 }
 
+// The mesh in malloc'd arrays for the C ABI (freed by orc_free).
+int mesh_out(const McMesh& M, double** vout, double** nout, int32_t** tout, int64_t* nv, int64_t* nt) {
+    *nv = (int64_t)M.v.size() / 3;
+    *nt = (int64_t)M.t.size() / 3;
+    *vout = (double*)std::malloc(sizeof(double) * std::max<size_t>(M.v.size(), 1));
+    *nout = (double*)std::malloc(sizeof(double) * std::max<size_t>(M.n.size(), 1));
+    *tout = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(M.t.size(), 1));
+    std::copy(M.v.begin(), M.v.end(), *vout);
+    std::copy(M.n.begin(), M.n.end(), *nout);
+    std::copy(M.t.begin(), M.t.end(), *tout);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// MarchingCubesSolver::BuildSurface with the reference's own case tables
+// (src/3d/marching_cubes_solver.cpp:25-609, tables :611-952 in
+// fluide_engine_amd/csrc/fe_mc_table.hpp), restated step by step: cubes in
+// i, j, k loop order (k innermost), the case from the corner order of
+// SolveSingleCube, each crossed edge's vertex placed from the first cube that
+// meets it (BoundingBox3D::Corner of that cube, the edge's table direction),
+// triangles in table order; then the caps of SolveXY (back, front), SolveYZ
+// (left, right) and SolveXZ (down, up), each plane with its own vertex map,
+// corner order and loop order, triangulated by the square table.
+// ---------------------------------------------------------------------------
+int surface_reference(orc_scene* s, McMesh& M) {
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
+    const double ih[3] = {1.0 / s->h[0], 1.0 / s->h[1], 1.0 / s->h[2]};
+    Arr3 sdf = s->sdf;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                if (s->sdf(i, j, k) < 0 && s->csdf(i, j, k) < 0) sdf(i, j, k) = s->csdf(i, j, k) * (-1);
+    auto hexv = [](char c) { return c <= '9' ? c - '0' : c - 'a' + 10; };
+    // the reference's cube corners as (x, y, z) offsets, and the bounding-box
+    // corner each maps to (indexMap, :256)
+    static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                  {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+    static const int indexMap[8] = {0, 1, 5, 4, 2, 3, 7, 6};
+    std::map<int64_t, int32_t> vmap;
+    auto edge_key = [&](int axis, int i, int j, int k) {
+        return (int64_t)axis + 3 * ((int64_t)i + (int64_t)(nx + 1) * ((int64_t)j + (int64_t)(ny + 1) * k));
+    };
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 0; j < ny - 1; ++j)
+            for (int k = 0; k < nz - 1; ++k) {
+                double data[8], nrm8[8][3];
+                int cs = 0;
+                for (int v = 0; v < 8; ++v) {
+                    data[v] = sdf(i + off[v][0], j + off[v][1], k + off[v][2]);
+                    mc_gradient(sdf, ih, i + off[v][0], j + off[v][1], k + off[v][2], nrm8[v]);
+                    if (data[v] <= 0.0) cs |= 1 << v;
+                }
+                if (cs == 0 || cs == 255) continue;
+                const double oc[3] = {so[0] + i * s->h[0], so[1] + j * s->h[1], so[2] + k * s->h[2]};
+                const char* t = fe::kMcRefCube[cs];
+                const int n = (int)std::strlen(t);
+                for (int q = 0; q + 2 < n; q += 3) {
+                    int32_t face[3];
+                    for (int r = 0; r < 3; ++r) {
+                        const int e = hexv(t[q + r]);
+                        const int a = fe::kMcRefEdge[e][0], b = fe::kMcRefEdge[e][1];
+                        int axis = 0;
+                        for (int x = 0; x < 3; ++x)
+                            if (off[a][x] != off[b][x]) axis = x;
+                        const int64_t key = edge_key(axis, i + std::min(off[a][0], off[b][0]),
+                                                     j + std::min(off[a][1], off[b][1]),
+                                                     k + std::min(off[a][2], off[b][2]));
+                        auto it = vmap.find(key);
+                        if (it != vmap.end()) {
+                            face[r] = it->second;
+                            continue;
+                        }
+                        // DistanceToZeroLevelSet (:591-601) and the clamps (:263-268)
+                        const double phi0 = data[a], phi1 = data[b];
+                        double alpha = std::fabs(phi0) + std::fabs(phi1) > 0.000000001
+                                           ? std::fabs(phi0) / (std::fabs(phi0) + std::fabs(phi1))
+                                           : 0.5;
+                        if (alpha < 0.000001) alpha = 0.000001;
+                        if (alpha > 0.999999) alpha = 0.999999;
+                        double nn[3];
+                        for (int x = 0; x < 3; ++x) {
+                            // BoundingBox3D::Corner (src/3d/bounding_box_3d.cpp:23-31)
+                            const int ca = indexMap[a], cb = indexMap[b];
+                            const double p0 = (s->h[x] * (((ca >> x) & 1) ? 0.5 : -0.5) + oc[x]) + 0.5 * s->h[x];
+                            const double p1 = (s->h[x] * (((cb >> x) & 1) ? 0.5 : -0.5) + oc[x]) + 0.5 * s->h[x];
+                            M.v.push_back((1.0 - alpha) * p0 + alpha * p1);
+                            nn[x] = (1.0 - alpha) * nrm8[a][x] + alpha * nrm8[b][x];
+                        }
+                        const double len = std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+                        for (int x = 0; x < 3; ++x) M.n.push_back(len * len > 0.0 ? nn[x] / len : nn[x]);
+                        face[r] = (int32_t)(M.v.size() / 3 - 1);
+                        vmap[key] = face[r];
+                    }
+                    M.t.insert(M.t.end(), face, face + 3);
+                }
+            }
+    // caps: the reference's corner order per plane, as (i, j, k) offsets from
+    // the square's low node, and its loops
+    struct Plane {
+        int axis, side;
+        int corner[4][3];
+    };
+    static const Plane planes[6] = {
+        {2, 0, {{1, 0, 0}, {0, 0, 0}, {0, 1, 0}, {1, 1, 0}}},  // back (SolveXY)
+        {2, 1, {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}}},  // front
+        {0, 0, {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}}},  // left (SolveYZ)
+        {0, 1, {{0, 0, 1}, {0, 0, 0}, {0, 1, 0}, {0, 1, 1}}},  // right
+        {1, 0, {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}}},  // down (SolveXZ)
+        {1, 1, {{1, 0, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1}}},  // up
+    };
+    const int dim[3] = {nx, ny, nz};
+    for (const Plane& P : planes) {
+        const int u = P.axis == 0 ? 1 : 0, w = P.axis == 2 ? 1 : 2;
+        const double outward = P.side ? 1.0 : -1.0;
+        std::map<int64_t, int32_t> pmap;
+        // outer loop w (j for z planes, k otherwise), inner loop u
+        for (int iw = 0; iw + 1 < dim[w]; ++iw)
+            for (int iu = 0; iu + 1 < dim[u]; ++iu) {
+                int node[4][3];
+                double phi[4];
+                int cs = 0;
+                for (int t = 0; t < 4; ++t) {
+                    int base[3];
+                    base[P.axis] = P.side ? dim[P.axis] - 1 : 0;
+                    base[u] = iu;
+                    base[w] = iw;
+                    for (int x = 0; x < 3; ++x) node[t][x] = base[x] + (x == P.axis ? 0 : P.corner[t][x]);
+                    phi[t] = sdf(node[t][0], node[t][1], node[t][2]);
+                    if (phi[t] <= 0.0) cs |= 1 << t;
+                }
+                if (cs == 0) continue;
+                const char* t = fe::kMcRefSquare[cs];
+                const int n = (int)std::strlen(t);
+                for (int q = 0; q + 2 < n; q += 3) {
+                    int32_t face[3];
+                    for (int r = 0; r < 3; ++r) {
+                        const int slot = hexv(t[q + r]);
+                        int64_t key;
+                        double p[3];
+                        if (slot < 4) {
+                            key = 4 * ((int64_t)node[slot][0] + (int64_t)(nx + 1) *
+                                       ((int64_t)node[slot][1] + (int64_t)(ny + 1) * node[slot][2]));
+                            for (int x = 0; x < 3; ++x) p[x] = so[x] + node[slot][x] * s->h[x];
+                        } else {
+                            const int e0 = slot - 4, e1 = (e0 + 1) % 4;  // _edgeConnection2D
+                            int lo[3], axis = 0;
+                            for (int x = 0; x < 3; ++x) {
+                                lo[x] = std::min(node[e0][x], node[e1][x]);
+                                if (node[e0][x] != node[e1][x]) axis = x;
+                            }
+                            key = 4 * ((int64_t)lo[0] + (int64_t)(nx + 1) * ((int64_t)lo[1] + (int64_t)(ny + 1) * lo[2])) +
+                                  1 + axis;
+                            // SolveSingleSquare (:524-546)
+                            const double f0 = std::fabs(phi[e0]), f1 = std::fabs(phi[e1]);
+                            double alpha = f0 + f1 > 1e-12 ? f0 / (f0 + f1) : 0.5f;
+                            if (alpha < 0.000001f) alpha = 0.000001f;
+                            if (alpha > 0.999999f) alpha = 0.999999f;
+                            for (int x = 0; x < 3; ++x)
+                                p[x] = (1.f - alpha) * (so[x] + node[e0][x] * s->h[x]) +
+                                       alpha * (so[x] + node[e1][x] * s->h[x]);
+                        }
+                        auto it = pmap.find(key);
+                        if (it == pmap.end()) {
+                            for (int x = 0; x < 3; ++x) {
+                                M.v.push_back(p[x]);
+                                M.n.push_back(x == P.axis ? outward : 0.0);
+                            }
+                            it = pmap.emplace(key, (int32_t)(M.v.size() / 3 - 1)).first;
+                        }
+                        face[r] = it->second;
+                    }
+                    M.t.insert(M.t.end(), face, face + 3);
+                }
+            }
+    }
+    return 0;
+}
+
 int surface(orc_scene* s, double** vout, double** nout, int32_t** tout, int64_t* nv, int64_t* nt) {
+    if (s->surface_mode == ORC_SURFACE_REFERENCE) {
+        McMesh R;
+        surface_reference(s, R);
+        return mesh_out(R, vout, nout, tout, nv, nt);
+    }
     const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
     const double so[3] = {s->origin[0] + s->h[0] / 2.0, s->origin[1] + s->h[1] / 2.0, s->origin[2] + s->h[2] / 2.0};
     const double ih[3] = {1.0 / s->h[0], 1.0 / s->h[1], 1.0 / s->h[2]};
@@ -1475,15 +1661,7 @@ int surface(orc_scene* s, double** vout, double** nout, int32_t** tout, int64_t*
                 }
             }
     }
-    *nv = (int64_t)M.v.size() / 3;
-    *nt = (int64_t)M.t.size() / 3;
-    *vout = (double*)std::malloc(sizeof(double) * std::max<size_t>(M.v.size(), 1));
-    *nout = (double*)std::malloc(sizeof(double) * std::max<size_t>(M.n.size(), 1));
-    *tout = (int32_t*)std::malloc(sizeof(int32_t) * std::max<size_t>(M.t.size(), 1));
-    std::copy(M.v.begin(), M.v.end(), *vout);
-    std::copy(M.n.begin(), M.n.end(), *nout);
-    std::copy(M.t.begin(), M.t.end(), *tout);
-    return 0;
+    return mesh_out(M, vout, nout, tout, nv, nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1793,6 +1971,8 @@ void orc_set_threads(orc_scene* s, int nthreads) { s->nthreads = std::max(1, nth
 void orc_plant_mistake(orc_scene* s, int which) { s->planted = which; }
 
 void orc_set_fast_dots(orc_scene* s, int on) { s->fast_dots = on ? 1 : 0; }
+
+void orc_set_surface_mode(orc_scene* s, int mode) { s->surface_mode = mode; }
 
 void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np) {
     s->np = np;

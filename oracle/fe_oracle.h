@@ -58,6 +58,10 @@ void orc_plant_mistake(orc_scene* s, int which);
 // for any thread count) instead of the reference's serial i-outer order: the
 // same converged answer, fast enough for tests on 128^3 grids.  Off by default.
 void orc_set_fast_dots(orc_scene* s, int on);
+// Surface triangulation: the reference's case tables (the default) or the
+// closed-surface face rule.
+enum { ORC_SURFACE_REFERENCE = 0, ORC_SURFACE_CLOSED = 1 };
+void orc_set_surface_mode(orc_scene* s, int mode);
 void orc_set_particles(orc_scene* s, const double* x, const double* v, const double* C, int64_t np);
 void orc_get_particles(const orc_scene* s, double* x, double* v, double* C);
 int64_t orc_num_particles(const orc_scene* s);

@@ -20,6 +20,7 @@ PIC, FLIP, APIC = 0, 1, 2
 AIR, FLUID, BOUNDRY = 0, 1, 2
 U, V, W, MU, MV, MW, RU, RV, RW, SDF, FLAGS, PRESSURE, RHS, COLLIDER_SDF = range(14)
 JACOBI, CG, PCG_JACOBI = 0, 1, 2
+SURFACE_REFERENCE, SURFACE_CLOSED = 0, 1
 # deliberate restatement mistakes for the pinning tests (fe_oracle.h ORC_PLANT_*)
 PLANT_NONE, PLANT_APIC_TRANSPOSE, PLANT_FLIP_DELTA_SIGN, PLANT_EULER_ADVECTION, PLANT_APIC_FACE_DIMS = range(5)
 
@@ -67,6 +68,7 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.orc_plant_mistake.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_fast_dots.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_surface_mode.argtypes = [C.c_void_p, C.c_int]
         L.orc_set_particles.argtypes = [C.c_void_p, dp, dp, dp, C.c_int64]
         L.orc_get_particles.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_num_particles.argtypes = [C.c_void_p]
@@ -180,6 +182,10 @@ class Scene:
     def set_fast_dots(self, on=True):
         """Solver dot products in storage order over fixed blocks (tests on large grids)."""
         lib().orc_set_fast_dots(self._s, 1 if on else 0)
+
+    def set_surface_mode(self, mode):
+        """SURFACE_REFERENCE (the reference's case tables, the default) or SURFACE_CLOSED."""
+        lib().orc_set_surface_mode(self._s, mode)
 
     def plant_mistake(self, which):
         """Pinning tests only: switch on one deliberate restatement mistake (PLANT_*)."""
