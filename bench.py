@@ -106,7 +106,7 @@ def parse():
     ap.add_argument("--kernel", default="p2g", choices=sorted(KERNEL_IDS),
                     help="kernel of the roofline object (default: P2G, the largest share of the substep)")
     ap.add_argument("--solver", default="mgpcg", choices=sorted(SOLVERS))
-    ap.add_argument("--mg", default="split", choices=["fused", "split"],
+    ap.add_argument("--mg", default="fused", choices=["fused", "split"],
                     help="multigrid V-cycle kernels: one descent and one ascent per level, or one per step")
     # one substep (about 20 s on 16 host threads) keeps the CPU sample within
     # half a minute; the rate is per substep, so more substeps only average it

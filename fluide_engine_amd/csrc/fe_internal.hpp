@@ -415,7 +415,7 @@ struct fe_scene {
     hipStream_t stream = nullptr;
     int depth = 3;            // ceil(maxCfl): extrapolation / BC depth
     int move_substeps = 3;    // (unsigned)max(maxCfl, 1)
-    fe_tuning tune{FE_P2G_TILES, FE_G2P_QUADS, FE_MG_SPLIT};  // kernel variants
+    fe_tuning tune{FE_P2G_TILES, FE_G2P_QUADS, FE_MG_FUSED};  // kernel variants
 
     // grid (HBM)
     float* u[3] = {nullptr, nullptr, nullptr};       // face velocities

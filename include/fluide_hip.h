@@ -317,8 +317,8 @@ enum fe_p2g_variant { FE_P2G_TILES = 0, FE_P2G_FACES = 1 };
 enum fe_g2p_variant { FE_G2P_QUADS = 0, FE_G2P_PAIRS = 1 };
 /* The multigrid V-cycle of FE_SOLVER_MGPCG on the levels above its
  * one-workgroup tail: one descent and one ascent kernel per level over LDS
- * tiles with recomputed halos (FE_MG_FUSED), or a kernel per sweep, residual,
- * restriction and prolongation (FE_MG_SPLIT, the default).  Both apply the
+ * tiles with recomputed halos (FE_MG_FUSED, the default), or a kernel per
+ * sweep, residual, restriction and prolongation (FE_MG_SPLIT).  Both apply the
  * same preconditioner (the same values per cell); they sum r.z over
  * different blocks, so FP64 roundings of the CG scalars may differ.  Not
  * autotuned. */
