@@ -1059,9 +1059,10 @@ hipError_t mg_setup(fe_scene* s) {
 
 float* mg_first_sweep_target(fe_scene* s) { return s->mg.empty() ? nullptr : s->mg[0].x; }
 
-// z = M^-1 r with one V-cycle.  The caller has already written level 0's
-// first sweep (x1 = w r) into mg[0].x.  Leaves partials of r.z in prz.  Every
-// kernel returns at once when `st` says the solve has stopped.
+// The one-workgroup tail of a V-cycle (z = M^-1 r, mg_apply below).  Once `st`
+// says the solve has stopped, every multigrid kernel still issues its first
+// loads (the stop flag is read together with them) and then returns before its
+// first write; only the CG kernels exit before loading anything.
 static void launch_tail(fe_scene* s, SolverScalars* st) {
     std::vector<MGLevel>& lv = s->mg;
     const int L = (int)lv.size();
@@ -1125,6 +1126,9 @@ static hipError_t mg_apply_fused(fe_scene* s, const float* r, float* z, SolverSc
     return hipGetLastError();
 }
 
+// z = M^-1 r with one V-cycle, leaving partials of r.z in prz.  With the split
+// kernels the caller has already written level 0's first sweep (x1 = w r)
+// into mg[0].x; the fused descent sweeps from r itself.
 hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, double* prz, int tag,
                     const double* pconf, int nparts) {
     if (s->tune.mg == FE_MG_FUSED) return mg_apply_fused(s, r, z, st, prz, tag, pconf, nparts);

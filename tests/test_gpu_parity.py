@@ -233,9 +233,9 @@ def test_pressure_cg_reaches_tolerance(ctx, kind, shape):
 def test_first_substep_solve_converges(ctx, kind, n):
     """The first substep of a fresh dam break (zero start): the solve reaches
     the FP64-checked tolerance with the solver chunks that hold no FLUID cell
-    skipped.  48^3 and 64^3 have two multigrid levels below the finest one
-    before the one-workgroup tail, a layout the 32^3 and 128^3 cases do not
-    exercise."""
+    skipped and exact zeros left outside FLUID.  At 48^3 and 64^3 one
+    multigrid level (24^3, 32^3) lies between the finest level and the
+    one-workgroup tail; 128^3 has two."""
     print(f"[gpu] first-substep solve {n}^3 kind {kind}", flush=True)
     desc = capi.scene_desc((n, n, n), method=capi.FLIP, blend=0.03, max_particles=8 * n ** 3)
     x = capi.seed_particles(desc, [capi.box((-1, -1, -1), (0.4, 0.6, 2.0))], seed=0)
