@@ -117,6 +117,8 @@ def parse():
                     help="skip the plain-CG / multigrid-PCG comparison after the timed run")
     ap.add_argument("--batch-frames", type=int, default=10,
                     help="frames of the 96-scene batch workload after the headline run (0 skips it)")
+    ap.add_argument("--no-graph-comparison", action="store_true",
+                    help="skip the comparison with substeps run as captured HIP graphs")
     ap.add_argument("--batch-workers", type=int, nargs="+", default=[4],
                     help="host worker threads of the batch (several values: one measurement each)")
     ap.add_argument("--learned-steps", type=int, default=1500,
@@ -138,10 +140,10 @@ KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
-             "--batch-frames 0 --learned-steps 0")
+             "--no-graph-comparison --batch-frames 0 --learned-steps 0")
 PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{counter} -o pmc -- "
            "python3 bench.py --steps 3 --warmup 2 --kernel none --no-cpu-baseline --no-solver-comparison "
-           "--batch-frames 0 --learned-steps 0 --copy-calibration 256")
+           "--no-graph-comparison --batch-frames 0 --learned-steps 0 --copy-calibration 256")
 
 
 def short_name(kernel):
@@ -315,6 +317,7 @@ def main():
                                        traffic_for(kern) if on_profile else None,
                                        scene.mg_variant() == capi.MG_SPLIT)
     scene.kernel_timer(KERNEL_IDS["none"])
+    graphs = None if args.no_graph_comparison else graph_substeps(scene, dt, solver, min(args.steps, 50), npart)
 
     solve_cmp = None if (args.solver == "jacobi" or args.no_solver_comparison) else solver_comparison(scene, capi, dt)
     learned = None
@@ -358,6 +361,7 @@ def main():
         "rooflines_other": rooflines,
         "pressure_solver_comparison": solve_cmp,
         "learned_warm_start": learned,
+        "graph_substeps": graphs,
         "batch_96_scenes": batch,
         "cpu_baseline": cpu,
     }
@@ -472,6 +476,29 @@ def learned_warm_start(capi, ctx, scene, n, dt, steps):
     return out
 
 
+def graph_substeps(scene, dt, solver, steps, npart):
+    """The same scene continued with its substeps as captured HIP graphs
+    (fe_scene_set_graph; bit-identical results): frames per second against the
+    eager launches of the same number of frames just before."""
+    def run(k):
+        t0 = time.perf_counter()
+        sub = 0
+        for _ in range(k):
+            sub += scene.advance_frame(dt, solver)
+        scene.sync()
+        return time.perf_counter() - t0, sub
+
+    we, se = run(steps)
+    scene.set_graph(True)
+    run(2)  # one capture per parity of the particle buffers
+    wg, sg = run(steps)
+    n_graph, captures, resumed = scene.graph_stats()
+    scene.set_graph(False)
+    return {"frames": steps, "eager_ms_per_step": round(we / se * 1e3, 4), "graph_ms_per_step": round(wg / sg * 1e3, 4),
+            "graph_mparticle_substeps_per_s": round(npart * sg / wg / 1e6, 2), "graph_substeps": n_graph,
+            "captures": captures, "resumed": resumed}
+
+
 def batch_workload(capi, ctx, frames, workers, warmup=2):
     """96 independent FLIP dam breaks on cubic grids n ~ U{24,32,48,64,96}
     (seed 1), each one fluid box with extents ~ U[0.2, 0.7], 8 particles per
@@ -525,6 +552,15 @@ def batch_workload(capi, ctx, frames, workers, warmup=2):
     out["autotune_s"] = round(time.perf_counter() - t0, 3)
     out["autotune_picks"] = picks
     out["autotuned"] = timed()
+    # the same batch with every scene's substeps as captured HIP graphs
+    for sc, _ in scenes:
+        sc.set_graph(True)
+    for _ in range(2):  # captures (one per parity of each scene's particle buffers)
+        b.advance_frame(dt, solver)
+    out["graphs"] = timed()
+    st = [sc.graph_stats() for sc, _ in scenes]
+    out["graphs"].update({"graph_substeps": sum(a for a, _, _ in st), "captures": sum(c for _, c, _ in st),
+                          "resumed": sum(r for _, _, r in st)})
     b.close()
     for sc, _ in scenes:
         sc.close()

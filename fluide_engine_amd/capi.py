@@ -47,6 +47,7 @@ EXPORTED = [
     "fe_net_clear_samples", "fe_net_train", "fe_net_loss_and_grads", "fe_scene_set_warm_start_net",
     "fe_scene_warm_start_scale", "fe_net_set_scale_mode", "fe_net_set_loss", "fe_scene_get_tuning",
     "fe_scene_set_tuning", "fe_scene_autotune", "fe_mesh_sdf_noalloc", "fe_scene_set_surface_mode",
+    "fe_scene_set_graph", "fe_scene_graph_stats",
 ]
 (KERNEL_NONE, KERNEL_CG_SPMV, KERNEL_CG_UPDATE, KERNEL_P2G, KERNEL_G2P_MOVE, KERNEL_SDF, KERNEL_JACOBI,
  KERNEL_MG_SMOOTH, KERNEL_MG_DOWN, KERNEL_MG_UP) = range(10)
@@ -138,6 +139,8 @@ def load():
     L.fe_surface_build.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.fe_scene_clear_colliders.argtypes = [vp]
     L.fe_scene_set_sdf_search.argtypes = [vp, C.c_int32]
+    L.fe_scene_set_graph.argtypes = [vp, C.c_int32]
+    L.fe_scene_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     i32p = C.POINTER(C.c_int32)
     dp = C.POINTER(C.c_double)
     L.fe_scene_add_mesh_collider.argtypes = [vp, i32p, dp, dp, dp, C.c_int64, i32p, C.c_int64]
@@ -456,6 +459,17 @@ class Scene:
 
     def set_sdf_search(self, mode):
         self.ctx.check(self.L.fe_scene_set_sdf_search(self._h, mode), "fe_scene_set_sdf_search")
+
+    def set_graph(self, enable):
+        """Run qualifying substeps as captured HIP graphs (bit-identical results)."""
+        self.ctx.check(self.L.fe_scene_set_graph(self._h, 1 if enable else 0), "fe_scene_set_graph")
+
+    def graph_stats(self):
+        """(graph substeps run, captures made, substeps that needed host iterations)."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.ctx.check(self.L.fe_scene_graph_stats(self._h, C.byref(a), C.byref(b), C.byref(c)),
+                       "fe_scene_graph_stats")
+        return a.value, b.value, c.value
 
     def add_mesh_collider(self, size, origin, spacing, vertices, triangles):
         """AddCollider(TriangleMeshCollider(size, origin, spacing, mesh)); size is (nx, ny, nz)."""

@@ -174,30 +174,34 @@ int ensure_sorted(fe_scene* s) {
     return FE_OK;
 }
 
+// The pressure solve's stats from the host mirror of its scalars and the
+// final FP64 residual sums.
+void solve_stats(fe_scene* s, const fe_solver_desc* d, double rr, double bb, float ms, fe_solve_stats* st) {
+    const SolverScalars& h = *s->sc_host;
+    fe_solve_stats& ls = s->last_stats;
+    ls.iterations = d->kind == FE_SOLVER_JACOBI ? (h.done ? h.it : d->max_iterations) : h.it;
+    ls.abs_residual = std::sqrt(rr);
+    ls.rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    ls.ms = ms;
+    ls.converged = h.converged;
+    if (st) *st = ls;
+}
+
 // Pressure stage: SinglePhasePressureSolver::Solve (single_phase_pressure_solver.cpp:21-28)
 // minus the markers (built with the SDF), plus the FP64 residual for the stats.
 int pressure(fe_scene* s, const fe_solver_desc* d, bool walls_after, fe_solve_stats* st, int ev_start) {
     if (d->warm_start == FE_WARM_START_LEARNED && !s->net)
         return set_err(s->ctx, FE_E_INVALID, "FE_WARM_START_LEARNED needs a network (fe_scene_set_warm_start_net)");
     HIPCHK(s->ctx, launch_pressure_solve(s, d));
-    const SolverScalars& h = *s->sc_host;
-    const int iters = d->kind == FE_SOLVER_JACOBI ? (h.done ? h.it : d->max_iterations) : h.it;
-    const int converged = h.converged;
     HIPCHK(s->ctx, launch_final_residual(s));
     HIPCHK(s->ctx, launch_apply_pressure(s, walls_after));
     hipEvent_t stop = s->ev[kMaxStages];
     HIPCHK(s->ctx, hipEventRecord(stop, s->stream));
     double rr = 0.0, bb = 0.0;
     HIPCHK(s->ctx, read_final_residual(s, &rr, &bb));
-    fe_solve_stats& ls = s->last_stats;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, s->ev[ev_start], stop);
-    ls.iterations = iters;
-    ls.abs_residual = std::sqrt(rr);
-    ls.rel_residual = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-    ls.ms = ms;
-    ls.converged = converged;
-    if (st) *st = ls;
+    solve_stats(s, d, rr, bb, ms, st);
     return FE_OK;
 }
 
@@ -210,7 +214,216 @@ int emit(fe_scene* s) {
     return FE_OK;
 }
 
+// The substep up to the pressure solve: OnBeginAdvanceTimeStep
+// (src/pic_simulator.cpp:156-196), ApplyBoundryCondition, ComputeExternalForces
+// (:203-212) and ComputeDiffusion (:214-222).  Launches only.
+int substep_head(fe_scene* s, double dt, StageClock* clk) {
+    fe_ctx* ctx = s->ctx;
+    // particles binned by cell (keys come fused from the previous advection)
+    if (clk) clk->begin(FE_STAGE_SORT);
+    if (!s->sorted) HIPCHK(ctx, launch_sort(s, s->keys_ready));
+    s->keys_ready = false;
+    if (clk) clk->begin(FE_STAGE_P2G);
+    HIPCHK(ctx, launch_p2g(s));
+    if (clk) clk->begin(FE_STAGE_SDF);
+    HIPCHK(ctx, launch_sdf(s));
+    HIPCHK(ctx, launch_markers(s));  // BuildMarkers reads only the SDF: done once here
+    // ApplyBoundryCondition is the wall zeroing without a collider (fused into
+    // the gravity and pressure-gradient kernels) and the full
+    // ConstrainVelocity with one
+    const bool col = s->has_collider;
+    if (clk) clk->begin(FE_STAGE_EXTRAPOLATE);
+    HIPCHK(ctx, launch_extrapolate(s));
+    HIPCHK(ctx, launch_bc(s));
+    if (clk) clk->begin(FE_STAGE_GRAVITY);
+    HIPCHK(ctx, launch_gravity(s, dt, !col));
+    if (col) HIPCHK(ctx, launch_bc(s));
+    // ComputeDiffusion when viscosity > 0, then ApplyBoundryCondition
+    if (s->desc.viscosity > 0.0) {
+        if (clk) clk->begin(FE_STAGE_DIFFUSION);
+        HIPCHK(ctx, launch_diffusion(s, dt, &s->diff_desc));
+        HIPCHK(ctx, launch_bc(s));
+    }
+    return FE_OK;
+}
+
+// The substep after the pressure solve and its application:
+// ComputeAdvection (src/pic_simulator.cpp:231-237).  Launches only.
+int substep_tail(fe_scene* s, double dt, StageClock* clk) {
+    fe_ctx* ctx = s->ctx;
+    const bool col = s->has_collider;
+    if (col) HIPCHK(ctx, launch_bc(s));
+    if (clk) clk->begin(FE_STAGE_EXTRAPOLATE);
+    HIPCHK(ctx, launch_extrapolate(s));
+    HIPCHK(ctx, launch_bc(s));
+    if (clk) clk->begin(FE_STAGE_G2P);
+    HIPCHK(ctx, launch_g2p_move(s, dt, true, true, !col));
+    HIPCHK(ctx, launch_collide(s));  // moved particles inside the collider (keys recomputed by the sort)
+    s->keys_ready = !col;
+    return FE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Substeps as HIP graphs.  A scene in graph mode runs each qualifying substep
+// as two captured graphs: (a) the stages up to the pressure solve and the
+// solve's first iterations, ending with a copy of the solver state to the
+// host; (b) the final residual, the pressure application and the advection,
+// ending with a copy of the residual sums.  Between them the host adds
+// iterations if the solve has not stopped (the same kernels, launched as in
+// an eager substep) and moves the solution into s->p if the iteration count
+// left it in the partner buffer.  dt reaches the captured kernels through
+// device memory, written by a one-thread kernel before (a).  Each graph reads
+// the buffers its capture saw, and the particle ping-pong alternates with
+// every sort, so a scene keeps one pair per parity of s->cur.  The number of
+// captured iterations is the previous solve's count + 2 (at least 4); a pair
+// is captured again when anything it baked in changes (the key below), after
+// two substeps in a row needed host iterations, or after eight that stopped
+// more than four iterations early.
+// ---------------------------------------------------------------------------
+__global__ void k_set_f64(double* __restrict__ a, double v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a = v;
+}
+
+std::mutex g_capture_mu;  // one capture at a time in the process
+
+struct GraphKey {
+    uint64_t gen;
+    fe_solver_desc desc;
+    double* p;
+    int64_t np, cap;
+    float* px0;
+    fe_tuning tune;
+};
+
+bool graph_eligible(const fe_scene* s, const fe_solver_desc* d) {
+    return s->graph_mode && s->emitters.empty() && !(s->desc.viscosity > 0.0) && !s->has_collider &&
+           d->kind != FE_SOLVER_JACOBI && d->warm_start != FE_WARM_START_LEARNED && s->keys_ready && !s->sorted &&
+           s->np > 0 && s->kt_id == FE_KERNEL_NONE && (d->kind != FE_SOLVER_MGPCG || !s->mg.empty());
+}
+
+bool same_solver(const fe_solver_desc& a, const fe_solver_desc& b) {
+    return a.kind == b.kind && a.max_iterations == b.max_iterations && a.check_interval == b.check_interval &&
+           a.relative == b.relative && a.warm_start == b.warm_start && a.tolerance == b.tolerance;
+}
+
+void graph_release(fe::SubstepGraph& G) {
+    if (G.a) (void)hipGraphExecDestroy(G.a);
+    if (G.b) (void)hipGraphExecDestroy(G.b);
+    G = fe::SubstepGraph{};
+}
+
+bool graph_key_matches(const fe::SubstepGraph& G, const fe_scene* s, const fe_solver_desc* d) {
+    return G.valid && G.gen == s->graph_gen && same_solver(G.desc, *d) && G.p == s->p &&
+           G.np == s->np && G.cap == s->cap && G.px0 == s->P[0].x && G.tune.p2g == s->tune.p2g &&
+           G.tune.g2p == s->tune.g2p && G.tune.mg == s->tune.mg;
+}
+
+// Captures what `body` launches on the scene's stream into an executable graph.
+template <typename F>
+int capture(fe_scene* s, hipGraphExec_t* out, F body) {
+    fe_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> lock(g_capture_mu);
+    HIPCHK(ctx, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    s->capturing = true;
+    const int rc = body();
+    s->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s->stream, &g);
+    if (rc != FE_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return hip_fail(ctx, ei, "hipGraphInstantiate");
+    return FE_OK;
+}
+
+int substep_graph(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st) {
+    fe_ctx* ctx = s->ctx;
+    fe::SubstepGraph& G = s->graph[s->cur & 1];
+    const int precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
+    const bool fresh = !graph_key_matches(G, s, d) || G.resume_streak >= 2 || G.idle_streak >= 8;
+    ++s->substeps_run;
+    hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(64), 0, s->stream, s->dt_dev, dt);
+    HIPCHK(ctx, hipEventRecord(s->ev[0], s->stream));
+    int rc;
+    if (fresh) {
+        graph_release(G);
+        const int prev = s->last_cg_iters[precond];
+        G.iters = std::max(4, std::min(prev + 2, d->max_iterations + 1));
+        G.gen = s->graph_gen;
+        G.desc = *d;
+        G.p = s->p;
+        G.np = s->np;
+        G.cap = s->cap;
+        G.px0 = s->P[0].x;
+        G.tune = s->tune;
+        rc = capture(s, &G.a, [&]() -> int {
+            const int r = substep_head(s, dt, nullptr);
+            if (r != FE_OK) return r;
+            HIPCHK(ctx, capture_pressure_head(s, d, G.iters, &G.run_after_a));
+            return FE_OK;
+        });
+        if (rc != FE_OK) {
+            graph_release(G);
+            return rc;
+        }
+        // the host state the head leaves (the capture ran its launch functions)
+        G.cur_after = s->cur;
+        G.sdf_from_cells_after = s->sdf_from_cells;
+        ++s->graph_captures;
+    }
+    HIPCHK(ctx, hipGraphLaunch(G.a, s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    s->cur = G.cur_after;
+    s->sorted = true;
+    s->keys_ready = false;
+    s->sdf_from_cells = G.sdf_from_cells_after;
+    bool resumed = false;
+    HIPCHK(ctx, finish_pressure_after_head(s, d, G.run_after_a, &resumed));
+    const int iters = s->sc_host->it;
+    if (resumed) {
+        ++s->graph_resumed;
+        ++G.resume_streak;
+    } else {
+        G.resume_streak = 0;
+    }
+    G.idle_streak = (!resumed && G.iters > iters + 4) ? G.idle_streak + 1 : 0;
+    if (!G.b) {
+        const SolverShape sh = solver_shape_n(cell_count(s->g));
+        rc = capture(s, &G.b, [&]() -> int {
+            HIPCHK(ctx, launch_final_residual(s));
+            HIPCHK(ctx, launch_apply_pressure(s, true));
+            const int r = substep_tail(s, dt, nullptr);
+            if (r != FE_OK) return r;
+            HIPCHK(ctx, hipMemcpyAsync(s->res_host, s->partials + 2 * sh.blocks, 2 * sizeof(double),
+                                       hipMemcpyDeviceToHost, s->stream));
+            return FE_OK;
+        });
+        if (rc != FE_OK) {
+            graph_release(G);
+            return rc;
+        }
+        G.valid = true;
+    }
+    HIPCHK(ctx, hipGraphLaunch(G.b, s->stream));
+    HIPCHK(ctx, hipEventRecord(s->ev[1], s->stream));
+    HIPCHK(ctx, hipStreamSynchronize(s->stream));
+    s->sorted = false;
+    s->keys_ready = true;
+    ++s->graph_substeps;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);
+    for (int i = 0; i < kMaxStages; ++i) s->stage_ms[i] = 0.0;
+    solve_stats(s, d, s->res_host[0], s->res_host[1], ms, st);
+    kt_flush(s);
+    return FE_OK;
+}
+
 int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st) {
+    if (graph_eligible(s, d)) return substep_graph(s, dt, d, st);
     fe_ctx* ctx = s->ctx;
     // Emitter::Emitt (OnBeginAdvanceTimeStep, src/pic_simulator.cpp:158-163),
     // before BuildCollider: the first substep sees a collider SDF of 0
@@ -220,47 +433,16 @@ int substep(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stats* st)
     }
     ++s->substeps_run;
     StageClock clk(s);
-    // particles binned by cell (keys come fused from the previous advection)
-    clk.begin(FE_STAGE_SORT);
-    if (!s->sorted) HIPCHK(ctx, launch_sort(s, s->keys_ready));
-    s->keys_ready = false;
-    // OnBeginAdvanceTimeStep (src/pic_simulator.cpp:156-196)
-    clk.begin(FE_STAGE_P2G);
-    HIPCHK(ctx, launch_p2g(s));
-    clk.begin(FE_STAGE_SDF);
-    HIPCHK(ctx, launch_sdf(s));
-    HIPCHK(ctx, launch_markers(s));  // BuildMarkers reads only the SDF: done once here
-    // ApplyBoundryCondition is the wall zeroing without a collider (fused into
-    // the gravity and pressure-gradient kernels) and the full
-    // ConstrainVelocity with one
-    const bool col = s->has_collider;
-    clk.begin(FE_STAGE_EXTRAPOLATE);
-    HIPCHK(ctx, launch_extrapolate(s));
-    HIPCHK(ctx, launch_bc(s));
-    // ComputeExternalForces (:203-212) = forces, then ApplyBoundryCondition
-    clk.begin(FE_STAGE_GRAVITY);
-    HIPCHK(ctx, launch_gravity(s, dt, !col));
-    if (col) HIPCHK(ctx, launch_bc(s));
-    // ComputeDiffusion (:214-222) when viscosity > 0, then ApplyBoundryCondition
-    if (s->desc.viscosity > 0.0) {
-        clk.begin(FE_STAGE_DIFFUSION);
-        HIPCHK(ctx, launch_diffusion(s, dt, &s->diff_desc));
-        HIPCHK(ctx, launch_bc(s));
-    }
+    int rc = substep_head(s, dt, &clk);
+    if (rc != FE_OK) return rc;
     // ComputePressure (:224-229)
+    const bool col = s->has_collider;
     const int pev = clk.n;
     clk.begin(FE_STAGE_PRESSURE);
-    int rc = pressure(s, d, !col, st, pev);
+    rc = pressure(s, d, !col, st, pev);
     if (rc != FE_OK) return rc;
-    if (col) HIPCHK(ctx, launch_bc(s));
-    // ComputeAdvection (:231-237)
-    clk.begin(FE_STAGE_EXTRAPOLATE);
-    HIPCHK(ctx, launch_extrapolate(s));
-    HIPCHK(ctx, launch_bc(s));
-    clk.begin(FE_STAGE_G2P);
-    HIPCHK(ctx, launch_g2p_move(s, dt, true, true, !col));
-    HIPCHK(ctx, launch_collide(s));  // moved particles inside the collider (keys recomputed by the sort)
-    s->keys_ready = !col;
+    rc = substep_tail(s, dt, &clk);
+    if (rc != FE_OK) return rc;
     clk.end();
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
     clk.collect();
@@ -456,6 +638,8 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(dalloc(&s->sc, 2));  // two state slots (CG alternates them)
     chk(hipHostMalloc((void**)&s->sc_host, sizeof(SolverScalars), hipHostMallocDefault));
     chk(dalloc(&s->cflmax, 4));
+    chk(dalloc(&s->dt_dev, 1));
+    chk(hipHostMalloc((void**)&s->res_host, 2 * sizeof(double), hipHostMallocDefault));
     const int64_t cap = std::max<int64_t>(d->max_particles, 1);
     s->cap = cap;
     for (int b = 0; b < 2; ++b) {
@@ -517,6 +701,7 @@ int fe_scene_set_params(fe_scene* s, double blend, double max_cfl, const double 
     s->desc.density = density;
     s->depth = (int)std::ceil(max_cfl);                        // ceil(maxCfl), src/pic_simulator.cpp:491
     s->move_substeps = (int)(unsigned)std::max(max_cfl, 1.0);  // src/pic_simulator.cpp:255
+    ++s->graph_gen;  // captured substeps hold gravity, blend and the step counts by value
     return FE_OK;
 }
 
@@ -543,6 +728,10 @@ int fe_scene_destroy(fe_scene* s) {
     collider_free(s);
     sources_free(s);
     if (s->sc_host) (void)hipHostFree(s->sc_host);
+    if (s->res_host) (void)hipHostFree(s->res_host);
+    F(s->dt_dev);
+    graph_release(s->graph[0]);
+    graph_release(s->graph[1]);
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
     }
@@ -1013,6 +1202,7 @@ int fe_scene_set_tuning(fe_scene* s, const fe_tuning* t) {
         t->mg < FE_MG_FUSED || t->mg > FE_MG_SPLIT)
         return set_err(s->ctx, FE_E_INVALID, "unknown kernel variant");
     s->tune = *t;
+    ++s->graph_gen;
     return FE_OK;
 }
 
@@ -1021,6 +1211,7 @@ int fe_scene_autotune(fe_scene* s, int32_t reps, fe_tuning* out, fe_tuning_times
     if (reps < 1 || reps > 1000) return set_err(s->ctx, FE_E_INVALID, "reps must be in [1, 1000]");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     HIPCHK(s->ctx, autotune(s, reps, times));
+    ++s->graph_gen;
     if (out) *out = s->tune;
     return FE_OK;
 }
@@ -1031,6 +1222,7 @@ int fe_scene_reserve_particles(fe_scene* s, int64_t capacity) {
         return set_err(s->ctx, FE_E_INVALID, "capacity must be in [0, 2^31 - 1]");
     HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
     HIPCHK(s->ctx, reserve_particles(s, capacity));
+    ++s->graph_gen;
     return FE_OK;
 }
 
@@ -1299,6 +1491,28 @@ int fe_scene_set_sdf_search(fe_scene* s, int32_t mode) {
     if (mode != FE_SDF_SEARCH_CELLS && mode != FE_SDF_SEARCH_BUCKETS)
         return set_err(s->ctx, FE_E_INVALID, "unknown SDF search mode " + std::to_string(mode));
     s->sdf_buckets = mode == FE_SDF_SEARCH_BUCKETS;
+    ++s->graph_gen;
+    return FE_OK;
+}
+
+int fe_scene_set_graph(fe_scene* s, int32_t enable) {
+    if (!s) return FE_E_INVALID;
+    if (enable != 0 && enable != 1) return set_err(s->ctx, FE_E_INVALID, "enable must be 0 or 1");
+    HIPCHK(s->ctx, hipSetDevice(s->ctx->device));
+    HIPCHK(s->ctx, hipStreamSynchronize(s->stream));
+    s->graph_mode = enable;
+    if (!enable) {
+        graph_release(s->graph[0]);
+        graph_release(s->graph[1]);
+    }
+    return FE_OK;
+}
+
+int fe_scene_graph_stats(const fe_scene* s, int64_t* substeps, int64_t* captures, int64_t* resumed) {
+    if (!s) return FE_E_INVALID;
+    if (substeps) *substeps = s->graph_substeps;
+    if (captures) *captures = s->graph_captures;
+    if (resumed) *resumed = s->graph_resumed;
     return FE_OK;
 }
 

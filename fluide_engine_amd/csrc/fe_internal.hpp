@@ -342,6 +342,39 @@ struct SolverScalars {
 // ---------------------------------------------------------------------------
 // Host-side scene.
 // ---------------------------------------------------------------------------
+// The state of a conjugate-gradient solve between its launches (the
+// iterations' buffers rotate with the iteration count).
+struct CGRun {
+    int k = 0;        // spmv / update pairs launched
+    int dcur = 0;     // d[dcur] holds the current search direction
+    double* xa = nullptr;  // current solution
+    double* xb = nullptr;  // its ping-pong partner
+};
+
+// One captured substep: graph `a` runs the stages up to the pressure solve's
+// first `iters` spmv launches and copies the solver state to the host; graph
+// `b` runs the stages from the pressure application on.  The key fields say
+// which scene state the capture saw; the after-fields the host state that
+// running it leaves.
+struct SubstepGraph {
+    bool valid = false;
+    // key: what the capture baked in
+    uint64_t gen = 0;
+    fe_solver_desc desc{};
+    double* p = nullptr;
+    int64_t np = 0, cap = 0;
+    float* px0 = nullptr;
+    fe_tuning tune{};
+    int iters = 0;  // spmv launches in graph a
+    hipGraphExec_t a = nullptr, b = nullptr;
+    // host state after graph a
+    CGRun run_after_a{};
+    int cur_after = 0;
+    int sdf_from_cells_after = 0;
+    // substeps in a row that needed host iterations / stopped > 4 iterations early
+    int resume_streak = 0, idle_streak = 0;
+};
+
 struct Particles {
     float* x = nullptr;   // 3*cap: px | py | pz
     float* v = nullptr;   // 3*cap
@@ -508,6 +541,17 @@ struct fe_scene {
     int kt_used = 0;
     double kt_ms = 0.0;
     int64_t kt_launches = 0;
+
+    // Substeps as HIP graphs (fe_scene_set_graph): one pair per parity of the
+    // particle ping-pong at the substep's start, since the sort swaps the
+    // buffers every substep.
+    int graph_mode = 0;
+    bool capturing = false;      // launches go into a graph: no timers, dt from dt_dev
+    double* dt_dev = nullptr;    // the substep's dt on the device (graph substeps)
+    double* res_host = nullptr;  // pinned: ||b - A p||^2 and ||b||^2 of the last graph substep
+    uint64_t graph_gen = 1;      // bumped by every change a captured graph would not see
+    fe::SubstepGraph graph[2];
+    int64_t graph_substeps = 0, graph_captures = 0, graph_resumed = 0;
 };
 
 struct fe_batch {
@@ -538,9 +582,10 @@ inline void kt_flush(fe_scene* s, int limit = 1 << 30) {
 // Launch `kernel` on the scene's stream; when the timer selects `id`, the
 // start/stop events ride on the dispatch packet itself (hipExtLaunchKernelGGL),
 // so they time the kernel's execution as rocprofv3's kernel trace does.
+// Launches captured into a graph are never timed.
 template <typename... Args, typename F>
 inline void launch_timed(fe_scene* s, int32_t id, int tag, F kernel, dim3 grid, dim3 block, Args... args) {
-    if (s->kt_id == id && !s->kt_ev.empty()) {
+    if (s->kt_id == id && !s->kt_ev.empty() && !s->capturing) {
         if (s->kt_used == kTimerPairs) {
             (void)hipStreamSynchronize(s->stream);
             kt_flush(s);

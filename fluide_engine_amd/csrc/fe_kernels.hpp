@@ -47,6 +47,10 @@ hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n);
 
 // k_solver.hip
 hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d);  // rhs + solve (no apply)
+// a captured substep's part of a CG-family solve (start + `iters` spmv launches + the state slot to the host),
+// and what follows it on the host once the head has run
+hipError_t capture_pressure_head(fe_scene* s, const fe_solver_desc* d, int iters, CGRun* out);
+hipError_t finish_pressure_after_head(fe_scene* s, const fe_solver_desc* d, const CGRun& head, bool* resumed);
 hipError_t launch_final_residual(fe_scene* s);  // FP64 ||b - A p||^2 and ||b||^2 (device)
 hipError_t read_final_residual(fe_scene* s, double* rr, double* bb);  // waits for them
 

@@ -389,15 +389,18 @@ hipError_t launch_walls(fe_scene* s) {
 // Four consecutive faces of one component per thread, as one 16-byte vector
 // (face arrays are separate allocations, so f0 = 4t is 16-byte aligned).
 // Blocks [0, b0) take u, [b0, b1) v, [b1, ...) w.
-__global__ __launch_bounds__(kBlock) void k_gravity(Geom g, FaceSet a, float gx, float gy, float gz, int walls,
-                                                    int64_t b0, int64_t b1) {
+__global__ __launch_bounds__(kBlock) void k_gravity(Geom g, FaceSet a, float gx, float gy, float gz,
+                                                    const double* __restrict__ dtp, double g0, double g1, double g2,
+                                                    int walls, int64_t b0, int64_t b1) {
     const int64_t blk = blockIdx.x;
     const int c = blk < b0 ? 0 : (blk < b1 ? 1 : 2);
     const int64_t f0 = 4 * ((blk - (c == 0 ? 0 : (c == 1 ? b0 : b1))) * kBlock + threadIdx.x);
     const int64_t nf = face_count(g, c);
     if (f0 >= nf) return;
     float* __restrict__ A = c == 0 ? a.a[0] : (c == 1 ? a.a[1] : a.a[2]);
-    const float add = c == 0 ? gx : (c == 1 ? gy : gz);
+    // dt * g by value, or formed from dt in device memory in a captured substep
+    // (the same FP64 product rounded to FP32)
+    const float add = dtp ? (float)(*dtp * (c == 0 ? g0 : (c == 1 ? g1 : g2))) : (c == 0 ? gx : (c == 1 ? gy : gz));
     const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1);
     // index of face f on the component's normal axis (the wall layers are 0 and n)
     auto normal_index = [&](int64_t f) -> int {
@@ -436,7 +439,8 @@ hipError_t launch_gravity(fe_scene* s, double dt, bool walls_after) {
     FaceSet a{{s->u[0], s->u[1], s->u[2]}};
     hipLaunchKernelGGL(k_gravity, dim3((unsigned)(nb0 + nb1 + nb2)), dim3(kBlock), 0, s->stream, g, a,
                        (float)(dt * s->desc.gravity[0]), (float)(dt * s->desc.gravity[1]),
-                       (float)(dt * s->desc.gravity[2]), walls_after ? 1 : 0, nb0, nb0 + nb1);
+                       (float)(dt * s->desc.gravity[2]), (const double*)(s->capturing ? s->dt_dev : nullptr),
+                       s->desc.gravity[0], s->desc.gravity[1], s->desc.gravity[2], walls_after ? 1 : 0, nb0, nb0 + nb1);
     return hipGetLastError();
 }
 

@@ -619,9 +619,11 @@ __device__ inline void grad_contract(const float* __restrict__ a, const Geom& g,
 
 template <int METHOD, bool Quads>
 __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_t cap, int64_t np, FaceSet grid,
-                                                     GridSrc<Quads> gsrc, GridSrc<Quads> ssrc, float blend, float dt,
-                                                     int nsub, int do_g2p, int do_move, uint32_t* __restrict__ key,
-                                                     int32_t* __restrict__ count) {
+                                                     GridSrc<Quads> gsrc, GridSrc<Quads> ssrc, float blend, float dt_arg,
+                                                     const double* __restrict__ dtp, int nsub, int do_g2p, int do_move,
+                                                     uint32_t* __restrict__ key, int32_t* __restrict__ count) {
+    // dt comes by value, or from device memory in a captured substep (the same FP32 rounding)
+    const float dt = dtp ? (float)*dtp : dt_arg;
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = s < np;
     uint32_t k = 0;
@@ -878,18 +880,19 @@ hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bo
     uint32_t* key = emit_keys ? s->key : nullptr;
     int32_t* cnt = emit_keys ? s->cell_count : nullptr;
     const float blend = (float)s->desc.blend;
+    const double* dtp = s->capturing ? s->dt_dev : nullptr;
     if (quads) {
         auto kern = s->g.method == FE_PIC ? k_g2p_move<FE_PIC, true>
                                           : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP, true> : k_g2p_move<FE_APIC, true>);
         launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid,
-                     GridSrc<true>{grid, gq}, GridSrc<true>{snaps, sq}, blend, (float)dt, s->move_substeps,
+                     GridSrc<true>{grid, gq}, GridSrc<true>{snaps, sq}, blend, (float)dt, dtp, s->move_substeps,
                      (int)do_g2p, (int)do_move, key, cnt);
     } else {
         auto kern = s->g.method == FE_PIC
                         ? k_g2p_move<FE_PIC, false>
                         : (s->g.method == FE_FLIP ? k_g2p_move<FE_FLIP, false> : k_g2p_move<FE_APIC, false>);
         launch_timed(s, FE_KERNEL_G2P_MOVE, 0, kern, grid1(s->np), dim3(kBlock), s->g, P, s->cap, s->np, grid,
-                     GridSrc<false>{grid, gq}, GridSrc<false>{snaps, sq}, blend, (float)dt, s->move_substeps,
+                     GridSrc<false>{grid, gq}, GridSrc<false>{snaps, sq}, blend, (float)dt, dtp, s->move_substeps,
                      (int)do_g2p, (int)do_move, key, cnt);
     }
     if (do_move) s->sorted = false;

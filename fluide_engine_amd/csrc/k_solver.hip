@@ -487,149 +487,268 @@ hipError_t read_final_residual(fe_scene* s, double* rr, double* bb) {
     return hipSuccess;
 }
 
-hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
+__global__ void k_set_scalars(SolverScalars* __restrict__ sc, SolverScalars v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sc = v;
+}
+
+// The solver's start: its scalars, BuildSystem and the initial guess.
+static hipError_t solve_begin(fe_scene* s, const fe_solver_desc* d, SolverScalars& init) {
+    const Geom& g = s->g;
+    const int64_t nc = cell_count(g);
+    const SolverShape sh = solver_shape(g);
+    init = SolverScalars{};
+    init.tol = d->tolerance;
+    init.relative = d->relative;
+    init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
+    init.max_it = d->max_iterations;
+    init.it = -1;
+    // a kernel rather than a host copy, so a captured substep carries the values
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, s->stream, s->sc, init);
+    FaceSet u{{s->u[0], s->u[1], s->u[2]}};
+    hipLaunchKernelGGL(k_build_system, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->code,
+                       s->b, s->partials, s->chunk_fluid, sh.chunk, sh.blocks);
+    if (d->warm_start == FE_WARM_START_LEARNED) {
+        if (!s->net) return hipErrorInvalidValue;
+        hipError_t e = net_warm_start(s->net, s);
+        if (e != hipSuccess) return e;
+    } else if (!d->warm_start) {
+        hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p, nc);
+    }
+    return hipGetLastError();
+}
+
+// Jacobi sweeps (JacobiIterationSolver::Solve), host-driven.
+static hipError_t jacobi_solve(fe_scene* s, const fe_solver_desc* d) {
     const Geom& g = s->g;
     const int64_t nc = cell_count(g);
     const SolverShape sh = solver_shape(g);
     const int nparts = (int)sh.blocks;
-    double* PA = s->partials;                  // 2 per block
-    double* PB = s->partials + 2 * sh.blocks;  // 2 per block
-    // BuildSystem's invH * invH (single_phase_pressure_solver.cpp:102-103); exact
-    // in FP32 for power-of-two spacings
+    double* PA = s->partials;
+    double* PB = s->partials + 2 * sh.blocks;
     const float ih2[3] = {(float)(g.ihd[0] * g.ihd[0]), (float)(g.ihd[1] * g.ihd[1]), (float)(g.ihd[2] * g.ihd[2])};
     hipError_t e;
-
-    SolverScalars init{};
-    init.tol = d->tolerance;
-    init.relative = d->relative;
-    init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
-    const bool mg = init.precond == 2;
-    double* PZ = s->partials + 4 * sh.blocks;  // r.z of the V-cycle, 1 per block
-    init.max_it = d->max_iterations;
-    init.it = -1;
-    if ((e = hipMemcpyAsync(s->sc, &init, sizeof(SolverScalars), hipMemcpyHostToDevice, s->stream)) != hipSuccess)
-        return e;
-    FaceSet u{{s->u[0], s->u[1], s->u[2]}};
-    hipLaunchKernelGGL(k_build_system, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->code,
-                       s->b, PA, s->chunk_fluid, sh.chunk, sh.blocks);
-    if (d->warm_start == FE_WARM_START_LEARNED) {
-        if (!s->net) return hipErrorInvalidValue;
-        if ((e = net_warm_start(s->net, s)) != hipSuccess) return e;
-    } else if (!d->warm_start) {
-        hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p, nc);
-    }
-
-    if (d->kind == FE_SOLVER_JACOBI) {
-        // The reference's CudaJacobiIterationSolver uploads the previous x (warm start).
-        float* xa = s->r;
-        float* xb = s->z;
-        hipLaunchKernelGGL(k_f64_to_f32, grid1(nc), dim3(kBlock), 0, s->stream, s->p, xa, nc);
-        bool pending = false;
-        int pending_it = 0;
-        for (int it = 0; it < d->max_iterations; ++it) {
-            launch_timed(s, FE_KERNEL_JACOBI, 0, k_jacobi, dim3((unsigned)sh.blocks), dim3(kBlock), g,
-                         (const uint16_t*)s->code, (const float*)s->b, (const float*)xa, xb, s->sc,
-                         (const double*)(pending ? PB : nullptr), nparts, pending_it, ih2[0], ih2[1], ih2[2], sh.chunk,
-                         sh.blocks);
-            std::swap(xa, xb);
-            pending = false;
-            if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
-                hipLaunchKernelGGL(k_jacobi_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g,
-                                   s->code, s->b, xa, s->sc, PB, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
-                pending = true;
-                pending_it = it;
-            }
+    // The reference's CudaJacobiIterationSolver uploads the previous x (warm start).
+    float* xa = s->r;
+    float* xb = s->z;
+    hipLaunchKernelGGL(k_f64_to_f32, grid1(nc), dim3(kBlock), 0, s->stream, s->p, xa, nc);
+    bool pending = false;
+    int pending_it = 0;
+    for (int it = 0; it < d->max_iterations; ++it) {
+        launch_timed(s, FE_KERNEL_JACOBI, 0, k_jacobi, dim3((unsigned)sh.blocks), dim3(kBlock), g,
+                     (const uint16_t*)s->code, (const float*)s->b, (const float*)xa, xb, s->sc,
+                     (const double*)(pending ? PB : nullptr), nparts, pending_it, ih2[0], ih2[1], ih2[2], sh.chunk,
+                     sh.blocks);
+        std::swap(xa, xb);
+        pending = false;
+        if (it != 0 && d->check_interval > 0 && it % d->check_interval == 0) {
+            hipLaunchKernelGGL(k_jacobi_residual, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g,
+                               s->code, s->b, xa, s->sc, PB, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
+            pending = true;
+            pending_it = it;
         }
-        if (pending)  // a check after the last sweep: the loop ends anyway, only record convergence
-            hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kBlock), 0, s->stream, PB, nparts, PA);
-        hipLaunchKernelGGL(k_f32_to_f64, grid1(nc), dim3(kBlock), 0, s->stream, xa, s->p, nc);
+    }
+    if (pending)  // a check after the last sweep: the loop ends anyway, only record convergence
+        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kBlock), 0, s->stream, PB, nparts, PA);
+    hipLaunchKernelGGL(k_f32_to_f64, grid1(nc), dim3(kBlock), 0, s->stream, xa, s->p, nc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = read_scalars(s, 0)) != hipSuccess) return e;
+    if (pending && !s->sc_host->done) {
+        double tot = 0.0;
+        if ((e = hipMemcpy(&tot, PA, sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess) return e;
+        if (std::sqrt(tot) < d->tolerance) {
+            s->sc_host->done = 1;
+            s->sc_host->converged = 1;
+            s->sc_host->it = pending_it + 1;
+        }
+    }
+    kt_flush(s);
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// CG / PCG, host side.  spmv k decides the stop of update k-1, so every batch
+// of launches ends with an spmv whose state slot the host reads; launches
+// after the stop exit at once and are left out of the kernel timer (tags >=
+// the stopping iteration).
+// ---------------------------------------------------------------------------
+struct CGSetup {
+    Geom g;
+    SolverShape sh;
+    int nparts = 0;
+    double *PA = nullptr, *PB = nullptr, *PZ = nullptr;
+    float ih2[3] = {0, 0, 0};
+    int precond = 0;
+    bool mg = false, fused = false;
+    float* mgx = nullptr;
+    const float* zr = nullptr;
+    const double* pz = nullptr;
+    int nz = 0;
+};
+
+// The per-solve constants (after mg_setup: the fused V-cycle's partials are
+// allocated with its levels).
+static CGSetup cg_setup(fe_scene* s, int precond) {
+    CGSetup c;
+    c.g = s->g;
+    c.sh = solver_shape(s->g);
+    c.nparts = (int)c.sh.blocks;
+    c.PA = s->partials;                  // 2 per block
+    c.PB = s->partials + 2 * c.sh.blocks;  // 2 per block
+    c.PZ = s->partials + 4 * c.sh.blocks;  // r.z of the V-cycle, 1 per block
+    // BuildSystem's invH * invH (single_phase_pressure_solver.cpp:102-103); exact
+    // in FP32 for power-of-two spacings
+    for (int a = 0; a < 3; ++a) c.ih2[a] = (float)(s->g.ihd[a] * s->g.ihd[a]);
+    c.precond = precond;
+    c.mg = precond == 2;
+    c.fused = c.mg && s->tune.mg == FE_MG_FUSED;
+    if (c.fused) c.PZ = s->mg_pz;  // one r.z partial per workgroup of the level-0 ascent
+    c.mgx = c.mg && !c.fused ? mg_first_sweep_target(s) : nullptr;  // the fused descent sweeps from b itself
+    c.zr = precond ? s->z : s->r;
+    c.pz = c.mg ? c.PZ : nullptr;
+    c.nz = c.fused ? (int)s->mg_npz : c.nparts;
+    return c;
+}
+
+static void cg_update(fe_scene* s, const CGSetup& c, CGRun& R, int kk) {
+    R.dcur = 1 - R.dcur;
+    launch_timed(s, FE_KERNEL_CG_UPDATE, kk, k_cg_update, dim3((unsigned)c.sh.blocks), dim3(kBlock), c.g,
+                 (const uint16_t*)s->code, (const float*)s->b, (const double*)R.xa, R.xb, (const float*)s->d[R.dcur],
+                 (const float*)s->q, s->r, s->z, c.mgx, s->sc + ((kk + 1) & 1), (const double*)c.PA, c.PB, c.nparts,
+                 c.ih2[0], c.ih2[1], c.ih2[2], (const uint8_t*)s->chunk_fluid, c.sh.chunk, c.sh.blocks);
+    std::swap(R.xa, R.xb);
+    if (c.mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), c.PZ, kk, c.PB, c.nparts);
+}
+
+// nb spmv launches with an update (and V-cycle) between each two; R.k is the
+// last spmv's iteration afterwards.
+static void cg_batch(fe_scene* s, const CGSetup& c, CGRun& R, int nb) {
+    for (int t = 0; t < nb; ++t) {
+        launch_timed(s, FE_KERNEL_CG_SPMV, R.k, k_cg_spmv, dim3((unsigned)c.sh.blocks), dim3(kBlock), c.g,
+                     (const uint16_t*)s->code, c.zr, (const float*)s->d[R.dcur], s->d[1 - R.dcur], s->q,
+                     (const SolverScalars*)(s->sc + (R.k & 1)), s->sc + ((R.k + 1) & 1), (const double*)c.PB, c.pz,
+                     c.nz, c.PA, c.nparts, c.ih2[0], c.ih2[1], c.ih2[2], (const uint8_t*)s->chunk_fluid, c.sh.chunk,
+                     c.sh.blocks);
+        if (t == nb - 1) break;
+        cg_update(s, c, R, R.k);
+        ++R.k;
+    }
+}
+
+// The iterations' start: the solver vectors' zeros, the initial residual and
+// the first V-cycle.  The CG and level-0 multigrid kernels skip the chunks
+// without FLUID cells (s->chunk_fluid): every vector they would write there
+// starts at 0 and, FLUID rows coupling only to FLUID cells, stays 0
+// (k_cg_init writes r, d, the first sweep and the solution's zeros outside
+// FLUID itself).
+static hipError_t cg_start(fe_scene* s, const CGSetup& c, CGRun& R) {
+    const int64_t nc = cell_count(c.g);
+    hipError_t e;
+    float* zero32[4] = {s->z, s->q, s->d[1], c.mg ? s->mg[0].x2 : nullptr};
+    for (float* a : zero32)
+        if (a && (e = launch_fill_f32(s->stream, a, 0.0f, nc)) != hipSuccess) return e;
+    // the fused V-cycle skips level-0 tiles without FLUID cells: the coarse
+    // right-hand side below them stays 0
+    if (c.fused && (e = launch_fill_f32(s->stream, s->mg[1].b, 0.0f, s->mg[1].cells)) != hipSuccess) return e;
+    if (c.mg && (e = launch_fill_f32(s->stream, s->mg[0].r, 0.0f, nc)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p2, nc);
+    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)c.sh.blocks), dim3(kBlock), 0, s->stream, c.g, s->code, s->b, s->p,
+                       s->r, s->z, s->d[0], c.mgx, s->sc, c.PA, c.PB, c.nparts, c.ih2[0], c.ih2[1], c.ih2[2],
+                       c.sh.chunk, c.sh.blocks);
+    if (c.mg && (e = mg_apply(s, s->r, s->z, s->sc, c.PZ, -1, nullptr, 0)) != hipSuccess) return e;
+    R = CGRun{};
+    R.xa = s->p;
+    R.xb = s->p2;
+    return hipGetLastError();
+}
+
+// Host-driven iterations: batches of spmv launches, the host reading the last
+// one's state slot.  resume: the state slot read last said "not done" after a
+// batch launched elsewhere (a captured substep), so the next update comes
+// first.  first: spmv launches in the first batch.
+static hipError_t cg_host_loop(fe_scene* s, const fe_solver_desc* d, const CGSetup& c, CGRun& R, int first,
+                               bool resume) {
+    hipError_t e;
+    const int batch = c.mg ? 4 : 16;
+    if (resume) {
+        cg_update(s, c, R, R.k);
+        ++R.k;
+    }
+    for (;;) {
+        const int nb = std::min(first, d->max_iterations + 1 - R.k);
+        first = c.mg ? 2 : batch;
+        cg_batch(s, c, R, nb);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = read_scalars(s, 0)) != hipSuccess) return e;
-        if (pending && !s->sc_host->done) {
-            double tot = 0.0;
-            if ((e = hipMemcpy(&tot, PA, sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess) return e;
-            if (std::sqrt(tot) < d->tolerance) {
-                s->sc_host->done = 1;
-                s->sc_host->converged = 1;
-                s->sc_host->it = pending_it + 1;
-            }
-        }
-        kt_flush(s);
-        return hipSuccess;
+        if ((e = read_scalars(s, (R.k + 1) & 1)) != hipSuccess) return e;
+        kt_flush(s, s->sc_host->done ? s->sc_host->it : (1 << 30));
+        if (s->sc_host->done || nb <= 0) break;
+        cg_update(s, c, R, R.k);
+        ++R.k;
     }
+    return hipSuccess;
+}
 
-    // CG / PCG
-    if (mg && (e = mg_setup(s)) != hipSuccess) return e;
-    const bool fused = mg && s->tune.mg == FE_MG_FUSED;
-    if (fused) PZ = s->mg_pz;  // one r.z partial per workgroup of the level-0 ascent
-    // The CG and level-0 multigrid kernels skip the chunks without FLUID cells
-    // (s->chunk_fluid): every vector they would write there starts at 0 and,
-    // FLUID rows coupling only to FLUID cells, stays 0 (k_cg_init writes r, d,
-    // the first sweep and the solution's zeros outside FLUID itself).
-    {
-        float* zero32[4] = {s->z, s->q, s->d[1], mg ? s->mg[0].x2 : nullptr};
-        for (float* a : zero32)
-            if (a && (e = launch_fill_f32(s->stream, a, 0.0f, nc)) != hipSuccess) return e;
-        // the fused V-cycle skips level-0 tiles without FLUID cells: the coarse
-        // right-hand side below them stays 0
-        if (fused && (e = launch_fill_f32(s->stream, s->mg[1].b, 0.0f, s->mg[1].cells)) != hipSuccess) return e;
-        if (mg && (e = launch_fill_f32(s->stream, s->mg[0].r, 0.0f, nc)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p2, nc);
-    }
-    float* mgx = mg && !fused ? mg_first_sweep_target(s) : nullptr;  // the fused descent sweeps from b itself
-    hipLaunchKernelGGL(k_cg_init, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, s->code, s->b, s->p,
-                       s->r, s->z, s->d[0], mgx, s->sc, PA, PB, nparts, ih2[0], ih2[1], ih2[2], sh.chunk, sh.blocks);
-    if (mg && (e = mg_apply(s, s->r, s->z, s->sc, PZ, -1, nullptr, 0)) != hipSuccess) return e;
-    double* xa = s->p;
-    double* xb = s->p2;
-    int dcur = 0;
+hipError_t launch_pressure_solve(fe_scene* s, const fe_solver_desc* d) {
+    SolverScalars init;
+    hipError_t e = solve_begin(s, d, init);
+    if (e != hipSuccess) return e;
+    if (d->kind == FE_SOLVER_JACOBI) return jacobi_solve(s, d);
+    if (init.precond == 2 && (e = mg_setup(s)) != hipSuccess) return e;
+    const CGSetup c = cg_setup(s, init.precond);
+    CGRun R;
+    if ((e = cg_start(s, c, R)) != hipSuccess) return e;
     // iterations per host check: fewer when each carries a V-cycle; the first
     // check comes where the previous solve of this kind stopped (warm-started
     // solves of consecutive substeps take similar counts), so a typical solve
     // waits for the device once
-    const int batch = mg ? 4 : 16;
     const int prev = s->last_cg_iters[init.precond];
-    int first = prev > 0 ? std::min(prev + 1, 64) : batch;
-    const float* zr = init.precond ? s->z : s->r;
-    const double* pz = mg ? PZ : nullptr;
-    const int nz = fused ? (int)s->mg_npz : nparts;
-    // spmv k decides the stop of update k-1, so every batch ends with an spmv
-    // whose state slot the host reads; launches after the stop exit at once
-    // and are left out of the kernel timer (tags >= the stopping iteration).
-    int k = 0;
-    auto update = [&](int kk) {
-        dcur = 1 - dcur;
-        launch_timed(s, FE_KERNEL_CG_UPDATE, kk, k_cg_update, dim3((unsigned)sh.blocks), dim3(kBlock), g,
-                     (const uint16_t*)s->code, (const float*)s->b, (const double*)xa, xb, (const float*)s->d[dcur],
-                     (const float*)s->q, s->r, s->z, mgx, s->sc + ((kk + 1) & 1), (const double*)PA, PB, nparts, ih2[0],
-                     ih2[1], ih2[2], (const uint8_t*)s->chunk_fluid, sh.chunk, sh.blocks);
-        std::swap(xa, xb);
-        if (mg) (void)mg_apply(s, s->r, s->z, s->sc + ((kk + 1) & 1), PZ, kk, PB, nparts);
-    };
-    for (;;) {
-        const int nb = std::min(first, d->max_iterations + 1 - k);
-        first = mg ? 2 : batch;
-        for (int t = 0; t < nb; ++t) {
-            launch_timed(s, FE_KERNEL_CG_SPMV, k, k_cg_spmv, dim3((unsigned)sh.blocks), dim3(kBlock), g,
-                         (const uint16_t*)s->code, zr, (const float*)s->d[dcur], s->d[1 - dcur], s->q,
-                         (const SolverScalars*)(s->sc + (k & 1)), s->sc + ((k + 1) & 1), (const double*)PB, pz,
-                         nz, PA, nparts, ih2[0], ih2[1], ih2[2], (const uint8_t*)s->chunk_fluid, sh.chunk, sh.blocks);
-            if (t == nb - 1) break;
-            update(k);
-            ++k;
-        }
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = read_scalars(s, (k + 1) & 1)) != hipSuccess) return e;
-        kt_flush(s, s->sc_host->done ? s->sc_host->it : (1 << 30));
-        if (s->sc_host->done || nb <= 0) break;
-        update(k);
-        ++k;
-    }
-    if (xa != s->p) {
+    const int first = prev > 0 ? std::min(prev + 1, 64) : (c.mg ? 4 : 16);
+    if ((e = cg_host_loop(s, d, c, R, first, false)) != hipSuccess) return e;
+    if (R.xa != s->p) {
         s->p2 = s->p;
-        s->p = xa;
+        s->p = R.xa;
     }
     s->last_cg_iters[init.precond] = s->sc_host->done ? s->sc_host->it : d->max_iterations;
+    return hipSuccess;
+}
+
+// A captured substep's part of the solve: the start and the first `iters`
+// spmv launches, then the state slot of the last one copied into the pinned
+// host mirror.  Returns the run's state after them.
+hipError_t capture_pressure_head(fe_scene* s, const fe_solver_desc* d, int iters, CGRun* out) {
+    SolverScalars init;
+    hipError_t e = solve_begin(s, d, init);
+    if (e != hipSuccess) return e;
+    if (init.precond == 2 && (e = mg_setup(s)) != hipSuccess) return e;
+    const CGSetup c = cg_setup(s, init.precond);
+    CGRun R;
+    if ((e = cg_start(s, c, R)) != hipSuccess) return e;
+    cg_batch(s, c, R, std::max(1, std::min(iters, d->max_iterations + 1)));
+    if ((e = hipMemcpyAsync(s->sc_host, s->sc + ((R.k + 1) & 1), sizeof(SolverScalars), hipMemcpyDeviceToHost,
+                            s->stream)) != hipSuccess)
+        return e;
+    *out = R;
+    return hipGetLastError();
+}
+
+// After a captured head has run (s->sc_host holds its last state slot): the
+// host-driven iterations it left, if any, then the solution copied into s->p
+// when the iteration count left it in the partner buffer (a captured tail
+// reads s->p).  *resumed: whether iterations were added.
+hipError_t finish_pressure_after_head(fe_scene* s, const fe_solver_desc* d, const CGRun& head, bool* resumed) {
+    const int precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
+    const CGSetup c = cg_setup(s, precond);
+    CGRun R = head;
+    hipError_t e;
+    *resumed = false;
+    if (!s->sc_host->done && R.k + 1 <= d->max_iterations) {
+        *resumed = true;
+        if ((e = cg_host_loop(s, d, c, R, c.mg ? 2 : 16, true)) != hipSuccess) return e;
+    }
+    if (R.xa != s->p &&
+        (e = hipMemcpyAsync(s->p, R.xa, sizeof(double) * cell_count(s->g), hipMemcpyDeviceToDevice, s->stream)) !=
+            hipSuccess)
+        return e;
+    s->last_cg_iters[precond] = s->sc_host->done ? s->sc_host->it : d->max_iterations;
     return hipSuccess;
 }
 

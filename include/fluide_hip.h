@@ -203,6 +203,23 @@ int fe_advance_frame(fe_scene* scene, double frame_dt, const fe_solver_desc* sol
  * scene, in list order, the substep count and the wall time of its frame.
  * Returns the first failing scene's code. */
 typedef struct fe_batch fe_batch;
+
+/* Substeps as captured HIP graphs (no reference counterpart).  With enable = 1
+ * each substep of fe_substep / fe_advance_frame / fe_batch_advance_frame that
+ * qualifies (no emitter, collider or viscosity, a CG-family solver without the
+ * learned warm start, the kernel timer off, after one eager substep) runs as
+ * two graph launches instead of about 130 kernel launches: the stages up to
+ * the pressure solve with the solve's first iterations, then the pressure
+ * application and the advection.  The host adds iterations between them only
+ * when the solve has not stopped.  Results are the eager substep's, bit for
+ * bit.  In graph substeps fe_solve_stats.ms is the whole substep's device time
+ * and fe_stage_times reports zeros.  Graphs are captured on first use and
+ * again when a setting they hold changes; enable = 0 releases them.  Counters:
+ * graph substeps run, captures made, substeps that needed host iterations
+ * (pointers may be NULL). */
+int fe_scene_set_graph(fe_scene* scene, int32_t enable);
+int fe_scene_graph_stats(const fe_scene* scene, int64_t* substeps, int64_t* captures, int64_t* resumed);
+
 int fe_batch_create(fe_ctx* ctx, fe_scene* const* scenes, int32_t count, int32_t workers, fe_batch** out);
 int fe_batch_destroy(fe_batch* batch);
 int fe_batch_advance_frame(fe_batch* batch, double frame_dt, const fe_solver_desc* solver, int32_t* substeps,

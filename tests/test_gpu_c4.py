@@ -1,8 +1,8 @@
 """The batched-scenes workload at its specification: 96 FLIP dam breaks on
 cubic grids n ~ U{24, 32, 48, 64, 96} (seed 1), one fluid box each with
 extents ~ U[0.2, 0.7], 2x2x2 particles per fluid cell, multigrid-PCG to 1e-6,
-frame dt = 1/480.  Bars: every scene advanced in the batch ends bit for bit
-where it ends advanced alone; three sampled scenes (the first of 24^3, 48^3
+frame dt = 1/480.  Bars: every scene advanced in the batch, kernel by kernel
+or as captured HIP graphs, ends bit for bit where it ends advanced alone; three sampled scenes (the first of 24^3, 48^3
 and 96^3) agree with the FP64 oracle over the substep in which their falling
 box first needs a pressure solve (the oracle re-seeded from the GPU's
 particles and pressure just before it), with the 128^3 substep bars (markers
@@ -39,28 +39,48 @@ def _scene(ctx, n, lo, hi):
     return sc, x
 
 
-def test_c4_batch_bit_identical_to_alone(ctx):
+SOLVER = capi.solver(capi.SOLVER_MGPCG, 2000, 0, 1e-6, relative=True, warm_start=True)
+FRAMES = 4
+
+
+@pytest.fixture(scope="module")
+def alone(ctx):
+    """Every scene of the specification advanced alone, kernel by kernel."""
     specs = capi.batch_scene_specs()
     assert len(specs) == 96
-    solver = capi.solver(capi.SOLVER_MGPCG, 2000, 0, 1e-6, relative=True, warm_start=True)
-    frames = 2
-    print(f"[gpu] C4: 96 scenes alone, {frames} frames", flush=True)
-    alone = []
+    print(f"[gpu] C4: 96 scenes alone, {FRAMES} frames", flush=True)
+    out = []
     for n, lo, hi in specs:
         sc, _ = _scene(ctx, n, lo, hi)
-        for _ in range(frames):
-            sc.advance_frame(DT, solver)
-        alone.append(sc.download_particles()[:2])
+        for _ in range(FRAMES):
+            sc.advance_frame(DT, SOLVER)
+        out.append(sc.download_particles()[:2])
         sc.close()
-    print("[gpu] C4: the same 96 scenes batched (4 workers)", flush=True)
+    return out
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_c4_batch_bit_identical_to_alone(ctx, alone, graphs):
+    """Batched on 4 host threads, launched kernel by kernel or (graphs) with
+    every qualifying substep as captured HIP graphs, captured on the worker
+    threads while the other scenes run."""
+    specs = capi.batch_scene_specs()
+    print(f"[gpu] C4: the same 96 scenes batched (4 workers), graphs {graphs}", flush=True)
     scenes = [_scene(ctx, n, lo, hi)[0] for n, lo, hi in specs]
+    for sc in scenes:
+        sc.set_graph(graphs)
     b = capi.Batch(ctx, scenes, workers=4)
-    for _ in range(frames):
-        sub, _ = b.advance_frame(DT, solver)
+    for _ in range(FRAMES):
+        sub, _ = b.advance_frame(DT, SOLVER)
         assert all(s >= 1 for s in sub)
     for i, (sc, (ax, av)) in enumerate(zip(scenes, alone)):
         gx, gv, _ = sc.download_particles()
         assert np.array_equal(gx, ax) and np.array_equal(gv, av), f"scene {i} ({specs[i][0]}^3)"
+    if graphs:
+        st = [sc.graph_stats() for sc in scenes]
+        print(f"[gpu]   graph substeps {sum(a for a, _, _ in st)}, captures {sum(c for _, c, _ in st)}, "
+              f"resumed {sum(r for _, _, r in st)}", flush=True)
+        assert all(a >= FRAMES - 1 for a, _, _ in st)  # every substep after the first
     b.close()
     for sc in scenes:
         sc.close()
