@@ -481,20 +481,23 @@ def graph_substeps(scene, dt, solver, steps, npart):
     (fe_scene_set_graph; bit-identical results): frames per second against the
     eager launches of the same number of frames just before."""
     def run(k):
+        its = []
         t0 = time.perf_counter()
         sub = 0
         for _ in range(k):
             sub += scene.advance_frame(dt, solver)
+            its.append(scene.last_stats().iterations)
         scene.sync()
-        return time.perf_counter() - t0, sub
+        return time.perf_counter() - t0, sub, float(np.median(its)) if its else 0.0
 
-    we, se = run(steps)
+    we, se, ie = run(steps)
     scene.set_graph(True)
     run(2)  # one capture per parity of the particle buffers
-    wg, sg = run(steps)
+    wg, sg, ig = run(steps)
     n_graph, captures, resumed = scene.graph_stats()
     scene.set_graph(False)
-    return {"frames": steps, "eager_ms_per_step": round(we / se * 1e3, 4), "graph_ms_per_step": round(wg / sg * 1e3, 4),
+    return {"frames": steps, "eager_ms_per_step": round(we / se * 1e3, 4), "eager_iterations_median": ie,
+            "graph_ms_per_step": round(wg / sg * 1e3, 4), "graph_iterations_median": ig,
             "graph_mparticle_substeps_per_s": round(npart * sg / wg / 1e6, 2), "graph_substeps": n_graph,
             "captures": captures, "resumed": resumed}
 
