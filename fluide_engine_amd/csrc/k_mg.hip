@@ -10,7 +10,7 @@
 // BuildSystem's 7-point rule on those markers with twice the spacing.
 //
 // The V-cycle is symmetric, so the preconditioner is SPD and CG stays CG:
-//   * damped Jacobi smoothing (omega = 2/3), two sweeps before and two after;
+//   * damped Jacobi smoothing (omega = 6/7), two sweeps before and two after;
 //   * prolongation: trilinear interpolation of the cell-centred coarse
 //     correction (weights 3/4, 1/4 per axis; out-of-range coarse neighbours
 //     clamp onto the boundary cell, which is the Neumann wall);

@@ -181,9 +181,17 @@ __device__ inline void sort_bin_regs(int32_t* __restrict__ order, int s0, int n)
 }
 
 // Atomic slots give each bin its particles in arbitrary order; sorting every
-// bin by previous index restores a deterministic, stable order.
+// bin by previous index restores a deterministic, stable order.  Bins of up to
+// 32 particles sort in registers here; larger ones (particles pile up in a
+// few cells where the fluid is compressed: several hundred have been seen in a
+// 128^3 dam break) go to a list for k_sort_big_bins, one workgroup per bin.
+// The list lives in `cursor` (free once k_scatter has taken its slots):
+// cursor[0] counts, cursor[1 ..] holds the cells.  A bin that finds the list
+// full (more large bins than cells - 1, possible only on tiny grids) sorts by
+// insertion here.
+constexpr int kRegBin = 32;
 __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict__ start, int64_t ncells,
-                                                      int32_t* __restrict__ order) {
+                                                      int32_t* __restrict__ order, int32_t* __restrict__ big) {
     int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= ncells) return;
     const int s0 = start[c], s1 = start[c + 1];
@@ -197,6 +205,15 @@ __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict_
         sort_bin_regs<16>(order, s0, n);
         return;
     }
+    if (n <= kRegBin) {
+        sort_bin_regs<kRegBin>(order, s0, n);
+        return;
+    }
+    const int slot = atomicAdd(&big[0], 1);
+    if (slot < ncells - 1) {
+        big[1 + slot] = (int32_t)c;
+        return;
+    }
     for (int a = s0 + 1; a < s1; ++a) {
         const int32_t v = order[a];
         int b = a - 1;
@@ -205,6 +222,42 @@ __global__ __launch_bounds__(kBlock) void k_sort_bins(const int32_t* __restrict_
             --b;
         }
         order[b + 1] = v;
+    }
+}
+
+// The listed bins, one workgroup at a time each (the grid loops over the
+// list; every workgroup reads the same count, so all leave together).  The
+// entries of a bin are distinct previous indices, so each one's place is the
+// number of entries below it: ranks are counted against chunks of the bin
+// staged in LDS, the entries are written by rank into `scratch` and copied
+// back.
+constexpr int kRankChunk = 2048;
+__global__ __launch_bounds__(kBlock) void k_sort_big_bins(const int32_t* __restrict__ start,
+                                                          int32_t* __restrict__ order,
+                                                          const int32_t* __restrict__ big, int64_t ncells,
+                                                          int32_t* __restrict__ scratch) {
+    __shared__ int32_t chunk[kRankChunk];
+    const int nbig = min((int64_t)big[0], ncells - 1);
+    for (int i = blockIdx.x; i < nbig; i += gridDim.x) {
+        const int c = big[1 + i];
+        const int s0 = start[c], n = start[c + 1] - s0;
+        for (int e0 = 0; e0 < n; e0 += kBlock) {
+            const int e = e0 + threadIdx.x;
+            const int32_t v = e < n ? order[s0 + e] : 0;
+            int rank = 0;
+            for (int c0 = 0; c0 < n; c0 += kRankChunk) {
+                const int m = min(kRankChunk, n - c0);
+                __syncthreads();  // the previous chunk has been read
+                for (int u = threadIdx.x; u < m; u += kBlock) chunk[u] = order[s0 + c0 + u];
+                __syncthreads();
+                if (e < n)
+                    for (int u = 0; u < m; ++u) rank += chunk[u] < v ? 1 : 0;
+            }
+            if (e < n) scratch[s0 + rank] = v;
+        }
+        __syncthreads();  // every rank written before the copy reads them
+        for (int u = threadIdx.x; u < n; u += kBlock) order[s0 + u] = scratch[s0 + u];
+        __syncthreads();  // the copy done before the next bin's first chunk is staged
     }
 }
 
@@ -777,6 +830,8 @@ hipError_t launch_scan(fe_scene* s, const int32_t* in, int32_t* out, int64_t n) 
     return launch_scan_with(s->stream, in, out, n, s->scan_sums);
 }
 
+constexpr int kBigBinGroups = 256;  // workgroups looping over the large-bin list
+
 // Bins the particles of P[cur] by cell into P[1-cur]; keys and counts must be
 // ready when keys_ready (the fused advection kernel produced them).
 hipError_t launch_sort(fe_scene* s, bool keys_ready) {
@@ -791,7 +846,13 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
         if ((e = launch_fill_i32(s->stream, s->cursor, 0, nc)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->np, s->cell_start,
                            s->cursor, s->order);
-        hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order);
+        // the large-bin list in cursor: its count starts at 0
+        if ((e = launch_fill_i32(s->stream, s->cursor, 0, 1)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order,
+                           s->cursor);
+        hipLaunchKernelGGL(k_sort_big_bins, dim3(kBigBinGroups), dim3(kBlock), 0, s->stream,
+                           (const int32_t*)s->cell_start, s->order, (const int32_t*)s->cursor, nc,
+                           reinterpret_cast<int32_t*>(s->key));
         hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, s->order, s->np, s->cap,
                            s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4, s->st4);
         if ((e = hipGetLastError()) != hipSuccess) return e;
