@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
 
 // ---------------------------------------------------------------------------
 // Large levels, fused (FE_MG_FUSED): one descent and one ascent kernel per
-// level.  A workgroup owns a coarse tile of kRtx x kRty x kRtz cells and the
+// level.  A workgroup owns a coarse tile of T::rx x T::ry x T::rz cells and the
 // fine tile of twice that size below it.  It stages what the tile's sweeps
 // read, with halos, in LDS and recomputes the halo cells of each sweep itself
 // instead of waiting for a neighbour tile, so a level takes two launches
@@ -378,18 +378,32 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
 //             for the ascent), r = b - A x2, coarse b = R r;
 //   ascent:   y = x2 + P xc, x' = y + w (b - A y), out = x' + w (b - A x'),
 //             with level 0's out = z and the partials of r.z.
-// Window of halo H around the fine tile: coordinates 0 .. kFt + 2H - 1 stand
-// for fine indices f0 - H ...
+// Window of halo H around the fine tile: coordinates 0 .. T::fx + 2H - 1
+// stand for fine indices f0 - H ...
+// Two tile shapes: the finest level's work is throughput-bound and takes large
+// tiles (little halo per cell); the levels below are latency-bound (a 32^3
+// level has 16 large tiles for 256 CUs, and each workgroup's chain of LDS
+// phases sets the kernel's time), so they take small tiles: 2.5x fewer cells
+// per thread in every phase for more halo work in all.  A cell's value does
+// not depend on the tile it is computed in.
 // ---------------------------------------------------------------------------
-constexpr int kFtx = 2 * kRtx, kFty = 2 * kRty, kFtz = 2 * kRtz;
-template <int H>
+template <int RX, int RY, int RZ>
+struct FusedTile {
+    static constexpr int rx = RX, ry = RY, rz = RZ;              // coarse tile
+    static constexpr int fx = 2 * RX, fy = 2 * RY, fz = 2 * RZ;  // fine tile
+    static constexpr int coarse = RX * RY * RZ;
+};
+using TileFinest = FusedTile<8, 8, 4>;
+using TileCoarse = FusedTile<4, 4, 4>;
+template <class T, int H>
 struct FWin {
-    static constexpr int x = kFtx + 2 * H, y = kFty + 2 * H, z = kFtz + 2 * H;
+    static constexpr int x = T::fx + 2 * H, y = T::fy + 2 * H, z = T::fz + 2 * H;
     static constexpr int sy = x, sz = x * y, n = x * y * z;
 };
-// coarse window of the ascent: coarse indices I0 - 2 .. I0 + kRt + 1 per axis
+// coarse window of the ascent: coarse indices I0 - 2 .. I0 + T::rx + 1 per axis
+template <class T>
 struct CWin {
-    static constexpr int x = kRtx + 4, y = kRty + 4, z = kRtz + 4;
+    static constexpr int x = T::rx + 4, y = T::ry + 4, z = T::rz + 4;
     static constexpr int sy = x, sz = x * y, n = x * y * z;
 };
 
@@ -441,15 +455,15 @@ __device__ inline void sum_partials_fused(const double* __restrict__ part, int n
     __syncthreads();
 }
 
-template <bool L0>
+template <bool L0, class T>
 __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const float* __restrict__ b,
                                                     const uint16_t* __restrict__ code, float* __restrict__ x2out,
                                                     const uint16_t* __restrict__ ccode, float* __restrict__ cb,
                                                     const SolverScalars* st, const double* __restrict__ pconf,
                                                     int nconf, const uint8_t* __restrict__ tfl) {
-    using W3 = FWin<3>;
-    using W2 = FWin<2>;
-    using W1 = FWin<1>;
+    using W3 = FWin<T, 3>;
+    using W2 = FWin<T, 2>;
+    using W1 = FWin<T, 1>;
     __shared__ float sX1[W3::n];  // x1; later, over W1, the residual
     __shared__ float sB[W2::n];
     __shared__ uint16_t sC[W2::n];
@@ -479,12 +493,12 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
     if (L0 && tfl && !tfl[t]) return;
     const int TX = (int)(t % a.tiles[0]), TY = (int)((t / a.tiles[0]) % a.tiles[1]),
               TZ = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
-    const int I0 = TX * kRtx, J0 = TY * kRty, K0 = TZ * kRtz;
+    const int I0 = TX * T::rx, J0 = TY * T::ry, K0 = TZ * T::rz;
     const int fx0 = 2 * I0, fy0 = 2 * J0, fz0 = 2 * K0;
-    // this thread's coarse cell (the restriction; the first kRtx*kRty*kRtz threads)
-    const int tx = threadIdx.x % kRtx, ty = (threadIdx.x / kRtx) % kRty, tz = threadIdx.x / (kRtx * kRty);
+    // this thread's coarse cell (the restriction; the first T::coarse threads)
+    const int tx = threadIdx.x % T::rx, ty = (threadIdx.x / T::rx) % T::ry, tz = threadIdx.x / (T::rx * T::ry);
     const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
-    const bool cin = threadIdx.x < kRtx * kRty * kRtz && I < a.c[0] && J < a.c[1] && K < a.c[2];
+    const bool cin = threadIdx.x < T::coarse && I < a.c[0] && J < a.c[1] && K < a.c[2];
     const int64_t ci = (int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K);
     // b and the row codes on the tile + 3: every load issued before any LDS store
     constexpr int kPer = (W3::n + kFusedThreads - 1) / kFusedThreads;
@@ -529,7 +543,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
                              sX1[e3 + W3::sy], sX1[e3 - W3::sz], sX1[e3 + W3::sz]};
         const float x2 = mg_sweep_cell(sC[e], sX1[e3], sB[e], nb, a.ih2);
         sX2[e] = x2;
-        if (wx >= 2 && wx < 2 + kFtx && wy >= 2 && wy < 2 + kFty && wz >= 2 && wz < 2 + kFtz) {
+        if (wx >= 2 && wx < 2 + T::fx && wy >= 2 && wy < 2 + T::fy && wz >= 2 && wz < 2 + T::fz) {
             const int i = fx0 + wx - 2, j = fy0 + wy - 2, k = fz0 + wz - 2;
             if (fine_in(a, i, j, k)) x2out[(int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k)] = x2;
         }
@@ -572,19 +586,19 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
 
 // out = the level's corrected and post-smoothed solution.  With prz (level 0)
 // every launched workgroup leaves one partial of b . out (r.z).
-template <bool L0>
+template <bool L0, class T>
 __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const float* __restrict__ x2,
                                                   const float* __restrict__ xc, const float* __restrict__ b,
                                                   const uint16_t* __restrict__ code, float* __restrict__ out,
                                                   double* __restrict__ prz, const SolverScalars* __restrict__ st,
                                                   const uint8_t* __restrict__ tfl) {
-    using W2 = FWin<2>;
-    using W1 = FWin<1>;
+    using W2 = FWin<T, 2>;
+    using W1 = FWin<T, 1>;
     __shared__ float sY[W2::n];
     __shared__ uint16_t sC[W2::n];
     __shared__ float sB[W1::n];
     __shared__ float sXp[W1::n];
-    __shared__ float sXc[CWin::n];
+    __shared__ float sXc[CWin<T>::n];
     const int done = st->done;  // read with the data, tested before the first write
     const int64_t t = fused_tile(a);
     const bool real = t < (int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2] && !(L0 && tfl && !tfl[t]);
@@ -592,16 +606,16 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
     if (real) {
         const int TX = (int)(t % a.tiles[0]), TY = (int)((t / a.tiles[0]) % a.tiles[1]),
                   TZ = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
-        fx0 = 2 * TX * kRtx;
-        fy0 = 2 * TY * kRty;
-        fz0 = 2 * TZ * kRtz;
-        cx0 = TX * kRtx - 2;
-        cy0 = TY * kRty - 2;
-        cz0 = TZ * kRtz - 2;
+        fx0 = 2 * TX * T::rx;
+        fy0 = 2 * TY * T::ry;
+        fz0 = 2 * TZ * T::rz;
+        cx0 = TX * T::rx - 2;
+        cy0 = TY * T::ry - 2;
+        cz0 = TZ * T::rz - 2;
         // x2 and the row codes on the tile + 2, b on the tile + 1, the coarse
         // correction on its window: every load issued before any LDS store
         constexpr int kP2 = (W2::n + kFusedThreads - 1) / kFusedThreads;
-        constexpr int kPc = (CWin::n + kFusedThreads - 1) / kFusedThreads;
+        constexpr int kPc = (CWin<T>::n + kFusedThreads - 1) / kFusedThreads;
         float vx[kP2], vb[kP2];
         uint16_t vc[kP2];
         float vxc[kPc];
@@ -626,8 +640,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
         for (int u = 0; u < kPc; ++u) {
             const int e = threadIdx.x + u * kFusedThreads;
             vxc[u] = 0.0f;
-            if (e < CWin::n) {
-                const int wx = e % CWin::x, wy = (e / CWin::x) % CWin::y, wz = e / CWin::sz;
+            if (e < CWin<T>::n) {
+                const int wx = e % CWin<T>::x, wy = (e / CWin<T>::x) % CWin<T>::y, wz = e / CWin<T>::sz;
                 const int I = min(max(cx0 + wx, 0), a.c[0] - 1), J = min(max(cy0 + wy, 0), a.c[1] - 1),
                           K = min(max(cz0 + wz, 0), a.c[2] - 1);
                 vxc[u] = xc[(int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K)];
@@ -647,7 +661,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
 #pragma unroll
         for (int u = 0; u < kPc; ++u) {
             const int e = threadIdx.x + u * kFusedThreads;
-            if (e < CWin::n) sXc[e] = vxc[u];
+            if (e < CWin<T>::n) sXc[e] = vxc[u];
         }
     }
     __syncthreads();
@@ -660,7 +674,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
             const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
             const float add = prolong_at3(fx0 - 2 + wx, fy0 - 2 + wy, fz0 - 2 + wz, a.c[0], a.c[1], a.c[2],
                                           [&](int I, int J, int K) {
-                                              return sXc[(I - cx0) + CWin::sy * (J - cy0) + CWin::sz * (K - cz0)];
+                                              return sXc[(I - cx0) + CWin<T>::sy * (J - cy0) + CWin<T>::sz * (K - cz0)];
                                           });
             sY[e] = sY[e] + add;
         }
@@ -679,9 +693,9 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
     __syncthreads();
     if (real) {
         // out on the tile
-        constexpr int kTile = kFtx * kFty * kFtz;
+        constexpr int kTile = T::fx * T::fy * T::fz;
         for (int e = threadIdx.x; e < kTile; e += kFusedThreads) {
-            const int wx = e % kFtx, wy = (e / kFtx) % kFty, wz = e / (kFtx * kFty);
+            const int wx = e % T::fx, wy = (e / T::fx) % T::fy, wz = e / (T::fx * T::fy);
             const int i = fx0 + wx, j = fy0 + wy, k = fz0 + wz;
             if (!fine_in(a, i, j, k)) continue;
             const int e1 = (wx + 1) + W1::sy * (wy + 1) + W1::sz * (wz + 1);
@@ -715,8 +729,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_tile_fluid(const uint8_t* __restr
     const int64_t t = blockIdx.x;
     const int TX = (int)(t % tx), TY = (int)((t / tx) % ty), TZ = (int)(t / ((int64_t)tx * ty));
     int any = 0;
-    for (int e = threadIdx.x; e < kFtx * kFty * kFtz; e += kBlock) {
-        const int i = TX * kFtx + e % kFtx, j = TY * kFty + (e / kFtx) % kFty, k = TZ * kFtz + e / (kFtx * kFty);
+    using T = TileFinest;
+    for (int e = threadIdx.x; e < T::fx * T::fy * T::fz; e += kBlock) {
+        const int i = TX * T::fx + e % T::fx, j = TY * T::fy + (e / T::fx) % T::fy, k = TZ * T::fz + e / (T::fx * T::fy);
         if (i < nx && j < ny && k < nz && flags[(int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k)] == FE_FLUID)
             any = 1;
     }
@@ -938,6 +953,7 @@ static LevelArgs level_args(const MGLevel& l) {
     return a;
 }
 
+template <class T>
 static FusedArgs fused_args(const MGLevel& l, const MGLevel& c) {
     FusedArgs a;
     for (int q = 0; q < 3; ++q) {
@@ -945,9 +961,9 @@ static FusedArgs fused_args(const MGLevel& l, const MGLevel& c) {
         a.c[q] = c.n[q];
         a.ih2[q] = l.ih2[q];
     }
-    a.tiles[0] = (c.n[0] + kRtx - 1) / kRtx;
-    a.tiles[1] = (c.n[1] + kRty - 1) / kRty;
-    a.tiles[2] = (c.n[2] + kRtz - 1) / kRtz;
+    a.tiles[0] = (c.n[0] + T::rx - 1) / T::rx;
+    a.tiles[1] = (c.n[1] + T::ry - 1) / T::ry;
+    a.tiles[2] = (c.n[2] + T::rz - 1) / T::rz;
     a.padded = pad_blocks((int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2]);
     return a;
 }
@@ -1004,7 +1020,7 @@ static hipError_t mg_alloc(fe_scene* s) {
         if ((e = hipMalloc((void**)&l.r, n * sizeof(float))) != hipSuccess) break;
     }
     // the fused V-cycle's level-0 tile flags and r.z partials
-    const FusedArgs fa = fused_args(lv[0], lv[1]);
+    const FusedArgs fa = fused_args<TileFinest>(lv[0], lv[1]);
     s->mg_npz = fa.padded;
     if (e == hipSuccess) e = hipMalloc((void**)&s->mg_tfl, (size_t)fa.padded);
     if (e == hipSuccess) e = hipMalloc((void**)&s->mg_pz, sizeof(double) * (size_t)fa.padded);
@@ -1049,7 +1065,7 @@ hipError_t mg_setup(fe_scene* s) {
                            c.n[1], c.n[2], c.code);
     }
     if (s->tune.mg == FE_MG_FUSED) {
-        const FusedArgs fa = fused_args(s->mg[0], s->mg[1]);
+        const FusedArgs fa = fused_args<TileFinest>(s->mg[0], s->mg[1]);
         const int64_t nt = (int64_t)fa.tiles[0] * fa.tiles[1] * fa.tiles[2];
         hipLaunchKernelGGL(k_mg_tile_fluid, dim3((unsigned)nt), dim3(kBlock), 0, s->stream, (const uint8_t*)s->flags,
                            fa.n[0], fa.n[1], fa.n[2], fa.tiles[0], fa.tiles[1], fa.tiles[2], s->mg_tfl);
@@ -1096,14 +1112,14 @@ static hipError_t mg_apply_fused(fe_scene* s, const float* r, float* z, SolverSc
     for (int q = 0; q < T; ++q) {
         const MGLevel& l = lv[q];
         const MGLevel& c = lv[q + 1];
-        const FusedArgs a = fused_args(l, c);
+        const FusedArgs a = q == 0 ? fused_args<TileFinest>(l, c) : fused_args<TileCoarse>(l, c);
         const dim3 grid((unsigned)a.padded);
         if (q == 0)
-            launch_timed(s, FE_KERNEL_MG_DOWN, tag, k_mg_down<true>, grid, dim3(kFusedThreads), a, (const float*)l.b,
-                         (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b, (const SolverScalars*)st, pconf,
-                         nparts, (const uint8_t*)s->mg_tfl);
+            launch_timed(s, FE_KERNEL_MG_DOWN, tag, k_mg_down<true, TileFinest>, grid, dim3(kFusedThreads), a,
+                         (const float*)l.b, (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b,
+                         (const SolverScalars*)st, pconf, nparts, (const uint8_t*)s->mg_tfl);
         else
-            hipLaunchKernelGGL(k_mg_down<false>, grid, dim3(kFusedThreads), 0, s->stream, a, (const float*)l.b,
+            hipLaunchKernelGGL((k_mg_down<false, TileCoarse>), grid, dim3(kFusedThreads), 0, s->stream, a, (const float*)l.b,
                                (const uint16_t*)l.code, l.x2, (const uint16_t*)c.code, c.b, (const SolverScalars*)st,
                                (const double*)nullptr, 0, (const uint8_t*)nullptr);
     }
@@ -1111,15 +1127,15 @@ static hipError_t mg_apply_fused(fe_scene* s, const float* r, float* z, SolverSc
     for (int q = T - 1; q >= 0; --q) {
         const MGLevel& l = lv[q];
         const MGLevel& c = lv[q + 1];
-        const FusedArgs a = fused_args(l, c);
+        const FusedArgs a = q == 0 ? fused_args<TileFinest>(l, c) : fused_args<TileCoarse>(l, c);
         const dim3 grid((unsigned)a.padded);
         const float* xc = q + 1 == T ? c.x2 : c.x;
         if (q == 0)
-            launch_timed(s, FE_KERNEL_MG_UP, tag, k_mg_up<true>, grid, dim3(kFusedThreads), a, (const float*)l.x2, xc,
-                         (const float*)l.b, (const uint16_t*)l.code, z, prz, (const SolverScalars*)st,
-                         (const uint8_t*)s->mg_tfl);
+            launch_timed(s, FE_KERNEL_MG_UP, tag, k_mg_up<true, TileFinest>, grid, dim3(kFusedThreads), a,
+                         (const float*)l.x2, xc, (const float*)l.b, (const uint16_t*)l.code, z, prz,
+                         (const SolverScalars*)st, (const uint8_t*)s->mg_tfl);
         else
-            hipLaunchKernelGGL(k_mg_up<false>, grid, dim3(kFusedThreads), 0, s->stream, a, (const float*)l.x2, xc,
+            hipLaunchKernelGGL((k_mg_up<false, TileCoarse>), grid, dim3(kFusedThreads), 0, s->stream, a, (const float*)l.x2, xc,
                                (const float*)l.b, (const uint16_t*)l.code, l.x, (double*)nullptr,
                                (const SolverScalars*)st, (const uint8_t*)nullptr);
     }
