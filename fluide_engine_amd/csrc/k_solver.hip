@@ -33,7 +33,9 @@ namespace fe {
 __global__ __launch_bounds__(kBlock) void k_build_system(Geom g, FaceSet u, const uint8_t* __restrict__ flags,
                                                          uint16_t* __restrict__ code, float* __restrict__ b,
                                                          double* __restrict__ pout, uint8_t* __restrict__ cfl,
-                                                         int64_t chunk, int64_t padded) {
+                                                         int64_t chunk, int64_t padded, SolverScalars* __restrict__ sc,
+                                                         SolverScalars init) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sc = init;  // the solver's start state (read by the later kernels)
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
     const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1], nz = (uint32_t)g.n[2];
@@ -487,8 +489,35 @@ hipError_t read_final_residual(fe_scene* s, double* rr, double* bb) {
     return hipSuccess;
 }
 
-__global__ void k_set_scalars(SolverScalars* __restrict__ sc, SolverScalars v) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *sc = v;
+// Zeros for up to 8 buffers in one launch: buffer q takes blocks
+// [first[q], first[q + 1]), four 32-bit words per thread (one 16-byte store;
+// every buffer starts 256-byte aligned).
+struct ZeroList {
+    int n = 0;
+    uint32_t* p[8] = {};
+    int64_t words[8] = {};
+    int64_t first[9] = {};
+};
+
+__global__ __launch_bounds__(kBlock) void k_zero_many(ZeroList L) {
+    const int64_t blk = blockIdx.x;
+    int q = 0;
+    while (q + 1 < L.n && blk >= L.first[q + 1]) ++q;
+    const int64_t w0 = ((blk - L.first[q]) * kBlock + threadIdx.x) * 4;
+    uint32_t* __restrict__ p = L.p[q];
+    if (w0 + 4 <= L.words[q]) {
+        *reinterpret_cast<uint4*>(p + w0) = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        for (int64_t w = w0; w < L.words[q]; ++w) p[w] = 0u;
+    }
+}
+
+static void zero_add(ZeroList& L, void* p, int64_t bytes) {
+    if (!p || bytes <= 0) return;
+    L.p[L.n] = static_cast<uint32_t*>(p);
+    L.words[L.n] = bytes / 4;
+    L.first[L.n + 1] = L.first[L.n] + (L.words[L.n] + 4 * kBlock - 1) / (4 * kBlock);
+    ++L.n;
 }
 
 // The solver's start: its scalars, BuildSystem and the initial guess.
@@ -502,11 +531,11 @@ static hipError_t solve_begin(fe_scene* s, const fe_solver_desc* d, SolverScalar
     init.precond = d->kind == FE_SOLVER_PCG_JACOBI ? 1 : (d->kind == FE_SOLVER_MGPCG ? 2 : 0);
     init.max_it = d->max_iterations;
     init.it = -1;
-    // a kernel rather than a host copy, so a captured substep carries the values
-    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, s->stream, s->sc, init);
+    // the scalars' start goes with BuildSystem's launch (a kernel argument
+    // rather than a host copy, so a captured substep carries the values)
     FaceSet u{{s->u[0], s->u[1], s->u[2]}};
     hipLaunchKernelGGL(k_build_system, dim3((unsigned)sh.blocks), dim3(kBlock), 0, s->stream, g, u, s->flags, s->code,
-                       s->b, s->partials, s->chunk_fluid, sh.chunk, sh.blocks);
+                       s->b, s->partials, s->chunk_fluid, sh.chunk, sh.blocks, s->sc, init);
     if (d->warm_start == FE_WARM_START_LEARNED) {
         if (!s->net) return hipErrorInvalidValue;
         hipError_t e = net_warm_start(s->net, s);
@@ -643,14 +672,17 @@ static void cg_batch(fe_scene* s, const CGSetup& c, CGRun& R, int nb) {
 static hipError_t cg_start(fe_scene* s, const CGSetup& c, CGRun& R) {
     const int64_t nc = cell_count(c.g);
     hipError_t e;
-    float* zero32[4] = {s->z, s->q, s->d[1], c.mg ? s->mg[0].x2 : nullptr};
-    for (float* a : zero32)
-        if (a && (e = launch_fill_f32(s->stream, a, 0.0f, nc)) != hipSuccess) return e;
+    ZeroList L;
+    zero_add(L, s->z, 4 * nc);
+    zero_add(L, s->q, 4 * nc);
+    zero_add(L, s->d[1], 4 * nc);
+    if (c.mg) zero_add(L, s->mg[0].x2, 4 * nc);
     // the fused V-cycle skips level-0 tiles without FLUID cells: the coarse
     // right-hand side below them stays 0
-    if (c.fused && (e = launch_fill_f32(s->stream, s->mg[1].b, 0.0f, s->mg[1].cells)) != hipSuccess) return e;
-    if (c.mg && (e = launch_fill_f32(s->stream, s->mg[0].r, 0.0f, nc)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_zero_f64, grid1(nc), dim3(kBlock), 0, s->stream, s->p2, nc);
+    if (c.fused) zero_add(L, s->mg[1].b, 4 * s->mg[1].cells);
+    if (c.mg) zero_add(L, s->mg[0].r, 4 * nc);
+    zero_add(L, s->p2, 8 * nc);
+    hipLaunchKernelGGL(k_zero_many, dim3((unsigned)L.first[L.n]), dim3(kBlock), 0, s->stream, L);
     hipLaunchKernelGGL(k_cg_init, dim3((unsigned)c.sh.blocks), dim3(kBlock), 0, s->stream, c.g, s->code, s->b, s->p,
                        s->r, s->z, s->d[0], c.mgx, s->sc, c.PA, c.PB, c.nparts, c.ih2[0], c.ih2[1], c.ih2[2],
                        c.sh.chunk, c.sh.blocks);
