@@ -135,8 +135,8 @@ def parse():
 
 
 KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
-                  "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth", "mg_down": "k_mg_down",
-                  "mg_up": "k_mg_up"}
+                  "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth", "mg_down": "k_mg_down<finest>",
+                  "mg_up": "k_mg_up<finest>"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/kt -o kt -- "
              "python3 bench.py --steps 20 --warmup 5 --kernel none --no-cpu-baseline --no-solver-comparison "
@@ -147,10 +147,15 @@ PMC_CMD = ("rocprofv3 --pmc {counter} --output-format csv -d gpurun_out/pmc_{cou
 
 
 def short_name(kernel):
-    """'fe::k_cg_update(fe::Geom, ...)' -> 'k_cg_update'; template arguments dropped."""
+    """'fe::k_cg_update(fe::Geom, ...)' -> 'k_cg_update'; template arguments
+    dropped, except that the fused V-cycle kernels keep their level class
+    ('k_mg_down<finest>' for the finest level, 'k_mg_down<coarse>' below it)."""
     name = kernel.split("(")[0].replace("void ", "").strip()
-    name = name.split("::")[-1]
-    return name.split("<")[0]
+    base, _, targs = name.partition("<")
+    base = base.split("::")[-1]
+    if base in ("k_mg_down", "k_mg_up") and targs:
+        base += "<finest>" if targs.startswith("true") else "<coarse>"
+    return base
 
 
 def summarize_trace(stats_csv):
@@ -317,7 +322,9 @@ def main():
                                        traffic_for(kern) if on_profile else None,
                                        scene.mg_variant() == capi.MG_SPLIT)
     scene.kernel_timer(KERNEL_IDS["none"])
-    graphs = None if args.no_graph_comparison else graph_substeps(scene, dt, solver, min(args.steps, 50), npart)
+    graphs = None
+    if not args.no_graph_comparison:
+        graphs = graph_substeps(capi, ctx, desc, x, scene.mg_variant(), dt, solver, min(args.steps, 50), npart)
 
     solve_cmp = None if (args.solver == "jacobi" or args.no_solver_comparison) else solver_comparison(scene, capi, dt)
     learned = None
@@ -476,30 +483,48 @@ def learned_warm_start(capi, ctx, scene, n, dt, steps):
     return out
 
 
-def graph_substeps(scene, dt, solver, steps, npart):
-    """The same scene continued with its substeps as captured HIP graphs
-    (fe_scene_set_graph; bit-identical results): frames per second against the
-    eager launches of the same number of frames just before."""
-    def run(k):
+def graph_substeps(capi, ctx, desc, x, mg, dt, solver, steps, npart):
+    """Eager launches against captured HIP graphs (fe_scene_set_graph) on the
+    same work: two fresh scenes from the seeded particles, one in graph mode,
+    advance 3 untimed frames (the graph scene's first substep runs eagerly,
+    the next two capture one graph pair per parity of the particle buffers),
+    then `steps` timed frames each, one scene after the other.  The runs are
+    bit-identical (particles and pressure compared at the end), so both solve
+    the same systems with the same iteration counts."""
+    pair = []
+    for graph in (False, True):
+        sc = capi.Scene(ctx, desc)
+        sc.set_mg_variant(mg)
+        sc.upload_particles(x, np.zeros_like(x))
+        sc.set_graph(graph)
+        pair.append(sc)
+
+    def run(sc, k):
         its = []
         t0 = time.perf_counter()
         sub = 0
         for _ in range(k):
-            sub += scene.advance_frame(dt, solver)
-            its.append(scene.last_stats().iterations)
-        scene.sync()
-        return time.perf_counter() - t0, sub, float(np.median(its)) if its else 0.0
+            sub += sc.advance_frame(dt, solver)
+            its.append(sc.last_stats().iterations)
+        sc.sync()
+        return time.perf_counter() - t0, sub, its
 
-    we, se, ie = run(steps)
-    scene.set_graph(True)
-    run(2)  # one capture per parity of the particle buffers
-    wg, sg, ig = run(steps)
-    n_graph, captures, resumed = scene.graph_stats()
-    scene.set_graph(False)
-    return {"frames": steps, "eager_ms_per_step": round(we / se * 1e3, 4), "eager_iterations_median": ie,
-            "graph_ms_per_step": round(wg / sg * 1e3, 4), "graph_iterations_median": ig,
-            "graph_mparticle_substeps_per_s": round(npart * sg / wg / 1e6, 2), "graph_substeps": n_graph,
-            "captures": captures, "resumed": resumed}
+    eager, graph = pair
+    run(eager, 3)
+    run(graph, 3)
+    we, se, ie = run(eager, steps)
+    wg, sg, ig = run(graph, steps)
+    n_graph, captures, resumed = graph.graph_stats()
+    xe, ve, _ = eager.download_particles()
+    xg, vg, _ = graph.download_particles()
+    same = bool(np.array_equal(xe, xg) and np.array_equal(ve, vg) and ie == ig
+                and np.array_equal(eager.get(capi.GRID_PRESSURE), graph.get(capi.GRID_PRESSURE)))
+    for sc in pair:
+        sc.close()
+    return {"frames": steps, "after_frames": 3, "eager_ms_per_step": round(we / se * 1e3, 4),
+            "graph_ms_per_step": round(wg / sg * 1e3, 4), "iterations_median": float(np.median(ie)),
+            "graph_mparticle_substeps_per_s": round(npart * sg / wg / 1e6, 2), "bit_identical": same,
+            "graph_substeps": n_graph, "captures": captures, "resumed": resumed}
 
 
 def batch_workload(capi, ctx, frames, workers, warmup=2):
