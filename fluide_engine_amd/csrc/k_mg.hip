@@ -181,6 +181,65 @@ __device__ inline float mg_residual_cell(uint16_t cd, float xc, float bc, const 
     return (cd & kFluidBit) ? bc - code_apply_nb(cd, code_center(cd, ih2), ih2, xc, nb) : 0.0f;
 }
 
+// A level's diagonal and smoother weight for FLUID rows, one entry per lane:
+// lane n + 3 m + 9 p of every wavefront holds code_center and mg_weight of a
+// FLUID row with n, m and p non-BOUNDRY neighbours along x, y and z (the only
+// bits of a FLUID row's code that the two functions read), so the values are
+// theirs.  Each wavefront builds its own copy (no LDS, no barrier) and reads it
+// with a cross-lane permute.
+struct CodeLanes {
+    float cen, wgt;
+};
+
+__device__ inline CodeLanes code_lanes(const float ih2[3], float omega) {
+    const int L = (int)(threadIdx.x & 63) % 27;  // lanes 27..63 repeat entries, never read
+    const uint16_t cd = (uint16_t)(kFluidBit | ((L % 3) << 6) | (((L / 3) % 3) << 8) | ((L / 9) << 10));
+    return CodeLanes{code_center(cd, ih2), mg_weight(cd, ih2, omega)};
+}
+
+// Lane of a FLUID row's entry.  Every lane of the wavefront must execute the
+// permutes that read it (call outside divergent code).
+__device__ inline int code_lane(uint32_t cd) {
+    return (int)(((cd >> 6) & 3) + 3 * ((cd >> 8) & 3) + 9 * ((cd >> 10) & 3));
+}
+
+__device__ inline float lane_read(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+}
+
+// mg_weight of any row: a non-FLUID row's weight is omega (diagonal 1).
+__device__ inline float lanes_weight(uint32_t cd, const CodeLanes& t, float omega) {
+    const float w = lane_read(t.wgt, code_lane(cd));
+    return (cd & kFluidBit) ? w : omega;
+}
+
+// A x at a FLUID row whose neighbours outside FLUID hold 0: a FLUID row
+// couples to exactly its FLUID neighbours, and every vector of the V-cycle is
+// 0 outside FLUID cells (the CG start puts the solution's non-FLUID entries at
+// 0, the residual is 0 there from then on, restriction and prolongation skip
+// those cells), so each coupling term is taken without consulting the code:
+// an uncoupled neighbour adds -ih2 * 0.  The terms and their order are
+// code_apply_nb's, so the value is too.  cen = code_center of the row; nih2 =
+// -1/h^2 per axis.
+__device__ inline float mg_apply_fluid_row(float cen, float xc, const float (&nb)[6], const float nih2[3]) {
+    float acc = cen * xc;
+    acc = fmaf(nih2[0], nb[0], acc);
+    acc = fmaf(nih2[0], nb[1], acc);
+    acc = fmaf(nih2[1], nb[2], acc);
+    acc = fmaf(nih2[1], nb[3], acc);
+    acc = fmaf(nih2[2], nb[4], acc);
+    acc = fmaf(nih2[2], nb[5], acc);
+    return acc;
+}
+
+// mg_sweep_cell at a FLUID row under the same condition (w = mg_weight).  At a
+// non-FLUID cell (x and b 0) mg_sweep_cell gives 0, which the callers write
+// without reading the neighbours.
+__device__ inline float mg_sweep_fluid_row(float cen, float w, float xc, float bc, const float (&nb)[6],
+                                           const float nih2[3]) {
+    return xc + w * (bc - mg_apply_fluid_row(cen, xc, nb, nih2));
+}
+
 // ---------------------------------------------------------------------------
 // Large levels: one kernel per step
 // ---------------------------------------------------------------------------
@@ -371,9 +430,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(const float* __restrict__
 // fine tile of twice that size below it.  It stages what the tile's sweeps
 // read, with halos, in LDS and recomputes the halo cells of each sweep itself
 // instead of waiting for a neighbour tile, so a level takes two launches
-// instead of six.  Every value is the split kernels' (the same helpers in the
+// instead of six.  Every value is the split kernels' (the same terms in the
 // same order; only the source of a neighbour value differs, and a neighbour
-// outside the grid, which enters with weight 0, reads 0 here).
+// outside the grid, which enters with weight 0, reads 0 here).  The sweeps and
+// residuals take the FLUID-row forms (mg_sweep_fluid_row: diagonal and weight
+// from a per-wavefront lane table, coupling terms without the code's selects,
+// non-FLUID cells written as the 0 they are without reading their
+// neighbours): the finest level's phases are VALU-bound (16 waves of a tile
+// share one CU's four SIMDs), and these forms take about half the
+// instructions of the general ones.
 //   descent:  x1 = w b (first sweep from zero), x2 = x1 + w (b - A x1) (kept
 //             for the ascent), r = b - A x2, coarse b = R r;
 //   ascent:   y = x2 + P xc, x' = y + w (b - A y), out = x' + w (b - A x'),
@@ -421,10 +486,16 @@ __device__ inline bool fine_in(const FusedArgs& a, int i, int j, int k) {
     return i >= 0 && j >= 0 && k >= 0 && i < a.n[0] && j < a.n[1] && k < a.n[2];
 }
 
+// Linear index of an in-grid cell of a level with dims n: 32-bit arithmetic
+// (a scene holds at most 2^30 cells).
+__device__ inline uint32_t fused_index(int i, int j, int k, const int n[3]) {
+    return (uint32_t)i + (uint32_t)n[0] * ((uint32_t)j + (uint32_t)n[1] * (uint32_t)k);
+}
+
 // tfl (level 0 only): per tile, whether its fine tile holds a FLUID cell.  A
 // tile without one has every output 0 (its right-hand side and its coarse
 // cells' are 0), and those stay as the solve's start left them.
-constexpr int kFusedThreads = 1024;  // 16 waves per tile: the phases between barriers are LDS-latency bound
+constexpr int kFusedThreads = 1024;  // 16 waves per tile
 
 // sum_partials for a kFusedThreads workgroup: every workgroup adds the NV-wide
 // partials in the same fixed order (threads strided over the partials, lanes
@@ -488,22 +559,23 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
             }
         }
     }
-    const int64_t t = fused_tile(a);
-    if (t >= (int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2]) return;
+    const uint32_t t = (uint32_t)fused_tile(a);
+    if (t >= (uint32_t)(a.tiles[0] * a.tiles[1] * a.tiles[2])) return;
     if (L0 && tfl && !tfl[t]) return;
-    const int TX = (int)(t % a.tiles[0]), TY = (int)((t / a.tiles[0]) % a.tiles[1]),
-              TZ = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
+    const uint32_t tx0 = (uint32_t)a.tiles[0], txy = tx0 * (uint32_t)a.tiles[1];
+    const int TX = (int)(t % tx0), TY = (int)((t % txy) / tx0), TZ = (int)(t / txy);
     const int I0 = TX * T::rx, J0 = TY * T::ry, K0 = TZ * T::rz;
     const int fx0 = 2 * I0, fy0 = 2 * J0, fz0 = 2 * K0;
     // this thread's coarse cell (the restriction; the first T::coarse threads)
     const int tx = threadIdx.x % T::rx, ty = (threadIdx.x / T::rx) % T::ry, tz = threadIdx.x / (T::rx * T::ry);
     const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
     const bool cin = threadIdx.x < T::coarse && I < a.c[0] && J < a.c[1] && K < a.c[2];
-    const int64_t ci = (int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K);
-    // b and the row codes on the tile + 3: every load issued before any LDS store
+    const uint32_t ci = fused_index(I, J, K, a.c);
+    // b and the row codes on the tile + 3: every load issued before any LDS
+    // store (the codes in whole registers, so no load waits for another)
     constexpr int kPer = (W3::n + kFusedThreads - 1) / kFusedThreads;
     float vb[kPer];
-    uint16_t vc[kPer];
+    uint32_t vc[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = threadIdx.x + u * kFusedThreads;
@@ -513,50 +585,71 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
             const int wx = e % W3::x, wy = (e / W3::x) % W3::y, wz = e / W3::sz;
             const int i = fx0 - 3 + wx, j = fy0 - 3 + wy, k = fz0 - 3 + wz;
             if (fine_in(a, i, j, k)) {
-                const int64_t f = (int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k);
+                const uint32_t f = fused_index(i, j, k, a.n);
                 vb[u] = b[f];
                 vc[u] = code[f];
             }
         }
     }
-    const uint16_t ccd = cin ? ccode[ci] : (uint16_t)0;
+    const uint32_t ccd = cin ? (uint32_t)ccode[ci] : 0u;
+    const CodeLanes lanes = code_lanes(a.ih2, kMgOmega);
+    const float nih2[3] = {-a.ih2[0], -a.ih2[1], -a.ih2[2]};
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = threadIdx.x + u * kFusedThreads;
+        const float w = lanes_weight(vc[u], lanes, kMgOmega);  // every lane: a cross-lane read
         if (e < W3::n) {
-            sX1[e] = mg_weight(vc[u], a.ih2) * vb[u];  // the first sweep from zero
+            sX1[e] = w * vb[u];  // the first sweep from zero
             const int wx = e % W3::x, wy = (e / W3::x) % W3::y, wz = e / W3::sz;
             if (wx >= 1 && wx < W3::x - 1 && wy >= 1 && wy < W3::y - 1 && wz >= 1 && wz < W3::z - 1) {
                 const int e2 = (wx - 1) + W2::sy * (wy - 1) + W2::sz * (wz - 1);
                 sB[e2] = vb[u];
-                sC[e2] = vc[u];
+                sC[e2] = (uint16_t)vc[u];
             }
         }
     }
     __syncthreads();
     if (done) return;  // uniform: the solve has stopped
     // x2 on the tile + 2; the tile's own cells also go to global memory
-    for (int e = threadIdx.x; e < W2::n; e += kFusedThreads) {
+    for (int e0 = 0; e0 < W2::n; e0 += kFusedThreads) {
+        const int e = e0 + (int)threadIdx.x;
+        const bool on = e < W2::n;
+        const uint32_t cd = on ? sC[e] : 0u;
+        const int li = code_lane(cd);
+        const float cen = lane_read(lanes.cen, li), w = lane_read(lanes.wgt, li);
+        if (!on) continue;
         const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
         const int e3 = (wx + 1) + W3::sy * (wy + 1) + W3::sz * (wz + 1);
-        const float nb[6] = {sX1[e3 - 1],       sX1[e3 + 1],       sX1[e3 - W3::sy],
-                             sX1[e3 + W3::sy], sX1[e3 - W3::sz], sX1[e3 + W3::sz]};
-        const float x2 = mg_sweep_cell(sC[e], sX1[e3], sB[e], nb, a.ih2);
+        float x2 = 0.0f;
+        if (cd & kFluidBit) {
+            const float nb[6] = {sX1[e3 - 1],       sX1[e3 + 1],       sX1[e3 - W3::sy],
+                                 sX1[e3 + W3::sy], sX1[e3 - W3::sz], sX1[e3 + W3::sz]};
+            x2 = mg_sweep_fluid_row(cen, w, sX1[e3], sB[e], nb, nih2);
+        }
         sX2[e] = x2;
         if (wx >= 2 && wx < 2 + T::fx && wy >= 2 && wy < 2 + T::fy && wz >= 2 && wz < 2 + T::fz) {
             const int i = fx0 + wx - 2, j = fy0 + wy - 2, k = fz0 + wz - 2;
-            if (fine_in(a, i, j, k)) x2out[(int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k)] = x2;
+            if (fine_in(a, i, j, k)) x2out[fused_index(i, j, k, a.n)] = x2;
         }
     }
     __syncthreads();
     // r on the tile + 1 (into sX1, whose x1 is no longer read)
     float* sR = sX1;
-    for (int e = threadIdx.x; e < W1::n; e += kFusedThreads) {
+    for (int e0 = 0; e0 < W1::n; e0 += kFusedThreads) {
+        const int e = e0 + (int)threadIdx.x;
+        const bool on = e < W1::n;
         const int wx = e % W1::x, wy = (e / W1::x) % W1::y, wz = e / W1::sz;
         const int e2 = (wx + 1) + W2::sy * (wy + 1) + W2::sz * (wz + 1);
-        const float nb[6] = {sX2[e2 - 1],       sX2[e2 + 1],       sX2[e2 - W2::sy],
-                             sX2[e2 + W2::sy], sX2[e2 - W2::sz], sX2[e2 + W2::sz]};
-        sR[e] = mg_residual_cell(sC[e2], sX2[e2], sB[e2], nb, a.ih2);
+        const uint32_t cd = on ? sC[e2] : 0u;
+        const float cen = lane_read(lanes.cen, code_lane(cd));
+        if (!on) continue;
+        float rv = 0.0f;
+        if (cd & kFluidBit) {
+            const float nb[6] = {sX2[e2 - 1],       sX2[e2 + 1],       sX2[e2 - W2::sy],
+                                 sX2[e2 + W2::sy], sX2[e2 - W2::sz], sX2[e2 + W2::sz]};
+            rv = sB[e2] - mg_apply_fluid_row(cen, sX2[e2], nb, nih2);
+        }
+        sR[e] = rv;
     }
     __syncthreads();
     // coarse b = R r: fine index 2I-1+t sits at window coordinate 2*(I - I0) + t
@@ -600,12 +693,12 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
     __shared__ float sXp[W1::n];
     __shared__ float sXc[CWin<T>::n];
     const int done = st->done;  // read with the data, tested before the first write
-    const int64_t t = fused_tile(a);
-    const bool real = t < (int64_t)a.tiles[0] * a.tiles[1] * a.tiles[2] && !(L0 && tfl && !tfl[t]);
+    const uint32_t t = (uint32_t)fused_tile(a);
+    const bool real = t < (uint32_t)(a.tiles[0] * a.tiles[1] * a.tiles[2]) && !(L0 && tfl && !tfl[t]);
     int fx0 = 0, fy0 = 0, fz0 = 0, cx0 = 0, cy0 = 0, cz0 = 0;
     if (real) {
-        const int TX = (int)(t % a.tiles[0]), TY = (int)((t / a.tiles[0]) % a.tiles[1]),
-                  TZ = (int)(t / ((int64_t)a.tiles[0] * a.tiles[1]));
+        const uint32_t tx0 = (uint32_t)a.tiles[0], txy = tx0 * (uint32_t)a.tiles[1];
+        const int TX = (int)(t % tx0), TY = (int)((t % txy) / tx0), TZ = (int)(t / txy);
         fx0 = 2 * TX * T::rx;
         fy0 = 2 * TY * T::ry;
         fz0 = 2 * TZ * T::rz;
@@ -614,10 +707,11 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
         cz0 = TZ * T::rz - 2;
         // x2 and the row codes on the tile + 2, b on the tile + 1, the coarse
         // correction on its window: every load issued before any LDS store
+        // (the codes in whole registers, so no load waits for another)
         constexpr int kP2 = (W2::n + kFusedThreads - 1) / kFusedThreads;
         constexpr int kPc = (CWin<T>::n + kFusedThreads - 1) / kFusedThreads;
         float vx[kP2], vb[kP2];
-        uint16_t vc[kP2];
+        uint32_t vc[kP2];
         float vxc[kPc];
 #pragma unroll
         for (int u = 0; u < kP2; ++u) {
@@ -628,7 +722,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
                 const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
                 const int i = fx0 - 2 + wx, j = fy0 - 2 + wy, k = fz0 - 2 + wz;
                 if (fine_in(a, i, j, k)) {
-                    const int64_t f = (int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k);
+                    const uint32_t f = fused_index(i, j, k, a.n);
                     vx[u] = x2[f];
                     vc[u] = code[f];
                     const bool w1 = wx >= 1 && wx < W2::x - 1 && wy >= 1 && wy < W2::y - 1 && wz >= 1 && wz < W2::z - 1;
@@ -644,7 +738,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
                 const int wx = e % CWin<T>::x, wy = (e / CWin<T>::x) % CWin<T>::y, wz = e / CWin<T>::sz;
                 const int I = min(max(cx0 + wx, 0), a.c[0] - 1), J = min(max(cy0 + wy, 0), a.c[1] - 1),
                           K = min(max(cz0 + wz, 0), a.c[2] - 1);
-                vxc[u] = xc[(int64_t)I + (int64_t)a.c[0] * ((int64_t)J + (int64_t)a.c[1] * K)];
+                vxc[u] = xc[fused_index(I, J, K, a.c)];
             }
         }
 #pragma unroll
@@ -652,7 +746,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
             const int e = threadIdx.x + u * kFusedThreads;
             if (e < W2::n) {
                 sY[e] = vx[u];
-                sC[e] = vc[u];
+                sC[e] = (uint16_t)vc[u];
                 const int wx = e % W2::x, wy = (e / W2::x) % W2::y, wz = e / W2::sz;
                 if (wx >= 1 && wx < W2::x - 1 && wy >= 1 && wy < W2::y - 1 && wz >= 1 && wz < W2::z - 1)
                     sB[(wx - 1) + W1::sy * (wy - 1) + W1::sz * (wz - 1)] = vb[u];
@@ -664,6 +758,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
             if (e < CWin<T>::n) sXc[e] = vxc[u];
         }
     }
+    const CodeLanes lanes = code_lanes(a.ih2, kMgOmega);
+    const float nih2[3] = {-a.ih2[0], -a.ih2[1], -a.ih2[2]};
     __syncthreads();
     if (done) return;  // uniform: the solve has stopped
     double v[1] = {0.0};
@@ -682,30 +778,48 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
     __syncthreads();
     if (real) {
         // x' on the tile + 1
-        for (int e = threadIdx.x; e < W1::n; e += kFusedThreads) {
+        for (int e0 = 0; e0 < W1::n; e0 += kFusedThreads) {
+            const int e = e0 + (int)threadIdx.x;
+            const bool on = e < W1::n;
             const int wx = e % W1::x, wy = (e / W1::x) % W1::y, wz = e / W1::sz;
             const int e2 = (wx + 1) + W2::sy * (wy + 1) + W2::sz * (wz + 1);
-            const float nb[6] = {sY[e2 - 1],       sY[e2 + 1],       sY[e2 - W2::sy],
-                                 sY[e2 + W2::sy], sY[e2 - W2::sz], sY[e2 + W2::sz]};
-            sXp[e] = mg_sweep_cell(sC[e2], sY[e2], sB[e], nb, a.ih2);
+            const uint32_t cd = on ? sC[e2] : 0u;
+            const int li = code_lane(cd);
+            const float cen = lane_read(lanes.cen, li), w = lane_read(lanes.wgt, li);
+            if (!on) continue;
+            float xp = 0.0f;
+            if (cd & kFluidBit) {
+                const float nb[6] = {sY[e2 - 1],       sY[e2 + 1],       sY[e2 - W2::sy],
+                                     sY[e2 + W2::sy], sY[e2 - W2::sz], sY[e2 + W2::sz]};
+                xp = mg_sweep_fluid_row(cen, w, sY[e2], sB[e], nb, nih2);
+            }
+            sXp[e] = xp;
         }
     }
     __syncthreads();
     if (real) {
         // out on the tile
         constexpr int kTile = T::fx * T::fy * T::fz;
-        for (int e = threadIdx.x; e < kTile; e += kFusedThreads) {
+        for (int e0 = 0; e0 < kTile; e0 += kFusedThreads) {
+            const int e = e0 + (int)threadIdx.x;
+            const bool on = e < kTile;
             const int wx = e % T::fx, wy = (e / T::fx) % T::fy, wz = e / (T::fx * T::fy);
             const int i = fx0 + wx, j = fy0 + wy, k = fz0 + wz;
-            if (!fine_in(a, i, j, k)) continue;
             const int e1 = (wx + 1) + W1::sy * (wy + 1) + W1::sz * (wz + 1);
             const int e2 = (wx + 2) + W2::sy * (wy + 2) + W2::sz * (wz + 2);
-            const float nb[6] = {sXp[e1 - 1],       sXp[e1 + 1],       sXp[e1 - W1::sy],
-                                 sXp[e1 + W1::sy], sXp[e1 - W1::sz], sXp[e1 + W1::sz]};
-            const float bc = sB[e1];
-            const float o = mg_sweep_cell(sC[e2], sXp[e1], bc, nb, a.ih2);
-            out[(int64_t)i + (int64_t)a.n[0] * ((int64_t)j + (int64_t)a.n[1] * k)] = o;
-            if (L0) v[0] += (double)bc * (double)o;
+            const uint32_t cd = on ? sC[e2] : 0u;
+            const int li = code_lane(cd);
+            const float cen = lane_read(lanes.cen, li), w = lane_read(lanes.wgt, li);
+            if (!on || !fine_in(a, i, j, k)) continue;
+            float o = 0.0f;
+            if (cd & kFluidBit) {
+                const float nb[6] = {sXp[e1 - 1],       sXp[e1 + 1],       sXp[e1 - W1::sy],
+                                     sXp[e1 + W1::sy], sXp[e1 - W1::sz], sXp[e1 + W1::sz]};
+                const float bc = sB[e1];
+                o = mg_sweep_fluid_row(cen, w, sXp[e1], bc, nb, nih2);
+                if (L0) v[0] += (double)bc * (double)o;
+            }
+            out[fused_index(i, j, k, a.n)] = o;
         }
     }
     if (L0 && prz) {  // one partial per workgroup: waves in order, lanes by the shuffle tree
