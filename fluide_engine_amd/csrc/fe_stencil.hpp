@@ -109,6 +109,22 @@ __device__ inline float code_apply(uint16_t code, float center, I c, const Strid
 // consumes it.  A full group issues all G loads before the first use, so the
 // group costs one memory latency instead of G; a partial group at the end of
 // the span runs cell by cell.  Cells are used in ascending order either way.
+// span_groups from a thread's first cell `first` (threadIdx.x included) on.
+template <int G, typename T, typename Load, typename Use>
+__device__ inline void span_groups_from(int64_t first, int64_t c1, Load load, Use use) {
+    for (int64_t g0 = first; g0 < c1; g0 += (int64_t)G * kBlock) {
+        if (g0 + (int64_t)(G - 1) * kBlock < c1) {
+            T t[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) t[u] = load(g0 + (int64_t)u * kBlock);
+#pragma unroll
+            for (int u = 0; u < G; ++u) use(g0 + (int64_t)u * kBlock, t[u]);
+        } else {
+            for (int64_t c = g0; c < c1; c += kBlock) use(c, load(c));
+        }
+    }
+}
+
 template <int G, typename T, typename Load, typename Use>
 __device__ inline void span_groups(int64_t c0, int64_t c1, Load load, Use use) {
     for (int64_t g0 = c0 + threadIdx.x; g0 < c1; g0 += (int64_t)G * kBlock) {
@@ -220,16 +236,40 @@ __device__ inline void write_partials(double (&v)[NV], double* __restrict__ part
     }
 }
 
+// A thread's share of a partials array: entries b = first, first + step, ...
+// below n, added in that order.  The loads of U entries are issued before the
+// first add, so a share costs one memory latency per group
+// instead of one per entry; the order of the adds is unchanged.
+constexpr int kPartialGroup = 8;
+
+template <int NV, int U = kPartialGroup>
+__device__ inline void add_partials(const double* __restrict__ part, int stride, int off, int first, int step, int n,
+                                    double (&tot)[NV]) {
+    for (int b0 = first; b0 < n; b0 += U * step) {
+        double v[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + u * step;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[u][q] = b < n ? part[(int64_t)b * stride + off + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (b0 + u * step < n) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) tot[q] += v[u][q];
+            }
+        }
+    }
+}
+
 // Sum of the previous kernel's partials, identical in every block.
 template <int NV>
 __device__ inline void sum_partials(const double* __restrict__ part, int nparts, double (&tot)[NV]) {
     __shared__ double sm[NV][kBlock / 64];
 #pragma unroll
     for (int q = 0; q < NV; ++q) tot[q] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kBlock) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) tot[q] += part[(int64_t)b * NV + q];
-    }
+    add_partials<NV>(part, NV, 0, (int)threadIdx.x, kBlock, nparts, tot);
     block_reduce<NV>(tot, sm);
 }
 
@@ -238,12 +278,11 @@ __device__ inline void sum_partials(const double* __restrict__ part, int nparts,
 __device__ inline void sum_partials_two(const double* __restrict__ a, int sa, int oa, int na,
                                         const double* __restrict__ c, int sc, int oc, int nc, double (&tot)[2]) {
     __shared__ double sm[2][kBlock / 64];
-    tot[0] = tot[1] = 0.0;
-    const int n = na > nc ? na : nc;
-    for (int b = threadIdx.x; b < n; b += kBlock) {
-        if (b < na) tot[0] += a[(int64_t)b * sa + oa];
-        if (b < nc) tot[1] += c[(int64_t)b * sc + oc];
-    }
+    double ta[1] = {0.0}, tc[1] = {0.0};
+    add_partials<1>(a, sa, oa, (int)threadIdx.x, kBlock, na, ta);
+    add_partials<1>(c, sc, oc, (int)threadIdx.x, kBlock, nc, tc);
+    tot[0] = ta[0];
+    tot[1] = tc[0];
     block_reduce<2>(tot, sm);
 }
 

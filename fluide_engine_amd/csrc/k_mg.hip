@@ -23,9 +23,11 @@
 // Kernels per V-cycle on the large levels: the first sweep from zero is fused
 // into the kernel that produces the level's right-hand side (the CG update on
 // level 0, the restriction below it), then one sweep, the residual, the
-// restriction; on the way up the prolongation and two sweeps.  All levels of
-// at most 4096 cells run in one workgroup in LDS (k_mg_tail), which replaces
-// about ten latency-bound launches.
+// restriction; on the way up the prolongation and two sweeps.  The coarsest
+// level (at most 512 cells) runs in one workgroup in LDS (k_mg_tail).  The
+// tail once also took the next level up (16^3 at 128^3): one workgroup on one
+// CU then did 4096 cells' sweeps, about 15 us per V-cycle, where the fused
+// kernels' 8 workgroups take 11 us for that level's descent and ascent.
 #include <algorithm>
 
 #include "fe_stencil.hpp"
@@ -35,9 +37,9 @@ namespace fe {
 constexpr float kOmegaCoarse = 0.9f;  // coarsest-level Jacobi damping
 constexpr int kCoarseSweeps = 8;
 constexpr int kCoarseMax = 512;       // cells of the coarsest level
-constexpr int kTailMax = 4096;        // levels of at most this many cells run in k_mg_tail
-constexpr int kTailLds = 4608;        // cells of all tail levels together (3 floats + a row code each in LDS)
-constexpr int kTailThreads = 1024;
+constexpr int kTailMax = kCoarseMax;  // levels of at most this many cells run in k_mg_tail
+constexpr int kTailLds = kCoarseMax;  // cells of all tail levels together (5 floats + a row code each in LDS)
+constexpr int kTailThreads = kTailMax;  // one cell per thread
 constexpr int kMaxTail = 8;
 constexpr int kTailPer = (kTailMax + kTailThreads - 1) / kTailThreads;  // cells per thread on a tail level
 
@@ -505,10 +507,7 @@ __device__ inline void sum_partials_fused(const double* __restrict__ part, int n
     __shared__ double sm[NV][kFusedThreads / 64];
 #pragma unroll
     for (int q = 0; q < NV; ++q) tot[q] = 0.0;
-    for (int b = threadIdx.x; b < nparts; b += kFusedThreads) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) tot[q] += part[(int64_t)b * NV + q];
-    }
+    add_partials<NV, 1>(part, NV, 0, (int)threadIdx.x, kFusedThreads, nparts, tot);
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
         double x = tot[q];
@@ -540,17 +539,21 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
     __shared__ uint16_t sC[W2::n];
     __shared__ float sX2[W2::n];
     const int done = st->done;  // read with the data, tested before the first write
-    if (L0) {
-        // the V-cycle's first kernel: a relative stop pending confirmation is
-        // tested on the residual the CG update just recomputed (as k_mg_smooth)
-        if (pconf && done) return;
-        if (pconf && st->confirm) {
-            double t[2];
-            sum_partials_fused<2>(pconf, nconf, t);
-            if (stop_rule(st, st->bnorm2, t[0], t[1])) {
+    if (L0 && pconf) {
+        // the V-cycle's first kernel: after a stop it returns; a relative stop
+        // pending confirmation is tested on the residual the CG update just
+        // recomputed (as k_mg_smooth), by every workgroup (workgroup 0 records
+        // a stop).  Tested before the tile's loads: with the loads' registers
+        // live across the test the kernel needs more than 64 VGPRs, and two
+        // tiles no longer share a CU.
+        if (done) return;
+        if (st->confirm) {
+            double tt[2];
+            sum_partials_fused<2>(pconf, nconf, tt);
+            if (stop_rule(st, st->bnorm2, tt[0], tt[1])) {
                 if (blockIdx.x == 0 && threadIdx.x == 0) {
                     SolverScalars* w = const_cast<SolverScalars*>(st);
-                    w->rr = t[1];
+                    w->rr = tt[1];
                     w->it = st->it + 1;
                     w->converged = 1;
                     w->done = 1;
@@ -560,8 +563,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
         }
     }
     const uint32_t t = (uint32_t)fused_tile(a);
-    if (t >= (uint32_t)(a.tiles[0] * a.tiles[1] * a.tiles[2])) return;
-    if (L0 && tfl && !tfl[t]) return;
+    const bool real = t < (uint32_t)(a.tiles[0] * a.tiles[1] * a.tiles[2]) && !(L0 && tfl && !tfl[t]);
+    if (!real) return;
     const uint32_t tx0 = (uint32_t)a.tiles[0], txy = tx0 * (uint32_t)a.tiles[1];
     const int TX = (int)(t % tx0), TY = (int)((t % txy) / tx0), TZ = (int)(t / txy);
     const int I0 = TX * T::rx, J0 = TY * T::ry, K0 = TZ * T::rz;
@@ -910,7 +913,7 @@ __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, cons
 // Solves level T's right-hand side bT approximately with the V-cycle of the
 // tail levels; the result goes to xT.  Everything lives in LDS: five values
 // (two iterates, right-hand side, diagonal, smoother weight) and the row code
-// per cell of every tail level (at most 4608 cells, 99 KiB).
+// per cell of every tail level (at most kTailLds cells).
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const float* __restrict__ bT,
                                                           float* __restrict__ xT,
                                                           const SolverScalars* __restrict__ st) {

@@ -256,9 +256,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
                                                     const double* __restrict__ pz, int nz, double* __restrict__ pout,
                                                     int nparts, float ihx2, float ihy2, float ihz2,
                                                     const uint8_t* __restrict__ cfl, int64_t chunk, int64_t padded) {
-    if (in->done) {  // uniform: an earlier spmv stopped the solve
-        if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
-        return;
+    // The chunk's FLUID flag, then the loads of the thread's first group of
+    // cells, issued before the partial sums and the stop flag are waited for
+    // (they depend on neither; after a stop they go unused).
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
+    const bool body = !cfl || cfl[sp.lb] != 0;
+    constexpr int G = 4;
+    struct Raw {
+        uint16_t cd;
+        float zc, dc, zn[6], dn[6];
+    };
+    const int64_t g0 = sp.c0 + threadIdx.x;
+    const bool pre = body && g0 + (int64_t)(G - 1) * kBlock < sp.c1;
+    Raw raw[G];
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int64_t c = g0 + (int64_t)u * kBlock;
+            raw[u].cd = code[c];
+            raw[u].zc = z[c];
+            raw[u].dc = dold[c];
+            code_neighbours(c, st, [&](int64_t n) { return z[n]; }, raw[u].zn);
+            code_neighbours(c, st, [&](int64_t n) { return dold[n]; }, raw[u].dn);
+        }
     }
     // ---- state update from the last update's r.z and r.r (every block alike);
     // with the multigrid preconditioner r.z comes from the V-cycle's last sweep
@@ -270,6 +292,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         t[1] = u[1];
     } else {
         sum_partials<2>(pin, nparts, t);
+    }
+    if (in->done) {  // uniform: an earlier spmv stopped the solve
+        if (blockIdx.x == 0 && threadIdx.x == 0) *out = *in;
+        return;
     }
     const int it_prev = in->it;
     double beta, rr_ref;
@@ -314,14 +340,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
     }
     if (done) return;
     // ---- body
-    if (cfl && !cfl[block_span(cell_count(g), chunk, padded).lb]) {  // no FLUID cell: d, q stay 0
+    if (!body) {  // no FLUID cell: d, q stay 0
         if (threadIdx.x == 0) pout[blockIdx.x] = 0.0;
         return;
     }
-    const int64_t nc = cell_count(g);
-    const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
-    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     const float fb = (float)beta;
     auto D = [&](int64_t n) { return fb * dold[n] + z[n]; };  // AXpY(beta, d, s, &d)
     double v[1] = {0.0};
@@ -329,8 +352,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
         uint16_t cd;
         float dc, nb[6];
     };
-    span_groups<4, In>(
-        sp.c0, sp.c1,
+    auto use = [&](int64_t c, const In& t) {
+        const float qc = code_apply_nb(t.cd, code_center(t.cd, ih2), ih2, t.dc, t.nb);
+        dnew[c] = t.dc;
+        q[c] = qc;
+        v[0] += (double)t.dc * (double)qc;
+    };
+    if (pre) {  // the prefetched group: D from the raw values, as D(n) forms it
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            In t;
+            t.cd = raw[u].cd;
+            t.dc = fb * raw[u].dc + raw[u].zc;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t.nb[k] = fb * raw[u].dn[k] + raw[u].zn[k];
+            use(g0 + (int64_t)u * kBlock, t);
+        }
+    }
+    span_groups_from<G, In>(
+        pre ? g0 + (int64_t)G * kBlock : g0, sp.c1,
         [&](int64_t c) {
             In t;
             t.cd = code[c];
@@ -338,12 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(Geom g, const uint16_t* __re
             code_neighbours(c, st, D, t.nb);
             return t;
         },
-        [&](int64_t c, const In& t) {
-            const float qc = code_apply_nb(t.cd, code_center(t.cd, ih2), ih2, t.dc, t.nb);
-            dnew[c] = t.dc;
-            q[c] = qc;
-            v[0] += (double)t.dc * (double)qc;
-        });
+        use);
     write_partials<1>(v, pout);
 }
 
@@ -357,23 +392,48 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
                                                       const uint8_t* __restrict__ cfl, int64_t chunk, int64_t padded) {
     const int64_t nc = cell_count(g);
     const Span sp = block_span(nc, chunk, padded);
+    // The chunk's FLUID flag, then the loads of the thread's first group of
+    // cells, issued before the partial sum and the solver state are waited for
+    // (they depend on neither; after a stop, or when r is recomputed, they go
+    // unused).
+    const bool body = !cfl || cfl[sp.lb] != 0;
+    constexpr int G = 4;
+    struct In {
+        double x;
+        float d, q, r;
+        uint16_t cd;
+    };
+    const int64_t g0 = sp.c0 + threadIdx.x;
+    const bool pre = body && g0 + (int64_t)(G - 1) * kBlock < sp.c1;
+    In raw[G];
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int64_t c = g0 + (int64_t)u * kBlock;
+            raw[u].x = xo[c];
+            raw[u].d = d[c];
+            raw[u].q = q[c];
+            raw[u].r = r[c];
+            raw[u].cd = code[c];
+        }
+    }
+    double t[1];
+    sum_partials<1>(pin, nparts, t);
     if (sc->done) {  // uniform: keep the x rotation consistent
         for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) xn[c] = xo[c];
         return;
     }
-    double t[1];
-    sum_partials<1>(pin, nparts, t);
     const double alpha = sc->sigma / t[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->dq = t[0];
         sc->alpha = alpha;
     }
-    if (cfl && !cfl[sp.lb]) {  // no FLUID cell: x, r, z and the first sweep stay 0
+    if (!body) {  // no FLUID cell: x, r, z and the first sweep stay 0
         if (threadIdx.x == 0) pout[2 * blockIdx.x] = pout[2 * blockIdx.x + 1] = 0.0;
         return;
     }
     const bool replace = sc->replace != 0;
-    const int pre = sc->precond;
+    const int pre_kind = sc->precond;
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
     const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
@@ -383,12 +443,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
     auto finish = [&](int64_t c, uint16_t cd, float rc) {
         r[c] = rc;
         float zc = rc;
-        if (pre == 1) {
+        if (pre_kind == 1) {
             zc = rc / code_center(cd, ih2);
             z[c] = zc;
         }
         if (mgx) mgx[c] = mg_weight(cd, ih2) * rc;  // the V-cycle's first sweep from zero
-        if (pre != 2) v[0] += (double)rc * (double)zc;
+        if (pre_kind != 2) v[0] += (double)rc * (double)zc;
         v[1] += (double)rc * (double)rc;
     };
     if (replace) {  // r = b - A x recomputed in FP64
@@ -399,26 +459,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(Geom g, const uint16_t* __
             finish(c, cd, (float)((double)b[c] - code_apply64(cd, c, st, ih2d, xc, X)));
         }
     } else {
-        struct In {
-            double x;
-            float d, q, r;
-            uint16_t cd;
+        auto use = [&](int64_t c, const In& t) {
+            xn[c] = alpha * (double)t.d + t.x;
+            finish(c, t.cd, -fa * t.q + t.r);
         };
-        span_groups<4, In>(
-            sp.c0, sp.c1,
+        if (pre) {
+#pragma unroll
+            for (int u = 0; u < G; ++u) use(g0 + (int64_t)u * kBlock, raw[u]);
+        }
+        span_groups_from<G, In>(
+            pre ? g0 + (int64_t)G * kBlock : g0, sp.c1,
             [&](int64_t c) {
                 In t;
                 t.x = xo[c];
                 t.d = d[c];
                 t.q = q[c];
                 t.r = r[c];
-                t.cd = (pre == 1 || mgx) ? code[c] : (uint16_t)0;  // for z or the first sweep
+                t.cd = (pre_kind == 1 || mgx) ? code[c] : (uint16_t)0;  // for z or the first sweep
                 return t;
             },
-            [&](int64_t c, const In& t) {
-                xn[c] = alpha * (double)t.d + t.x;
-                finish(c, t.cd, -fa * t.q + t.r);
-            });
+            use);
     }
     write_partials<2>(v, pout);
 }
