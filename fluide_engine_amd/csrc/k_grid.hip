@@ -60,6 +60,13 @@ hipError_t launch_fill_f32(hipStream_t st, float* a, float v, int64_t n) {
 // buckets farther than 2r, which the distance test drops).  The scan covers the
 // cells overlapping that bucket block within 2r of the centre.
 // ---------------------------------------------------------------------------
+// Without buckets a workgroup takes a block of kSdfBX x kSdfBY x kSdfBZ cells
+// and first checks whether any particle lies in the cells its scans could
+// reach (the block grown by the scan extent): if none does, every cell of the
+// block gets the value of an empty scan, band - radius, without scanning.
+constexpr int kSdfBX = 8, kSdfBY = 8, kSdfBZ = 4;
+static_assert(kSdfBX * kSdfBY * kSdfBZ == kBlock, "one cell per thread");
+
 template <bool Buckets>
 __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict__ pos4,
                                                 const int32_t* __restrict__ start, double* __restrict__ sdf,
@@ -67,11 +74,40 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict
                                                 float slack, int64_t padded) {
 #pragma clang fp contract(off)
     const int64_t blk = xcd_remap(blockIdx.x, padded);
-    const int64_t c = blk * kBlock + threadIdx.x;
     const int64_t nc = cell_count(g);
-    if (c >= nc) return;
     int i, j, k;
-    split_index(c, g.n[0], g.n[1], i, j, k);
+    int64_t c;
+    if (Buckets) {
+        c = blk * kBlock + threadIdx.x;
+        if (c >= nc) return;
+        split_index(c, g.n[0], g.n[1], i, j, k);
+    } else {
+        const int tbx = (g.n[0] + kSdfBX - 1) / kSdfBX, tby = (g.n[1] + kSdfBY - 1) / kSdfBY,
+                  tbz = (g.n[2] + kSdfBZ - 1) / kSdfBZ;
+        if (blk >= (int64_t)tbx * tby * tbz) return;  // the whole workgroup
+        int bi, bj, bk;
+        split_index(blk, tbx, tby, bi, bj, bk);
+        const int t = (int)threadIdx.x;
+        i = bi * kSdfBX + t % kSdfBX;
+        j = bj * kSdfBY + (t / kSdfBX) % kSdfBY;
+        k = bk * kSdfBZ + t / (kSdfBX * kSdfBY);
+        const int xa = max(bi * kSdfBX - ex, 0), xb = min(bi * kSdfBX + kSdfBX - 1 + ex, g.n[0] - 1);
+        const int ya = max(bj * kSdfBY - ey, 0), yb = min(bj * kSdfBY + kSdfBY - 1 + ey, g.n[1] - 1);
+        const int za = max(bk * kSdfBZ - ez, 0), zb = min(bk * kSdfBZ + kSdfBZ - 1 + ez, g.n[2] - 1);
+        const int ylen = yb - ya + 1, rows = ylen * (zb - za + 1);
+        int any = 0;
+        for (int r = t; r < rows; r += kBlock) {
+            const int64_t row = (int64_t)g.n[0] * ((int64_t)(ya + r % ylen) + (int64_t)g.n[1] * (za + r / ylen));
+            any |= start[row + xb + 1] > start[row + xa] ? 1 : 0;
+        }
+        any = __syncthreads_or(any);
+        if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) return;
+        c = (int64_t)i + (int64_t)g.n[0] * ((int64_t)j + (int64_t)g.n[1] * k);
+        if (!any) {
+            sdf[c] = band - radius;  // minD = band, nothing found
+            return;
+        }
+    }
     // Grid3D::GridIndexToPosition on the SDF grid of origin o + h/2 (src/hybrid_simulator.cpp:13-15)
     const double gx = (g.od[0] + g.hd[0] / 2.0) + (double)i * g.hd[0];
     const double gy = (g.od[1] + g.hd[1] / 2.0) + (double)j * g.hd[1];
@@ -101,40 +137,82 @@ __global__ __launch_bounds__(kBlock) void k_sdf(Geom g, const float4* __restrict
     float best32 = __builtin_inff();
     double best = qr2;
     bool found = false;
-    for (int kk = k0; kk <= k1; ++kk) {
-        for (int jj = j0; jj <= j1; ++jj) {
-            const int64_t row = (int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk);
-            const int s0 = start[row + i0], s1 = start[row + i1 + 1];
-            auto consider = [&](const float4 p) {
-                const float fx = p.x - gx32, fy = p.y - gy32, fz = p.z - gz32;
-                const float d2f = fx * fx + fy * fy + fz * fz;
-                if (d2f > best32 + slack) return;
-                if (Buckets) {
-                    const int bx = (int)floor((double)p.x / sp), by = (int)floor((double)p.y / sp),
-                              bz = (int)floor((double)p.z / sp);
-                    if (bx < blo[0] || bx > bhi[0] || by < blo[1] || by > bhi[1] || bz < blo[2] || bz > bhi[2])
-                        return;
-                }
-                best32 = fminf(best32, d2f);
-                const double dx = (double)p.x - gx;
-                const double dy = (double)p.y - gy;
-                const double dz = (double)p.z - gz;
-                const double d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 <= qr2 && (!found || d2 < best)) {
-                    best = d2;
-                    found = true;
-                }
-            };
-            constexpr int kB = 8;  // candidate loads issued together
-            int s = s0;
-            for (; s + kB <= s1; s += kB) {
-                float4 p[kB];
+    auto consider = [&](const float4 p) {
+        const float fx = p.x - gx32, fy = p.y - gy32, fz = p.z - gz32;
+        const float d2f = fx * fx + fy * fy + fz * fz;
+        if (d2f > best32 + slack) return;
+        if (Buckets) {
+            const int bx = (int)floor((double)p.x / sp), by = (int)floor((double)p.y / sp),
+                      bz = (int)floor((double)p.z / sp);
+            if (bx < blo[0] || bx > bhi[0] || by < blo[1] || by > bhi[1] || bz < blo[2] || bz > bhi[2]) return;
+        }
+        best32 = fminf(best32, d2f);
+        const double dx = (double)p.x - gx;
+        const double dy = (double)p.y - gy;
+        const double dz = (double)p.z - gz;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 <= qr2 && (!found || d2 < best)) {
+            best = d2;
+            found = true;
+        }
+    };
+    auto scan = [&](int s0, int s1) {
+        constexpr int kB = 8;  // candidate loads issued together
+        int s = s0;
+        for (; s + kB <= s1; s += kB) {
+            float4 p[kB];
 #pragma unroll
-                for (int q = 0; q < kB; ++q) p[q] = pos4[s + q];
+            for (int q = 0; q < kB; ++q) p[q] = pos4[s + q];
 #pragma unroll
-                for (int q = 0; q < kB; ++q) consider(p[q]);
+            for (int q = 0; q < kB; ++q) consider(p[q]);
+        }
+        for (; s < s1; ++s) consider(pos4[s]);
+    };
+    if (Buckets) {
+        for (int kk = k0; kk <= k1; ++kk) {
+            for (int jj = j0; jj <= j1; ++jj) {
+                const int64_t row = (int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk);
+                scan(start[row + i0], start[row + i1 + 1]);
             }
-            for (; s < s1; ++s) consider(pos4[s]);
+        }
+    } else {
+        // The centre's own cell first; a neighbour cell is then skipped when
+        // no point of it is nearer than the best distance so far (or than the
+        // band while none is found): a particle binned in cell ii lies in
+        // [o + ii h, o + (ii + 1) h) on each axis, so its distance to the
+        // centre of cell i is at least (|ii - i| - 1/2) h there.  The bound is
+        // shrunk by 1e-9 (far beyond FP64 rounding), so a skipped cell holds no
+        // particle whose computed squared distance is below the best, and the
+        // minimum -- the result -- is the full scan's.  Inside the fluid the
+        // own cell's nearest particle (about 0.43 h away for 2 x 2 x 2 seeding)
+        // usually rules out all 26 neighbours (they are at least h/2 away).
+        // Per row of cells (fixed jj, kk) the bound grows with |ii - i|, so
+        // the cells left after skipping form one interval around i, scanned as
+        // one particle range (the centre's row: the two sides of the centre).
+        scan(start[c], start[c + 1]);
+        auto bound = [&](int d, double h) { return d == 0 ? 0.0 : ((double)d - 0.5) * h; };
+        for (int kk = k0; kk <= k1; ++kk) {
+            const double lz = bound(abs(kk - k), g.hd[2]);
+            for (int jj = j0; jj <= j1; ++jj) {
+                const double ly = bound(abs(jj - j), g.hd[1]);
+                const double lyz2 = ly * ly + lz * lz;
+                // best is qr2 until a particle is found
+                auto skip = [&](int ii) {
+                    const double lx = bound(abs(ii - i), g.hd[0]);
+                    return (lx * lx + lyz2) * (1.0 - 1e-9) > best;
+                };
+                if (skip(i)) continue;
+                int a = i0, b = i1;
+                while (a < i && skip(a)) ++a;
+                while (b > i && skip(b)) --b;
+                const int64_t row = (int64_t)g.n[0] * ((int64_t)jj + (int64_t)g.n[1] * kk);
+                if (jj == j && kk == k) {
+                    if (a < i) scan(start[row + a], start[row + i]);
+                    if (b > i) scan(start[row + i + 1], start[row + b + 1]);
+                } else {
+                    scan(start[row + a], start[row + b + 1]);
+                }
+            }
         }
     }
     double minD = band;
@@ -161,8 +239,11 @@ hipError_t launch_sdf(fe_scene* s, bool buckets) {
     const double E = 3.0 * (2.0 * D * delta + delta * delta) + 3.0 * eps * 3.0 * D * D;
     const float slack = (float)(4.0 * E);
     const int64_t nc = cell_count(g);
-    const int64_t padded = pad_blocks((nc + kBlock - 1) / kBlock);
     buckets = buckets || s->has_collider || s->sdf_buckets;
+    const int64_t nblk = buckets ? (nc + kBlock - 1) / kBlock
+                                 : (int64_t)((g.n[0] + kSdfBX - 1) / kSdfBX) * ((g.n[1] + kSdfBY - 1) / kSdfBY) *
+                                       ((g.n[2] + kSdfBZ - 1) / kSdfBZ);
+    const int64_t padded = pad_blocks(nblk);
     if (buckets)
         launch_timed(s, FE_KERNEL_SDF, 0, k_sdf<true>, dim3((unsigned)padded), dim3(kBlock), g,
                      (const float4*)s->pos4, (const int32_t*)s->cell_start, s->sdf, ext[0], ext[1], ext[2], radius,
