@@ -269,6 +269,48 @@ __device__ inline FaceStencil face_stencil(const Geom& g, int c, float px, float
     return t;
 }
 
+// The stencils of one point on the three face arrays share their axis
+// coordinates: on axis a there are only the normal one (component a: origin
+// o, size n + 1) and the transverse one (the other two components: origin
+// o + h/2, size n), so a point takes six fast_axis evaluations instead of
+// nine.  stencil_at(c) is face_stencil(c) of the same point, value for value,
+// with 32-bit offsets (a face array holds fewer than 2^31 values).
+struct PointAxes {
+    int in[3], it[3];
+    float fn[3], ft[3];
+};
+
+__device__ inline PointAxes point_axes(const Geom& g, float px, float py, float pz) {
+    const float p[3] = {px, py, pz};
+    PointAxes t;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        fast_axis(p[a], g.co[a][a], g.ih[a], g.n[a] + 1, t.in[a], t.fn[a]);
+        fast_axis(p[a], g.co[(a + 1) % 3][a], g.ih[a], g.n[a], t.it[a], t.ft[a]);
+    }
+    return t;
+}
+
+struct FaceStencil32 {
+    int r00, r10, r01, r11;  // as FaceStencil
+    float fx, fy, fz;
+};
+
+__device__ inline FaceStencil32 stencil_at(const Geom& g, int c, const PointAxes& t) {
+    const int sx = face_dim(g, c, 0), sy = face_dim(g, c, 1), sz = face_dim(g, c, 2);
+    FaceStencil32 s;
+    const int i = c == 0 ? t.in[0] : t.it[0], j = c == 1 ? t.in[1] : t.it[1], k = c == 2 ? t.in[2] : t.it[2];
+    s.fx = c == 0 ? t.fn[0] : t.ft[0];
+    s.fy = c == 1 ? t.fn[1] : t.ft[1];
+    s.fz = c == 2 ? t.fn[2] : t.ft[2];
+    const int jp1 = min(j + 1, sy - 1), kp1 = min(k + 1, sz - 1);
+    s.r00 = sx * (j + sy * k) + i;
+    s.r10 = sx * (jp1 + sy * k) + i;
+    s.r01 = sx * (j + sy * kp1) + i;
+    s.r11 = sx * (jp1 + sy * kp1) + i;
+    return s;
+}
+
 __device__ inline float2 load_pair(const float* __restrict__ p) {
     float2 v;
     __builtin_memcpy(&v, p, sizeof(v));  // 4-byte aligned 8-byte load
@@ -276,7 +318,8 @@ __device__ inline float2 load_pair(const float* __restrict__ p) {
 }
 
 // The reference's lerp nesting: x first, then y, then z.
-__device__ inline float trilerp_pairs(float2 v00, float2 v10, float2 v01, float2 v11, const FaceStencil& t) {
+template <typename S>
+__device__ inline float trilerp_pairs(float2 v00, float2 v10, float2 v01, float2 v11, const S& t) {
     return lerpf_ref(lerpf_ref(lerpf_ref(v00.x, v00.y, t.fx), lerpf_ref(v10.x, v10.y, t.fx), t.fy),
                      lerpf_ref(lerpf_ref(v01.x, v01.y, t.fx), lerpf_ref(v11.x, v11.y, t.fx), t.fy), t.fz);
 }
@@ -291,7 +334,8 @@ __device__ inline float sample_face(const float* __restrict__ a, const Geom& g, 
 // j1 = min(j+1, sy-1), the clamps of the stencil above.  A sample then takes
 // two 16-byte loads (planes k and k+1) instead of four 8-byte ones, and
 // interpolates the same values in the same order, so it is bit-identical.
-__device__ inline float trilerp_quads(float4 q0, float4 q1, const FaceStencil& t) {
+template <typename S>
+__device__ inline float trilerp_quads(float4 q0, float4 q1, const S& t) {
     return trilerp_pairs(make_float2(q0.x, q0.y), make_float2(q0.z, q0.w), make_float2(q1.x, q1.y),
                          make_float2(q1.z, q1.w), t);
 }

@@ -613,7 +613,16 @@ struct GridSrc {
         if (Quads) return sample_quad(q.a[c], g, c, px, py, pz);
         return sample_face(a.a[c], g, c, px, py, pz);
     }
-    __device__ void pairs(const FaceStencil& t, int c, float2 (&v)[4]) const {
+    // the same sample from the point's shared axis coordinates
+    __device__ float sample_at(const Geom& g, int c, const PointAxes& p) const {
+        const FaceStencil32 t = stencil_at(g, c, p);
+        if (Quads) return trilerp_quads(q.a[c][t.r00], q.a[c][t.r01], t);
+        const float* b = a.a[c];
+        return trilerp_pairs(load_pair(b + t.r00), load_pair(b + t.r10), load_pair(b + t.r01), load_pair(b + t.r11),
+                             t);
+    }
+    template <typename S>
+    __device__ void pairs(const S& t, int c, float2 (&v)[4]) const {
         if (Quads) {
             const float4 q0 = q.a[c][t.r00], q1 = q.a[c][t.r01];
             v[0] = make_float2(q0.x, q0.y);
@@ -632,8 +641,8 @@ struct GridSrc {
 
 template <bool Quads>
 __device__ inline void sample_pic_delta(const GridSrc<Quads>& ga, const GridSrc<Quads>& gs, const Geom& g, int c,
-                                        float px, float py, float pz, float& pic, float& delta) {
-    const FaceStencil t = face_stencil(g, c, px, py, pz);
+                                        const PointAxes& p, float& pic, float& delta) {
+    const FaceStencil32 t = stencil_at(g, c, p);
     float2 av[4], sv[4];
     ga.pairs(t, c, av);
     gs.pairs(t, c, sv);
@@ -683,23 +692,26 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
     if (active) {  // inactive lanes stay for the wavefront-wide key runs at the end
         float x = P.x[s], y = P.x[cap + s], z = P.x[2 * cap + s];
         float v[3] = {P.v[s], P.v[cap + s], P.v[2 * cap + s]};
+        // the grid velocity at the particle: G2P's PIC sample, and the first
+        // sample of the first midpoint step (the same point, the same grid)
+        float pic[3];
+        const PointAxes here = point_axes(g, x, y, z);
         if (do_g2p) {
-            float pic[3];
             if (METHOD == FE_PIC) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = gsrc.sample(g, c, x, y, z);
+                for (int c = 0; c < 3; ++c) v[c] = pic[c] = gsrc.sample_at(g, c, here);
             } else if (METHOD == FE_FLIP) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     float delta;
-                    sample_pic_delta(gsrc, ssrc, g, c, x, y, z, pic[c], delta);
+                    sample_pic_delta(gsrc, ssrc, g, c, here, pic[c], delta);
                     float fl = v[c] + delta;
                     if (blend > 0.0f) fl = (1.0f - blend) * fl + blend * pic[c];
                     v[c] = fl;
                 }
             } else {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) pic[c] = gsrc.sample(g, c, x, y, z);
+                for (int c = 0; c < 3; ++c) pic[c] = gsrc.sample_at(g, c, here);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = pic[c];
 #pragma unroll
@@ -724,12 +736,19 @@ __global__ __launch_bounds__(kBlock) void k_g2p_move(Geom g, Particles P, int64_
             float p0[3] = {x, y, z}, p1[3] = {x, y, z};
             for (int t = 0; t < nsub; ++t) {
                 float v0[3], mid[3], mv[3];
+                if (t == 0 && do_g2p) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v0[c] = gsrc.sample(g, c, p0[0], p0[1], p0[2]);
+                    for (int c = 0; c < 3; ++c) v0[c] = pic[c];
+                } else {
+                    const PointAxes pa = t == 0 ? here : point_axes(g, p0[0], p0[1], p0[2]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v0[c] = gsrc.sample_at(g, c, pa);
+                }
 #pragma unroll
                 for (int a = 0; a < 3; ++a) mid[a] = p0[a] + 0.5f * sdt * v0[a];
+                const PointAxes pm = point_axes(g, mid[0], mid[1], mid[2]);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) mv[c] = gsrc.sample(g, c, mid[0], mid[1], mid[2]);
+                for (int c = 0; c < 3; ++c) mv[c] = gsrc.sample_at(g, c, pm);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     p1[a] = p0[a] + sdt * mv[a];
