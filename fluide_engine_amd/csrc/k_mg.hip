@@ -1195,7 +1195,9 @@ float* mg_first_sweep_target(fe_scene* s) { return s->mg.empty() ? nullptr : s->
 // The one-workgroup tail of a V-cycle (z = M^-1 r, mg_apply below).  Once `st`
 // says the solve has stopped, every multigrid kernel still issues its first
 // loads (the stop flag is read together with them) and then returns before its
-// first write; only the CG kernels exit before loading anything.
+// first write, except the finest-level descent, which tests the flag before
+// its loads; the CG spmv and update issue their first group of cells' loads
+// and the partial-sum loads before testing it.
 static void launch_tail(fe_scene* s, SolverScalars* st) {
     std::vector<MGLevel>& lv = s->mg;
     const int L = (int)lv.size();

@@ -657,8 +657,8 @@ static hipError_t jacobi_solve(fe_scene* s, const fe_solver_desc* d) {
 // ---------------------------------------------------------------------------
 // CG / PCG, host side.  spmv k decides the stop of update k-1, so every batch
 // of launches ends with an spmv whose state slot the host reads; launches
-// after the stop exit at once and are left out of the kernel timer (tags >=
-// the stopping iteration).
+// after the stop leave before their first write (after at most one group of
+// loads) and are left out of the kernel timer (tags >= the stopping iteration).
 // ---------------------------------------------------------------------------
 struct CGSetup {
     Geom g;
