@@ -79,6 +79,125 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarsen(const uint8_t* __restrict
     cf[c] = air ? (uint8_t)FE_AIR : (fluid ? (uint8_t)FE_FLUID : (uint8_t)FE_BOUNDRY);
 }
 
+// The coarse markers of levels 1 .. m (m <= kPyr) in one launch: a
+// workgroup takes a block of 8 x 8 x 8 level-1 cells and the pyramid above
+// it (4^3, 2^3 and 1 cells of levels 2, 3, 4; a level-q cell's children are
+// level-(q-1) cells of the same block), each level from the one below in LDS
+// by k_mg_coarsen's rule -- AIR if a child is AIR, else FLUID if a child is
+// FLUID, else BOUNDRY, over the children that exist.  It also leaves the
+// fine-tile flags of the fused V-cycle's finest level (a block's level-0
+// region is two 16 x 16 x 8 tiles): whether the tile holds a FLUID cell.
+constexpr int kPyr = 4;
+constexpr int kPyrThreads = 512;
+constexpr uint8_t kNoCell = 255;  // a cell beyond its level's dims
+
+struct PyrArgs {
+    int levels;           // m, levels 1 .. m computed here
+    int n[kPyr + 1][3];   // dims of levels 0 .. m
+    uint8_t* flags[kPyr + 1];
+    uint8_t* tfl;         // tile flags of level 0 (null: not wanted)
+    int tiles[3];         // tile counts of level 0's fused tiles
+};
+
+__global__ __launch_bounds__(kPyrThreads) void k_mg_pyramid(PyrArgs a) {
+    __shared__ uint8_t L[2][512];
+    const int t = (int)threadIdx.x;
+    const int bx = (a.n[1][0] + 7) / 8, by = (a.n[1][1] + 7) / 8;
+    const int b = (int)blockIdx.x;
+    const int B[3] = {b % bx, (b / bx) % by, b / (bx * by)};
+    // level 1 from level 0, plus the tile flags
+    int fluid0 = 0;
+    {
+        const int I = 8 * B[0] + t % 8, J = 8 * B[1] + (t / 8) % 8, K = 8 * B[2] + t / 64;
+        uint8_t out = kNoCell;
+        if (I < a.n[1][0] && J < a.n[1][1] && K < a.n[1][2]) {
+            const int fx = a.n[0][0], fy = a.n[0][1], fz = a.n[0][2];
+            bool fluid = false, air = false;
+            for (int dk = 0; dk < 2; ++dk) {
+                const int k = 2 * K + dk;
+                if (k >= fz) continue;
+                for (int dj = 0; dj < 2; ++dj) {
+                    const int j = 2 * J + dj;
+                    if (j >= fy) continue;
+                    for (int di = 0; di < 2; ++di) {
+                        const int i = 2 * I + di;
+                        if (i >= fx) continue;
+                        const uint8_t f = a.flags[0][(int64_t)i + (int64_t)fx * ((int64_t)j + (int64_t)fy * k)];
+                        fluid |= f == FE_FLUID;
+                        air |= f == FE_AIR;
+                    }
+                }
+            }
+            out = air ? (uint8_t)FE_AIR : (fluid ? (uint8_t)FE_FLUID : (uint8_t)FE_BOUNDRY);
+            a.flags[1][(int64_t)I + (int64_t)a.n[1][0] * ((int64_t)J + (int64_t)a.n[1][1] * K)] = out;
+            fluid0 = fluid ? 1 : 0;
+        }
+        L[0][t] = out;
+    }
+    if (a.tfl) {
+        // tiles of 8 x 8 x 4 level-1 cells: this block's lower and upper halves in z
+        const int lower = __syncthreads_or(t < 256 ? fluid0 : 0);
+        const int upper = __syncthreads_or(t >= 256 ? fluid0 : 0);
+        if (t < 2) {
+            const int tz = 2 * B[2] + t;
+            if (B[0] < a.tiles[0] && B[1] < a.tiles[1] && tz < a.tiles[2])
+                a.tfl[(int64_t)B[0] + (int64_t)a.tiles[0] * ((int64_t)B[1] + (int64_t)a.tiles[1] * tz)] =
+                    (uint8_t)(t == 0 ? lower : upper);
+        }
+    } else {
+        __syncthreads();
+    }
+    // levels 2 .. m from the level below, in LDS
+    int side = 8;
+    for (int q = 2; q <= a.levels; ++q) {
+        const uint8_t* below = L[q & 1];
+        uint8_t* here = L[(q + 1) & 1];
+        const int cs = side / 2;  // this level's block side
+        if (t < cs * cs * cs) {
+            const int tx = t % cs, ty = (t / cs) % cs, tz = t / (cs * cs);
+            const int I = cs * B[0] + tx, J = cs * B[1] + ty, K = cs * B[2] + tz;
+            uint8_t out = kNoCell;
+            if (I < a.n[q][0] && J < a.n[q][1] && K < a.n[q][2]) {
+                bool fluid = false, air = false;
+                for (int d = 0; d < 8; ++d) {
+                    const uint8_t f = below[(2 * tx + (d & 1)) + side * ((2 * ty + ((d >> 1) & 1)) + side * (2 * tz + (d >> 2)))];
+                    fluid |= f == FE_FLUID;
+                    air |= f == FE_AIR;
+                }
+                out = air ? (uint8_t)FE_AIR : (fluid ? (uint8_t)FE_FLUID : (uint8_t)FE_BOUNDRY);
+                a.flags[q][(int64_t)I + (int64_t)a.n[q][0] * ((int64_t)J + (int64_t)a.n[q][1] * K)] = out;
+            }
+            here[t] = out;
+        }
+        __syncthreads();
+        side = cs;
+    }
+}
+
+// The row codes of levels 1 .. L-1 in one launch: thread c of the flattened
+// cell range, level by level (offsets in off[]).
+constexpr int kMaxLevels = 16;
+struct CodesArgs {
+    int levels;                  // coarse levels (1 .. levels)
+    int64_t off[kMaxLevels + 1]; // first flattened cell of each coarse level
+    int n[kMaxLevels][3];
+    const uint8_t* flags[kMaxLevels];
+    uint16_t* code[kMaxLevels];
+};
+
+__global__ __launch_bounds__(kBlock) void k_mg_codes_all(CodesArgs a) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= a.off[a.levels]) return;
+    int q = 0;
+    while (q + 1 < a.levels && g >= a.off[q + 1]) ++q;
+    const int64_t c = g - a.off[q];
+    const int nx = a.n[q][0], ny = a.n[q][1], nz = a.n[q][2];
+    int i, j, k;
+    cell_ijk(c, nx, ny, i, j, k);
+    a.code[q][c] = row_code(a.flags[q], c, (uint32_t)i, (uint32_t)j, (uint32_t)k, (uint32_t)nx, (uint32_t)ny,
+                            (uint32_t)nz);
+}
+
 __global__ __launch_bounds__(kBlock) void k_mg_codes(const uint8_t* __restrict__ flags, int nx, int ny, int nz,
                                                      uint16_t* __restrict__ code) {
     const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -840,21 +959,6 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
     }
 }
 
-// Level 0's tile flags: whether the fine tile of each coarse tile holds a FLUID cell.
-__global__ __launch_bounds__(kBlock) void k_mg_tile_fluid(const uint8_t* __restrict__ flags, int nx, int ny, int nz,
-                                                          int tx, int ty, int tz, uint8_t* __restrict__ tfl) {
-    const int64_t t = blockIdx.x;
-    const int TX = (int)(t % tx), TY = (int)((t / tx) % ty), TZ = (int)(t / ((int64_t)tx * ty));
-    int any = 0;
-    using T = TileFinest;
-    for (int e = threadIdx.x; e < T::fx * T::fy * T::fz; e += kBlock) {
-        const int i = TX * T::fx + e % T::fx, j = TY * T::fy + (e / T::fx) % T::fy, k = TZ * T::fz + e / (T::fx * T::fy);
-        if (i < nx && j < ny && k < nz && flags[(int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k)] == FE_FLUID)
-            any = 1;
-    }
-    any = __syncthreads_or(any);
-    if (threadIdx.x == 0) tfl[t] = (uint8_t)any;
-}
 
 // ---------------------------------------------------------------------------
 // Small levels: the rest of the V-cycle in one workgroup
@@ -1171,21 +1275,48 @@ void mg_free(fe_scene* s) {
 hipError_t mg_setup(fe_scene* s) {
     hipError_t e;
     if (s->mg.empty() && (e = mg_alloc(s)) != hipSuccess) return e;
-    s->mg[0].flags = s->flags;
-    s->mg[0].code = s->code;
-    for (size_t q = 1; q < s->mg.size(); ++q) {
-        const MGLevel& f = s->mg[q - 1];
-        MGLevel& c = s->mg[q];
-        hipLaunchKernelGGL(k_mg_coarsen, grid1(c.cells), dim3(kBlock), 0, s->stream, (const uint8_t*)f.flags, f.n[0],
-                           f.n[1], f.n[2], c.flags, c.n[0], c.n[1], c.n[2]);
-        hipLaunchKernelGGL(k_mg_codes, grid1(c.cells), dim3(kBlock), 0, s->stream, (const uint8_t*)c.flags, c.n[0],
-                           c.n[1], c.n[2], c.code);
+    std::vector<MGLevel>& lv = s->mg;
+    lv[0].flags = s->flags;
+    lv[0].code = s->code;
+    const int L = (int)lv.size();
+    // markers of levels 1 .. m and the finest tiles' flags in one launch,
+    // deeper levels (grids above 128^3) one launch each
+    const int m = std::min(L - 1, kPyr);
+    PyrArgs pa{};
+    pa.levels = m;
+    for (int q = 0; q <= m; ++q) {
+        for (int a = 0; a < 3; ++a) pa.n[q][a] = lv[q].n[a];
+        pa.flags[q] = lv[q].flags;
     }
     if (s->tune.mg == FE_MG_FUSED) {
-        const FusedArgs fa = fused_args<TileFinest>(s->mg[0], s->mg[1]);
-        const int64_t nt = (int64_t)fa.tiles[0] * fa.tiles[1] * fa.tiles[2];
-        hipLaunchKernelGGL(k_mg_tile_fluid, dim3((unsigned)nt), dim3(kBlock), 0, s->stream, (const uint8_t*)s->flags,
-                           fa.n[0], fa.n[1], fa.n[2], fa.tiles[0], fa.tiles[1], fa.tiles[2], s->mg_tfl);
+        const FusedArgs fa = fused_args<TileFinest>(lv[0], lv[1]);
+        pa.tfl = s->mg_tfl;
+        for (int a = 0; a < 3; ++a) pa.tiles[a] = fa.tiles[a];
+    }
+    const unsigned nb = (unsigned)(((lv[1].n[0] + 7) / 8) * ((lv[1].n[1] + 7) / 8) * ((lv[1].n[2] + 7) / 8));
+    hipLaunchKernelGGL(k_mg_pyramid, dim3(nb), dim3(kPyrThreads), 0, s->stream, pa);
+    for (int q = m + 1; q < L; ++q) {
+        const MGLevel& f = lv[q - 1];
+        MGLevel& c = lv[q];
+        hipLaunchKernelGGL(k_mg_coarsen, grid1(c.cells), dim3(kBlock), 0, s->stream, (const uint8_t*)f.flags, f.n[0],
+                           f.n[1], f.n[2], c.flags, c.n[0], c.n[1], c.n[2]);
+    }
+    // row codes of every coarse level in one launch
+    if (L - 1 <= kMaxLevels) {
+        CodesArgs ca{};
+        ca.levels = L - 1;
+        ca.off[0] = 0;
+        for (int q = 1; q < L; ++q) {
+            for (int a = 0; a < 3; ++a) ca.n[q - 1][a] = lv[q].n[a];
+            ca.flags[q - 1] = lv[q].flags;
+            ca.code[q - 1] = lv[q].code;
+            ca.off[q] = ca.off[q - 1] + lv[q].cells;
+        }
+        hipLaunchKernelGGL(k_mg_codes_all, grid1(ca.off[L - 1]), dim3(kBlock), 0, s->stream, ca);
+    } else {
+        for (int q = 1; q < L; ++q)
+            hipLaunchKernelGGL(k_mg_codes, grid1(lv[q].cells), dim3(kBlock), 0, s->stream, (const uint8_t*)lv[q].flags,
+                               lv[q].n[0], lv[q].n[1], lv[q].n[2], lv[q].code);
     }
     return hipGetLastError();
 }
