@@ -77,8 +77,11 @@ def kernel_bytes_per_launch(kernel, solver, n, npart, mg_split=False):
 def kernel_impl_bytes_per_launch(kernel, n, npart):
     """What this implementation's kernel reads and writes by design, where that
     differs from the algorithmic bytes (P2G: the per-particle stencil records)."""
-    if kernel == "p2g":  # float4 stencil record (16) + v_c (4) per particle, bin start (4) per cell,
-        return 20 * npart + 4 * n ** 3 + 9 * (n + 1) * n * n  # u + marker + FLIP snapshot (9) per face
+    if kernel == "p2g":  # the scatter reads a float4 stencil record (16) + v_c (4) per candidate: a particle is a
+        # candidate of about 900 / 512 face blocks (10 x 10 x 9 candidate cells per 8 x 8 x 8 faces); two bin
+        # starts (8) per row of candidate cells (100 rows per 512 faces); u + marker + FLIP snapshot (9) per face
+        faces = (n + 1) * n * n
+        return int(20 * npart * 900 / 512 + 8 * 100 * faces / 512 + 9 * faces)
     return None
 
 
@@ -134,7 +137,7 @@ def parse():
     return ap.parse_args()
 
 
-KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g",
+KERNEL_SYMBOLS = {"cg_update": "k_cg_update", "cg_spmv": "k_cg_spmv", "jacobi": "k_jacobi", "p2g": "k_p2g_scatter",
                   "g2p_move": "k_g2p_move", "sdf": "k_sdf", "mg_smooth": "k_mg_smooth", "mg_down": "k_mg_down<finest>",
                   "mg_up": "k_mg_up<finest>"}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "flip128_hbm_traffic.json")
@@ -566,15 +569,16 @@ def batch_workload(capi, ctx, frames, workers, warmup=2):
                 "wall_s": round(wall, 3)}
 
     out = {"scenes": len(specs), "particles": npart, "frames": frames, "workers": workers}
-    out.update(timed())  # default kernel variants (P2G tiles, G2P quads)
+    out.update(timed())  # default kernel variants (P2G scatter, G2P quads)
     # the autotuner picks per scene between bit-identical kernel variants
     t0 = time.perf_counter()
     picks = {}
     for (sc, _), (n, _, _) in zip(scenes, specs):
         (p2g, g2p), _ = sc.autotune(reps=3)
         key = f"{n}^3"
-        picks.setdefault(key, {"p2g_faces": 0, "g2p_pairs": 0, "scenes": 0})
+        picks.setdefault(key, {"p2g_tiles": 0, "p2g_faces": 0, "g2p_pairs": 0, "scenes": 0})
         picks[key]["scenes"] += 1
+        picks[key]["p2g_tiles"] += int(p2g == capi.P2G_TILES)
         picks[key]["p2g_faces"] += int(p2g == capi.P2G_FACES)
         picks[key]["g2p_pairs"] += int(g2p == capi.G2P_PAIRS)
     out["autotune_s"] = round(time.perf_counter() - t0, 3)

@@ -21,7 +21,7 @@ SDF_SEARCH_CELLS, SDF_SEARCH_BUCKETS = 0, 1
 WARM_START_ZERO, WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1, 2
 NET_SCALE_RESIDUAL, NET_SCALE_ENERGY = 0, 1
 NET_LOSS_MSE, NET_LOSS_ENERGY = 0, 1
-P2G_TILES, P2G_FACES = 0, 1
+P2G_TILES, P2G_FACES, P2G_SCATTER = 0, 1, 2
 G2P_QUADS, G2P_PAIRS = 0, 1
 MG_FUSED, MG_SPLIT = 0, 1
 SURFACE_REFERENCE, SURFACE_CLOSED = 0, 1
@@ -64,7 +64,7 @@ class Tuning(C.Structure):
 
 
 class TuningTimes(C.Structure):
-    _fields_ = [("p2g_ms", C.c_double * 2), ("g2p_ms", C.c_double * 2)]
+    _fields_ = [("p2g_ms", C.c_double * 3), ("g2p_ms", C.c_double * 2)]
 
 
 class SolveStats(C.Structure):
@@ -532,7 +532,7 @@ class Scene:
 
     def autotune(self, reps=5):
         """Times the kernel variants on the current particles and keeps the
-        fastest; returns ((p2g, g2p), {"p2g_ms": [tiles, faces], "g2p_ms": [quads, pairs]})."""
+        fastest; returns ((p2g, g2p), {"p2g_ms": [tiles, faces, scatter], "g2p_ms": [quads, pairs]})."""
         t, tm = Tuning(), TuningTimes()
         self.ctx.check(self.L.fe_scene_autotune(self._h, int(reps), C.byref(t), C.byref(tm)), "fe_scene_autotune")
         return (t.p2g, t.g2p), {"p2g_ms": list(tm.p2g_ms), "g2p_ms": list(tm.g2p_ms)}

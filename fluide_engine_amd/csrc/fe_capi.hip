@@ -656,6 +656,7 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
     chk(dalloc(&s->cell_start, nc + 1));
     chk(dalloc(&s->cursor, nc));
     chk(dalloc(&s->scan_sums, (nc + 1023) / 1024 + 1));
+    chk(dalloc(&s->p2g_bound, 3 * kP2GBoundSlots));
     for (int i = 0; i <= kMaxStages; ++i) chk(hipEventCreate(&s->ev[i]));
     if (e != hipSuccess) {
         int rc = hip_fail(ctx, e, "scene allocation");
@@ -736,6 +737,7 @@ int fe_scene_destroy(fe_scene* s) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
     }
     F(s->pos4); F(s->st4); F(s->key); F(s->order); F(s->cell_count); F(s->cell_start); F(s->cursor); F(s->scan_sums);
+    F(s->p2g_bound);
     for (int i = 0; i <= kMaxStages; ++i)
         if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);
     for (auto e : s->kt_ev)
@@ -1198,7 +1200,7 @@ int fe_scene_get_tuning(const fe_scene* s, fe_tuning* out) {
 
 int fe_scene_set_tuning(fe_scene* s, const fe_tuning* t) {
     if (!s || !t) return FE_E_INVALID;
-    if (t->p2g < FE_P2G_TILES || t->p2g > FE_P2G_FACES || t->g2p < FE_G2P_QUADS || t->g2p > FE_G2P_PAIRS ||
+    if (t->p2g < FE_P2G_TILES || t->p2g > FE_P2G_SCATTER || t->g2p < FE_G2P_QUADS || t->g2p > FE_G2P_PAIRS ||
         t->mg < FE_MG_FUSED || t->mg > FE_MG_SPLIT)
         return set_err(s->ctx, FE_E_INVALID, "unknown kernel variant");
     s->tune = *t;

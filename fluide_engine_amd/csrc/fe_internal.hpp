@@ -18,6 +18,7 @@ namespace fe {
 constexpr int kBlock = 256;        // threads per workgroup for element-wise kernels (4 waves of 64)
 constexpr int kXcds = 8;           // MI355X: workgroups are dealt round-robin to 8 XCDs, one L2 each
 constexpr int kMaxStages = 16;
+constexpr int kP2GVariants = 3;       // FE_P2G_TILES, FE_P2G_FACES, FE_P2G_SCATTER
 
 // ---------------------------------------------------------------------------
 // Geometry of a MAC grid, in the form the kernels need.
@@ -356,6 +357,11 @@ __device__ inline int64_t xcd_remap(int64_t bid, int64_t padded) {
 
 inline int64_t pad_blocks(int64_t nblocks) { return (nblocks + kXcds - 1) / kXcds * kXcds; }
 
+// Slots per component of P2G's value bound: the sort's workgroups raise slot
+// (block mod 64) with atomicMax and P2G takes the maximum over the slots, so
+// no one address takes every workgroup's atomic.
+constexpr int kP2GBoundSlots = 64;
+
 // ---------------------------------------------------------------------------
 // Device-resident solver scalars (FP64).  Reductions leave per-block partials;
 // the next kernel's blocks each sum them in the same fixed order (so every
@@ -492,7 +498,7 @@ struct fe_scene {
     hipStream_t stream = nullptr;
     int depth = 3;            // ceil(maxCfl): extrapolation / BC depth
     int move_substeps = 3;    // (unsigned)max(maxCfl, 1)
-    fe_tuning tune{FE_P2G_TILES, FE_G2P_QUADS, FE_MG_FUSED};  // kernel variants
+    fe_tuning tune{FE_P2G_SCATTER, FE_G2P_QUADS, FE_MG_FUSED};  // kernel variants
 
     // grid (HBM)
     float* u[3] = {nullptr, nullptr, nullptr};       // face velocities
@@ -531,6 +537,7 @@ struct fe_scene {
     int32_t* cursor = nullptr;       // n^3
     int32_t* order = nullptr;        // sorted slot -> old index
     int32_t* scan_sums = nullptr;
+    uint32_t* p2g_bound = nullptr;   // 3 x kP2GBoundSlots: FP32 bits of P2G's |value| bound (set by the sort)
     bool sorted = false;
     bool keys_ready = false;  // bin keys + histogram already produced by the advection kernel
 

@@ -144,9 +144,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t* __restrict
 
 // Slots in the bins: the head of each run reserves the run's slots with one
 // atomic and the run's lanes take them in lane order.
+// Workgroup 0 also clears P2G's value-bound slots, which k_gather raises.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key, int64_t np,
                                                     const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
-                                                    int32_t* __restrict__ order) {
+                                                    int32_t* __restrict__ order, uint32_t* __restrict__ bound) {
+    static_assert(3 * kP2GBoundSlots <= kBlock, "one workgroup clears the bound slots");
+    if (blockIdx.x == 0 && threadIdx.x < 3 * kP2GBoundSlots) bound[threadIdx.x] = 0u;
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = p < np;
     const uint32_t k = active ? key[p] : 0u;
@@ -269,65 +272,234 @@ __global__ __launch_bounds__(kBlock) void k_sort_big_bins(const int32_t* __restr
 // scans lies in a cell within 1 of F on each axis and its clamped base is
 // within 2 of its cell, so F - base is in [-3, 3] and is 0 or 1 exactly when
 // (F - base) mod 256 is.
-__device__ inline float4 stencil_record(const Geom& g, int c, const float p[3]) {
-    int i[3];
+__device__ inline float4 stencil_record(const Geom& g, int c, const float p[3], int base[3]) {
     float f[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) stencil_axis(g, c, a, p[a], stencil_size(g, c, a), i[a], f[a]);
-    const uint32_t b = ((uint32_t)i[0] & 255u) | (((uint32_t)i[1] & 255u) << 8) | (((uint32_t)i[2] & 255u) << 16);
+    for (int a = 0; a < 3; ++a) stencil_axis(g, c, a, p[a], stencil_size(g, c, a), base[a], f[a]);
+    const uint32_t b =
+        ((uint32_t)base[0] & 255u) | (((uint32_t)base[1] & 255u) << 8) | (((uint32_t)base[2] & 255u) << 16);
     return make_float4(f[0], f[1], f[2], __uint_as_float(b));
 }
 
+// It also raises P2G's bound on the values it transfers (the fixed-point
+// scale below): per component c the largest |v_c| and, for APIC, |v_c| plus
+// sum_a |C_ca| times the larger distance from the (clamped) particle to the
+// two face planes of its stencil on axis a, which bounds every affine value
+// v_c + C_c . (x_face - x_p) the particle hands to a face.  Workgroup maxima
+// go to slot (block mod kP2GBoundSlots) with atomicMax on the FP32 bits
+// (monotonic for non-negative floats; NaN and infinity sort above every
+// finite bound, so a non-finite velocity shows as a non-finite bound).
 __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __restrict__ order, int64_t np,
                                                    int64_t cap, Particles src, Particles dst, int with_c,
-                                                   float4* __restrict__ pos4, float4* __restrict__ st4) {
-    int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (s >= np) return;
-    const int64_t o = order[s];
-    float p[3];
+                                                   float4* __restrict__ pos4, float4* __restrict__ st4,
+                                                   uint32_t* __restrict__ bound) {
+    const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t bb[3] = {0u, 0u, 0u};
+    if (s < np) {
+        const int64_t o = order[s];
+        float p[3], v[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        p[a] = src.x[a * cap + o];
-        dst.x[a * cap + s] = p[a];
-        dst.v[a * cap + s] = src.v[a * cap + o];
+        for (int a = 0; a < 3; ++a) {
+            p[a] = src.x[a * cap + o];
+            v[a] = src.v[a * cap + o];
+            dst.x[a * cap + s] = p[a];
+            dst.v[a * cap + s] = v[a];
+        }
+        pos4[s] = make_float4(p[0], p[1], p[2], 0.0f);
+        int base[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            st4[c * cap + s] = stencil_record(g, c, p, base[c]);
+            bb[c] = __float_as_uint(fabsf(v[c]));
+        }
+        if (with_c) {
+            float cm[9];
+#pragma unroll
+            for (int a = 0; a < 9; ++a) {
+                cm[a] = src.C[a * cap + o];
+                dst.C[a * cap + s] = cm[a];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float t = fabsf(v[c]);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float pc = p[a];
+                    if (a != c) {
+                        const float clo = g.o[a] + 0.5f * g.h[a], chi = g.o[a] + g.L[a] - 0.5f * g.h[a];
+                        pc = pc < clo ? clo : (pc > chi ? chi : pc);
+                    }
+                    const float g0 = g.co[c][a] + g.h[a] * (float)base[c][a];
+                    const float g1 = g.co[c][a] + g.h[a] * (float)(base[c][a] + 1);
+                    t += fabsf(cm[3 * c + a]) * fmaxf(fabsf(g0 - pc), fabsf(g1 - pc));
+                }
+                bb[c] = __float_as_uint(t);
+            }
+        }
+        dst.pid[s] = src.pid[o];
     }
-    pos4[s] = make_float4(p[0], p[1], p[2], 0.0f);
+    __shared__ uint32_t wmax[kBlock / 64][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) st4[c * cap + s] = stencil_record(g, c, p);
-    if (with_c) {
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll
-        for (int a = 0; a < 9; ++a) dst.C[a * cap + s] = src.C[a * cap + o];
+        for (int off = 32; off > 0; off >>= 1) bb[c] = max(bb[c], (uint32_t)__shfl_xor((int)bb[c], off, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6][c] = bb[c];
     }
-    dst.pid[s] = src.pid[o];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t m = 0u;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) m = max(m, wmax[w][threadIdx.x]);
+        if (m) atomicMax(&bound[threadIdx.x * kP2GBoundSlots + (blockIdx.x % kP2GBoundSlots)], m);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// P2G: TransferParticles2Grid as a deterministic gather.
-// PIC: src/pic_simulator.cpp:356-437; FLIP: src/flip_simulator.cpp:25-119;
-// APIC: src/apic_simulator.cpp:15-118.
-// The particles whose stencil can touch face F lie in cells [F-1, F+1] on the
-// transverse axes and [F-1, F] on the normal axis (plus cell n-1 for face n-2
-// when PIC/APIC clamp the normal axis at n-2).  Faces are taken in 2x2x2
-// tiles; a tile's candidate cells are the union of its faces' (4 x 4 x 3
-// cells, against 3 x 3 x 2 per face), so each candidate is read once for 8
-// faces: 6 cells read per face instead of 18.  Four lanes share a tile, lane q
-// scanning the candidate cells of one z plane (one y row when z is the normal
-// axis); their partial sums are added in a fixed order (q0 + q1) + (q2 + q3),
-// so the result does not depend on timing.  Cells are x-fastest and particles sorted by cell, so each (j, k)
-// row of candidate cells is one contiguous particle range.  For every
-// candidate the reference's own stencil (GetCooridnatesAndWeights with the
-// scheme's stencil size, precomputed per particle by k_gather) decides: the
-// particle contributes to a face iff the face is one of its 8 nodes; the face
-// is marked even for a zero weight.
+// P2G: TransferParticles2Grid, u_F = sum_p ve_pF w_pF / sum_p w_pF over the
+// particles p whose trilinear stencil has face F as a node (PIC:
+// src/pic_simulator.cpp:356-437; FLIP: src/flip_simulator.cpp:25-119; APIC:
+// src/apic_simulator.cpp:15-118; ve_pF is v_p, or v_p + C_p . (x_F - x_p)
+// for APIC).
+//
+// The two sums are formed in 64-bit fixed point.  Every term is rounded once
+// to an integer and the integers are added; integer addition is exact and
+// associative, so a face's sums do not depend on the order its terms arrive
+// in.  That is what lets the scatter variant add terms with LDS atomics and
+// still give the bits of the gather variants, run after run.  Scales: a term
+// of the weight sum is w 2^H and one of the value sum ve w 2^(H-E), where
+// |ve| < 2^E over all particles (k_gather's bound; it is exact up to FP32
+// rounding) and H = min(61 - ceil(log2(np + 1)), 49), so every term is below
+// 2^(H+1) <= 2^50 in magnitude and a face's sum over at most np terms stays
+// below 2^62.  A term is the exact FP64 product of the FP32 ve and w, scaled
+// by a power of two (exact) and rounded to an integer by adding 1.5 * 2^52
+// (2^52 for the non-negative weight terms) in one fma: the integer is the
+// low mantissa bits.  The quotient is formed in FP64 and rounded to FP32.
+// Each term's rounding is 2^-H of the bound (2^-39 at 4M particles), far
+// below the FP32 rounding of ve and w themselves.  A non-finite velocity
+// (bound not finite) makes every face that has weight NaN.
+// ---------------------------------------------------------------------------
+struct P2GFixed {
+    double sv;    // 2^(H-E): value-term scale
+    double sw;    // 2^H: weight-term scale
+    double back;  // 2^E: (value sum / weight sum) * back is the face value
+    bool bad;     // the bound is not finite
+};
+
+// The bound of component c: the largest of its kP2GBoundSlots slots, read by
+// every lane of the wavefront (call before any lane leaves the kernel).
+__device__ inline P2GFixed p2g_fixed(const uint32_t* __restrict__ bound, int c, int head) {
+    static_assert(kP2GBoundSlots == 64, "one slot per lane");
+    uint32_t m = bound[c * kP2GBoundSlots + (threadIdx.x & 63)];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    P2GFixed fx;
+    fx.bad = m >= 0x7f800000u;
+    int e = 0;
+    if (m && !fx.bad) (void)frexpf(__uint_as_float(m), &e);  // bound < 2^e
+    fx.sv = ldexp(1.0, head - e);
+    fx.sw = ldexp(1.0, head);
+    fx.back = ldexp(1.0, e);
+    return fx;
+}
+
+// H of the scales for np particles (host side).
+inline int p2g_head(int64_t np) {
+    int c = 0;
+    while (c < 62 && (int64_t(1) << c) < np + 1) ++c;
+    return std::min(61 - c, 49);
+}
+
+// Value term round(ve w 2^(H-E)) from vs = ve * 2^(H-E) (exact in FP64).
+__device__ inline int64_t p2g_value_term(double vs, float w) {
+    const double r = fma(vs, (double)w, 6755399441055744.0);  // 1.5 * 2^52
+    return (int64_t)((uint64_t)__double_as_longlong(r) - 0x4338000000000000ull);
+}
+
+// Weight term round(w 2^H), w >= 0.
+__device__ inline int64_t p2g_weight_term(float w, double sw) {
+    const double r = fma((double)w, sw, 4503599627370496.0);  // 2^52
+    return (int64_t)((uint64_t)__double_as_longlong(r) - 0x4330000000000000ull);
+}
+
+__device__ inline float p2g_quotient(int64_t num, int64_t den, const P2GFixed& fx) {
+    if (den <= 0) return 0.0f;
+    if (fx.bad) return __int_as_float(0x7fc00000);
+    return (float)((double)num / (double)den * fx.back);
+}
+
+// APIC's value for the face at gp: val + C_c . (gp - pc), pc the particle
+// position clamped on the transverse axes (apic_simulator.cpp:54-63).
+__device__ inline float apic_value(float val, float c0, float c1, float c2, float gx, float gy, float gz,
+                                   const float pc[3]) {
+#pragma clang fp contract(off)
+    float t = c0 * (gx - pc[0]);
+    t = fmaf(c1, gy - pc[1], t);
+    t = fmaf(c2, gz - pc[2], t);
+    return val + t;
+}
+
+// APIC: the particle position clamped to [o + h/2, o + L - h/2] on the axes
+// transverse to component C.
+template <int C>
+__device__ inline void apic_clamped(const Geom& g, float4 pp, float pc[3]) {
+#pragma clang fp contract(off)
+    pc[0] = pp.x;
+    pc[1] = pp.y;
+    pc[2] = pp.z;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a == C) continue;
+        const float lo = g.o[a] + 0.5f * g.h[a], hi = g.o[a] + g.L[a] - 0.5f * g.h[a];
+        pc[a] = pc[a] < lo ? lo : (pc[a] > hi ? hi : pc[a]);
+    }
+}
+
+// Position of face index F on axis a of component C (GetXPos/GetYPos/GetZPos,
+// face_centered_grid3d.cpp:303-316); rounded as a product and a sum in every
+// variant (no contraction), so the variants' APIC values agree bit for bit.
+__device__ inline float face_pos(const Geom& g, int C, int a, int F) {
+#pragma clang fp contract(off)
+    return g.co[C][a] + g.h[a] * (float)F;
+}
+
+// Candidate cells of the faces [F0, F1] of component C on axis a: a face's
+// contributors lie in cells [F-1, F+1] on the transverse axes and [F-1, F]
+// on the normal axis, plus cell n-1 for face n-2 when PIC/APIC clamp the
+// normal axis's stencil at n-2 (their stencil size is the cell count).
+template <int METHOD>
+__device__ inline void candidate_cells(const Geom& g, int C, int a, int F0, int F1, int& lo, int& hi) {
+    lo = max(F0 - 1, 0);
+    if (a == C) {
+        hi = F1;
+        if (METHOD != FE_FLIP && F1 >= g.n[a] - 2) hi = g.n[a] - 1;
+    } else {
+        hi = F1 + 1;
+    }
+    hi = min(hi, g.n[a] - 1);
+}
+
+// ---------------------------------------------------------------------------
+// P2G gather over 2x2x2 tiles of faces from the sorted cell bins.  A tile's
+// candidate cells are the union of its faces' (4 x 4 x 3 cells, against
+// 3 x 3 x 2 per face), so each candidate is read once for 8 faces.  Four
+// lanes share a tile, lane q scanning the candidate cells of one z plane (one
+// y row when z is the normal axis); cells are x-fastest and particles sorted
+// by cell, so each (j, k) row of candidate cells is one contiguous particle
+// range.  For every candidate the reference's own stencil (GetCooridnatesAndWeights
+// with the scheme's stencil size, precomputed per particle by k_gather)
+// decides: the particle contributes to a face iff the face is one of its 8
+// nodes; the face is marked even for a zero weight.
 // ---------------------------------------------------------------------------
 constexpr int kP2GLanes = 4;  // lanes per tile (one candidate plane each)
 
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
                                                 const float4* __restrict__ pos4, const float4* __restrict__ st4,
-                                                int64_t cap, const int32_t* __restrict__ start, float* __restrict__ out,
+                                                int64_t cap, const int32_t* __restrict__ start,
+                                                const uint32_t* __restrict__ bound, int head, float* __restrict__ out,
                                                 uint8_t* __restrict__ mark, float* __restrict__ snap,
                                                 int64_t padded) {
+    const P2GFixed fx = p2g_fixed(bound, C, head);
     const int64_t blk = xcd_remap(blockIdx.x, padded);
     const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
     const int tx = (sx + 1) >> 1, ty = (sy + 1) >> 1, tz = (sz + 1) >> 1;
@@ -341,38 +513,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
     const int F0[3] = {2 * ti, 2 * tj, 2 * tk};
     int lo[3], hi[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = max(F0[a] - 1, 0);
-        if (a == C) {
-            hi[a] = F0[a] + 1;
-            if (METHOD != FE_FLIP && F0[a] + 1 == g.n[a] - 2) hi[a] = F0[a] + 2;
-        } else {
-            hi[a] = F0[a] + 2;
-        }
-        hi[a] = min(hi[a], g.n[a] - 1);
-    }
-    // APIC: face positions GetXPos/GetYPos/GetZPos (face_centered_grid3d.cpp:303-316)
-    // of the tile's first face and the clamp box [o + h/2, o + L - h/2] of the
-    // transverse axes (apic_simulator.cpp:54-63).
-    float gp0[3], gp1[3], clo[3], chi[3];
-    if (METHOD == FE_APIC) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            gp0[a] = g.co[C][a] + g.h[a] * (float)F0[a];
-            gp1[a] = g.co[C][a] + g.h[a] * (float)(F0[a] + 1);
-            clo[a] = g.o[a] + 0.5f * g.h[a];
-            chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
-        }
-    }
+    for (int a = 0; a < 3; ++a) candidate_cells<METHOD>(g, C, a, F0[a], F0[a] + 1, lo[a], hi[a]);
     // Face e = ex + 2 ey + 4 ez of the tile is (F0x + ex, F0y + ey, F0z + ez).
-    float num[8], den[8];
+    int64_t num[8], den[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) num[e] = den[e] = 0.0f;
+    for (int e = 0; e < 8; ++e) num[e] = den[e] = 0;
     uint32_t marked = 0;
     // One candidate, from its stencil record (fx, fy, fz, packed low bytes of
     // the base indices): face F takes it iff F - base is 0 or 1 on every axis,
-    // with weight factor 1 - f at offset 0 and f at offset 1 (0 when rejected,
-    // so a rejected candidate adds exact zeros).
+    // with weight factor 1 - f at offset 0 and f at offset 1.
     auto consider = [&](int s, float4 r, float val) {
         const uint32_t b = __float_as_uint(r.w);
         const float fr[3] = {r.x, r.y, r.z};
@@ -383,8 +532,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
             const uint32_t d0 = ((uint32_t)F0[a] - (b >> (8 * a))) & 255u;
             const uint32_t d1 = (d0 + 1u) & 255u;
             const bool a0 = d0 <= 1u, a1 = d1 <= 1u;
-            w0[a] = a0 ? (d0 ? fr[a] : 1.0f - fr[a]) : 0.0f;
-            w1[a] = a1 ? (d1 ? fr[a] : 1.0f - fr[a]) : 0.0f;
+            w0[a] = d0 ? fr[a] : 1.0f - fr[a];
+            w1[a] = d1 ? fr[a] : 1.0f - fr[a];
             ok |= (a0 ? 1u : 0u) << (2 * a);
             ok |= (a1 ? 1u : 0u) << (2 * a + 1);
         }
@@ -393,35 +542,30 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         const uint32_t my = ((ok & 4u) ? 0x33u : 0u) | ((ok & 8u) ? 0xCCu : 0u);
         const uint32_t mz = ((ok & 16u) ? 0x0Fu : 0u) | ((ok & 32u) ? 0xF0u : 0u);
         const uint32_t m = mx & my & mz;
+        if (!m) return;
+        float pc[3];
+        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
         if (METHOD == FE_APIC) {
-            if (!m) return;
-            const float4 pp = pos4[s];
-            float pc[3] = {pp.x, pp.y, pp.z};
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
+            apic_clamped<C>(g, pos4[s], pc);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
-            const float c0 = cr[0], c1 = cr[cap], c2 = cr[2 * cap];
+            c0 = cr[0];
+            c1 = cr[cap];
+            c2 = cr[2 * cap];
+        }
+        const double vs = (double)val * fx.sv;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                if (!((m >> e) & 1u)) continue;
-                const float gx = (e & 1) ? gp1[0] : gp0[0];
-                const float gy = (e & 2) ? gp1[1] : gp0[1];
-                const float gz = (e & 4) ? gp1[2] : gp0[2];
-                const float ve = val + (c0 * (gx - pc[0]) + c1 * (gy - pc[1]) + c2 * (gz - pc[2]));
-                const float w = (((e & 1) ? w1[0] : w0[0]) * ((e & 2) ? w1[1] : w0[1])) * ((e & 4) ? w1[2] : w0[2]);
-                num[e] += ve * w;
-                den[e] += w;
+        for (int e = 0; e < 8; ++e) {
+            if (!((m >> e) & 1u)) continue;
+            const float w = (((e & 1) ? w1[0] : w0[0]) * ((e & 2) ? w1[1] : w0[1])) * ((e & 4) ? w1[2] : w0[2]);
+            if (METHOD == FE_APIC) {
+                const float ve = apic_value(val, c0, c1, c2, face_pos(g, C, 0, F0[0] + (e & 1)),
+                                            face_pos(g, C, 1, F0[1] + ((e >> 1) & 1)),
+                                            face_pos(g, C, 2, F0[2] + (e >> 2)), pc);
+                num[e] += p2g_value_term((double)ve * fx.sv, w);
+            } else {
+                num[e] += p2g_value_term(vs, w);
             }
-        } else {
-            const float vm = m ? val : 0.0f;
-            const float wxy[4] = {w0[0] * w0[1], w1[0] * w0[1], w0[0] * w1[1], w1[0] * w1[1]};
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float w = wxy[e & 3] * ((e & 4) ? w1[2] : w0[2]);
-                num[e] += vm * w;
-                den[e] += w;
-            }
+            den[e] += p2g_weight_term(w, fx.sw);
         }
         marked |= m;
     };
@@ -453,7 +597,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
             for (; s < s1; ++s) consider(s, st[s], vc[s]);
         }
     }
-    // (q0 + q1) + (q2 + q3): the same value on all four lanes.
+    // the tile's four lanes' sums (exact integers: any order gives the same)
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         num[e] += __shfl_xor(num[e], 1, kP2GLanes);
@@ -472,14 +616,14 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         if (I >= sx) continue;
         const int e = ex + 2 * q;
         const int64_t f = (int64_t)I + (int64_t)sx * ((int64_t)J + (int64_t)sy * K);
-        float ne = 0.0f, de = 0.0f;
+        int64_t ne = 0, de = 0;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
             if (u == e) {
                 ne = num[u];
                 de = den[u];
             }
-        const float res = de > 0.0f ? ne / de : 0.0f;
+        const float res = p2g_quotient(ne, de, fx);
         out[f] = res;
         mark[f] = (uint8_t)((marked >> e) & 1u);
         if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
@@ -488,18 +632,14 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 
 // The same transfer with one thread per face (more threads, each face's
 // candidates read by the face itself): the variant that wins on small grids.
-// Its sums are bit-identical to the tile kernel's: a face's candidates form
-// one partial per lane coordinate (z plane, or y row for C = 2) in the tile
-// kernel's order (the tile's extra candidates add exact zeros there), and the
-// partials of the tile's lanes lo, lo+1, lo+2, lo+3 (lo = max((F & ~1) - 1, 0)
-// on the lane axis) are added as (p0 + p1) + (p2 + p3), lanes the face does
-// not read contributing +0.
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
                                                      const float4* __restrict__ pos4, const float4* __restrict__ st4,
                                                      int64_t cap, const int32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ bound, int head,
                                                      float* __restrict__ out, uint8_t* __restrict__ mark,
                                                      float* __restrict__ snap, int64_t padded) {
+    const P2GFixed fx = p2g_fixed(bound, C, head);
     const int64_t blk = xcd_remap(blockIdx.x, padded);
     const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
     const int64_t f = blk * kBlock + threadIdx.x;
@@ -509,26 +649,13 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
     const int F[3] = {I, J, K};
     int lo[3], hi[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = max(F[a] - 1, 0);
-        if (a == C) {
-            hi[a] = F[a];
-            if (METHOD != FE_FLIP && F[a] == g.n[a] - 2) hi[a] = F[a] + 1;
-        } else {
-            hi[a] = F[a] + 1;
-        }
-        hi[a] = min(hi[a], g.n[a] - 1);
-    }
-    float gp[3], clo[3], chi[3];
+    for (int a = 0; a < 3; ++a) candidate_cells<METHOD>(g, C, a, F[a], F[a], lo[a], hi[a]);
+    float gp[3];
     if (METHOD == FE_APIC) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            gp[a] = g.co[C][a] + g.h[a] * (float)F[a];
-            clo[a] = g.o[a] + 0.5f * g.h[a];
-            chi[a] = g.o[a] + g.L[a] - 0.5f * g.h[a];
-        }
+        for (int a = 0; a < 3; ++a) gp[a] = face_pos(g, C, a, F[a]);
     }
-    float num = 0.0f, den = 0.0f;
+    int64_t num = 0, den = 0;
     uint32_t marked = 0;
     auto consider = [&](int s, float4 r, float val) {
         const uint32_t b = __float_as_uint(r.w);
@@ -541,29 +668,20 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
         const float wz = dz ? r.z : 1.0f - r.z;
         const float w = (wx * wy) * wz;
         if (METHOD == FE_APIC) {
-            const float4 pp = pos4[s];
-            float pc[3] = {pp.x, pp.y, pp.z};
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                if (a != C) pc[a] = pc[a] < clo[a] ? clo[a] : (pc[a] > chi[a] ? chi[a] : pc[a]);
+            float pc[3];
+            apic_clamped<C>(g, pos4[s], pc);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
-            val = val + (cr[0] * (gp[0] - pc[0]) + cr[cap] * (gp[1] - pc[1]) + cr[2 * cap] * (gp[2] - pc[2]));
+            val = apic_value(val, cr[0], cr[cap], cr[2 * cap], gp[0], gp[1], gp[2], pc);
         }
         marked = 1;
-        num += val * w;
-        den += w;
+        num += p2g_value_term((double)val * fx.sv, w);
+        den += p2g_weight_term(w, fx.sw);
     };
     const float4* __restrict__ st = st4 + (int64_t)C * cap;
     const float* __restrict__ vc = V + (int64_t)C * cap;
-    constexpr int LA = C == 2 ? 1 : 2, OA = 3 - LA;  // the tile kernel's lane axis
-    const int pl = max((F[LA] & ~1) - 1, 0);
-    float pn[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     constexpr int kBatch = 8;  // candidates whose loads are issued together
-    for (int lc = lo[LA]; lc <= hi[LA]; ++lc) {
-        num = 0.0f;
-        den = 0.0f;
-        for (int oc = lo[OA]; oc <= hi[OA]; ++oc) {
-            const int jc = LA == 1 ? lc : oc, kc = LA == 2 ? lc : oc;
+    for (int kc = lo[2]; kc <= hi[2]; ++kc) {
+        for (int jc = lo[1]; jc <= hi[1]; ++jc) {
             const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
             const int s0 = start[row + lo[0]], s1 = start[row + hi[0] + 1];
             int s = s0;
@@ -580,20 +698,217 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
             }
             for (; s < s1; ++s) consider(s, st[s], vc[s]);
         }
-        const int q = lc - pl;  // 0..3
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (u == q) {
-                pn[u] = num;
-                pd[u] = den;
-            }
     }
-    const float nt = (pn[0] + pn[1]) + (pn[2] + pn[3]);
-    const float dt = (pd[0] + pd[1]) + (pd[2] + pd[3]);
-    const float res = dt > 0.0f ? nt / dt : 0.0f;
+    const float res = p2g_quotient(num, den, fx);
     out[f] = res;
     mark[f] = (uint8_t)marked;
     if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+}
+
+// ---------------------------------------------------------------------------
+// P2G as a scatter over blocks of faces: the variant whose loads coalesce.  A
+// workgroup owns an 8 x 8 x 8 block of faces of component C and walks the
+// particles of the block's candidate cells (candidate_cells over the block's
+// faces): each x row of candidate cells is one contiguous range of sorted
+// particles, and the workgroup takes the concatenated ranges with
+// consecutive threads on consecutive particles (a thread finds its row by a
+// binary search of the ranges' prefix sums in LDS).  Every stencil node of a
+// candidate that lies in the block adds the node's two fixed-point terms to
+// the block's LDS sums with 64-bit integer atomics; nodes outside the block
+// belong to the neighbouring block, which reads the same particle.  A face is
+// marked iff one of the 8 stencil bases that have it as a node holds a
+// particle (one bit per base of the block in LDS), the reference's rule
+// (marked even for a zero weight).  A particle is read by about 1.8 blocks
+// (10 x 10 x 9 candidate cells per 8 x 8 x 8 faces) against 6 tiles of the
+// 2x2x2 gather, and a wavefront's loads touch 8 cache lines instead of up to 64.
+// ---------------------------------------------------------------------------
+constexpr int kScB = 8;                                        // faces per block edge
+constexpr int kScFaces = kScB * kScB * kScB;                   // faces per block
+// LDS slot of block face (x, y, z): x + 9 y + 76 z.  A 64-bit sum spans two
+// of the 32 four-byte banks, so slot mod 16 picks its bank pair; the padded
+// strides put the four (y, z) parities of a wavefront's nodes at bank pairs
+// 0, 9, 12 and 5 instead of all at 0 and 8.
+constexpr int kScSy = 9, kScSz = 76;
+constexpr int kScRun = 1;    // consecutive candidates per lane group (see k_p2g_scatter)
+constexpr int kScAhead = 1;  // chunks whose loads are in flight ahead of the atomics
+constexpr int kScSlots = (kScB - 1) * (1 + kScSy + kScSz) + 1;
+constexpr int kScRows = (kScB + 2) * (kScB + 2);               // x rows of candidate cells, at most
+constexpr int kScBases = (kScB + 1) * (kScB + 1) * (kScB + 1); // stencil bases with a node in the block
+static_assert(kScRows <= 128, "one wavefront scans the row lengths, two per lane");
+
+template <int C, int METHOD>
+__global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __restrict__ V,
+                                                        const float* __restrict__ Cm,
+                                                        const float4* __restrict__ pos4,
+                                                        const float4* __restrict__ st4, int64_t cap,
+                                                        const int32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ bound, int head,
+                                                        float* __restrict__ out, uint8_t* __restrict__ mark,
+                                                        float* __restrict__ snap, int nbx, int nby, int nbz,
+                                                        int64_t padded) {
+    __shared__ unsigned long long s_num[kScSlots], s_den[kScSlots];
+    __shared__ uint32_t s_occ[(kScBases + 31) / 32];
+    __shared__ int32_t s_first[kScRows], s_pre[kScRows + 1];
+    const P2GFixed fx = p2g_fixed(bound, C, head);
+    const int64_t blk = xcd_remap(blockIdx.x, padded);
+    if (blk >= (int64_t)nbx * nby * nbz) return;  // the whole workgroup
+    int bi, bj, bk;
+    split_index(blk, nbx, nby, bi, bj, bk);
+    const int F0[3] = {bi * kScB, bj * kScB, bk * kScB};
+    const int sx = face_dim(g, C, 0), sy = face_dim(g, C, 1), sz = face_dim(g, C, 2);
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) candidate_cells<METHOD>(g, C, a, F0[a], F0[a] + kScB - 1, lo[a], hi[a]);
+    const int ny = hi[1] - lo[1] + 1, rows = ny * (hi[2] - lo[2] + 1);
+    for (int u = threadIdx.x; u < kScSlots; u += kBlock) s_num[u] = s_den[u] = 0ull;
+    for (int u = threadIdx.x; u < (kScBases + 31) / 32; u += kBlock) s_occ[u] = 0u;
+    if (threadIdx.x < 64) {
+        // row lengths and their exclusive prefix sums (rows 2l and 2l + 1 on lane l)
+        const int lane = threadIdx.x;
+        int len[2] = {0, 0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = 2 * lane + h;
+            if (r < rows) {
+                const int jc = lo[1] + r % ny, kc = lo[2] + r / ny;
+                const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
+                const int s0 = start[row + lo[0]];
+                len[h] = start[row + hi[0] + 1] - s0;
+                s_first[r] = s0;
+            }
+        }
+        const int pair = len[0] + len[1];
+        int inc = pair;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += o;
+        }
+        const int ex = inc - pair;
+        if (2 * lane < rows) s_pre[2 * lane] = ex;
+        if (2 * lane + 1 < rows) s_pre[2 * lane + 1] = ex + len[0];
+        if (lane == 63) s_pre[rows] = inc;
+    }
+    __syncthreads();
+    const int total = s_pre[rows];
+    const float4* __restrict__ st = st4 + (int64_t)C * cap;
+    const float* __restrict__ vc = V + (int64_t)C * cap;
+    // Thread (wavefront w, lane l) takes candidate kScRun (w + 4 (l / kScRun))
+    // + l % kScRun of each chunk of 256 (with kScRun = 1: w + 4 l), so a
+    // wavefront's atomics land on the nodes of 64 / kScRun places spread over
+    // the chunk instead of the few cells that 64 consecutive particles fill:
+    // same-address atomics in one instruction serialise, and spreading the
+    // lanes (at 16 B records 4 apart, 32 cache lines per load) measured
+    // faster than runs of 2, 4 or 8 consecutive particles (15, 23 and 60 %
+    // more time per launch at 128^3).  The records and values of the next
+    // kScAhead chunks are loaded before this chunk's atomics (one chunk
+    // ahead measured faster than two).
+    const int lane = threadIdx.x & 63;
+    const int slot = (lane % kScRun) + kScRun * ((threadIdx.x >> 6) + 4 * (lane / kScRun));
+    auto locate = [&](int q) {
+        int r = 0;
+#pragma unroll
+        for (int step = 64; step > 0; step >>= 1)
+            if (r + step < rows && s_pre[r + step] <= q) r += step;
+        return s_first[r] + (q - s_pre[r]);
+    };
+    auto process = [&](int s, float4 rec, float val) {
+        const uint32_t b = __float_as_uint(rec.w);
+        // base - F0 per axis (the base is within a few cells of the block)
+        int d[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) d[a] = (int)(int8_t)(uint8_t)((b >> (8 * a)) - (uint32_t)F0[a]);
+        if ((unsigned)(d[0] + 1) > (unsigned)kScB || (unsigned)(d[1] + 1) > (unsigned)kScB ||
+            (unsigned)(d[2] + 1) > (unsigned)kScB)
+            return;  // no node in the block
+        const int bit = (d[0] + 1) + (kScB + 1) * ((d[1] + 1) + (kScB + 1) * (d[2] + 1));
+        atomicOr(&s_occ[bit >> 5], 1u << (bit & 31));
+        const float fr[3] = {rec.x, rec.y, rec.z};
+        float pc[3];
+        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        if (METHOD == FE_APIC) {
+            apic_clamped<C>(g, pos4[s], pc);
+            const float* cr = Cm + (int64_t)(3 * C) * cap + s;
+            c0 = cr[0];
+            c1 = cr[cap];
+            c2 = cr[2 * cap];
+        }
+        const double vs = (double)val * fx.sv;
+        const int base = d[0] + kScSy * d[1] + kScSz * d[2];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+            if ((unsigned)(d[0] + ex) >= (unsigned)kScB || (unsigned)(d[1] + ey) >= (unsigned)kScB ||
+                (unsigned)(d[2] + ez) >= (unsigned)kScB)
+                continue;
+            const float w = ((ex ? fr[0] : 1.0f - fr[0]) * (ey ? fr[1] : 1.0f - fr[1])) * (ez ? fr[2] : 1.0f - fr[2]);
+            int64_t tn;
+            if (METHOD == FE_APIC) {
+                const float ve = apic_value(val, c0, c1, c2, face_pos(g, C, 0, F0[0] + d[0] + ex),
+                                            face_pos(g, C, 1, F0[1] + d[1] + ey), face_pos(g, C, 2, F0[2] + d[2] + ez),
+                                            pc);
+                tn = p2g_value_term((double)ve * fx.sv, w);
+            } else {
+                tn = p2g_value_term(vs, w);
+            }
+            const int li = base + ex + kScSy * ey + kScSz * ez;
+            atomicAdd(&s_num[li], (unsigned long long)tn);
+            atomicAdd(&s_den[li], (unsigned long long)p2g_weight_term(w, fx.sw));
+        }
+    };
+    int sp[kScAhead + 1];
+    float4 rec[kScAhead + 1];
+    float val[kScAhead + 1];
+#pragma unroll
+    for (int u = 0; u <= kScAhead; ++u) {
+        sp[u] = 0;
+        rec[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        val[u] = 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kScAhead; ++u) {
+        const int qa = slot + u * kBlock;
+        if (qa < total) {
+            sp[u] = locate(qa);
+            rec[u] = st[sp[u]];
+            val[u] = vc[sp[u]];
+        }
+    }
+    for (int q = slot; q < total; q += kBlock) {
+        const int qa = q + kScAhead * kBlock;
+        if (qa < total) {
+            sp[kScAhead] = locate(qa);
+            rec[kScAhead] = st[sp[kScAhead]];
+            val[kScAhead] = vc[sp[kScAhead]];
+        }
+        process(sp[0], rec[0], val[0]);
+#pragma unroll
+        for (int u = 0; u < kScAhead; ++u) {
+            sp[u] = sp[u + 1];
+            rec[u] = rec[u + 1];
+            val[u] = val[u + 1];
+        }
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < kScFaces; u += kBlock) {
+        const int l[3] = {u % kScB, (u / kScB) % kScB, u / (kScB * kScB)};
+        const int I = F0[0] + l[0], J = F0[1] + l[1], K = F0[2] + l[2];
+        if (I >= sx || J >= sy || K >= sz) continue;
+        uint32_t m = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            // base F - (ex, ey, ez), as a bit of the block's base range [F0 - 1, F0 + 7]
+            const int bit = (l[0] - (e & 1) + 1) + (kScB + 1) * ((l[1] - ((e >> 1) & 1) + 1) +
+                                                                 (kScB + 1) * (l[2] - (e >> 2) + 1));
+            m |= (s_occ[bit >> 5] >> (bit & 31)) & 1u;
+        }
+        const int li = l[0] + kScSy * l[1] + kScSz * l[2];
+        const float res = p2g_quotient((int64_t)s_num[li], (int64_t)s_den[li], fx);
+        const int64_t f = (int64_t)I + (int64_t)sx * ((int64_t)J + (int64_t)sy * K);
+        out[f] = res;
+        mark[f] = (uint8_t)m;
+        if (METHOD == FE_FLIP) snap[f] = res;  // FLIP snapshot right after P2G (flip_simulator.cpp:107-118)
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -864,7 +1179,7 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
     if (s->np > 0) {
         if ((e = launch_fill_i32(s->stream, s->cursor, 0, nc)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->np, s->cell_start,
-                           s->cursor, s->order);
+                           s->cursor, s->order, s->p2g_bound);
         // the large-bin list in cursor: its count starts at 0
         if ((e = launch_fill_i32(s->stream, s->cursor, 0, 1)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order,
@@ -873,7 +1188,8 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
                            (const int32_t*)s->cell_start, s->order, (const int32_t*)s->cursor, nc,
                            reinterpret_cast<int32_t*>(s->key));
         hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, s->order, s->np, s->cap,
-                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4, s->st4);
+                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4, s->st4,
+                           s->p2g_bound);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         s->cur = 1 - s->cur;
     }
@@ -886,13 +1202,27 @@ template <int METHOD>
 static hipError_t p2g_method(fe_scene* s) {
     const Particles& P = s->P[s->cur];
     const Geom& g = s->g;
+    const int head = p2g_head(s->np);
     for (int c = 0; c < 3; ++c) {
+        if (s->tune.p2g == FE_P2G_SCATTER) {
+            const int nb[3] = {(face_dim(g, c, 0) + kScB - 1) / kScB, (face_dim(g, c, 1) + kScB - 1) / kScB,
+                               (face_dim(g, c, 2) + kScB - 1) / kScB};
+            const int64_t padded = pad_blocks((int64_t)nb[0] * nb[1] * nb[2]);
+            auto kern = c == 0 ? k_p2g_scatter<0, METHOD>
+                               : (c == 1 ? k_p2g_scatter<1, METHOD> : k_p2g_scatter<2, METHOD>);
+            launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
+                         (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
+                         (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
+                         s->snap[c], nb[0], nb[1], nb[2], padded);
+            continue;
+        }
         if (s->tune.p2g == FE_P2G_FACES) {
             const int64_t padded = pad_blocks((face_count(g, c) + kBlock - 1) / kBlock);
             auto kern = c == 0 ? k_p2g_face<0, METHOD> : (c == 1 ? k_p2g_face<1, METHOD> : k_p2g_face<2, METHOD>);
             launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
                          (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
-                         (const int32_t*)s->cell_start, s->u[c], s->mark[c], s->snap[c], padded);
+                         (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
+                         s->snap[c], padded);
             continue;
         }
         const int64_t tiles = (int64_t)((face_dim(g, c, 0) + 1) / 2) * ((face_dim(g, c, 1) + 1) / 2) *
@@ -902,7 +1232,8 @@ static hipError_t p2g_method(fe_scene* s) {
         auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
         launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
                      (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
-                     (const int32_t*)s->cell_start, s->u[c], s->mark[c], s->snap[c], padded);
+                     (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
+                     s->snap[c], padded);
     }
     return hipGetLastError();
 }
@@ -1052,7 +1383,7 @@ hipError_t autotune(fe_scene* s, int reps, fe_tuning_times* times) {
     }
     if (!s->sorted && (e = launch_sort(s, false)) != hipSuccess) return e;
     const fe_tuning keep = s->tune;
-    for (int v = 0; v < 2; ++v) {
+    for (int v = 0; v < kP2GVariants; ++v) {
         s->tune.p2g = v;
         if ((e = time_median(s, reps, &t.p2g_ms[v], nullptr, [&] { return launch_p2g(s); })) != hipSuccess) break;
     }
@@ -1080,7 +1411,9 @@ hipError_t autotune(fe_scene* s, int reps, fe_tuning_times* times) {
         s->tune = keep;
         return e;
     }
-    s->tune.p2g = t.p2g_ms[FE_P2G_FACES] < t.p2g_ms[FE_P2G_TILES] ? FE_P2G_FACES : FE_P2G_TILES;
+    s->tune.p2g = FE_P2G_TILES;
+    for (int v = 1; v < kP2GVariants; ++v)
+        if (t.p2g_ms[v] < t.p2g_ms[s->tune.p2g]) s->tune.p2g = v;
     s->tune.g2p = t.g2p_ms[FE_G2P_PAIRS] < t.g2p_ms[FE_G2P_QUADS] ? FE_G2P_PAIRS : FE_G2P_QUADS;
     // leave the grid holding the chosen P2G of the current particles
     if ((e = launch_p2g(s)) != hipSuccess) return e;

@@ -320,8 +320,10 @@ int fe_scene_reserve_particles(fe_scene* scene, int64_t capacity);
 int fe_scene_particle_capacity(const fe_scene* scene, int64_t* capacity);
 
 /* Kernel variants and their autotuner (no reference counterpart).  P2G runs
- * as 2x2x2 face tiles (FE_P2G_TILES, four lanes per tile) or one thread per
- * face (FE_P2G_FACES); G2P and advection sample the face arrays directly
+ * as a gather over 2x2x2 face tiles (FE_P2G_TILES, four lanes per tile), a
+ * gather with one thread per face (FE_P2G_FACES) or a scatter over 8x8x8
+ * face blocks (FE_P2G_SCATTER, 64-bit integer LDS sums); all three add the
+ * same fixed-point terms, so their sums are equal.  G2P and advection sample the face arrays directly
  * (FE_G2P_PAIRS) or from a quad layout built first (FE_G2P_QUADS).  The
  * variants of each stage give bit-identical results, so the choice only
  * changes speed.  fe_scene_autotune times every variant `reps` times on the
@@ -329,8 +331,8 @@ int fe_scene_particle_capacity(const fe_scene* scene, int64_t* capacity);
  * velocities, markers and FLIP snapshot holding their P2G, which the next
  * substep recomputes anyway; the particles are untouched), keeps the fastest
  * of each stage and reports the choice and the median times in ms (times may
- * be NULL: p2g_ms[variant], g2p_ms[variant]).  Defaults: TILES, QUADS. */
-enum fe_p2g_variant { FE_P2G_TILES = 0, FE_P2G_FACES = 1 };
+ * be NULL: p2g_ms[variant], g2p_ms[variant]).  Defaults: SCATTER, QUADS. */
+enum fe_p2g_variant { FE_P2G_TILES = 0, FE_P2G_FACES = 1, FE_P2G_SCATTER = 2 };
 enum fe_g2p_variant { FE_G2P_QUADS = 0, FE_G2P_PAIRS = 1 };
 /* The multigrid V-cycle of FE_SOLVER_MGPCG on the levels above its
  * one-workgroup tail: one descent and one ascent kernel per level over LDS
@@ -346,7 +348,7 @@ typedef struct fe_tuning {
     int32_t mg;  /* fe_mg_variant */
 } fe_tuning;
 typedef struct fe_tuning_times {
-    double p2g_ms[2];
+    double p2g_ms[3];
     double g2p_ms[2];
 } fe_tuning_times;
 int fe_scene_get_tuning(const fe_scene* scene, fe_tuning* out);

@@ -40,13 +40,14 @@ def test_p2g_variants_bit_identical(ctx, method, shape):
     print(f"[gpu] p2g variants {NAMES[method]} {shape}", flush=True)
     s = _scene(ctx, shape, method, seed=3)
     out = {}
-    for variant in (capi.P2G_TILES, capi.P2G_FACES):
+    for variant in (capi.P2G_TILES, capi.P2G_FACES, capi.P2G_SCATTER):
         s.set_tuning(variant, capi.G2P_QUADS)
         s.run_stage(capi.STAGE_P2G)
         fields = GRIDS + ([capi.GRID_SU, capi.GRID_SV, capi.GRID_SW] if method == capi.FLIP else [])
         out[variant] = [s.get(f) for f in fields]
-    for a, b in zip(out[capi.P2G_TILES], out[capi.P2G_FACES]):
-        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    for other in (capi.P2G_FACES, capi.P2G_SCATTER):
+        for a, b in zip(out[capi.P2G_TILES], out[other]):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
     assert any(np.any(a != 0) for a in out[capi.P2G_TILES][:3])
     s.close()
 
@@ -86,7 +87,7 @@ def test_autotune_keeps_particles_and_results(ctx):
         s.substep(dt, solver)
     before = a.download_particles()
     (p2g, g2p), times = a.autotune(reps=3)
-    assert p2g in (capi.P2G_TILES, capi.P2G_FACES) and g2p in (capi.G2P_QUADS, capi.G2P_PAIRS)
+    assert p2g in (capi.P2G_TILES, capi.P2G_FACES, capi.P2G_SCATTER) and g2p in (capi.G2P_QUADS, capi.G2P_PAIRS)
     assert all(t > 0 for t in times["p2g_ms"] + times["g2p_ms"])
     assert a.tuning() == (p2g, g2p)
     after = a.download_particles()
