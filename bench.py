@@ -76,8 +76,8 @@ def kernel_bytes_per_launch(kernel, solver, n, npart, mg_split=False):
 
 def kernel_impl_bytes_per_launch(kernel, n, npart):
     """What this implementation's kernel reads and writes by design, where that
-    differs from the algorithmic bytes (P2G: the per-particle stencil records)."""
-    if kernel == "p2g":  # the scatter reads a float4 stencil record (16) + v_c (4) per candidate: a particle is a
+    differs from the algorithmic bytes (P2G: each particle read by about 1.76 face blocks)."""
+    if kernel == "p2g":  # the scatter reads a float4 position (16) + v_c (4) per candidate: a particle is a
         # candidate of about 900 / 512 face blocks (10 x 10 x 9 candidate cells per 8 x 8 x 8 faces); two bin
         # starts (8) per row of candidate cells (100 rows per 512 faces); u + marker + FLIP snapshot (9) per face
         faces = (n + 1) * n * n

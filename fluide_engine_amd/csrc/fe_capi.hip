@@ -649,7 +649,6 @@ int fe_scene_create(fe_ctx* ctx, const fe_scene_desc* d, fe_scene** out) {
         if (d->method == FE_APIC) chk(dalloc(&s->P[b].C, 9 * cap));
     }
     chk(dalloc(&s->pos4, cap));
-    chk(dalloc(&s->st4, 3 * cap));
     chk(dalloc(&s->key, cap));
     chk(dalloc(&s->order, cap));
     chk(dalloc(&s->cell_count, nc + 1));
@@ -736,7 +735,7 @@ int fe_scene_destroy(fe_scene* s) {
     for (int b = 0; b < 2; ++b) {
         F(s->P[b].x); F(s->P[b].v); F(s->P[b].C); F(s->P[b].pid);
     }
-    F(s->pos4); F(s->st4); F(s->key); F(s->order); F(s->cell_count); F(s->cell_start); F(s->cursor); F(s->scan_sums);
+    F(s->pos4); F(s->key); F(s->order); F(s->cell_count); F(s->cell_start); F(s->cursor); F(s->scan_sums);
     F(s->p2g_bound);
     for (int i = 0; i <= kMaxStages; ++i)
         if (s->ev[i]) (void)hipEventDestroy(s->ev[i]);

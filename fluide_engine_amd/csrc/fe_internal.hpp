@@ -529,7 +529,6 @@ struct fe_scene {
     int64_t np = 0, cap = 0;
     fe::Particles P[2];
     float4* pos4 = nullptr;  // positions of the sorted particles as (x, y, z, 0): one load per gathered candidate
-    float4* st4 = nullptr;   // 3 x cap stencil records of the sorted particles (see k_gather)
     int cur = 0;
     uint32_t* key = nullptr;
     int32_t* cell_count = nullptr;   // n^3 + 1

@@ -264,23 +264,18 @@ __global__ __launch_bounds__(kBlock) void k_sort_big_bins(const int32_t* __restr
     }
 }
 
-// Besides the reorder, k_gather writes what P2G reads per candidate: the
-// position as one float4 and, per face component c, the stencil record
-// st4[c*cap + s] = (fx, fy, fz, bits(i0 & 255 | (j0 & 255) << 8 | (k0 & 255) << 16))
-// of GetCooridnatesAndWeights (the same stencil_axis values P2G would compute).
-// Only the low 8 bits of each base index are kept: every candidate a face F
-// scans lies in a cell within 1 of F on each axis and its clamped base is
-// within 2 of its cell, so F - base is in [-3, 3] and is 0 or 1 exactly when
-// (F - base) mod 256 is.
-__device__ inline float4 stencil_record(const Geom& g, int c, const float p[3], int base[3]) {
-    float f[3];
+// The stencil of GetCooridnatesAndWeights for face component c at position
+// pp, with the scheme's stencil size per axis: base node index and fraction
+// per axis (the same stencil_axis values on every path).  P2G evaluates it per
+// candidate from the binned position, so all P2G variants use the same values.
+__device__ inline void p2g_stencil(const Geom& g, int c, float4 pp, int base[3], float fr[3]) {
+    const float p[3] = {pp.x, pp.y, pp.z};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) stencil_axis(g, c, a, p[a], stencil_size(g, c, a), base[a], f[a]);
-    const uint32_t b =
-        ((uint32_t)base[0] & 255u) | (((uint32_t)base[1] & 255u) << 8) | (((uint32_t)base[2] & 255u) << 16);
-    return make_float4(f[0], f[1], f[2], __uint_as_float(b));
+    for (int a = 0; a < 3; ++a) stencil_axis(g, c, a, p[a], stencil_size(g, c, a), base[a], fr[a]);
 }
 
+// Besides the reorder, k_gather writes the binned position as one float4 (the
+// one load per candidate of P2G and the SDF).
 // It also raises P2G's bound on the values it transfers (the fixed-point
 // scale below): per component c the largest |v_c| and, for APIC, |v_c| plus
 // sum_a |C_ca| times the larger distance from the (clamped) particle to the
@@ -291,8 +286,7 @@ __device__ inline float4 stencil_record(const Geom& g, int c, const float p[3], 
 // finite bound, so a non-finite velocity shows as a non-finite bound).
 __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __restrict__ order, int64_t np,
                                                    int64_t cap, Particles src, Particles dst, int with_c,
-                                                   float4* __restrict__ pos4, float4* __restrict__ st4,
-                                                   uint32_t* __restrict__ bound) {
+                                                   float4* __restrict__ pos4, uint32_t* __restrict__ bound) {
     const int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     uint32_t bb[3] = {0u, 0u, 0u};
     if (s < np) {
@@ -305,13 +299,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __rest
             dst.x[a * cap + s] = p[a];
             dst.v[a * cap + s] = v[a];
         }
-        pos4[s] = make_float4(p[0], p[1], p[2], 0.0f);
-        int base[3][3];
+        const float4 pp = make_float4(p[0], p[1], p[2], 0.0f);
+        pos4[s] = pp;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            st4[c * cap + s] = stencil_record(g, c, p, base[c]);
-            bb[c] = __float_as_uint(fabsf(v[c]));
-        }
+        for (int c = 0; c < 3; ++c) bb[c] = __float_as_uint(fabsf(v[c]));
         if (with_c) {
             float cm[9];
 #pragma unroll
@@ -321,6 +312,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __rest
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
+                int base[3];
+                float fr[3];
+                p2g_stencil(g, c, pp, base, fr);
                 float t = fabsf(v[c]);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
@@ -329,8 +323,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(Geom g, const int32_t* __rest
                         const float clo = g.o[a] + 0.5f * g.h[a], chi = g.o[a] + g.L[a] - 0.5f * g.h[a];
                         pc = pc < clo ? clo : (pc > chi ? chi : pc);
                     }
-                    const float g0 = g.co[c][a] + g.h[a] * (float)base[c][a];
-                    const float g1 = g.co[c][a] + g.h[a] * (float)(base[c][a] + 1);
+                    const float g0 = g.co[c][a] + g.h[a] * (float)base[a];
+                    const float g1 = g.co[c][a] + g.h[a] * (float)(base[a] + 1);
                     t += fabsf(cm[3 * c + a]) * fmaxf(fabsf(g0 - pc), fabsf(g1 - pc));
                 }
                 bb[c] = __float_as_uint(t);
@@ -494,8 +488,8 @@ constexpr int kP2GLanes = 4;  // lanes per tile (one candidate plane each)
 
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
-                                                const float4* __restrict__ pos4, const float4* __restrict__ st4,
-                                                int64_t cap, const int32_t* __restrict__ start,
+                                                const float4* __restrict__ pos4, int64_t cap,
+                                                const int32_t* __restrict__ start,
                                                 const uint32_t* __restrict__ bound, int head, float* __restrict__ out,
                                                 uint8_t* __restrict__ mark, float* __restrict__ snap,
                                                 int64_t padded) {
@@ -519,19 +513,19 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 #pragma unroll
     for (int e = 0; e < 8; ++e) num[e] = den[e] = 0;
     uint32_t marked = 0;
-    // One candidate, from its stencil record (fx, fy, fz, packed low bytes of
-    // the base indices): face F takes it iff F - base is 0 or 1 on every axis,
-    // with weight factor 1 - f at offset 0 and f at offset 1.
-    auto consider = [&](int s, float4 r, float val) {
-        const uint32_t b = __float_as_uint(r.w);
-        const float fr[3] = {r.x, r.y, r.z};
+    // One candidate, from its stencil (base node, fraction per axis): face F
+    // takes it iff F - base is 0 or 1 on every axis, with weight factor 1 - f
+    // at offset 0 and f at offset 1.
+    auto consider = [&](int s, float4 pp, float val) {
+        int bs[3];
+        float fr[3];
+        p2g_stencil(g, C, pp, bs, fr);
         float w0[3], w1[3];
         uint32_t ok = 0;  // bit 2a: face offset 0 on axis a accepts; bit 2a+1: offset 1
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const uint32_t d0 = ((uint32_t)F0[a] - (b >> (8 * a))) & 255u;
-            const uint32_t d1 = (d0 + 1u) & 255u;
-            const bool a0 = d0 <= 1u, a1 = d1 <= 1u;
+            const int d0 = F0[a] - bs[a], d1 = d0 + 1;
+            const bool a0 = (unsigned)d0 <= 1u, a1 = (unsigned)d1 <= 1u;
             w0[a] = d0 ? fr[a] : 1.0f - fr[a];
             w1[a] = d1 ? fr[a] : 1.0f - fr[a];
             ok |= (a0 ? 1u : 0u) << (2 * a);
@@ -546,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
         float pc[3];
         float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
         if (METHOD == FE_APIC) {
-            apic_clamped<C>(g, pos4[s], pc);
+            apic_clamped<C>(g, pp, pc);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
             c0 = cr[0];
             c1 = cr[cap];
@@ -575,7 +569,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
     constexpr int LA = C == 2 ? 1 : 2, OA = 3 - LA;
     const int lc = lo[LA] + q;
     if (lc <= hi[LA]) {
-        const float4* __restrict__ st = st4 + (int64_t)C * cap;
         const float* __restrict__ vc = V + (int64_t)C * cap;
         constexpr int kBatch = 8;  // candidates whose loads are issued together
         for (int oc = lo[OA]; oc <= hi[OA]; ++oc) {
@@ -588,13 +581,13 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
                 float pv[kBatch];
 #pragma unroll
                 for (int u = 0; u < kBatch; ++u) {
-                    rr[u] = st[s + u];
+                    rr[u] = pos4[s + u];
                     pv[u] = vc[s + u];
                 }
 #pragma unroll
                 for (int u = 0; u < kBatch; ++u) consider(s + u, rr[u], pv[u]);
             }
-            for (; s < s1; ++s) consider(s, st[s], vc[s]);
+            for (; s < s1; ++s) consider(s, pos4[s], vc[s]);
         }
     }
     // the tile's four lanes' sums (exact integers: any order gives the same)
@@ -634,8 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(Geom g, const float* __restrict_
 // candidates read by the face itself): the variant that wins on small grids.
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __restrict__ V, const float* __restrict__ Cm,
-                                                     const float4* __restrict__ pos4, const float4* __restrict__ st4,
-                                                     int64_t cap, const int32_t* __restrict__ start,
+                                                     const float4* __restrict__ pos4, int64_t cap,
+                                                     const int32_t* __restrict__ start,
                                                      const uint32_t* __restrict__ bound, int head,
                                                      float* __restrict__ out, uint8_t* __restrict__ mark,
                                                      float* __restrict__ snap, int64_t padded) {
@@ -657,19 +650,19 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
     }
     int64_t num = 0, den = 0;
     uint32_t marked = 0;
-    auto consider = [&](int s, float4 r, float val) {
-        const uint32_t b = __float_as_uint(r.w);
-        const uint32_t dx = ((uint32_t)I - b) & 255u;
-        const uint32_t dy = ((uint32_t)J - (b >> 8)) & 255u;
-        const uint32_t dz = ((uint32_t)K - (b >> 16)) & 255u;
+    auto consider = [&](int s, float4 pp, float val) {
+        int bs[3];
+        float fr[3];
+        p2g_stencil(g, C, pp, bs, fr);
+        const uint32_t dx = (uint32_t)(I - bs[0]), dy = (uint32_t)(J - bs[1]), dz = (uint32_t)(K - bs[2]);
         if ((dx | dy | dz) > 1u) return;
-        const float wx = dx ? r.x : 1.0f - r.x;
-        const float wy = dy ? r.y : 1.0f - r.y;
-        const float wz = dz ? r.z : 1.0f - r.z;
+        const float wx = dx ? fr[0] : 1.0f - fr[0];
+        const float wy = dy ? fr[1] : 1.0f - fr[1];
+        const float wz = dz ? fr[2] : 1.0f - fr[2];
         const float w = (wx * wy) * wz;
         if (METHOD == FE_APIC) {
             float pc[3];
-            apic_clamped<C>(g, pos4[s], pc);
+            apic_clamped<C>(g, pp, pc);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
             val = apic_value(val, cr[0], cr[cap], cr[2 * cap], gp[0], gp[1], gp[2], pc);
         }
@@ -677,7 +670,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
         num += p2g_value_term((double)val * fx.sv, w);
         den += p2g_weight_term(w, fx.sw);
     };
-    const float4* __restrict__ st = st4 + (int64_t)C * cap;
     const float* __restrict__ vc = V + (int64_t)C * cap;
     constexpr int kBatch = 8;  // candidates whose loads are issued together
     for (int kc = lo[2]; kc <= hi[2]; ++kc) {
@@ -690,13 +682,13 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
                 float pv[kBatch];
 #pragma unroll
                 for (int u = 0; u < kBatch; ++u) {
-                    rr[u] = st[s + u];
+                    rr[u] = pos4[s + u];
                     pv[u] = vc[s + u];
                 }
 #pragma unroll
                 for (int u = 0; u < kBatch; ++u) consider(s + u, rr[u], pv[u]);
             }
-            for (; s < s1; ++s) consider(s, st[s], vc[s]);
+            for (; s < s1; ++s) consider(s, pos4[s], vc[s]);
         }
     }
     const float res = p2g_quotient(num, den, fx);
@@ -739,8 +731,7 @@ static_assert(kScRows <= 128, "one wavefront scans the row lengths, two per lane
 template <int C, int METHOD>
 __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __restrict__ V,
                                                         const float* __restrict__ Cm,
-                                                        const float4* __restrict__ pos4,
-                                                        const float4* __restrict__ st4, int64_t cap,
+                                                        const float4* __restrict__ pos4, int64_t cap,
                                                         const int32_t* __restrict__ start,
                                                         const uint32_t* __restrict__ bound, int head,
                                                         float* __restrict__ out, uint8_t* __restrict__ mark,
@@ -791,16 +782,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
     }
     __syncthreads();
     const int total = s_pre[rows];
-    const float4* __restrict__ st = st4 + (int64_t)C * cap;
     const float* __restrict__ vc = V + (int64_t)C * cap;
     // Thread (wavefront w, lane l) takes candidate kScRun (w + 4 (l / kScRun))
     // + l % kScRun of each chunk of 256 (with kScRun = 1: w + 4 l), so a
     // wavefront's atomics land on the nodes of 64 / kScRun places spread over
     // the chunk instead of the few cells that 64 consecutive particles fill:
     // same-address atomics in one instruction serialise, and spreading the
-    // lanes (at 16 B records 4 apart, 32 cache lines per load) measured
+    // lanes (16 B positions 4 apart, 32 cache lines per load) measured
     // faster than runs of 2, 4 or 8 consecutive particles (15, 23 and 60 %
-    // more time per launch at 128^3).  The records and values of the next
+    // more time per launch at 128^3).  The positions and values of the next
     // kScAhead chunks are loaded before this chunk's atomics (one chunk
     // ahead measured faster than two).
     const int lane = threadIdx.x & 63;
@@ -812,22 +802,21 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
             if (r + step < rows && s_pre[r + step] <= q) r += step;
         return s_first[r] + (q - s_pre[r]);
     };
-    auto process = [&](int s, float4 rec, float val) {
-        const uint32_t b = __float_as_uint(rec.w);
-        // base - F0 per axis (the base is within a few cells of the block)
-        int d[3];
+    auto process = [&](int s, float4 pp, float val) {
+        int d[3];  // stencil base - F0 per axis
+        float fr[3];
+        p2g_stencil(g, C, pp, d, fr);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) d[a] = (int)(int8_t)(uint8_t)((b >> (8 * a)) - (uint32_t)F0[a]);
+        for (int a = 0; a < 3; ++a) d[a] -= F0[a];
         if ((unsigned)(d[0] + 1) > (unsigned)kScB || (unsigned)(d[1] + 1) > (unsigned)kScB ||
             (unsigned)(d[2] + 1) > (unsigned)kScB)
             return;  // no node in the block
         const int bit = (d[0] + 1) + (kScB + 1) * ((d[1] + 1) + (kScB + 1) * (d[2] + 1));
         atomicOr(&s_occ[bit >> 5], 1u << (bit & 31));
-        const float fr[3] = {rec.x, rec.y, rec.z};
         float pc[3];
         float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
         if (METHOD == FE_APIC) {
-            apic_clamped<C>(g, pos4[s], pc);
+            apic_clamped<C>(g, pp, pc);
             const float* cr = Cm + (int64_t)(3 * C) * cap + s;
             c0 = cr[0];
             c1 = cr[cap];
@@ -870,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
         const int qa = slot + u * kBlock;
         if (qa < total) {
             sp[u] = locate(qa);
-            rec[u] = st[sp[u]];
+            rec[u] = pos4[sp[u]];
             val[u] = vc[sp[u]];
         }
     }
@@ -878,7 +867,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
         const int qa = q + kScAhead * kBlock;
         if (qa < total) {
             sp[kScAhead] = locate(qa);
-            rec[kScAhead] = st[sp[kScAhead]];
+            rec[kScAhead] = pos4[sp[kScAhead]];
             val[kScAhead] = vc[sp[kScAhead]];
         }
         process(sp[0], rec[0], val[0]);
@@ -1188,8 +1177,7 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
                            (const int32_t*)s->cell_start, s->order, (const int32_t*)s->cursor, nc,
                            reinterpret_cast<int32_t*>(s->key));
         hipLaunchKernelGGL(k_gather, grid1(s->np), dim3(kBlock), 0, s->stream, s->g, s->order, s->np, s->cap,
-                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4, s->st4,
-                           s->p2g_bound);
+                           s->P[s->cur], s->P[1 - s->cur], s->g.method == FE_APIC ? 1 : 0, s->pos4, s->p2g_bound);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         s->cur = 1 - s->cur;
     }
@@ -1211,7 +1199,7 @@ static hipError_t p2g_method(fe_scene* s) {
             auto kern = c == 0 ? k_p2g_scatter<0, METHOD>
                                : (c == 1 ? k_p2g_scatter<1, METHOD> : k_p2g_scatter<2, METHOD>);
             launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
-                         (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
+                         (const float*)P.C, (const float4*)s->pos4, s->cap,
                          (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
                          s->snap[c], nb[0], nb[1], nb[2], padded);
             continue;
@@ -1220,7 +1208,7 @@ static hipError_t p2g_method(fe_scene* s) {
             const int64_t padded = pad_blocks((face_count(g, c) + kBlock - 1) / kBlock);
             auto kern = c == 0 ? k_p2g_face<0, METHOD> : (c == 1 ? k_p2g_face<1, METHOD> : k_p2g_face<2, METHOD>);
             launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
-                         (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
+                         (const float*)P.C, (const float4*)s->pos4, s->cap,
                          (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
                          s->snap[c], padded);
             continue;
@@ -1231,7 +1219,7 @@ static hipError_t p2g_method(fe_scene* s) {
         const int64_t padded = pad_blocks((tiles + per_block - 1) / per_block);
         auto kern = c == 0 ? k_p2g<0, METHOD> : (c == 1 ? k_p2g<1, METHOD> : k_p2g<2, METHOD>);
         launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
-                     (const float*)P.C, (const float4*)s->pos4, (const float4*)s->st4, s->cap,
+                     (const float*)P.C, (const float4*)s->pos4, s->cap,
                      (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
                      s->snap[c], padded);
     }

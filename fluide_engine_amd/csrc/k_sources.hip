@@ -152,13 +152,13 @@ hipError_t seed_from_counts(fe_scene* s, const NodeGrid3& E, uint64_t seed, uint
 
 // Grows every per-particle array to `capacity`, keeping the current particles
 // (positions, velocities, affine rows and ids of P[cur]); the derived arrays
-// (binned positions, stencil records, keys, order) are rebuilt by the next sort.
+// (binned positions, keys, order) are rebuilt by the next sort.
 hipError_t reserve_particles(fe_scene* s, int64_t capacity) {
     if (capacity <= s->cap) return hipSuccess;
     const bool apic = s->desc.method == FE_APIC;
     const int64_t oc = s->cap, np = s->np;
     Particles NP[2];
-    float4 *pos4 = nullptr, *st4 = nullptr;
+    float4* pos4 = nullptr;
     uint32_t* key = nullptr;
     int32_t* order = nullptr;
     hipError_t e = hipSuccess;
@@ -172,10 +172,9 @@ hipError_t reserve_particles(fe_scene* s, int64_t capacity) {
         if (apic) A(&NP[b].C, 9 * capacity);
     }
     A(&pos4, capacity);
-    A(&st4, 3 * capacity);
     A(&key, capacity);
     A(&order, capacity);
-    auto release = [](Particles* P, float4* a, float4* b, uint32_t* k, int32_t* o) {
+    auto release = [](Particles* P, float4* a, uint32_t* k, int32_t* o) {
         for (int q = 0; q < 2; ++q) {
             if (P[q].x) (void)hipFree(P[q].x);
             if (P[q].v) (void)hipFree(P[q].v);
@@ -183,12 +182,11 @@ hipError_t reserve_particles(fe_scene* s, int64_t capacity) {
             if (P[q].C) (void)hipFree(P[q].C);
         }
         if (a) (void)hipFree(a);
-        if (b) (void)hipFree(b);
         if (k) (void)hipFree(k);
         if (o) (void)hipFree(o);
     };
     if (e != hipSuccess) {
-        release(NP, pos4, st4, key, order);
+        release(NP, pos4, key, order);
         return e;
     }
     const Particles& P = s->P[s->cur];
@@ -205,14 +203,13 @@ hipError_t reserve_particles(fe_scene* s, int64_t capacity) {
         e = hipMemcpyAsync(Q.pid, P.pid, sizeof(int32_t) * np, hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
-        release(NP, pos4, st4, key, order);
+        release(NP, pos4, key, order);
         return e;
     }
-    release(s->P, s->pos4, s->st4, s->key, s->order);
+    release(s->P, s->pos4, s->key, s->order);
     s->P[0] = NP[0];
     s->P[1] = NP[1];
     s->pos4 = pos4;
-    s->st4 = st4;
     s->key = key;
     s->order = order;
     s->cap = capacity;
