@@ -380,6 +380,7 @@ int substep_graph(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stat
     s->cur = G.cur_after;
     s->sorted = true;
     s->keys_ready = false;
+    s->count_zero = true;  // the head's k_scatter counted the histogram down
     s->sdf_from_cells = G.sdf_from_cells_after;
     bool resumed = false;
     HIPCHK(ctx, finish_pressure_after_head(s, d, G.run_after_a, &resumed));
@@ -413,6 +414,7 @@ int substep_graph(fe_scene* s, double dt, const fe_solver_desc* d, fe_solve_stat
     HIPCHK(ctx, hipStreamSynchronize(s->stream));
     s->sorted = false;
     s->keys_ready = true;
+    s->count_zero = false;  // the advection's histogram of the next bins
     ++s->graph_substeps;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, s->ev[0], s->ev[1]);

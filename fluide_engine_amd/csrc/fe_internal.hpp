@@ -531,7 +531,8 @@ struct fe_scene {
     float4* pos4 = nullptr;  // positions of the sorted particles as (x, y, z, 0): one load per gathered candidate
     int cur = 0;
     uint32_t* key = nullptr;
-    int32_t* cell_count = nullptr;   // n^3 + 1
+    int32_t* cell_count = nullptr;   // n^3 + 1: bin histogram (all zero after the sort's k_scatter)
+    bool count_zero = false;         // cell_count is known to be all zero (no fill needed before a histogram)
     int32_t* cell_start = nullptr;   // n^3 + 1
     int32_t* cursor = nullptr;       // n^3
     int32_t* order = nullptr;        // sorted slot -> old index

@@ -143,19 +143,26 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int32_t* __restrict
 }
 
 // Slots in the bins: the head of each run reserves the run's slots with one
-// atomic and the run's lanes take them in lane order.
-// Workgroup 0 also clears P2G's value-bound slots, which k_gather raises.
+// atomic and the run's lanes take them in lane order.  The reservations count
+// the histogram down (a bin's slots are taken from its end), so it is all
+// zero afterwards, ready for the next histogram.  Workgroup 0 also clears
+// P2G's value-bound slots, which k_gather raises, and the count of
+// k_sort_bins' large-bin list.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key, int64_t np,
-                                                    const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
-                                                    int32_t* __restrict__ order, uint32_t* __restrict__ bound) {
+                                                    const int32_t* __restrict__ start, int32_t* __restrict__ count,
+                                                    int32_t* __restrict__ order, uint32_t* __restrict__ bound,
+                                                    int32_t* __restrict__ big) {
     static_assert(3 * kP2GBoundSlots <= kBlock, "one workgroup clears the bound slots");
-    if (blockIdx.x == 0 && threadIdx.x < 3 * kP2GBoundSlots) bound[threadIdx.x] = 0u;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 3 * kP2GBoundSlots) bound[threadIdx.x] = 0u;
+        if (threadIdx.x == 0) big[0] = 0;
+    }
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const bool active = p < np;
     const uint32_t k = active ? key[p] : 0u;
     const KeyRun r = key_run(k, active);
     int base = 0;
-    if (r.head) base = start[k] + atomicAdd(&cursor[k], r.len);
+    if (r.head) base = start[k] + atomicSub(&count[k], r.len) - r.len;
     base = __shfl(base, r.first, 64);
     if (active) order[base + ((int)(threadIdx.x & 63) - r.first)] = (int32_t)p;
 }
@@ -1161,16 +1168,16 @@ hipError_t launch_sort(fe_scene* s, bool keys_ready) {
     const int64_t nc = cell_count(s->g);
     hipError_t e;
     if (!keys_ready) {
-        if ((e = launch_fill_i32(s->stream, s->cell_count, 0, nc + 1)) != hipSuccess) return e;
+        if (!s->count_zero && (e = launch_fill_i32(s->stream, s->cell_count, 0, nc + 1)) != hipSuccess) return e;
         if ((e = launch_keys(s, true)) != hipSuccess) return e;
+        s->count_zero = false;
     }
     if ((e = launch_scan(s, s->cell_count, s->cell_start, nc)) != hipSuccess) return e;
     if (s->np > 0) {
-        if ((e = launch_fill_i32(s->stream, s->cursor, 0, nc)) != hipSuccess) return e;
+        // counts the histogram back to zero; clears the large-bin list's count (in cursor)
         hipLaunchKernelGGL(k_scatter, grid1(s->np), dim3(kBlock), 0, s->stream, s->key, s->np, s->cell_start,
-                           s->cursor, s->order, s->p2g_bound);
-        // the large-bin list in cursor: its count starts at 0
-        if ((e = launch_fill_i32(s->stream, s->cursor, 0, 1)) != hipSuccess) return e;
+                           s->cell_count, s->order, s->p2g_bound, s->cursor);
+        s->count_zero = true;
         hipLaunchKernelGGL(k_sort_bins, grid1(nc), dim3(kBlock), 0, s->stream, s->cell_start, nc, s->order,
                            s->cursor);
         hipLaunchKernelGGL(k_sort_big_bins, dim3(kBigBinGroups), dim3(kBlock), 0, s->stream,
@@ -1273,8 +1280,9 @@ hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bo
     hipError_t e;
     if (quads && (e = launch_quads(s, s->u, s->quad)) != hipSuccess) return e;
     if (quads && flip_g2p && (e = launch_quads(s, s->snap, s->squad)) != hipSuccess) return e;
-    if (emit_keys) {
+    if (emit_keys && !s->count_zero) {
         if ((e = launch_fill_i32(s->stream, s->cell_count, 0, cell_count(s->g) + 1)) != hipSuccess) return e;
+        s->count_zero = true;
     }
     uint32_t* key = emit_keys ? s->key : nullptr;
     int32_t* cnt = emit_keys ? s->cell_count : nullptr;
@@ -1294,6 +1302,7 @@ hipError_t launch_g2p_move(fe_scene* s, double dt, bool do_g2p, bool do_move, bo
                      GridSrc<false>{grid, gq}, GridSrc<false>{snaps, sq}, blend, (float)dt, dtp, s->move_substeps,
                      (int)do_g2p, (int)do_move, key, cnt);
     }
+    if (emit_keys) s->count_zero = false;  // the histogram of the next bins
     if (do_move) s->sorted = false;
     return hipGetLastError();
 }
