@@ -42,34 +42,67 @@ __global__ __launch_bounds__(kBlock) void k_build_system(Geom g, FaceSet u, cons
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double v[1] = {0.0};
     int fluid = 0;
-    for (int64_t c = sp.c0 + threadIdx.x; c < sp.c1; c += kBlock) {
-        float bb = 0.0f;
-        uint16_t cd = 0;
-        if (flags[c] == FE_FLUID) {
+    // Groups of G cells: their flags loaded together, then every FLUID cell's
+    // neighbour flags and faces together (two dependent load rounds per group
+    // instead of per cell); the cells are then written in order.
+    const FaceSet fs = u;
+    constexpr int G = 4;
+    for (int64_t c0 = sp.c0 + threadIdx.x; c0 < sp.c1; c0 += (int64_t)G * kBlock) {
+        uint8_t fc[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int64_t c = c0 + (int64_t)u * kBlock;
+            fc[u] = c < sp.c1 ? flags[c] : (uint8_t)FE_AIR;
+        }
+        uint8_t fn[G][6];
+        float fv[G][6];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int64_t c = c0 + (int64_t)u * kBlock;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                fn[u][q] = (uint8_t)FE_BOUNDRY;
+                fv[u][q] = 0.0f;
+            }
+            if (fc[u] != FE_FLUID) continue;
             const uint32_t cu = (uint32_t)c;  // grids hold fewer than 2^30 cells
             const uint32_t i = cu % nx, j = (cu / nx) % ny, k = cu / (nx * ny);
             const bool in[6] = {i > 0, i + 1 < nx, j > 0, j + 1 < ny, k > 0, k + 1 < nz};
             const int64_t nb[6] = {c - 1, c + 1, c - sy, c + sy, c - sz, c + sz};
-            cd = kFluidBit;
-            fluid = 1;
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                if (!in[q]) continue;
-                const uint8_t f = flags[nb[q]];
-                if (f == FE_BOUNDRY) continue;
-                cd = (uint16_t)(cd + (1u << (6 + 2 * (q >> 1))));
-                if (f == FE_FLUID) cd |= (uint16_t)(1u << q);
-            }
+            for (int q = 0; q < 6; ++q)
+                if (in[q]) fn[u][q] = flags[nb[q]];
             const int64_t iu = (int64_t)i + (int64_t)(nx + 1) * ((int64_t)j + (int64_t)ny * k);
             const int64_t iv = (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)(ny + 1) * k);
-            const float left = u.a[0][iu], right = u.a[0][iu + 1];
-            const float down = u.a[1][iv], up = u.a[1][iv + nx];
-            const float back = u.a[2][c], front = u.a[2][c + sz];
-            bb = (right - left) / g.h[0] + (up - down) / g.h[1] + (front - back) / g.h[2];
+            fv[u][0] = fs.a[0][iu];
+            fv[u][1] = fs.a[0][iu + 1];
+            fv[u][2] = fs.a[1][iv];
+            fv[u][3] = fs.a[1][iv + nx];
+            fv[u][4] = fs.a[2][c];
+            fv[u][5] = fs.a[2][c + sz];
         }
-        code[c] = cd;
-        b[c] = bb;
-        v[0] += (double)bb * (double)bb;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const int64_t c = c0 + (int64_t)u * kBlock;
+            if (c >= sp.c1) break;
+            float bb = 0.0f;
+            uint16_t cd = 0;
+            if (fc[u] == FE_FLUID) {
+                cd = kFluidBit;
+                fluid = 1;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const uint8_t f = fn[u][q];  // FE_BOUNDRY outside the grid
+                    if (f == FE_BOUNDRY) continue;
+                    cd = (uint16_t)(cd + (1u << (6 + 2 * (q >> 1))));
+                    if (f == FE_FLUID) cd |= (uint16_t)(1u << q);
+                }
+                bb = (fv[u][1] - fv[u][0]) / g.h[0] + (fv[u][3] - fv[u][2]) / g.h[1] + (fv[u][5] - fv[u][4]) / g.h[2];
+            }
+            code[c] = cd;
+            b[c] = bb;
+            v[0] += (double)bb * (double)bb;
+        }
     }
     // chunks without FLUID cells hold zeros in every CG and level-0 multigrid
     // vector throughout the solve, so those kernels skip them
