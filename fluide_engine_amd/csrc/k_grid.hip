@@ -597,8 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Geom g, FaceSet a, double dt, un
     int bad = 0;
     const int64_t nc = cell_count(g);
     const int64_t stride = (int64_t)gridDim.x * kBlock;
-    // four cells per step, all their face loads issued before any is used
-    constexpr int kG = 4;
+    // eight cells per step, all their face loads issued before any is used
+    constexpr int kG = 8;
     for (int64_t c0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; c0 < nc; c0 += kG * stride) {
         float f[kG][6];
 #pragma unroll

@@ -246,36 +246,51 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(Geom g, const uint16_t* __re
                                                     const double* __restrict__ pbb, double* __restrict__ pout,
                                                     int nparts, float ihx2, float ihy2, float ihz2, int64_t chunk,
                                                     int64_t padded) {
+    const int64_t nc = cell_count(g);
+    const Span sp = block_span(nc, chunk, padded);
+    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
+    // the thread's first group of cells loaded before the partial sum is
+    // waited for (it does not depend on it)
+    constexpr int G = 4;
+    const int64_t g0 = sp.c0 + threadIdx.x;
+    const bool early = g0 + (int64_t)(G - 1) * kBlock < sp.c1;
+    Stencil64In raw[G];
+    if (early) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) raw[u] = stencil64_in(g0 + (int64_t)u * kBlock, code, x, b, st);
+    }
     double bb[1];
     sum_partials<1>(pbb, nparts, bb);
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->bnorm2 = bb[0];
-    const int64_t nc = cell_count(g);
-    const Span sp = block_span(nc, chunk, padded);
     const float ih2[3] = {ihx2, ihy2, ihz2};
     const double ih2d[3] = {ihx2, ihy2, ihz2};
-    const Stride st = stride_of(g.n[0], g.n[1], g.n[2]);
     const int pre = sc->precond;  // 0 none, 1 diagonal, 2 multigrid (z and r.z come later)
     double v[2] = {0.0, 0.0};
-    span_groups<4, Stencil64In>(
-        sp.c0, sp.c1, [&](int64_t c) { return stencil64_in(c, code, x, b, st); },
-        [&](int64_t c, const Stencil64In& t) {
-            // the identity rows of non-FLUID cells have the solution 0: the
-            // start takes it there (FLUID rows couple only to FLUID cells), so
-            // r is 0 outside FLUID from the start and stays so
-            const bool fl = (t.cd & kFluidBit) != 0;
-            if (!fl && t.xc != 0.0) x[c] = 0.0;  // written only where it is not 0 already
-            const float rc = fl ? (float)((double)t.bc - code_apply64_nb(t.cd, ih2d, t.xc, t.nb)) : 0.0f;
-            r[c] = rc;
-            float zc = rc;
-            if (pre == 1) {
-                zc = rc / code_center(t.cd, ih2);
-                z[c] = zc;
-            }
-            if (mgx) mgx[c] = mg_weight(t.cd, ih2) * rc;  // the V-cycle's first sweep from zero
-            d0[c] = 0.0f;
-            if (pre != 2) v[0] += (double)rc * (double)zc;
-            v[1] += (double)rc * (double)rc;
-        });
+    auto use = [&](int64_t c, const Stencil64In& t) {
+        // the identity rows of non-FLUID cells have the solution 0: the
+        // start takes it there (FLUID rows couple only to FLUID cells), so
+        // r is 0 outside FLUID from the start and stays so
+        const bool fl = (t.cd & kFluidBit) != 0;
+        if (!fl && t.xc != 0.0) x[c] = 0.0;  // written only where it is not 0 already
+        const float rc = fl ? (float)((double)t.bc - code_apply64_nb(t.cd, ih2d, t.xc, t.nb)) : 0.0f;
+        r[c] = rc;
+        float zc = rc;
+        if (pre == 1) {
+            zc = rc / code_center(t.cd, ih2);
+            z[c] = zc;
+        }
+        if (mgx) mgx[c] = mg_weight(t.cd, ih2) * rc;  // the V-cycle's first sweep from zero
+        d0[c] = 0.0f;
+        if (pre != 2) v[0] += (double)rc * (double)zc;
+        v[1] += (double)rc * (double)rc;
+    };
+    if (early) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) use(g0 + (int64_t)u * kBlock, raw[u]);
+    }
+    span_groups_from<G, Stencil64In>(
+        early ? g0 + (int64_t)G * kBlock : g0, sp.c1, [&](int64_t c) { return stencil64_in(c, code, x, b, st); },
+        use);
     write_partials<2>(v, pout);
 }
 
