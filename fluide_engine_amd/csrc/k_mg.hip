@@ -15,7 +15,7 @@
 //     correction (weights 3/4, 1/4 per axis; out-of-range coarse neighbours
 //     clamp onto the boundary cell, which is the Neumann wall);
 //   * restriction: the transpose of that prolongation divided by 8;
-//   * coarsest level (<= 512 cells): 8 damped Jacobi sweeps (omega = 0.9)
+//   * coarsest level (<= 512 cells): 16 damped Jacobi sweeps (omega = 0.9)
 //     from zero, a fixed polynomial in the operator.
 // Non-FLUID cells carry no residual into the coarse levels and take no
 // correction; on level 0 they are smoothed like the identity rows they are.
@@ -35,7 +35,7 @@
 namespace fe {
 
 constexpr float kOmegaCoarse = 0.9f;  // coarsest-level Jacobi damping
-constexpr int kCoarseSweeps = 8;
+constexpr int kCoarseSweeps = 16;
 constexpr int kCoarseMax = 512;       // cells of the coarsest level
 constexpr int kTailMax = kCoarseMax;  // levels of at most this many cells run in k_mg_tail
 constexpr int kTailLds = kCoarseMax;  // cells of all tail levels together (5 floats + a row code each in LDS)
