@@ -15,7 +15,7 @@
 //     correction (weights 3/4, 1/4 per axis; out-of-range coarse neighbours
 //     clamp onto the boundary cell, which is the Neumann wall);
 //   * restriction: the transpose of that prolongation divided by 8;
-//   * coarsest level (<= 512 cells): 16 damped Jacobi sweeps (omega = 0.9)
+//   * coarsest level (<= 512 cells): 8 steps of Chebyshev iteration on D^-1 A over [0.05, 2]
 //     from zero, a fixed polynomial in the operator.
 // Non-FLUID cells carry no residual into the coarse levels and take no
 // correction; on level 0 they are smoothed like the identity rows they are.
@@ -34,8 +34,14 @@
 
 namespace fe {
 
-constexpr float kOmegaCoarse = 0.9f;  // coarsest-level Jacobi damping
-constexpr int kCoarseSweeps = 16;
+// Coarsest level: kCoarseSweeps steps of Chebyshev iteration from zero on
+// D^-1 A over [kChebLo, kChebHi].  Gershgorin bounds every eigenvalue of D^-1 A
+// (diagonal >= the sum of the off-diagonal magnitudes on every row) by 2, so
+// with kChebHi = 2 the residual polynomial's Chebyshev form lies in (0, 1] at
+// every eigenvalue, below kChebLo included: the step is a fixed symmetric
+// positive-definite polynomial of D^-1 A, as PCG needs.
+constexpr int kCoarseSweeps = 8;
+constexpr float kChebLo = 0.05f, kChebHi = 2.0f;
 constexpr int kCoarseMax = 512;       // cells of the coarsest level
 constexpr int kTailMax = kCoarseMax;  // levels of at most this many cells run in k_mg_tail
 constexpr int kTailLds = kCoarseMax;  // cells of all tail levels together (5 floats + a row code each in LDS)
@@ -992,6 +998,33 @@ __device__ inline TailLevel tail_level(const TailArgs& a, int t) {
 
 // One Jacobi sweep: out = in + w (b - A in), or out = w b when `in` is null.
 // dg, wt: the level's diagonal and smoother weight per cell (precomputed).
+// One Chebyshev step on the coarsest level (wt = 1/diagonal): d = c1 d +
+// c2 D^-1 (b - A in), out = in + d; from zero (in == nullptr) d = c2 D^-1 b.
+// d is per cell, so it is updated in place.
+__device__ inline void tail_cheb(const TailLevel& l, const uint16_t* code, const float* dg, const float* wt,
+                                 const float* in, float* out, const float* b, float* d, float c1, float c2) {
+#pragma unroll
+    for (int u = 0; u < kTailPer; ++u) {
+        const int c = threadIdx.x + u * kTailThreads;
+        if (c >= l.nc) break;
+        const uint16_t cd = code[c];
+        const float w = wt[c];
+        float xo, dn;
+        if (in) {
+            const float xc = in[c];
+            const float r = w * (b[c] - code_apply(cd, dg[c], c, l.st, l.ih2, xc, [&](int n) { return in[n]; }));
+            dn = c1 * d[c] + c2 * r;
+            xo = xc + dn;
+        } else {
+            dn = c2 * (w * b[c]);
+            xo = dn;
+        }
+        d[c] = dn;
+        out[c] = xo;
+    }
+    __syncthreads();
+}
+
 __device__ inline void tail_sweep(const TailLevel& l, const uint16_t* code, const float* dg, const float* wt,
                                   const float* in, float* out, const float* b) {
     // a tail level holds at most kTailMax cells: a fixed trip count lets the
@@ -1023,6 +1056,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
                                                           const SolverScalars* __restrict__ st) {
     const int done = st->done;  // tested after the loads below (see k_mg_smooth)
     __shared__ float X[kTailLds], Tm[kTailLds], B[kTailLds], Dg[kTailLds], Wt[kTailLds];
+    __shared__ float Dd[kCoarseMax];  // the coarsest level's Chebyshev direction
     __shared__ uint16_t Cd[kTailLds];
     // Row codes and the right-hand side: every load is issued before the
     // first LDS store (one memory latency in all).
@@ -1066,7 +1100,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
     // with its own damping)
     for (int t = 0; t < a.levels; ++t) {
         const TailLevel l = tail_level(a, t);
-        const float omega = t == a.levels - 1 ? kOmegaCoarse : kMgOmega;
+        const float omega = t == a.levels - 1 ? 1.0f : kMgOmega;  // the coarsest: 1/diagonal for Chebyshev
 #pragma unroll
         for (int u = 0; u < kTailPer; ++u) {
             const int c = threadIdx.x + u * kTailThreads;
@@ -1119,17 +1153,25 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
             __syncthreads();
         }
     }
-    // coarsest level: Jacobi sweeps from zero, ending in Tm
+    // coarsest level: Chebyshev iteration from zero, ending in Tm (the
+    // three-term recurrence with rho_k = 1 / (2 sigma - rho_(k-1)), rho_0 = 1 / sigma)
     {
         const TailLevel l = tail_level(a, a.levels - 1);
         const int o = l.o;
         const uint16_t* code = Cd + o;
-        static_assert(kCoarseSweeps % 2 == 0, "the sweeps ping-pong X -> Tm");
-        tail_sweep(l, code, Dg + o, Wt + o, nullptr, X + o, B + o);
-        tail_sweep(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o);
-        for (int s = 2; s < kCoarseSweeps; s += 2) {
-            tail_sweep(l, code, Dg + o, Wt + o, Tm + o, X + o, B + o);
-            tail_sweep(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o);
+        static_assert(kCoarseSweeps % 2 == 0, "the steps ping-pong X -> Tm");
+        const float theta = 0.5f * (kChebHi + kChebLo), delta = 0.5f * (kChebHi - kChebLo);
+        const float sigma = theta / delta;
+        float rho = 1.0f / sigma;
+        tail_cheb(l, code, Dg + o, Wt + o, nullptr, X + o, B + o, Dd, 0.0f, 1.0f / theta);
+        for (int k = 1; k < kCoarseSweeps; ++k) {
+            const float rn = 1.0f / (2.0f * sigma - rho);
+            const float c1 = rn * rho, c2 = 2.0f * rn / delta;
+            if (k & 1)
+                tail_cheb(l, code, Dg + o, Wt + o, X + o, Tm + o, B + o, Dd, c1, c2);
+            else
+                tail_cheb(l, code, Dg + o, Wt + o, Tm + o, X + o, B + o, Dd, c1, c2);
+            rho = rn;
         }
     }
     // up: prolongation, two sweeps Tm -> X -> Tm
