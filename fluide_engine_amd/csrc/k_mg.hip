@@ -10,7 +10,10 @@
 // BuildSystem's 7-point rule on those markers with twice the spacing.
 //
 // The V-cycle is symmetric, so the preconditioner is SPD and CG stays CG:
-//   * damped Jacobi smoothing (omega = 6/7), two sweeps before and two after;
+//   * smoothing: two steps of Chebyshev iteration on D^-1 A over [0.15, 2]
+//     before and the same two after (mg_cheb2; levels inside the
+//     one-workgroup tail on small grids: two damped Jacobi sweeps with the
+//     first step's weight before and after);
 //   * prolongation: trilinear interpolation of the cell-centred coarse
 //     correction (weights 3/4, 1/4 per axis; out-of-range coarse neighbours
 //     clamp onto the boundary cell, which is the Neumann wall);
@@ -302,6 +305,13 @@ __device__ inline float mg_sweep_cell(uint16_t cd, float xc, float bc, const flo
     const float w = mg_weight(cd, ih2);
     return xc + w * (bc - code_apply_nb(cd, dgn, ih2, xc, nb));
 }
+// The second smoothing step at cell c (mg_cheb2 with the cell's weights).
+__device__ inline float mg_cheb2_cell(uint16_t cd, float xc, float x0, float bc, const float (&nb)[6],
+                                      const float ih2[3]) {
+    const float dgn = code_center(cd, ih2);
+    const float w = mg_weight(cd, ih2);
+    return mg_cheb2(xc, x0, kMgW21 * w, bc - code_apply_nb(cd, dgn, ih2, xc, nb));
+}
 
 // The residual b - A x of a FLUID cell, 0 elsewhere.
 __device__ inline float mg_residual_cell(uint16_t cd, float xc, float bc, const float (&nb)[6], const float ih2[3]) {
@@ -403,9 +413,10 @@ __device__ inline StencilIn stencil_in(int64_t c, const uint16_t* __restrict__ c
 // solver chunks holding a FLUID cell; the others hold zeros and are skipped.
 __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_t* __restrict__ code,
                                                       const float* __restrict__ b, const float* __restrict__ xin,
-                                                      float* __restrict__ xout, const SolverScalars* st,
+                                                      float* xout, const SolverScalars* st,
                                                       double* __restrict__ prz, const double* __restrict__ pconf,
-                                                      int nparts, const uint8_t* __restrict__ cfl) {
+                                                      int nparts, const uint8_t* __restrict__ cfl, int second,
+                                                      const float* x0) {
     // The stop flag is read with the data and tested before the first write,
     // so its load adds no latency of its own (the loads before it are in bounds).
     const int done = st->done;
@@ -436,7 +447,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
         sp.c0, sp.c1, [&](int64_t c) { return stencil_in(c, code, xin, b, stv, X); },
         [&](int64_t c, const StencilIn& t) {
             if (done) return;  // uniform: the solve has stopped
-            const float xo = mg_sweep_cell(t.cd, t.xc, t.bc, t.nb, L.ih2);
+            // second: the second smoothing step, from x0 (null: 0); x0 may be
+            // xout (each cell reads its own x0 before writing it)
+            const float xo = second ? mg_cheb2_cell(t.cd, t.xc, x0 ? x0[c] : 0.0f, t.bc, t.nb, L.ih2)
+                                    : mg_sweep_cell(t.cd, t.xc, t.bc, t.nb, L.ih2);
             xout[c] = xo;
             v[0] += (double)t.bc * (double)xo;
         });
@@ -752,7 +766,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
         if (cd & kFluidBit) {
             const float nb[6] = {sX1[e3 - 1],       sX1[e3 + 1],       sX1[e3 - W3::sy],
                                  sX1[e3 + W3::sy], sX1[e3 - W3::sz], sX1[e3 + W3::sz]};
-            x2 = mg_sweep_fluid_row(cen, w, sX1[e3], sB[e], nb, nih2);
+            const float x1 = sX1[e3];
+            x2 = mg_cheb2(x1, 0.0f, kMgW21 * w, sB[e] - mg_apply_fluid_row(cen, x1, nb, nih2));
         }
         sX2[e] = x2;
         if (wx >= 2 && wx < 2 + T::fx && wy >= 2 && wy < 2 + T::fy && wz >= 2 && wz < 2 + T::fz) {
@@ -944,7 +959,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
                 const float nb[6] = {sXp[e1 - 1],       sXp[e1 + 1],       sXp[e1 - W1::sy],
                                      sXp[e1 + W1::sy], sXp[e1 - W1::sz], sXp[e1 + W1::sz]};
                 const float bc = sB[e1];
-                o = mg_sweep_fluid_row(cen, w, sXp[e1], bc, nb, nih2);
+                const float x1 = sXp[e1];
+                o = mg_cheb2(x1, sY[e2], kMgW21 * w, bc - mg_apply_fluid_row(cen, x1, nb, nih2));
                 if (L0) v[0] += (double)bc * (double)o;
             }
             out[fused_index(i, j, k, a.n)] = o;
@@ -1451,11 +1467,12 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
         const dim3 grid((unsigned)a.padded);
         if (q == 0)
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, pconf, nparts, cfl0);
+                         (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, pconf, nparts, cfl0, 1,
+                         (const float*)nullptr);
         else
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x, l.x2, (const SolverScalars*)st, (double*)nullptr,
-                               (const double*)nullptr, 0, (const uint8_t*)nullptr);
+                               (const double*)nullptr, 0, (const uint8_t*)nullptr, 1, (const float*)nullptr);
         hipLaunchKernelGGL(k_mg_residual, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                            (const float*)l.b, (const float*)l.x2, l.r, st, q == 0 ? cfl0 : (const uint8_t*)nullptr);
         const dim3 rgrid((unsigned)((c.n[0] + kRtx - 1) / kRtx), (unsigned)((c.n[1] + kRty - 1) / kRty),
@@ -1475,16 +1492,17 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
         if (q == 0) {
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
                          (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0,
-                         cfl0);
+                         cfl0, 0, (const float*)nullptr);
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
-                         (const float*)l.b, (const float*)l.x, z, st, prz, (const double*)nullptr, 0, cfl0);
+                         (const float*)l.b, (const float*)l.x, z, st, prz, (const double*)nullptr, 0, cfl0, 1,
+                         (const float*)l.x2);
         } else {
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0,
-                               (const uint8_t*)nullptr);
+                               (const uint8_t*)nullptr, 0, (const float*)nullptr);
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, (const double*)nullptr, 0,
-                               (const uint8_t*)nullptr);
+                               (const uint8_t*)nullptr, 1, (const float*)l.x2);
         }
     }
     return hipGetLastError();
