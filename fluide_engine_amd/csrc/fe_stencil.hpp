@@ -175,27 +175,37 @@ __device__ inline double code_apply64(uint16_t code, int64_t c, const Stride& st
 // 1, so the smoother stays convergent and the V-cycle SPD.  Measured on the
 // 128^3 dam break: 11 multigrid-PCG iterations against 12 with omega = 2/3.
 // Multigrid smoothing: two steps of Chebyshev iteration on D^-1 A over
-// [kMgSmLo, 2] per smoothing (2 bounds every eigenvalue of D^-1 A, each row's
+// [lo, 2] per smoothing (2 bounds every eigenvalue of D^-1 A, each row's
 // diagonal being at least the sum of its off-diagonal magnitudes):
-//   x1 = x0 + kMgOmega D^-1 (b - A x0),
-//   x2 = x1 + kMgBeta (x1 - x0) + w2 D^-1 (b - A x1),  w2 = kMgW21 * (kMgOmega / diag).
+//   x1 = x0 + omega D^-1 (b - A x0),
+//   x2 = x1 + beta (x1 - x0) + w2 D^-1 (b - A x1),  w2 = w21 * (omega / diag)
+// (MgSmooth; the finest level and the levels below it may use different
+// intervals).
 // The same polynomial before and after the coarse correction keeps the
 // V-cycle symmetric, and its residual polynomial lies in (0, 1) on (0, 2], so
 // the preconditioner is SPD.
-constexpr double kMgSmLo = 0.15, kMgSmHi = 2.0;
-constexpr double kMgSmTheta = 0.5 * (kMgSmLo + kMgSmHi), kMgSmDelta = 0.5 * (kMgSmHi - kMgSmLo);
-constexpr double kMgSmRho0 = kMgSmDelta / kMgSmTheta;
-constexpr double kMgSmRho1 = 1.0 / (2.0 * kMgSmTheta / kMgSmDelta - kMgSmRho0);
-constexpr float kMgOmega = (float)(1.0 / kMgSmTheta);  // the first step's weight (times 1/diag)
-constexpr float kMgBeta = (float)(kMgSmRho1 * kMgSmRho0);
-constexpr float kMgW21 = (float)(2.0 * kMgSmRho1 / kMgSmDelta * kMgSmTheta);  // second step weight / first
+constexpr double kMgSmHi = 2.0;
+constexpr double kMgSmLo = 0.15;        // the finest level's interval [kMgSmLo, 2]
+constexpr double kMgSmLoCoarse = 0.07;  // the levels below it
+struct MgSmooth {
+    float omega;  // the first step's weight (times 1/diag): 1/theta
+    float beta;   // the second step's (x1 - x0) coefficient
+    float w21;    // the second step's weight over the first's
+};
+constexpr MgSmooth mg_smooth_of(double lo) {
+    const double theta = 0.5 * (lo + kMgSmHi), delta = 0.5 * (kMgSmHi - lo);
+    const double rho0 = delta / theta, rho1 = 1.0 / (2.0 * theta / delta - rho0);
+    return MgSmooth{(float)(1.0 / theta), (float)(rho1 * rho0), (float)(2.0 * rho1 / delta * theta)};
+}
+constexpr MgSmooth kMgFine = mg_smooth_of(kMgSmLo), kMgCoarse = mg_smooth_of(kMgSmLoCoarse);
+constexpr float kMgOmega = kMgFine.omega;  // the finest level's first step (level 0's first sweep from zero)
 
 // The second smoothing step from x1 (x0 the value before smoothing, r the
-// residual b - A x1, w2 = kMgW21 * the first step's weight); fixed rounding
+// residual b - A x1, w2 = w21 * the first step's weight); fixed rounding
 // (no contraction), so every kernel that smooths gives the same bits.
-__device__ inline float mg_cheb2(float x1, float x0, float w2, float r) {
+__device__ inline float mg_cheb2(float x1, float x0, float w2, float r, float beta) {
 #pragma clang fp contract(off)
-    return fmaf(w2, r, fmaf(kMgBeta, x1 - x0, x1));
+    return fmaf(w2, r, fmaf(beta, x1 - x0, x1));
 }
 
 __device__ inline float mg_weight(uint16_t code, const float ih2[3], float omega = kMgOmega) {

@@ -11,7 +11,7 @@
 //
 // The V-cycle is symmetric, so the preconditioner is SPD and CG stays CG:
 //   * smoothing: two steps of Chebyshev iteration on D^-1 A over [0.15, 2]
-//     before and the same two after (mg_cheb2; levels inside the
+//     on the finest level and [0.07, 2] below it, before and the same two after (mg_cheb2; levels inside the
 //     one-workgroup tail on small grids: two damped Jacobi sweeps with the
 //     first step's weight before and after);
 //   * prolongation: trilinear interpolation of the cell-centred coarse
@@ -300,17 +300,18 @@ __device__ inline float prolong_at(int i, int j, int k, int cx, int cy, int cz, 
 
 // One damped-Jacobi update of a cell, x + w (b - A x): the smoother of every
 // level and kernel variant (the same expression, so the same rounding).
-__device__ inline float mg_sweep_cell(uint16_t cd, float xc, float bc, const float (&nb)[6], const float ih2[3]) {
+__device__ inline float mg_sweep_cell(uint16_t cd, float xc, float bc, const float (&nb)[6], const float ih2[3],
+                                     float omega) {
     const float dgn = code_center(cd, ih2);
-    const float w = mg_weight(cd, ih2);
+    const float w = mg_weight(cd, ih2, omega);
     return xc + w * (bc - code_apply_nb(cd, dgn, ih2, xc, nb));
 }
 // The second smoothing step at cell c (mg_cheb2 with the cell's weights).
 __device__ inline float mg_cheb2_cell(uint16_t cd, float xc, float x0, float bc, const float (&nb)[6],
-                                      const float ih2[3]) {
+                                      const float ih2[3], const MgSmooth& sm) {
     const float dgn = code_center(cd, ih2);
-    const float w = mg_weight(cd, ih2);
-    return mg_cheb2(xc, x0, kMgW21 * w, bc - code_apply_nb(cd, dgn, ih2, xc, nb));
+    const float w = mg_weight(cd, ih2, sm.omega);
+    return mg_cheb2(xc, x0, sm.w21 * w, bc - code_apply_nb(cd, dgn, ih2, xc, nb), sm.beta);
 }
 
 // The residual b - A x of a FLUID cell, 0 elsewhere.
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
                                                       float* xout, const SolverScalars* st,
                                                       double* __restrict__ prz, const double* __restrict__ pconf,
                                                       int nparts, const uint8_t* __restrict__ cfl, int second,
-                                                      const float* x0) {
+                                                      const float* x0, int fine) {
     // The stop flag is read with the data and tested before the first write,
     // so its load adds no latency of its own (the loads before it are in bounds).
     const int done = st->done;
@@ -449,8 +450,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(LevelArgs L, const uint16_
             if (done) return;  // uniform: the solve has stopped
             // second: the second smoothing step, from x0 (null: 0); x0 may be
             // xout (each cell reads its own x0 before writing it)
-            const float xo = second ? mg_cheb2_cell(t.cd, t.xc, x0 ? x0[c] : 0.0f, t.bc, t.nb, L.ih2)
-                                    : mg_sweep_cell(t.cd, t.xc, t.bc, t.nb, L.ih2);
+            const MgSmooth& sm = fine ? kMgFine : kMgCoarse;
+            const float xo = second ? mg_cheb2_cell(t.cd, t.xc, x0 ? x0[c] : 0.0f, t.bc, t.nb, L.ih2, sm)
+                                    : mg_sweep_cell(t.cd, t.xc, t.bc, t.nb, L.ih2, sm.omega);
             xout[c] = xo;
             v[0] += (double)t.bc * (double)xo;
         });
@@ -543,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(const float* __restrict_
     bc[c] = v;
     if (x1) {
         const float ih2[3] = {cihx2, cihy2, cihz2};
-        x1[c] = mg_weight(cd, ih2) * v;
+        x1[c] = mg_weight(cd, ih2, kMgCoarse.omega) * v;
     }
 }
 
@@ -734,12 +736,13 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
         }
     }
     const uint32_t ccd = cin ? (uint32_t)ccode[ci] : 0u;
-    const CodeLanes lanes = code_lanes(a.ih2, kMgOmega);
+    constexpr MgSmooth SM = L0 ? kMgFine : kMgCoarse;
+    const CodeLanes lanes = code_lanes(a.ih2, SM.omega);
     const float nih2[3] = {-a.ih2[0], -a.ih2[1], -a.ih2[2]};
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = threadIdx.x + u * kFusedThreads;
-        const float w = lanes_weight(vc[u], lanes, kMgOmega);  // every lane: a cross-lane read
+        const float w = lanes_weight(vc[u], lanes, SM.omega);  // every lane: a cross-lane read
         if (e < W3::n) {
             sX1[e] = w * vb[u];  // the first sweep from zero
             const int wx = e % W3::x, wy = (e / W3::x) % W3::y, wz = e / W3::sz;
@@ -767,7 +770,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_down(FusedArgs a, const fl
             const float nb[6] = {sX1[e3 - 1],       sX1[e3 + 1],       sX1[e3 - W3::sy],
                                  sX1[e3 + W3::sy], sX1[e3 - W3::sz], sX1[e3 + W3::sz]};
             const float x1 = sX1[e3];
-            x2 = mg_cheb2(x1, 0.0f, kMgW21 * w, sB[e] - mg_apply_fluid_row(cen, x1, nb, nih2));
+            x2 = mg_cheb2(x1, 0.0f, SM.w21 * w, sB[e] - mg_apply_fluid_row(cen, x1, nb, nih2), SM.beta);
         }
         sX2[e] = x2;
         if (wx >= 2 && wx < 2 + T::fx && wy >= 2 && wy < 2 + T::fy && wz >= 2 && wz < 2 + T::fz) {
@@ -901,7 +904,8 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
             if (e < CWin<T>::n) sXc[e] = vxc[u];
         }
     }
-    const CodeLanes lanes = code_lanes(a.ih2, kMgOmega);
+    constexpr MgSmooth SM = L0 ? kMgFine : kMgCoarse;
+    const CodeLanes lanes = code_lanes(a.ih2, SM.omega);
     const float nih2[3] = {-a.ih2[0], -a.ih2[1], -a.ih2[2]};
     __syncthreads();
     if (done) return;  // uniform: the solve has stopped
@@ -960,7 +964,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_mg_up(FusedArgs a, const floa
                                      sXp[e1 + W1::sy], sXp[e1 - W1::sz], sXp[e1 + W1::sz]};
                 const float bc = sB[e1];
                 const float x1 = sXp[e1];
-                o = mg_cheb2(x1, sY[e2], kMgW21 * w, bc - mg_apply_fluid_row(cen, x1, nb, nih2));
+                o = mg_cheb2(x1, sY[e2], SM.w21 * w, bc - mg_apply_fluid_row(cen, x1, nb, nih2), SM.beta);
                 if (L0) v[0] += (double)bc * (double)o;
             }
             out[fused_index(i, j, k, a.n)] = o;
@@ -1116,7 +1120,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(TailArgs a, const floa
     // with its own damping)
     for (int t = 0; t < a.levels; ++t) {
         const TailLevel l = tail_level(a, t);
-        const float omega = t == a.levels - 1 ? 1.0f : kMgOmega;  // the coarsest: 1/diagonal for Chebyshev
+        const float omega = t == a.levels - 1 ? 1.0f : kMgCoarse.omega;  // the coarsest: 1/diagonal for Chebyshev
 #pragma unroll
         for (int u = 0; u < kTailPer; ++u) {
             const int c = threadIdx.x + u * kTailThreads;
@@ -1468,11 +1472,11 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
         if (q == 0)
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
                          (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, pconf, nparts, cfl0, 1,
-                         (const float*)nullptr);
+                         (const float*)nullptr, 1);
         else
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x, l.x2, (const SolverScalars*)st, (double*)nullptr,
-                               (const double*)nullptr, 0, (const uint8_t*)nullptr, 1, (const float*)nullptr);
+                               (const double*)nullptr, 0, (const uint8_t*)nullptr, 1, (const float*)nullptr, 0);
         hipLaunchKernelGGL(k_mg_residual, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                            (const float*)l.b, (const float*)l.x2, l.r, st, q == 0 ? cfl0 : (const uint8_t*)nullptr);
         const dim3 rgrid((unsigned)((c.n[0] + kRtx - 1) / kRtx), (unsigned)((c.n[1] + kRty - 1) / kRty),
@@ -1492,17 +1496,17 @@ hipError_t mg_apply(fe_scene* s, const float* r, float* z, SolverScalars* st, do
         if (q == 0) {
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
                          (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0,
-                         cfl0, 0, (const float*)nullptr);
+                         cfl0, 0, (const float*)nullptr, 1);
             launch_timed(s, FE_KERNEL_MG_SMOOTH, tag, k_mg_smooth, grid, dim3(kBlock), a, (const uint16_t*)l.code,
                          (const float*)l.b, (const float*)l.x, z, st, prz, (const double*)nullptr, 0, cfl0, 1,
-                         (const float*)l.x2);
+                         (const float*)l.x2, 1);
         } else {
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x2, l.x, st, (double*)nullptr, (const double*)nullptr, 0,
-                               (const uint8_t*)nullptr, 0, (const float*)nullptr);
+                               (const uint8_t*)nullptr, 0, (const float*)nullptr, 0);
             hipLaunchKernelGGL(k_mg_smooth, grid, dim3(kBlock), 0, s->stream, a, (const uint16_t*)l.code,
                                (const float*)l.b, (const float*)l.x, l.x2, st, (double*)nullptr, (const double*)nullptr, 0,
-                               (const uint8_t*)nullptr, 1, (const float*)l.x2);
+                               (const uint8_t*)nullptr, 1, (const float*)l.x2, 0);
         }
     }
     return hipGetLastError();
