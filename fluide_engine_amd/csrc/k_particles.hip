@@ -757,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
     int lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) candidate_cells<METHOD>(g, C, a, F0[a], F0[a] + kScB - 1, lo[a], hi[a]);
-    const int ny = hi[1] - lo[1] + 1, rows = ny * (hi[2] - lo[2] + 1);
+    const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1, rows = ny * nz;
     for (int u = threadIdx.x; u < kScSlots; u += kBlock) s_num[u] = s_den[u] = 0ull;
     for (int u = threadIdx.x; u < (kScBases + 31) / 32; u += kBlock) s_occ[u] = 0u;
     if (threadIdx.x < 64) {
@@ -768,7 +768,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
         for (int h = 0; h < 2; ++h) {
             const int r = 2 * lane + h;
             if (r < rows) {
-                const int jc = lo[1] + r % ny, kc = lo[2] + r / ny;
+                // rows y-fastest; z-fastest for the w faces, whose stencil z
+                // base is the cell's own: consecutive rows then differ in that
+                // base, which spreads a chunk's atomics over more slots (the
+                // sums do not depend on the order)
+                const int jc = C == 2 ? lo[1] + r / nz : lo[1] + r % ny;
+                const int kc = C == 2 ? lo[2] + r % nz : lo[2] + r / ny;
                 const int64_t row = (int64_t)g.n[0] * ((int64_t)jc + (int64_t)g.n[1] * kc);
                 const int s0 = start[row + lo[0]];
                 len[h] = start[row + hi[0] + 1] - s0;
