@@ -731,6 +731,7 @@ constexpr int kScSy = 9, kScSz = 76;
 constexpr int kScRun = 1;    // consecutive candidates per lane group (see k_p2g_scatter)
 constexpr int kScAhead = 1;  // chunks whose loads are in flight ahead of the atomics
 constexpr int kScSlots = (kScB - 1) * (1 + kScSy + kScSz) + 1;
+static_assert(kScSy >= kScB && kScSz >= (kScB - 1) * (1 + kScSy) + 1, "faces of a block never share a slot");
 constexpr int kScRows = (kScB + 2) * (kScB + 2);               // x rows of candidate cells, at most
 constexpr int kScBases = (kScB + 1) * (kScB + 1) * (kScB + 1); // stencil bases with a node in the block
 static_assert(kScRows <= 128, "one wavefront scans the row lengths, two per lane");
