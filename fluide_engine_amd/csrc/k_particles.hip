@@ -709,9 +709,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
 // workgroup owns an 8 x 8 x 8 block of faces of component C and walks the
 // particles of the block's candidate cells (candidate_cells over the block's
 // faces): each x row of candidate cells is one contiguous range of sorted
-// particles, and the workgroup takes the concatenated ranges with
-// consecutive threads on consecutive particles (a thread finds its row by a
-// binary search of the ranges' prefix sums in LDS).  Every stencil node of a
+// particles, and the workgroup takes the concatenated ranges 256 candidates
+// at a time, the lanes of a wavefront 4 candidates apart (a thread finds its
+// row by a binary search of the ranges' prefix sums in LDS).  Every stencil node of a
 // candidate that lies in the block adds the node's two fixed-point terms to
 // the block's LDS sums with 64-bit integer atomics; nodes outside the block
 // belong to the neighbouring block, which reads the same particle.  A face is
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
 // particle (one bit per base of the block in LDS), the reference's rule
 // (marked even for a zero weight).  A particle is read by about 1.8 blocks
 // (10 x 10 x 9 candidate cells per 8 x 8 x 8 faces) against 6 tiles of the
-// 2x2x2 gather, and a wavefront's loads touch 8 cache lines instead of up to 64.
+// 2x2x2 gather, and a wavefront's loads touch 32 cache lines, not up to 64.
 // ---------------------------------------------------------------------------
 constexpr int kScB = 8;                                        // faces per block edge
 constexpr int kScFaces = kScB * kScB * kScB;                   // faces per block
