@@ -709,8 +709,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
 // workgroup owns an 8 x 8 x 8 block of faces of component C and walks the
 // particles of the block's candidate cells (candidate_cells over the block's
 // faces): each x row of candidate cells is one contiguous range of sorted
-// particles, and the workgroup takes the concatenated ranges 256 candidates
-// at a time, the lanes of a wavefront 4 candidates apart (a thread finds its
+// particles, and the workgroup takes the concatenated ranges 1024 candidates
+// at a time, the lanes of a wavefront 16 candidates apart (a thread finds its
 // row by a binary search of the ranges' prefix sums in LDS).  Every stencil node of a
 // candidate that lies in the block adds the node's two fixed-point terms to
 // the block's LDS sums with 64-bit integer atomics; nodes outside the block
@@ -719,7 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_face(Geom g, const float* __rest
 // particle (one bit per base of the block in LDS), the reference's rule
 // (marked even for a zero weight).  A particle is read by about 1.8 blocks
 // (10 x 10 x 9 candidate cells per 8 x 8 x 8 faces) against 6 tiles of the
-// 2x2x2 gather, and a wavefront's loads touch 32 cache lines, not up to 64.
+// 2x2x2 gather.
 // ---------------------------------------------------------------------------
 constexpr int kScB = 8;                                        // faces per block edge
 constexpr int kScFaces = kScB * kScB * kScB;                   // faces per block
@@ -730,6 +730,7 @@ constexpr int kScFaces = kScB * kScB * kScB;                   // faces per bloc
 constexpr int kScSy = 9, kScSz = 76;
 constexpr int kScRun = 1;    // consecutive candidates per lane group (see k_p2g_scatter)
 constexpr int kScAhead = 1;  // chunks whose loads are in flight ahead of the atomics
+constexpr int kScThreads = 1024;  // threads per workgroup, one chunk of candidates per iteration
 constexpr int kScSlots = (kScB - 1) * (1 + kScSy + kScSz) + 1;
 static_assert(kScSy >= kScB && kScSz >= (kScB - 1) * (1 + kScSy) + 1, "faces of a block never share a slot");
 constexpr int kScRows = (kScB + 2) * (kScB + 2);               // x rows of candidate cells, at most
@@ -737,7 +738,7 @@ constexpr int kScBases = (kScB + 1) * (kScB + 1) * (kScB + 1); // stencil bases 
 static_assert(kScRows <= 128, "one wavefront scans the row lengths, two per lane");
 
 template <int C, int METHOD>
-__global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __restrict__ V,
+__global__ __launch_bounds__(kScThreads) void k_p2g_scatter(Geom g, const float* __restrict__ V,
                                                         const float* __restrict__ Cm,
                                                         const float4* __restrict__ pos4, int64_t cap,
                                                         const int32_t* __restrict__ start,
@@ -759,8 +760,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
 #pragma unroll
     for (int a = 0; a < 3; ++a) candidate_cells<METHOD>(g, C, a, F0[a], F0[a] + kScB - 1, lo[a], hi[a]);
     const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1, rows = ny * nz;
-    for (int u = threadIdx.x; u < kScSlots; u += kBlock) s_num[u] = s_den[u] = 0ull;
-    for (int u = threadIdx.x; u < (kScBases + 31) / 32; u += kBlock) s_occ[u] = 0u;
+    for (int u = threadIdx.x; u < kScSlots; u += kScThreads) s_num[u] = s_den[u] = 0ull;
+    for (int u = threadIdx.x; u < (kScBases + 31) / 32; u += kScThreads) s_occ[u] = 0u;
     if (threadIdx.x < 64) {
         // row lengths and their exclusive prefix sums (rows 2l and 2l + 1 on lane l)
         const int lane = threadIdx.x;
@@ -796,18 +797,19 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
     __syncthreads();
     const int total = s_pre[rows];
     const float* __restrict__ vc = V + (int64_t)C * cap;
-    // Thread (wavefront w, lane l) takes candidate kScRun (w + 4 (l / kScRun))
-    // + l % kScRun of each chunk of 256 (with kScRun = 1: w + 4 l), so a
-    // wavefront's atomics land on the nodes of 64 / kScRun places spread over
-    // the chunk instead of the few cells that 64 consecutive particles fill:
-    // same-address atomics in one instruction serialise, and spreading the
-    // lanes (16 B positions 4 apart, 32 cache lines per load) measured
-    // faster than runs of 2, 4 or 8 consecutive particles (15, 23 and 60 %
-    // more time per launch at 128^3).  The positions and values of the next
+    // Thread (wavefront w of W = kScThreads / 64, lane l) takes candidate
+    // kScRun (w + W (l / kScRun)) + l % kScRun of each chunk of kScThreads
+    // (with kScRun = 1: w + W l), so a wavefront's atomics land on the nodes
+    // of 64 / kScRun places spread over the chunk instead of the few cells
+    // that 64 consecutive particles fill: same-address atomics in one
+    // instruction serialise.  Spreading the lanes measured faster than runs of
+    // 2, 4 or 8 consecutive particles (15, 23 and 60 % more time per launch
+    // at 128^3 with 256 threads), and 1024 threads (lanes 16 apart) faster than
+    // 512 and 256 (254, 268 and 339 us for the three launches).  The positions and values of the next
     // kScAhead chunks are loaded before this chunk's atomics (one chunk
     // ahead measured faster than two).
     const int lane = threadIdx.x & 63;
-    const int slot = (lane % kScRun) + kScRun * ((threadIdx.x >> 6) + 4 * (lane / kScRun));
+    const int slot = (lane % kScRun) + kScRun * ((threadIdx.x >> 6) + (kScThreads / 64) * (lane / kScRun));
     auto locate = [&](int q) {
         int r = 0;
 #pragma unroll
@@ -869,15 +871,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
     }
 #pragma unroll
     for (int u = 0; u < kScAhead; ++u) {
-        const int qa = slot + u * kBlock;
+        const int qa = slot + u * kScThreads;
         if (qa < total) {
             sp[u] = locate(qa);
             rec[u] = pos4[sp[u]];
             val[u] = vc[sp[u]];
         }
     }
-    for (int q = slot; q < total; q += kBlock) {
-        const int qa = q + kScAhead * kBlock;
+    for (int q = slot; q < total; q += kScThreads) {
+        const int qa = q + kScAhead * kScThreads;
         if (qa < total) {
             sp[kScAhead] = locate(qa);
             rec[kScAhead] = pos4[sp[kScAhead]];
@@ -892,7 +894,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_scatter(Geom g, const float* __r
         }
     }
     __syncthreads();
-    for (int u = threadIdx.x; u < kScFaces; u += kBlock) {
+    for (int u = threadIdx.x; u < kScFaces; u += kScThreads) {
         const int l[3] = {u % kScB, (u / kScB) % kScB, u / (kScB * kScB)};
         const int I = F0[0] + l[0], J = F0[1] + l[1], K = F0[2] + l[2];
         if (I >= sx || J >= sy || K >= sz) continue;
@@ -1211,7 +1213,7 @@ static hipError_t p2g_method(fe_scene* s) {
             const int64_t padded = pad_blocks((int64_t)nb[0] * nb[1] * nb[2]);
             auto kern = c == 0 ? k_p2g_scatter<0, METHOD>
                                : (c == 1 ? k_p2g_scatter<1, METHOD> : k_p2g_scatter<2, METHOD>);
-            launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kBlock), g, (const float*)P.v,
+            launch_timed(s, FE_KERNEL_P2G, 0, kern, dim3((unsigned)padded), dim3(kScThreads), g, (const float*)P.v,
                          (const float*)P.C, (const float4*)s->pos4, s->cap,
                          (const int32_t*)s->cell_start, (const uint32_t*)s->p2g_bound, head, s->u[c], s->mark[c],
                          s->snap[c], nb[0], nb[1], nb[2], padded);
